@@ -1,0 +1,122 @@
+#!/usr/bin/env python3
+"""Benchmark of the D3Q19 fluid + temperature streaming step on one MI355X.
+
+Workload: the 2-D Rayleigh-Benard configuration, 512 x 1 x 256 lattice
+(stored as NX=512, NY=256 vertical, NZ=1), D3Q19 fluid + D3Q19 temperature
+populations, Boussinesq buoyancy at Ra = 1e8, Pr = 1, no-slip and fixed-T
+walls, initial T linear plus seeded uniform noise 1e-3.  The timed path is the
+FP32-deviation fused stream+collide kernel (value); the FP64 plain path is
+timed beside it.  A step = one fused kernel launch advancing every node of
+both population sets one time step; inputs are resident in HBM.
+
+Prints one JSON line (see README of the driver contract in DESIGN.md).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "ch4-project_amd"), os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "D3Q19 fluid+T MLUPS, % HBM roofline; served-ensemble MLUPS + p99 latency"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_NODE = {"fp32dev": 2 * (19 + 19) * 4, "fp64": 2 * (19 + 19) * 8}  # populations read once + written once
+
+
+def gpu_leg(precision, steps, warmup):
+    import configs
+    from ch4g import Lattice
+    cfg = configs.rb2d(precision=precision)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.step(warmup)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    ms = lat.step_timed(steps)
+    lat.synchronize()
+    wall = time.perf_counter() - t0
+    nodes = cfg.nodes()
+    out = {"nodes": nodes, "wall_s": wall, "event_ms": ms, "mlups": nodes * steps / wall / 1e6,
+           "avg_launch_us": ms * 1e3 / steps}
+    h = lat.dump_averages()
+    out["nusselt"] = h["nuy"]
+    lat.close()
+    return out
+
+
+def cpu_leg(target_s=12.0):
+    """The oracle (FP64, reference-structured un-fused sweeps, OpenMP) on
+    this host's cores, on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import configs
+    from oracle import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    cfg = configs.rb2d()
+    o = Oracle(cfg)
+    o.initial_conditions()
+    t0 = time.perf_counter()
+    o.step(5)
+    probe = time.perf_counter() - t0
+    n = max(5, min(2000, int(target_s / max(probe / 5, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(n)
+    dt = time.perf_counter() - t0
+    return {"value": round(cfg.nodes() * n / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"2-D Rayleigh-Benard 512x256 fluid+T FP64, {n} steps ({dt:.1f} s), oracle/ch4_oracle.c "
+                      f"un-fused reference-structured sweeps, OpenMP {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-path leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (one lattice per GPU; multi-GPU slab decomposition "
+              "is a design in DESIGN.md, not built). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    from ch4g import abi
+    r32 = gpu_leg(abi.FP32_DEV, args.steps, args.warmup)
+    bpn = BYTES_PER_NODE["fp32dev"]
+    achieved = bpn * r32["nodes"] / (r32["avg_launch_us"] * 1e-6) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(r32["mlups"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["wall_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "2-D Rayleigh-Benard 512x1x256 (NX=512, NY=256 vertical), D3Q19 fluid + D3Q19 "
+                               "temperature, Ra=1e8, Pr=1, FP32-deviation storage",
+                   "nodes": r32["nodes"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_node": bpn, "avg_launch_us": round(r32["avg_launch_us"], 3)},
+    }
+    if not args.no_fp64:
+        r64 = gpu_leg(abi.FP64, max(args.steps // 2, 1), args.warmup)
+        a64 = BYTES_PER_NODE["fp64"] * r64["nodes"] / (r64["avg_launch_us"] * 1e-6) / 1e9
+        line["fp64_plain"] = {"mlups": round(r64["mlups"], 2), "avg_launch_us": round(r64["avg_launch_us"], 3),
+                              "roofline_frac": round(a64 / HBM_PEAK_GBS, 4),
+                              "fp32dev_speedup": round(r32["mlups"] / r64["mlups"], 3)}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_leg()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

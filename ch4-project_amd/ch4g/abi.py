@@ -1,0 +1,94 @@
+"""ctypes mirror of include/ch4g.h (structs, flags, precision modes).
+
+Kept field-for-field identical to the C header; tests/test_abi.py checks the
+sizes and offsets against the compiled library.
+"""
+import ctypes as C
+
+NPOP = 19
+HIT_NK = 6
+
+# enum ch4g_flag (reference define.h switch names)
+LB_FLUID = 1 << 0
+LB_TEMPERATURE = 1 << 1
+FLUID_FORCING_POISEUILLE = 1 << 2
+FLUID_FORCING_CHANNEL = 1 << 3
+FLUID_FORCING_HIT = 1 << 4
+TEMPERATURE_BUOYANCY = 1 << 5
+FLUID_FORCING_NOZEROMODE = 1 << 6
+FLUID_BC_Y = 1 << 7
+TEMPERATURE_BC_Y = 1 << 8
+FLUID_LES = 1 << 9
+FLUID_LES_LOCAL_STRAIN = 1 << 10
+TEMPERATURE_BUOYANCY_T0_REF = 1 << 11
+TEMPERATURE_INITIAL_LINEAR = 1 << 12
+TEMPERATURE_INITIAL_ADD_NOISE = 1 << 13
+FLUID_INITIAL_UNIT_DENSITY = 1 << 14
+INITIAL_BAROMETRIC = 1 << 15
+LAGRANGE = 1 << 16
+LAGRANGE_TWOWAY = 1 << 17
+
+FP64 = 0
+FP32_DEV = 1
+FP32_STORE = 2
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("NX", C.c_int32), ("NY", C.c_int32), ("NZ", C.c_int32),
+        ("precision", C.c_int32),
+        ("flags", C.c_uint32),
+        ("seed", C.c_int32),
+        ("tau_u", C.c_double), ("tau_t", C.c_double),
+        ("Amp_x", C.c_double), ("Amp_y", C.c_double), ("Amp_z", C.c_double),
+        ("gravity_x", C.c_double), ("gravity_y", C.c_double), ("gravity_z", C.c_double),
+        ("beta_t", C.c_double), ("beta2_t", C.c_double),
+        ("T_bot", C.c_double), ("T_top", C.c_double), ("T_ref", C.c_double),
+        ("noise_t", C.c_double),
+        ("les_cs", C.c_double),
+        ("rho0", C.c_double), ("T0", C.c_double),
+    ]
+
+    def copy(self, **kw):
+        c = Config()
+        C.pointer(c)[0] = self
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+    def nodes(self):
+        return int(self.NX) * int(self.NY) * int(self.NZ)
+
+    def has(self, flag):
+        return (int(self.flags) & flag) != 0
+
+
+OUTPUT_FIELDS = ["x", "y", "z", "ux", "uy", "uz", "ux2", "uy2", "uz2", "rho", "ene", "eps", "rho2",
+                 "t", "t2", "epst", "dxt", "dyt", "dzt", "uxt", "uyt", "uzt", "nux", "nuy", "nuz", "lb"]
+
+
+class Output(C.Structure):
+    _fields_ = [(n, C.c_double) for n in OUTPUT_FIELDS]
+
+    def as_dict(self):
+        return {n: float(getattr(self, n)) for n in OUTPUT_FIELDS}
+
+
+def make_config(NX, NY, NZ, flags, precision=FP64, **kw):
+    """Config with the reference's defaults (rho0 = 1, T0 = mean wall T)."""
+    c = Config()
+    c.NX, c.NY, c.NZ = NX, NY, NZ
+    c.flags = flags
+    c.precision = precision
+    c.seed = kw.pop("seed", 12345)
+    c.tau_u = kw.pop("tau_u", 1.0)
+    c.tau_t = kw.pop("tau_t", 1.0)
+    c.rho0 = kw.pop("rho0", 1.0)
+    c.les_cs = kw.pop("les_cs", 0.18)
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError(k)
+        setattr(c, k, v)
+    if "T0" not in kw:
+        c.T0 = 0.5 * (c.T_bot + c.T_top)
+    return c

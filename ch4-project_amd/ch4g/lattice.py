@@ -1,0 +1,178 @@
+"""ctypes binding of lib/libch4g.so (include/ch4g.h).
+
+There is no fallback: if the HIP library is missing or no GPU is visible the
+calls raise.  Arrays are numpy float64, shaped [NZ, NY, NX, ...] like the
+reference's interior nodes (x fastest).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import NPOP, Config, Output
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libch4g.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(PKG_ROOT), "include")
+
+_lib = None
+
+# every function include/ch4g.h declares (checked by tests/test_abi.py)
+EXPORTS = [
+    "ch4g_last_error", "ch4g_device_count", "ch4g_create", "ch4g_destroy", "ch4g_initial_conditions",
+    "ch4g_set_populations", "ch4g_get_populations", "ch4g_step", "ch4g_hydro_fields", "ch4g_dump_averages",
+    "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_step_timed", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
+    "ch4g_footprint_bytes",
+]
+
+
+class Ch4gError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Ch4gError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    L.ch4g_last_error.restype = C.c_char_p
+    L.ch4g_device_count.restype = C.c_int
+    L.ch4g_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+    L.ch4g_destroy.argtypes = [P]
+    L.ch4g_destroy.restype = None
+    L.ch4g_initial_conditions.argtypes = [P]
+    L.ch4g_set_populations.argtypes = [P, dp, dp]
+    L.ch4g_get_populations.argtypes = [P, dp, dp]
+    L.ch4g_step.argtypes = [P, C.c_int]
+    L.ch4g_hydro_fields.argtypes = [P, dp, dp, dp]
+    L.ch4g_dump_averages.argtypes = [P, C.POINTER(Output)]
+    L.ch4g_get_hit_phases.argtypes = [P, dp]
+    L.ch4g_set_hit_phases.argtypes = [P, dp]
+    L.ch4g_step_timed.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
+    L.ch4g_set_graphs.argtypes = [P, C.c_int]
+    L.ch4g_synchronize.argtypes = [P]
+    L.ch4g_itime.argtypes = [P]
+    L.ch4g_itime.restype = C.c_int64
+    L.ch4g_device_bytes.argtypes = [P]
+    L.ch4g_device_bytes.restype = C.c_int64
+    L.ch4g_footprint_bytes.argtypes = [C.POINTER(Config)]
+    L.ch4g_footprint_bytes.restype = C.c_int64
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def check(rc):
+    if rc != 0:
+        raise Ch4gError(f"ch4g error {rc}: {lib().ch4g_last_error().decode()}")
+
+
+def device_count():
+    return lib().ch4g_device_count()
+
+
+def footprint_bytes(cfg):
+    return lib().ch4g_footprint_bytes(C.byref(cfg))
+
+
+class Lattice:
+    """A device-resident lattice (one ch4g_lattice)."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self._L = lib()
+        h = C.c_void_p()
+        check(self._L.ch4g_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.shape = (cfg.NZ, cfg.NY, cfg.NX)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ch4g_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def initial_conditions(self):
+        check(self._L.ch4g_initial_conditions(self._h))
+
+    def set_populations(self, p, g=None):
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        g = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+        check(self._L.ch4g_set_populations(self._h, _ptr(p), _ptr(g)))
+
+    def get_populations(self):
+        n = self.cfg.nodes()
+        p = np.zeros((n, NPOP))
+        g = np.zeros((n, NPOP))
+        check(self._L.ch4g_get_populations(self._h, _ptr(p), _ptr(g)))
+        return p.reshape(self.shape + (NPOP,)), g.reshape(self.shape + (NPOP,))
+
+    def step(self, n=1):
+        check(self._L.ch4g_step(self._h, int(n)))
+
+    def step_timed(self, n):
+        """Run n steps; return their device time in ms (HIP events on the
+        lattice stream)."""
+        ms = C.c_double()
+        check(self._L.ch4g_step_timed(self._h, int(n), C.byref(ms)))
+        return ms.value
+
+    def set_graphs(self, enable):
+        check(self._L.ch4g_set_graphs(self._h, 1 if enable else 0))
+
+    def synchronize(self):
+        check(self._L.ch4g_synchronize(self._h))
+
+    @property
+    def itime(self):
+        return self._L.ch4g_itime(self._h)
+
+    @property
+    def device_bytes(self):
+        return self._L.ch4g_device_bytes(self._h)
+
+    def hydro_fields(self):
+        n = self.cfg.nodes()
+        d = np.zeros(n)
+        u = np.zeros((n, 3))
+        t = np.zeros(n)
+        check(self._L.ch4g_hydro_fields(self._h, _ptr(d), _ptr(u), _ptr(t)))
+        s = self.shape
+        return {"dens": d.reshape(s), "u": u.reshape(s + (3,)), "t": t.reshape(s)}
+
+    def dump_averages(self):
+        out = Output()
+        check(self._L.ch4g_dump_averages(self._h, C.byref(out)))
+        return out.as_dict()
+
+    def hit_phases(self):
+        a = np.zeros((6, 3))
+        check(self._L.ch4g_get_hit_phases(self._h, _ptr(a)))
+        return a
+
+    def set_hit_phases(self, phi):
+        a = np.ascontiguousarray(phi, dtype=np.float64).reshape(6, 3)
+        check(self._L.ch4g_set_hit_phases(self._h, _ptr(a)))

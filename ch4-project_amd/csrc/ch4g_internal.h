@@ -1,0 +1,156 @@
+// Internal declarations of the HIP lattice-Boltzmann library (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+#include <utility>
+
+#include "../../include/ch4g.h"
+
+namespace ch4g {
+
+constexpr int Q = 19;
+
+// D3Q19 velocities in the reference's population order (src/lb.c:163-181).
+__host__ __device__ constexpr int cxq(int q)
+{
+    constexpr int v[Q] = {0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+    return v[q];
+}
+__host__ __device__ constexpr int cyq(int q)
+{
+    constexpr int v[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1};
+    return v[q];
+}
+__host__ __device__ constexpr int czq(int q)
+{
+    constexpr int v[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, -1, 1, -1};
+    return v[q];
+}
+// opposite direction (src/lb.c:188-206)
+__host__ __device__ constexpr int invq(int q)
+{
+    constexpr int v[Q] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+    return v[q];
+}
+// weights 1/3, 1/18, 1/36 (src/lb.c:141-159), as the reference writes them
+__host__ __device__ constexpr double wq(int q) { return q == 0 ? 1. / 3. : (q <= 6 ? 1. / 18. : 1. / 36.); }
+
+// HIT forcing: per-step tables of sin(2 pi (k x / L + phi)) for the six modes,
+// layout [6][NX] | [6][NY] | [6][NZ].
+struct HitTable {
+    const double *sx, *sy, *sz;
+};
+
+// Everything a step / gather kernel needs, passed by value.
+struct KParams {
+    int NX, NY, NZ;
+    int64_t N;          // interior nodes
+    int fluid, thermal; // population sets present
+    int wall_f, wall_t; // y walls: fluid no-slip, temperature fixed
+    int forced;         // any fluid forcing (hydro_fields half-force shift + Guo)
+    int first;          // inputs are pre-collision p_0/g_0: no gather
+    int les;            // local-strain Smagorinsky
+    int itime;          // step being computed (1 for the first step)
+    // sound-speed constants computed on the host exactly as the reference does
+    double invcs2, invcs4, invtwocs2, invtwocs4, cs2;
+    double invtau_u, invtau_t, guo_fac, nu, tau_u, Cs2;
+    // forcing
+    double Fc[3];       // constant part (Poiseuille + channel), summed in reference order
+    int hit;
+    double hit_fa[3][CH4G_HIT_NK];  // pow(k^2,-1/3) * Amp_{x,y,z}
+    HitTable hit_prev, hit_now;
+    int buoy;
+    double T0b, beta_t, beta2_t, grav[3];
+    double Tw_top, Tw_bot;  // (T_wall - T_ref)
+    // FP32-deviation base state: per-row profiles rho_b(y), T_b(y).
+    // row32[y*8 + k]: 0 rho_b, 1 T_b, 2 rho_b(y-1)-rho_b(y), 3 rho_b(y+1)-rho_b(y),
+    //                 4 T_b(y-1)-T_b(y), 5 T_b(y+1)-T_b(y), 6 T_b(y)-T0b, 7 unused
+    // (rows y-1, y+1 periodic); row64[y*2 + k]: 0 rho_b, 1 T_b in double.
+    const float *row32;
+    const double *row64;
+    double cw_top, cw_bot;  // (T_wall - T_ref) - T_b(wall row)
+    double rho0, T0;
+    int arith64;            // float storage with double collision arithmetic
+};
+
+// Neighbour coordinates of a node for the pull gather: the source of
+// population q is x - c_q, periodic in every direction; crossings of y walls
+// are flagged (uniform across a wavefront, whose lanes share y and z).
+struct Nbr {
+    int x, y, z;
+    int xm, xp, ym, yp, zm, zp;
+    bool bot, top;  // y == 0, y == NY-1
+};
+
+__device__ __forceinline__ Nbr make_nbr(int x, int y, int z, int NX, int NY, int NZ)
+{
+    Nbr b;
+    b.x = x;
+    b.y = y;
+    b.z = z;
+    b.xm = x == 0 ? NX - 1 : x - 1;
+    b.xp = x == NX - 1 ? 0 : x + 1;
+    b.ym = y == 0 ? NY - 1 : y - 1;
+    b.yp = y == NY - 1 ? 0 : y + 1;
+    b.zm = z == 0 ? NZ - 1 : z - 1;
+    b.zp = z == NZ - 1 ? 0 : z + 1;
+    b.bot = y == 0;
+    b.top = y == NY - 1;
+    return b;
+}
+
+// linear index of the pull source x - c_q (periodic wrap)
+template <int q>
+__device__ __forceinline__ uint32_t src_index(const Nbr &b, int NX, int NY)
+{
+    const int sx = cxq(q) > 0 ? b.xm : (cxq(q) < 0 ? b.xp : b.x);
+    const int sy = cyq(q) > 0 ? b.ym : (cyq(q) < 0 ? b.yp : b.y);
+    const int sz = czq(q) > 0 ? b.zm : (czq(q) < 0 ? b.zp : b.z);
+    return ((uint32_t)sz * (uint32_t)NY + (uint32_t)sy) * (uint32_t)NX + (uint32_t)sx;
+}
+
+// does the pull source of q lie beyond a y wall?
+template <int q>
+__device__ __forceinline__ bool crosses_wall(const Nbr &b)
+{
+    return (cyq(q) > 0 && b.bot) || (cyq(q) < 0 && b.top);
+}
+
+// compile-time loop over q
+template <int q, int Q_, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (q < Q_) {
+        f(std::integral_constant<int, q>{});
+        static_for<q + 1, Q_>(f);
+    }
+}
+
+// Kernel launchers (stream-ordered).
+void launch_step_fp64(const KParams &P, const double *fin, const double *gin, double *fout, double *gout,
+                      double *tlag, double *taules, hipStream_t s);
+void launch_gather_fp64(const KParams &P, const double *fin, const double *gin, const double *tlag,
+                        double *fpost, double *gpost, double *dens, double *u, double *t, hipStream_t s);
+void launch_step_fp32(const KParams &P, const float *fin, const float *gin, float *fout, float *gout, float *tlag,
+                      float *taules, hipStream_t s);
+void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,
+                        double *gpost, double *dens, double *u, double *t, hipStream_t s);
+
+// Initial state from a per-row fluid profile prow[NY][19] and a temperature
+// field t0[N]; restart load from FP64 SoA staging.  dev = FP32-deviation.
+void launch_init(const KParams &P, const double *prow, const double *t0, void *f, void *g, void *tlag, int dev,
+                 hipStream_t s);
+void launch_load(const KParams &P, const double *fs, const double *gs, void *f, void *g, void *tlag, int dev,
+                 hipStream_t s);
+
+// Reductions for dump_averages: FP64 per-block partials then a fixed-order
+// final sum (deterministic for a given lattice size).
+constexpr int NOUT = 26;
+int reduce_blocks(const KParams &P);
+void launch_averages(const KParams &P, const double *dens, const double *u, const double *t, double nu, double kappa,
+                     double *partials, double *result, hipStream_t s);
+
+}  // namespace ch4g

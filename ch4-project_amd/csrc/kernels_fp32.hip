@@ -1,0 +1,399 @@
+// FP32-deviation fused stream+collide kernel and its gather kernel.
+//
+// Storage: df_i = f_i - w_i rho_b(y) and dg_i = g_i - w_i T_b(y) in float,
+// SoA planes [q][z][y][x], where rho_b(y), T_b(y) is a per-row base state
+// chosen on the host (uniform rho0/T0, or the conduction profile with its
+// hydrostatic density for buoyancy-driven runs).  Subtracting the base keeps
+// the stored numbers of the size of the dynamics, so float rounding acts on
+// the signal and not on the O(1) background.  Streaming a population from row
+// y' to row y adds w_i (rho_b(y') - rho_b(y)) (exact algebra; the row
+// differences are tabulated).  Collision arithmetic is float:
+//   drho = sum df,  j = sum c df,  rho = rho_b + drho,  u = j/rho + F_prev/2
+//   df_eq = w (drho + rho (3 c.u + 9/2 (c.u)^2 - 3/2 u^2))
+//   df*   = df - (df - df_eq)/tau + (1 - 1/(2 tau)) w rho (3 (c-u).F + 9 (c.u)(c.F))
+// and the same for the temperature (T = T_b + dT, no forcing term).  Step
+// semantics (lag scalar, wall rules, force lag) are those of kernels_fp64.hip.
+#include "ch4g_internal.h"
+
+namespace ch4g {
+
+namespace {
+
+__device__ __forceinline__ float wf(int q) { return q == 0 ? 1.f / 3.f : (q <= 6 ? 1.f / 18.f : 1.f / 36.f); }
+
+// per-row constants of the base state
+struct Row {
+    float rb, tb, dr_up, dr_dn, dt_up, dt_dn, offT;
+};
+
+__device__ __forceinline__ Row row_of(const KParams &P, int y)
+{
+    const float *r = P.row32 + 8 * y;
+    Row w;
+    w.rb = r[0];
+    w.tb = r[1];
+    w.dr_up = r[2];
+    w.dr_dn = r[3];
+    w.dt_up = r[4];
+    w.dt_dn = r[5];
+    w.offT = r[6];
+    return w;
+}
+
+// force of one node from its temperature deviation dT (buoyancy) and the
+// HIT tables; same term order as build_forcing.
+__device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab, int x, int y, int z, float theta,
+                                           float F[3])
+{
+    F[0] = (float)P.Fc[0];
+    F[1] = (float)P.Fc[1];
+    F[2] = (float)P.Fc[2];
+    if (P.hit) {
+#pragma unroll
+        for (int ii = 0; ii < CH4G_HIT_NK; ii++) {
+            float ax = (float)tab.sx[ii * P.NX + x];
+            float ay = (float)tab.sy[ii * P.NY + y];
+            float az = (float)tab.sz[ii * P.NZ + z];
+            F[0] += (float)P.hit_fa[0][ii] * (ay + az);
+            F[1] += (float)P.hit_fa[1][ii] * (ax + az);
+            F[2] += (float)P.hit_fa[2][ii] * (ay + ax);
+        }
+    }
+    if (P.buoy) {
+        float fac = (float)P.beta_t * theta + (float)P.beta2_t * theta * theta;
+        F[0] += fac * (float)P.grav[0];
+        F[1] += fac * (float)P.grav[1];
+        F[2] += fac * (float)P.grav[2];
+    }
+}
+
+// Streaming of the previous step (pull) with halfway bounce-back at y walls.
+// Branch-free: every load address is computed first (wall crossings are a
+// wave-uniform select), so all 19 loads are in flight together.
+template <bool FIRST>
+__device__ __forceinline__ void gather_f32(const KParams &P, const Row &R, const Nbr &b, const float *__restrict__ in,
+                                           uint32_t n, float f[Q])
+{
+    const size_t N = (size_t)P.N;
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (FIRST) {
+            f[q] = in[(size_t)q * N + n];
+        } else {
+            const bool cross = P.wall_f && crosses_wall<q>(b);
+            const float *base = in + (size_t)(cross ? invq(q) : q) * N;
+            const uint32_t off = cross ? n : src_index<q>(b, P.NX, P.NY);
+            float v = base[off];
+            if constexpr (cyq(q) > 0) v += cross ? 0.f : wf(q) * R.dr_up;
+            if constexpr (cyq(q) < 0) v += cross ? 0.f : wf(q) * R.dr_dn;
+            f[q] = v;
+        }
+    });
+}
+
+// Fixed-T y walls: same-direction population + w 2((T_w - T_ref) - T_lag).
+template <bool FIRST>
+__device__ __forceinline__ void gather_g32(const KParams &P, const Row &R, const Nbr &b, const float *__restrict__ in,
+                                           uint32_t n, float dTlag, float g[Q])
+{
+    const size_t N = (size_t)P.N;
+    const float fac_top = 2.f * ((float)P.cw_top - dTlag);
+    const float fac_bot = 2.f * ((float)P.cw_bot - dTlag);
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (FIRST) {
+            g[q] = in[(size_t)q * N + n];
+        } else {
+            const bool cross = P.wall_t && crosses_wall<q>(b);
+            const uint32_t off = cross ? n : src_index<q>(b, P.NX, P.NY);
+            float v = in[(size_t)q * N + off];
+            if constexpr (cyq(q) > 0) v += cross ? wf(q) * fac_bot : wf(q) * R.dt_up;
+            if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
+            g[q] = v;
+        }
+    });
+}
+
+__device__ __forceinline__ float sum19(const float *p)
+{
+    // pairwise: keeps the float rounding of the moment small
+    float a = (p[1] + p[2]) + (p[3] + p[4]) + (p[5] + p[6]);
+    float b = ((p[7] + p[8]) + (p[9] + p[10])) + ((p[11] + p[12]) + (p[13] + p[14]));
+    float c = (p[15] + p[16]) + (p[17] + p[18]);
+    return p[0] + (a + (b + c));
+}
+
+}  // namespace
+
+// representative q of the k-th (q, inv q) pair
+__host__ __device__ constexpr int pair_q(int k)
+{
+    constexpr int v[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
+    return v[k];
+}
+
+template <typename A>
+__device__ __forceinline__ A wa(int q)
+{
+    return q == 0 ? A(1) / A(3) : (q <= 6 ? A(1) / A(18) : A(1) / A(36));
+}
+
+// A = arithmetic type of the collision (float for CH4G_FP32_DEV, double for
+// CH4G_FP32_STORE); storage is float in both.
+template <bool FIRST, bool FLUID, bool THERMAL, typename A>
+__global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__restrict__ fin,
+                                                   const float *__restrict__ gin, float *__restrict__ fout,
+                                                   float *__restrict__ gout, float *__restrict__ tlag,
+                                                   float *__restrict__ taules)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= P.NX) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    const size_t N = (size_t)P.N;
+    const Row R = row_of(P, y);
+    const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
+
+    float dTlag = 0.f;
+    float g[Q];
+    if constexpr (THERMAL) {
+        dTlag = tlag[n];
+        gather_g32<FIRST>(P, R, b, gin, n, dTlag, g);
+    }
+    float f[Q];
+    if constexpr (FLUID) gather_f32<FIRST>(P, R, b, fin, n, f);
+    A dT = A(0);
+    if constexpr (THERMAL) {
+        if constexpr (sizeof(A) == 4) {
+            dT = sum19(g);
+        } else {
+            for (int q = 0; q < Q; q++) dT += (A)g[q];
+        }
+    }
+
+    A ux = 0, uy = 0, uz = 0;
+    if constexpr (FLUID) {
+        A fa[Q];
+        for (int q = 0; q < Q; q++) fa[q] = (A)f[q];
+        A drho;
+        if constexpr (sizeof(A) == 4) {
+            drho = sum19(f);
+        } else {
+            drho = 0;
+            for (int q = 0; q < Q; q++) drho += fa[q];
+        }
+        const A rho = (sizeof(A) == 4 ? (A)R.rb : (A)P.row64[2 * y]) + drho;
+        // momentum from (q, inv q) differences: exact for nearby values
+        const A jx = (fa[1] - fa[2]) + (fa[7] - fa[10]) + (fa[8] - fa[9]) + (fa[11] - fa[14]) + (fa[13] - fa[12]);
+        const A jy = (fa[3] - fa[4]) + (fa[7] - fa[10]) + (fa[9] - fa[8]) + (fa[15] - fa[18]) + (fa[16] - fa[17]);
+        const A jz = (fa[5] - fa[6]) + (fa[11] - fa[14]) + (fa[12] - fa[13]) + (fa[15] - fa[18]) + (fa[17] - fa[16]);
+        const A irho = A(1) / rho;
+        ux = jx * irho;
+        uy = jy * irho;
+        uz = jz * irho;
+        if (P.forced) {
+            float Fp[3];
+            force_at32(P, P.hit_prev, x, y, z, R.offT + dTlag, Fp);
+            ux += A(0.5) * Fp[0];
+            uy += A(0.5) * Fp[1];
+            uz += A(0.5) * Fp[2];
+        }
+        const A u2 = ux * ux + uy * uy + uz * uz;
+        // weight-class products (w rho, w drho) shared by the populations
+        const A wr[3] = {wa<A>(0) * rho, wa<A>(1) * rho, wa<A>(7) * rho};
+        const A wd[3] = {wa<A>(0) * drho, wa<A>(1) * drho, wa<A>(7) * drho};
+        A omega = (A)P.invtau_u;
+        if (P.les) {
+            A xx = 0, yy = 0, xy = 0, xz = 0, yz = 0;
+#pragma unroll
+            for (int q = 1; q < Q; q++) {
+                const int wc = q <= 6 ? 1 : 2;
+                const A cx = cxq(q), cy = cyq(q), cz = czq(q);
+                const A cu = cx * ux + cy * uy + cz * uz;
+                const A eq = fma(wr[wc], A(3) * cu + A(4.5) * cu * cu - A(1.5) * u2, wd[wc]);
+                const A d = fa[q] - eq;
+                xx += cx * cx * d;
+                yy += cy * cy * d;
+                xy += cx * cy * d;
+                xz += cx * cz * d;
+                yz += cy * cz * d;
+            }
+            // stress_tensor of gradients.c:65-67: S.zz takes S.xz, S.zx stays 0
+            const A ss = xx * xx + A(2) * xy * xy + xz * xz + yy * yy + A(2) * yz * yz + xz * xz;
+            A tau_prev = (P.itime == 1) ? (A)P.tau_u : (A)taules[n];
+            A fac = A(1) / (A(2) * (A)P.cs2 * rho * tau_prev);
+            A tau = A(3) * (fac * sqrt(A(2) * ss) * (A)P.Cs2 + (A)P.nu) + A(0.5);
+            taules[n] = (float)tau;
+            omega = A(1) / tau;
+        }
+        float Fn[3] = {0.f, 0.f, 0.f};
+        if (P.forced) force_at32(P, P.hit_now, x, y, z, R.offT + (float)dT, Fn);
+        const A uF = ux * Fn[0] + uy * Fn[1] + uz * Fn[2];
+        // Guo term folded into the equilibrium: f* = f - omega (f - eqG) with
+        // eqG = f_eq + G/omega, so the O(population) value is rounded twice
+        const A kg = (A)P.guo_fac / omega;
+        A r[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int wc = q == 0 ? 0 : (q <= 6 ? 1 : 2);
+            const A cx = cxq(q), cy = cyq(q), cz = czq(q);
+            const A cu = cx * ux + cy * uy + cz * uz;
+            A poly = A(3) * cu + A(4.5) * cu * cu - A(1.5) * u2;
+            if (P.forced) {
+                const A cF = cx * Fn[0] + cy * Fn[1] + cz * Fn[2];
+                poly += kg * (A(3) * (cF - uF) + A(9) * cu * cF);
+            }
+            const A eq = fma(wr[wc], poly, wd[wc]);
+            r[q] = fma(-omega, fa[q] - eq, fa[q]);
+        }
+        if constexpr (sizeof(A) == 4) {
+            // Float weights miss the lattice identities (18 fl(1/18) != 1, ...)
+            // by ~1e-8, which would scale mass and momentum systematically at
+            // every step.  Restore the exact collision invariants: mass is
+            // conserved; momentum becomes j + omega rho F_prev/2 + guo rho F,
+            // both targets evaluated at velocity magnitude.
+            A Fp2[3] = {A(0), A(0), A(0)};
+            if (P.forced) {
+                Fp2[0] = (ux - jx * irho) * rho;  // = rho F_prev / 2
+                Fp2[1] = (uy - jy * irho) * rho;
+                Fp2[2] = (uz - jz * irho) * rho;
+            }
+            const A tx = jx + omega * Fp2[0] + (A)P.guo_fac * rho * (A)Fn[0];
+            const A ty = jy + omega * Fp2[1] + (A)P.guo_fac * rho * (A)Fn[1];
+            const A tz = jz + omega * Fp2[2] + (A)P.guo_fac * rho * (A)Fn[2];
+            const A mx = (r[1] - r[2]) + (r[7] - r[10]) + (r[8] - r[9]) + (r[11] - r[14]) + (r[13] - r[12]);
+            const A my = (r[3] - r[4]) + (r[7] - r[10]) + (r[9] - r[8]) + (r[15] - r[18]) + (r[16] - r[17]);
+            const A mz = (r[5] - r[6]) + (r[11] - r[14]) + (r[12] - r[13]) + (r[15] - r[18]) + (r[17] - r[16]);
+            const A hx = A(0.5) * (tx - mx), hy = A(0.5) * (ty - my), hz = A(0.5) * (tz - mz);
+            r[1] += hx;
+            r[2] -= hx;
+            r[3] += hy;
+            r[4] -= hy;
+            r[5] += hz;
+            r[6] -= hz;
+            r[0] += drho - sum19(r);
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) fout[(size_t)q * N + n] = (float)r[q];
+    }
+
+    if constexpr (THERMAL) {
+        const A u2 = ux * ux + uy * uy + uz * uz;
+        const A T = (sizeof(A) == 4 ? (A)R.tb : (A)P.row64[2 * y + 1]) + dT;
+        const A om = (A)P.invtau_t;
+        const A wT[3] = {wa<A>(0) * T, wa<A>(1) * T, wa<A>(7) * T};
+        const A wdT[3] = {wa<A>(0) * dT, wa<A>(1) * dT, wa<A>(7) * dT};
+        A r[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int wc = q == 0 ? 0 : (q <= 6 ? 1 : 2);
+            const A cu = cxq(q) * ux + cyq(q) * uy + czq(q) * uz;
+            const A eq = fma(wT[wc], A(3) * cu + A(4.5) * cu * cu - A(1.5) * u2, wdT[wc]);
+            const A gq = (A)g[q];
+            r[q] = fma(-om, gq - eq, gq);
+        }
+        // the collision conserves the temperature exactly (float weights do not)
+        if constexpr (sizeof(A) == 4) r[0] += dT - sum19(r);
+#pragma unroll
+        for (int q = 0; q < Q; q++) gout[(size_t)q * N + n] = (float)r[q];
+        tlag[n] = (float)dT;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__restrict__ fin,
+                                                     const float *__restrict__ gin, const float *__restrict__ tlag,
+                                                     double *__restrict__ fpost, double *__restrict__ gpost,
+                                                     double *__restrict__ dens, double *__restrict__ u,
+                                                     double *__restrict__ t)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= P.NX) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    const size_t N = (size_t)P.N;
+    const Row R = row_of(P, y);
+    const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
+    const double rb = P.row64[2 * y], tb = P.row64[2 * y + 1];
+    float dTlag = 0.f;
+    if (P.thermal) {
+        float g[Q];
+        dTlag = tlag[n];
+        if (P.first)
+            gather_g32<true>(P, R, b, gin, n, dTlag, g);
+        else
+            gather_g32<false>(P, R, b, gin, n, dTlag, g);
+        double s = 0.0;
+        for (int q = 0; q < Q; q++) {
+            s += (double)g[q];
+            if (gpost) gpost[q * N + n] = wq(q) * tb + (double)g[q];
+        }
+        if (t) t[n] = tb + s;
+    }
+    if (P.fluid) {
+        float f[Q];
+        if (P.first)
+            gather_f32<true>(P, R, b, fin, n, f);
+        else
+            gather_f32<false>(P, R, b, fin, n, f);
+        double drho = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+        for (int q = 0; q < Q; q++) {
+            double d = (double)f[q];
+            drho += d;
+            jx += cxq(q) * d;
+            jy += cyq(q) * d;
+            jz += czq(q) * d;
+            if (fpost) fpost[q * N + n] = wq(q) * rb + d;
+        }
+        double rho = rb + drho;
+        double ux = jx / rho, uy = jy / rho, uz = jz / rho;
+        if (P.forced) {
+            float F[3];
+            force_at32(P, P.hit_now, x, y, z, R.offT + dTlag, F);
+            ux += 0.5 * (double)F[0];
+            uy += 0.5 * (double)F[1];
+            uz += 0.5 * (double)F[2];
+        }
+        if (dens) dens[n] = rho;
+        if (u) {
+            u[3 * n] = ux;
+            u[3 * n + 1] = uy;
+            u[3 * n + 2] = uz;
+        }
+    }
+}
+
+static int block_x32(const KParams &P) { return P.NX >= 256 ? 256 : (P.NX >= 128 ? 128 : 64); }
+
+void launch_step_fp32(const KParams &P, const float *fin, const float *gin, float *fout, float *gout, float *tlag,
+                      float *taules, hipStream_t s)
+{
+    int bx = block_x32(P);
+    dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ);
+#define CH4G_LAUNCH32(F_, FL_, TH_)                                                                                  \
+    do {                                                                                                              \
+        if (P.arith64)                                                                                                \
+            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, double>), grid, dim3(bx), 0, s, P, fin, gin, fout, gout,   \
+                               tlag, taules);                                                                         \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, float>), grid, dim3(bx), 0, s, P, fin, gin, fout, gout,    \
+                               tlag, taules);                                                                         \
+    } while (0)
+    if (P.fluid && P.thermal) {
+        if (P.first) CH4G_LAUNCH32(true, true, true); else CH4G_LAUNCH32(false, true, true);
+    } else if (P.fluid) {
+        if (P.first) CH4G_LAUNCH32(true, true, false); else CH4G_LAUNCH32(false, true, false);
+    } else {
+        if (P.first) CH4G_LAUNCH32(true, false, true); else CH4G_LAUNCH32(false, false, true);
+    }
+#undef CH4G_LAUNCH32
+}
+
+void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,
+                        double *gpost, double *dens, double *u, double *t, hipStream_t s)
+{
+    int bx = block_x32(P);
+    dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ);
+    hipLaunchKernelGGL(k_gather_fp32, grid, dim3(bx), 0, s, P, fin, gin, tlag, fpost, gpost, dens, u, t);
+}
+
+}  // namespace ch4g

@@ -1,0 +1,314 @@
+// FP64 fused stream+collide kernel and gather (hydro_fields) kernel.
+//
+// Arithmetic is written in the reference's evaluation order and this file is
+// compiled with -ffp-contract=off (the reference builds with -std=c11, which
+// disables contraction), so every field is bit-identical to the reference's
+// FP64 step on the same inputs.
+//
+// State between steps (pull scheme): the post-collision populations of the
+// last step (the reference's rhs_p/rhs_g after add_forcing), plus one lag
+// scalar per node, tlag = the temperature of the previous step.  The kernel
+// for step s first performs the streaming of step s-1 (lb.c:544-583) with the
+// wall rules of boundary_conditions.c applied as a gather, reconstructs
+// rho, u, T of step s-1 (lb.c:420-540; the u shift uses the force built from
+// the lagged temperature, as main.c's ordering implies), then collides
+// (fluid.c:1033-1225) and adds the Guo term (forcing.c:1424-1435) of step s.
+#include "ch4g_internal.h"
+
+namespace ch4g {
+
+// typedef.h:286-291, strict left-to-right sums
+__device__ __forceinline__ double m_sum(const double *p)
+{
+    return p[0] + p[1] + p[2] + p[3] + p[4] + p[5] + p[6] + p[7] + p[8] + p[9] + p[10] + p[11] + p[12] + p[13] +
+           p[14] + p[15] + p[16] + p[17] + p[18];
+}
+__device__ __forceinline__ double m_x(const double *p)
+{
+    return p[1] - p[2] + p[7] + p[8] - p[9] - p[10] + p[11] - p[12] + p[13] - p[14];
+}
+__device__ __forceinline__ double m_y(const double *p)
+{
+    return p[3] - p[4] + p[7] - p[8] + p[9] - p[10] + p[15] + p[16] - p[17] - p[18];
+}
+__device__ __forceinline__ double m_z(const double *p)
+{
+    return p[5] - p[6] + p[11] + p[12] - p[13] - p[14] + p[15] - p[16] + p[17] - p[18];
+}
+
+// build_forcing for one node (forcing.c:361-667), terms added in source order.
+__device__ __forceinline__ void force_at(const KParams &P, const HitTable &tab, int x, int y, int z, double T,
+                                         double F[3])
+{
+    F[0] = P.Fc[0];
+    F[1] = P.Fc[1];
+    F[2] = P.Fc[2];
+    if (P.hit) {
+#pragma unroll
+        for (int ii = 0; ii < CH4G_HIT_NK; ii++) {
+            double ax = tab.sx[ii * P.NX + x];
+            double ay = tab.sy[ii * P.NY + y];
+            double az = tab.sz[ii * P.NZ + z];
+            F[0] += P.hit_fa[0][ii] * (ay + az);
+            F[1] += P.hit_fa[1][ii] * (ax + az);
+            F[2] += P.hit_fa[2][ii] * (ay + ax);
+        }
+    }
+    if (P.buoy) {
+        double temp = T - P.T0b;
+        double fac = P.beta_t * temp;
+        if (P.beta2_t != 0.0) fac += P.beta2_t * temp * temp;
+        F[0] += fac * P.grav[0];
+        F[1] += fac * P.grav[1];
+        F[2] += fac * P.grav[2];
+    }
+}
+
+// Streaming of the previous step as a pull, with halfway bounce-back at y
+// walls (boundary_conditions.c:714-873: the ghost receives the node's own
+// post-collision population of the opposite direction).  Branch-free so all
+// loads are in flight together.
+template <bool FIRST>
+__device__ __forceinline__ void gather_fluid(const KParams &P, const Nbr &b, const double *__restrict__ in,
+                                             uint32_t n, double f[Q])
+{
+    const size_t N = (size_t)P.N;
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (FIRST) {
+            f[q] = in[(size_t)q * N + n];
+        } else {
+            const bool cross = P.wall_f && crosses_wall<q>(b);
+            const double *base = in + (size_t)(cross ? invq(q) : q) * N;
+            const uint32_t off = cross ? n : src_index<q>(b, P.NX, P.NY);
+            f[q] = base[off];
+        }
+    });
+}
+
+// Fixed-temperature y walls (boundary_conditions.c:966-1078): the ghost
+// receives the node's own post-collision population of the SAME direction
+// plus w_q * 2((T_wall - T_ref) - t), t being the node temperature of the
+// previous hydro_fields call (the lag scalar).
+template <bool FIRST>
+__device__ __forceinline__ void gather_thermal(const KParams &P, const Nbr &b, const double *__restrict__ in,
+                                               uint32_t n, double Tlag, double g[Q])
+{
+    const size_t N = (size_t)P.N;
+    const double fac_top = 2.0 * (P.Tw_top - Tlag);
+    const double fac_bot = 2.0 * (P.Tw_bot - Tlag);
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (FIRST) {
+            g[q] = in[(size_t)q * N + n];
+        } else {
+            const bool cross = P.wall_t && crosses_wall<q>(b);
+            const uint32_t off = cross ? n : src_index<q>(b, P.NX, P.NY);
+            const double v = in[(size_t)q * N + off];
+            if constexpr (cyq(q) > 0)
+                g[q] = cross ? v + wq(q) * fac_bot : v;
+            else if constexpr (cyq(q) < 0)
+                g[q] = cross ? v + wq(q) * fac_top : v;
+            else
+                g[q] = v;
+        }
+    });
+}
+
+// equilibrium (lb.c:382-415)
+__device__ __forceinline__ double feq_q(const KParams &P, int q, double rho, double ux, double uy, double uz,
+                                        double u2)
+{
+    double cu = ((double)cxq(q) * ux + (double)cyq(q) * uy + (double)czq(q) * uz);
+    return rho * wq(q) * (1.0 + P.invcs2 * cu + P.invtwocs4 * cu * cu - P.invtwocs2 * u2);
+}
+
+// tau_u_les, local-strain branch (les.c:20-30,:144-180,:253) with the
+// stress_tensor assignments of gradients.c:42-69.
+__device__ __forceinline__ double les_tau(const KParams &P, const double f[Q], double rho, double ux, double uy,
+                                          double uz, double u2, double tau_prev)
+{
+    double xx = 0.0, yy = 0.0, zz = 0.0, xy = 0.0, xz = 0.0, yz = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        double d = f[q] - feq_q(P, q, rho, ux, uy, uz, u2);
+        const double cx = cxq(q), cy = cyq(q), cz = czq(q);
+        xx += cx * cx * d;
+        yy += cy * cy * d;
+        zz += cz * cz * d;
+        xy += cx * cy * d;
+        xz += cx * cz * d;
+        yz += cy * cz * d;
+    }
+    (void)zz;
+    const double yx = xy, zx = 0.0, zy = yz;
+    zz = xz;
+    double fac = -1.0 / (2 * P.cs2 * rho * tau_prev * 1.0);
+    double s_norm = fabs(fac) * sqrt(2.0 * (xx * xx + xy * xy + xz * xz + yx * yx + yy * yy + yz * yz + zx * zx +
+                                            zy * zy + zz * zz));
+    double nu_les = s_norm * P.Cs2;
+    return 3.0 * (nu_les + P.nu) + 0.5;
+}
+
+template <bool FIRST, bool FLUID, bool THERMAL>
+__global__ void __launch_bounds__(256) k_step_fp64(KParams P, const double *__restrict__ fin,
+                                                   const double *__restrict__ gin, double *__restrict__ fout,
+                                                   double *__restrict__ gout, double *__restrict__ tlag,
+                                                   double *__restrict__ taules)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= P.NX) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    const size_t N = (size_t)P.N;
+    const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
+
+    double Tlag = 0.0, T = 0.0;
+    double g[Q], f[Q];
+    if constexpr (THERMAL) {
+        Tlag = tlag[n];
+        gather_thermal<FIRST>(P, b, gin, n, Tlag, g);
+    }
+    if constexpr (FLUID) gather_fluid<FIRST>(P, b, fin, n, f);
+    if constexpr (THERMAL) T = m_sum(g);
+
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    if constexpr (FLUID) {
+        const double rho = m_sum(f);
+        if (P.forced) {
+            double Fp[3];
+            force_at(P, P.hit_prev, x, y, z, Tlag, Fp);
+            ux = m_x(f) / rho + 0.5 * 1.0 * Fp[0];
+            uy = m_y(f) / rho + 0.5 * 1.0 * Fp[1];
+            uz = m_z(f) / rho + 0.5 * 1.0 * Fp[2];
+        } else {
+            ux = m_x(f) / rho;
+            uy = m_y(f) / rho;
+            uz = m_z(f) / rho;
+        }
+        const double u2 = (ux * ux + uy * uy + uz * uz);
+        double invtau = P.invtau_u;
+        if (P.les) {
+            double tau_prev = (P.itime == 1) ? P.tau_u : taules[n];
+            double tau = les_tau(P, f, rho, ux, uy, uz, u2, tau_prev);
+            taules[n] = tau;
+            invtau = 1.0 / tau;
+        }
+        double Fn[3] = {0.0, 0.0, 0.0};
+        if (P.forced) force_at(P, P.hit_now, x, y, z, T, Fn);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            double r = f[q];
+            double fcoll = -invtau * (f[q] - feq_q(P, q, rho, ux, uy, uz, u2));
+            r += fcoll;
+            if (P.forced) {
+                const double cx = cxq(q), cy = cyq(q), cz = czq(q);
+                double cu = (cx * ux + cy * uy + cz * uz);
+                double dx = (cx - ux) * P.invcs2 + cx * cu * P.invcs4;
+                double dy = (cy - uy) * P.invcs2 + cy * cu * P.invcs4;
+                double dz = (cz - uz) * P.invcs2 + cz * cu * P.invcs4;
+                r += P.guo_fac * wq(q) * rho * Fn[0] * dx;
+                r += P.guo_fac * wq(q) * rho * Fn[1] * dy;
+                r += P.guo_fac * wq(q) * rho * Fn[2] * dz;
+            }
+            fout[(size_t)q * N + n] = r;
+        }
+    }
+
+    if constexpr (THERMAL) {
+        const double u2 = (ux * ux + uy * uy + uz * uz);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            double r = g[q];
+            r += -P.invtau_t * (g[q] - feq_q(P, q, T, ux, uy, uz, u2));
+            gout[(size_t)q * N + n] = r;
+        }
+        tlag[n] = T;
+    }
+}
+
+// Streaming of the last step + hydro_fields, without a new collision:
+// produces the reference's p, g (AoS order not applied here: SoA planes),
+// dens, u and t of the current step.  Any output may be null.
+__global__ void __launch_bounds__(256) k_gather_fp64(KParams P, const double *__restrict__ fin,
+                                                     const double *__restrict__ gin,
+                                                     const double *__restrict__ tlag, double *__restrict__ fpost,
+                                                     double *__restrict__ gpost, double *__restrict__ dens,
+                                                     double *__restrict__ u, double *__restrict__ t)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= P.NX) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    const size_t N = (size_t)P.N;
+    const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
+    double Tlag = 0.0;
+    if (P.thermal) {
+        double g[Q];
+        Tlag = tlag[n];
+        if (P.first)
+            gather_thermal<true>(P, b, gin, n, Tlag, g);
+        else
+            gather_thermal<false>(P, b, gin, n, Tlag, g);
+        if (gpost)
+            for (int q = 0; q < Q; q++) gpost[q * N + n] = g[q];
+        if (t) t[n] = m_sum(g);
+    }
+    if (P.fluid) {
+        double f[Q];
+        if (P.first)
+            gather_fluid<true>(P, b, fin, n, f);
+        else
+            gather_fluid<false>(P, b, fin, n, f);
+        if (fpost)
+            for (int q = 0; q < Q; q++) fpost[q * N + n] = f[q];
+        const double rho = m_sum(f);
+        double ux, uy, uz;
+        if (P.forced) {
+            double F[3];
+            force_at(P, P.hit_now, x, y, z, Tlag, F);
+            ux = m_x(f) / rho + 0.5 * 1.0 * F[0];
+            uy = m_y(f) / rho + 0.5 * 1.0 * F[1];
+            uz = m_z(f) / rho + 0.5 * 1.0 * F[2];
+        } else {
+            ux = m_x(f) / rho;
+            uy = m_y(f) / rho;
+            uz = m_z(f) / rho;
+        }
+        if (dens) dens[n] = rho;
+        if (u) {
+            u[3 * n] = ux;
+            u[3 * n + 1] = uy;
+            u[3 * n + 2] = uz;
+        }
+    }
+}
+
+static dim3 grid_of(const KParams &P, int bx) { return dim3((P.NX + bx - 1) / bx, P.NY, P.NZ); }
+static int block_x(const KParams &P) { return P.NX >= 256 ? 256 : (P.NX >= 128 ? 128 : 64); }
+
+void launch_step_fp64(const KParams &P, const double *fin, const double *gin, double *fout, double *gout,
+                      double *tlag, double *taules, hipStream_t s)
+{
+    int bx = block_x(P);
+    dim3 grid = grid_of(P, bx);
+#define CH4G_LAUNCH64(F_, FL_, TH_) \
+    hipLaunchKernelGGL((k_step_fp64<F_, FL_, TH_>), grid, dim3(bx), 0, s, P, fin, gin, fout, gout, tlag, taules)
+    if (P.fluid && P.thermal) {
+        if (P.first) CH4G_LAUNCH64(true, true, true); else CH4G_LAUNCH64(false, true, true);
+    } else if (P.fluid) {
+        if (P.first) CH4G_LAUNCH64(true, true, false); else CH4G_LAUNCH64(false, true, false);
+    } else {
+        if (P.first) CH4G_LAUNCH64(true, false, true); else CH4G_LAUNCH64(false, false, true);
+    }
+#undef CH4G_LAUNCH64
+}
+
+void launch_gather_fp64(const KParams &P, const double *fin, const double *gin, const double *tlag, double *fpost,
+                        double *gpost, double *dens, double *u, double *t, hipStream_t s)
+{
+    int bx = block_x(P);
+    hipLaunchKernelGGL(k_gather_fp64, grid_of(P, bx), dim3(bx), 0, s, P, fin, gin, tlag, fpost, gpost, dens, u, t);
+}
+
+}  // namespace ch4g

@@ -1,0 +1,184 @@
+/*
+ * ch4g.h — C-ABI of the MI355X lattice-Boltzmann time step.
+ *
+ * One lattice = one device-resident D3Q19 fluid (+ optional D3Q19 temperature)
+ * population set on a single GPU, advanced by fused HIP kernels.  Every entry
+ * point takes plain pointers and sizes, returns 0 on success and a negative
+ * status on failure (message via ch4g_last_error()).  All calls for one lattice
+ * must come from one host thread; work is ordered on the lattice's own stream.
+ *
+ * The reference (ch4-project, C/MPI) has no plugin API: its boundary is the set
+ * of global-state functions declared in src/functions.h and called in a fixed
+ * order by src/main.c:67-196.  Each entry point below names the reference
+ * function(s) it replaces; include/ch4g_dropin.h re-exports the reference names
+ * themselves on top of this header.
+ *
+ * Coordinates follow the reference: x is the fastest index
+ * (src/typedef.h:342 IDX), y is the wall-normal / vertical axis of channel and
+ * Rayleigh-Benard runs (src/forcing.c:665, src/output.c:338), cell centres sit
+ * at i+1/2 (src/grid.c:183-185).
+ *
+ * Host population buffers are AoS, interior nodes only, layout
+ * [NZ][NY][NX][19] (x fastest), population order and velocities of
+ * src/lb.c:139-181 (D3Q19).
+ */
+#ifndef CH4G_H
+#define CH4G_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CH4G_NPOP 19
+#define CH4G_HIT_NK 6
+
+/* Compile-time switches of the reference (src/define.h), carried at run time.
+ * Names keep the reference's macro names without the LB_ prefix.          */
+enum ch4g_flag {
+    CH4G_LB_FLUID                      = 1u << 0,  /* define.h LB_FLUID */
+    CH4G_LB_TEMPERATURE                = 1u << 1,  /* LB_TEMPERATURE */
+    CH4G_FLUID_FORCING_POISEUILLE      = 1u << 2,  /* forcing.c:366-371 */
+    CH4G_FLUID_FORCING_CHANNEL         = 1u << 3,  /* forcing.c:374-379 */
+    CH4G_FLUID_FORCING_HIT             = 1u << 4,  /* forcing.c:530-592 default branch */
+    CH4G_TEMPERATURE_BUOYANCY          = 1u << 5,  /* forcing.c:620-667 */
+    CH4G_FLUID_FORCING_NOZEROMODE      = 1u << 6,  /* forcing.c:1123-1152 */
+    CH4G_FLUID_BC_Y                    = 1u << 7,  /* no-slip walls at y, boundary_conditions.c:714-873 */
+    CH4G_TEMPERATURE_BC_Y              = 1u << 8,  /* fixed-T walls at y, boundary_conditions.c:966-1078 */
+    CH4G_FLUID_LES                     = 1u << 9,  /* Smagorinsky, les.c:5-269 */
+    CH4G_FLUID_LES_LOCAL_STRAIN        = 1u << 10, /* |S| from the local stress, les.c:20-30 */
+    CH4G_TEMPERATURE_BUOYANCY_T0_REF   = 1u << 11, /* theta = t - T_ref, forcing.c:623-624 */
+    CH4G_TEMPERATURE_INITIAL_LINEAR    = 1u << 12, /* initial_conditions.c:303-308 */
+    CH4G_TEMPERATURE_INITIAL_ADD_NOISE = 1u << 13, /* initial_conditions.c:421-424 */
+    CH4G_FLUID_INITIAL_UNIT_DENSITY    = 1u << 14, /* initial_conditions.c:256-279 */
+    CH4G_INITIAL_BAROMETRIC            = 1u << 15, /* initial_conditions.c:72-84 */
+    CH4G_LAGRANGE                      = 1u << 16, /* point particles, lagrange.c */
+    CH4G_LAGRANGE_TWOWAY               = 1u << 17  /* particle feedback, lagrange_feedback.c */
+};
+
+/* Storage / arithmetic of the populations.  Reductions (Nusselt, energy,
+ * profiles) are always accumulated in FP64.                                  */
+enum ch4g_precision {
+    CH4G_FP64     = 0,  /* f_i stored and collided in double (reference my_double) */
+    CH4G_FP32_DEV = 1,  /* f_i - w_i*rho_b and g_i - w_i*T_b stored and collided in float */
+    CH4G_FP32_STORE = 2 /* same float deviation storage, collision arithmetic in double */
+};
+
+/* Run parameters: the subset of the reference `property` struct
+ * (src/typedef.h:75-224, read by parameters.c:41) used on the streaming step,
+ * plus the run-time flag word above.  Lattice spacing and time step are 1
+ * (parameters.c:135 requires time_dt == SX/NX under METHOD_STREAMING).      */
+typedef struct ch4g_config {
+    int32_t  NX, NY, NZ;
+    int32_t  precision;        /* enum ch4g_precision */
+    uint32_t flags;            /* OR of enum ch4g_flag */
+    int32_t  seed;             /* srand48 seed (reference: rank + time(NULL), parallel.c:238-240) */
+    double   tau_u, tau_t;     /* relaxation times; nu=(tau_u-1/2)/3, kappa=(tau_t-1/2)/3 (parameters.c:179,285) */
+    double   Amp_x, Amp_y, Amp_z;
+    double   gravity_x, gravity_y, gravity_z;
+    double   beta_t, beta2_t;
+    double   T_bot, T_top, T_ref;
+    double   noise_t;          /* initial temperature noise amplitude */
+    double   les_cs;           /* Smagorinsky constant (reference hard-codes 0.18, les.c:10) */
+    double   rho0, T0;         /* base state subtracted in CH4G_FP32_DEV storage */
+} ch4g_config;
+
+/* Volume averages written by dump_averages (src/typedef.h:233-258 `output`,
+ * fluid + temperature members, same order).  nux/nuy/nuz are Nusselt numbers
+ * normalised by kappa*deltaT/SY (output.c:577-579).                          */
+typedef struct ch4g_output {
+    double x, y, z;
+    double ux, uy, uz;
+    double ux2, uy2, uz2;
+    double rho, ene, eps;
+    double rho2;
+    double t, t2, epst;
+    double dxt, dyt, dzt;
+    double uxt, uyt, uzt;
+    double nux, nuy, nuz;
+    double lb;
+} ch4g_output;
+
+typedef struct ch4g_lattice ch4g_lattice;
+
+/* Last error message of the calling thread ("" if none). */
+const char *ch4g_last_error(void);
+
+/* Number of GPUs visible to the library (0 if none). */
+int ch4g_device_count(void);
+
+/* Allocate a device-resident lattice on the current device.
+ * Replaces allocate_fields (parameters.c:917) + design_lb (lb.c:3).
+ * Fails with a negative status for unsupported flag combinations.           */
+int ch4g_create(const ch4g_config *cfg, ch4g_lattice **out);
+
+/* Release all device memory of the lattice.  Replaces free_fields. */
+void ch4g_destroy(ch4g_lattice *lat);
+
+/* Build the initial state on the device from the reference's initial
+ * conditions: p = w_i (initial_conditions.c:45, optional unit-density
+ * renormalisation :256-279, barometric :72-84), T linear + uniform noise drawn
+ * with drand48 seeded by cfg->seed in k,j,i order (:303-308, :421-424),
+ * g = w_i T (:486-487); then the first hydro_fields/build_forcing/hydro_fields
+ * sequence of main.c:26-48.  HIT phases are drawn after the temperature field
+ * (initialization_forcing, initial_conditions.c:866-871).                   */
+int ch4g_initial_conditions(ch4g_lattice *lat);
+
+/* Load / read back the pre-collision populations p (and g) of the current
+ * step, AoS [NZ][NY][NX][19].  g may be NULL without LB_TEMPERATURE.
+ * set = restart from a population dump (read_pop_h5, output_h5.c:540);
+ * get = write_pop_h5 (output_h5.c:347) payload.  `set` recomputes
+ * hydro_fields/build_forcing/hydro_fields as main.c:26-48 does.            */
+int ch4g_set_populations(ch4g_lattice *lat, const double *p, const double *g);
+int ch4g_get_populations(ch4g_lattice *lat, double *p, double *g);
+
+/* Advance nsteps time steps of main.c:92-161 (compute_advection, add_collision
+ * x2, build_forcing, add_forcing, boundary_and_pbc_conditions_for_streaming,
+ * time_stepping x2, hydro_fields x2) with one fused kernel per step.
+ * Asynchronous on the lattice stream.                                        */
+int ch4g_step(ch4g_lattice *lat, int nsteps);
+
+/* hydro_fields (lb.c:420-540): density, velocity (with the Guo half-force
+ * shift) and temperature of the current step, [NZ][NY][NX] (u: x,y,z
+ * interleaved).  Any pointer may be NULL.  Synchronous.                     */
+int ch4g_hydro_fields(ch4g_lattice *lat, double *dens, double *u, double *t);
+
+/* dump_averages (output.c:63-620): FP64 volume averages of the current step
+ * including the Nusselt numbers.  Synchronous.                               */
+int ch4g_dump_averages(ch4g_lattice *lat, ch4g_output *out);
+
+/* HIT forcing phases phi_u[6][3] of the current step (forcing.c:49-79).
+ * get/set let a host that owns the random stream drive the phase walk.      */
+int ch4g_get_hit_phases(ch4g_lattice *lat, double *phi);
+int ch4g_set_hit_phases(ch4g_lattice *lat, const double *phi);
+
+/* ch4g_step bracketed by HIP events recorded on the lattice stream; waits
+ * for the steps and returns their device time in *elapsed_ms.  Replaces the
+ * MPI_Wtime bracket of main.c:54-56,198-201 and measure_time
+ * (parallel.c:618-638, run_time.dat "time per step and grid point").      */
+int ch4g_step_timed(ch4g_lattice *lat, int nsteps, double *elapsed_ms);
+
+/* Replay steady-state steps from captured HIP graphs (default on).  Graphs
+ * hold GRAPH_STEPS steps and are used once the first step has run, for runs
+ * without per-step host forcing tables (HIT); results are identical either
+ * way.  No reference counterpart (launch mechanics of this library).        */
+int ch4g_set_graphs(ch4g_lattice *lat, int enable);
+
+/* Wait for all work queued on the lattice stream. */
+int ch4g_synchronize(ch4g_lattice *lat);
+
+/* Current step counter (itime of main.c:69). */
+int64_t ch4g_itime(const ch4g_lattice *lat);
+
+/* Bytes of device memory held by the lattice. */
+int64_t ch4g_device_bytes(const ch4g_lattice *lat);
+
+/* Device bytes a lattice with this configuration would hold (admission). */
+int64_t ch4g_footprint_bytes(const ch4g_config *cfg);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CH4G_H */

@@ -1,0 +1,133 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/ch4_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker, and by bench.py's cpu_baseline leg.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib", "libch4_oracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.oracle_create.restype = P
+        L.oracle_create.argtypes = [P]
+        L.oracle_destroy.argtypes = [P]
+        L.oracle_initial_conditions.argtypes = [P]
+        L.oracle_set_populations.argtypes = [P, dp, dp]
+        L.oracle_get_populations.argtypes = [P, dp, dp]
+        L.oracle_step.argtypes = [P, C.c_int]
+        L.oracle_itime.argtypes = [P]
+        L.oracle_itime.restype = C.c_long
+        L.oracle_hydro_fields.argtypes = [P, dp, dp, dp, dp]
+        L.oracle_get_hit_phases.argtypes = [P, dp]
+        L.oracle_dump_averages.argtypes = [P, P]
+        L.oracle_srand48.argtypes = [P, C.c_long]
+        L.oracle_drand48.argtypes = [P]
+        L.oracle_drand48.restype = C.c_double
+        L.oracle_random_gauss.argtypes = [P, C.c_double, C.c_double]
+        L.oracle_random_gauss.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """One reference-structured FP64 lattice on the CPU."""
+
+    def __init__(self, cfg):
+        from ch4g.abi import LB_TEMPERATURE  # noqa: F401  (config layout lives in the product header)
+        self.cfg = cfg
+        self._L = lib()
+        self._h = self._L.oracle_create(C.byref(cfg))
+        if not self._h:
+            raise MemoryError("oracle_create failed")
+        self.shape = (cfg.NZ, cfg.NY, cfg.NX)
+
+    def close(self):
+        if self._h:
+            self._L.oracle_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def initial_conditions(self):
+        self._L.oracle_initial_conditions(self._h)
+
+    def set_populations(self, p, g=None):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        g = None if g is None else np.ascontiguousarray(g, dtype=np.float64)
+        self._L.oracle_set_populations(self._h, _ptr(p), _ptr(g))
+
+    def get_populations(self):
+        n = self.cfg.nodes()
+        p = np.empty((n, 19))
+        g = np.empty((n, 19))
+        self._L.oracle_get_populations(self._h, _ptr(p), _ptr(g))
+        return p.reshape(self.shape + (19,)), g.reshape(self.shape + (19,))
+
+    def step(self, n=1):
+        self._L.oracle_step(self._h, int(n))
+
+    @property
+    def itime(self):
+        return self._L.oracle_itime(self._h)
+
+    def hydro_fields(self):
+        n = self.cfg.nodes()
+        d = np.empty(n)
+        u = np.empty((n, 3))
+        t = np.empty(n)
+        f = np.empty((n, 3))
+        self._L.oracle_hydro_fields(self._h, _ptr(d), _ptr(u), _ptr(t), _ptr(f))
+        s = self.shape
+        return {"dens": d.reshape(s), "u": u.reshape(s + (3,)), "t": t.reshape(s),
+                "force": f.reshape(s + (3,))}
+
+    def hit_phases(self):
+        a = np.empty((6, 3))
+        self._L.oracle_get_hit_phases(self._h, _ptr(a))
+        return a
+
+    def dump_averages(self):
+        from ch4g.abi import Output
+        out = Output()
+        self._L.oracle_dump_averages(self._h, C.byref(out))
+        return out.as_dict()
+
+
+class RNG:
+    """The oracle's drand48 / random_gauss restatement, for pinning tests."""
+
+    def __init__(self, seed):
+        from ch4g.abi import make_config
+        self._o = Oracle(make_config(1, 1, 1, 0, seed=seed))
+        self._L = self._o._L
+
+    def drand48(self):
+        return self._L.oracle_drand48(self._o._h)
+
+    def random_gauss(self, mu=0.0, sigma=1.0):
+        return self._L.oracle_random_gauss(self._o._h, mu, sigma)

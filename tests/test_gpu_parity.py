@@ -1,0 +1,162 @@
+"""GPU parity: the HIP library (through the C-ABI) against the FP64 oracle.
+
+FP64 device path: the bar is max|gpu - oracle| / max|oracle| <= 1e-12 per
+field; the kernels follow the reference's evaluation order without
+contraction, so the fields are expected (and checked) to be bit-identical.
+FP32-deviation path: relative L2 error per field against the FP64 oracle,
+velocity <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+from ch4g import abi, Lattice
+from oracle import Oracle
+import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_max(a, b):
+    s = np.abs(b).max()
+    return np.abs(a - b).max() / (s if s > 0 else 1.0)
+
+
+def rel_l2(a, b):
+    s = np.sqrt((b * b).sum())
+    return np.sqrt(((a - b) ** 2).sum()) / (s if s > 0 else 1.0)
+
+
+def both(cfg, steps):
+    o = Oracle(cfg)
+    o.initial_conditions()
+    o.step(steps)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.step(steps)
+    return o, lat
+
+
+def compare_fp64(o, lat, exact=True):
+    po, go = o.get_populations()
+    pg, gg = lat.get_populations()
+    ho, hg = o.hydro_fields(), lat.hydro_fields()
+    fields = {"p": (pg, po), "dens": (hg["dens"], ho["dens"]), "u": (hg["u"], ho["u"])}
+    if lat.cfg.has(abi.LB_TEMPERATURE):
+        fields["g"] = (gg, go)
+        fields["t"] = (hg["t"], ho["t"])
+    for name, (a, b) in fields.items():
+        err = rel_max(a, b)
+        print(f"  {name}: max rel err {err:.3e}  bit-identical={np.array_equal(a, b)}")
+        assert err <= 1e-12, (name, err)
+        if exact:
+            assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("steps", [0, 1, 2, 10, 1000])
+def test_poiseuille_fp64_matches_oracle(steps):
+    o, lat = both(configs.poiseuille(4, 64, 4), steps)
+    assert lat.itime == steps
+    compare_fp64(o, lat)
+
+
+def test_rb2d_fp64_matches_oracle_small():
+    o, lat = both(configs.rb2d(NX=128, NY=64), 1000)
+    compare_fp64(o, lat)
+
+
+def test_rb2d_fp64_matches_oracle_full_size():
+    """512x256 (the 2-D Rayleigh-Benard configuration), 200 steps."""
+    o, lat = both(configs.rb2d(), 200)
+    compare_fp64(o, lat)
+
+
+def test_hit_les_fp64_matches_oracle():
+    """Periodic HIT forcing (phase walk on the drand48 stream, host sin tables)
+    with local-strain Smagorinsky; LES needs sqrt on the device."""
+    cfg = configs.hit(N=16, tau=0.51, amp=1e-4)
+    o, lat = both(cfg, 100)
+    np.testing.assert_array_equal(lat.hit_phases(), o.hit_phases())
+    compare_fp64(o, lat, exact=False)
+
+
+def test_dump_averages_fp64_matches_oracle():
+    o, lat = both(configs.rb2d(NX=96, NY=48, Ra=1e6), 300)
+    ao, ag = o.dump_averages(), lat.dump_averages()
+    for k, v in ao.items():
+        if k in ("x", "y", "z"):
+            continue
+        scale = max(abs(v), 1e-300)
+        print(f"  {k}: oracle {v:.15e} gpu {ag[k]:.15e}")
+        assert abs(ag[k] - v) <= 1e-11 * scale + 1e-16, k  # floor: cancelling sums (periodic means ~ 0)
+
+
+def test_set_get_populations_roundtrip():
+    rng = np.random.default_rng(5)
+    cfg = configs.rb2d(NX=16, NY=8, NZ=3)
+    lat = Lattice(cfg)
+    W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    p = W * (1 + 0.01 * rng.standard_normal((3, 8, 16, 19)))
+    g = W * (0.5 + 0.01 * rng.standard_normal((3, 8, 16, 19)))
+    lat.set_populations(p.reshape(-1, 19), g.reshape(-1, 19))
+    p2, g2 = lat.get_populations()
+    assert np.array_equal(p2, p) and np.array_equal(g2, g)
+
+
+@pytest.mark.parametrize("name,cfg,steps", [
+    ("poiseuille 4x64x4", configs.poiseuille(4, 64, 4, precision=abi.FP32_DEV), 1000),
+    # same tau_u = 0.50384 as the 512x256 Ra=1e8 case
+    ("rb2d 128x64 Ra=6.25e6", configs.rb2d(NX=128, NY=64, Ra=6.25e6, precision=abi.FP32_DEV), 1000),
+    ("rb2d 512x256 Ra=1e8", configs.rb2d(precision=abi.FP32_DEV), 1000),
+    ("rb2d 128x64 Ra=6.25e6 store", configs.rb2d(NX=128, NY=64, Ra=6.25e6, precision=abi.FP32_STORE), 1000),
+    ("rb2d 512x256 Ra=1e8 store", configs.rb2d(precision=abi.FP32_STORE), 1000),
+])
+def test_fp32_deviation_against_fp64_oracle(name, cfg, steps):
+    """Relative L2 error per field of the FP32-deviation path against the
+    FP64 oracle on the same inputs; bar: velocity <= 1e-5."""
+    o, lat = both(cfg, steps)
+    ho, hg = o.hydro_fields(), lat.hydro_fields()
+    eu = rel_l2(hg["u"], ho["u"])
+    ed = rel_l2(hg["dens"], ho["dens"])
+    print(f"  {name}: L2 rel err u {eu:.3e}  rho {ed:.3e}")
+    if cfg.has(abi.LB_TEMPERATURE):
+        et = rel_l2(hg["t"], ho["t"])
+        print(f"  {name}: L2 rel err T {et:.3e}")
+        assert et <= 1e-5
+    assert eu <= 1e-5
+
+
+def test_poiseuille_fp32_closed_form():
+    """FP32-deviation storage reaches the exact lattice Poiseuille solution
+    (parabola + bounce-back slip) to FP32 accuracy."""
+    H, g, tau = 16, 1e-6, 0.8
+    nu = (tau - 0.5) / 3
+    lat = Lattice(configs.poiseuille(4, H, 1, tau=tau, g=g, precision=abi.FP32_DEV))
+    lat.initial_conditions()
+    lat.step(int(40 / (np.pi ** 2 * nu / H ** 2)))
+    u = lat.hydro_fields()["u"][0, :, 0, 0]
+    y = np.arange(H) + 0.5
+    ua = g / (2 * nu) * y * (H - y) + (g / nu) * (16 * tau ** 2 - 16 * tau + 1) / 24
+    assert np.abs(u - ua).max() / ua.max() < 2e-5
+
+
+def test_rb2d_full_size_invariants_fp32():
+    """512x256 Rayleigh-Benard in FP32-deviation storage: the initial
+    averages equal the oracle's (the initial Nusselt number is not 1: the
+    hydro velocity carries the Guo half-force, so <u_y T> adds
+    (beta g / 2) Var(T) / (kappa dT / H) ~ 0.08), mass is conserved over
+    2000 steps and the fields stay finite and bounded."""
+    cfg = configs.rb2d(precision=abi.FP32_DEV)
+    o = Oracle(configs.rb2d())
+    o.initial_conditions()
+    ao = o.dump_averages()
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    a0 = lat.dump_averages()
+    assert abs(a0["nuy"] - ao["nuy"]) < 1e-6 * abs(ao["nuy"])
+    assert abs(a0["t"] - ao["t"]) < 1e-7
+    lat.step(2000)
+    a1 = lat.dump_averages()
+    assert abs(a1["rho"] - a0["rho"]) < 1e-6
+    h = lat.hydro_fields()
+    assert np.isfinite(h["u"]).all() and np.isfinite(h["t"]).all()
+    assert h["t"].min() > -0.05 and h["t"].max() < 1.05
