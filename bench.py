@@ -27,10 +27,34 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_NODE = {"fp32dev": 2 * (19 + 19) * 4, "fp64": 2 * (19 + 19) * 8}  # populations read once + written once
 
 
-def gpu_leg(precision, steps, warmup):
+WORKLOADS = {
+    # name: (description, config factory(precision))
+    "rb2d": ("2-D Rayleigh-Benard 512x1x256 (NX=512, NY=256 vertical), D3Q19 fluid + D3Q19 temperature, "
+             "Ra=1e8, Pr=1", lambda c, p: c.rb2d(precision=p)),
+    "rb3d-256": ("3-D Rayleigh-Benard 256^3, D3Q19 fluid + D3Q19 temperature, Ra=1e9, Pr=1",
+                 lambda c, p: c.rb2d(NX=256, NY=256, NZ=256, Ra=1e9, precision=p)),
+    "rb3d-512": ("3-D Rayleigh-Benard 512^3, D3Q19 fluid + D3Q19 temperature, Ra=1e9, Pr=1",
+                 lambda c, p: c.rb2d(NX=512, NY=512, NZ=512, Ra=1e9, precision=p)),
+    "channel-192": ("3-D channel 192^3, D3Q19 fluid, body force 1e-6", lambda c, p: c.channel3d(192, 192, 192,
+                                                                                               precision=p)),
+}
+
+
+def pmc_traffic(workload):
+    """HBM bytes per k_step_fp32 launch measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (calibrated on a plain copy; profiles/k_step_fp32_pmc_traffic.md)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_k_step_fp32.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)[workload]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def gpu_leg(precision, steps, warmup, workload="rb2d"):
     import configs
     from ch4g import Lattice
-    cfg = configs.rb2d(precision=precision)
+    cfg = WORKLOADS[workload][1](configs, precision)
     lat = Lattice(cfg)
     lat.initial_conditions()
     lat.step(warmup)
@@ -40,7 +64,7 @@ def gpu_leg(precision, steps, warmup):
     lat.synchronize()
     wall = time.perf_counter() - t0
     nodes = cfg.nodes()
-    out = {"nodes": nodes, "wall_s": wall, "event_ms": ms, "mlups": nodes * steps / wall / 1e6,
+    out = {"nodes": nodes, "thermal": cfg.has(1 << 1), "wall_s": wall, "event_ms": ms, "mlups": nodes * steps / wall / 1e6,
            "avg_launch_us": ms * 1e3 / steps}
     h = lat.dump_averages()
     out["nusselt"] = h["nuy"]
@@ -78,14 +102,27 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-path leg")
+    ap.add_argument("--workload", default="rb2d", choices=sorted(WORKLOADS))
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run plain 1 GiB device-to-device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": a.numel() * 4, "copies": 5}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (one lattice per GPU; multi-GPU slab decomposition "
               "is a design in DESIGN.md, not built). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     from ch4g import abi
-    r32 = gpu_leg(abi.FP32_DEV, args.steps, args.warmup)
-    bpn = BYTES_PER_NODE["fp32dev"]
+    r32 = gpu_leg(abi.FP32_DEV, args.steps, args.warmup, args.workload)
+    bpn = BYTES_PER_NODE["fp32dev"] if r32["thermal"] else BYTES_PER_NODE["fp32dev"] // 2
     achieved = bpn * r32["nodes"] / (r32["avg_launch_us"] * 1e-6) / 1e9
     line = {
         "metric": METRIC,
@@ -100,20 +137,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "2-D Rayleigh-Benard 512x1x256 (NX=512, NY=256 vertical), D3Q19 fluid + D3Q19 "
-                               "temperature, Ra=1e8, Pr=1, FP32-deviation storage",
+        "config": {"workload": WORKLOADS[args.workload][0] + ", FP32-deviation storage",
                    "nodes": r32["nodes"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload),
                      "bytes_per_node": bpn, "avg_launch_us": round(r32["avg_launch_us"], 3)},
     }
     if not args.no_fp64:
-        r64 = gpu_leg(abi.FP64, max(args.steps // 2, 1), args.warmup)
-        a64 = BYTES_PER_NODE["fp64"] * r64["nodes"] / (r64["avg_launch_us"] * 1e-6) / 1e9
+        r64 = gpu_leg(abi.FP64, max(args.steps // 2, 1), args.warmup, args.workload)
+        b64 = BYTES_PER_NODE["fp64"] if r64["thermal"] else BYTES_PER_NODE["fp64"] // 2
+        a64 = b64 * r64["nodes"] / (r64["avg_launch_us"] * 1e-6) / 1e9
         line["fp64_plain"] = {"mlups": round(r64["mlups"], 2), "avg_launch_us": round(r64["avg_launch_us"], 3),
                               "roofline_frac": round(a64 / HBM_PEAK_GBS, 4),
                               "fp32dev_speedup": round(r32["mlups"] / r64["mlups"], 3)}
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "rb2d":
         line["cpu_baseline"] = cpu_leg()
     print(json.dumps(line), flush=True)
 

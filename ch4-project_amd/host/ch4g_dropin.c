@@ -1,0 +1,189 @@
+/*
+ * Drop-in implementation of the reference's per-step functions
+ * (include/ch4g_dropin.h) on one device-resident lattice.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../../include/ch4g_dropin.h"
+
+/* stages of one reference time step (main.c:92-161) */
+enum { ST_IDLE = 0, ST_ADVECTION, ST_COLLISION, ST_FORCING, ST_BC, ST_STREAMED, ST_HYDRO };
+
+static struct {
+    ch4g_run_params rp;
+    ch4g_lattice *lat;
+    int stage;
+    int itime;
+    int launched;   /* the fused step of the current itime has been queued */
+    double t1, t2;  /* wall-clock bracket of the time loop (main.c:54-56, 198-201) */
+    int timing;
+} S;
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static void die(const char *what)
+{
+    const char *e = ch4g_last_error();
+    fprintf(stderr, "Error message -> %s%s%s. Exit.\n", what, (e && *e) ? ": " : "", (e && *e) ? e : "");
+    exit(-1);
+}
+
+static void need_lattice(const char *fn)
+{
+    if (!S.lat) {
+        fprintf(stderr, "Error message -> %s called before ch4g_dropin_init. Exit.\n", fn);
+        exit(-1);
+    }
+}
+
+static void order(const char *fn, int allowed_from, int allowed_to, int next)
+{
+    need_lattice(fn);
+    if (S.stage < allowed_from || S.stage > allowed_to) {
+        fprintf(stderr, "Error message -> %s called out of the main.c step order (stage %d). Exit.\n", fn, S.stage);
+        exit(-1);
+    }
+    if (next > S.stage) S.stage = next;
+}
+
+int ch4g_dropin_init(const ch4g_run_params *rp)
+{
+    if (S.lat) ch4g_dropin_finalize();
+    memset(&S, 0, sizeof S);
+    S.rp = *rp;
+    return ch4g_create(&rp->cfg, &S.lat);
+}
+
+void ch4g_dropin_finalize(void)
+{
+    if (S.lat) ch4g_destroy(S.lat);
+    S.lat = NULL;
+}
+
+ch4g_lattice *ch4g_dropin_lattice(void) { return S.lat; }
+
+int ch4g_dropin_itime(void) { return S.itime; }
+
+/* initial_conditions + the first hydro_fields / build_forcing / hydro_fields
+ * of main.c:26-48 (the device lattice builds all of them together).  A
+ * restart (resume 1) is loaded with ch4g_set_populations instead.         */
+void initial_conditions(int restart)
+{
+    need_lattice("initial_conditions");
+    if (restart) {
+        fprintf(stderr, "Error message -> resume: load the population dump with ch4g_set_populations. Exit.\n");
+        exit(-1);
+    }
+    if (ch4g_initial_conditions(S.lat)) die("initial_conditions");
+    S.itime = 0;
+    S.stage = ST_IDLE;
+}
+
+/* HIT phases are drawn inside ch4g_initial_conditions in the reference's
+ * order (after the temperature noise), so this call has nothing left to do. */
+void initialization_forcing(void) { need_lattice("initialization_forcing"); }
+
+void compute_advection(pop *f, pop *rhs_f, my_double tau, pop *f_eq, char which_pop)
+{
+    (void)f, (void)rhs_f, (void)tau, (void)f_eq, (void)which_pop;
+    if (S.stage == ST_HYDRO || S.stage == ST_STREAMED) S.stage = ST_IDLE; /* a new step begins */
+    order("compute_advection", ST_IDLE, ST_COLLISION, ST_ADVECTION);
+    if (!S.timing) {
+        S.t1 = now_s();
+        S.timing = 1;
+    }
+}
+
+void add_collision(pop *f, pop *rhs_f, my_double tau, pop *f_eq, char which_pop)
+{
+    (void)f, (void)rhs_f, (void)tau, (void)f_eq, (void)which_pop;
+    order("add_collision", ST_ADVECTION, ST_COLLISION, ST_COLLISION);
+}
+
+void build_forcing(void) { order("build_forcing", ST_COLLISION, ST_FORCING, ST_FORCING); }
+
+void add_forcing(void) { order("add_forcing", ST_FORCING, ST_FORCING, ST_FORCING); }
+
+void boundary_and_pbc_conditions_for_streaming(void)
+{
+    order("boundary_and_pbc_conditions_for_streaming", ST_COLLISION, ST_FORCING, ST_BC);
+}
+
+void time_stepping(pop *f, pop *rhs_f, pop *old_rhs_f, pop *old_old_rhs_f, my_double tau, pop *f_eq,
+                   char which_pop)
+{
+    (void)f, (void)rhs_f, (void)old_rhs_f, (void)old_old_rhs_f, (void)tau, (void)f_eq, (void)which_pop;
+    if (S.stage == ST_STREAMED) return; /* second population set of the same step */
+    order("time_stepping", ST_BC, ST_BC, ST_STREAMED);
+    if (ch4g_step(S.lat, 1)) die("time_stepping");
+    S.itime++;
+}
+
+void hydro_fields(char which_pop)
+{
+    (void)which_pop;
+    order("hydro_fields", ST_STREAMED, ST_HYDRO, ST_HYDRO);
+}
+
+/* dump_averages (output.c:63-1101): every time_dump_diagn the FP64 volume
+ * averages, appended as in output.c:773 and :864.                          */
+void dump_averages(void)
+{
+    need_lattice("dump_averages");
+    const ch4g_run_params *rp = &S.rp;
+    int every = (int)(rp->time_dump_diagn / rp->time_dt);
+    if (every <= 0 || S.itime % every != 0) return;
+    ch4g_output o;
+    if (ch4g_dump_averages(S.lat, &o)) die("dump_averages");
+    double time_now = rp->time_start + S.itime * rp->time_dt;
+    if (rp->cfg.flags & CH4G_LB_FLUID) {
+        FILE *fo = fopen("velocity_averages.dat", "a");
+        if (fo) {
+            fprintf(fo, "%e %e %e %e %e %e %e %e %e %e %e\n", time_now, o.ene, o.rho, o.ux, o.uy, o.uz, o.ux2, o.uy2,
+                    o.uz2, o.eps, o.rho2);
+            fclose(fo);
+        }
+    }
+    if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
+        FILE *fo = fopen("temperature_averages.dat", "a");
+        if (fo) {
+            fprintf(fo, "%e %e %e %e %e %e %e %e %e %e %e %e %e %e\n", time_now, o.t, o.t2, o.epst, o.dxt, o.dyt,
+                    o.dzt, o.uxt, o.uyt, o.uzt, o.nux, o.nuy, o.nuz, o.lb);
+            fclose(fo);
+        }
+    }
+}
+
+/* measure_time (parallel.c:618-638): run_time.dat of the time loop. */
+void measure_time(void)
+{
+    need_lattice("measure_time");
+    if (ch4g_synchronize(S.lat)) die("measure_time");
+    S.t2 = now_s();
+    double dt = S.timing ? S.t2 - S.t1 : 0.0;
+    const ch4g_config *c = &S.rp.cfg;
+    double npts = (double)c->NX * (double)c->NY * (double)c->NZ;
+    int it = S.itime > 0 ? S.itime : 1;
+    remove("run_time.dat");
+    FILE *fo = fopen("run_time.dat", "w");
+    if (!fo) return;
+    fprintf(fo, "Total execution time %e\n", dt);
+    fprintf(stdout, "Total execution time %e\n", dt);
+    fprintf(fo, "Time steps %d\n", S.itime);
+    fprintf(fo, "Execution time per time step %e\n", dt / (double)it);
+    fprintf(fo, "Execution time per grid point %e\n", dt / npts);
+    fprintf(fo, "Execution time per time step and grid point %e\n", dt / ((double)it * npts));
+    fprintf(fo, "Time tick on this machine %e\n", 1e-9);
+    fprintf(fo, "Number of processes for this run %d\n", 1);
+    fprintf(fo, "Execution time per process %e\n", dt);
+    fclose(fo);
+}

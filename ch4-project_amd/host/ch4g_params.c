@@ -1,0 +1,239 @@
+/*
+ * Reader of a reference run description: the active #define names of a
+ * define.h (src/define.h, examples/<case>/define.h) and the name/value pairs
+ * of param.in (src/parameters.c:3-39 read_parameter, :41-800
+ * assign_parameters).  Switches that change the streaming step in ways the
+ * device lattice does not implement are rejected by name.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <ctype.h>
+#include <errno.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "../../include/ch4g_dropin.h"
+
+/* messages go to stderr, as read_parameter (parameters.c:15-34) reports */
+static char g_msg[512];
+static int perr(const char *fmt, const char *a)
+{
+    snprintf(g_msg, sizeof g_msg, fmt, a);
+    fprintf(stderr, "Error message -> %s\n", g_msg);
+    return -1;
+}
+
+/* define.h switches mapped onto ch4g flags */
+static const struct { const char *name; unsigned flag; } FLAGS[] = {
+    {"LB_FLUID", CH4G_LB_FLUID},
+    {"LB_TEMPERATURE", CH4G_LB_TEMPERATURE},
+    {"LB_FLUID_FORCING_POISEUILLE", CH4G_FLUID_FORCING_POISEUILLE},
+    {"LB_FLUID_FORCING_CHANNEL", CH4G_FLUID_FORCING_CHANNEL},
+    {"LB_FLUID_FORCING_HIT", CH4G_FLUID_FORCING_HIT},
+    {"LB_TEMPERATURE_BUOYANCY", CH4G_TEMPERATURE_BUOYANCY},
+    {"LB_FLUID_FORCING_NOZEROMODE", CH4G_FLUID_FORCING_NOZEROMODE},
+    {"LB_FLUID_BC_Y", CH4G_FLUID_BC_Y},
+    {"LB_TEMPERATURE_BC_Y", CH4G_TEMPERATURE_BC_Y},
+    {"LB_FLUID_LES", CH4G_FLUID_LES},
+    {"LB_FLUID_LES_LOCAL_STRAIN", CH4G_FLUID_LES_LOCAL_STRAIN},
+    {"LB_TEMPERATURE_BUOYANCY_T0_REF", CH4G_TEMPERATURE_BUOYANCY_T0_REF},
+    {"LB_TEMPERATURE_INITIAL_LINEAR", CH4G_TEMPERATURE_INITIAL_LINEAR},
+    {"LB_TEMPERATURE_INITIAL_ADD_NOISE", CH4G_TEMPERATURE_INITIAL_ADD_NOISE},
+    {"LB_FLUID_INITIAL_UNIT_DENSITY", CH4G_FLUID_INITIAL_UNIT_DENSITY},
+    {"LB_INITIAL_BAROMETRIC", CH4G_INITIAL_BAROMETRIC},
+};
+
+/* switches with no effect on the streaming step of these configurations
+ * (build, I/O, timing, umbrella names, choices implied by METHOD_STREAMING) */
+static const char *NEUTRAL[] = {
+    "NEW_SENDRECV", "OUTPUT_ASCII", "OUTPUT_H5", "OUTPUT_H5_GRID", "TIMING", "SYSTEM_TIMING", "DEBUG",
+    "GRID_REGULAR", "LB", "LB_FLUID_FORCING", "LB_FLUID_BC", "LB_TEMPERATURE_BC", "LB_FLUID_FORCING_GRAVITY",
+    "METHOD_EDGES_AND_CORNERS", "METHOD_FORCING_GUO", "METHOD_STEPPING_EULER", "METHOD_MYQUICK",
+    "METHOD_MYQUICK_CARTESIAN", "SYSTEM_RANDOM48", "PRECISION_DOUBLE", "OUTPUT_AT_START",
+    "METHOD_STREAMING", "GRID_POP_D3Q19", "NO_XMLHEADERS",
+};
+
+static int in_list(const char *n, const char *const *l, size_t k)
+{
+    for (size_t i = 0; i < k; i++)
+        if (!strcmp(n, l[i])) return 1;
+    return 0;
+}
+
+/* active #define names (block and line comments skipped) */
+static int read_defines(const char *path, char names[][64], int maxn)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return perr("%s file is missing", path);
+    int n = 0, in_block = 0;
+    char line[1024];
+    while (fgets(line, sizeof line, f)) {
+        char buf[1024];
+        size_t o = 0;
+        for (size_t i = 0; line[i] && o + 1 < sizeof buf; i++) {
+            if (in_block) {
+                if (line[i] == '*' && line[i + 1] == '/') {
+                    in_block = 0;
+                    i++;
+                }
+                continue;
+            }
+            if (line[i] == '/' && line[i + 1] == '*') {
+                in_block = 1;
+                i++;
+                continue;
+            }
+            if (line[i] == '/' && line[i + 1] == '/') break;
+            buf[o++] = line[i];
+        }
+        buf[o] = 0;
+        char *p = buf;
+        while (isspace((unsigned char)*p)) p++;
+        if (strncmp(p, "#define", 7) || !isspace((unsigned char)p[7])) continue;
+        p += 7;
+        while (isspace((unsigned char)*p)) p++;
+        size_t k = 0;
+        while ((isalnum((unsigned char)p[k]) || p[k] == '_') && k < 63) k++;
+        if (!k || n >= maxn) continue;
+        memcpy(names[n], p, k);
+        names[n][k] = 0;
+        n++;
+    }
+    fclose(f);
+    return n;
+}
+
+/* param.in tokens, scanned like read_parameter's fscanf("%s %lf") loop */
+typedef struct { char name[256]; double val; } kv;
+
+static int read_param_file(const char *path, kv *out, int maxn)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return perr("%s file is missing", path);
+    int n = 0;
+    char tok[256];
+    char pending[256] = "";
+    int have_pending = 0;
+    for (;;) {
+        if (have_pending) {
+            strcpy(tok, pending);
+            have_pending = 0;
+        } else if (fscanf(f, "%255s", tok) != 1) {
+            break;
+        }
+        double v = 0.0;
+        char nxt[256];
+        if (fscanf(f, "%255s", nxt) == 1) {
+            char *end;
+            errno = 0;
+            v = strtod(nxt, &end);
+            if (end == nxt || *end) { /* not a number: it is the next name */
+                v = 0.0;
+                strcpy(pending, nxt);
+                have_pending = 1;
+            }
+        }
+        if (n < maxn) {
+            snprintf(out[n].name, sizeof out[n].name, "%s", tok);
+            out[n].val = v;
+            n++;
+        }
+    }
+    fclose(f);
+    return n;
+}
+
+static int getp(const kv *t, int n, const char *name, double *v)
+{
+    for (int i = 0; i < n; i++)
+        if (!strcmp(t[i].name, name)) {
+            *v = t[i].val;
+            return 1;
+        }
+    return 0;
+}
+
+static int need(const kv *t, int n, const char *name, double *v)
+{
+    if (!getp(t, n, name, v)) return perr("%s not found. End of File", name);
+    return 0;
+}
+
+int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_params *out)
+{
+    static char names[512][64];
+    int nd = read_defines(define_h, names, 512);
+    if (nd < 0) return -1;
+    memset(out, 0, sizeof *out);
+    ch4g_config *c = &out->cfg;
+    int streaming = 0, d3q19 = 0;
+    for (int i = 0; i < nd; i++) {
+        const char *nm = names[i];
+        int known = 0;
+        for (size_t k = 0; k < sizeof FLAGS / sizeof FLAGS[0]; k++)
+            if (!strcmp(nm, FLAGS[k].name)) {
+                c->flags |= FLAGS[k].flag;
+                known = 1;
+            }
+        if (!strcmp(nm, "METHOD_STREAMING")) streaming = 1;
+        if (!strcmp(nm, "GRID_POP_D3Q19")) d3q19 = 1;
+        if (!strcmp(nm, "OUTPUT_AT_START")) out->output_at_start = 1;
+        if (!strncmp(nm, "GRID_POP_", 9) && strcmp(nm, "GRID_POP_D3Q19"))
+            return perr("%.63s: only GRID_POP_D3Q19 is implemented on the device lattice", nm);
+        if (!known && !in_list(nm, NEUTRAL, sizeof NEUTRAL / sizeof NEUTRAL[0]))
+            return perr("define.h switch %.63s changes the step in a way this lattice does not implement", nm);
+    }
+    if (!streaming) return perr("%s", "METHOD_STREAMING is required");
+    if (!d3q19) return perr("%s", "GRID_POP_D3Q19 is required");
+
+    static kv t[1024];
+    int np = read_param_file(param_in, t, 1024);
+    if (np < 0) return -1;
+    double v, SX, SY, SZ;
+    if (need(t, np, "NX", &v)) return -1;
+    c->NX = (int)v;
+    if (need(t, np, "NY", &v)) return -1;
+    c->NY = (int)v;
+    if (need(t, np, "NZ", &v)) return -1;
+    c->NZ = (int)v;
+    if (need(t, np, "SX", &SX) || need(t, np, "SY", &SY) || need(t, np, "SZ", &SZ)) return -1;
+    if (need(t, np, "time_dt", &out->time_dt) || need(t, np, "time_start", &out->time_start) ||
+        need(t, np, "time_end", &out->time_end) || need(t, np, "time_dump_field", &out->time_dump_field) ||
+        need(t, np, "time_dump_diagn", &out->time_dump_diagn))
+        return -1;
+    /* parameters.c:135: under streaming the time step equals the spacing */
+    if (out->time_dt != SX / c->NX || SX / c->NX != 1.0 || SY / c->NY != 1.0 || SZ / c->NZ != 1.0)
+        return perr("%s", "time_dt != Delta X or Y or Z (streaming needs unit spacing and time step)");
+    if (c->flags & CH4G_LB_FLUID) {
+        if (need(t, np, "tau_u", &c->tau_u)) return -1;
+        if (c->flags & (CH4G_FLUID_FORCING_POISEUILLE | CH4G_FLUID_FORCING_CHANNEL | CH4G_FLUID_FORCING_HIT |
+                        CH4G_TEMPERATURE_BUOYANCY)) {
+            if (need(t, np, "Amp_x", &c->Amp_x) || need(t, np, "Amp_y", &c->Amp_y) || need(t, np, "Amp_z", &c->Amp_z))
+                return -1;
+        }
+    }
+    if (c->flags & CH4G_LB_TEMPERATURE) {
+        if (need(t, np, "tau_t", &c->tau_t) || need(t, np, "T_bot", &c->T_bot) || need(t, np, "T_top", &c->T_top) ||
+            need(t, np, "T_ref", &c->T_ref))
+            return -1;
+        if (c->flags & CH4G_TEMPERATURE_BUOYANCY) {
+            if (need(t, np, "beta_t", &c->beta_t) || need(t, np, "beta2_t", &c->beta2_t) ||
+                need(t, np, "gravity_x", &c->gravity_x) || need(t, np, "gravity_y", &c->gravity_y) ||
+                need(t, np, "gravity_z", &c->gravity_z))
+                return -1;
+        }
+        if (c->flags & CH4G_TEMPERATURE_INITIAL_ADD_NOISE)
+            if (need(t, np, "noise_t", &c->noise_t)) return -1;
+    }
+    /* keys of this implementation (optional) */
+    c->precision = CH4G_FP64;
+    if (getp(t, np, "precision", &v)) c->precision = (int)v;
+    c->les_cs = 0.18; /* les.c:10 */
+    if (getp(t, np, "les_cs", &v)) c->les_cs = v;
+    c->rho0 = 1.0;
+    c->T0 = 0.5 * (c->T_bot + c->T_top);
+    /* parallel.c:238-240 seeds srand48 with rank + time(NULL); `seed` fixes it */
+    c->seed = getp(t, np, "seed", &v) ? (int)v : (int)time(NULL);
+    return 0;
+}
