@@ -19,6 +19,21 @@ namespace ch4g {
 
 namespace {
 
+// Population planes are streamed: every value is read once and written once
+// per step.  Stores are always non-temporal; loads are non-temporal when the
+// two lattices fit the last-level cache (measured faster there), plain
+// otherwise (x-shifted reads of neighbouring wavefronts share L2 lines).
+template <bool NTL>
+__device__ __forceinline__ float ldp(const float *p)
+{
+    if constexpr (NTL)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+__device__ __forceinline__ void stp(float *p, float v) { __builtin_nontemporal_store(v, p); }
+
 __device__ __forceinline__ float wf(int q) { return q == 0 ? 1.f / 3.f : (q <= 6 ? 1.f / 18.f : 1.f / 36.f); }
 
 // per-row constants of the base state
@@ -70,7 +85,7 @@ __device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab
 // Streaming of the previous step (pull) with halfway bounce-back at y walls.
 // Branch-free: every load address is computed first (wall crossings are a
 // wave-uniform select), so all 19 loads are in flight together.
-template <bool FIRST>
+template <bool FIRST, bool NTL>
 __device__ __forceinline__ void gather_f32(const KParams &P, const Row &R, const Nbr &b, const float *__restrict__ in,
                                            uint32_t n, float f[Q])
 {
@@ -78,12 +93,12 @@ __device__ __forceinline__ void gather_f32(const KParams &P, const Row &R, const
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (FIRST) {
-            f[q] = in[(size_t)q * N + n];
+            f[q] = ldp<NTL>(in + (size_t)q * N + n);
         } else {
             const bool cross = P.wall_f && crosses_wall<q>(b);
             const float *base = in + (size_t)(cross ? invq(q) : q) * N;
             const uint32_t off = cross ? n : src_index<q>(b, P.NX, P.NY);
-            float v = base[off];
+            float v = ldp<NTL>(base + off);
             if constexpr (cyq(q) > 0) v += cross ? 0.f : wf(q) * R.dr_up;
             if constexpr (cyq(q) < 0) v += cross ? 0.f : wf(q) * R.dr_dn;
             f[q] = v;
@@ -92,7 +107,7 @@ __device__ __forceinline__ void gather_f32(const KParams &P, const Row &R, const
 }
 
 // Fixed-T y walls: same-direction population + w 2((T_w - T_ref) - T_lag).
-template <bool FIRST>
+template <bool FIRST, bool NTL>
 __device__ __forceinline__ void gather_g32(const KParams &P, const Row &R, const Nbr &b, const float *__restrict__ in,
                                            uint32_t n, float dTlag, float g[Q])
 {
@@ -102,11 +117,11 @@ __device__ __forceinline__ void gather_g32(const KParams &P, const Row &R, const
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (FIRST) {
-            g[q] = in[(size_t)q * N + n];
+            g[q] = ldp<NTL>(in + (size_t)q * N + n);
         } else {
             const bool cross = P.wall_t && crosses_wall<q>(b);
             const uint32_t off = cross ? n : src_index<q>(b, P.NX, P.NY);
-            float v = in[(size_t)q * N + off];
+            float v = ldp<NTL>(in + (size_t)q * N + off);
             if constexpr (cyq(q) > 0) v += cross ? wf(q) * fac_bot : wf(q) * R.dt_up;
             if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
             g[q] = v;
@@ -138,41 +153,17 @@ __device__ __forceinline__ A wa(int q)
     return q == 0 ? A(1) / A(3) : (q <= 6 ? A(1) / A(18) : A(1) / A(36));
 }
 
-// A = arithmetic type of the collision (float for CH4G_FP32_DEV, double for
-// CH4G_FP32_STORE); storage is float in both.
-template <bool FIRST, bool FLUID, bool THERMAL, typename A>
-__global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__restrict__ fin,
-                                                   const float *__restrict__ gin, float *__restrict__ fout,
-                                                   float *__restrict__ gout, float *__restrict__ tlag,
-                                                   float *__restrict__ taules)
+// Fluid part of the step for one node: moments, velocity with the lagged
+// half force, optional LES, Guo-forced BGK in FMA form, restoration of the
+// collision invariants, non-temporal stores; the velocity used is returned
+// in ux, uy, uz.
+template <typename A>
+__device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, int x, int y, int z, uint32_t n,
+                                              const float f[Q], float dTlag, A dT, float *__restrict__ fout,
+                                              float *__restrict__ taules, A &ux, A &uy, A &uz)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= P.NX) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
     const size_t N = (size_t)P.N;
-    const Row R = row_of(P, y);
-    const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
-
-    float dTlag = 0.f;
-    float g[Q];
-    if constexpr (THERMAL) {
-        dTlag = tlag[n];
-        gather_g32<FIRST>(P, R, b, gin, n, dTlag, g);
-    }
-    float f[Q];
-    if constexpr (FLUID) gather_f32<FIRST>(P, R, b, fin, n, f);
-    A dT = A(0);
-    if constexpr (THERMAL) {
-        if constexpr (sizeof(A) == 4) {
-            dT = sum19(g);
-        } else {
-            for (int q = 0; q < Q; q++) dT += (A)g[q];
-        }
-    }
-
-    A ux = 0, uy = 0, uz = 0;
-    if constexpr (FLUID) {
+    {
         A fa[Q];
         for (int q = 0; q < Q; q++) fa[q] = (A)f[q];
         A drho;
@@ -274,10 +265,20 @@ __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__res
             r[0] += drho - sum19(r);
         }
 #pragma unroll
-        for (int q = 0; q < Q; q++) fout[(size_t)q * N + n] = (float)r[q];
+        for (int q = 0; q < Q; q++) stp(fout + (size_t)q * N + n, (float)r[q]);
     }
 
-    if constexpr (THERMAL) {
+}
+
+// Temperature part of the step for one node: BGK with the fluid velocity,
+// exact conservation of the temperature, non-temporal stores.
+template <typename A>
+__device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, int y, uint32_t n, const float g[Q],
+                                                A dT, A ux, A uy, A uz, float *__restrict__ gout,
+                                                float *__restrict__ tlag)
+{
+    const size_t N = (size_t)P.N;
+    {
         const A u2 = ux * ux + uy * uy + uz * uz;
         const A T = (sizeof(A) == 4 ? (A)R.tb : (A)P.row64[2 * y + 1]) + dT;
         const A om = (A)P.invtau_t;
@@ -295,9 +296,45 @@ __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__res
         // the collision conserves the temperature exactly (float weights do not)
         if constexpr (sizeof(A) == 4) r[0] += dT - sum19(r);
 #pragma unroll
-        for (int q = 0; q < Q; q++) gout[(size_t)q * N + n] = (float)r[q];
+        for (int q = 0; q < Q; q++) stp(gout + (size_t)q * N + n, (float)r[q]);
         tlag[n] = (float)dT;
     }
+}
+
+// A = arithmetic type of the collision (float for CH4G_FP32_DEV, double for
+// CH4G_FP32_STORE); storage is float in both.
+template <bool FIRST, bool FLUID, bool THERMAL, typename A, bool NTL>
+__global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__restrict__ fin,
+                                                   const float *__restrict__ gin, float *__restrict__ fout,
+                                                   float *__restrict__ gout, float *__restrict__ tlag,
+                                                   float *__restrict__ taules)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= P.NX) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    const Row R = row_of(P, y);
+    const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
+
+    float dTlag = 0.f;
+    float g[Q];
+    if constexpr (THERMAL) {
+        dTlag = tlag[n];
+        gather_g32<FIRST, NTL>(P, R, b, gin, n, dTlag, g);
+    }
+    float f[Q];
+    if constexpr (FLUID) gather_f32<FIRST, NTL>(P, R, b, fin, n, f);
+    A dT = A(0);
+    if constexpr (THERMAL) {
+        if constexpr (sizeof(A) == 4) {
+            dT = sum19(g);
+        } else {
+            for (int q = 0; q < Q; q++) dT += (A)g[q];
+        }
+    }
+    A ux = 0, uy = 0, uz = 0;
+    if constexpr (FLUID) collide_fluid<A>(P, R, x, y, z, n, f, dTlag, dT, fout, taules, ux, uy, uz);
+    if constexpr (THERMAL) collide_thermal<A>(P, R, y, n, g, dT, ux, uy, uz, gout, tlag);
 }
 
 __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__restrict__ fin,
@@ -319,9 +356,9 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
         float g[Q];
         dTlag = tlag[n];
         if (P.first)
-            gather_g32<true>(P, R, b, gin, n, dTlag, g);
+            gather_g32<true, false>(P, R, b, gin, n, dTlag, g);
         else
-            gather_g32<false>(P, R, b, gin, n, dTlag, g);
+            gather_g32<false, false>(P, R, b, gin, n, dTlag, g);
         double s = 0.0;
         for (int q = 0; q < Q; q++) {
             s += (double)g[q];
@@ -332,9 +369,9 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
     if (P.fluid) {
         float f[Q];
         if (P.first)
-            gather_f32<true>(P, R, b, fin, n, f);
+            gather_f32<true, false>(P, R, b, fin, n, f);
         else
-            gather_f32<false>(P, R, b, fin, n, f);
+            gather_f32<false, false>(P, R, b, fin, n, f);
         double drho = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
         for (int q = 0; q < Q; q++) {
             double d = (double)f[q];
@@ -362,21 +399,39 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
     }
 }
 
-static int block_x32(const KParams &P) { return P.NX >= 256 ? 256 : (P.NX >= 128 ? 128 : 64); }
+#ifndef CH4G_BX32
+#define CH4G_BX32 256
+#endif
+static int block_x32(const KParams &P)
+{
+    int bx = CH4G_BX32;
+    while (bx > 64 && bx / 2 >= P.NX) bx /= 2;
+    return bx;
+}
 
 void launch_step_fp32(const KParams &P, const float *fin, const float *gin, float *fout, float *gout, float *tlag,
                       float *taules, hipStream_t s)
 {
     int bx = block_x32(P);
     dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ);
+    // both lattices resident in the 256 MB last-level cache?
+    const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
+    const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
+#define CH4G_LAUNCH32_(F_, FL_, TH_, A_)                                                                                 \
+    do {                                                                                                              \
+        if (ntl)                                                                                                      \
+            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, true>), grid, dim3(bx), 0, s, P, fin, gin, fout, gout, \
+                               tlag, taules);                                                                         \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, false>), grid, dim3(bx), 0, s, P, fin, gin, fout,      \
+                               gout, tlag, taules);                                                                   \
+    } while (0)
 #define CH4G_LAUNCH32(F_, FL_, TH_)                                                                                  \
     do {                                                                                                              \
         if (P.arith64)                                                                                                \
-            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, double>), grid, dim3(bx), 0, s, P, fin, gin, fout, gout,   \
-                               tlag, taules);                                                                         \
+            CH4G_LAUNCH32_(F_, FL_, TH_, double);                                                                     \
         else                                                                                                          \
-            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, float>), grid, dim3(bx), 0, s, P, fin, gin, fout, gout,    \
-                               tlag, taules);                                                                         \
+            CH4G_LAUNCH32_(F_, FL_, TH_, float);                                                                      \
     } while (0)
     if (P.fluid && P.thermal) {
         if (P.first) CH4G_LAUNCH32(true, true, true); else CH4G_LAUNCH32(false, true, true);
@@ -386,6 +441,7 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
         if (P.first) CH4G_LAUNCH32(true, false, true); else CH4G_LAUNCH32(false, false, true);
     }
 #undef CH4G_LAUNCH32
+#undef CH4G_LAUNCH32_
 }
 
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,
