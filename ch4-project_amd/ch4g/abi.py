@@ -25,8 +25,24 @@ TEMPERATURE_INITIAL_LINEAR = 1 << 12
 TEMPERATURE_INITIAL_ADD_NOISE = 1 << 13
 FLUID_INITIAL_UNIT_DENSITY = 1 << 14
 INITIAL_BAROMETRIC = 1 << 15
-LAGRANGE = 1 << 16
-LAGRANGE_TWOWAY = 1 << 17
+
+# enum ch4g_lagrange_flag (reference LAGRANGE_* switch names)
+LAGRANGE_ADD_TRACER = 1 << 0
+LAGRANGE_WRAP = 1 << 1
+LAGRANGE_GRADIENT = 1 << 2
+LAGRANGE_ADDEDMASS = 1 << 3
+LAGRANGE_GRAVITY = 1 << 4
+LAGRANGE_BC_INELASTIC = 1 << 5
+LAGRANGE_BC_INELASTIC_REINJECT = 1 << 6
+LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM = 1 << 7
+LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO = 1 << 8
+LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF = 1 << 9
+LAGRANGE_SMALLTAUD_BETA = 1 << 10
+LAGRANGE_SMALLTAUD_FLOATER = 1 << 11
+LAGRANGE_TWOWAY_MOMENTUM = 1 << 12
+LAGRANGE_TWOWAY_SMALLTAUD = 1 << 13
+LAGRANGE_TAUDRAG_INCREMENT_LOG = 1 << 14
+LAGRANGE_ADDEDMASS_INCREMENT_LOG = 1 << 15
 
 FP64 = 0
 FP32_DEV = 1
@@ -92,3 +108,41 @@ def make_config(NX, NY, NZ, flags, precision=FP64, **kw):
     if "T0" not in kw:
         c.T0 = 0.5 * (c.T_bot + c.T_top)
     return c
+
+
+class ParticleConfig(C.Structure):
+    _fields_ = [
+        ("flags", C.c_uint32),
+        ("particle_number", C.c_int32),
+        ("tau_drag_types", C.c_double), ("tau_drag_min", C.c_double), ("tau_drag_max", C.c_double),
+        ("beta_coeff_types", C.c_double), ("beta_coeff_min", C.c_double), ("beta_coeff_max", C.c_double),
+        ("time_dt_lagr_factor", C.c_double),
+    ]
+
+
+def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_coeff=(1, 1.0, 1.0),
+                         time_dt_lagr_factor=1.0):
+    """ParticleConfig from (types, min, max) ladders as param.in gives them."""
+    pc = ParticleConfig()
+    pc.flags = flags
+    pc.particle_number = particle_number
+    pc.tau_drag_types, pc.tau_drag_min, pc.tau_drag_max = (float(v) for v in tau_drag)
+    pc.beta_coeff_types, pc.beta_coeff_min, pc.beta_coeff_max = (float(v) for v in beta_coeff)
+    pc.time_dt_lagr_factor = float(time_dt_lagr_factor)
+    return pc
+
+
+PARTICLE_FIELDS = ["name", "x", "y", "z", "vx", "vy", "vz", "vx_old", "vy_old", "vz_old",
+                   "ax", "ay", "az", "ax_old", "ay_old", "az_old", "ux", "uy", "uz", "ux_old", "uy_old", "uz_old",
+                   "dx_ux", "dy_ux", "dz_ux", "dx_uy", "dy_uy", "dz_uy", "dx_uz", "dy_uz", "dz_uz",
+                   "Dt_ux", "Dt_uy", "Dt_uz", "t", "t_old", "dt_t", "tau_drag", "beta_coeff"]
+
+
+class Particle(C.Structure):
+    _fields_ = [(n, C.c_double) for n in PARTICLE_FIELDS]
+
+
+def particle_dtype():
+    """numpy structured dtype with the layout of struct ch4g_particle."""
+    import numpy as np
+    return np.dtype([(n, np.float64) for n in PARTICLE_FIELDS])

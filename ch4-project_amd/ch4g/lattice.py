@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from .abi import NPOP, Config, Output
+from .abi import NPOP, Config, Output, ParticleConfig, particle_dtype
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libch4g.so")
@@ -22,7 +22,8 @@ EXPORTS = [
     "ch4g_last_error", "ch4g_device_count", "ch4g_create", "ch4g_destroy", "ch4g_initial_conditions",
     "ch4g_set_populations", "ch4g_get_populations", "ch4g_step", "ch4g_hydro_fields", "ch4g_dump_averages",
     "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_step_timed", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
-    "ch4g_footprint_bytes",
+    "ch4g_footprint_bytes", "ch4g_particles_init", "ch4g_particles_count", "ch4g_get_particles",
+    "ch4g_set_particles",
 ]
 
 
@@ -62,6 +63,10 @@ def lib():
     L.ch4g_device_bytes.restype = C.c_int64
     L.ch4g_footprint_bytes.argtypes = [C.POINTER(Config)]
     L.ch4g_footprint_bytes.restype = C.c_int64
+    L.ch4g_particles_init.argtypes = [P, C.POINTER(ParticleConfig)]
+    L.ch4g_particles_count.argtypes = [P, C.POINTER(C.c_int64)]
+    L.ch4g_get_particles.argtypes = [P, P, C.c_int64, C.POINTER(C.c_int64)]
+    L.ch4g_set_particles.argtypes = [P, P, C.c_int64]
     _lib = L
     return L
 
@@ -176,3 +181,25 @@ class Lattice:
     def set_hit_phases(self, phi):
         a = np.ascontiguousarray(phi, dtype=np.float64).reshape(6, 3)
         check(self._L.ch4g_set_hit_phases(self._h, _ptr(a)))
+
+    # ---- Lagrangian particles
+    def particles_init(self, pcfg):
+        check(self._L.ch4g_particles_init(self._h, C.byref(pcfg)))
+
+    def particles_count(self):
+        n = C.c_int64()
+        check(self._L.ch4g_particles_count(self._h, C.byref(n)))
+        return n.value
+
+    def particles(self):
+        """Particle records in array order (numpy structured array with the
+        fields of struct ch4g_particle)."""
+        n = self.particles_count()
+        a = np.zeros(n, dtype=particle_dtype())
+        got = C.c_int64()
+        check(self._L.ch4g_get_particles(self._h, a.ctypes.data if n else None, n, C.byref(got)))
+        return a[:got.value]
+
+    def set_particles(self, a):
+        a = np.ascontiguousarray(a, dtype=particle_dtype())
+        check(self._L.ch4g_set_particles(self._h, a.ctypes.data if len(a) else None, len(a)))

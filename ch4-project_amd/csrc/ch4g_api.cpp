@@ -16,11 +16,11 @@
 #include <new>
 #include <vector>
 
-#include "ch4g_internal.h"
+#include "ch4g_lattice.h"
 
 using namespace ch4g;
 
-namespace {
+namespace ch4g {
 
 thread_local char g_err[512] = "";
 
@@ -33,60 +33,9 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(call)                                                                                     \
-    do {                                                                                                 \
-        hipError_t e_ = (call);                                                                          \
-        if (e_ != hipSuccess) return fail(-2, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),    \
-                                          __FILE__, __LINE__);                                           \
-    } while (0)
+}  // namespace ch4g
 
-constexpr int HIT_RING = 8;
-// steps per captured graph (even: the graph starts and ends on the same buffer)
-constexpr int GRAPH_STEPS = 32;
-
-}  // namespace
-
-struct ch4g_lattice {
-    ch4g_config cfg;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int64_t N = 0;
-    int dev32 = 0;           // FP32-deviation storage
-    size_t rsz = 8;          // bytes per stored real
-    void *f[2] = {nullptr, nullptr};
-    void *g[2] = {nullptr, nullptr};
-    void *tlag = nullptr;
-    void *taules = nullptr;
-    int cur = 0;             // buffer holding the current state
-    int first = 1;           // state holds pre-collision populations (no gather)
-    int64_t itime = 0;
-    // constants of the step
-    KParams base;
-    double nu = 0, kappa = 0, deltaT = 0;
-    // HIT forcing: phases of the current force (Phi_{itime+1}), device tables
-    // for the two most recent steps, pinned staging ring
-    double phi[CH4G_HIT_NK][3];
-    double vk[CH4G_HIT_NK][3];
-    double *d_hit[2] = {nullptr, nullptr};
-    double *h_ring[HIT_RING] = {};
-    hipEvent_t ev_ring[HIT_RING] = {};
-    int ring = 0;
-    size_t hit_len = 0;      // doubles per table
-    unsigned short xsubi[3]; // drand48 stream (srand48(seed) state)
-    // lazily allocated FP64 scratch
-    double *d_dens = nullptr, *d_u = nullptr, *d_t = nullptr;
-    double *d_part = nullptr, *d_res = nullptr;
-    double *d_fs = nullptr, *d_gs = nullptr;
-    // FP32-deviation per-row base state
-    float *d_row32 = nullptr;
-    double *d_row64 = nullptr;
-    // captured GRAPH_STEPS-step sequences, one per starting buffer
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int use_graphs = 1;
-    int64_t bytes = 0;
-};
-
-namespace {
+namespace ch4g {
 
 int dalloc(ch4g_lattice *L, void **p, size_t bytes)
 {
@@ -98,6 +47,10 @@ int dalloc(ch4g_lattice *L, void **p, size_t bytes)
     L->bytes += (int64_t)bytes;
     return 0;
 }
+
+}  // namespace ch4g
+
+namespace {
 
 bool has(const ch4g_config &c, uint32_t f) { return (c.flags & f) != 0; }
 
@@ -180,11 +133,20 @@ int first_build_forcing(ch4g_lattice *L)
     return hit_upload(L, (int)((L->itime + 1) & 1));
 }
 
+}  // namespace
+
+namespace ch4g {
+
 KParams params_for(const ch4g_lattice *L, int first, int itime)
 {
     KParams P = L->base;
     P.first = first;
     P.itime = itime;
+    if (P.twoway) {
+        // feedback of step s lives in plane set s & 1 (zero at s = 0, 1)
+        P.fb_prev = L->part->d_fb[(itime + 1) & 1];
+        P.fb_now = L->part->d_fb[itime & 1];
+    }
     if (P.hit) {
         const int NX = L->cfg.NX, NY = L->cfg.NY;
         int sp = itime & 1, sn = (itime + 1) & 1;
@@ -235,14 +197,40 @@ void gather(ch4g_lattice *L, double *fpost, double *gpost, double *dens, double 
                            (const double *)L->tlag, fpost, gpost, dens, u, t, L->stream);
 }
 
+// Graphs capture kernel arguments: drop them when the step sequence changes.
+void destroy_graphs(ch4g_lattice *L)
+{
+    for (int b = 0; b < 2; b++)
+        if (L->graph[b]) {
+            (void)hipGraphExecDestroy(L->graph[b]);
+            L->graph[b] = nullptr;
+        }
+}
+
+}  // namespace ch4g
+
+namespace {
+
 // One step: host forcing tables (HIT), then the fused kernel.
 int launch_one_step(ch4g_lattice *L)
 {
     int itime = (int)L->itime + 1;
     if (L->base.hit) {
         // build_forcing of this step: Phi_{itime+1}
+        if (L->part && L->part->rng_on_device) {
+            int rc = rng_to_host(L);
+            if (rc) return rc;
+        }
         hit_walk(L);
+        if (L->part && L->part->rng_on_device) {
+            int rc = rng_to_device(L);
+            if (rc) return rc;
+        }
         int rc = hit_upload(L, (itime + 1) & 1);
+        if (rc) return rc;
+    }
+    if (L->part) {
+        int rc = particles_before_step(L, itime);
         if (rc) return rc;
     }
     KParams P = params_for(L, L->first, itime);
@@ -257,6 +245,7 @@ int launch_one_step(ch4g_lattice *L)
     L->cur = out;
     L->first = 0;
     L->itime = itime;
+    if (L->part) return particles_after_step(L);
     return 0;
 }
 
@@ -332,7 +321,6 @@ int validate(const ch4g_config *c)
     if (forced(*c) && !has(*c, CH4G_LB_FLUID)) return fail(-1, "fluid forcing requires LB_FLUID");
     if (has(*c, CH4G_FLUID_LES) && !has(*c, CH4G_FLUID_LES_LOCAL_STRAIN))
         return fail(-1, "LB_FLUID_LES is supported in its LB_FLUID_LES_LOCAL_STRAIN form only");
-    if (has(*c, CH4G_LAGRANGE | CH4G_LAGRANGE_TWOWAY)) return fail(-1, "particles are driven through ch4g_particles_*");
     if (c->tau_u <= 0.5 && has(*c, CH4G_LB_FLUID)) return fail(-1, "tau_u must exceed 1/2");
     if (c->tau_t <= 0.5 && has(*c, CH4G_LB_TEMPERATURE)) return fail(-1, "tau_t must exceed 1/2");
     if ((has(*c, CH4G_FLUID_BC_Y) || has(*c, CH4G_TEMPERATURE_BC_Y)) && c->NY < 3)
@@ -535,8 +523,8 @@ void ch4g_destroy(ch4g_lattice *L)
         if (L->h_ring[r]) (void)hipHostFree(L->h_ring[r]);
         if (L->ev_ring[r]) (void)hipEventDestroy(L->ev_ring[r]);
     }
-    for (int b = 0; b < 2; b++)
-        if (L->graph[b]) (void)hipGraphExecDestroy(L->graph[b]);
+    destroy_graphs(L);
+    particles_free(L);
     void *scr[] = {L->tlag, L->taules, L->d_dens, L->d_u, L->d_t, L->d_part, L->d_res, L->d_fs, L->d_gs,
                    L->d_row32, L->d_row64};
     for (void *p : scr)
@@ -548,6 +536,7 @@ void ch4g_destroy(ch4g_lattice *L)
 int ch4g_initial_conditions(ch4g_lattice *L)
 {
     if (!L) return fail(-1, "null lattice");
+    particles_free(L);
     const ch4g_config &c = L->cfg;
     const int NX = c.NX, NY = c.NY, NZ = c.NZ;
     const double LY = NY;
@@ -615,6 +604,7 @@ int ch4g_initial_conditions(ch4g_lattice *L)
 int ch4g_set_populations(ch4g_lattice *L, const double *p, const double *g)
 {
     if (!L) return fail(-1, "null lattice");
+    particles_free(L);
     int rc = ensure_stage(L);
     if (rc) return rc;
     const size_t N = (size_t)L->N;
@@ -669,7 +659,7 @@ int ch4g_step(ch4g_lattice *L, int nsteps)
         // Steady-state steps (not the first, not the LES start-up step, no
         // per-step host forcing tables) depend only on the buffer parity:
         // replay a captured GRAPH_STEPS-step graph instead of launching.
-        if (L->use_graphs && !L->first && L->itime >= 1 && !L->base.hit && nsteps - s >= GRAPH_STEPS) {
+        if (L->use_graphs && !L->first && L->itime >= 1 && !L->base.hit && !L->part && nsteps - s >= GRAPH_STEPS) {
             int rc = replay_graph(L);
             if (rc) return rc;
             s += GRAPH_STEPS;

@@ -74,6 +74,10 @@ struct KParams {
     double cw_top, cw_bot;  // (T_wall - T_ref) - T_b(wall row)
     double rho0, T0;
     int arith64;            // float storage with double collision arithmetic
+    // two-way particle feedback added to the built force (lagrange_feedback.c:245-254):
+    // planes [3][N] of the previous step (u shift) and of this step (Guo term)
+    int twoway;
+    const double *fb_prev, *fb_now;
 };
 
 // Neighbour coordinates of a node for the pull gather: the source of
@@ -128,6 +132,57 @@ __device__ __forceinline__ void static_for(F &&f)
         static_for<q + 1, Q_>(f);
     }
 }
+
+// ---------------------------------------------------------------------------
+// Lagrangian particles: FP64 structure-of-arrays rows of one device buffer.
+enum PartField {
+    PX, PY, PZ, PVX, PVY, PVZ, PVXO, PVYO, PVZO, PAX, PAY, PAZ, PAXO, PAYO, PAZO,
+    PUX, PUY, PUZ, PUXO, PUYO, PUZO,
+    PG,                 // 9 rows: dx_ux, dy_ux, dz_ux, dx_uy, dy_uy, dz_uy, dx_uz, dy_uz, dz_uz
+    PDTU = PG + 9,      // 3 rows: Dt_ux, Dt_uy, Dt_uz
+    PT = PDTU + 3, PTOLD, PDTT,
+    PTAU, PBETA,
+    PVOL,               // (4/3) pi (3 nu beta tau)^(3/2), host pow (lagrange_feedback.c:87-90)
+    PDRAT,              // (3 - beta) / (2 beta) (lagrange_feedback.c:84)
+    PNAME,
+    PF_COUNT
+};
+
+struct PartParams {
+    int NX, NY, NZ;
+    int64_t N;
+    int64_t np, ld;          // particles, row stride
+    double *s;               // state rows
+    int *alive;
+    int *ndraw;
+    int *bsum;
+    uint64_t *rng;
+    const double *u, *t;     // fields of the step: u interleaved [3N], t [N]
+    uint32_t flags;          // ch4g_lagrange_flag
+    int fluid, thermal, wall_f, wall_t;
+    double Tw_bot, Tw_top;   // T_wall - T_ref
+    int first;               // itime == 1 (Euler start)
+    double dt;               // time_dt_lagr
+    double grav[3];
+    double beta_t;
+};
+
+// one sub-step: interpolation + move + walls (+ re-injection draws) + wrap
+int launch_particles(const PartParams &P, hipStream_t s);
+// feedback deposits of the particles into fb[3][N] (sorted per node, summed
+// in particle order); clears the nodes listed in old_keys first
+struct FeedbackBufs {
+    uint32_t *kin, *kout, *vin, *vout;
+    void *tmp;
+    size_t tmp_bytes;
+    int key_bits;
+    const uint32_t *old_keys;
+    int64_t old_n;
+    double *fb;
+    double gv[3];            // gravity vector of the feedback (g_x, -g_y, g_z) or 0
+};
+int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s);
+size_t feedback_sort_bytes(int64_t nitems, int key_bits);
 
 // Kernel launchers (stream-ordered).
 void launch_step_fp64(const KParams &P, const double *fin, const double *gin, double *fout, double *gout,

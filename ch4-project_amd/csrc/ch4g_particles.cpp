@@ -1,0 +1,488 @@
+// C-ABI of the Lagrangian particles (include/ch4g.h, ch4g_particles_*):
+// allocation and initial conditions on the host with the lattice's drand48
+// stream and glibc libm (so that the per-particle constants are the
+// reference's bits), and the hooks the step sequence calls around the fused
+// lattice kernel: the feedback deposit before it, the hydro fields and the
+// particle sub-steps after it.
+//
+// Compiled with -ffp-contract=off (host arithmetic included).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "ch4g_lattice.h"
+
+using namespace ch4g;
+
+namespace {
+
+bool lhas(const ch4g_particle_config &pc, uint32_t f) { return (pc.flags & f) != 0; }
+
+bool random_reinjection(const ch4g_lattice *L, const ch4g_particle_config &pc)
+{
+    return L->base.wall_f && lhas(pc, CH4G_LAGRANGE_BC_INELASTIC) && lhas(pc, CH4G_LAGRANGE_BC_INELASTIC_REINJECT) &&
+           !lhas(pc, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM);
+}
+
+// Switch combinations whose reference result is defined and restated.
+int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
+{
+    if (!pc) return fail(-1, "null particle configuration");
+    if (L->itime != 0) return fail(-1, "particles are initialised before the first step (itime 0)");
+    if (pc->particle_number < 0 || (int64_t)pc->particle_number * 8 > (int64_t)1 << 31)
+        return fail(-1, "particle_number out of range");
+    if (!L->base.fluid) return fail(-1, "particles need LB_FLUID");
+    // parameters.c:566, :581, :167
+    if (pc->tau_drag_types < 1 || pc->tau_drag_max < pc->tau_drag_min) return fail(-1, "Error in tau_drag parameters");
+    const bool am = lhas(*pc, CH4G_LAGRANGE_ADDEDMASS);
+    if (am && (pc->beta_coeff_types < 1 || pc->beta_coeff_max < pc->beta_coeff_min))
+        return fail(-1, "Error in beta_coeff parameters");
+    if (!(pc->time_dt_lagr_factor >= 1.0)) return fail(-1, "time_dt_lagr > time_dt (time_dt_lagr_factor < 1)");
+    // the reference reads beta_coeff and the gradient only when both switches
+    // are on (lagrange.c:239-262 allocates beta under GRADIENT && ADDEDMASS)
+    if (am && !lhas(*pc, CH4G_LAGRANGE_GRADIENT))
+        return fail(-1, "LAGRANGE_ADDEDMASS needs LAGRANGE_GRADIENT (beta_coeff is set only with both)");
+    const bool uses_dtu = lhas(*pc, CH4G_LAGRANGE_SMALLTAUD_BETA) || lhas(*pc, CH4G_LAGRANGE_SMALLTAUD_FLOATER) ||
+                          lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM);
+    // these read Du/Dt, set only for beta != 0 with added mass; otherwise the
+    // reference reads a value left over from another particle
+    if (uses_dtu && (!am || pc->beta_coeff_min <= 0.0))
+        return fail(-1, "LAGRANGE_SMALLTAUD_* and LAGRANGE_TWOWAY_MOMENTUM need LAGRANGE_ADDEDMASS with beta_coeff_min > 0");
+    if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_SMALLTAUD) && !lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM))
+        return fail(-1, "LAGRANGE_TWOWAY_SMALLTAUD needs LAGRANGE_TWOWAY_MOMENTUM");
+    if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM) && L->N >= ((int64_t)1 << 32) - 1)
+        return fail(-1, "two-way coupling needs fewer than 2^32 - 1 nodes");
+    return 0;
+}
+
+// One property ladder (lagrange.c:264-298, same loop for beta :305-336):
+// the values of the `types` steps repeat with period `repetitions` over the
+// particle types after the fluid tracers.
+void ladder(std::vector<double> &tab, int ntr, double trval, double types, double vmin, double vmax, bool logs,
+            double repetitions)
+{
+    const int nt = (int)tab.size();
+    double cycles = (nt - ntr) / types;
+    double step;
+    if (logs)
+        step = types > 1 ? pow(vmax / vmin, 1.0 / (types - 1.0)) : 0;
+    else
+        step = types > 1 ? (vmax - vmin) / (types - 1.0) : 0;
+    for (int n = 0; n < ntr; n++) tab[n] = trval;
+    const int reps = (int)repetitions, ntyp = (int)types;
+    for (int k = 0; k < (int)(cycles / repetitions); k++)
+        for (int i = 0; i < ntyp; i++)
+            for (int j = 0; j < reps; j++) {
+                const int at = ntr + j + i * reps + k * reps * ntyp;
+                if (at < nt) tab[at] = logs ? vmin * pow(step, (double)i) : vmin + i * step;
+            }
+}
+
+// particle volume and density ratio of the feedback (lagrange_feedback.c:84-90)
+void feedback_constants(double nu, double tau, double beta, double *vol, double *drat)
+{
+    double squared_radius = 3.0 * nu * beta * tau;
+    *vol = (4. / 3.) * 3.14159265359 * pow(squared_radius, 3. / 2.);
+    *drat = (3.0 - beta) / (2.0 * beta);
+}
+
+uint64_t xs_get(const unsigned short xs[3])
+{
+    return (uint64_t)xs[0] | ((uint64_t)xs[1] << 16) | ((uint64_t)xs[2] << 32);
+}
+
+void xs_set(unsigned short xs[3], uint64_t v)
+{
+    xs[0] = (unsigned short)(v & 0xFFFF);
+    xs[1] = (unsigned short)((v >> 16) & 0xFFFF);
+    xs[2] = (unsigned short)((v >> 32) & 0xFFFF);
+}
+
+void free_state(Particles *P)
+{
+    void *ptrs[] = {P->d_state, P->d_alive, P->d_ndraw, P->d_bsum, P->d_kin, P->d_vin, P->d_vout, P->d_sort_tmp,
+                    P->d_keys[0], P->d_keys[1]};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    P->d_state = nullptr;
+    P->d_alive = P->d_ndraw = P->d_bsum = nullptr;
+    P->d_kin = P->d_vin = P->d_vout = nullptr;
+    P->d_sort_tmp = nullptr;
+    P->d_keys[0] = P->d_keys[1] = nullptr;
+    P->nkeys[0] = P->nkeys[1] = 0;
+    P->cap = 0;
+}
+
+// (Re)allocate every array sized by the particle capacity.
+int alloc_state(ch4g_lattice *L, int64_t cap)
+{
+    Particles *P = L->part;
+    free_state(P);
+    const int64_t c = cap > 0 ? cap : 1;
+    int rc;
+    if ((rc = dalloc(L, (void **)&P->d_state, (size_t)PF_COUNT * c * sizeof(double)))) return rc;
+    if ((rc = dalloc(L, (void **)&P->d_alive, (size_t)c * sizeof(int)))) return rc;
+    if ((rc = dalloc(L, (void **)&P->d_ndraw, (size_t)c * sizeof(int)))) return rc;
+    if (random_reinjection(L, P->pc))
+        if ((rc = dalloc(L, (void **)&P->d_bsum, (size_t)((c + 255) / 256) * sizeof(int)))) return rc;
+    if (lhas(P->pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM)) {
+        const size_t M = (size_t)8 * c;
+        if ((rc = dalloc(L, (void **)&P->d_kin, M * sizeof(uint32_t)))) return rc;
+        if ((rc = dalloc(L, (void **)&P->d_vin, M * sizeof(uint32_t)))) return rc;
+        if ((rc = dalloc(L, (void **)&P->d_vout, M * sizeof(uint32_t)))) return rc;
+        for (int b = 0; b < 2; b++)
+            if ((rc = dalloc(L, (void **)&P->d_keys[b], M * sizeof(uint32_t)))) return rc;
+        P->key_bits = 1;
+        while (P->key_bits < 32 && ((uint64_t)1 << P->key_bits) <= (uint64_t)L->N) P->key_bits++;
+        P->sort_tmp_bytes = feedback_sort_bytes((int64_t)M, P->key_bits);
+        if ((rc = dalloc(L, &P->d_sort_tmp, P->sort_tmp_bytes > 0 ? P->sort_tmp_bytes : 1))) return rc;
+    }
+    P->cap = c;
+    return 0;
+}
+
+PartParams part_params(ch4g_lattice *L)
+{
+    Particles *Q = L->part;
+    PartParams P;
+    memset(&P, 0, sizeof P);
+    P.NX = L->cfg.NX;
+    P.NY = L->cfg.NY;
+    P.NZ = L->cfg.NZ;
+    P.N = L->N;
+    P.np = Q->np;
+    P.ld = Q->cap;
+    P.s = Q->d_state;
+    P.alive = Q->d_alive;
+    P.ndraw = Q->d_ndraw;
+    P.bsum = Q->d_bsum;
+    P.rng = Q->d_rng;
+    P.u = L->d_u;
+    P.t = L->d_t;
+    P.flags = Q->pc.flags;
+    P.fluid = L->base.fluid;
+    P.thermal = L->base.thermal;
+    P.wall_f = L->base.wall_f;
+    P.wall_t = L->base.wall_t;
+    P.Tw_bot = L->base.Tw_bot;
+    P.Tw_top = L->base.Tw_top;
+    P.first = L->itime == 1;
+    P.dt = Q->dt;
+    P.grav[0] = L->cfg.gravity_x;
+    P.grav[1] = L->cfg.gravity_y;
+    P.grav[2] = L->cfg.gravity_z;
+    P.beta_t = L->cfg.beta_t;
+    return P;
+}
+
+// Host records -> device rows (constants of the feedback recomputed).
+int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
+{
+    Particles *Q = L->part;
+    const int64_t cap = Q->cap;
+    std::vector<double> rows((size_t)PF_COUNT * cap, 0.0);
+    std::vector<int> alive((size_t)cap, 0);
+    for (int64_t p = 0; p < n; p++) {
+        const ch4g_particle &r = in[p];
+        auto S = [&](int f) -> double & { return rows[(size_t)f * cap + p]; };
+        S(PX) = r.x;
+        S(PY) = r.y;
+        S(PZ) = r.z;
+        S(PVX) = r.vx;
+        S(PVY) = r.vy;
+        S(PVZ) = r.vz;
+        S(PVXO) = r.vx_old;
+        S(PVYO) = r.vy_old;
+        S(PVZO) = r.vz_old;
+        S(PAX) = r.ax;
+        S(PAY) = r.ay;
+        S(PAZ) = r.az;
+        S(PAXO) = r.ax_old;
+        S(PAYO) = r.ay_old;
+        S(PAZO) = r.az_old;
+        S(PUX) = r.ux;
+        S(PUY) = r.uy;
+        S(PUZ) = r.uz;
+        S(PUXO) = r.ux_old;
+        S(PUYO) = r.uy_old;
+        S(PUZO) = r.uz_old;
+        const double g[9] = {r.dx_ux, r.dy_ux, r.dz_ux, r.dx_uy, r.dy_uy, r.dz_uy, r.dx_uz, r.dy_uz, r.dz_uz};
+        for (int m = 0; m < 9; m++) S(PG + m) = g[m];
+        S(PDTU) = r.Dt_ux;
+        S(PDTU + 1) = r.Dt_uy;
+        S(PDTU + 2) = r.Dt_uz;
+        S(PT) = r.t;
+        S(PTOLD) = r.t_old;
+        S(PDTT) = r.dt_t;
+        S(PTAU) = r.tau_drag;
+        S(PBETA) = r.beta_coeff;
+        double vol = 0.0, drat = 0.0;
+        if (lhas(Q->pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM)) feedback_constants(L->nu, r.tau_drag, r.beta_coeff, &vol, &drat);
+        S(PVOL) = vol;
+        S(PDRAT) = drat;
+        S(PNAME) = r.name;
+        alive[(size_t)p] = 1;
+    }
+    HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
+    HIPCHK(hipMemcpyAsync(Q->d_alive, alive.data(), alive.size() * sizeof(int), hipMemcpyHostToDevice, L->stream));
+    HIPCHK(hipMemsetAsync(Q->d_ndraw, 0, (size_t)cap * sizeof(int), L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    Q->np = n;
+    return 0;
+}
+
+}  // namespace
+
+namespace ch4g {
+
+int rng_to_host(ch4g_lattice *L)
+{
+    uint64_t v = 0;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    HIPCHK(hipMemcpy(&v, L->part->d_rng, sizeof v, hipMemcpyDeviceToHost));
+    xs_set(L->xsubi, v);
+    return 0;
+}
+
+int rng_to_device(ch4g_lattice *L)
+{
+    uint64_t v = xs_get(L->xsubi);
+    HIPCHK(hipStreamSynchronize(L->stream));
+    HIPCHK(hipMemcpy(L->part->d_rng, &v, sizeof v, hipMemcpyHostToDevice));
+    return 0;
+}
+
+void particles_free(ch4g_lattice *L)
+{
+    Particles *P = L->part;
+    if (!P) return;
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    if (P->rng_on_device && P->d_rng) {
+        uint64_t v = 0;
+        if (hipMemcpy(&v, P->d_rng, sizeof v, hipMemcpyDeviceToHost) == hipSuccess) xs_set(L->xsubi, v);
+    }
+    free_state(P);
+    for (int b = 0; b < 2; b++)
+        if (P->d_fb[b]) (void)hipFree(P->d_fb[b]);
+    if (P->d_rng) (void)hipFree(P->d_rng);
+    delete P;
+    L->part = nullptr;
+    L->base.twoway = 0;
+    L->base.fb_prev = L->base.fb_now = nullptr;
+}
+
+// add_particle_feedbacks (main.c:125-130): from the second step on, the
+// deposits of the particles as moved at the end of the previous step.
+int particles_before_step(ch4g_lattice *L, int itime)
+{
+    Particles *Q = L->part;
+    if (!lhas(Q->pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM) || itime < 2) return 0;
+    const int par = itime & 1;
+    PartParams P = part_params(L);
+    FeedbackBufs B;
+    memset(&B, 0, sizeof B);
+    B.kin = Q->d_kin;
+    B.kout = Q->d_keys[par];
+    B.vin = Q->d_vin;
+    B.vout = Q->d_vout;
+    B.tmp = Q->d_sort_tmp;
+    B.tmp_bytes = Q->sort_tmp_bytes;
+    B.key_bits = Q->key_bits;
+    B.old_keys = Q->d_keys[par];
+    B.old_n = Q->nkeys[par];
+    B.fb = Q->d_fb[par];
+    if (lhas(Q->pc, CH4G_LAGRANGE_GRAVITY)) {
+        B.gv[0] = L->cfg.gravity_x;
+        B.gv[1] = -L->cfg.gravity_y;
+        B.gv[2] = L->cfg.gravity_z;
+    }
+    if (launch_feedback(P, B, L->stream)) return fail(-2, "feedback launch failed: %s", hipGetErrorString(hipGetLastError()));
+    Q->nkeys[par] = 8 * Q->np;
+    return 0;
+}
+
+// main.c:163-183: hydro fields of the step (with their halo rules applied in
+// the particle kernel), then time_dt_lagr_factor sub-steps.
+int particles_after_step(ch4g_lattice *L)
+{
+    int rc = ensure_scratch(L);
+    if (rc) return rc;
+    gather(L, nullptr, nullptr, nullptr, L->base.fluid ? L->d_u : nullptr, L->base.thermal ? L->d_t : nullptr);
+    HIPCHK(hipGetLastError());
+    PartParams P = part_params(L);
+    for (int s = 0; s < L->part->substeps; s++)
+        if (launch_particles(P, L->stream)) return fail(-2, "particle launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+}  // namespace ch4g
+
+extern "C" {
+
+int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
+{
+    if (!L) return fail(-1, "null lattice");
+    int rc = validate_particles(L, pc);
+    if (rc) return rc;
+    particles_free(L);
+    destroy_graphs(L);
+    L->part = new (std::nothrow) Particles();
+    if (!L->part) return fail(-3, "out of host memory");
+    Particles *Q = L->part;
+    Q->pc = *pc;
+    Q->dt = 1.0 / pc->time_dt_lagr_factor; // parameters.c:164 with time_dt = 1
+    Q->substeps = (int)pc->time_dt_lagr_factor;
+
+    // particle types (parameters.c:559-584, :743-755; lagrange.c:264-336)
+    const bool am = lhas(*pc, CH4G_LAGRANGE_GRADIENT) && lhas(*pc, CH4G_LAGRANGE_ADDEDMASS);
+    double types = pc->tau_drag_types;
+    if (am) types *= pc->beta_coeff_types;
+    const int ntr = lhas(*pc, CH4G_LAGRANGE_ADD_TRACER) ? 1 : 0;
+    types += ntr;
+    const int nt = (int)types;
+    std::vector<double> tau((size_t)nt, 0.0), beta((size_t)nt, 0.0);
+    double rep = 1.0;
+    ladder(tau, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
+           lhas(*pc, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
+    rep *= pc->tau_drag_types;
+    if (am) {
+        ladder(beta, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
+               lhas(*pc, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
+        rep *= pc->beta_coeff_types;
+    }
+
+    // names, properties and positions (lagrange.c:634-641, :711-771): three
+    // draws per particle from the lattice's stream, zero velocity
+    const int64_t np = pc->particle_number;
+    std::vector<ch4g_particle> h((size_t)(np > 0 ? np : 1));
+    memset(h.data(), 0, h.size() * sizeof(ch4g_particle));
+    const double LNX = L->cfg.NX, LNY = L->cfg.NY, LNZ = L->cfg.NZ;
+    for (int64_t i = 0; i < np; i++) {
+        ch4g_particle &r = h[(size_t)i];
+        r.name = (double)i;
+        const int type = ((int)r.name) % nt;
+        r.tau_drag = tau[(size_t)type];
+        if (am) r.beta_coeff = beta[(size_t)type];
+        r.x = 0 + erand48(L->xsubi) * LNX;
+        r.y = 0 + erand48(L->xsubi) * LNY;
+        r.z = 0 + erand48(L->xsubi) * LNZ;
+    }
+    if ((rc = alloc_state(L, np))) goto err;
+    if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM)) {
+        for (int b = 0; b < 2; b++) {
+            if ((rc = dalloc(L, (void **)&Q->d_fb[b], (size_t)3 * L->N * sizeof(double)))) goto err;
+            if (hipMemsetAsync(Q->d_fb[b], 0, (size_t)3 * L->N * sizeof(double), L->stream) != hipSuccess) {
+                rc = fail(-2, "hipMemsetAsync failed");
+                goto err;
+            }
+        }
+        L->base.twoway = 1;
+        L->base.forced = 1; // the feedback enters through the force field
+    }
+    if (random_reinjection(L, *pc)) {
+        if ((rc = dalloc(L, (void **)&Q->d_rng, 2 * sizeof(uint64_t)))) goto err;
+        Q->rng_on_device = 1;
+        if ((rc = rng_to_device(L))) goto err;
+    }
+    if ((rc = upload(L, h.data(), np))) goto err;
+    return 0;
+err:
+    particles_free(L);
+    return rc;
+}
+
+int ch4g_particles_count(ch4g_lattice *L, int64_t *n)
+{
+    if (!L || !n) return fail(-1, "null argument");
+    *n = 0;
+    Particles *Q = L->part;
+    if (!Q || Q->np == 0) return 0;
+    std::vector<int> alive((size_t)Q->np);
+    HIPCHK(hipMemcpyAsync(alive.data(), Q->d_alive, alive.size() * sizeof(int), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    int64_t c = 0;
+    for (int a : alive) c += a != 0;
+    *n = c;
+    return 0;
+}
+
+int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t *n)
+{
+    if (!L || !n || (cap > 0 && !out)) return fail(-1, "null argument");
+    *n = 0;
+    Particles *Q = L->part;
+    if (!Q || Q->np == 0) return 0;
+    const int64_t ld = Q->cap, np = Q->np;
+    std::vector<double> rows((size_t)PF_COUNT * ld);
+    std::vector<int> alive((size_t)np);
+    HIPCHK(hipMemcpyAsync(rows.data(), Q->d_state, rows.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipMemcpyAsync(alive.data(), Q->d_alive, alive.size() * sizeof(int), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    int64_t k = 0;
+    for (int64_t p = 0; p < np; p++) {
+        if (!alive[(size_t)p]) continue;
+        if (k < cap) {
+            auto S = [&](int f) { return rows[(size_t)f * ld + p]; };
+            ch4g_particle &r = out[k];
+            r.name = S(PNAME);
+            r.x = S(PX);
+            r.y = S(PY);
+            r.z = S(PZ);
+            r.vx = S(PVX);
+            r.vy = S(PVY);
+            r.vz = S(PVZ);
+            r.vx_old = S(PVXO);
+            r.vy_old = S(PVYO);
+            r.vz_old = S(PVZO);
+            r.ax = S(PAX);
+            r.ay = S(PAY);
+            r.az = S(PAZ);
+            r.ax_old = S(PAXO);
+            r.ay_old = S(PAYO);
+            r.az_old = S(PAZO);
+            r.ux = S(PUX);
+            r.uy = S(PUY);
+            r.uz = S(PUZ);
+            r.ux_old = S(PUXO);
+            r.uy_old = S(PUYO);
+            r.uz_old = S(PUZO);
+            double *g = &r.dx_ux;
+            for (int m = 0; m < 9; m++) g[m] = S(PG + m);
+            r.Dt_ux = S(PDTU);
+            r.Dt_uy = S(PDTU + 1);
+            r.Dt_uz = S(PDTU + 2);
+            r.t = S(PT);
+            r.t_old = S(PTOLD);
+            r.dt_t = S(PDTT);
+            r.tau_drag = S(PTAU);
+            r.beta_coeff = S(PBETA);
+        }
+        k++;
+    }
+    *n = k;
+    return 0;
+}
+
+int ch4g_set_particles(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
+{
+    if (!L || (n > 0 && !in)) return fail(-1, "null argument");
+    if (!L->part) return fail(-1, "ch4g_particles_init first");
+    if (n < 0 || n * 8 > (int64_t)1 << 31) return fail(-1, "particle count out of range");
+    int rc;
+    if (n > L->part->cap) {
+        HIPCHK(hipStreamSynchronize(L->stream));
+        if ((rc = alloc_state(L, n))) return rc;
+        // the deposit keys that located the feedback nodes are gone
+        for (int b = 0; b < 2; b++)
+            if (L->part->d_fb[b])
+                HIPCHK(hipMemsetAsync(L->part->d_fb[b], 0, (size_t)3 * L->N * sizeof(double), L->stream));
+    }
+    return upload(L, in, n);
+}
+
+}  // extern "C"

@@ -57,8 +57,8 @@ __device__ __forceinline__ Row row_of(const KParams &P, int y)
 
 // force of one node from its temperature deviation dT (buoyancy) and the
 // HIT tables; same term order as build_forcing.
-__device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab, int x, int y, int z, float theta,
-                                           float F[3])
+__device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab, const double *fb, uint32_t n, int x,
+                                           int y, int z, float theta, float F[3])
 {
     F[0] = (float)P.Fc[0];
     F[1] = (float)P.Fc[1];
@@ -79,6 +79,12 @@ __device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab
         F[0] += fac * (float)P.grav[0];
         F[1] += fac * (float)P.grav[1];
         F[2] += fac * (float)P.grav[2];
+    }
+    if (P.twoway) {
+        const size_t N = (size_t)P.N;
+        F[0] += (float)fb[n];
+        F[1] += (float)fb[N + n];
+        F[2] += (float)fb[2 * N + n];
     }
 }
 
@@ -184,7 +190,7 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
         uz = jz * irho;
         if (P.forced) {
             float Fp[3];
-            force_at32(P, P.hit_prev, x, y, z, R.offT + dTlag, Fp);
+            force_at32(P, P.hit_prev, P.fb_prev, n, x, y, z, R.offT + dTlag, Fp);
             ux += A(0.5) * Fp[0];
             uy += A(0.5) * Fp[1];
             uz += A(0.5) * Fp[2];
@@ -218,7 +224,7 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
             omega = A(1) / tau;
         }
         float Fn[3] = {0.f, 0.f, 0.f};
-        if (P.forced) force_at32(P, P.hit_now, x, y, z, R.offT + (float)dT, Fn);
+        if (P.forced) force_at32(P, P.hit_now, P.fb_now, n, x, y, z, R.offT + (float)dT, Fn);
         const A uF = ux * Fn[0] + uy * Fn[1] + uz * Fn[2];
         // Guo term folded into the equilibrium: f* = f - omega (f - eqG) with
         // eqG = f_eq + G/omega, so the O(population) value is rounded twice
@@ -385,7 +391,7 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
         double ux = jx / rho, uy = jy / rho, uz = jz / rho;
         if (P.forced) {
             float F[3];
-            force_at32(P, P.hit_now, x, y, z, R.offT + dTlag, F);
+            force_at32(P, P.hit_now, P.fb_now, n, x, y, z, R.offT + dTlag, F);
             ux += 0.5 * (double)F[0];
             uy += 0.5 * (double)F[1];
             uz += 0.5 * (double)F[2];

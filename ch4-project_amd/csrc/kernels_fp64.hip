@@ -37,8 +37,8 @@ __device__ __forceinline__ double m_z(const double *p)
 }
 
 // build_forcing for one node (forcing.c:361-667), terms added in source order.
-__device__ __forceinline__ void force_at(const KParams &P, const HitTable &tab, int x, int y, int z, double T,
-                                         double F[3])
+__device__ __forceinline__ void force_at(const KParams &P, const HitTable &tab, const double *fb, uint32_t n, int x,
+                                         int y, int z, double T, double F[3])
 {
     F[0] = P.Fc[0];
     F[1] = P.Fc[1];
@@ -61,6 +61,12 @@ __device__ __forceinline__ void force_at(const KParams &P, const HitTable &tab, 
         F[0] += fac * P.grav[0];
         F[1] += fac * P.grav[1];
         F[2] += fac * P.grav[2];
+    }
+    if (P.twoway) {
+        const size_t N = (size_t)P.N;
+        F[0] += fb[n];
+        F[1] += fb[N + n];
+        F[2] += fb[2 * N + n];
     }
 }
 
@@ -177,7 +183,7 @@ __global__ void __launch_bounds__(256) k_step_fp64(KParams P, const double *__re
         const double rho = m_sum(f);
         if (P.forced) {
             double Fp[3];
-            force_at(P, P.hit_prev, x, y, z, Tlag, Fp);
+            force_at(P, P.hit_prev, P.fb_prev, n, x, y, z, Tlag, Fp);
             ux = m_x(f) / rho + 0.5 * 1.0 * Fp[0];
             uy = m_y(f) / rho + 0.5 * 1.0 * Fp[1];
             uz = m_z(f) / rho + 0.5 * 1.0 * Fp[2];
@@ -195,7 +201,7 @@ __global__ void __launch_bounds__(256) k_step_fp64(KParams P, const double *__re
             invtau = 1.0 / tau;
         }
         double Fn[3] = {0.0, 0.0, 0.0};
-        if (P.forced) force_at(P, P.hit_now, x, y, z, T, Fn);
+        if (P.forced) force_at(P, P.hit_now, P.fb_now, n, x, y, z, T, Fn);
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             double r = f[q];
@@ -266,7 +272,7 @@ __global__ void __launch_bounds__(256) k_gather_fp64(KParams P, const double *__
         double ux, uy, uz;
         if (P.forced) {
             double F[3];
-            force_at(P, P.hit_now, x, y, z, Tlag, F);
+            force_at(P, P.hit_now, P.fb_now, n, x, y, z, Tlag, F);
             ux = m_x(f) / rho + 0.5 * 1.0 * F[0];
             uy = m_y(f) / rho + 0.5 * 1.0 * F[1];
             uz = m_z(f) / rho + 0.5 * 1.0 * F[2];

@@ -1,0 +1,730 @@
+// Lagrangian point particles on the device: one thread per particle for the
+// interpolation + advance + wall rules, an ordered re-injection pass that
+// hands out drand48 draws in particle order by jumping the 48-bit LCG, and
+// the two-way feedback as a sort of the deposits by node followed by
+// per-node sums in particle order.
+//
+// Compiled with -ffp-contract=off: every expression is evaluated in the
+// reference's order (lagrange.c, lagrange_feedback.c, gradients.c), so
+// positions, velocities and deposits are bit-identical to the reference's
+// FP64 loops on the same fields.
+#include <hipcub/hipcub.hpp>
+
+#include "ch4g_internal.h"
+
+namespace ch4g {
+namespace {
+
+constexpr int PB = 256;                         // particles per block
+constexpr double FLT_EPS = 1.1920928955078125e-07;  // FLT_EPSILON (float.h) as a double
+constexpr uint64_t MASK48 = (1ull << 48) - 1;
+
+__device__ __forceinline__ bool lflag(const PartParams &P, uint32_t f) { return (P.flags & f) != 0; }
+
+__device__ __forceinline__ int wrapi(int i, int n)
+{
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// wrap(x, s) = x - floor(x / s) * s (typedef.h:378)
+__device__ __forceinline__ double wrapd(double x, double s) { return x - floor(x / s) * s; }
+
+// drand48 state after k steps from x: X <- a X + c (mod 2^48) composed by
+// squaring (all maps are powers of the same affine map, so they commute).
+__device__ __forceinline__ uint64_t lcg_jump(uint64_t x, uint64_t k)
+{
+    uint64_t am = 0x5DEECE66Dull, ac = 0xBull, m = 1, c = 0;
+    while (k) {
+        if (k & 1) {
+            m = m * am;
+            c = c * am + ac;
+        }
+        ac = (am + 1) * ac;
+        am = am * am;
+        k >>= 1;
+    }
+    return (m * x + c) & MASK48;
+}
+
+__device__ __forceinline__ double lcg_value(uint64_t x) { return ldexp((double)x, -48); }
+
+// Corners of the trilinear stencil in the order of the reference's sums
+// (lagrange.c:1293-1319): (m,m,m) (p,m,m) (m,p,m) (m,m,p) (p,p,m) (m,p,p) (p,m,p) (p,p,p).
+__device__ __forceinline__ int cdx(int c) { return (c == 1 || c == 4 || c == 6 || c == 7) ? 1 : 0; }
+__device__ __forceinline__ int cdy(int c) { return (c == 2 || c == 4 || c == 5 || c == 7) ? 1 : 0; }
+__device__ __forceinline__ int cdz(int c) { return (c == 3 || c == 5 || c == 6 || c == 7) ? 1 : 0; }
+
+struct Stencil {
+    int im, jm, km;   // logical cell of the (m,m,m) corner, -1 .. N-1
+    double w[8];
+};
+
+// Cell lookup and volume coefficients (lagrange.c:1232-1292).  The
+// reference scans the cell centres c + 1/2 for c + 1/2 <= x < c + 3/2;
+// floor(x - 1/2) is that cell exactly (x - 1/2 is exact for x >= 1/4 and
+// cannot cross an integer below it).
+__device__ __forceinline__ bool stencil(const PartParams &P, double x, double y, double z, Stencil &st)
+{
+    const double px = wrapd(x, (double)P.NX), py = wrapd(y, (double)P.NY), pz = wrapd(z, (double)P.NZ);
+    const double fx = floor(px - 0.5), fy = floor(py - 0.5), fz = floor(pz - 0.5);
+    if (!(fx >= -1.0 && fx <= P.NX - 1.0 && fy >= -1.0 && fy <= P.NY - 1.0 && fz >= -1.0 && fz <= P.NZ - 1.0))
+        return false;
+    st.im = (int)fx;
+    st.jm = (int)fy;
+    st.km = (int)fz;
+    const double dxm = px - ((double)st.im + 0.5), dxp = ((double)(st.im + 1) + 0.5) - px;
+    const double dym = py - ((double)st.jm + 0.5), dyp = ((double)(st.jm + 1) + 0.5) - py;
+    const double dzm = pz - ((double)st.km + 0.5), dzp = ((double)(st.km + 1) + 0.5) - pz;
+    st.w[0] = dxp * dyp * dzp;
+    st.w[1] = dxm * dyp * dzp;
+    st.w[2] = dxp * dym * dzp;
+    st.w[3] = dxp * dyp * dzm;
+    st.w[4] = dxm * dym * dzp;
+    st.w[5] = dxp * dym * dzm;
+    st.w[6] = dxm * dyp * dzm;
+    st.w[7] = dxm * dym * dzm;
+    return true;
+}
+
+// Velocity at a logical node including the halo of boundary_conditions_hydro
+// (boundary_conditions.c:2786-2875): periodic images, then u_g = -u on the
+// first ghost plane of each no-slip y wall.
+__device__ __forceinline__ void fetch_u(const PartParams &P, int i, int j, int k, double v[3])
+{
+    i = wrapi(i, P.NX);
+    k = wrapi(k, P.NZ);
+    bool ghost = false;
+    if (P.wall_f && j == -1) {
+        j = 0;
+        ghost = true;
+    } else if (P.wall_f && j == P.NY) {
+        j = P.NY - 1;
+        ghost = true;
+    } else {
+        j = wrapi(j, P.NY);
+    }
+    const size_t n = ((size_t)k * P.NY + j) * P.NX + i;
+    const double *u = P.u + 3 * n;
+    if (ghost) {
+        v[0] = -1.0 * u[0];
+        v[1] = -1.0 * u[1];
+        v[2] = -1.0 * u[2];
+    } else {
+        v[0] = u[0];
+        v[1] = u[1];
+        v[2] = u[2];
+    }
+}
+
+// Temperature with the fixed-T ghost t_g = (2 (T_w - T_ref) / t - 1) t
+// (boundary_conditions.c:2928-2945).
+__device__ __forceinline__ double fetch_t(const PartParams &P, int i, int j, int k)
+{
+    i = wrapi(i, P.NX);
+    k = wrapi(k, P.NZ);
+    int side = 0;
+    if (P.wall_t && j == -1) {
+        j = 0;
+        side = -1;
+    } else if (P.wall_t && j == P.NY) {
+        j = P.NY - 1;
+        side = 1;
+    } else {
+        j = wrapi(j, P.NY);
+    }
+    const double t = P.t[((size_t)k * P.NY + j) * P.NX + i];
+    if (side == 0) return t;
+    const double Tw = side < 0 ? P.Tw_bot : P.Tw_top;
+    const double fac = 2.0 * Tw / t - 1.0;
+    return fac * t;
+}
+
+// second-order one-sided combinations with unit spacing: h1 = h2 = 1 gives
+// a0 = -3/2, a1 = 2, a2 = -1/2 forward and the negatives backward
+// (gradients.c:201-225, :372-486).
+__device__ __forceinline__ void one_sided(const double a[3], const double b[3], const double c[3], bool fwd,
+                                          double &X, double &Y, double &Z)
+{
+    const double a0 = fwd ? -1.5 : 1.5, a1 = fwd ? 2.0 : -2.0, a2 = fwd ? -0.5 : 0.5;
+    X = a0 * a[0] + a1 * b[0] + a2 * c[0];
+    Y = a0 * a[1] + a1 * b[1] + a2 * c[1];
+    Z = a0 * a[2] + a1 * b[2] + a2 * c[2];
+}
+
+// gradient_vector with LAGRANGE on a two-node halo (gradients.c:78-499).
+// G = xx, xy, xz, yx, yy, yz, zx, zy, zz (first letter: component of u,
+// second: direction of the derivative).
+__device__ void grad_at(const PartParams &P, int i, int j, int k, double G[9])
+{
+    double a[3], b[3], c[3];
+    // centred differences: both neighbours always exist for nodes -1..N
+    fetch_u(P, i + 1, j, k, a);
+    fetch_u(P, i - 1, j, k, b);
+    G[0] = (a[0] - b[0]) / 2.0;
+    G[3] = (a[1] - b[1]) / 2.0;
+    G[6] = (a[2] - b[2]) / 2.0;
+    fetch_u(P, i, j + 1, k, a);
+    fetch_u(P, i, j - 1, k, b);
+    G[1] = (a[0] - b[0]) / 2.0;
+    G[4] = (a[1] - b[1]) / 2.0;
+    G[7] = (a[2] - b[2]) / 2.0;
+    fetch_u(P, i, j, k + 1, a);
+    fetch_u(P, i, j, k - 1, b);
+    G[2] = (a[0] - b[0]) / 2.0;
+    G[5] = (a[1] - b[1]) / 2.0;
+    G[8] = (a[2] - b[2]) / 2.0;
+    if (P.wall_f) {
+        // LB_FLUID_BC_Y: one-sided at the first and last fluid rows
+        if (j == 0) {
+            fetch_u(P, i, 0, k, a);
+            fetch_u(P, i, 1, k, b);
+            fetch_u(P, i, 2, k, c);
+            one_sided(a, b, c, true, G[1], G[4], G[7]);
+        }
+        if (j == P.NY - 1) {
+            fetch_u(P, i, P.NY - 1, k, a);
+            fetch_u(P, i, P.NY - 2, k, b);
+            fetch_u(P, i, P.NY - 3, k, c);
+            one_sided(a, b, c, false, G[1], G[4], G[7]);
+        }
+        // LAGRANGE_INTERPOLATION_FIX: ghost nodes next to the interior take
+        // the one-sided gradient of the first interior node; the moved index
+        // is kept for the directions that follow
+        if (i == -1) {
+            i = 0;
+            fetch_u(P, 0, j, k, a);
+            fetch_u(P, 1, j, k, b);
+            fetch_u(P, 2, j, k, c);
+            one_sided(a, b, c, true, G[0], G[3], G[6]);
+        }
+        if (i == P.NX) {
+            i = P.NX - 1;
+            fetch_u(P, i, j, k, a);
+            fetch_u(P, i - 1, j, k, b);
+            fetch_u(P, i - 2, j, k, c);
+            one_sided(a, b, c, false, G[0], G[3], G[6]);
+        }
+        if (j == -1) {
+            j = 0;
+            fetch_u(P, i, 0, k, a);
+            fetch_u(P, i, 1, k, b);
+            fetch_u(P, i, 2, k, c);
+            one_sided(a, b, c, true, G[1], G[4], G[7]);
+        }
+        if (j == P.NY) {
+            j = P.NY - 1;
+            fetch_u(P, i, j, k, a);
+            fetch_u(P, i, j - 1, k, b);
+            fetch_u(P, i, j - 2, k, c);
+            one_sided(a, b, c, false, G[1], G[4], G[7]);
+        }
+        if (k == -1) {
+            k = 0;
+            fetch_u(P, i, j, 0, a);
+            fetch_u(P, i, j, 1, b);
+            fetch_u(P, i, j, 2, c);
+            one_sided(a, b, c, true, G[2], G[5], G[8]);
+        }
+        if (k == P.NZ) {
+            k = P.NZ - 1;
+            fetch_u(P, i, j, k, a);
+            fetch_u(P, i, j, k - 1, b);
+            fetch_u(P, i, j, k - 2, c);
+            one_sided(a, b, c, false, G[2], G[5], G[8]);
+        }
+    }
+    if (P.NX == 1) G[0] = G[3] = G[6] = 0.0;
+    if (P.NY == 1) G[1] = G[4] = G[7] = 0.0;
+    if (P.NZ == 1) G[2] = G[5] = G[8] = 0.0;
+}
+
+// sendrecv_particles on one rank (lagrange.c:4060-4116): wrap (with
+// LAGRANGE_WRAP), keep the particle when its wrapped position lies in [0, S).
+__device__ __forceinline__ void wrap_keep(const PartParams &P, double &x, double &y, double &z, int64_t p)
+{
+    const double SX = P.NX, SY = P.NY, SZ = P.NZ;
+    const double px = wrapd(x, SX), py = wrapd(y, SY), pz = wrapd(z, SZ);
+    if (lflag(P, CH4G_LAGRANGE_WRAP)) {
+        x = wrapd(x, SX);
+        z = wrapd(z, SZ);
+        y = wrapd(y, SY);
+    }
+    if (!(px >= 0.0 && px < SX && py >= 0.0 && py < SY && pz >= 0.0 && pz < SZ)) P.alive[p] = 0;
+}
+
+// One sub-step of every particle: interpolate_vector_at_particles(u),
+// interpolate_scalar_at_particles(t) (lagrange.c:1211-1593), move_particles
+// (lagrange.c:2934-4025) and the wrap of sendrecv_particles.
+__global__ void __launch_bounds__(PB) k_particles(PartParams P)
+{
+    const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
+    int nd = 0;
+    if (p < P.np && P.alive[p]) {
+        double *S = P.s;
+        const int64_t ld = P.ld;
+#define R(f) S[(int64_t)(f) * ld + p]
+        double x = R(PX), y = R(PY), z = R(PZ);
+        double vx = R(PVX), vy = R(PVY), vz = R(PVZ);
+        double vxo = R(PVXO), vyo = R(PVYO), vzo = R(PVZO);
+        double ax = R(PAX), ay = R(PAY), az = R(PAZ);
+        double axo = R(PAXO), ayo = R(PAYO), azo = R(PAZO);
+        double ux = R(PUX), uy = R(PUY), uz = R(PUZ);
+        double uxo = R(PUXO), uyo = R(PUYO), uzo = R(PUZO);
+        double G[9];
+        for (int m = 0; m < 9; m++) G[m] = R(PG + m);
+        double dtu[3] = {R(PDTU), R(PDTU + 1), R(PDTU + 2)};
+        double t = R(PT), told = R(PTOLD), dtt = R(PDTT);
+        const double tau = R(PTAU), beta = R(PBETA);
+
+        // ---- interpolation at the (wrapped) position
+        Stencil st;
+        if (stencil(P, x, y, z, st)) {
+            if (P.fluid) {
+                double cu[3];
+                double v0 = 0, v1 = 0, v2 = 0;
+                for (int c = 0; c < 8; c++) {
+                    fetch_u(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), cu);
+                    if (c == 0) {
+                        v0 = cu[0] * st.w[0];
+                        v1 = cu[1] * st.w[0];
+                        v2 = cu[2] * st.w[0];
+                    } else {
+                        v0 += cu[0] * st.w[c];
+                        v1 += cu[1] * st.w[c];
+                        v2 += cu[2] * st.w[c];
+                    }
+                }
+                ux = v0;
+                uy = v1;
+                uz = v2;
+                if (lflag(P, CH4G_LAGRANGE_GRADIENT)) {
+                    double acc[9], Gc[9];
+                    for (int c = 0; c < 8; c++) {
+                        grad_at(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), Gc);
+                        for (int m = 0; m < 9; m++) acc[m] = (c == 0) ? Gc[m] * st.w[0] : acc[m] + Gc[m] * st.w[c];
+                    }
+                    for (int m = 0; m < 9; m++) G[m] = acc[m];
+                }
+            }
+            if (P.thermal) {
+                double s = 0;
+                for (int c = 0; c < 8; c++) {
+                    double tc = fetch_t(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c));
+                    s = (c == 0) ? tc * st.w[0] : s + tc * st.w[c];
+                }
+                t = s;
+            }
+        }
+
+        // ---- move_particles
+        const double dt = P.dt;
+        const bool first = P.first;
+        const double gx = P.grav[0], gy = P.grav[1], gz = P.grav[2];
+        if (P.thermal) {
+            if (first) {
+                told = t;
+            } else {
+                dtt = (t - told) / dt;
+                told = t;
+            }
+        }
+        if (tau == 0.0) {  // fluid tracer (lagrange.c:3010-3227)
+            vx = ux;
+            vy = uy;
+            vz = uz;
+            if (first) {
+                vxo = vx;
+                vyo = vy;
+                vzo = vz;
+            }
+            ax = (vx - vxo) / dt;
+            ay = (vy - vyo) / dt;
+            az = (vz - vzo) / dt;
+            if (first) {
+                x += dt * vx;
+                y += dt * vy;
+                z += dt * vz;
+            } else {
+                x += dt * 0.5 * (3.0 * vx - vxo);
+                y += dt * 0.5 * (3.0 * vy - vyo);
+                z += dt * 0.5 * (3.0 * vz - vzo);
+            }
+            vxo = vx;
+            vyo = vy;
+            vzo = vz;
+            uxo = ux;
+            uyo = uy;
+            uzo = uz;
+        }
+        if (tau > 0.0) {  // inertial particle (lagrange.c:3229-3466)
+            double dux = 0.0, duy = 0.0, duz = 0.0;
+            const double invtau = 1.0 / tau;
+            ax = (ux - vx) * invtau;
+            ay = (uy - vy) * invtau;
+            az = (uz - vz) * invtau;
+            const bool grav = lflag(P, CH4G_LAGRANGE_GRAVITY);
+            if (grav) {
+                ax -= gx;
+                ay -= gy;
+                az -= gz;
+            }
+            if (lflag(P, CH4G_LAGRANGE_ADDEDMASS) && beta != 0.0) {
+                if (grav) {
+                    ax -= (-beta) * gx;
+                    ay -= (-beta) * gy;
+                    az -= (-beta) * gz;
+                }
+                if (first) {
+                    uxo = ux;
+                    uyo = uy;
+                    uzo = uz;
+                }
+                dux = (ux - uxo) / dt + (ux - vx) * G[0] + (uy - vy) * G[1] + (uz - vz) * G[2];
+                duy = (uy - uyo) / dt + (ux - vx) * G[3] + (uy - vy) * G[4] + (uz - vz) * G[5];
+                duz = (uz - uzo) / dt + (ux - vx) * G[6] + (uy - vy) * G[7] + (uz - vz) * G[8];
+                ax += dux * beta;
+                ay += duy * beta;
+                az += duz * beta;
+                dtu[0] = dux;
+                dtu[1] = duy;
+                dtu[2] = duz;
+            }
+            if (first) {
+                x += dt * vx;
+                y += dt * vy;
+                z += dt * vz;
+            } else {
+                x += dt * 0.5 * (3.0 * vx - vxo);
+                y += dt * 0.5 * (3.0 * vy - vyo);
+                z += dt * 0.5 * (3.0 * vz - vzo);
+            }
+            vxo = vx;
+            vyo = vy;
+            vzo = vz;
+            uxo = ux;
+            uyo = uy;
+            uzo = uz;
+            if (first) {
+                vx += dt * ax;
+                vy += dt * ay;
+                vz += dt * az;
+            } else {
+                vx += dt * 0.5 * (3.0 * ax - axo);
+                vy += dt * 0.5 * (3.0 * ay - ayo);
+                vz += dt * 0.5 * (3.0 * az - azo);
+            }
+            axo = ax;
+            ayo = ay;
+            azo = az;
+            if (lflag(P, CH4G_LAGRANGE_SMALLTAUD_BETA) && tau < 10.0 && tau > 0.0) {
+                vx = ux;
+                vy = uy;
+                vz = uz;
+                vx -= gx * (1.0 - beta) * tau;
+                vy -= gy * (1.0 - beta) * tau;
+                vz -= gz * (1.0 - beta) * tau;
+                vx -= dux * (1.0 - beta) * tau;
+                vy -= duy * (1.0 - beta) * tau;
+                vz -= duz * (1.0 - beta) * tau;
+            }
+            if (lflag(P, CH4G_LAGRANGE_SMALLTAUD_FLOATER) && tau > 0.0) {
+                vx = ux;
+                vy = uy;
+                vz = uz;
+                const double f1 = tau * (1.0 - beta + 2. / 3. * P.beta_t * (t - 0.4));
+                vx += f1 * (-gx - dtu[0]);
+                vy += f1 * (-gy - dtu[1]);
+                vz += f1 * (-gz - dtu[2]);
+            }
+        }
+
+        // ---- wall rules at y (lagrange.c:3754-3993); x and z of a random
+        // re-injection are filled in by the ordered draw pass
+        if (P.wall_f) {
+            const bool inel = lflag(P, CH4G_LAGRANGE_BC_INELASTIC);
+            const bool reinj = lflag(P, CH4G_LAGRANGE_BC_INELASTIC_REINJECT);
+            const bool norand = lflag(P, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM);
+            const bool vzero = lflag(P, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO);
+            const double fac1 = inel ? 0.0 : -1.0, fac2 = inel ? 0.0 : 1.0;
+            const double radius = 0.0;
+            const double SY = P.NY;
+            if (y < radius) {
+                if (!reinj) {
+                    y = fac1 * (y - radius) + radius;
+                    vx *= fac2;
+                    if (vy < 0.0) vy *= fac1;
+                    vz *= fac2;
+                }
+                if (inel && reinj) {
+                    y += SY;
+                    if (!norand) nd += 2;
+                    if (vzero) {
+                        vx = 0.0;
+                        vy = 0.0;
+                        vz = 0.0;
+                    } else {
+                        vx = -gx * (1.0 - beta) * tau;
+                        vy = -gy * (1.0 - beta) * tau;
+                        vz = -gz * (1.0 - beta) * tau;
+                    }
+                    if (lflag(P, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF)) {
+                        vx /= 2.0;
+                        vy /= 2.0;
+                        vz /= 2.0;
+                    }
+                }
+            }
+            if (y >= SY - radius) {
+                if (!reinj) {
+                    y = fac1 * (y - (SY - radius)) + (SY - radius);
+                    vx *= fac2;
+                    if (vy > 0.0) vy *= fac1;
+                    vz *= fac2;
+                }
+                if (inel && reinj) {
+                    y -= (SY - FLT_EPS);
+                    if (!norand) nd += 2;
+                    if (vzero) {
+                        vx = 0.0;
+                        vy = 0.0;
+                        vz = 0.0;
+                    } else {
+                        vx = -gx * (1.0 - beta) * tau;
+                        vy = -gy * (1.0 - beta) * tau;
+                        vz = -gz * (1.0 - beta) * tau;
+                    }
+                }
+            }
+        }
+        if (nd == 0) wrap_keep(P, x, y, z, p);
+
+        R(PX) = x;
+        R(PY) = y;
+        R(PZ) = z;
+        R(PVX) = vx;
+        R(PVY) = vy;
+        R(PVZ) = vz;
+        R(PVXO) = vxo;
+        R(PVYO) = vyo;
+        R(PVZO) = vzo;
+        R(PAX) = ax;
+        R(PAY) = ay;
+        R(PAZ) = az;
+        R(PAXO) = axo;
+        R(PAYO) = ayo;
+        R(PAZO) = azo;
+        R(PUX) = ux;
+        R(PUY) = uy;
+        R(PUZ) = uz;
+        R(PUXO) = uxo;
+        R(PUYO) = uyo;
+        R(PUZO) = uzo;
+        for (int m = 0; m < 9; m++) R(PG + m) = G[m];
+        R(PDTU) = dtu[0];
+        R(PDTU + 1) = dtu[1];
+        R(PDTU + 2) = dtu[2];
+        R(PT) = t;
+        R(PTOLD) = told;
+        R(PDTT) = dtt;
+#undef R
+    }
+    if (P.bsum) {
+        // draws of this block, for the ordered re-injection pass
+        if (p < P.np) P.ndraw[p] = nd;
+        __shared__ int red[PB / 64];
+        int v = nd;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int s = 0;
+            for (int w = 0; w < PB / 64; w++) s += red[w];
+            P.bsum[blockIdx.x] = s;
+        }
+    }
+}
+
+// Exclusive scan of the per-block draw counts (one block), then the stream
+// state advances past every draw of the sub-step.  rng[1] keeps the state
+// the draws start from.
+__global__ void __launch_bounds__(1024) k_draw_offsets(int *bsum, int nb, uint64_t *rng)
+{
+    __shared__ int part[1024];
+    __shared__ long long carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? bsum[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int a = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+            __syncthreads();
+            part[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = (int)(carry + part[threadIdx.x] - v);
+        __syncthreads();
+        if (threadIdx.x == 0) carry += part[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        rng[1] = rng[0];
+        rng[0] = lcg_jump(rng[0], (uint64_t)carry);
+    }
+}
+
+// Re-injected particles take x = SX r, z = SZ r from their draws (the last
+// pair wins when a particle crosses both walls), then sendrecv's wrap.
+__global__ void __launch_bounds__(PB) k_draw_apply(PartParams P)
+{
+    const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
+    const int nd = p < P.np ? P.ndraw[p] : 0;
+    __shared__ int sc[PB];
+    sc[threadIdx.x] = nd;
+    __syncthreads();
+    for (int o = 1; o < PB; o <<= 1) {
+        int a = threadIdx.x >= o ? sc[threadIdx.x - o] : 0;
+        __syncthreads();
+        sc[threadIdx.x] += a;
+        __syncthreads();
+    }
+    if (nd == 0) return;
+    const uint64_t off = (uint64_t)P.bsum[blockIdx.x] + (uint64_t)(sc[threadIdx.x] - nd);
+    const uint64_t base = P.rng[1];
+    const double rx = lcg_value(lcg_jump(base, off + nd - 1));
+    const double rz = lcg_value(lcg_jump(base, off + nd));
+    double *S = P.s;
+    double x = P.NX * rx, z = P.NZ * rz, y = S[(int64_t)PY * P.ld + p];
+    wrap_keep(P, x, y, z, p);
+    S[(int64_t)PX * P.ld + p] = x;
+    S[(int64_t)PY * P.ld + p] = y;
+    S[(int64_t)PZ * P.ld + p] = z;
+}
+
+// ---------------------------------------------------------------------------
+// Two-way feedback (lagrange_feedback.c:11-256).
+
+__global__ void k_fb_clear(const uint32_t *keys, int64_t n, double *fb, int64_t N)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t k = keys[e];
+    if ((int64_t)k >= N) return;
+    fb[k] = 0.0;
+    fb[N + k] = 0.0;
+    fb[2 * N + k] = 0.0;
+}
+
+// Deposit keys: the node of each of the eight corners when it is an
+// interior node (add_sendrecv_borders_vector only ever adds into ghost
+// nodes, lagrange_feedback.c:525-836, so ghost deposits never reach the
+// force); N (sorts last) otherwise.  Particles with zero volume (tracers)
+// deposit exact zeros and are skipped.
+__global__ void __launch_bounds__(PB) k_fb_keys(PartParams P, uint32_t *kin, uint32_t *vin)
+{
+    const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (p >= P.np) return;
+    const uint32_t sent = (uint32_t)P.N;
+    Stencil st;
+    const double *S = P.s;
+    bool ok = P.alive[p] && S[(int64_t)PVOL * P.ld + p] != 0.0 &&
+              stencil(P, S[(int64_t)PX * P.ld + p], S[(int64_t)PY * P.ld + p], S[(int64_t)PZ * P.ld + p], st);
+    for (int c = 0; c < 8; c++) {
+        uint32_t key = sent;
+        if (ok) {
+            const int i = st.im + cdx(c), j = st.jm + cdy(c), k = st.km + cdz(c);
+            if (i >= 0 && i < P.NX && j >= 0 && j < P.NY && k >= 0 && k < P.NZ)
+                key = (uint32_t)(((int64_t)k * P.NY + j) * P.NX + i);
+        }
+        kin[8 * p + c] = key;
+        vin[8 * p + c] = (uint32_t)(8 * p + c);
+    }
+}
+
+// Per node, the deposits in particle order (the sort is stable and the
+// input is particle-major): force_twoway starts at 0 and adds
+// fac * fp * vol for each particle (lagrange_feedback.c:149-176).
+__global__ void __launch_bounds__(PB) k_fb_sum(PartParams P, const uint32_t *keys, const uint32_t *vals, int64_t M,
+                                               double *fb, double g0, double g1, double g2)
+{
+    const int64_t e = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (e >= M) return;
+    const uint32_t key = keys[e];
+    if ((int64_t)key >= P.N) return;
+    if (e > 0 && keys[e - 1] == key) return;
+    const double *S = P.s;
+    const int64_t ld = P.ld;
+    const bool small = lflag(P, CH4G_LAGRANGE_TWOWAY_SMALLTAUD);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int64_t j = e; j < M && keys[j] == key; j++) {
+        const uint32_t v = vals[j];
+        const int64_t p = v >> 3;
+        const int c = (int)(v & 7);
+        Stencil st;
+        stencil(P, S[PX * ld + p], S[PY * ld + p], S[PZ * ld + p], st);
+        const double fac = S[PVOL * ld + p], dr = S[PDRAT * ld + p];
+        double fpx, fpy, fpz;
+        if (small) {
+            fpx = (S[PDTU * ld + p] - g0) * (1 - dr);
+            fpy = (S[(PDTU + 1) * ld + p] - g1) * (1 - dr);
+            fpz = (S[(PDTU + 2) * ld + p] - g2) * (1 - dr);
+        } else {
+            fpx = S[PDTU * ld + p] - g0 + dr * (g0 - S[PAX * ld + p]);
+            fpy = S[(PDTU + 1) * ld + p] - g1 + dr * (g1 - S[PAY * ld + p]);
+            fpz = S[(PDTU + 2) * ld + p] - g2 + dr * (g2 - S[PAZ * ld + p]);
+        }
+        sx += fac * fpx * st.w[c];
+        sy += fac * fpy * st.w[c];
+        sz += fac * fpz * st.w[c];
+    }
+    const int64_t N = P.N;
+    fb[key] = sx;
+    fb[N + key] = sy;
+    fb[2 * N + key] = sz;
+}
+
+int blocks_of(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+int launch_particles(const PartParams &P, hipStream_t s)
+{
+    if (P.np == 0) return 0;
+    const int nb = blocks_of(P.np, PB);
+    hipLaunchKernelGGL(k_particles, dim3(nb), dim3(PB), 0, s, P);
+    if (P.bsum) {
+        hipLaunchKernelGGL(k_draw_offsets, dim3(1), dim3(1024), 0, s, P.bsum, nb, P.rng);
+        hipLaunchKernelGGL(k_draw_apply, dim3(nb), dim3(PB), 0, s, P);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+size_t feedback_sort_bytes(int64_t nitems, int key_bits)
+{
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                             (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)nitems, 0,
+                                             key_bits);
+    return bytes;
+}
+
+int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s)
+{
+    if (B.old_n > 0)
+        hipLaunchKernelGGL(k_fb_clear, dim3(blocks_of(B.old_n, 256)), dim3(256), 0, s, B.old_keys, B.old_n, B.fb, P.N);
+    if (P.np == 0) return hipGetLastError() == hipSuccess ? 0 : -2;
+    const int64_t M = 8 * P.np;
+    hipLaunchKernelGGL(k_fb_keys, dim3(blocks_of(P.np, PB)), dim3(PB), 0, s, P, B.kin, B.vin);
+    size_t bytes = B.tmp_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(B.tmp, bytes, B.kin, B.kout, B.vin, B.vout, (int)M, 0, B.key_bits, s) !=
+        hipSuccess)
+        return -2;
+    hipLaunchKernelGGL(k_fb_sum, dim3(blocks_of(M, PB)), dim3(PB), 0, s, P, (const uint32_t *)B.kout,
+                       (const uint32_t *)B.vout, M, B.fb, B.gv[0], B.gv[1], B.gv[2]);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace ch4g

@@ -52,9 +52,7 @@ enum ch4g_flag {
     CH4G_TEMPERATURE_INITIAL_LINEAR    = 1u << 12, /* initial_conditions.c:303-308 */
     CH4G_TEMPERATURE_INITIAL_ADD_NOISE = 1u << 13, /* initial_conditions.c:421-424 */
     CH4G_FLUID_INITIAL_UNIT_DENSITY    = 1u << 14, /* initial_conditions.c:256-279 */
-    CH4G_INITIAL_BAROMETRIC            = 1u << 15, /* initial_conditions.c:72-84 */
-    CH4G_LAGRANGE                      = 1u << 16, /* point particles, lagrange.c */
-    CH4G_LAGRANGE_TWOWAY               = 1u << 17  /* particle feedback, lagrange_feedback.c */
+    CH4G_INITIAL_BAROMETRIC            = 1u << 15  /* initial_conditions.c:72-84 */
 };
 
 /* Storage / arithmetic of the populations.  Reductions (Nusselt, energy,
@@ -173,6 +171,85 @@ int64_t ch4g_itime(const ch4g_lattice *lat);
 
 /* Bytes of device memory held by the lattice. */
 int64_t ch4g_device_bytes(const ch4g_lattice *lat);
+
+/* ------------------------------------------------------------------------ */
+/* Lagrangian point particles (src/lagrange.c, src/lagrange_feedback.c).
+ *
+ * Particles live on the lattice's device next to the populations and advance
+ * inside ch4g_step after the fluid step, in main.c's order (main.c:163-183):
+ * boundary_conditions_hydro, then time_dt_lagr_factor sub-steps of
+ * interpolate_vector_at_particles(u) [+ interpolate_scalar_at_particles(t)]
+ * and move_particles (with sendrecv_particles' wrap).  With two-way coupling
+ * add_particle_feedbacks enters the force of every step from the second on
+ * (main.c:125-130).  Positions and fields are FP64 in every precision mode.
+ * Interpolated fields use the two-node halo of the reference's particle build
+ * (define.h:333-334 LAGRANGE_GRADIENT_BRD2, typedef.h:315-316).            */
+
+/* Particle switches (define.h:316-382), carried at run time. */
+enum ch4g_lagrange_flag {
+    CH4G_LAGRANGE_ADD_TRACER                          = 1u << 0,  /* type 0 is a fluid tracer, parameters.c:743-755 */
+    CH4G_LAGRANGE_WRAP                                = 1u << 1,  /* positions wrapped into the box, lagrange.c:4075-4079 */
+    CH4G_LAGRANGE_GRADIENT                            = 1u << 2,  /* grad u at particles, lagrange.c:1331-1436 */
+    CH4G_LAGRANGE_ADDEDMASS                           = 1u << 3,  /* beta Du/Dt, lagrange.c:3325-3380 (needs GRADIENT) */
+    CH4G_LAGRANGE_GRAVITY                             = 1u << 4,  /* -g (+beta g), lagrange.c:3312-3340 */
+    CH4G_LAGRANGE_BC_INELASTIC                        = 1u << 5,  /* wall rule, lagrange.c:3760-3771 */
+    CH4G_LAGRANGE_BC_INELASTIC_REINJECT               = 1u << 6,  /* lagrange.c:3806-3845, :3887-3925 */
+    CH4G_LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM      = 1u << 7,
+    CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO = 1u << 8,
+    CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF = 1u << 9,
+    CH4G_LAGRANGE_SMALLTAUD_BETA                      = 1u << 10, /* lagrange.c:3426-3445 */
+    CH4G_LAGRANGE_SMALLTAUD_FLOATER                   = 1u << 11, /* lagrange.c:3446-3465 */
+    CH4G_LAGRANGE_TWOWAY_MOMENTUM                     = 1u << 12, /* lagrange_feedback.c:11-256 */
+    CH4G_LAGRANGE_TWOWAY_SMALLTAUD                    = 1u << 13, /* lagrange_feedback.c:106-110 */
+    CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG               = 1u << 14, /* lagrange.c:271-297 */
+    CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG             = 1u << 15  /* lagrange.c:313-336 */
+};
+
+/* Particle parameters of param.in (parameters.c:155-167, :524-584). */
+typedef struct ch4g_particle_config {
+    uint32_t flags;            /* OR of enum ch4g_lagrange_flag */
+    int32_t  particle_number;
+    double   tau_drag_types, tau_drag_min, tau_drag_max;
+    double   beta_coeff_types, beta_coeff_min, beta_coeff_max; /* read with LAGRANGE_ADDEDMASS */
+    double   time_dt_lagr_factor;                              /* sub-steps per lattice step */
+} ch4g_particle_config;
+
+/* One particle: the members of the reference's point_particle
+ * (typedef.h:391-481) that the supported switches read or write.           */
+typedef struct ch4g_particle {
+    double name;
+    double x, y, z;
+    double vx, vy, vz, vx_old, vy_old, vz_old;
+    double ax, ay, az, ax_old, ay_old, az_old;
+    double ux, uy, uz, ux_old, uy_old, uz_old;
+    double dx_ux, dy_ux, dz_ux, dx_uy, dy_uy, dz_uy, dx_uz, dy_uz, dz_uz;
+    double Dt_ux, Dt_uy, Dt_uz;
+    double t, t_old, dt_t;
+    double tau_drag, beta_coeff;
+} ch4g_particle;
+
+/* allocate_particles (lagrange.c:6-49) + initial_conditions_particles
+ * without restart (lagrange.c:72-1021): type ladders, name % types, positions
+ * drawn x, y, z per particle from the lattice's drand48 stream (which has
+ * already served initial_conditions and the first build_forcing, as in
+ * main.c:23-51), zero velocity.  Call after ch4g_initial_conditions (or
+ * ch4g_set_populations) and before the first step.                          */
+int ch4g_particles_init(ch4g_lattice *lat, const ch4g_particle_config *pcfg);
+
+/* Number of particles alive (sendrecv_particles drops a particle whose
+ * wrapped position rounds onto the upper box face, lagrange.c:4090-4116).
+ * Synchronous.                                                              */
+int ch4g_particles_count(ch4g_lattice *lat, int64_t *n);
+
+/* Read back the particles in the reference's array order (output_particles,
+ * lagrange.c:1598; write_point_particle_h5 payload).  At most `cap` records;
+ * *n receives the live count.  Synchronous.                                 */
+int ch4g_get_particles(ch4g_lattice *lat, ch4g_particle *out, int64_t cap, int64_t *n);
+
+/* Replace the particle state (read_point_particle_h5 restart,
+ * lagrange.c:128-131).  Requires ch4g_particles_init first; the count may
+ * differ from particle_number.                                              */
+int ch4g_set_particles(ch4g_lattice *lat, const ch4g_particle *in, int64_t n);
 
 /* Device bytes a lattice with this configuration would hold (admission). */
 int64_t ch4g_footprint_bytes(const ch4g_config *cfg);

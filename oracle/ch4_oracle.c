@@ -23,40 +23,7 @@
 #include <string.h>
 #include <stdio.h>
 
-#include "../include/ch4g.h"
-
-#define NP 19
-#define BRD 1
-
-typedef struct { double p[NP]; } opop;
-typedef struct { double x, y, z; } ovec;
-typedef struct { double xx, xy, xz, yx, yy, yz, zx, zy, zz; } oten;
-
-typedef struct oracle {
-    ch4g_config cfg;
-    int NX, NY, NZ;         /* interior sizes */
-    int GX, GY, GZ;         /* with ghosts */
-    size_t ng;              /* ghost-extended node count */
-    opop *p, *rhs_p, *g, *rhs_g;
-    ovec *u, *force;
-    double *dens, *t, *tau_les;
-    /* lattice, lb.c:139-230 and :368-377 */
-    double wgt[NP], cx[NP], cy[NP], cz[NP];
-    int inv[NP];
-    double cs, cs2, cs4, invcs, invcs2, invcs4, twocs2, twocs4, invtwocs2, invtwocs4;
-    double nu, kappa, deltaT;
-    /* HIT forcing, initial_conditions.c:822-875 */
-    int nk;
-    ovec vk[CH4G_HIT_NK];
-    double vk2[CH4G_HIT_NK];
-    ovec phi[CH4G_HIT_NK];
-    /* drand48 state (48-bit) */
-    uint64_t x48;
-    long itime;
-} oracle;
-
-#define IDX(o, i, j, k) ((((size_t)(k)) * (size_t)(o)->GY + (size_t)(j)) * (size_t)(o)->GX + (size_t)(i))
-#define HAS(o, f) (((o)->cfg.flags & (f)) != 0)
+#include "ch4_oracle_internal.h"
 
 /* ------------------------------------------------------------------------ */
 /* drand48: POSIX 48-bit LCG, X <- (a X + c) mod 2^48, a = 0x5DEECE66D,
@@ -204,10 +171,13 @@ static void halo_scal(oracle *o, double *f)
 /* ------------------------------------------------------------------------ */
 /* hydro_fields (lb.c:420-540). 'p': rho = m(p), u = mv/rho + dt/2 F under
  * METHOD_FORCING_GUO with LB_FLUID_FORCING (:464-466).  'g': t = m(g) (:486). */
+/* LB_FLUID_FORCING: any force term, or the particle feedback (which only
+ * enters through build_forcing's force field, main.c:118-131)              */
 static int has_fluid_forcing(const oracle *o)
 {
     return HAS(o, CH4G_FLUID_FORCING_POISEUILLE | CH4G_FLUID_FORCING_CHANNEL | CH4G_FLUID_FORCING_HIT |
-                      CH4G_TEMPERATURE_BUOYANCY);
+                      CH4G_TEMPERATURE_BUOYANCY) ||
+           olagr_twoway(o);
 }
 
 static void hydro_fields(oracle *o, char which)
@@ -493,6 +463,8 @@ static void one_step(oracle *o)
     if (HAS(o, CH4G_LB_TEMPERATURE)) advection_collision(o, o->g, o->rhs_g, o->cfg.tau_t, 'g');
     if (has_fluid_forcing(o)) {
         build_forcing(o);
+        /* main.c:125-130 without restart: from the second step on */
+        if (olagr_twoway(o) && o->itime > 1) olagr_feedback(o);
         add_forcing(o);
     }
     boundary_conditions_streaming(o);
@@ -500,6 +472,8 @@ static void one_step(oracle *o)
     if (HAS(o, CH4G_LB_TEMPERATURE)) time_stepping(o, o->g, o->rhs_g);
     if (HAS(o, CH4G_LB_FLUID)) hydro_fields(o, 'p');
     if (HAS(o, CH4G_LB_TEMPERATURE)) hydro_fields(o, 'g');
+    /* dump_averages (main.c:162) reads only the fields; then main.c:163-183 */
+    if (o->lagr) olagr_after_step(o);
 }
 
 /* First reconstruction of main.c:26-48. */
@@ -572,6 +546,7 @@ void oracle_destroy(oracle *o)
     free(o->dens);
     free(o->t);
     free(o->tau_les);
+    olagr_free(o);
     free(o);
 }
 

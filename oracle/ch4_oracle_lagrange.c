@@ -1,0 +1,717 @@
+/*
+ * ch4_oracle_lagrange.c — FP64 CPU restatement of the reference's Lagrangian
+ * point particles.  TEST INFRASTRUCTURE ONLY (the checker of the HIP particle
+ * kernels); nothing in the product links, loads or calls it.
+ *
+ * Restated with the reference's structure: particle records in one array,
+ * fields with a ghost halo filled by boundary_conditions_hydro, cell lookup
+ * by a linear scan of the cell centres, trilinear sums and finite
+ * differences in the reference's evaluation order, sequential feedback
+ * accumulation in particle order.  The halo is two nodes wide
+ * (LAGRANGE_GRADIENT_BRD2, define.h:334, typedef.h:315-316): with one node
+ * and no fluid walls the reference reads unset gradient components at the
+ * ghost corners (gradients.c:83-95 has no branch for them), which has no
+ * defined value to follow.
+ *
+ * Sources followed:
+ *   initial_conditions_particles  lagrange.c:72-1021 (no restart branch)
+ *   boundary_conditions_hydro     boundary_conditions.c:2786-2997
+ *   interpolate_*_at_particles    lagrange.c:1211-1593
+ *   gradient_vector (LAGRANGE)    gradients.c:78-499
+ *   move_particles                lagrange.c:2934-4025
+ *   sendrecv_particles (1 rank)   lagrange.c:4028-4221
+ *   add_particle_feedbacks        lagrange_feedback.c:11-256, 525-836
+ * Pins: tests/test_oracle_particles.py.  Parity unpinned: none of these
+ * functions calls libm except pow(r^2, 3/2) of the particle volume, which is
+ * computed with the same glibc call by the device library's host code.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ch4_oracle_internal.h"
+
+#define B2 2 /* halo width of the particle-side fields */
+
+typedef struct olagr {
+    ch4g_particle_config pc;
+    int npart;
+    ch4g_particle *tr;
+    int GX, GY, GZ;
+    size_t ng;
+    ovec *u, *ftw;     /* velocity with halo, force_twoway */
+    double *t;         /* temperature with halo */
+    double *cx, *cy, *cz; /* cell centres incl. ghosts (grid.c:183-185,498) */
+    double dt;         /* time_dt_lagr = time_dt / time_dt_lagr_factor (parameters.c:164) */
+} olagr;
+
+#define LHAS(L, f) (((L)->pc.flags & (f)) != 0)
+#define IDX2(L, i, j, k) ((((size_t)(k)) * (size_t)(L)->GY + (size_t)(j)) * (size_t)(L)->GX + (size_t)(i))
+#define WRAP(x, s) ((x) - floor((x) / (s)) * (s)) /* typedef.h:378 */
+
+int olagr_twoway(const oracle *o) { return o->lagr && LHAS(o->lagr, CH4G_LAGRANGE_TWOWAY_MOMENTUM); }
+
+void olagr_free(oracle *o)
+{
+    olagr *L = o->lagr;
+    if (!L) return;
+    free(L->tr);
+    free(L->u);
+    free(L->ftw);
+    free(L->t);
+    free(L->cx);
+    free(L->cy);
+    free(L->cz);
+    free(L);
+    o->lagr = NULL;
+}
+
+/* One property ladder of lagrange.c:264-298 (and its copies for beta):
+ * values vary with period `rep` over the particle types that follow the
+ * fluid tracers.                                                            */
+static void ladder(double *tab, int ntypes, int ntr, double trval, double types, double vmin, double vmax, int logs,
+                   double repetitions)
+{
+    double cycles = (ntypes - ntr) / types;
+    double step;
+    if (logs)
+        step = types > 1 ? pow(vmax / vmin, 1.0 / (types - 1.0)) : 0;
+    else
+        step = types > 1 ? (vmax - vmin) / (types - 1.0) : 0;
+    for (int n = 0; n < ntr; n++) tab[n] = trval;
+    for (int k = 0; k < (int)(cycles / repetitions); k++)
+        for (int i = 0; i < (int)types; i++)
+            for (int j = 0; j < (int)repetitions; j++) {
+                int at = ntr + j + i * (int)repetitions + k * (int)repetitions * (int)types;
+                tab[at] = logs ? vmin * pow(step, (double)i) : vmin + i * step;
+            }
+}
+
+int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
+{
+    olagr_free(o);
+    olagr *L = (olagr *)calloc(1, sizeof(olagr));
+    if (!L) return -1;
+    o->lagr = L;
+    L->pc = *pc;
+    L->GX = o->NX + 2 * B2;
+    L->GY = o->NY + 2 * B2;
+    L->GZ = o->NZ + 2 * B2;
+    L->ng = (size_t)L->GX * L->GY * L->GZ;
+    L->u = (ovec *)calloc(L->ng, sizeof(ovec));
+    L->ftw = (ovec *)calloc(L->ng, sizeof(ovec));
+    L->t = (double *)calloc(L->ng, sizeof(double));
+    L->cx = (double *)malloc(sizeof(double) * L->GX);
+    L->cy = (double *)malloc(sizeof(double) * L->GY);
+    L->cz = (double *)malloc(sizeof(double) * L->GZ);
+    if (!L->u || !L->ftw || !L->t || !L->cx || !L->cy || !L->cz) return -1;
+    for (int i = 0; i < L->GX; i++) L->cx[i] = (double)(i - B2) + 0.5;
+    for (int j = 0; j < L->GY; j++) L->cy[j] = (double)(j - B2) + 0.5;
+    for (int k = 0; k < L->GZ; k++) L->cz[k] = (double)(k - B2) + 0.5;
+    L->dt = 1.0 / pc->time_dt_lagr_factor;
+
+    /* particle_types (parameters.c:559-584, :743-755) */
+    const int am = LHAS(L, CH4G_LAGRANGE_GRADIENT) && LHAS(L, CH4G_LAGRANGE_ADDEDMASS);
+    double types = pc->tau_drag_types;
+    if (am) types *= pc->beta_coeff_types;
+    int ntr = LHAS(L, CH4G_LAGRANGE_ADD_TRACER) ? 1 : 0;
+    types += ntr;
+    const int nt = (int)types;
+    double *tau = (double *)calloc(nt, sizeof(double)), *beta = (double *)calloc(nt, sizeof(double));
+    if (!tau || !beta) return -1;
+    double rep = 1.0;
+    ladder(tau, nt, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
+           LHAS(L, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
+    rep *= pc->tau_drag_types;
+    if (am) {
+        ladder(beta, nt, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
+               LHAS(L, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
+        rep *= pc->beta_coeff_types;
+    }
+
+    L->npart = pc->particle_number;
+    L->tr = (ch4g_particle *)calloc(L->npart > 0 ? L->npart : 1, sizeof(ch4g_particle));
+    if (!L->tr) return -1;
+    for (int i = 0; i < L->npart; i++) L->tr[i].name = i; /* lagrange.c:634-641, one rank */
+    const double LNX = o->NX, LNY = o->NY, LNZ = o->NZ;
+    for (int i = 0; i < L->npart; i++) {
+        ch4g_particle *p = &L->tr[i];
+        int type = ((int)p->name) % nt;
+        p->tau_drag = tau[type];
+        if (am) p->beta_coeff = beta[type];
+        /* lagrange.c:767-769: three draws per particle, particle order */
+        p->x = 0 + oracle_drand48(o) * LNX;
+        p->y = 0 + oracle_drand48(o) * LNY;
+        p->z = 0 + oracle_drand48(o) * LNZ;
+        p->vx = p->vy = p->vz = 0.0;
+        p->t = p->t_old = p->dt_t = 0.0;
+    }
+    free(tau);
+    free(beta);
+    return 0;
+}
+
+/* boundary_conditions_hydro (boundary_conditions.c:2786-2997) on one rank:
+ * sendrecv_borders_vector/_scalar fill every ghost node with its periodic
+ * image, then the no-slip rule u_g = -u and the fixed-temperature rule
+ * t_g = (2 (T_wall - T_ref) / t - 1) t set the first ghost plane of each
+ * y wall, over all i, k including ghosts.                                  */
+static void bc_hydro(oracle *o)
+{
+    olagr *L = o->lagr;
+    const int NX = o->NX, NY = o->NY, NZ = o->NZ;
+    for (int k = 0; k < L->GZ; k++)
+        for (int j = 0; j < L->GY; j++)
+            for (int i = 0; i < L->GX; i++) {
+                int si = ((i - B2) % NX + NX) % NX, sj = ((j - B2) % NY + NY) % NY, sk = ((k - B2) % NZ + NZ) % NZ;
+                size_t m = IDX(o, si + BRD, sj + BRD, sk + BRD), n = IDX2(L, i, j, k);
+                if (HAS(o, CH4G_LB_FLUID)) L->u[n] = o->u[m];
+                if (HAS(o, CH4G_LB_TEMPERATURE)) L->t[n] = o->t[m];
+            }
+    if (HAS(o, CH4G_LB_FLUID) && HAS(o, CH4G_FLUID_BC_Y))
+        for (int i = 0; i < L->GX; i++)
+            for (int k = 0; k < L->GZ; k++) {
+                const double cm = -1.0, cp = -1.0;
+                int j = B2;
+                ovec a = L->u[IDX2(L, i, j, k)];
+                L->u[IDX2(L, i, j - 1, k)] = (ovec){cm * a.x, cm * a.y, cm * a.z};
+                j = NY + B2 - 1;
+                a = L->u[IDX2(L, i, j, k)];
+                L->u[IDX2(L, i, j + 1, k)] = (ovec){cp * a.x, cp * a.y, cp * a.z};
+            }
+    if (HAS(o, CH4G_LB_TEMPERATURE) && HAS(o, CH4G_TEMPERATURE_BC_Y))
+        for (int i = 0; i < L->GX; i++)
+            for (int k = 0; k < L->GZ; k++) {
+                int j = B2;
+                double T_wall = o->cfg.T_bot;
+                double fac = 2.0 * (T_wall - o->cfg.T_ref) / L->t[IDX2(L, i, j, k)] - 1.0;
+                L->t[IDX2(L, i, j - 1, k)] = fac * L->t[IDX2(L, i, j, k)];
+                j = NY + B2 - 1;
+                T_wall = o->cfg.T_top;
+                fac = 2.0 * (T_wall - o->cfg.T_ref) / L->t[IDX2(L, i, j, k)] - 1.0;
+                L->t[IDX2(L, i, j + 1, k)] = fac * L->t[IDX2(L, i, j, k)];
+            }
+    (void)NZ;
+}
+
+/* linear scan of lagrange.c:1253-1264 (last match wins) */
+static int locate(const double *c, int n, double x)
+{
+    int im = -1;
+    for (int i = 0; i < n - 1; i++)
+        if (c[i] <= x && x < c[i + 1]) im = i;
+    return im;
+}
+
+typedef struct {
+    int im, jm, km;
+    double w[8];
+    size_t n[8];
+} ostencil;
+
+/* cell and volume coefficients of lagrange.c:1232-1292, corners in the
+ * order of the reference's sums                                            */
+static int stencil(const oracle *o, const ch4g_particle *p, ostencil *s)
+{
+    const olagr *L = o->lagr;
+    double px = WRAP(p->x, (double)o->NX), py = WRAP(p->y, (double)o->NY), pz = WRAP(p->z, (double)o->NZ);
+    int im = locate(L->cx, L->GX, px), jm = locate(L->cy, L->GY, py), km = locate(L->cz, L->GZ, pz);
+    if (im < 0 || jm < 0 || km < 0) return -1;
+    int ip = im + 1, jp = jm + 1, kp = km + 1;
+    double dxm = px - L->cx[im], dxp = L->cx[ip] - px;
+    double dym = py - L->cy[jm], dyp = L->cy[jp] - py;
+    double dzm = pz - L->cz[km], dzp = L->cz[kp] - pz;
+    s->im = im;
+    s->jm = jm;
+    s->km = km;
+    s->w[0] = dxp * dyp * dzp; /* (im,jm,km) */
+    s->w[1] = dxm * dyp * dzp; /* (ip,jm,km) */
+    s->w[2] = dxp * dym * dzp; /* (im,jp,km) */
+    s->w[3] = dxp * dyp * dzm; /* (im,jm,kp) */
+    s->w[4] = dxm * dym * dzp; /* (ip,jp,km) */
+    s->w[5] = dxp * dym * dzm; /* (im,jp,kp) */
+    s->w[6] = dxm * dyp * dzm; /* (ip,jm,kp) */
+    s->w[7] = dxm * dym * dzm; /* (ip,jp,kp) */
+    s->n[0] = IDX2(L, im, jm, km);
+    s->n[1] = IDX2(L, ip, jm, km);
+    s->n[2] = IDX2(L, im, jp, km);
+    s->n[3] = IDX2(L, im, jm, kp);
+    s->n[4] = IDX2(L, ip, jp, km);
+    s->n[5] = IDX2(L, im, jp, kp);
+    s->n[6] = IDX2(L, ip, jm, kp);
+    s->n[7] = IDX2(L, ip, jp, kp);
+    return 0;
+}
+
+static const int CORNER[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {1, 1, 1}};
+
+/* gradient_vector with LAGRANGE (gradients.c:78-499): centred differences
+ * where both neighbours exist; with fluid walls (LB_FLUID_BC) one-sided
+ * second order at the first and last fluid rows of y (:201-225) and the
+ * LAGRANGE_INTERPOLATION_FIX at the ghost cells next to the interior in
+ * every direction (:372-486), which moves the evaluation point one cell
+ * inwards and keeps the moved index for the directions that follow; then
+ * components along a size-one axis are zeroed (:494-496).                   */
+static oten grad_vec(const oracle *o, int i, int j, int k)
+{
+    const olagr *L = o->lagr;
+    const ovec *t = L->u;
+    const int NX = o->NX, NY = o->NY, NZ = o->NZ;
+    oten g;
+    memset(&g, 0, sizeof g);
+    double h1, h2, a0, a1, a2;
+    if (i > 0 && i < L->GX - 1) {
+        size_t p = IDX2(L, i + 1, j, k), m = IDX2(L, i - 1, j, k);
+        double d = L->cx[i + 1] - L->cx[i - 1];
+        g.xx = (t[p].x - t[m].x) / d;
+        g.yx = (t[p].y - t[m].y) / d;
+        g.zx = (t[p].z - t[m].z) / d;
+    }
+    if (j > 0 && j < L->GY - 1) {
+        size_t p = IDX2(L, i, j + 1, k), m = IDX2(L, i, j - 1, k);
+        double d = L->cy[j + 1] - L->cy[j - 1];
+        g.xy = (t[p].x - t[m].x) / d;
+        g.yy = (t[p].y - t[m].y) / d;
+        g.zy = (t[p].z - t[m].z) / d;
+    }
+    if (k > 0 && k < L->GZ - 1) {
+        size_t p = IDX2(L, i, j, k + 1), m = IDX2(L, i, j, k - 1);
+        double d = L->cz[k + 1] - L->cz[k - 1];
+        g.xz = (t[p].x - t[m].x) / d;
+        g.yz = (t[p].y - t[m].y) / d;
+        g.zz = (t[p].z - t[m].z) / d;
+    }
+    if (HAS(o, CH4G_FLUID_BC_Y)) {
+#define FWD(c, a)                                                              \
+    h1 = c[a + 1] - c[a];                                                      \
+    h2 = c[a + 2] - c[a + 1];                                                  \
+    a0 = -(2.0 * h1 + h2) / (h1 * (h1 + h2));                                  \
+    a1 = (h1 + h2) / (h1 * h2);                                                \
+    a2 = -(h1) / (h2 * (h1 + h2));
+#define BWD(c, a)                                                              \
+    h1 = c[a] - c[a - 1];                                                      \
+    h2 = c[a - 1] - c[a - 2];                                                  \
+    a0 = (2.0 * h1 + h2) / (h1 * (h1 + h2));                                   \
+    a1 = -(h1 + h2) / (h1 * h2);                                               \
+    a2 = (h1) / (h2 * (h1 + h2));
+#define COMB(n0, n1, n2, X, Y, Z)                                              \
+    X = a0 * t[n0].x + a1 * t[n1].x + a2 * t[n2].x;                            \
+    Y = a0 * t[n0].y + a1 * t[n1].y + a2 * t[n2].y;                            \
+    Z = a0 * t[n0].z + a1 * t[n1].z + a2 * t[n2].z;
+        /* LB_FLUID_BC_Y, SECOND_ORDER_GRAD */
+        if (j == B2) {
+            FWD(L->cy, j)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j + 1, k), IDX2(L, i, j + 2, k), g.xy, g.yy, g.zy)
+        }
+        if (j == NY + B2 - 1) {
+            BWD(L->cy, j)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j - 1, k), IDX2(L, i, j - 2, k), g.xy, g.yy, g.zy)
+        }
+        /* LAGRANGE_INTERPOLATION_FIX, SECOND_ORDER_GRAD */
+        if (i == B2 - 1) {
+            i = i + 1;
+            FWD(L->cx, i)
+            COMB(IDX2(L, i, j, k), IDX2(L, i + 1, j, k), IDX2(L, i + 2, j, k), g.xx, g.yx, g.zx)
+        }
+        if (i == NX + B2) {
+            i = i - 1;
+            BWD(L->cx, i)
+            COMB(IDX2(L, i, j, k), IDX2(L, i - 1, j, k), IDX2(L, i - 2, j, k), g.xx, g.yx, g.zx)
+        }
+        if (j == B2 - 1) {
+            j = j + 1;
+            FWD(L->cy, j)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j + 1, k), IDX2(L, i, j + 2, k), g.xy, g.yy, g.zy)
+        }
+        if (j == NY + B2) {
+            j = j - 1;
+            BWD(L->cy, j)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j - 1, k), IDX2(L, i, j - 2, k), g.xy, g.yy, g.zy)
+        }
+        if (k == B2 - 1) {
+            k = k + 1;
+            FWD(L->cz, k)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j, k + 1), IDX2(L, i, j, k + 2), g.xz, g.yz, g.zz)
+        }
+        if (k == NZ + B2) {
+            k = k - 1;
+            BWD(L->cz, k)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j, k - 1), IDX2(L, i, j, k - 2), g.xz, g.yz, g.zz)
+        }
+#undef FWD
+#undef BWD
+#undef COMB
+    }
+    if (NX == 1) g.xx = g.yx = g.zx = 0.0;
+    if (NY == 1) g.xy = g.yy = g.zy = 0.0;
+    if (NZ == 1) g.xz = g.yz = g.zz = 0.0;
+    return g;
+}
+
+/* interpolate_vector_at_particles(u,'u') and interpolate_scalar_at_particles
+ * (t,'t') (lagrange.c:1211-1593); the scalar's gradient feeds only
+ * particle-temperature switches that are not restated.                     */
+static void interpolate(oracle *o)
+{
+    olagr *L = o->lagr;
+    const int grad = LHAS(L, CH4G_LAGRANGE_GRADIENT);
+    for (int ip = 0; ip < L->npart; ip++) {
+        ch4g_particle *p = &L->tr[ip];
+        ostencil s;
+        if (stencil(o, p, &s)) continue;
+        if (HAS(o, CH4G_LB_FLUID)) {
+            double vx = L->u[s.n[0]].x * s.w[0], vy = L->u[s.n[0]].y * s.w[0], vz = L->u[s.n[0]].z * s.w[0];
+            for (int c = 1; c < 8; c++) {
+                vx += L->u[s.n[c]].x * s.w[c];
+                vy += L->u[s.n[c]].y * s.w[c];
+                vz += L->u[s.n[c]].z * s.w[c];
+            }
+            p->ux = vx;
+            p->uy = vy;
+            p->uz = vz;
+            if (grad) {
+                oten G[8];
+                for (int c = 0; c < 8; c++) G[c] = grad_vec(o, s.im + CORNER[c][0], s.jm + CORNER[c][1], s.km + CORNER[c][2]);
+                double a[9];
+                for (int m = 0; m < 9; m++) {
+                    double acc = ((const double *)&G[0])[m] * s.w[0];
+                    for (int c = 1; c < 8; c++) acc += ((const double *)&G[c])[m] * s.w[c];
+                    a[m] = acc;
+                }
+                /* oten order xx,xy,xz,yx,yy,yz,zx,zy,zz; dy_ux = grad.xy (lagrange.c:1420-1428) */
+                p->dx_ux = a[0];
+                p->dy_ux = a[1];
+                p->dz_ux = a[2];
+                p->dx_uy = a[3];
+                p->dy_uy = a[4];
+                p->dz_uy = a[5];
+                p->dx_uz = a[6];
+                p->dy_uz = a[7];
+                p->dz_uz = a[8];
+            }
+        }
+        if (HAS(o, CH4G_LB_TEMPERATURE)) {
+            double v = L->t[s.n[0]] * s.w[0];
+            for (int c = 1; c < 8; c++) v += L->t[s.n[c]] * s.w[c];
+            p->t = v;
+        }
+    }
+}
+
+/* move_particles (lagrange.c:2934-4025) for the switches of ch4g.h, then
+ * sendrecv_particles (lagrange.c:4028-4221) on one rank.                   */
+static void move(oracle *o)
+{
+    olagr *L = o->lagr;
+    const double dt = L->dt;
+    const int first = (o->itime == 1); /* itime == 1 && resume == 0 */
+    const double gx = o->cfg.gravity_x, gy = o->cfg.gravity_y, gz = o->cfg.gravity_z;
+    const double SX = o->NX, SY = o->NY, SZ = o->NZ;
+    for (int ip = 0; ip < L->npart; ip++) {
+        ch4g_particle *p = &L->tr[ip];
+        if (HAS(o, CH4G_LB_TEMPERATURE)) {
+            if (first) {
+                p->t_old = p->t;
+            } else {
+                p->dt_t = (p->t - p->t_old) / dt;
+                p->t_old = p->t;
+            }
+        }
+        if (p->tau_drag == 0.0) { /* fluid tracer (lagrange.c:3010-3227) */
+            p->vx = p->ux;
+            p->vy = p->uy;
+            p->vz = p->uz;
+            if (first) {
+                p->vx_old = p->vx;
+                p->vy_old = p->vy;
+                p->vz_old = p->vz;
+            }
+            p->ax = (p->vx - p->vx_old) / dt;
+            p->ay = (p->vy - p->vy_old) / dt;
+            p->az = (p->vz - p->vz_old) / dt;
+            if (first) {
+                p->x += dt * p->vx;
+                p->y += dt * p->vy;
+                p->z += dt * p->vz;
+            } else {
+                p->x += dt * 0.5 * (3.0 * p->vx - p->vx_old);
+                p->y += dt * 0.5 * (3.0 * p->vy - p->vy_old);
+                p->z += dt * 0.5 * (3.0 * p->vz - p->vz_old);
+            }
+            p->vx_old = p->vx;
+            p->vy_old = p->vy;
+            p->vz_old = p->vz;
+            p->ux_old = p->ux;
+            p->uy_old = p->uy;
+            p->uz_old = p->uz;
+        }
+        if (p->tau_drag > 0.0) { /* inertial particle (lagrange.c:3229-3466) */
+            double dux = 0.0, duy = 0.0, duz = 0.0; /* Dt_u */
+            double invtau = 1.0 / p->tau_drag;
+            p->ax = (p->ux - p->vx) * invtau;
+            p->ay = (p->uy - p->vy) * invtau;
+            p->az = (p->uz - p->vz) * invtau;
+            if (LHAS(L, CH4G_LAGRANGE_GRAVITY)) {
+                p->ax -= gx;
+                p->ay -= gy;
+                p->az -= gz;
+            }
+            if (LHAS(L, CH4G_LAGRANGE_ADDEDMASS) && p->beta_coeff != 0.0) {
+                if (LHAS(L, CH4G_LAGRANGE_GRAVITY)) {
+                    p->ax -= (-p->beta_coeff) * gx;
+                    p->ay -= (-p->beta_coeff) * gy;
+                    p->az -= (-p->beta_coeff) * gz;
+                }
+                if (first) {
+                    p->ux_old = p->ux;
+                    p->uy_old = p->uy;
+                    p->uz_old = p->uz;
+                }
+                dux = (p->ux - p->ux_old) / dt + (p->ux - p->vx) * p->dx_ux + (p->uy - p->vy) * p->dy_ux +
+                      (p->uz - p->vz) * p->dz_ux;
+                duy = (p->uy - p->uy_old) / dt + (p->ux - p->vx) * p->dx_uy + (p->uy - p->vy) * p->dy_uy +
+                      (p->uz - p->vz) * p->dz_uy;
+                duz = (p->uz - p->uz_old) / dt + (p->ux - p->vx) * p->dx_uz + (p->uy - p->vy) * p->dy_uz +
+                      (p->uz - p->vz) * p->dz_uz;
+                p->ax += dux * p->beta_coeff;
+                p->ay += duy * p->beta_coeff;
+                p->az += duz * p->beta_coeff;
+                p->Dt_ux = dux;
+                p->Dt_uy = duy;
+                p->Dt_uz = duz;
+            }
+            if (first) {
+                p->x += dt * p->vx;
+                p->y += dt * p->vy;
+                p->z += dt * p->vz;
+            } else {
+                p->x += dt * 0.5 * (3.0 * p->vx - p->vx_old);
+                p->y += dt * 0.5 * (3.0 * p->vy - p->vy_old);
+                p->z += dt * 0.5 * (3.0 * p->vz - p->vz_old);
+            }
+            p->vx_old = p->vx;
+            p->vy_old = p->vy;
+            p->vz_old = p->vz;
+            p->ux_old = p->ux;
+            p->uy_old = p->uy;
+            p->uz_old = p->uz;
+            if (first) {
+                p->vx += dt * p->ax;
+                p->vy += dt * p->ay;
+                p->vz += dt * p->az;
+            } else {
+                p->vx += dt * 0.5 * (3.0 * p->ax - p->ax_old);
+                p->vy += dt * 0.5 * (3.0 * p->ay - p->ay_old);
+                p->vz += dt * 0.5 * (3.0 * p->az - p->az_old);
+            }
+            p->ax_old = p->ax;
+            p->ay_old = p->ay;
+            p->az_old = p->az;
+            if (LHAS(L, CH4G_LAGRANGE_SMALLTAUD_BETA) && p->tau_drag < 10.0 && p->tau_drag > 0.0) {
+                p->vx = p->ux;
+                p->vy = p->uy;
+                p->vz = p->uz;
+                p->vx -= gx * (1.0 - p->beta_coeff) * p->tau_drag;
+                p->vy -= gy * (1.0 - p->beta_coeff) * p->tau_drag;
+                p->vz -= gz * (1.0 - p->beta_coeff) * p->tau_drag;
+                p->vx -= dux * (1.0 - p->beta_coeff) * p->tau_drag;
+                p->vy -= duy * (1.0 - p->beta_coeff) * p->tau_drag;
+                p->vz -= duz * (1.0 - p->beta_coeff) * p->tau_drag;
+            }
+            if (LHAS(L, CH4G_LAGRANGE_SMALLTAUD_FLOATER) && p->tau_drag > 0.0) {
+                p->vx = p->ux;
+                p->vy = p->uy;
+                p->vz = p->uz;
+                double f1 = p->tau_drag * (1.0 - p->beta_coeff + 2. / 3. * o->cfg.beta_t * (p->t - 0.4));
+                p->vx += f1 * (-gx - p->Dt_ux);
+                p->vy += f1 * (-gy - p->Dt_uy);
+                p->vz += f1 * (-gz - p->Dt_uz);
+            }
+        }
+        /* wall rules (lagrange.c:3754-3993), LB_FLUID_BC_Y only */
+        if (HAS(o, CH4G_FLUID_BC_Y)) {
+            const int inel = LHAS(L, CH4G_LAGRANGE_BC_INELASTIC);
+            const int reinj = LHAS(L, CH4G_LAGRANGE_BC_INELASTIC_REINJECT);
+            const double fac1 = inel ? 0.0 : -1.0, fac2 = inel ? 0.0 : 1.0;
+            const double radius = 0.0;
+            if (p->y < radius) {
+                if (!reinj) {
+                    p->y = fac1 * (p->y - radius) + radius;
+                    p->vx *= fac2;
+                    if (p->vy < 0.0) p->vy *= fac1;
+                    p->vz *= fac2;
+                }
+                if (inel && reinj) {
+                    if (LHAS(L, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM)) {
+                        p->y += SY;
+                    } else {
+                        p->x = SX * oracle_drand48(o);
+                        p->y += SY;
+                        p->z = SZ * oracle_drand48(o);
+                    }
+                    if (LHAS(L, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO)) {
+                        p->vx = 0.0;
+                        p->vy = 0.0;
+                        p->vz = 0.0;
+                    } else {
+                        p->vx = -gx * (1.0 - p->beta_coeff) * p->tau_drag;
+                        p->vy = -gy * (1.0 - p->beta_coeff) * p->tau_drag;
+                        p->vz = -gz * (1.0 - p->beta_coeff) * p->tau_drag;
+                    }
+                    if (LHAS(L, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF)) {
+                        p->vx /= 2.0;
+                        p->vy /= 2.0;
+                        p->vz /= 2.0;
+                    }
+                }
+            }
+            if (p->y >= SY - radius) {
+                if (!reinj) {
+                    p->y = fac1 * (p->y - (SY - radius)) + (SY - radius);
+                    p->vx *= fac2;
+                    if (p->vy > 0.0) p->vy *= fac1;
+                    p->vz *= fac2;
+                }
+                if (inel && reinj) {
+                    if (LHAS(L, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM)) {
+                        p->y -= (SY - FLT_EPSILON);
+                    } else {
+                        p->x = SX * oracle_drand48(o);
+                        p->y -= (SY - FLT_EPSILON);
+                        p->z = SZ * oracle_drand48(o);
+                    }
+                    if (LHAS(L, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO)) {
+                        p->vx = 0.0;
+                        p->vy = 0.0;
+                        p->vz = 0.0;
+                    } else {
+                        p->vx = -gx * (1.0 - p->beta_coeff) * p->tau_drag;
+                        p->vy = -gy * (1.0 - p->beta_coeff) * p->tau_drag;
+                        p->vz = -gz * (1.0 - p->beta_coeff) * p->tau_drag;
+                    }
+                }
+            }
+        }
+    }
+    /* sendrecv_particles: wrap, keep the particles inside [0, S) (mesh
+     * bounds, lagrange.c:4090-4092); on one rank the others are dropped
+     * (they come back from the Allgatherv and fail the same test)          */
+    int kept = 0;
+    for (int ip = 0; ip < L->npart; ip++) {
+        ch4g_particle *p = &L->tr[ip];
+        double px = WRAP(p->x, SX), py = WRAP(p->y, SY), pz = WRAP(p->z, SZ);
+        if (LHAS(L, CH4G_LAGRANGE_WRAP)) {
+            p->x = WRAP(p->x, SX);
+            p->z = WRAP(p->z, SZ);
+            p->y = WRAP(p->y, SY);
+        }
+        if (px >= 0.0 && px < SX && py >= 0.0 && py < SY && pz >= 0.0 && pz < SZ) L->tr[kept++] = *p;
+    }
+    L->npart = kept;
+}
+
+void olagr_after_step(oracle *o)
+{
+    olagr *L = o->lagr;
+    bc_hydro(o);
+    for (int s = 0; s < (int)L->pc.time_dt_lagr_factor; s++) {
+        interpolate(o);
+        move(o);
+    }
+}
+
+/* add_particle_feedbacks (lagrange_feedback.c:11-256), trilinear
+ * extrapolation, momentum part.  add_sendrecv_borders_vector (:525-836)
+ * only ever adds into ghost nodes (it sends interior values and accumulates
+ * what it receives into the halo), so the interior force receives the
+ * deposits made directly on interior nodes; deposits on ghost nodes do not
+ * reach it.                                                                 */
+void olagr_feedback(oracle *o)
+{
+    olagr *L = o->lagr;
+    memset(L->ftw, 0, L->ng * sizeof(ovec));
+    const double nu = o->nu;
+    ovec gv = {0.0, 0.0, 0.0};
+    if (LHAS(L, CH4G_LAGRANGE_GRAVITY)) {
+        gv.x = o->cfg.gravity_x;
+        gv.y = -o->cfg.gravity_y;
+        gv.z = o->cfg.gravity_z;
+    }
+    for (int ip = 0; ip < L->npart; ip++) {
+        const ch4g_particle *p = &L->tr[ip];
+        ostencil s;
+        if (stencil(o, p, &s)) continue;
+        double density_ratio = (3.0 - p->beta_coeff) / (2.0 * p->beta_coeff);
+        double squared_radius = 3.0 * nu * p->beta_coeff * p->tau_drag;
+        double fac = (4. / 3.) * 3.14159265359 * pow(squared_radius, 3. / 2.);
+        ovec fp;
+        if (LHAS(L, CH4G_LAGRANGE_TWOWAY_SMALLTAUD)) {
+            fp.x = (p->Dt_ux - gv.x) * (1 - density_ratio);
+            fp.y = (p->Dt_uy - gv.y) * (1 - density_ratio);
+            fp.z = (p->Dt_uz - gv.z) * (1 - density_ratio);
+        } else {
+            fp.x = p->Dt_ux - gv.x + density_ratio * (gv.x - p->ax);
+            fp.y = p->Dt_uy - gv.y + density_ratio * (gv.y - p->ay);
+            fp.z = p->Dt_uz - gv.z + density_ratio * (gv.z - p->az);
+        }
+        for (int c = 0; c < 8; c++) L->ftw[s.n[c]].x += fac * fp.x * s.w[c];
+        for (int c = 0; c < 8; c++) L->ftw[s.n[c]].y += fac * fp.y * s.w[c];
+        for (int c = 0; c < 8; c++) L->ftw[s.n[c]].z += fac * fp.z * s.w[c];
+    }
+    for (int k = 0; k < o->NZ; k++)
+        for (int j = 0; j < o->NY; j++)
+            for (int i = 0; i < o->NX; i++) {
+                size_t n = IDX(o, i + BRD, j + BRD, k + BRD), m = IDX2(L, i + B2, j + B2, k + B2);
+                o->force[n].x += L->ftw[m].x;
+                o->force[n].y += L->ftw[m].y;
+                o->force[n].z += L->ftw[m].z;
+            }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Public API (ctypes). */
+long oracle_particles_count(const oracle *o) { return o->lagr ? o->lagr->npart : 0; }
+
+long oracle_get_particles(const oracle *o, ch4g_particle *out, long cap)
+{
+    if (!o->lagr) return 0;
+    long n = o->lagr->npart < cap ? o->lagr->npart : cap;
+    memcpy(out, o->lagr->tr, (size_t)n * sizeof(ch4g_particle));
+    return o->lagr->npart;
+}
+
+int oracle_set_particles(oracle *o, const ch4g_particle *in, long n)
+{
+    olagr *L = o->lagr;
+    if (!L) return -1;
+    ch4g_particle *t = (ch4g_particle *)realloc(L->tr, (size_t)(n > 0 ? n : 1) * sizeof(ch4g_particle));
+    if (!t) return -1;
+    L->tr = t;
+    memcpy(L->tr, in, (size_t)n * sizeof(ch4g_particle));
+    L->npart = (int)n;
+    return 0;
+}
+
+/* Interpolation probe for pinning tests: velocity / temperature fields of
+ * the interior given directly ([NZ][NY][NX], u interleaved), halo rules of
+ * boundary_conditions_hydro, interpolation at the particles' positions.     */
+int oracle_particles_probe(oracle *o, const double *u, const double *t)
+{
+    if (!o->lagr) return -1;
+    size_t m = 0;
+    for (int k = BRD; k < o->NZ + BRD; k++)
+        for (int j = BRD; j < o->NY + BRD; j++)
+            for (int i = BRD; i < o->NX + BRD; i++, m++) {
+                size_t n = IDX(o, i, j, k);
+                if (u) {
+                    o->u[n].x = u[3 * m];
+                    o->u[n].y = u[3 * m + 1];
+                    o->u[n].z = u[3 * m + 2];
+                }
+                if (t) o->t[n] = t[m];
+            }
+    bc_hydro(o);
+    interpolate(o);
+    return 0;
+}

@@ -44,6 +44,13 @@ def lib():
         L.oracle_drand48.restype = C.c_double
         L.oracle_random_gauss.argtypes = [P, C.c_double, C.c_double]
         L.oracle_random_gauss.restype = C.c_double
+        L.oracle_particles_init.argtypes = [P, P]
+        L.oracle_particles_count.argtypes = [P]
+        L.oracle_particles_count.restype = C.c_long
+        L.oracle_get_particles.argtypes = [P, P, C.c_long]
+        L.oracle_get_particles.restype = C.c_long
+        L.oracle_set_particles.argtypes = [P, P, C.c_long]
+        L.oracle_particles_probe.argtypes = [P, dp, dp]
         _lib = L
     return _lib
 
@@ -110,6 +117,30 @@ class Oracle:
         a = np.empty((6, 3))
         self._L.oracle_get_hit_phases(self._h, _ptr(a))
         return a
+
+    def particles_init(self, pcfg):
+        if self._L.oracle_particles_init(self._h, C.byref(pcfg)) != 0:
+            raise MemoryError("oracle_particles_init failed")
+
+    def particles(self):
+        """Particle records in array order, numpy structured array."""
+        from ch4g.abi import particle_dtype
+        n = self._L.oracle_particles_count(self._h)
+        a = np.zeros(n, dtype=particle_dtype())
+        self._L.oracle_get_particles(self._h, a.ctypes.data, n)
+        return a
+
+    def set_particles(self, a):
+        from ch4g.abi import particle_dtype
+        a = np.ascontiguousarray(a, dtype=particle_dtype())
+        self._L.oracle_set_particles(self._h, a.ctypes.data, len(a))
+
+    def probe(self, u=None, t=None):
+        """Interpolate given interior fields u [NZ,NY,NX,3], t [NZ,NY,NX] at the particles."""
+        u = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        t = None if t is None else np.ascontiguousarray(t, dtype=np.float64)
+        self._L.oracle_particles_probe(self._h, _ptr(u), _ptr(t))
+        return self.particles()
 
     def dump_averages(self):
         from ch4g.abi import Output

@@ -1,0 +1,172 @@
+"""GPU parity of the Lagrangian particles: the HIP kernels through the C-ABI
+against the oracle's restatement of lagrange.c / lagrange_feedback.c.
+
+FP64 lattices: every particle field, and with two-way coupling the fluid
+populations, must be bit-identical to the oracle (both evaluate the
+reference's expressions in its order without contraction; the feedback is
+summed per node in particle order, as the reference's sequential loop does).
+FP32-deviation lattice: particle positions within 1e-6 (absolute, lattice
+units) of the FP64 oracle after 100 steps, velocities within 1e-5 of the
+largest particle velocity component.
+"""
+import numpy as np
+import pytest
+
+from ch4g import abi, Lattice
+from oracle import Oracle
+import configs
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = abi.PARTICLE_FIELDS
+
+
+def rb_small(NX=64, NY=32, precision=abi.FP64, g=1e-3, seed=2024):
+    """Rayleigh-Benard at Ra 1e5 with a realistic particle gravity: beta_t g
+    is kept, g itself set to `g`."""
+    c = configs.rb2d(NX, NY, 1, Ra=1e5, precision=precision, seed=seed)
+    return c.copy(gravity_y=g, beta_t=c.beta_t * c.gravity_y / g)
+
+
+def pair(cfg, pcfg, steps, records=None):
+    o = Oracle(cfg)
+    o.initial_conditions()
+    o.particles_init(pcfg)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.particles_init(pcfg)
+    if records is not None:
+        o.set_particles(records)
+        lat.set_particles(records)
+    o.step(steps)
+    lat.step(steps)
+    return o, lat
+
+
+def compare_particles(o, lat, exact=True, pos_atol=0.0, rel=0.0):
+    a, b = o.particles(), lat.particles()
+    assert len(a) == len(b), (len(a), len(b))
+    bad = []
+    for f in FIELDS:
+        x, y = a[f], b[f]
+        if exact:
+            if not np.array_equal(x, y):
+                i = int(np.argmax(x != y))
+                bad.append(f"{f}[{i}] oracle {x[i]!r} gpu {y[i]!r}")
+    assert not bad, bad[:8]
+    if not exact:
+        for f in ("x", "y", "z"):
+            d = np.abs(a[f] - b[f])
+            S = {"x": lat.cfg.NX, "y": lat.cfg.NY, "z": lat.cfg.NZ}[f]
+            d = np.minimum(d, S - d)
+            assert d.max() <= pos_atol, (f, d.max())
+        # velocity error relative to the largest particle speed component
+        s = max(np.abs(a[f]).max() for f in ("vx", "vy", "vz"))
+        for f in ("vx", "vy", "vz"):
+            err = np.abs(a[f] - b[f]).max() / (s if s > 0 else 1.0)
+            print(f"  {f}: max error / max|v| = {err:.3e}")
+            assert err <= rel, f
+
+
+def compare_fluid(o, lat):
+    po, go = o.get_populations()
+    pg, gg = lat.get_populations()
+    assert np.array_equal(po, pg)
+    if lat.cfg.has(abi.LB_TEMPERATURE):
+        assert np.array_equal(go, gg)
+
+
+HEAVY = abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_WRAP | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS | \
+    abi.LAGRANGE_GRAVITY
+
+
+def test_rb_tracers_and_inertial_particles_fp64():
+    pc = abi.make_particle_config(HEAVY, 400, tau_drag=(3, 2.0, 8.0), beta_coeff=(2, 0.6, 1.2))
+    o, lat = pair(rb_small(), pc, 200)
+    print("particles alive", lat.particles_count())
+    compare_particles(o, lat)
+    compare_fluid(o, lat)
+
+
+def test_hit_les_particles_fp64():
+    cfg = configs.hit(16, tau=0.51, amp=1e-4)
+    flags = abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_WRAP | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS
+    pc = abi.make_particle_config(flags, 500, tau_drag=(4, 2.0, 10.0), beta_coeff=(1, 0.0, 0.0))
+    o, lat = pair(cfg, pc, 100)
+    compare_particles(o, lat)
+
+
+@pytest.mark.parametrize("extra", [
+    0,
+    abi.LAGRANGE_BC_INELASTIC,
+    abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT,
+    abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT |
+    abi.LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF,
+    abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT | abi.LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM,
+    abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT |
+    abi.LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO,
+    abi.LAGRANGE_SMALLTAUD_BETA,
+    abi.LAGRANGE_SMALLTAUD_FLOATER,
+], ids=["elastic", "inelastic", "reinject", "reinject_half", "reinject_norandom", "reinject_vzero",
+        "smalltaud_beta", "smalltaud_floater"])
+def test_wall_rules_and_small_tau_fp64(extra):
+    # strong settling so that particles reach the walls within the run
+    pc = abi.make_particle_config(HEAVY | extra, 256, tau_drag=(2, 2.0, 20.0), beta_coeff=(2, 0.2, 0.5))
+    o, lat = pair(rb_small(32, 16, g=5e-3), pc, 120)
+    compare_particles(o, lat)
+
+
+def test_twoway_momentum_fp64():
+    pc = abi.make_particle_config(HEAVY | abi.LAGRANGE_TWOWAY_MOMENTUM, 300, tau_drag=(2, 1.0, 4.0),
+                                  beta_coeff=(2, 0.5, 0.8))
+    o, lat = pair(rb_small(48, 24, g=2e-3), pc, 80)
+    compare_particles(o, lat)
+    compare_fluid(o, lat)
+
+
+def test_twoway_smalltaud_feedback_fp64():
+    flags = HEAVY | abi.LAGRANGE_TWOWAY_MOMENTUM | abi.LAGRANGE_TWOWAY_SMALLTAUD | abi.LAGRANGE_SMALLTAUD_BETA
+    pc = abi.make_particle_config(flags, 200, tau_drag=(2, 1.0, 4.0), beta_coeff=(1, 0.5, 0.5))
+    o, lat = pair(rb_small(32, 16, g=2e-3), pc, 50)
+    compare_particles(o, lat)
+    compare_fluid(o, lat)
+
+
+def test_substeps_fp64():
+    pc = abi.make_particle_config(HEAVY, 128, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.7, 0.7),
+                                  time_dt_lagr_factor=3)
+    o, lat = pair(configs.poiseuille(16, 32, 8, tau=0.8, g=1e-5), pc, 60)
+    compare_particles(o, lat)
+
+
+def test_particles_fp32_dev_close_to_fp64():
+    pc = abi.make_particle_config(HEAVY, 400, tau_drag=(3, 2.0, 8.0), beta_coeff=(2, 0.6, 1.2))
+    cfg64 = rb_small()
+    o = Oracle(cfg64)
+    o.initial_conditions()
+    o.particles_init(pc)
+    o.step(100)
+    lat = Lattice(rb_small(precision=abi.FP32_DEV))
+    lat.initial_conditions()
+    lat.particles_init(pc)
+    lat.step(100)
+    compare_particles(o, lat, exact=False, pos_atol=1e-6, rel=1e-5)
+
+
+def test_particle_restart_roundtrip():
+    pc = abi.make_particle_config(HEAVY, 64, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.7, 0.7))
+    cfg = rb_small(32, 16)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.particles_init(pc)
+    lat.step(5)
+    a = lat.particles()
+    lat.set_particles(a)
+    assert np.array_equal(lat.particles(), a)
+    # a larger set than the initial capacity
+    b = np.concatenate([a, a])
+    b["name"] = np.arange(len(b))
+    lat.set_particles(b)
+    assert lat.particles_count() == len(b)
+    lat.step(3)
+    assert np.isfinite(lat.particles()["x"]).all()

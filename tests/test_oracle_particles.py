@@ -1,0 +1,295 @@
+"""Pins of the oracle's Lagrangian particles (CPU only).
+
+The reference ships no particle tests or fixtures, so each part of the
+restatement (oracle/ch4_oracle_lagrange.c) is pinned by what the mathematics
+or a library fixes, each chosen so that a plausible slip (a swapped weight,
+a wrong ghost sign, a missing factor, an off-by-one cell) fails it:
+
+* type ladders, names, positions  <- the parameter arithmetic of
+                                     param.in and glibc drand48 itself
+* trilinear interpolation         <- exact for linear fields; the no-slip
+                                     ghost keeps a wall-vanishing linear
+                                     field linear; periodic halo = image
+* velocity gradient               <- centred and one-sided second-order
+                                     differences are exact for quadratic
+                                     fields; the interpolation fix gives the
+                                     first-cell gradient in the ghost half-cell
+* tracer advance                  <- uniform flow: x_n = x_0 + n u dt
+* inertial advance                <- Stokes relaxation v -> u (1 - e^{-t/tau})
+                                     (second-order AB2 error); terminal
+                                     velocity -g tau (1 - beta)
+* walls and re-injection          <- mirror / stop rules; glibc drand48 draws
+* two-way feedback                <- deposits sum to V_p f_p (weights sum to
+                                     one) with f_p in closed form
+"""
+import ctypes
+import ctypes.util
+import math
+
+import numpy as np
+import pytest
+
+from ch4g import abi
+from oracle import Oracle
+
+LIBC = ctypes.CDLL(ctypes.util.find_library("c"))
+LIBC.drand48.restype = ctypes.c_double
+
+
+def glibc_stream(seed, n):
+    LIBC.srand48(ctypes.c_long(seed))
+    return np.array([LIBC.drand48() for _ in range(n)])
+
+
+def quiescent(NX, NY, NZ, flags=0, tau=0.8, seed=7, **kw):
+    cfg = abi.make_config(NX, NY, NZ, abi.LB_FLUID | flags, tau_u=tau, seed=seed, **kw)
+    o = Oracle(cfg)
+    o.initial_conditions()
+    return o, cfg
+
+
+def uniform_flow(o, cfg, u0):
+    W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    CV = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0],
+                   [1, -1, 0], [-1, 1, 0], [-1, -1, 0], [1, 0, 1], [-1, 0, 1], [1, 0, -1], [-1, 0, -1], [0, 1, 1],
+                   [0, 1, -1], [0, -1, 1], [0, -1, -1]], dtype=float)
+    cu = CV @ np.asarray(u0)
+    feq = W * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * float(np.dot(u0, u0)))
+    p = np.broadcast_to(feq, (cfg.NZ, cfg.NY, cfg.NX, 19)).copy()
+    o.set_populations(p)
+
+
+def records(n):
+    return np.zeros(n, dtype=abi.particle_dtype())
+
+
+# --------------------------------------------------------------------------
+def test_type_ladders_and_names():
+    o, _ = quiescent(8, 8, 8)
+    flags = abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS
+    o.particles_init(abi.make_particle_config(flags, 30, tau_drag=(3, 0.5, 1.5), beta_coeff=(2, 0.5, 1.0)))
+    a = o.particles()
+    assert len(a) == 30
+    assert np.array_equal(a["name"], np.arange(30))
+    ntypes = 3 * 2 + 1
+    for r in a:
+        t = int(r["name"]) % ntypes
+        if t == 0:
+            assert r["tau_drag"] == 0.0 and r["beta_coeff"] == 1.0
+        else:
+            h = t - 1
+            assert r["tau_drag"] == 0.5 + (h % 3) * 0.5
+            assert r["beta_coeff"] == 0.5 + ((h // 3) % 2) * 0.5
+    # geometric ladder: min * step^i with step = (max/min)^(1/(types-1))
+    o2, _ = quiescent(8, 8, 8)
+    o2.particles_init(abi.make_particle_config(abi.LAGRANGE_TAUDRAG_INCREMENT_LOG, 8, tau_drag=(4, 0.1, 10.0)))
+    t = o2.particles()["tau_drag"][:4]
+    step = (10.0 / 0.1) ** (1 / 3)
+    assert np.allclose(t, 0.1 * step ** np.arange(4), rtol=1e-15)
+    assert t[-1] == pytest.approx(10.0, rel=1e-14)
+
+
+def test_positions_follow_the_drand48_stream():
+    # fluid only, no noise: the first draws after srand48 are the positions
+    o, cfg = quiescent(16, 8, 4, seed=99)
+    o.particles_init(abi.make_particle_config(0, 20, tau_drag=(1, 2.0, 2.0)))
+    a = o.particles()
+    r = glibc_stream(99, 60).reshape(20, 3)
+    assert np.array_equal(a["x"], r[:, 0] * 16)
+    assert np.array_equal(a["y"], r[:, 1] * 8)
+    assert np.array_equal(a["z"], r[:, 2] * 4)
+    # with temperature noise the k,j,i noise draws come first
+    cfg2 = abi.make_config(4, 4, 2, abi.LB_FLUID | abi.LB_TEMPERATURE | abi.TEMPERATURE_INITIAL_ADD_NOISE,
+                           seed=5, noise_t=1e-3)
+    o2 = Oracle(cfg2)
+    o2.initial_conditions()
+    o2.particles_init(abi.make_particle_config(0, 3, tau_drag=(1, 1.0, 1.0)))
+    r2 = glibc_stream(5, 32 + 9)[32:].reshape(3, 3)
+    assert np.array_equal(o2.particles()["x"], r2[:, 0] * 4)
+
+
+def _probe(o, pos, u=None, t=None):
+    a = records(len(pos))
+    a["x"], a["y"], a["z"] = pos[:, 0], pos[:, 1], pos[:, 2]
+    o.set_particles(a)
+    return o.probe(u, t)
+
+
+def _centres(cfg):
+    z, y, x = np.meshgrid(np.arange(cfg.NZ) + 0.5, np.arange(cfg.NY) + 0.5, np.arange(cfg.NX) + 0.5, indexing="ij")
+    return x, y, z
+
+
+def test_trilinear_exact_for_linear_fields():
+    o, cfg = quiescent(12, 10, 8, flags=abi.LB_TEMPERATURE)
+    o.particles_init(abi.make_particle_config(abi.LAGRANGE_GRADIENT, 1, tau_drag=(1, 1.0, 1.0)))
+    x, y, z = _centres(cfg)
+    a0, B = np.array([0.1, -0.2, 0.3]), np.array([[0.01, 0.02, -0.03], [0.005, -0.01, 0.02], [0.03, 0.01, 0.004]])
+    u = a0 + np.stack([x, y, z], -1) @ B.T
+    t = 0.7 + 0.05 * x - 0.02 * y + 0.01 * z
+    rng = np.random.default_rng(0)
+    pos = np.column_stack([rng.uniform(0.5, 11.5, 200), rng.uniform(0.5, 9.5, 200), rng.uniform(0.5, 7.5, 200)])
+    r = _probe(o, pos, u, t)
+    want = a0 + pos @ B.T
+    got = np.column_stack([r["ux"], r["uy"], r["uz"]])
+    assert np.allclose(got, want, rtol=0, atol=1e-14)
+    assert np.allclose(r["t"], 0.7 + 0.05 * pos[:, 0] - 0.02 * pos[:, 1] + 0.01 * pos[:, 2], atol=1e-14)
+    # gradient of a linear field: centred differences are exact away from the periodic seams
+    inner = (pos[:, 0] > 1.5) & (pos[:, 0] < 10.5) & (pos[:, 1] > 1.5) & (pos[:, 1] < 8.5) & \
+            (pos[:, 2] > 1.5) & (pos[:, 2] < 6.5)
+    G = np.column_stack([r[n] for n in ["dx_ux", "dy_ux", "dz_ux", "dx_uy", "dy_uy", "dz_uy",
+                                        "dx_uz", "dy_uz", "dz_uz"]])[inner]
+    assert np.allclose(G, B.reshape(1, 9), atol=1e-14)
+
+
+def test_periodic_halo_is_the_image():
+    o, cfg = quiescent(8, 6, 4)
+    o.particles_init(abi.make_particle_config(0, 1, tau_drag=(1, 1.0, 1.0)))
+    x, y, z = _centres(cfg)
+    f = np.sin(2 * np.pi * x / 8) + 0.3 * np.cos(2 * np.pi * y / 6) + 0.1 * z
+    u = np.stack([f, 2 * f, -f], -1)
+    # across the x seam, y and z on cell centres: blend of u(NX-1) and u(0)
+    pos = np.array([[7.8, 2.5, 1.5], [0.2, 2.5, 1.5], [8.3, 2.5, 1.5], [-0.2, 2.5, 1.5]])
+    r = _probe(o, pos, u)
+    fl, fr = f[1, 2, 7], f[1, 2, 0]
+    for p, got in zip(pos[:, 0], r["ux"]):
+        xw = p % 8
+        w = (xw - 7.5) if xw >= 7.5 else (xw + 0.5)
+        assert got == pytest.approx((1 - w) * fl + w * fr, abs=1e-15)
+
+
+def test_noslip_ghost_and_gradient_near_wall():
+    o, cfg = quiescent(6, 8, 6, flags=abi.FLUID_BC_Y)
+    o.particles_init(abi.make_particle_config(abi.LAGRANGE_GRADIENT, 1, tau_drag=(1, 1.0, 1.0)))
+    x, y, z = _centres(cfg)
+    # u_x = a y vanishes at the bottom wall: the ghost u_g = -u keeps it linear
+    a = 0.01
+    u = np.stack([a * y, 0 * y, 0 * y], -1)
+    pos = np.array([[2.7, 0.1, 3.2], [1.1, 0.45, 2.9], [4.0, 0.3, 0.7]])
+    r = _probe(o, pos, u)
+    assert np.allclose(r["ux"], a * pos[:, 1], atol=1e-16)
+    # quadratic field: centred and one-sided second-order differences are exact
+    q = lambda X, Y, Z: 0.01 * X * X - 0.02 * X * Y + 0.003 * Y * Y + 0.004 * Z * Z + 0.05 * Y  # noqa: E731
+    dqdy = lambda X, Y, Z: -0.02 * X + 0.006 * Y + 0.05  # noqa: E731
+    u = np.stack([q(x, y, z), 0 * y, 0 * y], -1)
+    pos = np.array([[2.2, 0.7, 2.5], [3.1, 7.2, 2.6], [2.9, 3.3, 3.1]])
+    r = _probe(o, pos, u)
+    assert np.allclose(r["dy_ux"], dqdy(pos[:, 0], pos[:, 1], pos[:, 2]), atol=1e-14)
+    # ghost half-cell (y < 1/2): the interpolation fix uses the first-row gradient
+    pos = np.array([[2.5, 0.2, 2.5]])
+    r = _probe(o, pos, u)
+    assert r["dy_ux"][0] == pytest.approx(dqdy(2.5, 0.5, 2.5), abs=1e-14)
+
+
+def test_tracers_move_with_uniform_flow():
+    o, cfg = quiescent(16, 8, 8, tau=0.9)
+    u0 = np.array([0.031, -0.017, 0.009])
+    uniform_flow(o, cfg, u0)
+    o.particles_init(abi.make_particle_config(abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_WRAP, 10,
+                                              tau_drag=(1, 0.0, 0.0)))
+    a0 = o.particles()
+    n = 40
+    o.step(n)
+    a = o.particles()
+    for c, S, u in zip("xyz", (16, 8, 8), u0):
+        want = np.mod(a0[c] + n * u, S)
+        d = np.abs(a[c] - want)
+        assert np.all(np.minimum(d, S - d) < 1e-12)
+        assert np.all((a[c] >= 0) & (a[c] < S))
+    assert np.allclose(a["vx"], u0[0], atol=1e-15)
+
+
+def test_stokes_relaxation_and_terminal_velocity():
+    tau_p = 20.0
+    o, cfg = quiescent(8, 8, 8, tau=0.8)
+    u0 = np.array([0.02, 0.0, -0.01])
+    uniform_flow(o, cfg, u0)
+    o.particles_init(abi.make_particle_config(0, 4, tau_drag=(1, tau_p, tau_p)))
+    n = 60
+    o.step(n)
+    v = np.column_stack([o.particles()[c] for c in ("vx", "vy", "vz")])
+    exact = u0 * (1 - math.exp(-n / tau_p))
+    # AB2 after an Euler start: O(dt^2/tau^2) relative error of the transient
+    assert np.allclose(v, exact, rtol=2e-3, atol=1e-6)
+    assert not np.allclose(v, u0 * (1 - math.exp(-n / (2 * tau_p))), rtol=2e-3)
+    # settling in a quiescent fluid: -g tau (1 - beta), Du/Dt = 0
+    g, tau_p, beta = 1e-3, 5.0, 0.6
+    o2, _ = quiescent(6, 6, 6, gravity_y=g)
+    flags = abi.LAGRANGE_GRAVITY | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS
+    o2.particles_init(abi.make_particle_config(flags, 3, tau_drag=(1, tau_p, tau_p), beta_coeff=(1, beta, beta)))
+    o2.step(400)
+    a = o2.particles()
+    assert np.allclose(a["vy"], -g * tau_p * (1 - beta), rtol=1e-12)
+    assert np.allclose(a["vx"], 0.0, atol=1e-20)
+
+
+def _one_particle(o, **fields):
+    a = records(1)
+    for k, v in fields.items():
+        a[k] = v
+    o.set_particles(a)
+
+
+def test_walls_mirror_stop_and_reinject():
+    g = 1e-3
+    # elastic: mirror position, reverse the normal velocity
+    o, cfg = quiescent(6, 8, 6, flags=abi.FLUID_BC_Y, gravity_y=g)
+    o.particles_init(abi.make_particle_config(0, 1, tau_drag=(1, 1e9, 1e9)))
+    _one_particle(o, x=2.0, y=0.2, z=3.0, vy=-0.3, vx=0.01, tau_drag=1e9)
+    o.step(1)
+    p = o.particles()[0]
+    assert p["y"] == pytest.approx(0.1, abs=1e-12) and p["vy"] == pytest.approx(0.3, rel=1e-8)
+    assert p["vx"] == pytest.approx(0.01, rel=1e-8)
+    _one_particle(o, x=2.0, y=7.9, z=3.0, vy=0.3, tau_drag=1e9, vy_old=0.3)
+    o.step(1)
+    p = o.particles()[0]
+    assert p["y"] == pytest.approx(7.8, abs=1e-12) and p["vy"] < 0
+    # inelastic: stop on the wall
+    o, _ = quiescent(6, 8, 6, flags=abi.FLUID_BC_Y)
+    o.particles_init(abi.make_particle_config(abi.LAGRANGE_BC_INELASTIC, 1, tau_drag=(1, 1e9, 1e9)))
+    _one_particle(o, x=2.0, y=0.2, z=3.0, vy=-0.3, vx=0.01, tau_drag=1e9)
+    o.step(1)
+    p = o.particles()[0]
+    assert p["y"] == 0.0 and p["vx"] == 0.0 and p["vy"] == 0.0
+    # re-injection: x = SX r, y + SY, z = SZ r with the next two drand48 draws;
+    # Stokes velocity -g (1-beta) tau, halved
+    seed = 31
+    o, _ = quiescent(6, 8, 4, flags=abi.FLUID_BC_Y, seed=seed, gravity_y=g)
+    flags = (abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT | abi.LAGRANGE_GRAVITY |
+             abi.LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF | abi.LAGRANGE_WRAP)
+    o.particles_init(abi.make_particle_config(flags, 2, tau_drag=(1, 4.0, 4.0)))
+    # consumed: 6 draws of positions; then overwrite with a falling particle
+    a = records(2)
+    a["x"], a["y"], a["z"] = [1.0, 3.0], [0.05, 4.0], [1.0, 2.0]
+    a["vy"], a["tau_drag"] = [-0.2, 0.0], [4.0, 4.0]
+    o.set_particles(a)
+    o.step(1)
+    r = glibc_stream(seed, 8)[6:]
+    p = o.particles()
+    assert p["x"][0] == 6 * r[0] and p["z"][0] == 4 * r[1]
+    assert p["y"][0] == pytest.approx(0.05 - 0.2 + 8, abs=1e-12)
+    assert p["vy"][0] == -g * (1 - 0.0) * 4.0 / 2.0
+    assert p["y"][1] == 4.0 and p["x"][1] == 3.0
+
+
+def test_twoway_feedback_deposit():
+    g, tau_p, beta = 1e-4, 3.0, 0.6
+    o, cfg = quiescent(8, 8, 8, tau=0.8, gravity_y=g)
+    flags = (abi.LAGRANGE_GRAVITY | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS |
+             abi.LAGRANGE_TWOWAY_MOMENTUM)
+    o.particles_init(abi.make_particle_config(flags, 1, tau_drag=(1, tau_p, tau_p), beta_coeff=(1, beta, beta)))
+    _one_particle(o, x=3.3, y=4.6, z=2.8, tau_drag=tau_p, beta_coeff=beta)
+    o.step(1)  # first move: a = -g (1 - beta) in y, Du/Dt = 0
+    p = o.particles()[0]
+    assert p["ay"] == pytest.approx(-g * (1 - beta), rel=1e-12)
+    o.step(1)  # second step: deposit on the force field
+    F = o.hydro_fields()["force"]
+    nu = (0.8 - 0.5) / 3
+    vol = 4 / 3 * 3.14159265359 * (3 * nu * beta * tau_p) ** 1.5
+    dr = (3 - beta) / (2 * beta)
+    fpy = 0.0 - (-g) + dr * (-g - p["ay"])
+    assert F[..., 1].sum() == pytest.approx(vol * fpy, rel=1e-12)
+    assert np.count_nonzero(F[..., 1]) == 8
+    # trilinear weights: the node at (3,4,2) (centre 3.5,4.5,2.5) gets (1-0.2)(1-0.1)(1-0.3)
+    assert F[2, 4, 3, 1] == pytest.approx(vol * fpy * 0.8 * 0.9 * 0.7, rel=1e-12)
+    assert F[..., 0].sum() == pytest.approx(0.0, abs=1e-30)
