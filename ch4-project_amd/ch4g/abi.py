@@ -146,3 +146,30 @@ def particle_dtype():
     """numpy structured dtype with the layout of struct ch4g_particle."""
     import numpy as np
     return np.dtype([(n, np.float64) for n in PARTICLE_FIELDS])
+
+
+SERVE_BATCHED = 0
+SERVE_SEQUENTIAL = 1
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("cfg", Config),
+        ("nsteps", C.c_int64),
+        ("arrival_s", C.c_double),
+        ("admit_s", C.c_double),
+        ("finish_s", C.c_double),
+        ("out", Output),
+        ("status", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class ServeParams(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_int64),
+        ("mode", C.c_int32),
+        ("chunk_steps", C.c_int32),
+        ("max_active", C.c_int32),
+        ("use_graphs", C.c_int32),
+    ]

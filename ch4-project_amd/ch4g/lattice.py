@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from .abi import NPOP, Config, Output, ParticleConfig, particle_dtype
+from .abi import NPOP, Config, Job, Output, ParticleConfig, ServeParams, particle_dtype
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libch4g.so")
@@ -23,7 +23,7 @@ EXPORTS = [
     "ch4g_set_populations", "ch4g_get_populations", "ch4g_step", "ch4g_hydro_fields", "ch4g_dump_averages",
     "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_step_timed", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
     "ch4g_footprint_bytes", "ch4g_particles_init", "ch4g_particles_count", "ch4g_get_particles",
-    "ch4g_set_particles",
+    "ch4g_set_particles", "ch4g_step_batch", "ch4g_serve",
 ]
 
 
@@ -67,6 +67,8 @@ def lib():
     L.ch4g_particles_count.argtypes = [P, C.POINTER(C.c_int64)]
     L.ch4g_get_particles.argtypes = [P, P, C.c_int64, C.POINTER(C.c_int64)]
     L.ch4g_set_particles.argtypes = [P, P, C.c_int64]
+    L.ch4g_step_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
+    L.ch4g_serve.argtypes = [C.POINTER(ServeParams), C.POINTER(Job), C.c_int32]
     _lib = L
     return L
 
@@ -86,6 +88,23 @@ def device_count():
 
 def footprint_bytes(cfg):
     return lib().ch4g_footprint_bytes(C.byref(cfg))
+
+
+def step_batch(lattices, nsteps):
+    """Advance several lattices together (one launch per step for all)."""
+    arr = (C.c_void_p * len(lattices))(*[l.handle for l in lattices])
+    check(lib().ch4g_step_batch(arr, len(lattices), int(nsteps)))
+
+
+def serve(jobs, mode=0, hbm_budget=0, chunk_steps=0, max_active=0, use_graphs=True):
+    """Run a request stream (list of abi.Job, modified in place with the
+    admission/finish times, averages and status); returns the list."""
+    arr = (Job * len(jobs))(*jobs)
+    sp = ServeParams()
+    sp.hbm_budget, sp.mode, sp.chunk_steps, sp.max_active = int(hbm_budget), int(mode), int(chunk_steps), int(max_active)
+    sp.use_graphs = 1 if use_graphs else 0
+    check(lib().ch4g_serve(C.byref(sp), arr, len(jobs)))
+    return list(arr)
 
 
 class Lattice:

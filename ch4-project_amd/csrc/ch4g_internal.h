@@ -184,6 +184,20 @@ struct FeedbackBufs {
 int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s);
 size_t feedback_sort_bytes(int64_t nitems, int key_bits);
 
+// One lattice of a batched step: its constants, its two state buffers and
+// the range of 256-node tiles it owns in the batch's tile space.
+struct BatchEntry {
+    KParams P;
+    void *f[2], *g[2];
+    void *tlag, *taules;
+    int cur;            // buffer holding the state at parity 0
+    int64_t tile0;      // first tile (entries sorted by tile0)
+};
+void launch_batch_fp32(const BatchEntry *tab, int ninst, int64_t ntiles, int fluid, int thermal, int arith64,
+                       bool ntl, int parity, hipStream_t s);
+void launch_batch_fp64(const BatchEntry *tab, int ninst, int64_t ntiles, int fluid, int thermal, int parity,
+                       hipStream_t s);
+
 // Kernel launchers (stream-ordered).
 void launch_step_fp64(const KParams &P, const double *fin, const double *gin, double *fout, double *gout,
                       double *tlag, double *taules, hipStream_t s);

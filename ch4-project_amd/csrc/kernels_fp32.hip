@@ -307,18 +307,16 @@ __device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, 
     }
 }
 
+// One node of the fused step; shared by the single-lattice kernel and the
+// batched kernel, so a lattice advances bit-identically in either.
 // A = arithmetic type of the collision (float for CH4G_FP32_DEV, double for
 // CH4G_FP32_STORE); storage is float in both.
 template <bool FIRST, bool FLUID, bool THERMAL, typename A, bool NTL>
-__global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__restrict__ fin,
-                                                   const float *__restrict__ gin, float *__restrict__ fout,
-                                                   float *__restrict__ gout, float *__restrict__ tlag,
-                                                   float *__restrict__ taules)
+__device__ __forceinline__ void step_node32(const KParams &P, const float *__restrict__ fin,
+                                            const float *__restrict__ gin, float *__restrict__ fout,
+                                            float *__restrict__ gout, float *__restrict__ tlag,
+                                            float *__restrict__ taules, int x, int y, int z, uint32_t n)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= P.NX) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
     const Row R = row_of(P, y);
     const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
 
@@ -341,6 +339,47 @@ __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__res
     A ux = 0, uy = 0, uz = 0;
     if constexpr (FLUID) collide_fluid<A>(P, R, x, y, z, n, f, dTlag, dT, fout, taules, ux, uy, uz);
     if constexpr (THERMAL) collide_thermal<A>(P, R, y, n, g, dT, ux, uy, uz, gout, tlag);
+}
+
+template <bool FIRST, bool FLUID, bool THERMAL, typename A, bool NTL>
+__global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__restrict__ fin,
+                                                   const float *__restrict__ gin, float *__restrict__ fout,
+                                                   float *__restrict__ gout, float *__restrict__ tlag,
+                                                   float *__restrict__ taules)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= P.NX) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    step_node32<FIRST, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
+}
+
+// Many lattices in one launch: block b advances 256 consecutive nodes of the
+// lattice whose tile range holds b (table sorted by first tile, found by a
+// block-uniform binary search); the state buffer of each lattice alternates
+// with the step parity.
+template <bool FLUID, bool THERMAL, typename A, bool NTL>
+__global__ void __launch_bounds__(256) k_step_batch32(const BatchEntry *__restrict__ tab, int ninst, int parity)
+{
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].tile0 <= (int64_t)blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchEntry &E = tab[lo];
+    const KParams P = E.P;
+    const int64_t m = ((int64_t)blockIdx.x - E.tile0) * 256 + threadIdx.x;
+    if (m >= P.N) return;
+    const uint32_t n = (uint32_t)m;
+    const uint32_t x = n % (uint32_t)P.NX, r = n / (uint32_t)P.NX;
+    const uint32_t y = r % (uint32_t)P.NY, z = r / (uint32_t)P.NY;
+    const int in = E.cur ^ parity;
+    step_node32<false, FLUID, THERMAL, A, NTL>(P, (const float *)E.f[in], (const float *)E.g[in],
+                                               (float *)E.f[in ^ 1], (float *)E.g[in ^ 1], (float *)E.tlag,
+                                               (float *)E.taules, (int)x, (int)y, (int)z, n);
 }
 
 __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__restrict__ fin,
@@ -448,6 +487,34 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
     }
 #undef CH4G_LAUNCH32
 #undef CH4G_LAUNCH32_
+}
+
+void launch_batch_fp32(const BatchEntry *tab, int ninst, int64_t ntiles, int fluid, int thermal, int arith64,
+                       bool ntl, int parity, hipStream_t s)
+{
+    dim3 grid((unsigned)ntiles), block(256);
+#define CH4G_BATCH32_(FL_, TH_, A_)                                                                                     \
+    do {                                                                                                              \
+        if (ntl)                                                                                                      \
+            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, true>), grid, block, 0, s, tab, ninst, parity);          \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, false>), grid, block, 0, s, tab, ninst, parity);         \
+    } while (0)
+#define CH4G_BATCH32(FL_, TH_)                                                                                        \
+    do {                                                                                                              \
+        if (arith64)                                                                                                  \
+            CH4G_BATCH32_(FL_, TH_, double);                                                                          \
+        else                                                                                                          \
+            CH4G_BATCH32_(FL_, TH_, float);                                                                           \
+    } while (0)
+    if (fluid && thermal)
+        CH4G_BATCH32(true, true);
+    else if (fluid)
+        CH4G_BATCH32(true, false);
+    else
+        CH4G_BATCH32(false, true);
+#undef CH4G_BATCH32
+#undef CH4G_BATCH32_
 }
 
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,

@@ -156,16 +156,14 @@ __device__ __forceinline__ double les_tau(const KParams &P, const double f[Q], d
     return 3.0 * (nu_les + P.nu) + 0.5;
 }
 
+// One node of the fused step (shared by the single-lattice and the batched
+// kernels, so a lattice advances bit-identically in either).
 template <bool FIRST, bool FLUID, bool THERMAL>
-__global__ void __launch_bounds__(256) k_step_fp64(KParams P, const double *__restrict__ fin,
-                                                   const double *__restrict__ gin, double *__restrict__ fout,
-                                                   double *__restrict__ gout, double *__restrict__ tlag,
-                                                   double *__restrict__ taules)
+__device__ __forceinline__ void step_node64(const KParams &P, const double *__restrict__ fin,
+                                            const double *__restrict__ gin, double *__restrict__ fout,
+                                            double *__restrict__ gout, double *__restrict__ tlag,
+                                            double *__restrict__ taules, int x, int y, int z, uint32_t n)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= P.NX) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
     const size_t N = (size_t)P.N;
     const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
 
@@ -231,6 +229,44 @@ __global__ void __launch_bounds__(256) k_step_fp64(KParams P, const double *__re
         }
         tlag[n] = T;
     }
+}
+
+template <bool FIRST, bool FLUID, bool THERMAL>
+__global__ void __launch_bounds__(256) k_step_fp64(KParams P, const double *__restrict__ fin,
+                                                   const double *__restrict__ gin, double *__restrict__ fout,
+                                                   double *__restrict__ gout, double *__restrict__ tlag,
+                                                   double *__restrict__ taules)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= P.NX) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    step_node64<FIRST, FLUID, THERMAL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
+}
+
+// Many lattices in one launch (see k_step_batch32).
+template <bool FLUID, bool THERMAL>
+__global__ void __launch_bounds__(256) k_step_batch64(const BatchEntry *__restrict__ tab, int ninst, int parity)
+{
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].tile0 <= (int64_t)blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchEntry &E = tab[lo];
+    const KParams P = E.P;
+    const int64_t m = ((int64_t)blockIdx.x - E.tile0) * 256 + threadIdx.x;
+    if (m >= P.N) return;
+    const uint32_t n = (uint32_t)m;
+    const uint32_t x = n % (uint32_t)P.NX, r = n / (uint32_t)P.NX;
+    const uint32_t y = r % (uint32_t)P.NY, z = r / (uint32_t)P.NY;
+    const int in = E.cur ^ parity;
+    step_node64<false, FLUID, THERMAL>(P, (const double *)E.f[in], (const double *)E.g[in], (double *)E.f[in ^ 1],
+                                       (double *)E.g[in ^ 1], (double *)E.tlag, (double *)E.taules, (int)x, (int)y,
+                                       (int)z, n);
 }
 
 // Streaming of the last step + hydro_fields, without a new collision:
@@ -308,6 +344,18 @@ void launch_step_fp64(const KParams &P, const double *fin, const double *gin, do
         if (P.first) CH4G_LAUNCH64(true, false, true); else CH4G_LAUNCH64(false, false, true);
     }
 #undef CH4G_LAUNCH64
+}
+
+void launch_batch_fp64(const BatchEntry *tab, int ninst, int64_t ntiles, int fluid, int thermal, int parity,
+                       hipStream_t s)
+{
+    dim3 grid((unsigned)ntiles), block(256);
+    if (fluid && thermal)
+        hipLaunchKernelGGL((k_step_batch64<true, true>), grid, block, 0, s, tab, ninst, parity);
+    else if (fluid)
+        hipLaunchKernelGGL((k_step_batch64<true, false>), grid, block, 0, s, tab, ninst, parity);
+    else
+        hipLaunchKernelGGL((k_step_batch64<false, true>), grid, block, 0, s, tab, ninst, parity);
 }
 
 void launch_gather_fp64(const KParams &P, const double *fin, const double *gin, const double *tlag, double *fpost,

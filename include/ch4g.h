@@ -251,6 +251,49 @@ int ch4g_get_particles(ch4g_lattice *lat, ch4g_particle *out, int64_t cap, int64
  * differ from particle_number.                                              */
 int ch4g_set_particles(ch4g_lattice *lat, const ch4g_particle *in, int64_t n);
 
+/* ------------------------------------------------------------------------ */
+/* Serving: many independent lattices on one GPU.  No reference counterpart
+ * (the reference runs one job per MPI launch, main.c); these entry points
+ * are for hosts that run ensembles of the reference's jobs.                 */
+
+/* Advance n lattices by nsteps steps together: one kernel launch per step
+ * (per storage/population variant) covers every lattice through a table of
+ * 256-node tiles.  Each lattice's fields are bit-identical to ch4g_step on
+ * it alone.  Lattices must be past their first step (ch4g_step(lat, 1)),
+ * without HIT forcing or particles, on the current device.                  */
+int ch4g_step_batch(ch4g_lattice *const *lats, int n, int nsteps);
+
+/* One job of a served ensemble. */
+typedef struct ch4g_job {
+    ch4g_config cfg;      /* lattice of the job */
+    int64_t nsteps;       /* steps to run (main.c time loop length) */
+    double arrival_s;     /* release time, seconds after ch4g_serve starts */
+    double admit_s;       /* out: state resident, first step queued */
+    double finish_s;      /* out: last step done, averages read back */
+    ch4g_output out;      /* out: dump_averages of the last step */
+    int32_t status;       /* out: 0 done, negative: not run (message in ch4g_last_error) */
+    int32_t reserved;
+} ch4g_job;
+
+enum ch4g_serve_mode {
+    CH4G_SERVE_BATCHED    = 0, /* continuous batching: admit while others run, one launch per step for all */
+    CH4G_SERVE_SEQUENTIAL = 1  /* one job at a time through ch4g_step (the plain single-job path) */
+};
+
+typedef struct ch4g_serve_params {
+    int64_t hbm_budget;   /* device bytes the jobs may hold at once (0: 90% of free memory) */
+    int32_t mode;         /* enum ch4g_serve_mode */
+    int32_t chunk_steps;  /* steps between admission points (0: 32); replayed from a captured graph */
+    int32_t max_active;   /* lattices resident at once (0: no limit but the budget) */
+    int32_t use_graphs;   /* capture the chunk's launches in a hipGraph and replay it */
+} ch4g_serve_params;
+
+/* Run a request stream: jobs are released at arrival_s (wall clock from the
+ * call), admitted in arrival order while their device footprint fits the
+ * budget, advanced together, and retired (averages read, memory freed) as
+ * they finish.  Returns when every job is done or rejected.                */
+int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs);
+
 /* Device bytes a lattice with this configuration would hold (admission). */
 int64_t ch4g_footprint_bytes(const ch4g_config *cfg);
 

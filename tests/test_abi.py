@@ -36,13 +36,16 @@ def test_struct_layout_matches_header(tmp_path):
         "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(ch4g_config), offsetof(ch4g_config, tau_u),"
         " offsetof(ch4g_config, T0), sizeof(ch4g_output), offsetof(ch4g_output, nuy));"
         " printf(\"%zu %zu %zu %zu\\n\", sizeof(ch4g_particle_config), offsetof(ch4g_particle_config, tau_drag_min),"
-        " sizeof(ch4g_particle), offsetof(ch4g_particle, Dt_uy)); return 0;}\n")
+        " sizeof(ch4g_particle), offsetof(ch4g_particle, Dt_uy));"
+        " printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(ch4g_job), offsetof(ch4g_job, out), offsetof(ch4g_job, status),"
+        " sizeof(ch4g_serve_params), offsetof(ch4g_serve_params, use_graphs)); return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", INCLUDE_DIR, str(src), "-o", str(exe)], check=True)
     got = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()))
     want = [C.sizeof(abi.Config), abi.Config.tau_u.offset, abi.Config.T0.offset, C.sizeof(abi.Output),
             abi.Output.nuy.offset, C.sizeof(abi.ParticleConfig), abi.ParticleConfig.tau_drag_min.offset,
-            C.sizeof(abi.Particle), abi.Particle.Dt_uy.offset]
+            C.sizeof(abi.Particle), abi.Particle.Dt_uy.offset, C.sizeof(abi.Job), abi.Job.out.offset,
+            abi.Job.status.offset, C.sizeof(abi.ServeParams), abi.ServeParams.use_graphs.offset]
     assert abi.particle_dtype().itemsize == C.sizeof(abi.Particle)
     assert got == want
 

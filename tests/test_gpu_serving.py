@@ -1,0 +1,124 @@
+"""GPU checks of the serving path: batched steps over many lattices, and the
+continuous-batching request loop.
+
+The batched kernel runs the same per-node code as the single-lattice kernel,
+so the bar is bit-identity: a lattice advanced in a batch, alone or in a
+mix, ends with exactly the populations (and averages) of ch4g_step.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from ch4g import abi, Lattice, serve, step_batch, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_configs():
+    return [
+        workloads.rb2d(64, 32, 1, Ra=1e5, precision=abi.FP32_DEV, seed=11),
+        workloads.channel3d(24, 20, 16, precision=abi.FP32_DEV),
+        workloads.rb2d(32, 16, 1, Ra=1e5, precision=abi.FP64, seed=12),
+        workloads.channel3d(4, 64, 4, tau=1.0, precision=abi.FP64),
+        workloads.rb2d(40, 24, 1, Ra=1e5, precision=abi.FP32_STORE, seed=13),
+        workloads.rb2d(300, 40, 1, Ra=1e6, precision=abi.FP32_DEV, seed=14),
+    ]
+
+
+def started(cfg):
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.step(1)
+    return lat
+
+
+def same_bits(a, b):
+    return a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def test_step_batch_is_bit_identical_to_single_steps():
+    cfgs = mixed_configs()
+    alone = [started(c) for c in cfgs]
+    mixed = [started(c) for c in cfgs]
+    for n in (7, 10):  # odd then even: buffer parity of every lattice
+        for lat in alone:
+            lat.step(n)
+        step_batch(mixed, n)
+    for a, b in zip(alone, mixed):
+        assert a.itime == b.itime == 18
+        pa, ga = a.get_populations()
+        pb, gb = b.get_populations()
+        assert same_bits(pa, pb)
+        if a.cfg.has(abi.LB_TEMPERATURE):
+            assert same_bits(ga, gb)
+
+
+def test_step_batch_subset_and_rejections():
+    cfgs = mixed_configs()[:3]
+    lats = [started(c) for c in cfgs]
+    ref = started(cfgs[1])
+    step_batch([lats[1]], 5)
+    ref.step(5)
+    assert same_bits(ref.get_populations()[0], lats[1].get_populations()[0])
+    fresh = Lattice(cfgs[0])
+    fresh.initial_conditions()
+    with pytest.raises(Exception, match="first step"):
+        step_batch([fresh], 2)
+
+
+def small_jobs(n=6, seed=5):
+    jobs = workloads.ensemble(n, seed=seed, steps=(40, 160))
+    # small grids for a quick check: rescale every job
+    for i, j in enumerate(jobs):
+        if j.cfg.NZ == 1:
+            j.cfg = workloads.rb2d(48 + 8 * i, 24 + 4 * i, 1, Ra=2e5, precision=abi.FP32_DEV, seed=100 + i)
+        else:
+            j.cfg = workloads.channel3d(16 + 2 * i, 16, 12, precision=abi.FP32_DEV)
+        j.arrival_s = 0.0
+    return jobs
+
+
+def outputs(jobs):
+    return [np.frombuffer(bytes(j.out), dtype=np.float64).copy() for j in jobs]
+
+
+def test_serve_alone_equals_mix_and_plain():
+    jobs = small_jobs()
+    mix = serve([abi.Job.from_buffer_copy(j) for j in jobs], mode=abi.SERVE_BATCHED, chunk_steps=16)
+    assert all(j.status == 0 for j in mix)
+    plain = serve([abi.Job.from_buffer_copy(j) for j in jobs], mode=abi.SERVE_SEQUENTIAL)
+    assert all(j.status == 0 for j in plain)
+    om, op = outputs(mix), outputs(plain)
+    for i, j in enumerate(jobs):
+        alone = serve([abi.Job.from_buffer_copy(j)], mode=abi.SERVE_BATCHED, chunk_steps=16)[0]
+        oa = np.frombuffer(bytes(alone.out), dtype=np.float64)
+        assert oa.tobytes() == om[i].tobytes(), i
+        assert op[i].tobytes() == om[i].tobytes(), i
+    # jobs ran concurrently in the batched mode and one at a time in the plain one
+    adm = np.array([j.admit_s for j in mix])
+    fin = np.array([j.finish_s for j in mix])
+    assert (adm < fin.min()).sum() >= 2
+    seq = sorted(plain, key=lambda j: j.admit_s)
+    for a, b in zip(seq, seq[1:]):
+        assert b.admit_s >= a.finish_s
+
+
+def test_serve_admission_respects_the_footprint():
+    jobs = small_jobs(8, seed=9)
+    lib = __import__("ch4g").lib()
+    need = [lib.ch4g_footprint_bytes(C.byref(j.cfg)) + 40 * j.cfg.nodes() + (1 << 20) for j in jobs]
+    budget = int(2.5 * max(need))
+    done = serve([abi.Job.from_buffer_copy(j) for j in jobs], mode=abi.SERVE_BATCHED, chunk_steps=8,
+                 hbm_budget=budget)
+    assert all(j.status == 0 for j in done)
+    # at every admission instant, the resident jobs fit the budget
+    for j in done:
+        t = j.admit_s
+        resident = [k for k, d in enumerate(done) if d.admit_s <= t < d.finish_s]
+        assert sum(need[k] for k in resident) <= budget
+    # a job larger than the budget is rejected, the others still run
+    big = abi.Job.from_buffer_copy(jobs[0])
+    big.cfg = workloads.channel3d(64, 64, 64)
+    res = serve([big] + [abi.Job.from_buffer_copy(j) for j in jobs[1:3]], hbm_budget=budget)
+    assert res[0].status < 0 and all(r.status == 0 for r in res[1:])
