@@ -95,6 +95,65 @@ def cpu_leg(target_s=12.0):
                       f"un-fused reference-structured sweeps, OpenMP {threads} threads"}
 
 
+def serve_leg(njobs, seed=2026):
+    """Served ensemble: the request stream of workloads.ensemble (Poisson
+    arrivals at 20 jobs/s, 2-D RB and 3-D channel jobs of log-uniform sizes
+    and step counts 1e3..1e5, FP32-deviation) through ch4g_serve, batched
+    (continuous batching) and one job at a time on the plain path."""
+    import numpy as np
+    from ch4g import abi, serve, workloads
+    res = {"workload": f"{njobs} jobs of the served-ensemble stream (workloads.ensemble seed {seed}): Poisson "
+                       "20 jobs/s, 2-D Rayleigh-Benard 64x64..1024x512 Ra 1e6..1e9 and 3-D channel 64^3..192^3, "
+                       "steps log-uniform 1e3..1e5, FP32-deviation"}
+    for name, mode in (("batched", abi.SERVE_BATCHED), ("sequential", abi.SERVE_SEQUENTIAL)):
+        done = serve(workloads.ensemble(njobs, seed=seed), mode=mode)
+        ok = [j for j in done if j.status == 0]
+        work = sum(workloads.job_work(j) for j in ok)
+        span = max(j.finish_s for j in ok)
+        lat = np.array([j.finish_s - j.arrival_s for j in ok])
+        res[name] = {"mlups": round(work / span / 1e6, 1), "makespan_s": round(span, 3),
+                     "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+                     "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "jobs_done": len(ok),
+                     "node_updates": work}
+    res["speedup"] = round(res["batched"]["mlups"] / res["sequential"]["mlups"], 3)
+    return res
+
+
+def particle_leg(steps=40, npart=2_000_000, N=256):
+    """HIT 256^3 (FP32-deviation, LES Cs 0.1) with 1e6 tracers + 1e6 inertial
+    particles: device time per step without and with the particles."""
+    import numpy as np
+    import configs
+    from ch4g import Lattice, abi
+    cfg = configs.hit(N, tau=0.5015, amp=1e-5, les_cs=0.1, precision=abi.FP32_DEV)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.step(5)
+    fluid_ms = lat.step_timed(steps) / steps
+    lat.close()
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.particles_init(abi.make_particle_config(abi.LAGRANGE_WRAP, 1, tau_drag=(1, 1.0, 1.0)))
+    rng = np.random.default_rng(7)
+    a = np.zeros(npart, dtype=abi.particle_dtype())
+    a["name"] = np.arange(npart)
+    for c in ("x", "y", "z"):
+        a[c] = rng.uniform(0.0, N, npart)
+    half = npart // 2
+    a["tau_drag"][half:] = rng.uniform(1.0, 50.0, npart - half)
+    lat.set_particles(a)
+    lat.step(5)
+    both_ms = lat.step_timed(steps) / steps
+    alive = lat.particles_count()
+    lat.close()
+    pms = both_ms - fluid_ms
+    return {"workload": f"HIT {N}^3 FP32-deviation LES Cs=0.1 + {half} tracers + {npart - half} inertial "
+                        "particles (tau_p uniform in 1..50, positions uniform), trilinear interpolation, AB2",
+            "fluid_ms_per_step": round(fluid_ms, 4), "with_particles_ms_per_step": round(both_ms, 4),
+            "particle_ms_per_step": round(pms, 4),
+            "particle_updates_per_s": round(npart / (pms * 1e-3), 1) if pms > 0 else None, "alive": alive}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +162,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-path leg")
     ap.add_argument("--workload", default="rb2d", choices=sorted(WORKLOADS))
+    ap.add_argument("--serve-jobs", type=int, default=16, help="jobs of the served-ensemble leg (0: skip)")
+    ap.add_argument("--no-particles", action="store_true", help="skip the particle leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run plain 1 GiB device-to-device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -150,6 +211,10 @@ def main():
         line["fp64_plain"] = {"mlups": round(r64["mlups"], 2), "avg_launch_us": round(r64["avg_launch_us"], 3),
                               "roofline_frac": round(a64 / HBM_PEAK_GBS, 4),
                               "fp32dev_speedup": round(r32["mlups"] / r64["mlups"], 3)}
+    if not args.no_particles:
+        line["particles"] = particle_leg()
+    if args.serve_jobs > 0:
+        line["serving"] = serve_leg(args.serve_jobs)
     if not args.no_cpu and args.workload == "rb2d":
         line["cpu_baseline"] = cpu_leg()
     print(json.dumps(line), flush=True)

@@ -3,6 +3,7 @@
  * (include/ch4g_dropin.h) on one device-resident lattice.
  */
 #define _POSIX_C_SOURCE 200809L
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -10,8 +11,8 @@
 
 #include "../../include/ch4g_dropin.h"
 
-/* stages of one reference time step (main.c:92-161) */
-enum { ST_IDLE = 0, ST_ADVECTION, ST_COLLISION, ST_FORCING, ST_BC, ST_STREAMED, ST_HYDRO };
+/* stages of one reference time step (main.c:92-183) */
+enum { ST_IDLE = 0, ST_ADVECTION, ST_COLLISION, ST_FORCING, ST_BC, ST_STREAMED, ST_HYDRO, ST_PARTICLES };
 
 static struct {
     ch4g_run_params rp;
@@ -21,6 +22,8 @@ static struct {
     int launched;   /* the fused step of the current itime has been queued */
     double t1, t2;  /* wall-clock bracket of the time loop (main.c:54-56, 198-201) */
     int timing;
+    int particles;  /* initial_conditions_particles done */
+    int moves;      /* move_particles calls of the current step */
 } S;
 
 static double now_s(void)
@@ -95,7 +98,7 @@ void initialization_forcing(void) { need_lattice("initialization_forcing"); }
 void compute_advection(pop *f, pop *rhs_f, my_double tau, pop *f_eq, char which_pop)
 {
     (void)f, (void)rhs_f, (void)tau, (void)f_eq, (void)which_pop;
-    if (S.stage == ST_HYDRO || S.stage == ST_STREAMED) S.stage = ST_IDLE; /* a new step begins */
+    if (S.stage == ST_HYDRO || S.stage == ST_STREAMED || S.stage == ST_PARTICLES) S.stage = ST_IDLE; /* new step */
     order("compute_advection", ST_IDLE, ST_COLLISION, ST_ADVECTION);
     if (!S.timing) {
         S.t1 = now_s();
@@ -161,6 +164,156 @@ void dump_averages(void)
             fclose(fo);
         }
     }
+}
+
+/* ---- Lagrangian particles ---------------------------------------------- */
+
+static void need_particles(const char *fn)
+{
+    need_lattice(fn);
+    if (!S.rp.lagrange) {
+        fprintf(stderr, "Error message -> %s called without LAGRANGE in define.h. Exit.\n", fn);
+        exit(-1);
+    }
+}
+
+/* the particle arrays are allocated with the initial conditions */
+void allocate_particles(void) { need_particles("allocate_particles"); }
+
+void initial_conditions_particles(int restart)
+{
+    need_particles("initial_conditions_particles");
+    if (restart) {
+        fprintf(stderr, "Error message -> resume: load part.h5 records with ch4g_set_particles. Exit.\n");
+        exit(-1);
+    }
+    if (S.itime != 0) {
+        fprintf(stderr, "Error message -> initial_conditions_particles after the first step. Exit.\n");
+        exit(-1);
+    }
+    if (ch4g_particles_init(S.lat, &S.rp.pcfg)) die("initial_conditions_particles");
+    S.particles = 1;
+}
+
+/* from the second step on, between build_forcing and add_forcing
+ * (main.c:125-130); the deposit is part of the fused step */
+void add_particle_feedbacks(void)
+{
+    need_particles("add_particle_feedbacks");
+    order("add_particle_feedbacks", ST_FORCING, ST_FORCING, ST_FORCING);
+    if (!(S.rp.pcfg.flags & CH4G_LAGRANGE_TWOWAY_MOMENTUM)) {
+        fprintf(stderr, "Error message -> add_particle_feedbacks without LAGRANGE_TWOWAY(_MOMENTUM). Exit.\n");
+        exit(-1);
+    }
+}
+
+/* the halo rules are applied where the particle kernel reads the fields */
+void boundary_conditions_hydro(void)
+{
+    need_lattice("boundary_conditions_hydro");
+    order("boundary_conditions_hydro", ST_HYDRO, ST_PARTICLES, ST_PARTICLES);
+    S.moves = 0;
+}
+
+void interpolate_vector_at_particles(vector *f, char which_vector)
+{
+    (void)f, (void)which_vector;
+    need_particles("interpolate_vector_at_particles");
+    order("interpolate_vector_at_particles", ST_PARTICLES, ST_PARTICLES, ST_PARTICLES);
+}
+
+void interpolate_scalar_at_particles(my_double *f, char which_scalar)
+{
+    (void)f, (void)which_scalar;
+    need_particles("interpolate_scalar_at_particles");
+    order("interpolate_scalar_at_particles", ST_PARTICLES, ST_PARTICLES, ST_PARTICLES);
+}
+
+/* one of the time_dt_lagr_factor sub-steps (main.c:169-182) */
+void move_particles(void)
+{
+    need_particles("move_particles");
+    order("move_particles", ST_PARTICLES, ST_PARTICLES, ST_PARTICLES);
+    if (++S.moves > (int)S.rp.pcfg.time_dt_lagr_factor) {
+        fprintf(stderr, "Error message -> more move_particles calls than time_dt_lagr_factor. Exit.\n");
+        exit(-1);
+    }
+}
+
+/* part.h5 / binary particle dumps are not written by this build (see
+ * ch4g_get_particles for the records) */
+void output_particles(void) { need_particles("output_particles"); }
+
+/* dump_particle_averages (lagrange.c:2598-2930): per particle type, FP64
+ * averages of v, v^2, v^4, a, a^2, a^4 (and t, t^2, t^4, dt_t, dt_t^2 with
+ * LB_TEMPERATURE), accumulated in particle order, appended to
+ * particle_averages.dat every time_dump_diagn.                             */
+void dump_particle_averages(void)
+{
+    need_particles("dump_particle_averages");
+    const ch4g_run_params *rp = &S.rp;
+    int every = (int)(rp->time_dump_diagn / rp->time_dt);
+    if (every <= 0 || S.itime % every != 0) return;
+    int64_t n = 0;
+    if (ch4g_particles_count(S.lat, &n)) die("dump_particle_averages");
+    ch4g_particle *a = (ch4g_particle *)malloc(sizeof(ch4g_particle) * (size_t)(n > 0 ? n : 1));
+    if (!a) die("dump_particle_averages: out of memory");
+    if (ch4g_get_particles(S.lat, a, n, &n)) die("dump_particle_averages");
+    const ch4g_particle_config *pc = &rp->pcfg;
+    double types = pc->tau_drag_types;
+    if ((pc->flags & CH4G_LAGRANGE_GRADIENT) && (pc->flags & CH4G_LAGRANGE_ADDEDMASS)) types *= pc->beta_coeff_types;
+    if (pc->flags & CH4G_LAGRANGE_ADD_TRACER) types += 1.0;
+    const int nt = (int)types;
+    enum { NV = 23 };
+    double *acc = (double *)calloc((size_t)nt * NV, sizeof(double));
+    double *cnt = (double *)calloc((size_t)nt, sizeof(double));
+    if (!acc || !cnt) die("dump_particle_averages: out of memory");
+    for (int64_t i = 0; i < n; i++) {
+        const ch4g_particle *p = &a[i];
+        int type = ((int)p->name) % nt;
+        double *o = acc + (size_t)type * NV;
+        cnt[type] += 1.0;
+        o[0] += p->vx;
+        o[1] += p->vy;
+        o[2] += p->vz;
+        o[3] += p->vx * p->vx;
+        o[4] += p->vy * p->vy;
+        o[5] += p->vz * p->vz;
+        o[6] += pow(p->vx, 4.0);
+        o[7] += pow(p->vy, 4.0);
+        o[8] += pow(p->vz, 4.0);
+        o[9] += p->ax;
+        o[10] += p->ay;
+        o[11] += p->az;
+        o[12] += p->ax * p->ax;
+        o[13] += p->ay * p->ay;
+        o[14] += p->az * p->az;
+        o[15] += pow(p->ax, 4.0);
+        o[16] += pow(p->ay, 4.0);
+        o[17] += pow(p->az, 4.0);
+        o[18] += p->t;
+        o[19] += p->t * p->t;
+        o[20] += pow(p->t, 4.0);
+        o[21] += p->dt_t;
+        o[22] += p->dt_t * p->dt_t;
+    }
+    FILE *fo = fopen("particle_averages.dat", "a");
+    if (fo) {
+        double time_now = rp->time_start + S.itime * rp->time_dt;
+        for (int type = 0; type < nt; type++) {
+            double norm = 1.0 / cnt[type];
+            double *o = acc + (size_t)type * NV;
+            fprintf(fo, "%e %e ", time_now, (double)type);
+            for (int k = 0; k < 18; k++) fprintf(fo, "%e ", o[k] * norm);
+            if (rp->cfg.flags & CH4G_LB_TEMPERATURE)
+                for (int k = 18; k < 23; k++) fprintf(fo, "%e ", o[k] * norm);
+            fprintf(fo, "\n");
+        }
+        fclose(fo);
+    }
+    free(acc);
+    free(cnt);
+    free(a);
 }
 
 /* measure_time (parallel.c:618-638): run_time.dat of the time loop. */

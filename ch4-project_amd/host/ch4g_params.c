@@ -44,6 +44,27 @@ static const struct { const char *name; unsigned flag; } FLAGS[] = {
     {"LB_INITIAL_BAROMETRIC", CH4G_INITIAL_BAROMETRIC},
 };
 
+/* particle switches (define.h:316-382) onto ch4g_lagrange_flag; umbrella
+ * switches (LAGRANGE_SMALLTAUD, LAGRANGE_TWOWAY) are resolved below      */
+static const struct { const char *name; unsigned flag; } LFLAGS[] = {
+    {"LAGRANGE_ADD_TRACER", CH4G_LAGRANGE_ADD_TRACER},
+    {"LAGRANGE_WRAP", CH4G_LAGRANGE_WRAP},
+    {"LAGRANGE_GRADIENT", CH4G_LAGRANGE_GRADIENT},
+    {"LAGRANGE_ADDEDMASS", CH4G_LAGRANGE_ADDEDMASS},
+    {"LAGRANGE_GRAVITY", CH4G_LAGRANGE_GRAVITY},
+    {"LAGRANGE_BC_INELASTIC", CH4G_LAGRANGE_BC_INELASTIC},
+    {"LAGRANGE_BC_INELASTIC_REINJECT", CH4G_LAGRANGE_BC_INELASTIC_REINJECT},
+    {"LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM", CH4G_LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM},
+    {"LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO", CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO},
+    {"LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF", CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF},
+    {"LAGRANGE_SMALLTAUD_BETA", CH4G_LAGRANGE_SMALLTAUD_BETA},
+    {"LAGRANGE_SMALLTAUD_FLOATER", CH4G_LAGRANGE_SMALLTAUD_FLOATER},
+    {"LAGRANGE_TWOWAY_MOMENTUM", CH4G_LAGRANGE_TWOWAY_MOMENTUM},
+    {"LAGRANGE_TWOWAY_SMALLTAUD", CH4G_LAGRANGE_TWOWAY_SMALLTAUD},
+    {"LAGRANGE_TAUDRAG_INCREMENT_LOG", CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG},
+    {"LAGRANGE_ADDEDMASS_INCREMENT_LOG", CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG},
+};
+
 /* switches with no effect on the streaming step of these configurations
  * (build, I/O, timing, umbrella names, choices implied by METHOD_STREAMING) */
 static const char *NEUTRAL[] = {
@@ -52,6 +73,10 @@ static const char *NEUTRAL[] = {
     "METHOD_EDGES_AND_CORNERS", "METHOD_FORCING_GUO", "METHOD_STEPPING_EULER", "METHOD_MYQUICK",
     "METHOD_MYQUICK_CARTESIAN", "SYSTEM_RANDOM48", "PRECISION_DOUBLE", "OUTPUT_AT_START",
     "METHOD_STREAMING", "GRID_POP_D3Q19", "NO_XMLHEADERS",
+    /* particles: umbrella switches, debug output, and the two-node halo the
+       particle fields always use here */
+    "LAGRANGE", "LAGRANGE_SMALLTAUD", "LAGRANGE_TWOWAY", "LAGRANGE_GRADIENT_BRD2", "LAGRANGE_DEBUG",
+    "LAGRANGE_OUTPUT_DEBUG",
 };
 
 static int in_list(const char *n, const char *const *l, size_t k)
@@ -167,7 +192,7 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     if (nd < 0) return -1;
     memset(out, 0, sizeof *out);
     ch4g_config *c = &out->cfg;
-    int streaming = 0, d3q19 = 0;
+    int streaming = 0, d3q19 = 0, forcing_umbrella = 0, smalltaud = 0, twoway = 0;
     for (int i = 0; i < nd; i++) {
         const char *nm = names[i];
         int known = 0;
@@ -176,6 +201,15 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
                 c->flags |= FLAGS[k].flag;
                 known = 1;
             }
+        for (size_t k = 0; k < sizeof LFLAGS / sizeof LFLAGS[0]; k++)
+            if (!strcmp(nm, LFLAGS[k].name)) {
+                out->pcfg.flags |= LFLAGS[k].flag;
+                known = 1;
+            }
+        if (!strcmp(nm, "LAGRANGE")) out->lagrange = 1;
+        if (!strcmp(nm, "LAGRANGE_SMALLTAUD")) smalltaud = 1;
+        if (!strcmp(nm, "LAGRANGE_TWOWAY")) twoway = 1;
+        if (!strcmp(nm, "LB_FLUID_FORCING")) forcing_umbrella = 1;
         if (!strcmp(nm, "METHOD_STREAMING")) streaming = 1;
         if (!strcmp(nm, "GRID_POP_D3Q19")) d3q19 = 1;
         if (!strcmp(nm, "OUTPUT_AT_START")) out->output_at_start = 1;
@@ -186,6 +220,13 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     }
     if (!streaming) return perr("%s", "METHOD_STREAMING is required");
     if (!d3q19) return perr("%s", "GRID_POP_D3Q19 is required");
+    /* the small-tau variants act inside #ifdef LAGRANGE_SMALLTAUD
+       (lagrange.c:3424), the feedback inside #ifdef LAGRANGE_TWOWAY of
+       lagrange_feedback.c and only when build_forcing runs (main.c:118-131) */
+    if (!smalltaud) out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_SMALLTAUD_BETA | CH4G_LAGRANGE_SMALLTAUD_FLOATER);
+    if (!twoway || !forcing_umbrella)
+        out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_TWOWAY_MOMENTUM | CH4G_LAGRANGE_TWOWAY_SMALLTAUD);
+    if (!out->lagrange && out->pcfg.flags) return perr("%s", "LAGRANGE_* switches without LAGRANGE");
 
     static kv t[1024];
     int np = read_param_file(param_in, t, 1024);
@@ -225,6 +266,23 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         }
         if (c->flags & CH4G_TEMPERATURE_INITIAL_ADD_NOISE)
             if (need(t, np, "noise_t", &c->noise_t)) return -1;
+    }
+    if (out->lagrange) {
+        ch4g_particle_config *pc = &out->pcfg;
+        double v2;
+        /* parameters.c:155-167, :524-584 */
+        if (need(t, np, "time_dump_lagr", &out->time_dump_lagr) ||
+            need(t, np, "time_dt_lagr_factor", &pc->time_dt_lagr_factor) || need(t, np, "particle_number", &v2))
+            return -1;
+        pc->particle_number = (int32_t)v2;
+        if (need(t, np, "tau_drag_types", &pc->tau_drag_types) || need(t, np, "tau_drag_min", &pc->tau_drag_min) ||
+            need(t, np, "tau_drag_max", &pc->tau_drag_max))
+            return -1;
+        if ((pc->flags & CH4G_LAGRANGE_GRADIENT) && (pc->flags & CH4G_LAGRANGE_ADDEDMASS)) {
+            if (need(t, np, "beta_coeff_types", &pc->beta_coeff_types) ||
+                need(t, np, "beta_coeff_min", &pc->beta_coeff_min) || need(t, np, "beta_coeff_max", &pc->beta_coeff_max))
+                return -1;
+        }
     }
     /* keys of this implementation (optional) */
     c->precision = CH4G_FP64;

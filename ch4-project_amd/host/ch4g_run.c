@@ -1,7 +1,8 @@
 /*
  * ch4g_run — the reference's main program (src/main.c) for the streaming
  * lattice-Boltzmann step, on the device lattice through the drop-in
- * functions of include/ch4g_dropin.h.  Same call order as main.c:7-221.
+ * functions of include/ch4g_dropin.h.  Same call order as main.c:7-221,
+ * particles included.
  *
  *   ch4g_run [define.h] [param.in]      (defaults: ./define.h ./param.in)
  */
@@ -26,14 +27,24 @@ int main(int argc, char **argv)
                                          CH4G_FLUID_FORCING_HIT | CH4G_TEMPERATURE_BUOYANCY)) != 0;
     pop *p = NULL, *rhs_p = NULL, *g = NULL, *rhs_g = NULL, *old = NULL, *eq = NULL; /* state lives on the device */
 
+    const int lagr = rp.lagrange;
+    const int twoway = (rp.pcfg.flags & CH4G_LAGRANGE_TWOWAY_MOMENTUM) != 0;
+    if (lagr) allocate_particles();
     initial_conditions(0);
     initialization_forcing();
     /* the first hydro_fields / build_forcing / hydro_fields of main.c:26-48
        are part of initial_conditions on the device lattice */
+    if (lagr) initial_conditions_particles(0);
 
     int itime = 0;
     double time_now;
-    if (rp.output_at_start) dump_averages();
+    if (rp.output_at_start) {
+        dump_averages();
+        if (lagr) {
+            output_particles();
+            dump_particle_averages();
+        }
+    }
     for (time_now = rp.time_start + rp.time_dt; time_now <= rp.time_end; time_now += rp.time_dt) {
         itime++;
         if (itime % 10 == 0) fprintf(stderr, "time step %d\n", itime);
@@ -45,8 +56,9 @@ int main(int argc, char **argv)
             compute_advection(g, rhs_g, rp.cfg.tau_t, eq, 'g');
             add_collision(g, rhs_g, rp.cfg.tau_t, eq, 'g');
         }
-        if (forcing) {
+        if (forcing || twoway) {
             build_forcing();
+            if (twoway && itime > 1) add_particle_feedbacks();
             add_forcing();
         }
         boundary_and_pbc_conditions_for_streaming();
@@ -55,6 +67,16 @@ int main(int argc, char **argv)
         if (fluid) hydro_fields('p');
         if (thermal) hydro_fields('g');
         dump_averages();
+        if (lagr) {
+            boundary_conditions_hydro();
+            for (int s = 0; s < (int)rp.pcfg.time_dt_lagr_factor; s++) {
+                interpolate_vector_at_particles(NULL, 'u');
+                if (thermal) interpolate_scalar_at_particles(NULL, 't');
+                move_particles();
+            }
+            output_particles();
+            dump_particle_averages();
+        }
     }
     measure_time();
     ch4g_dropin_finalize();

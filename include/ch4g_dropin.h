@@ -35,6 +35,7 @@ extern "C" {
 
 typedef double my_double;                    /* src/typedef.h:7-14 default */
 typedef struct { my_double p[CH4G_NPOP]; } pop; /* src/typedef.h:66-68, D3Q19 */
+typedef struct { my_double x, y, z; } vector;   /* src/typedef.h vector */
 
 /* Run description: the lattice configuration plus the time-loop and output
  * parameters of the reference's `property` (src/typedef.h:75-224).        */
@@ -42,13 +43,17 @@ typedef struct ch4g_run_params {
     ch4g_config cfg;
     double time_dt, time_start, time_end, time_dump_field, time_dump_diagn;
     int output_at_start; /* define.h OUTPUT_AT_START */
+    int lagrange;        /* define.h LAGRANGE */
+    ch4g_particle_config pcfg;
+    double time_dump_lagr;
 } ch4g_run_params;
 
 /* Read a reference-style define.h (active #define names) and param.in
  * (name value pairs, first match wins; parameters.c:3-39, :41-800).
  * Returns 0, or -1 after printing the reason on stderr as the reference does.
  * Optional keys of this implementation: precision (0 FP64, 1 FP32-deviation,
- * 2 float storage with double arithmetic), les_cs, seed.                  */
+ * 2 float storage with double arithmetic), les_cs, seed.  With LAGRANGE the
+ * particle keys of parameters.c:155-167, :524-584 are read as well.        */
 int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_params *out);
 
 /* allocate_fields (parameters.c:917) + design_lb (lb.c:3) on the device. */
@@ -73,6 +78,20 @@ void time_stepping(pop *f, pop *rhs_f, pop *old_rhs_f, pop *old_old_rhs_f, my_do
 void hydro_fields(char which_pop);                                                 /* lb.c:420 */
 void dump_averages(void);                                                          /* output.c:63 */
 void measure_time(void);                                                           /* parallel.c:618 */
+
+/* Lagrangian particles (src/functions.h).  The particles advance inside the
+ * fused step launched by time_stepping (feedback before the collision,
+ * interpolation + move after the streaming, in main.c's order); these calls
+ * check the call order and do the host-side work the reference does there. */
+void allocate_particles(void);                                          /* lagrange.c:6 */
+void initial_conditions_particles(int restart);                         /* lagrange.c:72 */
+void add_particle_feedbacks(void);                                      /* lagrange_feedback.c:11 */
+void boundary_conditions_hydro(void);                                   /* boundary_conditions.c:2786 */
+void interpolate_vector_at_particles(vector *f, char which_vector);      /* lagrange.c:1211 */
+void interpolate_scalar_at_particles(my_double *f, char which_scalar);  /* lagrange.c:1442 */
+void move_particles(void);                                              /* lagrange.c:2934 */
+void output_particles(void);                                            /* lagrange.c:1598 */
+void dump_particle_averages(void);                                      /* lagrange.c:2598 */
 
 #ifdef __cplusplus
 }

@@ -13,18 +13,23 @@ from ch4g.lattice import PKG_ROOT
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIX = os.path.join(HERE, "golden", "poiseuille_4x64x4")
+FIXP = os.path.join(HERE, "golden", "rb_particles_32x16")
 DROPIN = os.path.join(PKG_ROOT, "lib", "libch4g_dropin.so")
 RUN = os.path.join(PKG_ROOT, "bin", "ch4g_run")
 
 REFERENCE_NAMES = ["initial_conditions", "initialization_forcing", "compute_advection", "add_collision",
                    "build_forcing", "add_forcing", "boundary_and_pbc_conditions_for_streaming", "time_stepping",
-                   "hydro_fields", "dump_averages", "measure_time"]
+                   "hydro_fields", "dump_averages", "measure_time", "allocate_particles",
+                   "initial_conditions_particles", "add_particle_feedbacks", "boundary_conditions_hydro",
+                   "interpolate_vector_at_particles", "interpolate_scalar_at_particles", "move_particles",
+                   "output_particles", "dump_particle_averages"]
 
 
 class RunParams(C.Structure):
     _fields_ = [("cfg", abi.Config), ("time_dt", C.c_double), ("time_start", C.c_double),
                 ("time_end", C.c_double), ("time_dump_field", C.c_double), ("time_dump_diagn", C.c_double),
-                ("output_at_start", C.c_int)]
+                ("output_at_start", C.c_int), ("lagrange", C.c_int), ("pcfg", abi.ParticleConfig),
+                ("time_dump_lagr", C.c_double)]
 
 
 def test_dropin_exports_reference_names():
@@ -44,6 +49,31 @@ def test_read_run_params_poiseuille():
     assert c.flags == abi.LB_FLUID | abi.FLUID_FORCING_POISEUILLE | abi.FLUID_BC_Y
     assert c.tau_u == 1.0 and c.Amp_x == 0.003072 and c.seed == 1
     assert rp.time_end == 1000.0 and rp.time_dump_diagn == 250.0 and rp.output_at_start == 1
+
+
+def test_read_run_params_particles():
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    assert lib.ch4g_read_run_params(os.path.join(FIXP, "define.h").encode(),
+                                    os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == 0
+    assert rp.lagrange == 1
+    pc = rp.pcfg
+    want = (abi.LAGRANGE_WRAP | abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS |
+            abi.LAGRANGE_GRAVITY | abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT |
+            abi.LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF | abi.LAGRANGE_TWOWAY_MOMENTUM)
+    assert pc.flags == want
+    assert (pc.particle_number, pc.tau_drag_types, pc.tau_drag_max, pc.beta_coeff_min) == (96, 2.0, 12.0, 0.3)
+    assert pc.time_dt_lagr_factor == 1.0 and rp.time_dump_lagr == 40.0
+    # without the LAGRANGE_TWOWAY umbrella the feedback switch has no effect
+    text = open(os.path.join(FIXP, "define.h")).read().replace("#define LAGRANGE_TWOWAY\n", "")
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        d = os.path.join(td, "define.h")
+        open(d, "w").write(text)
+        assert lib.ch4g_read_run_params(d.encode(), os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == 0
+        assert not rp.pcfg.flags & abi.LAGRANGE_TWOWAY_MOMENTUM
+        open(d, "w").write(text + "#define LAGRANGE_ORIENTATION\n")
+        assert lib.ch4g_read_run_params(d.encode(), os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == -1
 
 
 def test_read_run_params_rejects_unimplemented_switch(tmp_path):
@@ -87,3 +117,68 @@ def test_ch4g_run_matches_oracle(tmp_path):
         assert t == 250 * k
         np.testing.assert_allclose(rows[k, 3], a["ux"], rtol=1e-6)  # %e output: 7 significant digits
         np.testing.assert_allclose(rows[k, 1], a["ene"], rtol=1e-6, atol=1e-30)
+
+
+def particle_type_averages(a, ntypes, thermal):
+    """dump_particle_averages (lagrange.c:2598-2930) restated: per type, in
+    particle order, sequential FP64 sums of v, v^2, v^4, a, a^2, a^4 (and t,
+    t^2, t^4, dt_t, dt_t^2), divided by the count."""
+    rows = []
+    for t in range(ntypes):
+        sel = [r for r in a if int(r["name"]) % ntypes == t]
+        cols = []
+        for f in ("vx", "vy", "vz"):
+            cols.append([float(r[f]) for r in sel])
+        for f in ("vx", "vy", "vz"):
+            cols.append([float(r[f]) * float(r[f]) for r in sel])
+        for f in ("vx", "vy", "vz"):
+            cols.append([float(r[f]) ** 4.0 for r in sel])
+        for f in ("ax", "ay", "az"):
+            cols.append([float(r[f]) for r in sel])
+        for f in ("ax", "ay", "az"):
+            cols.append([float(r[f]) * float(r[f]) for r in sel])
+        for f in ("ax", "ay", "az"):
+            cols.append([float(r[f]) ** 4.0 for r in sel])
+        if thermal:
+            cols.append([float(r["t"]) for r in sel])
+            cols.append([float(r["t"]) * float(r["t"]) for r in sel])
+            cols.append([float(r["t"]) ** 4.0 for r in sel])
+            cols.append([float(r["dt_t"]) for r in sel])
+            cols.append([float(r["dt_t"]) * float(r["dt_t"]) for r in sel])
+        row = []
+        for c in cols:
+            acc = 0.0
+            for v in c:
+                acc += v
+            row.append(acc * (1.0 / len(sel)))
+        rows.append(row)
+    return np.array(rows)
+
+
+@pytest.mark.gpu
+def test_ch4g_run_particles_matches_oracle(tmp_path):
+    """ch4g_run with particles (inelastic re-injection, added mass, two-way
+    coupling) writes particle_averages.dat at t = 0, 40, 80, 120 equal (to
+    the printed digits) to the oracle's particles at those steps."""
+    from oracle import Oracle
+    for f in ("define.h", "param.in"):
+        shutil.copy(os.path.join(FIXP, f), tmp_path / f)
+    r = subprocess.run([RUN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rows = np.loadtxt(tmp_path / "particle_averages.dat")
+    ntypes = 2 * 2 + 1
+    assert rows.shape == (4 * ntypes, 2 + 18 + 5)
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    assert lib.ch4g_read_run_params(os.path.join(FIXP, "define.h").encode(),
+                                    os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == 0
+    o = Oracle(rp.cfg)
+    o.initial_conditions()
+    o.particles_init(rp.pcfg)
+    for k in range(4):
+        if k:
+            o.step(40)
+        want = particle_type_averages(o.particles(), ntypes, True)
+        got = rows[k * ntypes:(k + 1) * ntypes]
+        assert np.all(got[:, 0] == 40 * k) and np.array_equal(got[:, 1], np.arange(ntypes))
+        np.testing.assert_allclose(got[:, 2:], want, rtol=2e-6, atol=1e-300)
