@@ -172,4 +172,10 @@ class ServeParams(C.Structure):
         ("chunk_steps", C.c_int32),
         ("max_active", C.c_int32),
         ("use_graphs", C.c_int32),
+        ("schedule", C.c_int32),
+        ("reserved", C.c_int32),
     ]
+
+
+SCHEDULE_PROPORTIONAL = 0
+SCHEDULE_LOCKSTEP = 1

@@ -96,13 +96,14 @@ def step_batch(lattices, nsteps):
     check(lib().ch4g_step_batch(arr, len(lattices), int(nsteps)))
 
 
-def serve(jobs, mode=0, hbm_budget=0, chunk_steps=0, max_active=0, use_graphs=True):
+def serve(jobs, mode=0, hbm_budget=0, chunk_steps=0, max_active=0, use_graphs=True, schedule=0):
     """Run a request stream (list of abi.Job, modified in place with the
     admission/finish times, averages and status); returns the list."""
     arr = (Job * len(jobs))(*jobs)
     sp = ServeParams()
     sp.hbm_budget, sp.mode, sp.chunk_steps, sp.max_active = int(hbm_budget), int(mode), int(chunk_steps), int(max_active)
     sp.use_graphs = 1 if use_graphs else 0
+    sp.schedule = int(schedule)
     check(lib().ch4g_serve(C.byref(sp), arr, len(jobs)))
     return list(arr)
 

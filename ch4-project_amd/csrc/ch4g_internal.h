@@ -184,19 +184,27 @@ struct FeedbackBufs {
 int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s);
 size_t feedback_sort_bytes(int64_t nitems, int key_bits);
 
-// One lattice of a batched step: its constants, its two state buffers and
-// the range of 256-node tiles it owns in the batch's tile space.
+// nodes per tile (one block) of the batched step
+constexpr int BATCH_TILE = 1024;
+
+// One lattice of a batched step: its constants and its two state buffers.
 struct BatchEntry {
     KParams P;
     void *f[2], *g[2];
     void *tlag, *taules;
-    int cur;            // buffer holding the state at parity 0
-    int64_t tile0;      // first tile (entries sorted by tile0)
+    int cur;            // buffer holding the state at the start of the chunk
 };
-void launch_batch_fp32(const BatchEntry *tab, int ninst, int64_t ntiles, int fluid, int thermal, int arith64,
-                       bool ntl, int parity, hipStream_t s);
-void launch_batch_fp64(const BatchEntry *tab, int ninst, int64_t ntiles, int fluid, int thermal, int parity,
+// One block of a batched launch: entry (bit 31: parity of that lattice's
+// step within the chunk) and the lattice's BATCH_TILE-node tile.
+struct BatchBlock {
+    uint32_t entry;
+    uint32_t tile;
+};
+// one launch covers every block of a storage kind (FP32 float/double
+// arithmetic, FP64) listed for it
+void launch_batch_fp32(const BatchEntry *tab, const BatchBlock *blocks, int64_t nblocks, int arith64, bool ntl,
                        hipStream_t s);
+void launch_batch_fp64(const BatchEntry *tab, const BatchBlock *blocks, int64_t nblocks, hipStream_t s);
 
 // Kernel launchers (stream-ordered).
 void launch_step_fp64(const KParams &P, const double *fin, const double *gin, double *fout, double *gout,

@@ -20,24 +20,29 @@ using namespace ch4g;
 
 namespace {
 
-// A set of lattices stepped together, grouped by kernel variant; the device
-// copy of the table is owned by the group set and reused across calls.
+// A set of lattices stepped together.  A chunk is `chunk` launches per
+// storage kind; lattice i takes steps[i] of them, spread evenly over the
+// chunk, so every launch carries a similar share of the work.  Each launch
+// has its own block table (lattice, tile, parity of that lattice's step).
 struct Batch {
     struct Group {
-        int fluid, thermal, dev32, arith64;
+        int dev32, arith64;
         bool ntl;
-        int first, count;   // range in the host table
-        int64_t ntiles;
+        int first, count;                 // range in the entry table
+        std::vector<int64_t> off, cnt;    // per launch: range in the block table
     };
     std::vector<BatchEntry> host;
+    std::vector<BatchBlock> blocks;
     std::vector<Group> groups;
     BatchEntry *dev = nullptr;
-    size_t dev_cap = 0;     // entries
+    size_t dev_cap = 0;       // entries
+    BatchBlock *dev_blocks = nullptr;
+    size_t blocks_cap = 0;    // blocks
     hipStream_t stream = nullptr;
     std::vector<ch4g_lattice *> lats;
+    std::vector<int> steps;   // steps of each lattice per chunk
+    int chunk = 0;            // launches per chunk = max steps
     hipGraphExec_t graph = nullptr;
-    int graph_steps = 0;
-    int graph_parity0 = 0;
     hipEvent_t ev = nullptr;
 
     ~Batch() { release(); }
@@ -45,7 +50,6 @@ struct Batch {
     {
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
-        graph_steps = 0;
     }
     void release()
     {
@@ -53,6 +57,9 @@ struct Batch {
         if (dev) (void)hipFree(dev);
         dev = nullptr;
         dev_cap = 0;
+        if (dev_blocks) (void)hipFree(dev_blocks);
+        dev_blocks = nullptr;
+        blocks_cap = 0;
         if (ev) (void)hipEventDestroy(ev);
         ev = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
@@ -69,36 +76,42 @@ int check_member(const ch4g_lattice *L)
     return 0;
 }
 
-// Table of the lattices, grouped by variant, tiles numbered per group.
-int build(Batch &B, const std::vector<ch4g_lattice *> &lats)
+// Entry and block tables of the lattices, grouped by storage kind.
+int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<int> &steps)
 {
     B.drop_graph();
     B.lats = lats;
+    B.steps = steps;
     B.host.clear();
+    B.blocks.clear();
     B.groups.clear();
-    for (ch4g_lattice *L : lats) {
-        int rc = check_member(L);
+    B.chunk = 0;
+    for (size_t i = 0; i < lats.size(); i++) {
+        int rc = check_member(lats[i]);
         if (rc) return rc;
+        if (steps[i] < 1) return fail(-1, "steps per chunk must be >= 1");
+        B.chunk = std::max(B.chunk, steps[i]);
     }
-    std::vector<ch4g_lattice *> order = lats;
-    auto key = [](const ch4g_lattice *L) {
-        return (L->base.fluid << 3) | (L->base.thermal << 2) | (L->dev32 << 1) | L->base.arith64;
-    };
-    std::stable_sort(order.begin(), order.end(),
-                     [&](const ch4g_lattice *a, const ch4g_lattice *b) { return key(a) < key(b); });
+    const int K = B.chunk;
+    std::vector<int> order(lats.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    auto key = [&](int i) { return (lats[(size_t)i]->dev32 << 1) | lats[(size_t)i]->base.arith64; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
     for (size_t i = 0; i < order.size();) {
         Batch::Group G;
-        const ch4g_lattice *L0 = order[i];
-        G.fluid = L0->base.fluid;
-        G.thermal = L0->base.thermal;
+        const int k0 = key(order[i]);
+        const ch4g_lattice *L0 = lats[(size_t)order[i]];
         G.dev32 = L0->dev32;
         G.arith64 = L0->base.arith64;
         G.first = (int)B.host.size();
         G.count = 0;
-        G.ntiles = 0;
         int64_t bytes = 0;
-        for (; i < order.size() && key(order[i]) == key(L0); i++) {
-            ch4g_lattice *L = order[i];
+        // step index of each member at each launch (-1: not in that launch)
+        std::vector<std::vector<int>> at;
+        std::vector<int64_t> ntile;
+        for (; i < order.size() && key(order[i]) == k0; i++) {
+            ch4g_lattice *L = lats[(size_t)order[i]];
+            const int st = steps[(size_t)order[i]];
             BatchEntry E;
             memset(&E, 0, sizeof E);
             E.P = params_for(L, 0, (int)(L->itime + 1));
@@ -109,11 +122,25 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats)
             E.tlag = L->tlag;
             E.taules = L->taules;
             E.cur = L->cur;
-            E.tile0 = G.ntiles;
-            G.ntiles += (L->N + 255) / 256;
-            bytes += 2 * (int64_t)Q * L->N * 4 * ((L->base.fluid ? 1 : 0) + (L->base.thermal ? 1 : 0));
             B.host.push_back(E);
+            std::vector<int> a((size_t)K, -1);
+            for (int s_ = 0; s_ < st; s_++) a[(size_t)(((int64_t)s_ * K) / st)] = s_;
+            at.push_back(a);
+            ntile.push_back((L->N + BATCH_TILE - 1) / BATCH_TILE);
+            bytes += 2 * (int64_t)Q * L->N * (L->dev32 ? 4 : 8) * ((L->base.fluid ? 1 : 0) + (L->base.thermal ? 1 : 0));
             G.count++;
+        }
+        G.off.assign((size_t)K, 0);
+        G.cnt.assign((size_t)K, 0);
+        for (int k = 0; k < K; k++) {
+            G.off[(size_t)k] = (int64_t)B.blocks.size();
+            for (int e = 0; e < G.count; e++) {
+                const int s_ = at[(size_t)e][(size_t)k];
+                if (s_ < 0) continue;
+                const uint32_t tag = (uint32_t)e | ((uint32_t)(s_ & 1) << 31);
+                for (int64_t t = 0; t < ntile[(size_t)e]; t++) B.blocks.push_back({tag, (uint32_t)t});
+            }
+            G.cnt[(size_t)k] = (int64_t)B.blocks.size() - G.off[(size_t)k];
         }
         // non-temporal loads when the group's lattices fit the last-level cache
         G.ntl = bytes <= ((int64_t)192 << 20);
@@ -121,7 +148,7 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats)
     }
     if (!B.stream) HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
     if (!B.ev) HIPCHK(hipEventCreateWithFlags(&B.ev, hipEventDisableTiming));
-    // the previous table may still be read by queued kernels
+    // the previous tables may still be read by queued kernels
     HIPCHK(hipStreamSynchronize(B.stream));
     if (B.host.size() > B.dev_cap) {
         if (B.dev) (void)hipFree(B.dev);
@@ -130,43 +157,53 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats)
         HIPCHK(hipMalloc((void **)&B.dev, cap * sizeof(BatchEntry)));
         B.dev_cap = cap;
     }
+    if (B.blocks.size() > B.blocks_cap) {
+        if (B.dev_blocks) (void)hipFree(B.dev_blocks);
+        B.dev_blocks = nullptr;
+        const size_t cap = std::max<size_t>(B.blocks.size() + B.blocks.size() / 2, 4096);
+        HIPCHK(hipMalloc((void **)&B.dev_blocks, cap * sizeof(BatchBlock)));
+        B.blocks_cap = cap;
+    }
     if (!B.host.empty()) {
         HIPCHK(hipMemcpyAsync(B.dev, B.host.data(), B.host.size() * sizeof(BatchEntry), hipMemcpyHostToDevice,
                               B.stream));
+        HIPCHK(hipMemcpyAsync(B.dev_blocks, B.blocks.data(), B.blocks.size() * sizeof(BatchBlock),
+                              hipMemcpyHostToDevice, B.stream));
         HIPCHK(hipStreamSynchronize(B.stream));
     }
     return 0;
 }
 
-void launch_steps(Batch &B, int nsteps, int parity0)
+void launch_steps(Batch &B)
 {
-    for (int k = 0; k < nsteps; k++) {
-        const int parity = (parity0 + k) & 1;
+    for (int k = 0; k < B.chunk; k++)
         for (const Batch::Group &G : B.groups) {
+            const int64_t nb = G.cnt[(size_t)k];
+            if (nb == 0) continue;
             const BatchEntry *tab = B.dev + G.first;
+            const BatchBlock *blk = B.dev_blocks + G.off[(size_t)k];
             if (G.dev32)
-                launch_batch_fp32(tab, G.count, G.ntiles, G.fluid, G.thermal, G.arith64, G.ntl, parity, B.stream);
+                launch_batch_fp32(tab, blk, nb, G.arith64, G.ntl, B.stream);
             else
-                launch_batch_fp64(tab, G.count, G.ntiles, G.fluid, G.thermal, parity, B.stream);
+                launch_batch_fp64(tab, blk, nb, B.stream);
         }
-    }
 }
 
-// Advance every lattice of the table by nsteps (ordered after each lattice's
-// own queued work, and before any work queued on it later).
-int advance(Batch &B, int nsteps, int parity0, bool use_graph)
+// Advance every lattice of the table by its steps of one chunk (ordered
+// after each lattice's own queued work, and before any work queued on it
+// later); the launches are replayed from a captured graph when asked.
+int advance(Batch &B, bool use_graph)
 {
-    if (B.lats.empty() || nsteps <= 0) return 0;
+    if (B.lats.empty() || B.chunk <= 0) return 0;
     for (ch4g_lattice *L : B.lats) {
         HIPCHK(hipEventRecord(B.ev, L->stream));
         HIPCHK(hipStreamWaitEvent(B.stream, B.ev, 0));
     }
-    if (use_graph && nsteps % 2 == 0) {
-        if (!B.graph || B.graph_steps != nsteps || B.graph_parity0 != parity0) {
-            B.drop_graph();
+    if (use_graph) {
+        if (!B.graph) {
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-            launch_steps(B, nsteps, parity0);
+            launch_steps(B);
             hipError_t e = hipStreamEndCapture(B.stream, &g);
             if (e != hipSuccess) return fail(-2, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
             e = hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0);
@@ -175,12 +212,10 @@ int advance(Batch &B, int nsteps, int parity0, bool use_graph)
                 B.graph = nullptr;
                 return fail(-2, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
             }
-            B.graph_steps = nsteps;
-            B.graph_parity0 = parity0;
         }
         HIPCHK(hipGraphLaunch(B.graph, B.stream));
     } else {
-        launch_steps(B, nsteps, parity0);
+        launch_steps(B);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(B.ev, B.stream));
@@ -188,12 +223,14 @@ int advance(Batch &B, int nsteps, int parity0, bool use_graph)
     return 0;
 }
 
-// The lattices' own state after nsteps batched steps.
-void commit(Batch &B, int nsteps)
+// The lattices' own state after a chunk.
+void commit(Batch &B)
 {
-    for (ch4g_lattice *L : B.lats) {
-        if (nsteps & 1) L->cur ^= 1;
-        L->itime += nsteps;
+    for (size_t i = 0; i < B.lats.size(); i++) {
+        ch4g_lattice *L = B.lats[i];
+        const int st = B.steps[i];
+        if (st & 1) L->cur ^= 1;
+        L->itime += st;
     }
 }
 
@@ -223,13 +260,12 @@ int ch4g_step_batch(ch4g_lattice *const *lats, int n, int nsteps)
         for (int j = 0; j < i; j++)
             if (v[i] == v[j]) return fail(-1, "a lattice appears twice in the batch");
     Batch B;
-    int rc = build(B, v);
+    int rc = build(B, v, std::vector<int>((size_t)n, nsteps));
     if (rc) return rc;
-    // parity 0 = each lattice's current buffer (E.cur)
-    rc = advance(B, nsteps, 0, false);
+    rc = advance(B, false);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(B.stream));
-    commit(B, nsteps);
+    commit(B);
     return 0;
 }
 
@@ -263,6 +299,7 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
     std::vector<Active> active;
     Batch B;
     bool table_dirty = true;
+    int chunks_since_share = 0;
     int64_t used = 0;
     size_t next = 0;
     int done = 0;
@@ -328,28 +365,48 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
             if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
             continue;
         }
-        // ---- one chunk for every active lattice (no job overshoots); the
-        // plain mode runs its single job to the end in one call
-        int64_t m = batched ? chunk : active[0].remaining;
-        for (const Active &A : active) m = std::min(m, A.remaining);
+        // ---- one chunk.  Batched: every resident job takes part, with all
+        // the chunk's steps (lock-step) or a share proportional to its
+        // remaining steps, spread over the chunk's launches (even shares keep
+        // each lattice's buffer parity, so the tables and their captured
+        // graph stay valid until the set changes; a job that can finish
+        // does); shares are refreshed every 16 chunks.  Plain: the single
+        // job runs to its end in one call.
         if (batched) {
-            if (table_dirty) {
+            if (table_dirty || ++chunks_since_share >= 16) {
+                int64_t rmax = 1;
+                for (const Active &A : active) rmax = std::max(rmax, A.remaining);
                 std::vector<ch4g_lattice *> v;
-                for (const Active &A : active) v.push_back(A.L);
-                if ((rc = build(B, v))) break;
+                std::vector<int> st;
+                for (const Active &A : active) {
+                    int64_t s_ = chunk;
+                    if (sp->schedule == CH4G_SCHEDULE_PROPORTIONAL) {
+                        s_ = (chunk * A.remaining + rmax - 1) / rmax;
+                        if (s_ & 1) s_ += 1;
+                    }
+                    s_ = std::max<int64_t>(1, std::min<int64_t>(s_, A.remaining));
+                    v.push_back(A.L);
+                    st.push_back((int)s_);
+                }
+                if ((rc = build(B, v, st))) break;
                 table_dirty = false;
+                chunks_since_share = 0;
             }
-            if ((rc = advance(B, (int)m, 0, sp->use_graphs != 0))) break;
-            commit(B, (int)m);
-            // the table holds each lattice's current buffer: after an odd
-            // chunk it is rebuilt
-            table_dirty = (m & 1) != 0;
+            if ((rc = advance(B, sp->use_graphs != 0))) break;
+            commit(B);
             HIPCHK(hipStreamSynchronize(B.stream));
+            for (size_t a = 0; a < active.size(); a++) {
+                active[a].remaining -= B.steps[a];
+                // an odd share flips the lattice's buffer: the table follows
+                if (B.steps[a] & 1) table_dirty = true;
+                if (active[a].remaining < B.steps[a]) table_dirty = true;
+            }
         } else {
+            const int64_t m = active[0].remaining;
             if ((rc = ch4g_step(active[0].L, (int)m))) break;
             if ((rc = ch4g_synchronize(active[0].L))) break;
+            active[0].remaining -= m;
         }
-        for (Active &A : active) A.remaining -= m;
         for (size_t a = 0; a < active.size();) {
             if (active[a].remaining == 0)
                 retire(a);

@@ -354,32 +354,39 @@ __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__res
     step_node32<FIRST, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
 }
 
-// Many lattices in one launch: block b advances 256 consecutive nodes of the
-// lattice whose tile range holds b (table sorted by first tile, found by a
-// block-uniform binary search); the state buffer of each lattice alternates
+// Many lattices in one launch: block b advances BATCH_TILE consecutive nodes
+// (256 at a time, the lattice's constants loaded once per block) of the
+// lattice and tile named by the launch's block table, with that lattice's
+// step parity (tile -> lattice map), whatever its population
+// sets (a block-uniform dispatch), so small and large lattices of every kind
+// share one launch per step; the state buffer of each lattice alternates
 // with the step parity.
-template <bool FLUID, bool THERMAL, typename A, bool NTL>
-__global__ void __launch_bounds__(256) k_step_batch32(const BatchEntry *__restrict__ tab, int ninst, int parity)
+template <typename A, bool NTL>
+__global__ void __launch_bounds__(256) k_step_batch32(const BatchEntry *__restrict__ tab,
+                                                      const BatchBlock *__restrict__ blocks)
 {
-    int lo = 0, hi = ninst - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (tab[mid].tile0 <= (int64_t)blockIdx.x)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const BatchEntry &E = tab[lo];
+    const BatchBlock bb = blocks[blockIdx.x];
+    const BatchEntry &E = tab[bb.entry & 0x7fffffffu];
     const KParams P = E.P;
-    const int64_t m = ((int64_t)blockIdx.x - E.tile0) * 256 + threadIdx.x;
-    if (m >= P.N) return;
-    const uint32_t n = (uint32_t)m;
-    const uint32_t x = n % (uint32_t)P.NX, r = n / (uint32_t)P.NX;
-    const uint32_t y = r % (uint32_t)P.NY, z = r / (uint32_t)P.NY;
-    const int in = E.cur ^ parity;
-    step_node32<false, FLUID, THERMAL, A, NTL>(P, (const float *)E.f[in], (const float *)E.g[in],
-                                               (float *)E.f[in ^ 1], (float *)E.g[in ^ 1], (float *)E.tlag,
-                                               (float *)E.taules, (int)x, (int)y, (int)z, n);
+    const int in = E.cur ^ (int)(bb.entry >> 31);
+    const float *fin = (const float *)E.f[in], *gin = (const float *)E.g[in];
+    float *fout = (float *)E.f[in ^ 1], *gout = (float *)E.g[in ^ 1];
+    float *tl = (float *)E.tlag, *tau = (float *)E.taules;
+    const int64_t base = (int64_t)bb.tile * BATCH_TILE;
+#pragma unroll 1
+    for (int it = 0; it < BATCH_TILE / 256; it++) {
+        const int64_t m = base + it * 256 + threadIdx.x;
+        if (m >= P.N) return;
+        const uint32_t n = (uint32_t)m;
+        const uint32_t x = n % (uint32_t)P.NX, r = n / (uint32_t)P.NX;
+        const uint32_t y = r % (uint32_t)P.NY, z = r / (uint32_t)P.NY;
+        if (P.fluid && P.thermal)
+            step_node32<false, true, true, A, NTL>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
+        else if (P.fluid)
+            step_node32<false, true, false, A, NTL>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
+        else
+            step_node32<false, false, true, A, NTL>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__restrict__ fin,
@@ -489,32 +496,21 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 #undef CH4G_LAUNCH32_
 }
 
-void launch_batch_fp32(const BatchEntry *tab, int ninst, int64_t ntiles, int fluid, int thermal, int arith64,
-                       bool ntl, int parity, hipStream_t s)
+void launch_batch_fp32(const BatchEntry *tab, const BatchBlock *blocks, int64_t nblocks, int arith64, bool ntl,
+                       hipStream_t s)
 {
-    dim3 grid((unsigned)ntiles), block(256);
-#define CH4G_BATCH32_(FL_, TH_, A_)                                                                                     \
-    do {                                                                                                              \
-        if (ntl)                                                                                                      \
-            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, true>), grid, block, 0, s, tab, ninst, parity);          \
-        else                                                                                                          \
-            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, false>), grid, block, 0, s, tab, ninst, parity);         \
-    } while (0)
-#define CH4G_BATCH32(FL_, TH_)                                                                                        \
-    do {                                                                                                              \
-        if (arith64)                                                                                                  \
-            CH4G_BATCH32_(FL_, TH_, double);                                                                          \
-        else                                                                                                          \
-            CH4G_BATCH32_(FL_, TH_, float);                                                                           \
-    } while (0)
-    if (fluid && thermal)
-        CH4G_BATCH32(true, true);
-    else if (fluid)
-        CH4G_BATCH32(true, false);
-    else
-        CH4G_BATCH32(false, true);
-#undef CH4G_BATCH32
-#undef CH4G_BATCH32_
+    dim3 grid((unsigned)nblocks), block(256);
+    if (arith64) {
+        if (ntl)
+            hipLaunchKernelGGL((k_step_batch32<double, true>), grid, block, 0, s, tab, blocks);
+        else
+            hipLaunchKernelGGL((k_step_batch32<double, false>), grid, block, 0, s, tab, blocks);
+    } else {
+        if (ntl)
+            hipLaunchKernelGGL((k_step_batch32<float, true>), grid, block, 0, s, tab, blocks);
+        else
+            hipLaunchKernelGGL((k_step_batch32<float, false>), grid, block, 0, s, tab, blocks);
+    }
 }
 
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,

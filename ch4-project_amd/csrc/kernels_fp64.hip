@@ -245,28 +245,31 @@ __global__ void __launch_bounds__(256) k_step_fp64(KParams P, const double *__re
 }
 
 // Many lattices in one launch (see k_step_batch32).
-template <bool FLUID, bool THERMAL>
-__global__ void __launch_bounds__(256) k_step_batch64(const BatchEntry *__restrict__ tab, int ninst, int parity)
+__global__ void __launch_bounds__(256) k_step_batch64(const BatchEntry *__restrict__ tab,
+                                                      const BatchBlock *__restrict__ blocks)
 {
-    int lo = 0, hi = ninst - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (tab[mid].tile0 <= (int64_t)blockIdx.x)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    const BatchEntry &E = tab[lo];
+    const BatchBlock bb = blocks[blockIdx.x];
+    const BatchEntry &E = tab[bb.entry & 0x7fffffffu];
     const KParams P = E.P;
-    const int64_t m = ((int64_t)blockIdx.x - E.tile0) * 256 + threadIdx.x;
-    if (m >= P.N) return;
-    const uint32_t n = (uint32_t)m;
-    const uint32_t x = n % (uint32_t)P.NX, r = n / (uint32_t)P.NX;
-    const uint32_t y = r % (uint32_t)P.NY, z = r / (uint32_t)P.NY;
-    const int in = E.cur ^ parity;
-    step_node64<false, FLUID, THERMAL>(P, (const double *)E.f[in], (const double *)E.g[in], (double *)E.f[in ^ 1],
-                                       (double *)E.g[in ^ 1], (double *)E.tlag, (double *)E.taules, (int)x, (int)y,
-                                       (int)z, n);
+    const int in = E.cur ^ (int)(bb.entry >> 31);
+    const double *fin = (const double *)E.f[in], *gin = (const double *)E.g[in];
+    double *fout = (double *)E.f[in ^ 1], *gout = (double *)E.g[in ^ 1];
+    double *tl = (double *)E.tlag, *tau = (double *)E.taules;
+    const int64_t base = (int64_t)bb.tile * BATCH_TILE;
+#pragma unroll 1
+    for (int it = 0; it < BATCH_TILE / 256; it++) {
+        const int64_t m = base + it * 256 + threadIdx.x;
+        if (m >= P.N) return;
+        const uint32_t n = (uint32_t)m;
+        const uint32_t x = n % (uint32_t)P.NX, r = n / (uint32_t)P.NX;
+        const uint32_t y = r % (uint32_t)P.NY, z = r / (uint32_t)P.NY;
+        if (P.fluid && P.thermal)
+            step_node64<false, true, true>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
+        else if (P.fluid)
+            step_node64<false, true, false>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
+        else
+            step_node64<false, false, true>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
+    }
 }
 
 // Streaming of the last step + hydro_fields, without a new collision:
@@ -346,16 +349,9 @@ void launch_step_fp64(const KParams &P, const double *fin, const double *gin, do
 #undef CH4G_LAUNCH64
 }
 
-void launch_batch_fp64(const BatchEntry *tab, int ninst, int64_t ntiles, int fluid, int thermal, int parity,
-                       hipStream_t s)
+void launch_batch_fp64(const BatchEntry *tab, const BatchBlock *blocks, int64_t nblocks, hipStream_t s)
 {
-    dim3 grid((unsigned)ntiles), block(256);
-    if (fluid && thermal)
-        hipLaunchKernelGGL((k_step_batch64<true, true>), grid, block, 0, s, tab, ninst, parity);
-    else if (fluid)
-        hipLaunchKernelGGL((k_step_batch64<true, false>), grid, block, 0, s, tab, ninst, parity);
-    else
-        hipLaunchKernelGGL((k_step_batch64<false, true>), grid, block, 0, s, tab, ninst, parity);
+    hipLaunchKernelGGL(k_step_batch64, dim3((unsigned)nblocks), dim3(256), 0, s, tab, blocks);
 }
 
 void launch_gather_fp64(const KParams &P, const double *fin, const double *gin, const double *tlag, double *fpost,

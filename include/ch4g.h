@@ -280,12 +280,19 @@ enum ch4g_serve_mode {
     CH4G_SERVE_SEQUENTIAL = 1  /* one job at a time through ch4g_step (the plain single-job path) */
 };
 
+enum ch4g_schedule {
+    CH4G_SCHEDULE_PROPORTIONAL = 0, /* each job's steps per chunk proportional to its remaining steps */
+    CH4G_SCHEDULE_LOCKSTEP     = 1  /* every job takes every step of the chunk */
+};
+
 typedef struct ch4g_serve_params {
     int64_t hbm_budget;   /* device bytes the jobs may hold at once (0: 90% of free memory) */
     int32_t mode;         /* enum ch4g_serve_mode */
-    int32_t chunk_steps;  /* steps between admission points (0: 32); replayed from a captured graph */
+    int32_t chunk_steps;  /* launches between admission points (0: 32); replayed from a captured graph */
     int32_t max_active;   /* lattices resident at once (0: no limit but the budget) */
     int32_t use_graphs;   /* capture the chunk's launches in a hipGraph and replay it */
+    int32_t schedule;     /* enum ch4g_schedule */
+    int32_t reserved;
 } ch4g_serve_params;
 
 /* Run a request stream: jobs are released at arrival_s (wall clock from the
