@@ -148,6 +148,10 @@ def particle_dtype():
     return np.dtype([(n, np.float64) for n in PARTICLE_FIELDS])
 
 
+# steady-state launch mode flags of ch4g_set_graphs
+LAUNCH_GRAPHS = 1
+LAUNCH_COOP = 2
+
 SERVE_BATCHED = 0
 SERVE_SEQUENTIAL = 1
 

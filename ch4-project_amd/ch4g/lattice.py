@@ -9,6 +9,7 @@ import os
 
 import numpy as np
 
+from . import abi
 from .abi import NPOP, Config, Job, Output, ParticleConfig, ServeParams, particle_dtype
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -165,8 +166,12 @@ class Lattice:
         check(self._L.ch4g_step_timed(self._h, int(n), C.byref(ms)))
         return ms.value
 
-    def set_graphs(self, enable):
-        check(self._L.ch4g_set_graphs(self._h, 1 if enable else 0))
+    def set_graphs(self, enable, coop=None):
+        """Launch mode of steady-state steps: graph replay when enable, and
+        cooperative multi-step launches when coop (default: same as enable)."""
+        coop = enable if coop is None else coop
+        flags = (abi.LAUNCH_GRAPHS if enable else 0) | (abi.LAUNCH_COOP if coop else 0)
+        check(self._L.ch4g_set_graphs(self._h, flags))
 
     def synchronize(self):
         check(self._L.ch4g_synchronize(self._h))

@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -656,10 +657,33 @@ int ch4g_step(ch4g_lattice *L, int nsteps)
     if (nsteps < 0) return fail(-1, "negative step count");
     int s = 0;
     while (s < nsteps) {
-        // Steady-state steps (not the first, not the LES start-up step, no
-        // per-step host forcing tables) depend only on the buffer parity:
+        // Steady-state steps (not the first, no per-step host forcing tables,
+        // no particles) of a lattice whose blocks all fit the GPU at once run
+        // COOP_STEPS at a time in one cooperative launch.
+        if ((L->use_graphs & CH4G_LAUNCH_COOP) && L->coop_ok && !L->first && L->itime >= 1 && !L->base.hit && !L->part &&
+            nsteps - s >= 2) {
+            const int m = std::min(COOP_STEPS, nsteps - s);
+            KParams P = params_for(L, 0, (int)L->itime + 1);
+            int rc;
+            if (L->dev32)
+                rc = launch_steps_coop_fp32(P, (float *)L->f[0], (float *)L->f[1], (float *)L->g[0], (float *)L->g[1],
+                                            (float *)L->tlag, (float *)L->taules, L->cur, m, L->stream);
+            else
+                rc = launch_steps_coop_fp64(P, (double *)L->f[0], (double *)L->f[1], (double *)L->g[0],
+                                            (double *)L->g[1], (double *)L->tlag, (double *)L->taules, L->cur, m,
+                                            L->stream);
+            if (rc < 0) return fail(-2, "cooperative launch failed: %s", hipGetErrorString(hipGetLastError()));
+            if (rc == 0) {
+                if (m & 1) L->cur ^= 1;
+                L->itime += m;
+                s += m;
+                continue;
+            }
+            L->coop_ok = 0; // too large to be resident at once: graphs below
+        }
+        // Otherwise steady-state steps depend only on the buffer parity:
         // replay a captured GRAPH_STEPS-step graph instead of launching.
-        if (L->use_graphs && !L->first && L->itime >= 1 && !L->base.hit && !L->part && nsteps - s >= GRAPH_STEPS) {
+        if ((L->use_graphs & CH4G_LAUNCH_GRAPHS) && !L->first && L->itime >= 1 && !L->base.hit && !L->part && nsteps - s >= GRAPH_STEPS) {
             int rc = replay_graph(L);
             if (rc) return rc;
             s += GRAPH_STEPS;
@@ -672,10 +696,10 @@ int ch4g_step(ch4g_lattice *L, int nsteps)
     return 0;
 }
 
-int ch4g_set_graphs(ch4g_lattice *L, int enable)
+int ch4g_set_graphs(ch4g_lattice *L, int flags)
 {
     if (!L) return fail(-1, "null lattice");
-    L->use_graphs = enable != 0;
+    L->use_graphs = flags & (CH4G_LAUNCH_GRAPHS | CH4G_LAUNCH_COOP);
     return 0;
 }
 

@@ -185,7 +185,7 @@ int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s);
 size_t feedback_sort_bytes(int64_t nitems, int key_bits);
 
 // nodes per tile (one block) of the batched step
-constexpr int BATCH_TILE = 1024;
+constexpr int BATCH_TILE = 256;
 
 // One lattice of a batched step: its constants and its two state buffers.
 struct BatchEntry {
@@ -194,17 +194,19 @@ struct BatchEntry {
     void *tlag, *taules;
     int cur;            // buffer holding the state at the start of the chunk
 };
-// One block of a batched launch: entry (bit 31: parity of that lattice's
-// step within the chunk) and the lattice's BATCH_TILE-node tile.
-struct BatchBlock {
+// One lattice taking part in a batched launch: entry (bit 31: parity of
+// that lattice's step within the chunk) and its first block in the launch.
+// A launch's slots are sorted by tile0; the last one is a sentinel whose
+// tile0 is the launch's block count.
+struct BatchSlot {
     uint32_t entry;
-    uint32_t tile;
+    uint32_t tile0;
 };
-// one launch covers every block of a storage kind (FP32 float/double
-// arithmetic, FP64) listed for it
-void launch_batch_fp32(const BatchEntry *tab, const BatchBlock *blocks, int64_t nblocks, int arith64, bool ntl,
-                       hipStream_t s);
-void launch_batch_fp64(const BatchEntry *tab, const BatchBlock *blocks, int64_t nblocks, hipStream_t s);
+// one launch covers the lattices of a storage kind (FP32 float/double
+// arithmetic, FP64) listed in its slots
+void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int arith64,
+                       bool ntl, hipStream_t s);
+void launch_batch_fp64(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, hipStream_t s);
 
 // Kernel launchers (stream-ordered).
 void launch_step_fp64(const KParams &P, const double *fin, const double *gin, double *fout, double *gout,
@@ -215,6 +217,14 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
                       float *taules, hipStream_t s);
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,
                         double *gpost, double *dens, double *u, double *t, hipStream_t s);
+
+// Up to nsteps steady-state steps in one cooperative launch (grid barrier
+// between steps) when every block of the lattice can be resident; returns 1
+// when that does not hold (nothing launched), 0 on launch, negative on error.
+int launch_steps_coop_fp32(const KParams &P, float *f0, float *f1, float *g0, float *g1, float *tlag,
+                           float *taules, int cur, int nsteps, hipStream_t s);
+int launch_steps_coop_fp64(const KParams &P, double *f0, double *f1, double *g0, double *g1, double *tlag,
+                           double *taules, int cur, int nsteps, hipStream_t s);
 
 // Initial state from a per-row fluid profile prow[NY][19] and a temperature
 // field t0[N]; restart load from FP64 SoA staging.  dev = FP32-deviation.

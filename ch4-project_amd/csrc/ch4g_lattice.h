@@ -11,6 +11,8 @@ namespace ch4g {
 constexpr int HIT_RING = 8;
 // steps per captured graph (even: the graph starts and ends on the same buffer)
 constexpr int GRAPH_STEPS = 32;
+// steps per cooperative multi-step launch
+constexpr int COOP_STEPS = 64;
 
 // Lagrangian particles of one lattice (device SoA, see PartField).
 struct Particles {
@@ -74,7 +76,8 @@ struct ch4g_lattice {
     double *d_row64 = nullptr;
     // captured GRAPH_STEPS-step sequences, one per starting buffer
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int use_graphs = 1;
+    int use_graphs = CH4G_LAUNCH_GRAPHS | CH4G_LAUNCH_COOP;
+    int coop_ok = 1;         // cooperative multi-step launches possible (all blocks resident)
     int64_t bytes = 0;
     ch4g::Particles *part = nullptr;
 };

@@ -23,21 +23,23 @@ namespace {
 // A set of lattices stepped together.  A chunk is `chunk` launches per
 // storage kind; lattice i takes steps[i] of them, spread evenly over the
 // chunk, so every launch carries a similar share of the work.  Each launch
-// has its own block table (lattice, tile, parity of that lattice's step).
+// has its own short slot list (lattice, parity of its step, first block).
 struct Batch {
     struct Group {
         int dev32, arith64;
         bool ntl;
         int first, count;                 // range in the entry table
-        std::vector<int64_t> off, cnt;    // per launch: range in the block table
+        std::vector<int64_t> off;         // per launch: first slot in the slot table
+        std::vector<int> nslot;           // per launch: slots (without the sentinel)
+        std::vector<int64_t> nblk;        // per launch: blocks
     };
     std::vector<BatchEntry> host;
-    std::vector<BatchBlock> blocks;
+    std::vector<BatchSlot> slots;
     std::vector<Group> groups;
     BatchEntry *dev = nullptr;
     size_t dev_cap = 0;       // entries
-    BatchBlock *dev_blocks = nullptr;
-    size_t blocks_cap = 0;    // blocks
+    BatchSlot *dev_slots = nullptr;
+    size_t slots_cap = 0;
     hipStream_t stream = nullptr;
     std::vector<ch4g_lattice *> lats;
     std::vector<int> steps;   // steps of each lattice per chunk
@@ -57,9 +59,9 @@ struct Batch {
         if (dev) (void)hipFree(dev);
         dev = nullptr;
         dev_cap = 0;
-        if (dev_blocks) (void)hipFree(dev_blocks);
-        dev_blocks = nullptr;
-        blocks_cap = 0;
+        if (dev_slots) (void)hipFree(dev_slots);
+        dev_slots = nullptr;
+        slots_cap = 0;
         if (ev) (void)hipEventDestroy(ev);
         ev = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
@@ -83,7 +85,7 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
     B.lats = lats;
     B.steps = steps;
     B.host.clear();
-    B.blocks.clear();
+    B.slots.clear();
     B.groups.clear();
     B.chunk = 0;
     for (size_t i = 0; i < lats.size(); i++) {
@@ -131,16 +133,20 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
             G.count++;
         }
         G.off.assign((size_t)K, 0);
-        G.cnt.assign((size_t)K, 0);
+        G.nslot.assign((size_t)K, 0);
+        G.nblk.assign((size_t)K, 0);
         for (int k = 0; k < K; k++) {
-            G.off[(size_t)k] = (int64_t)B.blocks.size();
+            G.off[(size_t)k] = (int64_t)B.slots.size();
+            int64_t t0 = 0;
             for (int e = 0; e < G.count; e++) {
                 const int s_ = at[(size_t)e][(size_t)k];
                 if (s_ < 0) continue;
-                const uint32_t tag = (uint32_t)e | ((uint32_t)(s_ & 1) << 31);
-                for (int64_t t = 0; t < ntile[(size_t)e]; t++) B.blocks.push_back({tag, (uint32_t)t});
+                B.slots.push_back({(uint32_t)e | ((uint32_t)(s_ & 1) << 31), (uint32_t)t0});
+                t0 += ntile[(size_t)e];
             }
-            G.cnt[(size_t)k] = (int64_t)B.blocks.size() - G.off[(size_t)k];
+            G.nslot[(size_t)k] = (int)((int64_t)B.slots.size() - G.off[(size_t)k]);
+            G.nblk[(size_t)k] = t0;
+            B.slots.push_back({0u, (uint32_t)t0}); // sentinel
         }
         // non-temporal loads when the group's lattices fit the last-level cache
         G.ntl = bytes <= ((int64_t)192 << 20);
@@ -157,17 +163,17 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
         HIPCHK(hipMalloc((void **)&B.dev, cap * sizeof(BatchEntry)));
         B.dev_cap = cap;
     }
-    if (B.blocks.size() > B.blocks_cap) {
-        if (B.dev_blocks) (void)hipFree(B.dev_blocks);
-        B.dev_blocks = nullptr;
-        const size_t cap = std::max<size_t>(B.blocks.size() + B.blocks.size() / 2, 4096);
-        HIPCHK(hipMalloc((void **)&B.dev_blocks, cap * sizeof(BatchBlock)));
-        B.blocks_cap = cap;
+    if (B.slots.size() > B.slots_cap) {
+        if (B.dev_slots) (void)hipFree(B.dev_slots);
+        B.dev_slots = nullptr;
+        const size_t cap = std::max<size_t>(B.slots.size() + B.slots.size() / 2, 1024);
+        HIPCHK(hipMalloc((void **)&B.dev_slots, cap * sizeof(BatchSlot)));
+        B.slots_cap = cap;
     }
     if (!B.host.empty()) {
         HIPCHK(hipMemcpyAsync(B.dev, B.host.data(), B.host.size() * sizeof(BatchEntry), hipMemcpyHostToDevice,
                               B.stream));
-        HIPCHK(hipMemcpyAsync(B.dev_blocks, B.blocks.data(), B.blocks.size() * sizeof(BatchBlock),
+        HIPCHK(hipMemcpyAsync(B.dev_slots, B.slots.data(), B.slots.size() * sizeof(BatchSlot),
                               hipMemcpyHostToDevice, B.stream));
         HIPCHK(hipStreamSynchronize(B.stream));
     }
@@ -178,14 +184,15 @@ void launch_steps(Batch &B)
 {
     for (int k = 0; k < B.chunk; k++)
         for (const Batch::Group &G : B.groups) {
-            const int64_t nb = G.cnt[(size_t)k];
+            const int64_t nb = G.nblk[(size_t)k];
             if (nb == 0) continue;
             const BatchEntry *tab = B.dev + G.first;
-            const BatchBlock *blk = B.dev_blocks + G.off[(size_t)k];
+            const BatchSlot *sl = B.dev_slots + G.off[(size_t)k];
+            const int ns = G.nslot[(size_t)k];
             if (G.dev32)
-                launch_batch_fp32(tab, blk, nb, G.arith64, G.ntl, B.stream);
+                launch_batch_fp32(tab, sl, ns, nb, G.arith64, G.ntl, B.stream);
             else
-                launch_batch_fp64(tab, blk, nb, B.stream);
+                launch_batch_fp64(tab, sl, ns, nb, B.stream);
         }
 }
 
@@ -259,7 +266,9 @@ int ch4g_step_batch(ch4g_lattice *const *lats, int n, int nsteps)
     for (int i = 0; i < n; i++)
         for (int j = 0; j < i; j++)
             if (v[i] == v[j]) return fail(-1, "a lattice appears twice in the batch");
-    Batch B;
+    // the batch context (stream, device tables) is reused by later calls
+    // from the same thread
+    thread_local Batch B;
     int rc = build(B, v, std::vector<int>((size_t)n, nsteps));
     if (rc) return rc;
     rc = advance(B, false);
@@ -320,8 +329,8 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
         return 0;
     };
 
-    while (done < njobs) {
-        // ---- admission, in arrival order, while the footprint fits
+    // ---- admission, in arrival order, while the footprint fits
+    auto admit_ready = [&]() {
         const double now = seconds_since(t0);
         while (next < order.size() && jobs[order[next]].arrival_s <= now) {
             ch4g_job &J = jobs[order[next]];
@@ -352,6 +361,10 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
             next++;
             table_dirty = true;
         }
+    };
+
+    while (done < njobs) {
+        admit_ready();
         // jobs with nothing (left) to run retire at once
         for (size_t a = 0; a < active.size();) {
             if (active[a].remaining == 0)
@@ -370,8 +383,9 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
         // remaining steps, spread over the chunk's launches (even shares keep
         // each lattice's buffer parity, so the tables and their captured
         // graph stay valid until the set changes; a job that can finish
-        // does); shares are refreshed every 16 chunks.  Plain: the single
-        // job runs to its end in one call.
+        // does); shares are refreshed every 16 chunks.  Jobs arriving while
+        // the chunk runs are set up meanwhile and join the next one.  Plain:
+        // the single job runs to its end in one call.
         if (batched) {
             if (table_dirty || ++chunks_since_share >= 16) {
                 int64_t rmax = 1;
@@ -394,8 +408,10 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
             }
             if ((rc = advance(B, sp->use_graphs != 0))) break;
             commit(B);
+            const size_t in_chunk = B.lats.size(); // active[0 .. in_chunk) take part
+            admit_ready();
             HIPCHK(hipStreamSynchronize(B.stream));
-            for (size_t a = 0; a < active.size(); a++) {
+            for (size_t a = 0; a < in_chunk; a++) {
                 active[a].remaining -= B.steps[a];
                 // an odd share flips the lattice's buffer: the table follows
                 if (B.steps[a] & 1) table_dirty = true;

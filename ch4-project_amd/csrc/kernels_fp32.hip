@@ -13,6 +13,8 @@
 //   df*   = df - (df - df_eq)/tau + (1 - 1/(2 tau)) w rho (3 (c-u).F + 9 (c.u)(c.F))
 // and the same for the temperature (T = T_b + dT, no forcing term).  Step
 // semantics (lag scalar, wall rules, force lag) are those of kernels_fp64.hip.
+#include <hip/hip_cooperative_groups.h>
+
 #include "ch4g_internal.h"
 
 namespace ch4g {
@@ -354,25 +356,57 @@ __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__res
     step_node32<FIRST, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
 }
 
+// Several steps of one lattice in one cooperative launch: every block is
+// resident (hipLaunchCooperativeKernel refuses the launch otherwise), each
+// runs the same fixed number of steps and meets the others at a grid
+// barrier between steps, so the launch gap between steps disappears for
+// lattices that fit the GPU in one wave of blocks.
+template <bool FLUID, bool THERMAL, typename A, bool NTL>
+__global__ void __launch_bounds__(256) k_steps_coop32(KParams P, float *f0, float *f1, float *g0, float *g1,
+                                                      float *tlag, float *taules, int cur, int nsteps)
+{
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const bool node = x < P.NX;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + (node ? x : 0);
+#pragma unroll 1
+    for (int s = 0; s < nsteps; s++) {
+        const int in = cur ^ (s & 1);
+        const float *fin = in ? f1 : f0, *gin = in ? g1 : g0;
+        float *fout = in ? f0 : f1, *gout = in ? g0 : g1;
+        if (node) step_node32<false, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
+        grid.sync();
+    }
+}
+
 // Many lattices in one launch: block b advances BATCH_TILE consecutive nodes
-// (256 at a time, the lattice's constants loaded once per block) of the
-// lattice and tile named by the launch's block table, with that lattice's
-// step parity (tile -> lattice map), whatever its population
+// of the lattice whose block range in this launch holds b (block-uniform
+// binary search over the launch's few slots), with that lattice's step
+// parity (tile -> lattice map), whatever its population
 // sets (a block-uniform dispatch), so small and large lattices of every kind
 // share one launch per step; the state buffer of each lattice alternates
 // with the step parity.
 template <typename A, bool NTL>
 __global__ void __launch_bounds__(256) k_step_batch32(const BatchEntry *__restrict__ tab,
-                                                      const BatchBlock *__restrict__ blocks)
+                                                      const BatchSlot *__restrict__ slots, int nslots)
 {
-    const BatchBlock bb = blocks[blockIdx.x];
-    const BatchEntry &E = tab[bb.entry & 0x7fffffffu];
+    int lo = 0, hi = nslots - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (slots[mid].tile0 <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchSlot sl = slots[lo];
+    const BatchEntry &E = tab[sl.entry & 0x7fffffffu];
     const KParams P = E.P;
-    const int in = E.cur ^ (int)(bb.entry >> 31);
+    const int in = E.cur ^ (int)(sl.entry >> 31);
     const float *fin = (const float *)E.f[in], *gin = (const float *)E.g[in];
     float *fout = (float *)E.f[in ^ 1], *gout = (float *)E.g[in ^ 1];
     float *tl = (float *)E.tlag, *tau = (float *)E.taules;
-    const int64_t base = (int64_t)bb.tile * BATCH_TILE;
+    const int64_t base = (int64_t)(blockIdx.x - sl.tile0) * BATCH_TILE;
 #pragma unroll 1
     for (int it = 0; it < BATCH_TILE / 256; it++) {
         const int64_t m = base + it * 256 + threadIdx.x;
@@ -496,20 +530,57 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 #undef CH4G_LAUNCH32_
 }
 
-void launch_batch_fp32(const BatchEntry *tab, const BatchBlock *blocks, int64_t nblocks, int arith64, bool ntl,
-                       hipStream_t s)
+// Cooperative multi-step launch when the lattice's blocks can all be
+// resident; returns 1 if it cannot be used (the caller launches per step).
+int launch_steps_coop_fp32(const KParams &P, float *f0, float *f1, float *g0, float *g1, float *tlag,
+                           float *taules, int cur, int nsteps, hipStream_t s)
+{
+    int bx = block_x32(P);
+    dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ), block(bx);
+    const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
+    const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
+    const void *fn = nullptr;
+#define CH4G_COOP_(FL_, TH_, A_)                                                                                      \
+    fn = ntl ? (const void *)&k_steps_coop32<FL_, TH_, A_, true> : (const void *)&k_steps_coop32<FL_, TH_, A_, false>
+#define CH4G_COOP(FL_, TH_)                                                                                           \
+    do {                                                                                                              \
+        if (P.arith64)                                                                                                \
+            CH4G_COOP_(FL_, TH_, double);                                                                             \
+        else                                                                                                          \
+            CH4G_COOP_(FL_, TH_, float);                                                                              \
+    } while (0)
+    if (P.fluid && P.thermal)
+        CH4G_COOP(true, true);
+    else if (P.fluid)
+        CH4G_COOP(true, false);
+    else
+        CH4G_COOP(false, true);
+#undef CH4G_COOP
+#undef CH4G_COOP_
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bx, 0) != hipSuccess) return 1;
+    if ((int64_t)grid.x * grid.y * grid.z > (int64_t)per_cu * cus) return 1;
+    KParams Pk = P;
+    void *args[] = {&Pk, &f0, &f1, &g0, &g1, &tlag, &taules, &cur, &nsteps};
+    return hipLaunchCooperativeKernel(fn, grid, block, args, 0, s) == hipSuccess ? 0 : -2;
+}
+
+void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int arith64,
+                       bool ntl, hipStream_t s)
 {
     dim3 grid((unsigned)nblocks), block(256);
     if (arith64) {
         if (ntl)
-            hipLaunchKernelGGL((k_step_batch32<double, true>), grid, block, 0, s, tab, blocks);
+            hipLaunchKernelGGL((k_step_batch32<double, true>), grid, block, 0, s, tab, slots, nslots);
         else
-            hipLaunchKernelGGL((k_step_batch32<double, false>), grid, block, 0, s, tab, blocks);
+            hipLaunchKernelGGL((k_step_batch32<double, false>), grid, block, 0, s, tab, slots, nslots);
     } else {
         if (ntl)
-            hipLaunchKernelGGL((k_step_batch32<float, true>), grid, block, 0, s, tab, blocks);
+            hipLaunchKernelGGL((k_step_batch32<float, true>), grid, block, 0, s, tab, slots, nslots);
         else
-            hipLaunchKernelGGL((k_step_batch32<float, false>), grid, block, 0, s, tab, blocks);
+            hipLaunchKernelGGL((k_step_batch32<float, false>), grid, block, 0, s, tab, slots, nslots);
     }
 }
 

@@ -13,6 +13,8 @@
 // rho, u, T of step s-1 (lb.c:420-540; the u shift uses the force built from
 // the lagged temperature, as main.c's ordering implies), then collides
 // (fluid.c:1033-1225) and adds the Guo term (forcing.c:1424-1435) of step s.
+#include <hip/hip_cooperative_groups.h>
+
 #include "ch4g_internal.h"
 
 namespace ch4g {
@@ -244,18 +246,46 @@ __global__ void __launch_bounds__(256) k_step_fp64(KParams P, const double *__re
     step_node64<FIRST, FLUID, THERMAL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
 }
 
+// Several steps of one lattice in one cooperative launch (see k_steps_coop32).
+template <bool FLUID, bool THERMAL>
+__global__ void __launch_bounds__(256) k_steps_coop64(KParams P, double *f0, double *f1, double *g0, double *g1,
+                                                      double *tlag, double *taules, int cur, int nsteps)
+{
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const bool node = x < P.NX;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + (node ? x : 0);
+#pragma unroll 1
+    for (int s = 0; s < nsteps; s++) {
+        const int in = cur ^ (s & 1);
+        const double *fin = in ? f1 : f0, *gin = in ? g1 : g0;
+        double *fout = in ? f0 : f1, *gout = in ? g0 : g1;
+        if (node) step_node64<false, FLUID, THERMAL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
+        grid.sync();
+    }
+}
+
 // Many lattices in one launch (see k_step_batch32).
 __global__ void __launch_bounds__(256) k_step_batch64(const BatchEntry *__restrict__ tab,
-                                                      const BatchBlock *__restrict__ blocks)
+                                                      const BatchSlot *__restrict__ slots, int nslots)
 {
-    const BatchBlock bb = blocks[blockIdx.x];
-    const BatchEntry &E = tab[bb.entry & 0x7fffffffu];
+    int lo = 0, hi = nslots - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (slots[mid].tile0 <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchSlot sl = slots[lo];
+    const BatchEntry &E = tab[sl.entry & 0x7fffffffu];
     const KParams P = E.P;
-    const int in = E.cur ^ (int)(bb.entry >> 31);
+    const int in = E.cur ^ (int)(sl.entry >> 31);
     const double *fin = (const double *)E.f[in], *gin = (const double *)E.g[in];
     double *fout = (double *)E.f[in ^ 1], *gout = (double *)E.g[in ^ 1];
     double *tl = (double *)E.tlag, *tau = (double *)E.taules;
-    const int64_t base = (int64_t)bb.tile * BATCH_TILE;
+    const int64_t base = (int64_t)(blockIdx.x - sl.tile0) * BATCH_TILE;
 #pragma unroll 1
     for (int it = 0; it < BATCH_TILE / 256; it++) {
         const int64_t m = base + it * 256 + threadIdx.x;
@@ -349,9 +379,31 @@ void launch_step_fp64(const KParams &P, const double *fin, const double *gin, do
 #undef CH4G_LAUNCH64
 }
 
-void launch_batch_fp64(const BatchEntry *tab, const BatchBlock *blocks, int64_t nblocks, hipStream_t s)
+int launch_steps_coop_fp64(const KParams &P, double *f0, double *f1, double *g0, double *g1, double *tlag,
+                           double *taules, int cur, int nsteps, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_batch64, dim3((unsigned)nblocks), dim3(256), 0, s, tab, blocks);
+    int bx = block_x(P);
+    dim3 grid = grid_of(P, bx), block(bx);
+    const void *fn;
+    if (P.fluid && P.thermal)
+        fn = (const void *)&k_steps_coop64<true, true>;
+    else if (P.fluid)
+        fn = (const void *)&k_steps_coop64<true, false>;
+    else
+        fn = (const void *)&k_steps_coop64<false, true>;
+    int dev = 0, per_cu = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bx, 0) != hipSuccess) return 1;
+    if ((int64_t)grid.x * grid.y * grid.z > (int64_t)per_cu * cus) return 1;
+    KParams Pk = P;
+    void *args[] = {&Pk, &f0, &f1, &g0, &g1, &tlag, &taules, &cur, &nsteps};
+    return hipLaunchCooperativeKernel(fn, grid, block, args, 0, s) == hipSuccess ? 0 : -2;
+}
+
+void launch_batch_fp64(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_batch64, dim3((unsigned)nblocks), dim3(256), 0, s, tab, slots, nslots);
 }
 
 void launch_gather_fp64(const KParams &P, const double *fin, const double *gin, const double *tlag, double *fpost,

@@ -256,6 +256,7 @@ __device__ __forceinline__ void wrap_keep(const PartParams &P, double &x, double
 // One sub-step of every particle: interpolate_vector_at_particles(u),
 // interpolate_scalar_at_particles(t) (lagrange.c:1211-1593), move_particles
 // (lagrange.c:2934-4025) and the wrap of sendrecv_particles.
+template <bool GRAD>
 __global__ void __launch_bounds__(PB) k_particles(PartParams P)
 {
     const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
@@ -298,7 +299,7 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
                 ux = v0;
                 uy = v1;
                 uz = v2;
-                if (lflag(P, CH4G_LAGRANGE_GRADIENT)) {
+                if constexpr (GRAD) {
                     double acc[9], Gc[9];
                     for (int c = 0; c < 8; c++) {
                         grad_at(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), Gc);
@@ -694,7 +695,10 @@ int launch_particles(const PartParams &P, hipStream_t s)
 {
     if (P.np == 0) return 0;
     const int nb = blocks_of(P.np, PB);
-    hipLaunchKernelGGL(k_particles, dim3(nb), dim3(PB), 0, s, P);
+    if (P.flags & CH4G_LAGRANGE_GRADIENT)
+        hipLaunchKernelGGL(k_particles<true>, dim3(nb), dim3(PB), 0, s, P);
+    else
+        hipLaunchKernelGGL(k_particles<false>, dim3(nb), dim3(PB), 0, s, P);
     if (P.bsum) {
         hipLaunchKernelGGL(k_draw_offsets, dim3(1), dim3(1024), 0, s, P.bsum, nb, P.rng);
         hipLaunchKernelGGL(k_draw_apply, dim3(nb), dim3(PB), 0, s, P);

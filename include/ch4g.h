@@ -157,11 +157,17 @@ int ch4g_set_hit_phases(ch4g_lattice *lat, const double *phi);
  * (parallel.c:618-638, run_time.dat "time per step and grid point").      */
 int ch4g_step_timed(ch4g_lattice *lat, int nsteps, double *elapsed_ms);
 
-/* Replay steady-state steps from captured HIP graphs (default on).  Graphs
- * hold GRAPH_STEPS steps and are used once the first step has run, for runs
- * without per-step host forcing tables (HIT); results are identical either
- * way.  No reference counterpart (launch mechanics of this library).        */
-int ch4g_set_graphs(ch4g_lattice *lat, int enable);
+/* How steady-state steps (after the first, without per-step host forcing
+ * tables (HIT) or particles) are launched; flags, default both:
+ *   CH4G_LAUNCH_GRAPHS  replay captured 32-step HIP graphs;
+ *   CH4G_LAUNCH_COOP    run up to 64 steps in one cooperative launch with a
+ *                       grid barrier between steps, when every block of the
+ *                       lattice can be resident at once (small lattices,
+ *                       where per-launch gaps dominate); else graphs.
+ * 0 launches every step on its own.  Results are identical in every mode.
+ * No reference counterpart (launch mechanics of this library).             */
+enum { CH4G_LAUNCH_GRAPHS = 1, CH4G_LAUNCH_COOP = 2 };
+int ch4g_set_graphs(ch4g_lattice *lat, int flags);
 
 /* Wait for all work queued on the lattice stream. */
 int ch4g_synchronize(ch4g_lattice *lat);

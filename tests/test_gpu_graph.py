@@ -9,13 +9,17 @@ import configs
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("coop", [False, True])
 @pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_DEV])
-def test_graph_replay_matches_eager(precision):
+def test_graph_replay_matches_eager(precision, coop):
+    """Graph replay alone, and cooperative multi-step launches (the lattice
+    fits the GPU at once) with graphs behind them."""
     cfg = configs.rb2d(NX=64, NY=32, Ra=1e6, precision=precision)
     eager = Lattice(cfg)
     eager.set_graphs(False)
     eager.initial_conditions()
     graph = Lattice(cfg)
+    graph.set_graphs(True, coop=coop)
     graph.initial_conditions()
     for rep in range(4):
         n = 70 if rep % 2 == 0 else 64  # 2 graphs + 6 eager steps / exactly 2 graphs
@@ -28,13 +32,15 @@ def test_graph_replay_matches_eager(precision):
         graph.dump_averages()  # eager work between replays
 
 
-def test_graph_replay_les_matches_eager():
+@pytest.mark.parametrize("coop", [False, True])
+def test_graph_replay_les_matches_eager(coop):
     cfg = configs.poiseuille(8, 16, 8, tau=0.52, g=1e-5)
     cfg.flags |= abi.FLUID_LES | abi.FLUID_LES_LOCAL_STRAIN
     eager = Lattice(cfg)
     eager.set_graphs(False)
     eager.initial_conditions()
     graph = Lattice(cfg)
+    graph.set_graphs(True, coop=coop)
     graph.initial_conditions()
     for _ in range(3):
         eager.step(65)
@@ -42,3 +48,22 @@ def test_graph_replay_les_matches_eager():
         pe, _ = eager.get_populations()
         pg, _ = graph.get_populations()
         assert np.array_equal(pe, pg)
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_DEV])
+def test_coop_fluid_only_odd_counts(precision):
+    """Cooperative launches of a fluid-only channel over odd step counts,
+    which leave the state in the other buffer, and a count below two."""
+    cfg = configs.poiseuille(12, 20, 6, tau=0.8, g=1e-5, precision=precision)
+    eager = Lattice(cfg)
+    eager.set_graphs(False)
+    eager.initial_conditions()
+    coop = Lattice(cfg)
+    coop.set_graphs(False, coop=True)
+    coop.initial_conditions()
+    for n in (1, 5, 67, 1, 3):
+        eager.step(n)
+        coop.step(n)
+        pe, _ = eager.get_populations()
+        pc, _ = coop.get_populations()
+        assert np.array_equal(pe, pc), n
