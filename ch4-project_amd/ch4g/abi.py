@@ -25,6 +25,7 @@ TEMPERATURE_INITIAL_LINEAR = 1 << 12
 TEMPERATURE_INITIAL_ADD_NOISE = 1 << 13
 FLUID_INITIAL_UNIT_DENSITY = 1 << 14
 INITIAL_BAROMETRIC = 1 << 15
+TEMPERATURE_LES = 1 << 16
 
 # enum ch4g_lagrange_flag (reference LAGRANGE_* switch names)
 LAGRANGE_ADD_TRACER = 1 << 0
@@ -90,6 +91,12 @@ class Output(C.Structure):
         return {n: float(getattr(self, n)) for n in OUTPUT_FIELDS}
 
 
+def output_dtype():
+    """numpy record layout of ch4g_output (rulers are arrays of it)."""
+    import numpy as np
+    return np.dtype([(n, np.float64) for n in OUTPUT_FIELDS])
+
+
 def make_config(NX, NY, NZ, flags, precision=FP64, **kw):
     """Config with the reference's defaults (rho0 = 1, T0 = mean wall T)."""
     c = Config()
@@ -147,10 +154,6 @@ def particle_dtype():
     import numpy as np
     return np.dtype([(n, np.float64) for n in PARTICLE_FIELDS])
 
-
-# steady-state launch mode flags of ch4g_set_graphs
-LAUNCH_GRAPHS = 1
-LAUNCH_COOP = 2
 
 SERVE_BATCHED = 0
 SERVE_SEQUENTIAL = 1

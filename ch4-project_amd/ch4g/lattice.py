@@ -9,8 +9,7 @@ import os
 
 import numpy as np
 
-from . import abi
-from .abi import NPOP, Config, Job, Output, ParticleConfig, ServeParams, particle_dtype
+from .abi import NPOP, Config, Job, Output, ParticleConfig, ServeParams, output_dtype, particle_dtype
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libch4g.so")
@@ -22,7 +21,7 @@ _lib = None
 EXPORTS = [
     "ch4g_last_error", "ch4g_device_count", "ch4g_create", "ch4g_destroy", "ch4g_initial_conditions",
     "ch4g_set_populations", "ch4g_get_populations", "ch4g_step", "ch4g_hydro_fields", "ch4g_dump_averages",
-    "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_step_timed", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
+    "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_dump_rulers", "ch4g_step_timed", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
     "ch4g_footprint_bytes", "ch4g_particles_init", "ch4g_particles_count", "ch4g_get_particles",
     "ch4g_set_particles", "ch4g_step_batch", "ch4g_serve",
 ]
@@ -53,6 +52,7 @@ def lib():
     L.ch4g_step.argtypes = [P, C.c_int]
     L.ch4g_hydro_fields.argtypes = [P, dp, dp, dp]
     L.ch4g_dump_averages.argtypes = [P, C.POINTER(Output)]
+    L.ch4g_dump_rulers.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
     L.ch4g_get_hit_phases.argtypes = [P, dp]
     L.ch4g_set_hit_phases.argtypes = [P, dp]
     L.ch4g_step_timed.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
@@ -166,12 +166,8 @@ class Lattice:
         check(self._L.ch4g_step_timed(self._h, int(n), C.byref(ms)))
         return ms.value
 
-    def set_graphs(self, enable, coop=None):
-        """Launch mode of steady-state steps: graph replay when enable, and
-        cooperative multi-step launches when coop (default: same as enable)."""
-        coop = enable if coop is None else coop
-        flags = (abi.LAUNCH_GRAPHS if enable else 0) | (abi.LAUNCH_COOP if coop else 0)
-        check(self._L.ch4g_set_graphs(self._h, flags))
+    def set_graphs(self, enable):
+        check(self._L.ch4g_set_graphs(self._h, 1 if enable else 0))
 
     def synchronize(self):
         check(self._L.ch4g_synchronize(self._h))
@@ -197,6 +193,13 @@ class Lattice:
         out = Output()
         check(self._L.ch4g_dump_averages(self._h, C.byref(out)))
         return out.as_dict()
+
+    def dump_rulers(self):
+        """x, y, z rulers (plane averages) as numpy arrays of output records."""
+        c = self.cfg
+        r = [np.zeros(n, dtype=output_dtype()) for n in (c.NX, c.NY, c.NZ)]
+        check(self._L.ch4g_dump_rulers(self._h, *[x.ctypes.data_as(C.c_void_p) for x in r]))
+        return tuple(r)
 
     def hit_phases(self):
         a = np.zeros((6, 3))

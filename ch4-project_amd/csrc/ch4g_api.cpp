@@ -13,7 +13,6 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -54,6 +53,13 @@ int dalloc(ch4g_lattice *L, void **p, size_t bytes)
 namespace {
 
 bool has(const ch4g_config &c, uint32_t f) { return (c.flags & f) != 0; }
+
+// Smagorinsky closure from finite differences of u (the reference's default
+// LB_FLUID_LES form, les.c:31-43)
+bool les_fd(const ch4g_config &c)
+{
+    return has(c, CH4G_FLUID_LES) && !has(c, CH4G_FLUID_LES_LOCAL_STRAIN) && has(c, CH4G_LB_FLUID);
+}
 
 bool forced(const ch4g_config &c)
 {
@@ -236,6 +242,13 @@ int launch_one_step(ch4g_lattice *L)
     }
     KParams P = params_for(L, L->first, itime);
     int in = L->cur, out = 1 - L->cur;
+    if (P.les_fd) {
+        // u of this step at every node first: the closure differentiates it
+        if (L->dev32)
+            launch_velocity_fp32(P, (const float *)L->f[in], (const float *)L->tlag, L->stream);
+        else
+            launch_velocity_fp64(P, (const double *)L->f[in], (const double *)L->tlag, L->stream);
+    }
     if (L->dev32)
         launch_step_fp32(P, (const float *)L->f[in], (const float *)L->g[in], (float *)L->f[out], (float *)L->g[out],
                          (float *)L->tlag, (float *)L->taules, L->stream);
@@ -320,8 +333,6 @@ int validate(const ch4g_config *c)
     if (has(*c, CH4G_TEMPERATURE_BUOYANCY) && !has(*c, CH4G_LB_TEMPERATURE))
         return fail(-1, "LB_TEMPERATURE_BUOYANCY requires LB_TEMPERATURE");
     if (forced(*c) && !has(*c, CH4G_LB_FLUID)) return fail(-1, "fluid forcing requires LB_FLUID");
-    if (has(*c, CH4G_FLUID_LES) && !has(*c, CH4G_FLUID_LES_LOCAL_STRAIN))
-        return fail(-1, "LB_FLUID_LES is supported in its LB_FLUID_LES_LOCAL_STRAIN form only");
     if (c->tau_u <= 0.5 && has(*c, CH4G_LB_FLUID)) return fail(-1, "tau_u must exceed 1/2");
     if (c->tau_t <= 0.5 && has(*c, CH4G_LB_TEMPERATURE)) return fail(-1, "tau_t must exceed 1/2");
     if ((has(*c, CH4G_FLUID_BC_Y) || has(*c, CH4G_TEMPERATURE_BC_Y)) && c->NY < 3)
@@ -351,6 +362,7 @@ int64_t ch4g_footprint_bytes(const ch4g_config *c)
     if (has(*c, CH4G_LB_FLUID)) b += 2 * Q * N * r;
     if (has(*c, CH4G_LB_TEMPERATURE)) b += 2 * Q * N * r + N * r;
     if (has(*c, CH4G_FLUID_LES)) b += N * r;
+    if (les_fd(*c)) b += 3 * N * r;
     return b;
 }
 
@@ -383,6 +395,8 @@ int ch4g_create(const ch4g_config *cfg, ch4g_lattice **out)
     }
     if (has(*cfg, CH4G_FLUID_LES))
         if ((rc = dalloc(L, &L->taules, N * L->rsz))) goto err;
+    if (les_fd(*cfg))
+        if ((rc = dalloc(L, &L->vel, 3 * N * L->rsz))) goto err;
 
     {
         // design_lb constants (lb.c:368-377) and parameters.c:179,285
@@ -398,6 +412,9 @@ int ch4g_create(const ch4g_config *cfg, ch4g_lattice **out)
         P.wall_t = has(*cfg, CH4G_TEMPERATURE_BC_Y);
         P.forced = forced(*cfg);
         P.les = has(*cfg, CH4G_FLUID_LES);
+        P.les_fd = les_fd(*cfg);
+        P.les_t = has(*cfg, CH4G_FLUID_LES) && has(*cfg, CH4G_TEMPERATURE_LES) && th && fl;
+        P.vel = L->vel;
         double cs = 1.0 / sqrt(3.0);
         double cs2 = pow(cs, 2.0), cs4 = pow(cs, 4.0);
         P.cs2 = cs2;
@@ -414,6 +431,7 @@ int ch4g_create(const ch4g_config *cfg, ch4g_lattice **out)
         P.invtau_t = 1.0 / cfg->tau_t;
         P.guo_fac = (1.0 - 0.5 * 1.0 * (1.0 / cfg->tau_u));
         P.Cs2 = pow(cfg->les_cs * 1.0, 2.0);
+        P.kappa = L->kappa;
         const double LX = cfg->NX, LY = cfg->NY;
         double Fc[3] = {0.0, 0.0, 0.0};
         if (has(*cfg, CH4G_FLUID_FORCING_POISEUILLE)) {
@@ -526,7 +544,7 @@ void ch4g_destroy(ch4g_lattice *L)
     }
     destroy_graphs(L);
     particles_free(L);
-    void *scr[] = {L->tlag, L->taules, L->d_dens, L->d_u, L->d_t, L->d_part, L->d_res, L->d_fs, L->d_gs,
+    void *scr[] = {L->tlag, L->taules, L->vel, L->d_rul, L->d_dens, L->d_u, L->d_t, L->d_part, L->d_res, L->d_fs, L->d_gs,
                    L->d_row32, L->d_row64};
     for (void *p : scr)
         if (p) (void)hipFree(p);
@@ -658,32 +676,9 @@ int ch4g_step(ch4g_lattice *L, int nsteps)
     int s = 0;
     while (s < nsteps) {
         // Steady-state steps (not the first, no per-step host forcing tables,
-        // no particles) of a lattice whose blocks all fit the GPU at once run
-        // COOP_STEPS at a time in one cooperative launch.
-        if ((L->use_graphs & CH4G_LAUNCH_COOP) && L->coop_ok && !L->first && L->itime >= 1 && !L->base.hit && !L->part &&
-            nsteps - s >= 2) {
-            const int m = std::min(COOP_STEPS, nsteps - s);
-            KParams P = params_for(L, 0, (int)L->itime + 1);
-            int rc;
-            if (L->dev32)
-                rc = launch_steps_coop_fp32(P, (float *)L->f[0], (float *)L->f[1], (float *)L->g[0], (float *)L->g[1],
-                                            (float *)L->tlag, (float *)L->taules, L->cur, m, L->stream);
-            else
-                rc = launch_steps_coop_fp64(P, (double *)L->f[0], (double *)L->f[1], (double *)L->g[0],
-                                            (double *)L->g[1], (double *)L->tlag, (double *)L->taules, L->cur, m,
-                                            L->stream);
-            if (rc < 0) return fail(-2, "cooperative launch failed: %s", hipGetErrorString(hipGetLastError()));
-            if (rc == 0) {
-                if (m & 1) L->cur ^= 1;
-                L->itime += m;
-                s += m;
-                continue;
-            }
-            L->coop_ok = 0; // too large to be resident at once: graphs below
-        }
-        // Otherwise steady-state steps depend only on the buffer parity:
-        // replay a captured GRAPH_STEPS-step graph instead of launching.
-        if ((L->use_graphs & CH4G_LAUNCH_GRAPHS) && !L->first && L->itime >= 1 && !L->base.hit && !L->part && nsteps - s >= GRAPH_STEPS) {
+        // no particles) depend only on the buffer parity: replay a captured
+        // GRAPH_STEPS-step graph instead of launching.
+        if (L->use_graphs && !L->first && L->itime >= 1 && !L->base.hit && !L->part && nsteps - s >= GRAPH_STEPS) {
             int rc = replay_graph(L);
             if (rc) return rc;
             s += GRAPH_STEPS;
@@ -696,10 +691,10 @@ int ch4g_step(ch4g_lattice *L, int nsteps)
     return 0;
 }
 
-int ch4g_set_graphs(ch4g_lattice *L, int flags)
+int ch4g_set_graphs(ch4g_lattice *L, int enable)
 {
     if (!L) return fail(-1, "null lattice");
-    L->use_graphs = flags & (CH4G_LAUNCH_GRAPHS | CH4G_LAUNCH_COOP);
+    L->use_graphs = enable != 0;
     return 0;
 }
 
@@ -743,6 +738,33 @@ int ch4g_hydro_fields(ch4g_lattice *L, double *dens, double *u, double *t)
     return 0;
 }
 
+namespace {
+
+// normalisation of output.c:499-690: fluid members by fnorm, temperature
+// members by tnorm, Nusselt numbers by tnorm / (kappa deltaT / SY), lb from
+// the normalised eps and epst with buoyancy
+void normalise(const ch4g_lattice *L, ch4g_output &o, double fnorm, double tnorm)
+{
+    const ch4g_config &c = L->cfg;
+    if (L->base.fluid) {
+        double *v[] = {&o.ene, &o.rho, &o.ux, &o.uy, &o.uz, &o.ux2, &o.uy2, &o.uz2, &o.eps, &o.rho2};
+        for (double *x : v) *x *= fnorm;
+    }
+    if (L->base.thermal) {
+        double *v[] = {&o.t, &o.t2, &o.epst, &o.dxt, &o.dyt, &o.dzt, &o.uxt, &o.uyt, &o.uzt};
+        for (double *x : v) *x *= tnorm;
+        double nn = tnorm / (L->kappa * L->deltaT / (double)c.NY);
+        o.nux *= nn;
+        o.nuy *= nn;
+        o.nuz *= nn;
+        o.lb = 0.0;
+        if (L->base.buoy)
+            o.lb = pow(o.eps, 5. / 4.) * pow(o.epst, -3. / 4.) * pow(c.gravity_y * c.beta_t, -3. / 2.);
+    }
+}
+
+}  // namespace
+
 int ch4g_dump_averages(ch4g_lattice *L, ch4g_output *out)
 {
     if (!L || !out) return fail(-1, "null argument");
@@ -754,27 +776,60 @@ int ch4g_dump_averages(ch4g_lattice *L, ch4g_output *out)
     double a[NOUT];
     HIPCHK(hipMemcpyAsync(a, L->d_res, sizeof a, hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
-    // normalisation of output.c:499-579
     const ch4g_config &c = L->cfg;
-    double norm = 1.0 / ((double)c.NX * (double)c.NY * (double)c.NZ);
+    const double norm = 1.0 / ((double)c.NX * (double)c.NY * (double)c.NZ);
     ch4g_output o;
     memcpy(&o, a, sizeof o);
-    if (L->base.fluid) {
-        double *v[] = {&o.ene, &o.rho, &o.ux, &o.uy, &o.uz, &o.ux2, &o.uy2, &o.uz2, &o.eps, &o.rho2};
-        for (double *x : v) *x *= norm;
-    }
-    if (L->base.thermal) {
-        double *v[] = {&o.t, &o.t2, &o.epst, &o.dxt, &o.dyt, &o.dzt, &o.uxt, &o.uyt, &o.uzt};
-        for (double *x : v) *x *= norm;
-        double nn = norm / (L->kappa * L->deltaT / (double)c.NY);
-        o.nux *= nn;
-        o.nuy *= nn;
-        o.nuz *= nn;
-        o.lb = 0.0;
-        if (L->base.buoy)
-            o.lb = pow(o.eps, 5. / 4.) * pow(o.epst, -3. / 4.) * pow(c.gravity_y * c.beta_t, -3. / 2.);
-    }
+    normalise(L, o, norm, norm);
     *out = o;
+    return 0;
+}
+
+int ch4g_dump_rulers(ch4g_lattice *L, ch4g_output *rx, ch4g_output *ry, ch4g_output *rz)
+{
+    if (!L) return fail(-1, "null lattice");
+    int rc = ensure_scratch(L);
+    if (rc) return rc;
+    const ch4g_config &c = L->cfg;
+    const size_t planes = (size_t)c.NX + c.NY + c.NZ;
+    if (!L->d_rul && (rc = dalloc(L, (void **)&L->d_rul, planes * NOUT * sizeof(double)))) return rc;
+    double *dx = L->d_rul, *dy = dx + (size_t)c.NX * NOUT, *dz = dy + (size_t)c.NY * NOUT;
+    gather(L, nullptr, nullptr, L->d_dens, L->d_u, L->d_t);
+    launch_rulers(L->base, L->d_dens, L->d_u, L->d_t, L->nu, L->kappa, rx ? dx : nullptr, ry ? dy : nullptr,
+                  rz ? dz : nullptr, L->stream);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(planes * NOUT);
+    HIPCHK(hipMemcpyAsync(h.data(), L->d_rul, h.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    static_assert(sizeof(ch4g_output) == NOUT * sizeof(double), "ch4g_output layout");
+    const double SX = c.NX, SY = c.NY, SZ = c.NZ;
+    // output.c:520-650: fluid 1/(SY SZ), 1/(SX SZ), 1/(SX SY); temperature
+    // 1/(SY SZ), 1/(SX SZ) and again 1/(SY SZ) for z; the ruler's own
+    // coordinate by 1/(NY NZ), 1/(NX NZ), 1/(NX NY)
+    struct Axis {
+        ch4g_output *r;
+        int n;
+        const double *src;
+        double fnorm, tnorm, cnorm;
+        int coord;
+    } ax[3] = {{rx, c.NX, h.data(), 1.0 / (SY * SZ), 1.0 / (SY * SZ), 1.0 / (double)(c.NY * c.NZ), 0},
+               {ry, c.NY, h.data() + (size_t)c.NX * NOUT, 1.0 / (SX * SZ), 1.0 / (SX * SZ),
+                1.0 / (double)(c.NX * c.NZ), 1},
+               {rz, c.NZ, h.data() + (size_t)(c.NX + c.NY) * NOUT, 1.0 / (SX * SY), 1.0 / (SY * SZ),
+                1.0 / (double)(c.NX * c.NY), 2}};
+    for (const Axis &A : ax) {
+        if (!A.r) continue;
+        for (int i = 0; i < A.n; i++) {
+            ch4g_output o;
+            memcpy(&o, A.src + (size_t)i * NOUT, sizeof o);
+            double *xyz = &o.x;
+            const double cc = L->base.fluid ? xyz[A.coord] * A.cnorm : 0.0;
+            normalise(L, o, A.fnorm, A.tnorm);
+            o.x = o.y = o.z = 0.0;
+            xyz[A.coord] = cc;
+            A.r[i] = o;
+        }
+    }
     return 0;
 }
 

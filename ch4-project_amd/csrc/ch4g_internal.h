@@ -52,11 +52,16 @@ struct KParams {
     int wall_f, wall_t; // y walls: fluid no-slip, temperature fixed
     int forced;         // any fluid forcing (hydro_fields half-force shift + Guo)
     int first;          // inputs are pre-collision p_0/g_0: no gather
-    int les;            // local-strain Smagorinsky
+    int les;            // Smagorinsky relaxation time per node (tau_u_les)
+    int les_fd;         // |S| from finite differences of the stored velocity (else local strain)
+    int les_t;          // temperature relaxation time from tau_u_les (tau_t_les)
     int itime;          // step being computed (1 for the first step)
     // sound-speed constants computed on the host exactly as the reference does
     double invcs2, invcs4, invtwocs2, invtwocs4, cs2;
-    double invtau_u, invtau_t, guo_fac, nu, tau_u, Cs2;
+    double invtau_u, invtau_t, guo_fac, nu, tau_u, Cs2, kappa;
+    // velocity planes [3][N] of the step (double, or float for float storage),
+    // written by the velocity kernel for the finite-difference LES closure
+    const void *vel;
     // forcing
     double Fc[3];       // constant part (Poiseuille + channel), summed in reference order
     int hit;
@@ -184,8 +189,30 @@ struct FeedbackBufs {
 int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s);
 size_t feedback_sort_bytes(int64_t nitems, int key_bits);
 
-// nodes per tile (one block) of the batched step
-constexpr int BATCH_TILE = 256;
+// The batched step works in wave tiles of BATCH_WAVE lanes: up to 64 x of
+// one (y, z) row, so the row, its base state and its wall flags are uniform
+// across the wave as in the single-lattice kernel; or, for "dense" lattices
+// (dense_rows), 64 consecutive nodes in linear order, each lane decoding its
+// own row.  A block runs BATCH_WAVES_PER_BLOCK tiles.
+constexpr int BATCH_WAVE = 64;
+constexpr int BATCH_WAVES_PER_BLOCK = 4;
+
+// Lattices whose rows fill less than 90 % of the 64-lane waves that cover
+// them (NX = 78: 61 %) are stepped with one node per lane in linear order,
+// each lane decoding its own row, instead of row-aligned waves.
+inline bool dense_rows(int NX)
+{
+    const int64_t lanes = (int64_t)(NX + BATCH_WAVE - 1) / BATCH_WAVE * BATCH_WAVE;
+    return (int64_t)NX * 10 < lanes * 9;
+}
+
+// wave tiles of a lattice in the batched step
+inline int64_t batch_tiles(int NX, int NY, int NZ)
+{
+    const int64_t N = (int64_t)NX * NY * NZ;
+    if (dense_rows(NX)) return (N + BATCH_WAVE - 1) / BATCH_WAVE;
+    return (int64_t)NY * NZ * ((NX + BATCH_WAVE - 1) / BATCH_WAVE);
+}
 
 // One lattice of a batched step: its constants and its two state buffers.
 struct BatchEntry {
@@ -195,18 +222,21 @@ struct BatchEntry {
     int cur;            // buffer holding the state at the start of the chunk
 };
 // One lattice taking part in a batched launch: entry (bit 31: parity of
-// that lattice's step within the chunk) and its first block in the launch.
-// A launch's slots are sorted by tile0; the last one is a sentinel whose
-// tile0 is the launch's block count.
+// that lattice's step within the chunk) and its first wave tile in the
+// launch.  A launch's slots are sorted by tile0; the last one is a sentinel
+// whose tile0 is the launch's wave-tile count.
 struct BatchSlot {
     uint32_t entry;
     uint32_t tile0;
 };
 // one launch covers the lattices of a storage kind (FP32 float/double
 // arithmetic, FP64) listed in its slots
-void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int arith64,
-                       bool ntl, hipStream_t s);
-void launch_batch_fp64(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, hipStream_t s);
+// population sets of a batched launch (one kernel instance per kind)
+enum { BATCH_FLUID = 1, BATCH_THERMAL = 2, BATCH_FLUID_THERMAL = 3 };
+void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
+                       bool dense, int arith64, bool ntl, hipStream_t s);
+void launch_batch_fp64(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
+                       bool dense, hipStream_t s);
 
 // Kernel launchers (stream-ordered).
 void launch_step_fp64(const KParams &P, const double *fin, const double *gin, double *fout, double *gout,
@@ -218,13 +248,11 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,
                         double *gpost, double *dens, double *u, double *t, hipStream_t s);
 
-// Up to nsteps steady-state steps in one cooperative launch (grid barrier
-// between steps) when every block of the lattice can be resident; returns 1
-// when that does not hold (nothing launched), 0 on launch, negative on error.
-int launch_steps_coop_fp32(const KParams &P, float *f0, float *f1, float *g0, float *g1, float *tlag,
-                           float *taules, int cur, int nsteps, hipStream_t s);
-int launch_steps_coop_fp64(const KParams &P, double *f0, double *f1, double *g0, double *g1, double *tlag,
-                           double *taules, int cur, int nsteps, hipStream_t s);
+// Velocity of the step at every node into P.vel (the finite-difference LES
+// closure reads it at the six neighbours); same gather and moments as the
+// step kernel.
+void launch_velocity_fp64(const KParams &P, const double *fin, const double *tlag, hipStream_t s);
+void launch_velocity_fp32(const KParams &P, const float *fin, const float *tlag, hipStream_t s);
 
 // Initial state from a per-row fluid profile prow[NY][19] and a temperature
 // field t0[N]; restart load from FP64 SoA staging.  dev = FP32-deviation.
@@ -239,5 +267,8 @@ constexpr int NOUT = 26;
 int reduce_blocks(const KParams &P);
 void launch_averages(const KParams &P, const double *dens, const double *u, const double *t, double nu, double kappa,
                      double *partials, double *result, hipStream_t s);
+// plane sums of the same terms along x, y, z: [NX|NY|NZ][NOUT], null to skip
+void launch_rulers(const KParams &P, const double *dens, const double *u, const double *t, double nu, double kappa,
+                   double *rx, double *ry, double *rz, hipStream_t s);
 
 }  // namespace ch4g

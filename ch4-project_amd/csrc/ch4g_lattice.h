@@ -11,8 +11,6 @@ namespace ch4g {
 constexpr int HIT_RING = 8;
 // steps per captured graph (even: the graph starts and ends on the same buffer)
 constexpr int GRAPH_STEPS = 32;
-// steps per cooperative multi-step launch
-constexpr int COOP_STEPS = 64;
 
 // Lagrangian particles of one lattice (device SoA, see PartField).
 struct Particles {
@@ -51,6 +49,7 @@ struct ch4g_lattice {
     void *g[2] = {nullptr, nullptr};
     void *tlag = nullptr;
     void *taules = nullptr;
+    void *vel = nullptr;     // [3][N] velocity of the step (finite-difference LES)
     int cur = 0;             // buffer holding the current state
     int first = 1;           // state holds pre-collision populations (no gather)
     int64_t itime = 0;
@@ -70,14 +69,14 @@ struct ch4g_lattice {
     // lazily allocated FP64 scratch
     double *d_dens = nullptr, *d_u = nullptr, *d_t = nullptr;
     double *d_part = nullptr, *d_res = nullptr;
+    double *d_rul = nullptr;  // ruler plane sums [NX + NY + NZ][NOUT]
     double *d_fs = nullptr, *d_gs = nullptr;
     // FP32-deviation per-row base state
     float *d_row32 = nullptr;
     double *d_row64 = nullptr;
     // captured GRAPH_STEPS-step sequences, one per starting buffer
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int use_graphs = CH4G_LAUNCH_GRAPHS | CH4G_LAUNCH_COOP;
-    int coop_ok = 1;         // cooperative multi-step launches possible (all blocks resident)
+    int use_graphs = 1;
     int64_t bytes = 0;
     ch4g::Particles *part = nullptr;
 };

@@ -26,12 +26,13 @@ namespace {
 // has its own short slot list (lattice, parity of its step, first block).
 struct Batch {
     struct Group {
-        int dev32, arith64;
+        int dev32, arith64, kind;
+        bool dense;                       // lanes decode their own rows (dense_rows)
         bool ntl;
         int first, count;                 // range in the entry table
         std::vector<int64_t> off;         // per launch: first slot in the slot table
         std::vector<int> nslot;           // per launch: slots (without the sentinel)
-        std::vector<int64_t> nblk;        // per launch: blocks
+        std::vector<int64_t> nblk;        // per launch: wave tiles
     };
     std::vector<BatchEntry> host;
     std::vector<BatchSlot> slots;
@@ -41,6 +42,12 @@ struct Batch {
     BatchSlot *dev_slots = nullptr;
     size_t slots_cap = 0;
     hipStream_t stream = nullptr;
+    // one stream per group after the first: the groups' launches are
+    // independent (different lattices) and overlap, each group's tail
+    // filled by the others' blocks
+    std::vector<hipStream_t> gstream;
+    std::vector<hipEvent_t> gdone;
+    hipEvent_t fork = nullptr;
     std::vector<ch4g_lattice *> lats;
     std::vector<int> steps;   // steps of each lattice per chunk
     int chunk = 0;            // launches per chunk = max steps
@@ -64,6 +71,12 @@ struct Batch {
         slots_cap = 0;
         if (ev) (void)hipEventDestroy(ev);
         ev = nullptr;
+        if (fork) (void)hipEventDestroy(fork);
+        fork = nullptr;
+        for (hipEvent_t e : gdone) (void)hipEventDestroy(e);
+        gdone.clear();
+        for (hipStream_t t : gstream) (void)hipStreamDestroy(t);
+        gstream.clear();
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -75,10 +88,15 @@ int check_member(const ch4g_lattice *L)
     if (L->first || L->itime < 1) return fail(-1, "batched lattices must be past their first step (ch4g_step(lat, 1))");
     if (L->base.hit) return fail(-1, "HIT-forced lattices need per-step host tables and are not batched");
     if (L->part) return fail(-1, "lattices with particles are not batched");
+    if (L->base.les_fd) return fail(-1, "finite-difference LES lattices need a velocity pass and are not batched");
     return 0;
 }
 
-// Entry and block tables of the lattices, grouped by storage kind.
+// population sets of a lattice (BATCH_FLUID, BATCH_THERMAL or both)
+int kind_of(const ch4g_lattice *L) { return (L->base.fluid ? BATCH_FLUID : 0) | (L->base.thermal ? BATCH_THERMAL : 0); }
+
+// Entry and block tables of the lattices, grouped by storage kind and
+// population sets (one kernel instance per group).
 int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<int> &steps)
 {
     B.drop_graph();
@@ -97,7 +115,10 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
     const int K = B.chunk;
     std::vector<int> order(lats.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
-    auto key = [&](int i) { return (lats[(size_t)i]->dev32 << 1) | lats[(size_t)i]->base.arith64; };
+    auto key = [&](int i) {
+        const ch4g_lattice *L = lats[(size_t)i];
+        return (L->dev32 << 4) | (L->base.arith64 << 3) | ((int)dense_rows(L->cfg.NX) << 2) | kind_of(L);
+    };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
     for (size_t i = 0; i < order.size();) {
         Batch::Group G;
@@ -105,6 +126,8 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
         const ch4g_lattice *L0 = lats[(size_t)order[i]];
         G.dev32 = L0->dev32;
         G.arith64 = L0->base.arith64;
+        G.kind = kind_of(L0);
+        G.dense = dense_rows(L0->cfg.NX);
         G.first = (int)B.host.size();
         G.count = 0;
         int64_t bytes = 0;
@@ -128,7 +151,7 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
             std::vector<int> a((size_t)K, -1);
             for (int s_ = 0; s_ < st; s_++) a[(size_t)(((int64_t)s_ * K) / st)] = s_;
             at.push_back(a);
-            ntile.push_back((L->N + BATCH_TILE - 1) / BATCH_TILE);
+            ntile.push_back(batch_tiles(L->cfg.NX, L->cfg.NY, L->cfg.NZ));
             bytes += 2 * (int64_t)Q * L->N * (L->dev32 ? 4 : 8) * ((L->base.fluid ? 1 : 0) + (L->base.thermal ? 1 : 0));
             G.count++;
         }
@@ -154,6 +177,15 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
     }
     if (!B.stream) HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
     if (!B.ev) HIPCHK(hipEventCreateWithFlags(&B.ev, hipEventDisableTiming));
+    if (!B.fork) HIPCHK(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
+    while (B.gstream.size() + 1 < B.groups.size()) {
+        hipStream_t t = nullptr;
+        hipEvent_t e = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+        B.gstream.push_back(t);
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        B.gdone.push_back(e);
+    }
     // the previous tables may still be read by queued kernels
     HIPCHK(hipStreamSynchronize(B.stream));
     if (B.host.size() > B.dev_cap) {
@@ -180,20 +212,37 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
     return 0;
 }
 
-void launch_steps(Batch &B)
+// The chunk's launches: group 0 on the batch stream, group g > 0 on its own
+// stream forked from and joined back into the batch stream (under capture
+// this makes parallel branches of one graph).
+int launch_steps(Batch &B)
 {
-    for (int k = 0; k < B.chunk; k++)
-        for (const Batch::Group &G : B.groups) {
-            const int64_t nb = G.nblk[(size_t)k];
-            if (nb == 0) continue;
+    const size_t ng = B.groups.size();
+    if (ng > 1) {
+        HIPCHK(hipEventRecord(B.fork, B.stream));
+        for (size_t g = 1; g < ng; g++) HIPCHK(hipStreamWaitEvent(B.gstream[g - 1], B.fork, 0));
+    }
+    for (size_t g = 0; g < ng; g++) {
+        const Batch::Group &G = B.groups[g];
+        hipStream_t st = g == 0 ? B.stream : B.gstream[g - 1];
+        for (int k = 0; k < B.chunk; k++) {
+            const int64_t nt = G.nblk[(size_t)k];
+            if (nt == 0) continue;
+            const int64_t nb = (nt + BATCH_WAVES_PER_BLOCK - 1) / BATCH_WAVES_PER_BLOCK;
             const BatchEntry *tab = B.dev + G.first;
             const BatchSlot *sl = B.dev_slots + G.off[(size_t)k];
             const int ns = G.nslot[(size_t)k];
             if (G.dev32)
-                launch_batch_fp32(tab, sl, ns, nb, G.arith64, G.ntl, B.stream);
+                launch_batch_fp32(tab, sl, ns, nb, G.kind, G.dense, G.arith64, G.ntl, st);
             else
-                launch_batch_fp64(tab, sl, ns, nb, B.stream);
+                launch_batch_fp64(tab, sl, ns, nb, G.kind, G.dense, st);
         }
+    }
+    for (size_t g = 1; g < ng; g++) {
+        HIPCHK(hipEventRecord(B.gdone[g - 1], B.gstream[g - 1]));
+        HIPCHK(hipStreamWaitEvent(B.stream, B.gdone[g - 1], 0));
+    }
+    return 0;
 }
 
 // Advance every lattice of the table by its steps of one chunk (ordered
@@ -210,8 +259,12 @@ int advance(Batch &B, bool use_graph)
         if (!B.graph) {
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-            launch_steps(B);
+            int lrc = launch_steps(B);
             hipError_t e = hipStreamEndCapture(B.stream, &g);
+            if (lrc) {
+                if (g) (void)hipGraphDestroy(g);
+                return lrc;
+            }
             if (e != hipSuccess) return fail(-2, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
             e = hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
@@ -222,7 +275,8 @@ int advance(Batch &B, bool use_graph)
         }
         HIPCHK(hipGraphLaunch(B.graph, B.stream));
     } else {
-        launch_steps(B);
+        int lrc = launch_steps(B);
+        if (lrc) return lrc;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(B.ev, B.stream));

@@ -13,8 +13,6 @@
 //   df*   = df - (df - df_eq)/tau + (1 - 1/(2 tau)) w rho (3 (c-u).F + 9 (c.u)(c.F))
 // and the same for the temperature (T = T_b + dT, no forcing term).  Step
 // semantics (lag scalar, wall rules, force lag) are those of kernels_fp64.hip.
-#include <hip/hip_cooperative_groups.h>
-
 #include "ch4g_internal.h"
 
 namespace ch4g {
@@ -161,48 +159,111 @@ __device__ __forceinline__ A wa(int q)
     return q == 0 ? A(1) / A(3) : (q <= 6 ? A(1) / A(18) : A(1) / A(36));
 }
 
+// Moments of the gathered fluid populations (deviation form): density
+// deviation and density, momentum, and the velocity with the lagged half
+// force (hydro_fields, lb.c:420-540).
+template <typename A>
+struct Moments {
+    A fa[Q];
+    A drho, rho, jx, jy, jz, irho, ux, uy, uz;
+};
+
+template <typename A>
+__device__ __forceinline__ void fluid_moments(const KParams &P, const Row &R, int x, int y, int z, uint32_t n,
+                                              const float f[Q], float dTlag, Moments<A> &m)
+{
+    A *fa = m.fa;
+    for (int q = 0; q < Q; q++) fa[q] = (A)f[q];
+    if constexpr (sizeof(A) == 4) {
+        m.drho = sum19(f);
+    } else {
+        m.drho = 0;
+        for (int q = 0; q < Q; q++) m.drho += fa[q];
+    }
+    m.rho = (sizeof(A) == 4 ? (A)R.rb : (A)P.row64[2 * y]) + m.drho;
+    // momentum from (q, inv q) differences: exact for nearby values
+    m.jx = (fa[1] - fa[2]) + (fa[7] - fa[10]) + (fa[8] - fa[9]) + (fa[11] - fa[14]) + (fa[13] - fa[12]);
+    m.jy = (fa[3] - fa[4]) + (fa[7] - fa[10]) + (fa[9] - fa[8]) + (fa[15] - fa[18]) + (fa[16] - fa[17]);
+    m.jz = (fa[5] - fa[6]) + (fa[11] - fa[14]) + (fa[12] - fa[13]) + (fa[15] - fa[18]) + (fa[17] - fa[16]);
+    m.irho = A(1) / m.rho;
+    m.ux = m.jx * m.irho;
+    m.uy = m.jy * m.irho;
+    m.uz = m.jz * m.irho;
+    if (P.forced) {
+        float Fp[3];
+        force_at32(P, P.hit_prev, P.fb_prev, n, x, y, z, R.offT + dTlag, Fp);
+        m.ux += A(0.5) * Fp[0];
+        m.uy += A(0.5) * Fp[1];
+        m.uz += A(0.5) * Fp[2];
+    }
+}
+
+// tau_u_les, finite-difference branch (les.c:31-43): gradient_vector of the
+// stored velocity (centred, periodic images, one-sided second order at y
+// walls, size-one axes zeroed), |S| = sqrt(2 S_ij S_ij).
+template <typename A>
+__device__ __forceinline__ A les_tau_fd32(const KParams &P, const float *__restrict__ vel, int x, int y, int z,
+                                          uint32_t n)
+{
+    const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
+    const size_t N = (size_t)P.N;
+    const int NX = P.NX, NY = P.NY;
+    auto id = [&](int i, int j, int k) -> size_t { return ((size_t)k * NY + j) * NX + i; };
+    A G[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float *u = vel + c * N;
+        G[c][0] = A(0.5) * ((A)u[id(b.xp, y, z)] - (A)u[id(b.xm, y, z)]);
+        G[c][1] = A(0.5) * ((A)u[id(x, b.yp, z)] - (A)u[id(x, b.ym, z)]);
+        G[c][2] = A(0.5) * ((A)u[id(x, y, b.zp)] - (A)u[id(x, y, b.zm)]);
+        if (P.wall_f) {
+            if (b.bot) G[c][1] = A(-1.5) * (A)u[n] + A(2) * (A)u[id(x, 1, z)] - A(0.5) * (A)u[id(x, 2, z)];
+            if (b.top) G[c][1] = A(1.5) * (A)u[n] - A(2) * (A)u[id(x, NY - 2, z)] + A(0.5) * (A)u[id(x, NY - 3, z)];
+        }
+        if (P.NX == 1) G[c][0] = A(0);
+        if (NY == 1) G[c][1] = A(0);
+        if (P.NZ == 1) G[c][2] = A(0);
+    }
+    A ss = A(0);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const A e = G[c][d] + G[d][c];
+            ss += e * e;
+        }
+    return A(3) * (sqrt(A(0.5) * ss) * (A)P.Cs2 + (A)P.nu) + A(0.5);
+}
+
 // Fluid part of the step for one node: moments, velocity with the lagged
 // half force, optional LES, Guo-forced BGK in FMA form, restoration of the
 // collision invariants, non-temporal stores; the velocity used is returned
-// in ux, uy, uz.
+// in ux, uy, uz and the LES relaxation time (if any) in tau_les.
 template <typename A>
 __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, int x, int y, int z, uint32_t n,
                                               const float f[Q], float dTlag, A dT, float *__restrict__ fout,
-                                              float *__restrict__ taules, A &ux, A &uy, A &uz)
+                                              float *__restrict__ taules, A &ux, A &uy, A &uz, A &tau_les)
 {
     const size_t N = (size_t)P.N;
     {
-        A fa[Q];
-        for (int q = 0; q < Q; q++) fa[q] = (A)f[q];
-        A drho;
-        if constexpr (sizeof(A) == 4) {
-            drho = sum19(f);
-        } else {
-            drho = 0;
-            for (int q = 0; q < Q; q++) drho += fa[q];
-        }
-        const A rho = (sizeof(A) == 4 ? (A)R.rb : (A)P.row64[2 * y]) + drho;
-        // momentum from (q, inv q) differences: exact for nearby values
-        const A jx = (fa[1] - fa[2]) + (fa[7] - fa[10]) + (fa[8] - fa[9]) + (fa[11] - fa[14]) + (fa[13] - fa[12]);
-        const A jy = (fa[3] - fa[4]) + (fa[7] - fa[10]) + (fa[9] - fa[8]) + (fa[15] - fa[18]) + (fa[16] - fa[17]);
-        const A jz = (fa[5] - fa[6]) + (fa[11] - fa[14]) + (fa[12] - fa[13]) + (fa[15] - fa[18]) + (fa[17] - fa[16]);
-        const A irho = A(1) / rho;
-        ux = jx * irho;
-        uy = jy * irho;
-        uz = jz * irho;
-        if (P.forced) {
-            float Fp[3];
-            force_at32(P, P.hit_prev, P.fb_prev, n, x, y, z, R.offT + dTlag, Fp);
-            ux += A(0.5) * Fp[0];
-            uy += A(0.5) * Fp[1];
-            uz += A(0.5) * Fp[2];
-        }
+        Moments<A> m;
+        fluid_moments<A>(P, R, x, y, z, n, f, dTlag, m);
+        const A *fa = m.fa;
+        const A drho = m.drho, rho = m.rho, jx = m.jx, jy = m.jy, jz = m.jz, irho = m.irho;
+        ux = m.ux;
+        uy = m.uy;
+        uz = m.uz;
         const A u2 = ux * ux + uy * uy + uz * uz;
         // weight-class products (w rho, w drho) shared by the populations
         const A wr[3] = {wa<A>(0) * rho, wa<A>(1) * rho, wa<A>(7) * rho};
         const A wd[3] = {wa<A>(0) * drho, wa<A>(1) * drho, wa<A>(7) * drho};
         A omega = (A)P.invtau_u;
-        if (P.les) {
+        if (P.les && P.les_fd) {
+            const A tau = les_tau_fd32<A>(P, (const float *)P.vel, x, y, z, n);
+            taules[n] = (float)tau;
+            tau_les = tau;
+            omega = A(1) / tau;
+        } else if (P.les) {
             A xx = 0, yy = 0, xy = 0, xz = 0, yz = 0;
 #pragma unroll
             for (int q = 1; q < Q; q++) {
@@ -223,6 +284,7 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
             A fac = A(1) / (A(2) * (A)P.cs2 * rho * tau_prev);
             A tau = A(3) * (fac * sqrt(A(2) * ss) * (A)P.Cs2 + (A)P.nu) + A(0.5);
             taules[n] = (float)tau;
+            tau_les = tau;
             omega = A(1) / tau;
         }
         float Fn[3] = {0.f, 0.f, 0.f};
@@ -282,14 +344,17 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
 // exact conservation of the temperature, non-temporal stores.
 template <typename A>
 __device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, int y, uint32_t n, const float g[Q],
-                                                A dT, A ux, A uy, A uz, float *__restrict__ gout,
+                                                A dT, A ux, A uy, A uz, A tau_les, float *__restrict__ gout,
                                                 float *__restrict__ tlag)
 {
     const size_t N = (size_t)P.N;
     {
         const A u2 = ux * ux + uy * uy + uz * uz;
         const A T = (sizeof(A) == 4 ? (A)R.tb : (A)P.row64[2 * y + 1]) + dT;
-        const A om = (A)P.invtau_t;
+        // tau_t_les (les.c:273-297) with prandtl 1: tau_t = 3(kappa + nu_les) + 1/2
+        const A om = (P.les_t && P.fluid) ? A(1) / (A(3) * ((A)P.kappa + (tau_les - A(0.5) - A(3) * (A)P.nu) / A(3)) +
+                                                     A(0.5))
+                                          : (A)P.invtau_t;
         const A wT[3] = {wa<A>(0) * T, wa<A>(1) * T, wa<A>(7) * T};
         const A wdT[3] = {wa<A>(0) * dT, wa<A>(1) * dT, wa<A>(7) * dT};
         A r[Q];
@@ -338,89 +403,109 @@ __device__ __forceinline__ void step_node32(const KParams &P, const float *__res
             for (int q = 0; q < Q; q++) dT += (A)g[q];
         }
     }
-    A ux = 0, uy = 0, uz = 0;
-    if constexpr (FLUID) collide_fluid<A>(P, R, x, y, z, n, f, dTlag, dT, fout, taules, ux, uy, uz);
-    if constexpr (THERMAL) collide_thermal<A>(P, R, y, n, g, dT, ux, uy, uz, gout, tlag);
+    A ux = 0, uy = 0, uz = 0, tau_les = A(0);
+    if constexpr (FLUID) collide_fluid<A>(P, R, x, y, z, n, f, dTlag, dT, fout, taules, ux, uy, uz, tau_les);
+    if constexpr (THERMAL) collide_thermal<A>(P, R, y, n, g, dT, ux, uy, uz, tau_les, gout, tlag);
 }
 
-template <bool FIRST, bool FLUID, bool THERMAL, typename A, bool NTL>
+// Row grid (blocks along x of one (y, z) row: row quantities block-uniform)
+// or, DENSE, a linear grid with one node per lane (see dense_rows).
+template <bool FIRST, bool FLUID, bool THERMAL, typename A, bool NTL, bool DENSE>
 __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__restrict__ fin,
                                                    const float *__restrict__ gin, float *__restrict__ fout,
                                                    float *__restrict__ gout, float *__restrict__ tlag,
                                                    float *__restrict__ taules)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= P.NX) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    int x, y, z;
+    uint32_t n;
+    if constexpr (DENSE) {
+        n = blockIdx.x * blockDim.x + threadIdx.x;
+        if (n >= (uint32_t)P.N) return;
+        const uint32_t row = n / (uint32_t)P.NX;
+        x = (int)(n - row * (uint32_t)P.NX);
+        y = (int)(row % (uint32_t)P.NY);
+        z = (int)(row / (uint32_t)P.NY);
+    } else {
+        x = blockIdx.x * blockDim.x + threadIdx.x;
+        if (x >= P.NX) return;
+        y = blockIdx.y;
+        z = blockIdx.z;
+        n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    }
     step_node32<FIRST, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
 }
 
-// Several steps of one lattice in one cooperative launch: every block is
-// resident (hipLaunchCooperativeKernel refuses the launch otherwise), each
-// runs the same fixed number of steps and meets the others at a grid
-// barrier between steps, so the launch gap between steps disappears for
-// lattices that fit the GPU in one wave of blocks.
-template <bool FLUID, bool THERMAL, typename A, bool NTL>
-__global__ void __launch_bounds__(256) k_steps_coop32(KParams P, float *f0, float *f1, float *g0, float *g1,
-                                                      float *tlag, float *taules, int cur, int nsteps)
+// Velocity of the step at every node (the u[] that add_collision reads,
+// fluid.c:1056-1080), for the finite-difference LES closure.
+template <bool FIRST, typename A>
+__global__ void __launch_bounds__(256) k_velocity32(KParams P, const float *__restrict__ fin,
+                                                    const float *__restrict__ tlag)
 {
-    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
-    const bool node = x < P.NX;
-    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + (node ? x : 0);
-#pragma unroll 1
-    for (int s = 0; s < nsteps; s++) {
-        const int in = cur ^ (s & 1);
-        const float *fin = in ? f1 : f0, *gin = in ? g1 : g0;
-        float *fout = in ? f0 : f1, *gout = in ? g0 : g1;
-        if (node) step_node32<false, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
-        grid.sync();
-    }
+    if (x >= P.NX) return;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    const size_t N = (size_t)P.N;
+    const Row R = row_of(P, y);
+    const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
+    float f[Q];
+    gather_f32<FIRST, false>(P, R, b, fin, n, f);
+    const float dTlag = P.thermal ? tlag[n] : 0.f;
+    Moments<A> m;
+    fluid_moments<A>(P, R, x, y, z, n, f, dTlag, m);
+    float *vel = (float *)P.vel;
+    vel[n] = (float)m.ux;
+    vel[N + n] = (float)m.uy;
+    vel[2 * N + n] = (float)m.uz;
 }
 
-// Many lattices in one launch: block b advances BATCH_TILE consecutive nodes
-// of the lattice whose block range in this launch holds b (block-uniform
-// binary search over the launch's few slots), with that lattice's step
-// parity (tile -> lattice map), whatever its population
-// sets (a block-uniform dispatch), so small and large lattices of every kind
-// share one launch per step; the state buffer of each lattice alternates
-// with the step parity.
-template <typename A, bool NTL>
+// Many lattices in one launch: each wave advances one wave tile (up to 64
+// consecutive x of one row) of the lattice whose tile range in this launch
+// holds it (wave-uniform binary search over the launch's few slots), with
+// that lattice's step parity; the state buffer of each lattice alternates
+// with the step parity.  One instance per population-set kind.
+template <bool FLUID, bool THERMAL, typename A, bool NTL, bool DENSE>
 __global__ void __launch_bounds__(256) k_step_batch32(const BatchEntry *__restrict__ tab,
                                                       const BatchSlot *__restrict__ slots, int nslots)
 {
+    const uint32_t wt = __builtin_amdgcn_readfirstlane(blockIdx.x * BATCH_WAVES_PER_BLOCK + threadIdx.x / BATCH_WAVE);
+    if (wt >= slots[nslots].tile0) return;
     int lo = 0, hi = nslots - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (slots[mid].tile0 <= blockIdx.x)
+        if (slots[mid].tile0 <= wt)
             lo = mid;
         else
             hi = mid - 1;
     }
     const BatchSlot sl = slots[lo];
     const BatchEntry &E = tab[sl.entry & 0x7fffffffu];
-    const KParams P = E.P;
+    const KParams &P = E.P;
     const int in = E.cur ^ (int)(sl.entry >> 31);
     const float *fin = (const float *)E.f[in], *gin = (const float *)E.g[in];
     float *fout = (float *)E.f[in ^ 1], *gout = (float *)E.g[in ^ 1];
     float *tl = (float *)E.tlag, *tau = (float *)E.taules;
-    const int64_t base = (int64_t)(blockIdx.x - sl.tile0) * BATCH_TILE;
-#pragma unroll 1
-    for (int it = 0; it < BATCH_TILE / 256; it++) {
-        const int64_t m = base + it * 256 + threadIdx.x;
-        if (m >= P.N) return;
-        const uint32_t n = (uint32_t)m;
-        const uint32_t x = n % (uint32_t)P.NX, r = n / (uint32_t)P.NX;
-        const uint32_t y = r % (uint32_t)P.NY, z = r / (uint32_t)P.NY;
-        if (P.fluid && P.thermal)
-            step_node32<false, true, true, A, NTL>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
-        else if (P.fluid)
-            step_node32<false, true, false, A, NTL>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
-        else
-            step_node32<false, false, true, A, NTL>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
+    // DENSE: wave tile t = nodes 64 t .. 64 t + 63 of the lattice, each lane
+    // decoding its row; else tile t = up to 64 x of one row (row
+    // coordinates wave-uniform), ceil(NX / 64) tiles per row
+    const uint32_t t = wt - sl.tile0, lane = threadIdx.x % BATCH_WAVE;
+    uint32_t n, x, y, z;
+    if constexpr (DENSE) {
+        n = t * BATCH_WAVE + lane;
+        if (n >= (uint32_t)P.N) return;
+        const uint32_t row = n / (uint32_t)P.NX;
+        x = n - row * (uint32_t)P.NX;
+        y = row % (uint32_t)P.NY;
+        z = row / (uint32_t)P.NY;
+    } else {
+        const uint32_t xt = ((uint32_t)P.NX + BATCH_WAVE - 1) / BATCH_WAVE, row = t / xt;
+        x = (t - row * xt) * BATCH_WAVE + lane;
+        if (x >= (uint32_t)P.NX) return;
+        y = row % (uint32_t)P.NY;
+        z = row / (uint32_t)P.NY;
+        n = row * (uint32_t)P.NX + x;
     }
+    step_node32<false, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
 }
 
 __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__restrict__ fin,
@@ -499,18 +584,30 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
                       float *taules, hipStream_t s)
 {
     int bx = block_x32(P);
-    dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ);
+    const bool dense = !P.first && dense_rows(P.NX);
+    dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ), block(bx);
+    if (dense) {
+        grid = dim3((unsigned)((P.N + 255) / 256));
+        block = dim3(256);
+    }
     // both lattices resident in the 256 MB last-level cache?
     const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
     const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
-#define CH4G_LAUNCH32_(F_, FL_, TH_, A_)                                                                                 \
+#define CH4G_LAUNCH32D_(F_, FL_, TH_, A_, D_)                                                                          \
     do {                                                                                                              \
         if (ntl)                                                                                                      \
-            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, true>), grid, dim3(bx), 0, s, P, fin, gin, fout, gout, \
-                               tlag, taules);                                                                         \
-        else                                                                                                          \
-            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, false>), grid, dim3(bx), 0, s, P, fin, gin, fout,      \
+            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, true, D_>), grid, block, 0, s, P, fin, gin, fout,      \
                                gout, tlag, taules);                                                                   \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, false, D_>), grid, block, 0, s, P, fin, gin, fout,     \
+                               gout, tlag, taules);                                                                   \
+    } while (0)
+#define CH4G_LAUNCH32_(F_, FL_, TH_, A_)                                                                              \
+    do {                                                                                                              \
+        if (!(F_) && dense)                                                                                           \
+            CH4G_LAUNCH32D_(false, FL_, TH_, A_, true);                                                               \
+        else                                                                                                          \
+            CH4G_LAUNCH32D_(F_, FL_, TH_, A_, false);                                                                 \
     } while (0)
 #define CH4G_LAUNCH32(F_, FL_, TH_)                                                                                  \
     do {                                                                                                              \
@@ -528,60 +625,62 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
     }
 #undef CH4G_LAUNCH32
 #undef CH4G_LAUNCH32_
+#undef CH4G_LAUNCH32D_
 }
 
 // Cooperative multi-step launch when the lattice's blocks can all be
 // resident; returns 1 if it cannot be used (the caller launches per step).
-int launch_steps_coop_fp32(const KParams &P, float *f0, float *f1, float *g0, float *g1, float *tlag,
-                           float *taules, int cur, int nsteps, hipStream_t s)
+void launch_velocity_fp32(const KParams &P, const float *fin, const float *tlag, hipStream_t s)
 {
     int bx = block_x32(P);
-    dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ), block(bx);
-    const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
-    const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
-    const void *fn = nullptr;
-#define CH4G_COOP_(FL_, TH_, A_)                                                                                      \
-    fn = ntl ? (const void *)&k_steps_coop32<FL_, TH_, A_, true> : (const void *)&k_steps_coop32<FL_, TH_, A_, false>
-#define CH4G_COOP(FL_, TH_)                                                                                           \
-    do {                                                                                                              \
-        if (P.arith64)                                                                                                \
-            CH4G_COOP_(FL_, TH_, double);                                                                             \
-        else                                                                                                          \
-            CH4G_COOP_(FL_, TH_, float);                                                                              \
-    } while (0)
-    if (P.fluid && P.thermal)
-        CH4G_COOP(true, true);
-    else if (P.fluid)
-        CH4G_COOP(true, false);
-    else
-        CH4G_COOP(false, true);
-#undef CH4G_COOP
-#undef CH4G_COOP_
-    int dev = 0, per_cu = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bx, 0) != hipSuccess) return 1;
-    if ((int64_t)grid.x * grid.y * grid.z > (int64_t)per_cu * cus) return 1;
-    KParams Pk = P;
-    void *args[] = {&Pk, &f0, &f1, &g0, &g1, &tlag, &taules, &cur, &nsteps};
-    return hipLaunchCooperativeKernel(fn, grid, block, args, 0, s) == hipSuccess ? 0 : -2;
+    dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ);
+    if (P.arith64) {
+        if (P.first)
+            hipLaunchKernelGGL((k_velocity32<true, double>), grid, dim3(bx), 0, s, P, fin, tlag);
+        else
+            hipLaunchKernelGGL((k_velocity32<false, double>), grid, dim3(bx), 0, s, P, fin, tlag);
+    } else {
+        if (P.first)
+            hipLaunchKernelGGL((k_velocity32<true, float>), grid, dim3(bx), 0, s, P, fin, tlag);
+        else
+            hipLaunchKernelGGL((k_velocity32<false, float>), grid, dim3(bx), 0, s, P, fin, tlag);
+    }
 }
 
-void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int arith64,
-                       bool ntl, hipStream_t s)
+void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
+                       bool dense, int arith64, bool ntl, hipStream_t s)
 {
     dim3 grid((unsigned)nblocks), block(256);
-    if (arith64) {
-        if (ntl)
-            hipLaunchKernelGGL((k_step_batch32<double, true>), grid, block, 0, s, tab, slots, nslots);
-        else
-            hipLaunchKernelGGL((k_step_batch32<double, false>), grid, block, 0, s, tab, slots, nslots);
-    } else {
-        if (ntl)
-            hipLaunchKernelGGL((k_step_batch32<float, true>), grid, block, 0, s, tab, slots, nslots);
-        else
-            hipLaunchKernelGGL((k_step_batch32<float, false>), grid, block, 0, s, tab, slots, nslots);
-    }
+#define CH4G_BATCH32D_(FL_, TH_, A_, D_)                                                                              \
+    do {                                                                                                              \
+        if (ntl)                                                                                                      \
+            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, true, D_>), grid, block, 0, s, tab, slots, nslots);    \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, false, D_>), grid, block, 0, s, tab, slots, nslots);   \
+    } while (0)
+#define CH4G_BATCH32_(FL_, TH_, A_)                                                                                   \
+    do {                                                                                                              \
+        if (dense)                                                                                                    \
+            CH4G_BATCH32D_(FL_, TH_, A_, true);                                                                       \
+        else                                                                                                          \
+            CH4G_BATCH32D_(FL_, TH_, A_, false);                                                                      \
+    } while (0)
+#define CH4G_BATCH32(FL_, TH_)                                                                                        \
+    do {                                                                                                              \
+        if (arith64)                                                                                                  \
+            CH4G_BATCH32_(FL_, TH_, double);                                                                          \
+        else                                                                                                          \
+            CH4G_BATCH32_(FL_, TH_, float);                                                                           \
+    } while (0)
+    if (kind == BATCH_FLUID_THERMAL)
+        CH4G_BATCH32(true, true);
+    else if (kind == BATCH_FLUID)
+        CH4G_BATCH32(true, false);
+    else
+        CH4G_BATCH32(false, true);
+#undef CH4G_BATCH32
+#undef CH4G_BATCH32_
+#undef CH4G_BATCH32D_
 }
 
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,

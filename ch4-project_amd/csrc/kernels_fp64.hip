@@ -13,8 +13,6 @@
 // rho, u, T of step s-1 (lb.c:420-540; the u shift uses the force built from
 // the lagged temperature, as main.c's ordering implies), then collides
 // (fluid.c:1033-1225) and adds the Guo term (forcing.c:1424-1435) of step s.
-#include <hip/hip_cooperative_groups.h>
-
 #include "ch4g_internal.h"
 
 namespace ch4g {
@@ -158,6 +156,68 @@ __device__ __forceinline__ double les_tau(const KParams &P, const double f[Q], d
     return 3.0 * (nu_les + P.nu) + 0.5;
 }
 
+// tau_u_les, default finite-difference branch (les.c:31-43, :160-180, :253):
+// gradient_vector of the stored velocity (gradients.c:78-499: centred
+// differences with periodic images, one-sided second order at y walls,
+// components along a size-one axis zeroed), |S| = sqrt(2 S_ij S_ij) summed
+// in the reference's term order.  tens.ab = d u_a / d x_b.
+__device__ __forceinline__ double les_tau_fd(const KParams &P, const double *__restrict__ vel, const Nbr &b,
+                                             uint32_t n)
+{
+    const size_t N = (size_t)P.N;
+    const int NX = P.NX, NY = P.NY;
+    auto id = [&](int i, int j, int k) -> size_t { return ((size_t)k * NY + j) * NX + i; };
+    double G[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double *u = vel + c * N;
+        G[c][0] = (u[id(b.xp, b.y, b.z)] - u[id(b.xm, b.y, b.z)]) / 2.0;
+        G[c][1] = (u[id(b.x, b.yp, b.z)] - u[id(b.x, b.ym, b.z)]) / 2.0;
+        G[c][2] = (u[id(b.x, b.y, b.zp)] - u[id(b.x, b.y, b.zm)]) / 2.0;
+        if (P.wall_f) {
+            if (b.bot) G[c][1] = -1.5 * u[n] + 2.0 * u[id(b.x, 1, b.z)] + -0.5 * u[id(b.x, 2, b.z)];
+            if (b.top) G[c][1] = 1.5 * u[n] + -2.0 * u[id(b.x, NY - 2, b.z)] + 0.5 * u[id(b.x, NY - 3, b.z)];
+        }
+        if (P.NX == 1) G[c][0] = 0.0;
+        if (NY == 1) G[c][1] = 0.0;
+        if (P.NZ == 1) G[c][2] = 0.0;
+    }
+    const double xx = G[0][0], xy = G[0][1], xz = G[0][2];
+    const double yx = G[1][0], yy = G[1][1], yz = G[1][2];
+    const double zx = G[2][0], zy = G[2][1], zz = G[2][2];
+    double s_norm = sqrt(0.5 * ((xx + xx) * (xx + xx) + (xy + yx) * (xy + yx) + (xz + zx) * (xz + zx) +
+                                (yx + xy) * (yx + xy) + (yy + yy) * (yy + yy) + (yz + zy) * (yz + zy) +
+                                (zx + xz) * (zx + xz) + (zy + yz) * (zy + yz) + (zz + zz) * (zz + zz)));
+    double nu_les = s_norm * P.Cs2;
+    return 3.0 * (nu_les + P.nu) + 0.5;
+}
+
+// tau_t_les (les.c:273-297, METHOD_STREAMING, prandtl = 1.0)
+__device__ __forceinline__ double les_tau_t(const KParams &P, double tau_les)
+{
+    const double prandtl = 1.0;
+    return 3.0 * (P.kappa + (tau_les - 0.5 - 3 * P.nu) / (3 * prandtl)) + 0.5;
+}
+
+// hydro_fields of the previous step for one node (lb.c:420-540): density and
+// velocity with the half force built from the lagged temperature.
+__device__ __forceinline__ void fluid_velocity64(const KParams &P, const double f[Q], uint32_t n, int x, int y, int z,
+                                                 double Tlag, double &rho, double &ux, double &uy, double &uz)
+{
+    rho = m_sum(f);
+    if (P.forced) {
+        double Fp[3];
+        force_at(P, P.hit_prev, P.fb_prev, n, x, y, z, Tlag, Fp);
+        ux = m_x(f) / rho + 0.5 * 1.0 * Fp[0];
+        uy = m_y(f) / rho + 0.5 * 1.0 * Fp[1];
+        uz = m_z(f) / rho + 0.5 * 1.0 * Fp[2];
+    } else {
+        ux = m_x(f) / rho;
+        uy = m_y(f) / rho;
+        uz = m_z(f) / rho;
+    }
+}
+
 // One node of the fused step (shared by the single-lattice and the batched
 // kernels, so a lattice advances bit-identically in either).
 template <bool FIRST, bool FLUID, bool THERMAL>
@@ -179,26 +239,21 @@ __device__ __forceinline__ void step_node64(const KParams &P, const double *__re
     if constexpr (THERMAL) T = m_sum(g);
 
     double ux = 0.0, uy = 0.0, uz = 0.0;
+    double tau_les = 0.0;
     if constexpr (FLUID) {
-        const double rho = m_sum(f);
-        if (P.forced) {
-            double Fp[3];
-            force_at(P, P.hit_prev, P.fb_prev, n, x, y, z, Tlag, Fp);
-            ux = m_x(f) / rho + 0.5 * 1.0 * Fp[0];
-            uy = m_y(f) / rho + 0.5 * 1.0 * Fp[1];
-            uz = m_z(f) / rho + 0.5 * 1.0 * Fp[2];
-        } else {
-            ux = m_x(f) / rho;
-            uy = m_y(f) / rho;
-            uz = m_z(f) / rho;
-        }
+        double rho;
+        fluid_velocity64(P, f, n, x, y, z, Tlag, rho, ux, uy, uz);
         const double u2 = (ux * ux + uy * uy + uz * uz);
         double invtau = P.invtau_u;
         if (P.les) {
-            double tau_prev = (P.itime == 1) ? P.tau_u : taules[n];
-            double tau = les_tau(P, f, rho, ux, uy, uz, u2, tau_prev);
-            taules[n] = tau;
-            invtau = 1.0 / tau;
+            if (P.les_fd) {
+                tau_les = les_tau_fd(P, (const double *)P.vel, b, n);
+            } else {
+                double tau_prev = (P.itime == 1) ? P.tau_u : taules[n];
+                tau_les = les_tau(P, f, rho, ux, uy, uz, u2, tau_prev);
+            }
+            taules[n] = tau_les;
+            invtau = 1.0 / tau_les;
         }
         double Fn[3] = {0.0, 0.0, 0.0};
         if (P.forced) force_at(P, P.hit_now, P.fb_now, n, x, y, z, T, Fn);
@@ -223,83 +278,111 @@ __device__ __forceinline__ void step_node64(const KParams &P, const double *__re
 
     if constexpr (THERMAL) {
         const double u2 = (ux * ux + uy * uy + uz * uz);
+        double invtau_t = P.invtau_t;
+        if constexpr (FLUID)
+            if (P.les_t) invtau_t = 1.0 / les_tau_t(P, tau_les);
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             double r = g[q];
-            r += -P.invtau_t * (g[q] - feq_q(P, q, T, ux, uy, uz, u2));
+            r += -invtau_t * (g[q] - feq_q(P, q, T, ux, uy, uz, u2));
             gout[(size_t)q * N + n] = r;
         }
         tlag[n] = T;
     }
 }
 
-template <bool FIRST, bool FLUID, bool THERMAL>
+// Row grid or, DENSE, a linear grid with one node per lane (see dense_rows).
+template <bool FIRST, bool FLUID, bool THERMAL, bool DENSE>
 __global__ void __launch_bounds__(256) k_step_fp64(KParams P, const double *__restrict__ fin,
                                                    const double *__restrict__ gin, double *__restrict__ fout,
                                                    double *__restrict__ gout, double *__restrict__ tlag,
                                                    double *__restrict__ taules)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= P.NX) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    int x, y, z;
+    uint32_t n;
+    if constexpr (DENSE) {
+        n = blockIdx.x * blockDim.x + threadIdx.x;
+        if (n >= (uint32_t)P.N) return;
+        const uint32_t row = n / (uint32_t)P.NX;
+        x = (int)(n - row * (uint32_t)P.NX);
+        y = (int)(row % (uint32_t)P.NY);
+        z = (int)(row / (uint32_t)P.NY);
+    } else {
+        x = blockIdx.x * blockDim.x + threadIdx.x;
+        if (x >= P.NX) return;
+        y = blockIdx.y;
+        z = blockIdx.z;
+        n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    }
     step_node64<FIRST, FLUID, THERMAL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
 }
 
-// Several steps of one lattice in one cooperative launch (see k_steps_coop32).
-template <bool FLUID, bool THERMAL>
-__global__ void __launch_bounds__(256) k_steps_coop64(KParams P, double *f0, double *f1, double *g0, double *g1,
-                                                      double *tlag, double *taules, int cur, int nsteps)
+// Velocity of the step at every node (the u[] that add_collision reads,
+// fluid.c:1056-1080), for the finite-difference LES closure.
+template <bool FIRST>
+__global__ void __launch_bounds__(256) k_velocity64(KParams P, const double *__restrict__ fin,
+                                                    const double *__restrict__ tlag)
 {
-    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
-    const bool node = x < P.NX;
-    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + (node ? x : 0);
-#pragma unroll 1
-    for (int s = 0; s < nsteps; s++) {
-        const int in = cur ^ (s & 1);
-        const double *fin = in ? f1 : f0, *gin = in ? g1 : g0;
-        double *fout = in ? f0 : f1, *gout = in ? g0 : g1;
-        if (node) step_node64<false, FLUID, THERMAL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
-        grid.sync();
-    }
+    if (x >= P.NX) return;
+    const uint32_t n = ((uint32_t)z * P.NY + y) * P.NX + x;
+    const size_t N = (size_t)P.N;
+    const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
+    double f[Q];
+    gather_fluid<FIRST>(P, b, fin, n, f);
+    const double Tlag = P.thermal ? tlag[n] : 0.0;
+    double rho, ux, uy, uz;
+    fluid_velocity64(P, f, n, x, y, z, Tlag, rho, ux, uy, uz);
+    double *vel = (double *)P.vel;
+    vel[n] = ux;
+    vel[N + n] = uy;
+    vel[2 * N + n] = uz;
 }
 
 // Many lattices in one launch (see k_step_batch32).
+template <bool FLUID, bool THERMAL, bool DENSE>
 __global__ void __launch_bounds__(256) k_step_batch64(const BatchEntry *__restrict__ tab,
                                                       const BatchSlot *__restrict__ slots, int nslots)
 {
+    const uint32_t wt = __builtin_amdgcn_readfirstlane(blockIdx.x * BATCH_WAVES_PER_BLOCK + threadIdx.x / BATCH_WAVE);
+    if (wt >= slots[nslots].tile0) return;
     int lo = 0, hi = nslots - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (slots[mid].tile0 <= blockIdx.x)
+        if (slots[mid].tile0 <= wt)
             lo = mid;
         else
             hi = mid - 1;
     }
     const BatchSlot sl = slots[lo];
     const BatchEntry &E = tab[sl.entry & 0x7fffffffu];
-    const KParams P = E.P;
+    const KParams &P = E.P;
     const int in = E.cur ^ (int)(sl.entry >> 31);
     const double *fin = (const double *)E.f[in], *gin = (const double *)E.g[in];
     double *fout = (double *)E.f[in ^ 1], *gout = (double *)E.g[in ^ 1];
     double *tl = (double *)E.tlag, *tau = (double *)E.taules;
-    const int64_t base = (int64_t)(blockIdx.x - sl.tile0) * BATCH_TILE;
-#pragma unroll 1
-    for (int it = 0; it < BATCH_TILE / 256; it++) {
-        const int64_t m = base + it * 256 + threadIdx.x;
-        if (m >= P.N) return;
-        const uint32_t n = (uint32_t)m;
-        const uint32_t x = n % (uint32_t)P.NX, r = n / (uint32_t)P.NX;
-        const uint32_t y = r % (uint32_t)P.NY, z = r / (uint32_t)P.NY;
-        if (P.fluid && P.thermal)
-            step_node64<false, true, true>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
-        else if (P.fluid)
-            step_node64<false, true, false>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
-        else
-            step_node64<false, false, true>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
+    // DENSE: wave tile t = nodes 64 t .. 64 t + 63 of the lattice, each lane
+    // decoding its row; else tile t = up to 64 x of one row (row
+    // coordinates wave-uniform), ceil(NX / 64) tiles per row
+    const uint32_t t = wt - sl.tile0, lane = threadIdx.x % BATCH_WAVE;
+    uint32_t n, x, y, z;
+    if constexpr (DENSE) {
+        n = t * BATCH_WAVE + lane;
+        if (n >= (uint32_t)P.N) return;
+        const uint32_t row = n / (uint32_t)P.NX;
+        x = n - row * (uint32_t)P.NX;
+        y = row % (uint32_t)P.NY;
+        z = row / (uint32_t)P.NY;
+    } else {
+        const uint32_t xt = ((uint32_t)P.NX + BATCH_WAVE - 1) / BATCH_WAVE, row = t / xt;
+        x = (t - row * xt) * BATCH_WAVE + lane;
+        if (x >= (uint32_t)P.NX) return;
+        y = row % (uint32_t)P.NY;
+        z = row / (uint32_t)P.NY;
+        n = row * (uint32_t)P.NX + x;
     }
+    step_node64<false, FLUID, THERMAL>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
 }
 
 // Streaming of the last step + hydro_fields, without a new collision:
@@ -366,9 +449,21 @@ void launch_step_fp64(const KParams &P, const double *fin, const double *gin, do
                       double *tlag, double *taules, hipStream_t s)
 {
     int bx = block_x(P);
-    dim3 grid = grid_of(P, bx);
-#define CH4G_LAUNCH64(F_, FL_, TH_) \
-    hipLaunchKernelGGL((k_step_fp64<F_, FL_, TH_>), grid, dim3(bx), 0, s, P, fin, gin, fout, gout, tlag, taules)
+    const bool dense = !P.first && dense_rows(P.NX);
+    dim3 grid = grid_of(P, bx), block(bx);
+    if (dense) {
+        grid = dim3((unsigned)((P.N + 255) / 256));
+        block = dim3(256);
+    }
+#define CH4G_LAUNCH64(F_, FL_, TH_)                                                                                   \
+    do {                                                                                                              \
+        if (!(F_) && dense)                                                                                           \
+            hipLaunchKernelGGL((k_step_fp64<false, FL_, TH_, true>), grid, block, 0, s, P, fin, gin, fout, gout,     \
+                               tlag, taules);                                                                         \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step_fp64<F_, FL_, TH_, false>), grid, block, 0, s, P, fin, gin, fout, gout, tlag, \
+                               taules);                                                                               \
+    } while (0)
     if (P.fluid && P.thermal) {
         if (P.first) CH4G_LAUNCH64(true, true, true); else CH4G_LAUNCH64(false, true, true);
     } else if (P.fluid) {
@@ -379,31 +474,34 @@ void launch_step_fp64(const KParams &P, const double *fin, const double *gin, do
 #undef CH4G_LAUNCH64
 }
 
-int launch_steps_coop_fp64(const KParams &P, double *f0, double *f1, double *g0, double *g1, double *tlag,
-                           double *taules, int cur, int nsteps, hipStream_t s)
+void launch_velocity_fp64(const KParams &P, const double *fin, const double *tlag, hipStream_t s)
 {
     int bx = block_x(P);
-    dim3 grid = grid_of(P, bx), block(bx);
-    const void *fn;
-    if (P.fluid && P.thermal)
-        fn = (const void *)&k_steps_coop64<true, true>;
-    else if (P.fluid)
-        fn = (const void *)&k_steps_coop64<true, false>;
+    dim3 grid = grid_of(P, bx);
+    if (P.first)
+        hipLaunchKernelGGL(k_velocity64<true>, grid, dim3(bx), 0, s, P, fin, tlag);
     else
-        fn = (const void *)&k_steps_coop64<false, true>;
-    int dev = 0, per_cu = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, bx, 0) != hipSuccess) return 1;
-    if ((int64_t)grid.x * grid.y * grid.z > (int64_t)per_cu * cus) return 1;
-    KParams Pk = P;
-    void *args[] = {&Pk, &f0, &f1, &g0, &g1, &tlag, &taules, &cur, &nsteps};
-    return hipLaunchCooperativeKernel(fn, grid, block, args, 0, s) == hipSuccess ? 0 : -2;
+        hipLaunchKernelGGL(k_velocity64<false>, grid, dim3(bx), 0, s, P, fin, tlag);
 }
 
-void launch_batch_fp64(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, hipStream_t s)
+void launch_batch_fp64(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
+                       bool dense, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_batch64, dim3((unsigned)nblocks), dim3(256), 0, s, tab, slots, nslots);
+    dim3 grid((unsigned)nblocks), block(256);
+#define CH4G_BATCH64(FL_, TH_)                                                                                        \
+    do {                                                                                                              \
+        if (dense)                                                                                                    \
+            hipLaunchKernelGGL((k_step_batch64<FL_, TH_, true>), grid, block, 0, s, tab, slots, nslots);            \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step_batch64<FL_, TH_, false>), grid, block, 0, s, tab, slots, nslots);           \
+    } while (0)
+    if (kind == BATCH_FLUID_THERMAL)
+        CH4G_BATCH64(true, true);
+    else if (kind == BATCH_FLUID)
+        CH4G_BATCH64(true, false);
+    else
+        CH4G_BATCH64(false, true);
+#undef CH4G_BATCH64
 }
 
 void launch_gather_fp64(const KParams &P, const double *fin, const double *gin, const double *tlag, double *fpost,

@@ -52,7 +52,8 @@ enum ch4g_flag {
     CH4G_TEMPERATURE_INITIAL_LINEAR    = 1u << 12, /* initial_conditions.c:303-308 */
     CH4G_TEMPERATURE_INITIAL_ADD_NOISE = 1u << 13, /* initial_conditions.c:421-424 */
     CH4G_FLUID_INITIAL_UNIT_DENSITY    = 1u << 14, /* initial_conditions.c:256-279 */
-    CH4G_INITIAL_BAROMETRIC            = 1u << 15  /* initial_conditions.c:72-84 */
+    CH4G_INITIAL_BAROMETRIC            = 1u << 15, /* initial_conditions.c:72-84 */
+    CH4G_TEMPERATURE_LES               = 1u << 16  /* tau_t from tau_u_les, les.c:273-297 (needs FLUID_LES) */
 };
 
 /* Storage / arithmetic of the populations.  Reductions (Nusselt, energy,
@@ -146,6 +147,16 @@ int ch4g_hydro_fields(ch4g_lattice *lat, double *dens, double *u, double *t);
  * including the Nusselt numbers.  Synchronous.                               */
 int ch4g_dump_averages(ch4g_lattice *lat, ch4g_output *out);
 
+/* dump_averages rulers (output.c:212-491, normalisation :520-690): averages
+ * over the y-z planes (rx[NX]), x-z planes (ry[NY]) and x-y planes (rz[NZ])
+ * of the same members, with the reference's normalisation factors (its
+ * temperature members of the z ruler are scaled by 1/(SY SZ), as the
+ * reference does).  Each ruler's own coordinate member holds the plane
+ * centre, the two other coordinate members are 0.  Any pointer may be
+ * NULL.  These are the rows of velocity_averages_{x,y,z}.dat and
+ * temperature_averages_{x,y,z}.dat.  Synchronous.                           */
+int ch4g_dump_rulers(ch4g_lattice *lat, ch4g_output *rx, ch4g_output *ry, ch4g_output *rz);
+
 /* HIT forcing phases phi_u[6][3] of the current step (forcing.c:49-79).
  * get/set let a host that owns the random stream drive the phase walk.      */
 int ch4g_get_hit_phases(ch4g_lattice *lat, double *phi);
@@ -157,17 +168,12 @@ int ch4g_set_hit_phases(ch4g_lattice *lat, const double *phi);
  * (parallel.c:618-638, run_time.dat "time per step and grid point").      */
 int ch4g_step_timed(ch4g_lattice *lat, int nsteps, double *elapsed_ms);
 
-/* How steady-state steps (after the first, without per-step host forcing
- * tables (HIT) or particles) are launched; flags, default both:
- *   CH4G_LAUNCH_GRAPHS  replay captured 32-step HIP graphs;
- *   CH4G_LAUNCH_COOP    run up to 64 steps in one cooperative launch with a
- *                       grid barrier between steps, when every block of the
- *                       lattice can be resident at once (small lattices,
- *                       where per-launch gaps dominate); else graphs.
- * 0 launches every step on its own.  Results are identical in every mode.
- * No reference counterpart (launch mechanics of this library).             */
-enum { CH4G_LAUNCH_GRAPHS = 1, CH4G_LAUNCH_COOP = 2 };
-int ch4g_set_graphs(ch4g_lattice *lat, int flags);
+/* Replay steady-state steps from captured HIP graphs (default on).  Graphs
+ * hold GRAPH_STEPS steps and are used once the first step has run, for runs
+ * without per-step host forcing tables (HIT) or particles; results are
+ * identical either way.  No reference counterpart (launch mechanics of this
+ * library).                                                                 */
+int ch4g_set_graphs(ch4g_lattice *lat, int enable);
 
 /* Wait for all work queued on the lattice stream. */
 int ch4g_synchronize(ch4g_lattice *lat);

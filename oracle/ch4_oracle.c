@@ -339,12 +339,46 @@ static double tau_u_les(oracle *o, size_t n)
     return tau;
 }
 
+static oten gradient_vector(const oracle *o, const ovec *t, int i, int j, int k);
+
+/* tau_u_les, default finite-difference branch (les.c:31-43): |S| from the
+ * gradient_vector of u (centred, one-sided second order at y walls), then
+ * the same eddy viscosity and relaxation time as above.                    */
+static double tau_u_les_fd(oracle *o, int i, int j, int k)
+{
+    oten g = gradient_vector(o, o->u, i, j, k);
+    double s_norm = sqrt(0.5 * ((g.xx + g.xx) * (g.xx + g.xx) + (g.xy + g.yx) * (g.xy + g.yx) +
+                                (g.xz + g.zx) * (g.xz + g.zx) + (g.yx + g.xy) * (g.yx + g.xy) +
+                                (g.yy + g.yy) * (g.yy + g.yy) + (g.yz + g.zy) * (g.yz + g.zy) +
+                                (g.zx + g.xz) * (g.zx + g.xz) + (g.zy + g.yz) * (g.zy + g.yz) +
+                                (g.zz + g.zz) * (g.zz + g.zz)));
+    double vol = 1.0;
+    double nu_les = s_norm * pow(o->cfg.les_cs * vol, 2.0);
+    double tau = 3.0 * (nu_les + o->nu) + 0.5;
+    o->tau_les[IDX(o, i, j, k)] = tau;
+    return tau;
+}
+
+/* tau_t_les (les.c:273-297, METHOD_STREAMING branch, prandtl forced to 1):
+ * the temperature relaxation time from this step's tau_u_les_total.       */
+static double tau_t_les(const oracle *o, size_t n)
+{
+    double prandtl = o->nu / o->kappa;
+    prandtl = 1.0;
+    double tau_les = o->tau_les[n];
+    return 3.0 * (o->kappa + (tau_les - 0.5 - 3 * o->nu) / (3 * prandtl)) + 0.5;
+}
+
 /* compute_advection, streaming branch with dt = 1 (fluid.c:267-284):
  * rhs <- f, followed by add_collision SRT (fluid.c:1033-1225):
- * rhs += -(1/tau)(f - f_eq).                                                */
+ * rhs += -(1/tau)(f - f_eq), with the per-node LES relaxation times of
+ * fluid.c:1070-1088 (u halo first, fluid.c:1056-1058).                     */
 static void advection_collision(oracle *o, const opop *f, opop *rhs, double tau, char which)
 {
     const int les = (which == 'p') && HAS(o, CH4G_FLUID_LES);
+    const int les_fd = les && !HAS(o, CH4G_FLUID_LES_LOCAL_STRAIN);
+    const int les_t = (which == 'g') && HAS(o, CH4G_FLUID_LES) && HAS(o, CH4G_TEMPERATURE_LES);
+    if (HAS(o, CH4G_FLUID_LES)) halo_vec(o, o->u);
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = BRD; k < o->NZ + BRD; k++)
         for (int j = BRD; j < o->NY + BRD; j++)
@@ -352,7 +386,8 @@ static void advection_collision(oracle *o, const opop *f, opop *rhs, double tau,
                 size_t n = IDX(o, i, j, k);
                 rhs[n] = f[n];
                 double invtau = 1.0 / tau;
-                if (les) invtau = 1.0 / tau_u_les(o, n);
+                if (les) invtau = 1.0 / (les_fd ? tau_u_les_fd(o, i, j, k) : tau_u_les(o, n));
+                if (les_t) invtau = 1.0 / tau_t_les(o, n);
                 opop e = equilibrium(o, f, n);
                 for (int q = 0; q < NP; q++) {
                     double fcoll = -invtau * (f[n].p[q] - e.p[q]);
@@ -744,100 +779,147 @@ static ovec gradient_scalar(const oracle *o, const double *t, int i, int j, int 
     return g;
 }
 
-/* dump_averages global averages (output.c:63-620): FP64 sums over interior
- * nodes in i,j,k loop order of the fields of the current step, normalised by
- * SX*SY*SZ, Nusselt numbers by kappa*deltaT/SY (:577-579).                 */
-int oracle_dump_averages(oracle *o, ch4g_output *out)
+/* Per-node terms of dump_averages (output.c:183-491) at interior node
+ * (i,j,k), vol = 1: coordinates, u, u^2, rho, rho^2, kinetic energy,
+ * dissipation from gradient_vector, temperature, its gradient, u T and the
+ * Nusselt integrands -kappa dT + u T.  Fields must have their halos.      */
+static void node_terms(const oracle *o, int i, int j, int k, ch4g_output *v)
 {
-    ch4g_output a;
+    memset(v, 0, sizeof *v);
+    const int fluid = HAS(o, CH4G_LB_FLUID), temp = HAS(o, CH4G_LB_TEMPERATURE);
+    size_t n = IDX(o, i, j, k);
+    double vol = 1.0;
+    if (fluid) {
+        v->x = (double)(i - BRD) + 0.5;
+        v->y = (double)(j - BRD) + 0.5;
+        v->z = (double)(k - BRD) + 0.5;
+        v->ux = o->u[n].x * vol;
+        v->uy = o->u[n].y * vol;
+        v->uz = o->u[n].z * vol;
+        v->ux2 = o->u[n].x * o->u[n].x * vol;
+        v->uy2 = o->u[n].y * o->u[n].y * vol;
+        v->uz2 = o->u[n].z * o->u[n].z * vol;
+        v->rho = o->dens[n] * vol;
+        v->rho2 = o->dens[n] * o->dens[n] * vol;
+        v->ene = 0.5 * (o->u[n].x * o->u[n].x + o->u[n].y * o->u[n].y + o->u[n].z * o->u[n].z) * vol;
+        oten G = gradient_vector(o, o->u, i, j, k);
+        v->eps = ((G.xx + G.xx) * (G.xx + G.xx) + (G.xy + G.yx) * (G.xy + G.yx) + (G.xz + G.zx) * (G.xz + G.zx) +
+                  (G.yx + G.xy) * (G.yx + G.xy) + (G.yy + G.yy) * (G.yy + G.yy) + (G.yz + G.zy) * (G.yz + G.zy) +
+                  (G.zx + G.xz) * (G.zx + G.xz) + (G.zy + G.yz) * (G.zy + G.yz) + (G.zz + G.zz) * (G.zz + G.zz)) *
+                 0.5 * o->nu * vol;
+    }
+    if (temp) {
+        ovec gt = gradient_scalar(o, o->t, i, j, k);
+        v->dxt = gt.x * vol;
+        v->dyt = gt.y * vol;
+        v->dzt = gt.z * vol;
+        v->uxt = o->u[n].x * o->t[n] * vol;
+        v->uyt = o->u[n].y * o->t[n] * vol;
+        v->uzt = o->u[n].z * o->t[n] * vol;
+        v->nux = (-o->kappa * v->dxt + v->uxt);
+        v->nuy = (-o->kappa * v->dyt + v->uyt);
+        v->nuz = (-o->kappa * v->dzt + v->uzt);
+        v->t = o->t[n] * vol;
+        v->t2 = o->t[n] * o->t[n] * vol;
+        v->epst = o->kappa * (gt.x * gt.x + gt.y * gt.y + gt.z * gt.z) * vol;
+    }
+}
+
+/* r += v, member by member (the reference's += per field) */
+static void add_terms(ch4g_output *r, const ch4g_output *v)
+{
+    double *a = (double *)r;
+    const double *b = (const double *)v;
+    for (size_t m = 0; m < sizeof *r / sizeof(double); m++) a[m] += b[m];
+}
+
+/* normalisation of output.c:499-690: fluid members by fnorm, temperature
+ * members by tnorm, Nusselt numbers by tnorm / (kappa deltaT / SY), lb from
+ * the normalised eps and epst with buoyancy.                               */
+static void normalise(const oracle *o, ch4g_output *a, double fnorm, double tnorm)
+{
+    if (HAS(o, CH4G_LB_FLUID)) {
+        a->ene *= fnorm;
+        a->rho *= fnorm;
+        a->ux *= fnorm;
+        a->uy *= fnorm;
+        a->uz *= fnorm;
+        a->ux2 *= fnorm;
+        a->uy2 *= fnorm;
+        a->uz2 *= fnorm;
+        a->eps *= fnorm;
+        a->rho2 *= fnorm;
+    }
+    if (HAS(o, CH4G_LB_TEMPERATURE)) {
+        double nn = tnorm / (o->kappa * o->deltaT / (double)o->NY);
+        a->t *= tnorm;
+        a->t2 *= tnorm;
+        a->epst *= tnorm;
+        a->dxt *= tnorm;
+        a->dyt *= tnorm;
+        a->dzt *= tnorm;
+        a->uxt *= tnorm;
+        a->uyt *= tnorm;
+        a->uzt *= tnorm;
+        a->nux *= nn;
+        a->nuy *= nn;
+        a->nuz *= nn;
+        a->lb = 0.0;
+        if (HAS(o, CH4G_TEMPERATURE_BUOYANCY))
+            a->lb = pow(a->eps, 5. / 4.) * pow(a->epst, -3. / 4.) * pow(o->cfg.gravity_y * o->cfg.beta_t, -3. / 2.);
+    }
+}
+
+/* dump_averages (output.c:63-690): FP64 sums over interior nodes in i,j,k
+ * loop order of the fields of the current step; global averages normalised
+ * by SX*SY*SZ, Nusselt numbers by kappa*deltaT/SY (:577-579); the x, y, z
+ * rulers (plane sums, :212-491) when rx/ry/rz are given, normalised by the
+ * plane sizes with the reference's factors: fluid 1/(SY SZ), 1/(SX SZ),
+ * 1/(SX SY); temperature 1/(SY SZ), 1/(SX SZ) and, for z, 1/(SY SZ)
+ * (:585-650); the ruler's own coordinate by 1/(NY NZ) etc. and the two
+ * other coordinates left 0.                                               */
+int oracle_dump_rulers(oracle *o, ch4g_output *out, ch4g_output *rx, ch4g_output *ry, ch4g_output *rz)
+{
+    ch4g_output a, v;
     memset(&a, 0, sizeof a);
     const int fluid = HAS(o, CH4G_LB_FLUID), temp = HAS(o, CH4G_LB_TEMPERATURE);
+    if (rx) memset(rx, 0, sizeof(ch4g_output) * (size_t)o->NX);
+    if (ry) memset(ry, 0, sizeof(ch4g_output) * (size_t)o->NY);
+    if (rz) memset(rz, 0, sizeof(ch4g_output) * (size_t)o->NZ);
     if (fluid) halo_vec(o, o->u);
     if (temp) halo_scal(o, o->t);
     for (int i = BRD; i < o->NX + BRD; i++)
         for (int j = BRD; j < o->NY + BRD; j++)
             for (int k = BRD; k < o->NZ + BRD; k++) {
-                size_t n = IDX(o, i, j, k);
-                double vol = 1.0;
-                if (fluid) {
-                    double x, y, z, ux, uy, uz, ux2, uy2, uz2, rho, rho2, ene, eps;
-                    a.x += x = (double)(i - BRD) + 0.5;
-                    a.y += y = (double)(j - BRD) + 0.5;
-                    a.z += z = (double)(k - BRD) + 0.5;
-                    (void)x; (void)y; (void)z;
-                    a.ux += ux = o->u[n].x * vol;
-                    a.uy += uy = o->u[n].y * vol;
-                    a.uz += uz = o->u[n].z * vol;
-                    (void)ux; (void)uy; (void)uz;
-                    ux2 = o->u[n].x * o->u[n].x * vol;
-                    a.ux2 += ux2;
-                    uy2 = o->u[n].y * o->u[n].y * vol;
-                    a.uy2 += uy2;
-                    uz2 = o->u[n].z * o->u[n].z * vol;
-                    a.uz2 += uz2;
-                    a.rho += rho = o->dens[n] * vol;
-                    (void)rho;
-                    a.rho2 += rho2 = o->dens[n] * o->dens[n] * vol;
-                    (void)rho2;
-                    a.ene += ene = 0.5 * (o->u[n].x * o->u[n].x + o->u[n].y * o->u[n].y + o->u[n].z * o->u[n].z) * vol;
-                    (void)ene;
-                    oten G = gradient_vector(o, o->u, i, j, k);
-                    eps = ((G.xx + G.xx) * (G.xx + G.xx) + (G.xy + G.yx) * (G.xy + G.yx) + (G.xz + G.zx) * (G.xz + G.zx) +
-                           (G.yx + G.xy) * (G.yx + G.xy) + (G.yy + G.yy) * (G.yy + G.yy) + (G.yz + G.zy) * (G.yz + G.zy) +
-                           (G.zx + G.xz) * (G.zx + G.xz) + (G.zy + G.yz) * (G.zy + G.yz) + (G.zz + G.zz) * (G.zz + G.zz)) *
-                          0.5 * o->nu * vol;
-                    a.eps += eps;
-                }
-                if (temp) {
-                    ovec gt = gradient_scalar(o, o->t, i, j, k);
-                    double dxt = gt.x * vol, dyt = gt.y * vol, dzt = gt.z * vol;
-                    a.dxt += dxt;
-                    a.dyt += dyt;
-                    a.dzt += dzt;
-                    double uxt, uyt, uzt;
-                    a.uxt += uxt = o->u[n].x * o->t[n] * vol;
-                    a.uyt += uyt = o->u[n].y * o->t[n] * vol;
-                    a.uzt += uzt = o->u[n].z * o->t[n] * vol;
-                    a.nux += (-o->kappa * dxt + uxt);
-                    a.nuy += (-o->kappa * dyt + uyt);
-                    a.nuz += (-o->kappa * dzt + uzt);
-                    double tt;
-                    a.t += tt = o->t[n] * vol;
-                    (void)tt;
-                    a.t2 += o->t[n] * o->t[n] * vol;
-                    a.epst += o->kappa * (gt.x * gt.x + gt.y * gt.y + gt.z * gt.z) * vol;
-                }
+                node_terms(o, i, j, k, &v);
+                add_terms(&a, &v);
+                if (rx) add_terms(&rx[i - BRD], &v);
+                if (ry) add_terms(&ry[j - BRD], &v);
+                if (rz) add_terms(&rz[k - BRD], &v);
             }
-    double norm = 1.0 / ((double)o->NX * (double)o->NY * (double)o->NZ);
-    if (fluid) {
-        a.ene *= norm;
-        a.rho *= norm;
-        a.ux *= norm;
-        a.uy *= norm;
-        a.uz *= norm;
-        a.ux2 *= norm;
-        a.uy2 *= norm;
-        a.uz2 *= norm;
-        a.eps *= norm;
-        a.rho2 *= norm;
+    const double SX = o->NX, SY = o->NY, SZ = o->NZ;
+    normalise(o, &a, 1.0 / (SX * SY * SZ), 1.0 / (SX * SY * SZ));
+    if (!temp) a.lb = 0.0;
+    if (out) *out = a;
+    for (int i = 0; rx && i < o->NX; i++) {
+        double c = rx[i].x * (1.0 / (double)(o->NY * o->NZ));
+        normalise(o, &rx[i], 1.0 / (SY * SZ), 1.0 / (SY * SZ));
+        rx[i].x = fluid ? c : 0.0;
+        rx[i].y = rx[i].z = 0.0;
     }
-    if (temp) {
-        double nn = norm / (o->kappa * o->deltaT / (double)o->NY);
-        a.t *= norm;
-        a.t2 *= norm;
-        a.epst *= norm;
-        a.dxt *= norm;
-        a.dyt *= norm;
-        a.dzt *= norm;
-        a.uxt *= norm;
-        a.uyt *= norm;
-        a.uzt *= norm;
-        a.nux *= nn;
-        a.nuy *= nn;
-        a.nuz *= nn;
-        if (HAS(o, CH4G_TEMPERATURE_BUOYANCY))
-            a.lb = pow(a.eps, 5. / 4.) * pow(a.epst, -3. / 4.) * pow(o->cfg.gravity_y * o->cfg.beta_t, -3. / 2.);
+    for (int j = 0; ry && j < o->NY; j++) {
+        double c = ry[j].y * (1.0 / (double)(o->NX * o->NZ));
+        normalise(o, &ry[j], 1.0 / (SX * SZ), 1.0 / (SX * SZ));
+        ry[j].y = fluid ? c : 0.0;
+        ry[j].x = ry[j].z = 0.0;
     }
-    *out = a;
+    for (int k = 0; rz && k < o->NZ; k++) {
+        double c = rz[k].z * (1.0 / (double)(o->NX * o->NY));
+        normalise(o, &rz[k], 1.0 / (SX * SY), 1.0 / (SY * SZ));
+        rz[k].z = fluid ? c : 0.0;
+        rz[k].x = rz[k].y = 0.0;
+    }
     return 0;
 }
+
+int oracle_dump_averages(oracle *o, ch4g_output *out) { return oracle_dump_rulers(o, out, NULL, NULL, NULL); }

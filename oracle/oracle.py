@@ -39,6 +39,7 @@ def lib():
         L.oracle_hydro_fields.argtypes = [P, dp, dp, dp, dp]
         L.oracle_get_hit_phases.argtypes = [P, dp]
         L.oracle_dump_averages.argtypes = [P, P]
+        L.oracle_dump_rulers.argtypes = [P, P, P, P, P]
         L.oracle_srand48.argtypes = [P, C.c_long]
         L.oracle_drand48.argtypes = [P]
         L.oracle_drand48.restype = C.c_double
@@ -147,6 +148,15 @@ class Oracle:
         out = Output()
         self._L.oracle_dump_averages(self._h, C.byref(out))
         return out.as_dict()
+
+    def dump_rulers(self):
+        """(global averages dict, x ruler [NX], y ruler [NY], z ruler [NZ]);
+        rulers are numpy structured arrays with the output members."""
+        from ch4g.abi import Output, output_dtype
+        out = Output()
+        r = [np.zeros(n, dtype=output_dtype()) for n in (self.cfg.NX, self.cfg.NY, self.cfg.NZ)]
+        self._L.oracle_dump_rulers(self._h, C.byref(out), *[x.ctypes.data_as(C.c_void_p) for x in r])
+        return (out.as_dict(),) + tuple(r)
 
 
 class RNG:

@@ -34,10 +34,13 @@ def rb2d(NX=512, NY=256, NZ=1, Ra=1e8, Pr=1.0, Uff=0.05, precision=abi.FP64, see
 
 def hit(N=32, tau=0.5015, amp=1e-5, les_cs=0.1, precision=abi.FP64, seed=4321, les=True):
     """Forced homogeneous isotropic turbulence, periodic, 6 modes |k|^2 <= 2,
-    optional local-strain Smagorinsky."""
+    optional Smagorinsky closure: les=True or "local" for the local-strain
+    form, "fd" for the finite-difference form (the reference's default)."""
     flags = abi.LB_FLUID | abi.FLUID_FORCING_HIT
     if les:
-        flags |= abi.FLUID_LES | abi.FLUID_LES_LOCAL_STRAIN
+        flags |= abi.FLUID_LES
+        if les != "fd":
+            flags |= abi.FLUID_LES_LOCAL_STRAIN
     return abi.make_config(N, N, N, flags, precision=precision, seed=seed, tau_u=tau, Amp_x=amp, Amp_y=amp,
                            Amp_z=amp, les_cs=les_cs)
 

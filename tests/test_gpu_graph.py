@@ -9,17 +9,13 @@ import configs
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("coop", [False, True])
 @pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_DEV])
-def test_graph_replay_matches_eager(precision, coop):
-    """Graph replay alone, and cooperative multi-step launches (the lattice
-    fits the GPU at once) with graphs behind them."""
+def test_graph_replay_matches_eager(precision):
     cfg = configs.rb2d(NX=64, NY=32, Ra=1e6, precision=precision)
     eager = Lattice(cfg)
     eager.set_graphs(False)
     eager.initial_conditions()
     graph = Lattice(cfg)
-    graph.set_graphs(True, coop=coop)
     graph.initial_conditions()
     for rep in range(4):
         n = 70 if rep % 2 == 0 else 64  # 2 graphs + 6 eager steps / exactly 2 graphs
@@ -32,15 +28,13 @@ def test_graph_replay_matches_eager(precision, coop):
         graph.dump_averages()  # eager work between replays
 
 
-@pytest.mark.parametrize("coop", [False, True])
-def test_graph_replay_les_matches_eager(coop):
+def test_graph_replay_les_matches_eager():
     cfg = configs.poiseuille(8, 16, 8, tau=0.52, g=1e-5)
     cfg.flags |= abi.FLUID_LES | abi.FLUID_LES_LOCAL_STRAIN
     eager = Lattice(cfg)
     eager.set_graphs(False)
     eager.initial_conditions()
     graph = Lattice(cfg)
-    graph.set_graphs(True, coop=coop)
     graph.initial_conditions()
     for _ in range(3):
         eager.step(65)
@@ -51,19 +45,20 @@ def test_graph_replay_les_matches_eager(coop):
 
 
 @pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_DEV])
-def test_coop_fluid_only_odd_counts(precision):
-    """Cooperative launches of a fluid-only channel over odd step counts,
-    which leave the state in the other buffer, and a count below two."""
-    cfg = configs.poiseuille(12, 20, 6, tau=0.8, g=1e-5, precision=precision)
+def test_graph_replay_odd_counts_fd_les(precision):
+    """Graph replay of a fluid-only channel with the finite-difference LES
+    closure (two kernels per step) over step counts that leave the state in
+    either buffer."""
+    cfg = configs.poiseuille(12, 20, 6, tau=0.52, g=1e-5, precision=precision)
+    cfg.flags |= abi.FLUID_LES
     eager = Lattice(cfg)
     eager.set_graphs(False)
     eager.initial_conditions()
-    coop = Lattice(cfg)
-    coop.set_graphs(False, coop=True)
-    coop.initial_conditions()
-    for n in (1, 5, 67, 1, 3):
+    graph = Lattice(cfg)
+    graph.initial_conditions()
+    for n in (1, 33, 67, 1, 64):
         eager.step(n)
-        coop.step(n)
+        graph.step(n)
         pe, _ = eager.get_populations()
-        pc, _ = coop.get_populations()
-        assert np.array_equal(pe, pc), n
+        pg, _ = graph.get_populations()
+        assert np.array_equal(pe, pg), n

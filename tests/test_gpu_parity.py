@@ -79,6 +79,38 @@ def test_hit_les_fp64_matches_oracle():
     compare_fp64(o, lat, exact=False)
 
 
+def test_hit_les_fd_fp64_matches_oracle():
+    """HIT with the finite-difference Smagorinsky closure (the reference's
+    default LB_FLUID_LES): a velocity kernel stores u, the step kernel
+    differentiates it at the six neighbours."""
+    cfg = configs.hit(N=16, tau=0.505, amp=1e-4, les="fd")
+    o, lat = both(cfg, 100)
+    compare_fp64(o, lat, exact=False)
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.FP32_DEV])
+def test_rb2d_les_fd_walls_temperature_les(precision):
+    """Finite-difference LES with one-sided differences at the y walls and
+    the temperature relaxation time following it (LB_TEMPERATURE_LES)."""
+    def with_les(c):
+        c.flags |= abi.FLUID_LES | abi.TEMPERATURE_LES
+        c.les_cs = 0.18
+        return c
+
+    o = Oracle(with_les(configs.rb2d(NX=64, NY=32, Ra=1e7)))
+    lat = Lattice(with_les(configs.rb2d(NX=64, NY=32, Ra=1e7, precision=precision)))
+    for x in (o, lat):
+        x.initial_conditions()
+        x.step(300)
+    if precision == abi.FP64:
+        compare_fp64(o, lat, exact=False)
+    else:
+        ho, hg = o.hydro_fields(), lat.hydro_fields()
+        eu, et = rel_l2(hg["u"], ho["u"]), rel_l2(hg["t"], ho["t"])
+        print(f"  L2 rel err u {eu:.3e}  T {et:.3e}")
+        assert eu <= 1e-5 and et <= 1e-5
+
+
 def test_dump_averages_fp64_matches_oracle():
     o, lat = both(configs.rb2d(NX=96, NY=48, Ra=1e6), 300)
     ao, ag = o.dump_averages(), lat.dump_averages()
@@ -88,6 +120,21 @@ def test_dump_averages_fp64_matches_oracle():
         scale = max(abs(v), 1e-300)
         print(f"  {k}: oracle {v:.15e} gpu {ag[k]:.15e}")
         assert abs(ag[k] - v) <= 1e-11 * scale + 1e-16, k  # floor: cancelling sums (periodic means ~ 0)
+
+
+@pytest.mark.parametrize("cfg,steps", [(configs.rb2d(NX=96, NY=48, Ra=1e6), 300),
+                                       (configs.poiseuille(12, 16, 6, tau=0.7, g=1e-5), 50)])
+def test_dump_rulers_fp64_matches_oracle(cfg, steps):
+    """x, y, z rulers (plane averages of dump_averages) against the oracle's
+    sequential plane sums; bar 1e-11 of each column's scale."""
+    o, lat = both(cfg, steps)
+    _, ox, oy, oz = o.dump_rulers()
+    gx, gy, gz = lat.dump_rulers()
+    for axis, a, b in (("x", gx, ox), ("y", gy, oy), ("z", gz, oz)):
+        for m in b.dtype.names:
+            scale = np.abs(b[m]).max()
+            err = np.abs(a[m] - b[m]).max()
+            assert err <= 1e-11 * scale + 1e-16, (axis, m, err, scale)
 
 
 def test_set_get_populations_roundtrip():
@@ -109,6 +156,7 @@ def test_set_get_populations_roundtrip():
     ("rb2d 512x256 Ra=1e8", configs.rb2d(precision=abi.FP32_DEV), 1000),
     ("rb2d 128x64 Ra=6.25e6 store", configs.rb2d(NX=128, NY=64, Ra=6.25e6, precision=abi.FP32_STORE), 1000),
     ("rb2d 512x256 Ra=1e8 store", configs.rb2d(precision=abi.FP32_STORE), 1000),
+    ("hit 32^3 fd-les", configs.hit(N=32, tau=0.505, amp=1e-4, les="fd", precision=abi.FP32_DEV), 200),
 ])
 def test_fp32_deviation_against_fp64_oracle(name, cfg, steps):
     """Relative L2 error per field of the FP32-deviation path against the

@@ -65,6 +65,10 @@ def test_step_batch_subset_and_rejections():
     fresh.initial_conditions()
     with pytest.raises(Exception, match="first step"):
         step_batch([fresh], 2)
+    les = workloads.channel3d(16, 16, 8, precision=abi.FP32_DEV)
+    les.flags |= abi.FLUID_LES
+    with pytest.raises(Exception, match="finite-difference LES"):
+        step_batch([started(les)], 2)
 
 
 def small_jobs(n=6, seed=5):
