@@ -95,18 +95,14 @@ def cpu_leg(target_s=12.0):
                       f"un-fused reference-structured sweeps, OpenMP {threads} threads"}
 
 
-def serve_leg(njobs, seed=2026):
-    """Served ensemble: the request stream of workloads.ensemble (Poisson
-    arrivals at 20 jobs/s, 2-D RB and 3-D channel jobs of log-uniform sizes
-    and step counts 1e3..1e5, FP32-deviation) through ch4g_serve, batched
-    (continuous batching) and one job at a time on the plain path."""
+def _serve_both(make_jobs):
+    """ch4g_serve on the same request stream, batched (continuous batching,
+    lock-step chunks) and one job at a time on the plain path."""
     import numpy as np
     from ch4g import abi, serve, workloads
-    res = {"workload": f"{njobs} jobs of the served-ensemble stream (workloads.ensemble seed {seed}): Poisson "
-                       "20 jobs/s, 2-D Rayleigh-Benard 64x64..1024x512 Ra 1e6..1e9 and 3-D channel 64^3..192^3, "
-                       "steps log-uniform 1e3..1e5, FP32-deviation"}
+    res = {}
     for name, mode in (("batched", abi.SERVE_BATCHED), ("sequential", abi.SERVE_SEQUENTIAL)):
-        done = serve(workloads.ensemble(njobs, seed=seed), mode=mode)
+        done = serve(make_jobs(), mode=mode)
         ok = [j for j in done if j.status == 0]
         work = sum(workloads.job_work(j) for j in ok)
         span = max(j.finish_s for j in ok)
@@ -116,6 +112,25 @@ def serve_leg(njobs, seed=2026):
                      "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "jobs_done": len(ok),
                      "node_updates": work}
     res["speedup"] = round(res["batched"]["mlups"] / res["sequential"]["mlups"], 3)
+    return res
+
+
+def serve_leg(njobs, seed=2026, sweep_jobs=48):
+    """Served ensembles through ch4g_serve.  (1) The request stream of
+    workloads.ensemble: Poisson arrivals at 20 jobs/s, 2-D RB and 3-D channel
+    jobs of log-uniform sizes and step counts 1e3..1e5, FP32-deviation (most
+    of its work is in lattices large enough to stream at HBM speed alone).
+    (2) A parameter sweep (workloads.sweep): a burst of small 2-D RB jobs,
+    each too small to fill the GPU alone."""
+    from ch4g import workloads
+    res = _serve_both(lambda: workloads.ensemble(njobs, seed=seed))
+    res["workload"] = (f"{njobs} jobs of the served-ensemble stream (workloads.ensemble seed {seed}): Poisson "
+                       "20 jobs/s, 2-D Rayleigh-Benard 64x64..1024x512 Ra 1e6..1e9 and 3-D channel 64^3..192^3, "
+                       "steps log-uniform 1e3..1e5, FP32-deviation")
+    sw = _serve_both(lambda: workloads.sweep(sweep_jobs))
+    sw["workload"] = (f"{sweep_jobs} jobs of a parameter sweep (workloads.sweep seed 7): burst at 200 jobs/s of "
+                      "2-D Rayleigh-Benard 64x32..256x128, Ra 1e5..1e7, steps log-uniform 2e3..2e4, FP32-deviation")
+    res["sweep"] = sw
     return res
 
 

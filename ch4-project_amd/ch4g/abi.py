@@ -184,5 +184,5 @@ class ServeParams(C.Structure):
     ]
 
 
-SCHEDULE_PROPORTIONAL = 0
-SCHEDULE_LOCKSTEP = 1
+SCHEDULE_LOCKSTEP = 0
+SCHEDULE_PROPORTIONAL = 1

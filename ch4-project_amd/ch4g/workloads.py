@@ -67,6 +67,29 @@ def ensemble(njobs, seed=2026, rate=20.0, steps=(1e3, 1e5), precision=abi.FP32_D
     return jobs
 
 
+def sweep(njobs, seed=7, rate=200.0, steps=(2e3, 2e4), precision=abi.FP32_DEV):
+    """Parameter-sweep request stream: many small 2-D Rayleigh-Benard jobs
+    (grids 64x32, 96x48, 128x64, 192x96 or 256x128, Ra log-uniform in
+    1e5..1e7, steps log-uniform in `steps`) arriving as a burst (Poisson at
+    `rate` jobs/s).  Each job alone is too small to fill the GPU.  Returns a
+    list of abi.Job."""
+    rng = np.random.default_rng(seed)
+    grids = [(64, 32), (96, 48), (128, 64), (192, 96), (256, 128)]
+    t = 0.0
+    jobs = []
+    for _ in range(njobs):
+        t += rng.exponential(1.0 / rate)
+        NX, NY = grids[int(rng.integers(0, len(grids)))]
+        Ra = math.exp(rng.uniform(math.log(1e5), math.log(1e7)))
+        nsteps = int(round(math.exp(rng.uniform(math.log(steps[0]), math.log(steps[1])))))
+        j = abi.Job()
+        j.cfg = rb2d(NX, NY, 1, Ra=Ra, precision=precision, seed=int(rng.integers(1, 2 ** 31 - 1)))
+        j.nsteps = nsteps
+        j.arrival_s = t
+        jobs.append(j)
+    return jobs
+
+
 def job_work(j):
     """Node updates of a job."""
     return int(j.cfg.NX) * int(j.cfg.NY) * int(j.cfg.NZ) * int(j.nsteps)

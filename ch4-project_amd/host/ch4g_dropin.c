@@ -4,6 +4,7 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -24,6 +25,8 @@ static struct {
     int timing;
     int particles;  /* initial_conditions_particles done */
     int moves;      /* move_particles calls of the current step */
+    ch4g_output *ruler[3];   /* x, y, z rulers of the last dump_averages */
+    ch4g_output *running[3]; /* their running sums (ruler_*_running) */
 } S;
 
 static double now_s(void)
@@ -70,6 +73,11 @@ void ch4g_dropin_finalize(void)
 {
     if (S.lat) ch4g_destroy(S.lat);
     S.lat = NULL;
+    for (int a = 0; a < 3; a++) {
+        free(S.ruler[a]);
+        free(S.running[a]);
+        S.ruler[a] = S.running[a] = NULL;
+    }
 }
 
 ch4g_lattice *ch4g_dropin_lattice(void) { return S.lat; }
@@ -137,23 +145,93 @@ void hydro_fields(char which_pop)
     order("hydro_fields", ST_STREAMED, ST_HYDRO, ST_HYDRO);
 }
 
+/* one ruler file: rows of the ruler's coordinate and the members listed
+ * (offsets into ch4g_output), divided by irun for the running averages */
+static void write_ruler(const char *name, int append, const ch4g_output *r, int n, int coord, const size_t *m,
+                        int nm, double irun)
+{
+    FILE *fo = fopen(name, append ? "a" : "w");
+    if (!fo) return;
+    for (int i = 0; i < n; i++) {
+        const double *v = (const double *)&r[i];
+        fprintf(fo, "%e", v[coord] / irun);
+        for (int k = 0; k < nm; k++) fprintf(fo, " %e", v[m[k] / sizeof(double)] / irun);
+        fprintf(fo, "\n");
+    }
+    fclose(fo);
+}
+
+#define OFF(f) offsetof(ch4g_output, f)
+static const size_t VEL_COLS[] = {OFF(ene), OFF(rho), OFF(ux), OFF(uy), OFF(uz), OFF(ux2),
+                                  OFF(uy2), OFF(uz2), OFF(eps), OFF(rho2)};
+static const size_t TEMP_COLS[] = {OFF(t), OFF(t2), OFF(epst), OFF(dxt), OFF(dyt), OFF(dzt), OFF(uxt),
+                                   OFF(uyt), OFF(uzt), OFF(nux), OFF(nuy), OFF(nuz), OFF(lb)};
+#undef OFF
+
 /* dump_averages (output.c:63-1101): every time_dump_diagn the FP64 volume
- * averages, appended as in output.c:773 and :864.                          */
+ * averages appended to velocity_averages.dat / temperature_averages.dat
+ * (:773, :864), the x, y, z rulers to *_averages_{x,y,z}.dat (rewritten, or
+ * appended with OUTPUT_DIAGN_APPEND), their running sums divided by irun to
+ * *_averages_{x,y,z}_run.dat (:765-768, :802-829), and with HIT forcing the
+ * turbulence scales to velocity_turbulence.dat (:832-857).                  */
 void dump_averages(void)
 {
     need_lattice("dump_averages");
     const ch4g_run_params *rp = &S.rp;
     int every = (int)(rp->time_dump_diagn / rp->time_dt);
     if (every <= 0 || S.itime % every != 0) return;
+    int irun = (int)(S.itime / every);
+    if (rp->output_at_start) irun += 1;
     ch4g_output o;
     if (ch4g_dump_averages(S.lat, &o)) die("dump_averages");
+    const int n[3] = {rp->cfg.NX, rp->cfg.NY, rp->cfg.NZ};
+    if (!S.ruler[0]) {
+        for (int a = 0; a < 3; a++) {
+            S.ruler[a] = (ch4g_output *)calloc((size_t)n[a], sizeof(ch4g_output));
+            S.running[a] = (ch4g_output *)calloc((size_t)n[a], sizeof(ch4g_output));
+            if (!S.ruler[a] || !S.running[a]) die("dump_averages");
+        }
+    }
+    if (ch4g_dump_rulers(S.lat, S.ruler[0], S.ruler[1], S.ruler[2])) die("dump_averages");
+    for (int a = 0; a < 3; a++) /* add_output(ruler_*_running, ruler_*) */
+        for (int i = 0; i < n[a]; i++) {
+            double *d = (double *)&S.running[a][i];
+            const double *v = (const double *)&S.ruler[a][i];
+            for (size_t k = 0; k < sizeof(ch4g_output) / sizeof(double); k++) d[k] += v[k];
+        }
     double time_now = rp->time_start + S.itime * rp->time_dt;
+    static const char *AX[3] = {"x", "y", "z"};
+    char name[64];
     if (rp->cfg.flags & CH4G_LB_FLUID) {
         FILE *fo = fopen("velocity_averages.dat", "a");
         if (fo) {
             fprintf(fo, "%e %e %e %e %e %e %e %e %e %e %e\n", time_now, o.ene, o.rho, o.ux, o.uy, o.uz, o.ux2, o.uy2,
                     o.uz2, o.eps, o.rho2);
             fclose(fo);
+        }
+        for (int a = 0; a < 3; a++) {
+            snprintf(name, sizeof name, "velocity_averages_%s.dat", AX[a]);
+            write_ruler(name, rp->diagn_append, S.ruler[a], n[a], a, VEL_COLS, 10, 1.0);
+            snprintf(name, sizeof name, "velocity_averages_%s_run.dat", AX[a]);
+            write_ruler(name, rp->diagn_append, S.running[a], n[a], a, VEL_COLS, 10, (double)irun);
+        }
+        if (rp->cfg.flags & CH4G_FLUID_FORCING_HIT) {
+            const double nu = (rp->cfg.tau_u - 0.5 * rp->time_dt) / 3.0;
+            double u_prime = sqrt(((o.ux2 - o.ux * o.ux) + (o.uy2 - o.uy * o.uy) + (o.uz2 - o.uz * o.uz)) / 3.0);
+            double lambda = sqrt(15.0 * nu * u_prime * u_prime / o.eps);
+            double Re_lambda = lambda * u_prime / nu;
+            double lk = pow(pow(nu, 3.0) / o.eps, 0.25);
+            double tk = pow(nu / o.eps, 0.5);
+            double vk = pow(nu * o.eps, 0.25);
+            double k_turb = 1.5 * pow(u_prime, 2.0);
+            double lL = pow(u_prime, 3.0) / o.eps;
+            double tL = k_turb / o.eps;
+            double vL = u_prime;
+            FILE *ft = fopen("velocity_turbulence.dat", "a");
+            if (ft) {
+                fprintf(ft, "%e %e %e %e %e %e %e %e %e\n", time_now, Re_lambda, lambda, lk, tk, vk, lL, tL, vL);
+                fclose(ft);
+            }
         }
     }
     if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
@@ -162,6 +240,12 @@ void dump_averages(void)
             fprintf(fo, "%e %e %e %e %e %e %e %e %e %e %e %e %e %e\n", time_now, o.t, o.t2, o.epst, o.dxt, o.dyt,
                     o.dzt, o.uxt, o.uyt, o.uzt, o.nux, o.nuy, o.nuz, o.lb);
             fclose(fo);
+        }
+        for (int a = 0; a < 3; a++) {
+            snprintf(name, sizeof name, "temperature_averages_%s.dat", AX[a]);
+            write_ruler(name, rp->diagn_append, S.ruler[a], n[a], a, TEMP_COLS, 13, 1.0);
+            snprintf(name, sizeof name, "temperature_averages_%s_run.dat", AX[a]);
+            write_ruler(name, rp->diagn_append, S.running[a], n[a], a, TEMP_COLS, 13, (double)irun);
         }
     }
 }

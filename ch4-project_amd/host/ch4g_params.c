@@ -42,6 +42,7 @@ static const struct { const char *name; unsigned flag; } FLAGS[] = {
     {"LB_TEMPERATURE_INITIAL_ADD_NOISE", CH4G_TEMPERATURE_INITIAL_ADD_NOISE},
     {"LB_FLUID_INITIAL_UNIT_DENSITY", CH4G_FLUID_INITIAL_UNIT_DENSITY},
     {"LB_INITIAL_BAROMETRIC", CH4G_INITIAL_BAROMETRIC},
+    {"LB_TEMPERATURE_LES", CH4G_TEMPERATURE_LES},
 };
 
 /* particle switches (define.h:316-382) onto ch4g_lagrange_flag; umbrella
@@ -71,7 +72,7 @@ static const char *NEUTRAL[] = {
     "NEW_SENDRECV", "OUTPUT_ASCII", "OUTPUT_H5", "OUTPUT_H5_GRID", "TIMING", "SYSTEM_TIMING", "DEBUG",
     "GRID_REGULAR", "LB", "LB_FLUID_FORCING", "LB_FLUID_BC", "LB_TEMPERATURE_BC", "LB_FLUID_FORCING_GRAVITY",
     "METHOD_EDGES_AND_CORNERS", "METHOD_FORCING_GUO", "METHOD_STEPPING_EULER", "METHOD_MYQUICK",
-    "METHOD_MYQUICK_CARTESIAN", "SYSTEM_RANDOM48", "PRECISION_DOUBLE", "OUTPUT_AT_START",
+    "METHOD_MYQUICK_CARTESIAN", "SYSTEM_RANDOM48", "PRECISION_DOUBLE", "OUTPUT_AT_START", "OUTPUT_DIAGN_APPEND",
     "METHOD_STREAMING", "GRID_POP_D3Q19", "NO_XMLHEADERS",
     /* particles: umbrella switches, debug output, and the two-node halo the
        particle fields always use here */
@@ -213,6 +214,7 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         if (!strcmp(nm, "METHOD_STREAMING")) streaming = 1;
         if (!strcmp(nm, "GRID_POP_D3Q19")) d3q19 = 1;
         if (!strcmp(nm, "OUTPUT_AT_START")) out->output_at_start = 1;
+        if (!strcmp(nm, "OUTPUT_DIAGN_APPEND")) out->diagn_append = 1;
         if (!strncmp(nm, "GRID_POP_", 9) && strcmp(nm, "GRID_POP_D3Q19"))
             return perr("%.63s: only GRID_POP_D3Q19 is implemented on the device lattice", nm);
         if (!known && !in_list(nm, NEUTRAL, sizeof NEUTRAL / sizeof NEUTRAL[0]))

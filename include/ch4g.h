@@ -293,8 +293,8 @@ enum ch4g_serve_mode {
 };
 
 enum ch4g_schedule {
-    CH4G_SCHEDULE_PROPORTIONAL = 0, /* each job's steps per chunk proportional to its remaining steps */
-    CH4G_SCHEDULE_LOCKSTEP     = 1  /* every job takes every step of the chunk */
+    CH4G_SCHEDULE_LOCKSTEP     = 0, /* every job takes every step of the chunk (default) */
+    CH4G_SCHEDULE_PROPORTIONAL = 1  /* each job's steps per chunk proportional to its remaining steps */
 };
 
 typedef struct ch4g_serve_params {

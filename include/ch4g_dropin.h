@@ -43,6 +43,7 @@ typedef struct ch4g_run_params {
     ch4g_config cfg;
     double time_dt, time_start, time_end, time_dump_field, time_dump_diagn;
     int output_at_start; /* define.h OUTPUT_AT_START */
+    int diagn_append;    /* define.h OUTPUT_DIAGN_APPEND: ruler files appended, not rewritten */
     int lagrange;        /* define.h LAGRANGE */
     ch4g_particle_config pcfg;
     double time_dump_lagr;

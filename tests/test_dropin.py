@@ -28,7 +28,7 @@ REFERENCE_NAMES = ["initial_conditions", "initialization_forcing", "compute_adve
 class RunParams(C.Structure):
     _fields_ = [("cfg", abi.Config), ("time_dt", C.c_double), ("time_start", C.c_double),
                 ("time_end", C.c_double), ("time_dump_field", C.c_double), ("time_dump_diagn", C.c_double),
-                ("output_at_start", C.c_int), ("lagrange", C.c_int), ("pcfg", abi.ParticleConfig),
+                ("output_at_start", C.c_int), ("diagn_append", C.c_int), ("lagrange", C.c_int), ("pcfg", abi.ParticleConfig),
                 ("time_dump_lagr", C.c_double)]
 
 
@@ -110,6 +110,7 @@ def test_ch4g_run_matches_oracle(tmp_path):
     cfg.Amp_x = 0.003072
     o = Oracle(cfg)
     o.initial_conditions()
+    run = None
     for k, t in enumerate(rows[:, 0]):
         if k:
             o.step(250)
@@ -117,6 +118,17 @@ def test_ch4g_run_matches_oracle(tmp_path):
         assert t == 250 * k
         np.testing.assert_allclose(rows[k, 3], a["ux"], rtol=1e-6)  # %e output: 7 significant digits
         np.testing.assert_allclose(rows[k, 1], a["ene"], rtol=1e-6, atol=1e-30)
+        _, _, ry, _ = o.dump_rulers()
+        run = ry["ux"].copy() if run is None else run + ry["ux"]
+    # y ruler of the last dump (rewritten each time) and the running average
+    # over the 5 dumps (OUTPUT_AT_START: irun = itime / every + 1)
+    ruler = np.loadtxt(tmp_path / "velocity_averages_y.dat")
+    assert ruler.shape == (64, 11)
+    np.testing.assert_array_equal(ruler[:, 0], np.arange(64) + 0.5)
+    np.testing.assert_allclose(ruler[:, 3], ry["ux"], rtol=1e-6)
+    running = np.loadtxt(tmp_path / "velocity_averages_y_run.dat")
+    np.testing.assert_allclose(running[:, 3], run / 5, rtol=1e-6)
+    assert np.loadtxt(tmp_path / "velocity_averages_x.dat").shape == (4, 11)
 
 
 def particle_type_averages(a, ntypes, thermal):
