@@ -134,6 +134,63 @@ def serve_leg(njobs, seed=2026, sweep_jobs=48):
     return res
 
 
+def batch_compare(steps=320, graphs=False):
+    """Throughput (GLUPS) of sets of lattices stepped one by one (one
+    ch4g_step call per lattice, each lattice on its own stream) and together
+    through ch4g_step_batch (one launch per population kind and step, 32 steps
+    per call).  graphs=False launches every single-lattice step, so a kernel
+    trace sees every launch."""
+    import time
+    from ch4g import abi, Lattice, step_batch, workloads
+    ch, rb = workloads.channel3d, workloads.rb2d
+    cases = {
+        "channel 78^3": [ch(78, 78, 78)],
+        "4 x channel 78^3": [ch(78, 78, 78)] * 4,
+        "RB 234x169": [rb(234, 169, 1, Ra=1e7, precision=abi.FP32_DEV)],
+        "channel 160^3": [ch(160, 160, 160)],
+        "channel 192^3": [ch(192, 192, 192)],
+        "8 x channel 80^3": [ch(80, 80, 80)] * 8,
+        "RB 512x256": [rb(512, 256, 1, Ra=1e8, precision=abi.FP32_DEV)],
+        "16 x RB 512x256": [rb(512, 256, 1, Ra=1e8, precision=abi.FP32_DEV)] * 16,
+        "64 x RB 128x64": [rb(128, 64, 1, Ra=1e6, precision=abi.FP32_DEV)] * 64,
+        "mix": [ch(113, 113, 113), ch(78, 78, 78), rb(584, 336, 1, Ra=1e8, precision=abi.FP32_DEV),
+                ch(133, 133, 133)],
+    }
+
+    def started(cfg):
+        lat = Lattice(cfg)
+        lat.set_graphs(graphs)
+        lat.initial_conditions()
+        lat.step(1)
+        return lat
+
+    out = {}
+    for name, cfgs in cases.items():
+        nodes = sum(c.nodes() for c in cfgs)
+        ls = [started(c) for c in cfgs]
+        for lat in ls:
+            lat.step(32)
+            lat.synchronize()
+        t0 = time.perf_counter()
+        for lat in ls:
+            lat.step(steps)
+        for lat in ls:
+            lat.synchronize()
+        single = nodes * steps / (time.perf_counter() - t0) / 1e9
+        for lat in ls:
+            lat.close()
+        ls = [started(c) for c in cfgs]
+        step_batch(ls, 32)
+        t0 = time.perf_counter()
+        for _ in range(steps // 32):
+            step_batch(ls, 32)
+        batch = nodes * (steps // 32) * 32 / (time.perf_counter() - t0) / 1e9
+        for lat in ls:
+            lat.close()
+        out[name] = {"single_glups": round(single, 2), "batch_glups": round(batch, 2)}
+    return out
+
+
 def particle_leg(steps=40, npart=2_000_000, N=256):
     """HIT 256^3 (FP32-deviation, LES Cs 0.1) with 1e6 tracers + 1e6 inertial
     particles: device time per step without and with the particles."""
@@ -179,6 +236,8 @@ def main():
     ap.add_argument("--workload", default="rb2d", choices=sorted(WORKLOADS))
     ap.add_argument("--serve-jobs", type=int, default=16, help="jobs of the served-ensemble leg (0: skip)")
     ap.add_argument("--no-particles", action="store_true", help="skip the particle leg")
+    ap.add_argument("--batch-compare", action="store_true",
+                    help="only compare single-lattice and batched stepping of sets of lattices (no graphs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run plain 1 GiB device-to-device copies (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -191,6 +250,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": a.numel() * 4, "copies": 5}))
+        return
+    if args.batch_compare:
+        print(json.dumps({"batch_compare": batch_compare()}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (one lattice per GPU; multi-GPU slab decomposition "
