@@ -400,7 +400,9 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
             ch4g_lattice *L = nullptr;
             int r = ch4g_create(&J.cfg, &L);
             if (!r) r = ch4g_initial_conditions(L);
-            if (!r && batched && L->base.hit) r = fail(-1, "HIT-forced jobs are not served");
+            if (!r && batched && (L->base.hit || L->base.les_fd))
+                r = fail(-1, "HIT-forced and finite-difference LES jobs are not batched (serve them with "
+                             "CH4G_SERVE_SEQUENTIAL)");
             if (!r && J.nsteps > 0) r = ch4g_step(L, 1); // first step: reads the initial populations
             if (r) {
                 if (L) ch4g_destroy(L);

@@ -85,18 +85,26 @@ ch4g_lattice *ch4g_dropin_lattice(void) { return S.lat; }
 int ch4g_dropin_itime(void) { return S.itime; }
 
 /* initial_conditions + the first hydro_fields / build_forcing / hydro_fields
- * of main.c:26-48 (the device lattice builds all of them together).  A
- * restart (resume 1) is loaded with ch4g_set_populations instead.         */
+ * of main.c:26-48 (the device lattice builds all of them together).  With
+ * restart the populations (resume_u, resume_t) and the HIT phases (resume_u)
+ * then come from pop.h5 when it exists (initial_conditions.c:599-612); the
+ * 18 phase draws the reference skips with resume_u are still taken from the
+ * drand48 stream here (the reference's stream is time-seeded in any case). */
 void initial_conditions(int restart)
 {
     need_lattice("initial_conditions");
-    if (restart) {
-        fprintf(stderr, "Error message -> resume: load the population dump with ch4g_set_populations. Exit.\n");
-        exit(-1);
-    }
     if (ch4g_initial_conditions(S.lat)) die("initial_conditions");
     S.itime = 0;
     S.stage = ST_IDLE;
+    FILE *fin = fopen("pop.h5", "r");
+    if (restart && fin != NULL) {
+        fclose(fin);
+        read_pop_h5();
+        fprintf(stderr, "The pop.h5 file is present!\n Populations are initialized from file.\n");
+    } else {
+        if (fin) fclose(fin);
+        if (restart) fprintf(stderr, "Warning message -> pop.h5 file is missing!\n Populations are initialized from scratch.\n");
+    }
 }
 
 /* HIT phases are drawn inside ch4g_initial_conditions in the reference's
@@ -424,3 +432,165 @@ void measure_time(void)
     fprintf(fo, "Execution time per process %e\n", dt);
     fclose(fo);
 }
+
+/* ---- pop.h5 (output_h5.c:347-729) ------------------------------------- */
+
+#ifdef CH4G_HDF5
+#include <hdf5.h>
+
+/* compound type of one pop record: p0..p18 doubles */
+static hid_t pop_type(void)
+{
+    hid_t t = H5Tcreate(H5T_COMPOUND, sizeof(pop));
+    char label[16];
+    for (int i = 0; i < CH4G_NPOP; i++) {
+        snprintf(label, sizeof label, "p%d", i);
+        H5Tinsert(t, label, HOFFSET(pop, p[i]), H5T_NATIVE_DOUBLE);
+    }
+    return t;
+}
+
+static hid_t vector_type(void)
+{
+    hid_t t = H5Tcreate(H5T_COMPOUND, sizeof(vector));
+    H5Tinsert(t, "x", HOFFSET(vector, x), H5T_NATIVE_DOUBLE);
+    H5Tinsert(t, "y", HOFFSET(vector, y), H5T_NATIVE_DOUBLE);
+    H5Tinsert(t, "z", HOFFSET(vector, z), H5T_NATIVE_DOUBLE);
+    return t;
+}
+
+void write_pop_h5(void)
+{
+    need_lattice("write_pop_h5");
+    const ch4g_config *c = &S.rp.cfg;
+    const size_t n = (size_t)c->NX * c->NY * c->NZ;
+    const int fluid = (c->flags & CH4G_LB_FLUID) != 0, thermal = (c->flags & CH4G_LB_TEMPERATURE) != 0;
+    pop *p = fluid ? (pop *)malloc(n * sizeof(pop)) : NULL;
+    pop *g = thermal ? (pop *)malloc(n * sizeof(pop)) : NULL;
+    if ((fluid && !p) || (thermal && !g)) die("write_pop_h5");
+    if (ch4g_get_populations(S.lat, (double *)p, (double *)g)) die("write_pop_h5");
+    fprintf(stderr, "Writing file pop.h5\n");
+    hsize_t dims[3] = {(hsize_t)c->NZ, (hsize_t)c->NY, (hsize_t)c->NX};
+    hid_t file = H5Fcreate("pop.h5", H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (file < 0) die("write_pop_h5");
+    hid_t group = H5Gcreate2(file, "/population", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    hid_t space = H5Screate_simple(3, dims, NULL);
+    hid_t pt = pop_type();
+    const char *names[2] = {"velocity", "temperature"};
+    pop *src[2] = {p, g};
+    for (int k = 0; k < 2; k++) {
+        if (!src[k]) continue;
+        hid_t d = H5Dcreate2(group, names[k], pt, space, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+        if (d < 0 || H5Dwrite(d, pt, H5S_ALL, H5S_ALL, H5P_DEFAULT, src[k]) < 0) die("write_pop_h5");
+        H5Dclose(d);
+    }
+    H5Tclose(pt);
+    H5Sclose(space);
+    H5Gclose(group);
+    if (c->flags & CH4G_FLUID_FORCING_HIT) {
+        /* /forcing: scalar attribute nk and the phase array phi_u */
+        double phi[CH4G_HIT_NK][3];
+        if (ch4g_get_hit_phases(S.lat, &phi[0][0])) die("write_pop_h5");
+        vector v[CH4G_HIT_NK];
+        for (int i = 0; i < CH4G_HIT_NK; i++) {
+            v[i].x = phi[i][0];
+            v[i].y = phi[i][1];
+            v[i].z = phi[i][2];
+        }
+        int nk = CH4G_HIT_NK;
+        hid_t fg = H5Gcreate2(file, "/forcing", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+        hid_t a1s = H5Screate(H5S_SCALAR);
+        hid_t a1 = H5Acreate2(fg, "number of modes (nk)", H5T_NATIVE_INT, a1s, H5P_DEFAULT, H5P_DEFAULT);
+        H5Awrite(a1, H5T_NATIVE_INT, &nk);
+        H5Aclose(a1);
+        H5Sclose(a1s);
+        hsize_t adim[1] = {(hsize_t)nk};
+        hid_t a2s = H5Screate_simple(1, adim, NULL);
+        hid_t vt = vector_type();
+        hid_t a2 = H5Acreate2(fg, "phi_u", vt, a2s, H5P_DEFAULT, H5P_DEFAULT);
+        H5Awrite(a2, vt, v);
+        H5Aclose(a2);
+        H5Tclose(vt);
+        H5Sclose(a2s);
+        H5Gclose(fg);
+    }
+    H5Fclose(file);
+    free(p);
+    free(g);
+}
+
+void read_pop_h5(void)
+{
+    need_lattice("read_pop_h5");
+    const ch4g_run_params *rp = &S.rp;
+    const ch4g_config *c = &rp->cfg;
+    const size_t n = (size_t)c->NX * c->NY * c->NZ;
+    const int fluid = (c->flags & CH4G_LB_FLUID) && rp->resume_u;
+    const int thermal = (c->flags & CH4G_LB_TEMPERATURE) && rp->resume_t;
+    hid_t file = H5Fopen("pop.h5", H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (file < 0) {
+        fprintf(stderr, "Error message -> pop.h5 cannot be opened. Exit.\n");
+        exit(-1);
+    }
+    pop *buf[2] = {fluid ? (pop *)malloc(n * sizeof(pop)) : NULL, thermal ? (pop *)malloc(n * sizeof(pop)) : NULL};
+    const char *names[2] = {"/population/velocity", "/population/temperature"};
+    hid_t pt = pop_type();
+    for (int k = 0; k < 2; k++) {
+        if (!buf[k]) continue;
+        hid_t d = H5Dopen2(file, names[k], H5P_DEFAULT);
+        hsize_t dims[3] = {0, 0, 0};
+        hid_t sp = d >= 0 ? H5Dget_space(d) : -1;
+        if (d < 0 || sp < 0 || H5Sget_simple_extent_ndims(sp) != 3 || H5Sget_simple_extent_dims(sp, dims, NULL) != 3 ||
+            dims[0] != (hsize_t)c->NZ || dims[1] != (hsize_t)c->NY || dims[2] != (hsize_t)c->NX) {
+            fprintf(stderr, "Error message -> %s in pop.h5 is missing or not [NZ][NY][NX]. Exit.\n", names[k]);
+            exit(-1);
+        }
+        if (H5Dread(d, pt, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf[k]) < 0) die("read_pop_h5");
+        H5Sclose(sp);
+        H5Dclose(d);
+    }
+    H5Tclose(pt);
+    if (ch4g_set_populations(S.lat, (const double *)buf[0], (const double *)buf[1])) die("read_pop_h5");
+    /* /forcing (output_h5.c:653-700): HIT phases when resume_u */
+    if ((c->flags & CH4G_FLUID_FORCING_HIT) && rp->resume_u && H5Lexists(file, "/forcing", H5P_DEFAULT) > 0) {
+        hid_t fg = H5Gopen2(file, "/forcing", H5P_DEFAULT);
+        int nk = 0;
+        hid_t a1 = H5Aopen(fg, "number of modes (nk)", H5P_DEFAULT);
+        H5Aread(a1, H5T_NATIVE_INT, &nk);
+        H5Aclose(a1);
+        if (nk != CH4G_HIT_NK) {
+            fprintf(stderr, "Error message -> pop.h5 holds %d forcing modes, this lattice %d. Exit.\n", nk, CH4G_HIT_NK);
+            exit(-1);
+        }
+        vector v[CH4G_HIT_NK];
+        hid_t vt = vector_type();
+        hid_t a2 = H5Aopen(fg, "phi_u", H5P_DEFAULT);
+        if (a2 < 0 || H5Aread(a2, vt, v) < 0) die("read_pop_h5");
+        H5Aclose(a2);
+        H5Tclose(vt);
+        H5Gclose(fg);
+        double phi[CH4G_HIT_NK][3];
+        for (int i = 0; i < CH4G_HIT_NK; i++) {
+            phi[i][0] = v[i].x;
+            phi[i][1] = v[i].y;
+            phi[i][2] = v[i].z;
+        }
+        if (ch4g_set_hit_phases(S.lat, &phi[0][0])) die("read_pop_h5");
+    }
+    H5Fclose(file);
+    free(buf[0]);
+    free(buf[1]);
+}
+#else
+void write_pop_h5(void)
+{
+    fprintf(stderr, "Error message -> this drop-in layer was built without HDF5 (pop.h5). Exit.\n");
+    exit(-1);
+}
+
+void read_pop_h5(void)
+{
+    fprintf(stderr, "Error message -> this drop-in layer was built without HDF5 (pop.h5). Exit.\n");
+    exit(-1);
+}
+#endif

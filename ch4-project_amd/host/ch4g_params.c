@@ -215,6 +215,7 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         if (!strcmp(nm, "GRID_POP_D3Q19")) d3q19 = 1;
         if (!strcmp(nm, "OUTPUT_AT_START")) out->output_at_start = 1;
         if (!strcmp(nm, "OUTPUT_DIAGN_APPEND")) out->diagn_append = 1;
+        if (!strcmp(nm, "OUTPUT_H5")) out->output_h5 = 1;
         if (!strncmp(nm, "GRID_POP_", 9) && strcmp(nm, "GRID_POP_D3Q19"))
             return perr("%.63s: only GRID_POP_D3Q19 is implemented on the device lattice", nm);
         if (!known && !in_list(nm, NEUTRAL, sizeof NEUTRAL / sizeof NEUTRAL[0]))
@@ -234,6 +235,17 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     int np = read_param_file(param_in, t, 1024);
     if (np < 0) return -1;
     double v, SX, SY, SZ;
+    /* parameters.c:78-88 */
+    if (need(t, np, "resume", &v)) return -1;
+    out->resume = (int)v;
+    if (c->flags & CH4G_LB_FLUID) {
+        if (need(t, np, "resume_u", &v)) return -1;
+        out->resume_u = (int)v;
+    }
+    if (c->flags & CH4G_LB_TEMPERATURE) {
+        if (need(t, np, "resume_t", &v)) return -1;
+        out->resume_t = (int)v;
+    }
     if (need(t, np, "NX", &v)) return -1;
     c->NX = (int)v;
     if (need(t, np, "NY", &v)) return -1;

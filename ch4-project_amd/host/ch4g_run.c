@@ -30,7 +30,7 @@ int main(int argc, char **argv)
     const int lagr = rp.lagrange;
     const int twoway = (rp.pcfg.flags & CH4G_LAGRANGE_TWOWAY_MOMENTUM) != 0;
     if (lagr) allocate_particles();
-    initial_conditions(0);
+    initial_conditions(rp.resume);
     initialization_forcing();
     /* the first hydro_fields / build_forcing / hydro_fields of main.c:26-48
        are part of initial_conditions on the device lattice */
@@ -79,6 +79,7 @@ int main(int argc, char **argv)
         }
     }
     measure_time();
+    if (rp.output_h5) write_pop_h5(); /* main.c:203-204 */
     ch4g_dropin_finalize();
     fprintf(stdout, "The End\n");
     return 0;

@@ -44,6 +44,8 @@ typedef struct ch4g_run_params {
     double time_dt, time_start, time_end, time_dump_field, time_dump_diagn;
     int output_at_start; /* define.h OUTPUT_AT_START */
     int diagn_append;    /* define.h OUTPUT_DIAGN_APPEND: ruler files appended, not rewritten */
+    int output_h5;       /* define.h OUTPUT_H5: pop.h5 written at the end of the run */
+    int resume, resume_u, resume_t; /* param.in resume flags (parameters.c:78-88) */
     int lagrange;        /* define.h LAGRANGE */
     ch4g_particle_config pcfg;
     double time_dump_lagr;
@@ -78,6 +80,12 @@ void time_stepping(pop *f, pop *rhs_f, pop *old_rhs_f, pop *old_old_rhs_f, my_do
                    char which_pop);                                                /* lb.c:588 */
 void hydro_fields(char which_pop);                                                 /* lb.c:420 */
 void dump_averages(void);                                                          /* output.c:63 */
+/* pop.h5 (output_h5.c:347-540, :540-729): /population/velocity and
+ * /population/temperature, compound p0..p18 of doubles over [NZ][NY][NX],
+ * and with HIT forcing the /forcing group ("number of modes (nk)", phi_u).
+ * Serial HDF5; without HDF5 at build time both report an error and exit. */
+void write_pop_h5(void);
+void read_pop_h5(void);
 void measure_time(void);                                                           /* parallel.c:618 */
 
 /* Lagrangian particles (src/functions.h).  The particles advance inside the

@@ -28,7 +28,8 @@ REFERENCE_NAMES = ["initial_conditions", "initialization_forcing", "compute_adve
 class RunParams(C.Structure):
     _fields_ = [("cfg", abi.Config), ("time_dt", C.c_double), ("time_start", C.c_double),
                 ("time_end", C.c_double), ("time_dump_field", C.c_double), ("time_dump_diagn", C.c_double),
-                ("output_at_start", C.c_int), ("diagn_append", C.c_int), ("lagrange", C.c_int), ("pcfg", abi.ParticleConfig),
+                ("output_at_start", C.c_int), ("diagn_append", C.c_int), ("output_h5", C.c_int),
+                ("resume", C.c_int), ("resume_u", C.c_int), ("resume_t", C.c_int), ("lagrange", C.c_int), ("pcfg", abi.ParticleConfig),
                 ("time_dump_lagr", C.c_double)]
 
 
@@ -194,3 +195,81 @@ def test_ch4g_run_particles_matches_oracle(tmp_path):
         got = rows[k * ntypes:(k + 1) * ntypes]
         assert np.all(got[:, 0] == 40 * k) and np.array_equal(got[:, 1], np.arange(ntypes))
         np.testing.assert_allclose(got[:, 2:], want, rtol=2e-6, atol=1e-300)
+
+
+H5LIB = "/opt/conda/lib/libhdf5.so"
+
+
+def read_pop_h5(path, name, shape):
+    """Read one /population dataset through the HDF5 C library itself:
+    a [NZ][NY][NX] compound of doubles named p0..p18."""
+    h5 = C.CDLL(H5LIB)
+    hid = C.c_int64
+    h5.H5open()
+    native_double = hid.in_dll(h5, "H5T_NATIVE_DOUBLE_g").value
+    h5.H5Fopen.restype = hid
+    h5.H5Fopen.argtypes = [C.c_char_p, C.c_uint, hid]
+    h5.H5Dopen2.restype = hid
+    h5.H5Dopen2.argtypes = [hid, C.c_char_p, hid]
+    h5.H5Dget_space.restype = hid
+    h5.H5Dget_space.argtypes = [hid]
+    h5.H5Sget_simple_extent_dims.argtypes = [hid, C.POINTER(C.c_uint64), C.c_void_p]
+    h5.H5Tcreate.restype = hid
+    h5.H5Tcreate.argtypes = [C.c_int, C.c_size_t]
+    h5.H5Tinsert.argtypes = [hid, C.c_char_p, C.c_size_t, hid]
+    h5.H5Dread.argtypes = [hid, hid, hid, hid, hid, C.c_void_p]
+    for f in ("H5Dclose", "H5Sclose", "H5Tclose", "H5Fclose"):
+        getattr(h5, f).argtypes = [hid]
+    f = h5.H5Fopen(str(path).encode(), 0, 0)
+    assert f >= 0
+    d = h5.H5Dopen2(f, name.encode(), 0)
+    assert d >= 0, name
+    sp = h5.H5Dget_space(d)
+    dims = (C.c_uint64 * 3)()
+    assert h5.H5Sget_simple_extent_dims(sp, dims, None) == 3
+    assert tuple(dims) == shape
+    t = h5.H5Tcreate(6, 19 * 8)  # H5T_COMPOUND
+    for i in range(19):
+        assert h5.H5Tinsert(t, b"p%d" % i, 8 * i, native_double) >= 0
+    buf = np.empty(shape + (19,))
+    assert h5.H5Dread(d, t, 0, 0, 0, buf.ctypes.data) >= 0
+    for x, fn in ((t, "H5Tclose"), (sp, "H5Sclose"), (d, "H5Dclose"), (f, "H5Fclose")):
+        getattr(h5, fn)(x)
+    return buf
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(H5LIB), reason="serial HDF5 not installed")
+def test_pop_h5_dump_and_restart(tmp_path):
+    """OUTPUT_H5: ch4g_run writes pop.h5 with the reference's layout
+    (output_h5.c:347-460) and, with resume 1 / resume_u 1, a second run starts
+    from it: 500 + 500 steps give the averages of 1000 straight steps."""
+    from oracle import Oracle
+    import configs
+    d1, d2 = tmp_path / "a", tmp_path / "b"
+    define = open(os.path.join(FIX, "define.h")).read() + "#define OUTPUT_H5\n"
+    param = open(os.path.join(FIX, "param.in")).read().replace("time_end 1000.", "time_end 500.")
+    for d, prm in ((d1, param), (d2, param.replace("resume 0", "resume 1").replace("resume_u 0", "resume_u 1"))):
+        d.mkdir()
+        (d / "define.h").write_text(define)
+        (d / "param.in").write_text(prm)
+    r = subprocess.run([RUN], cwd=d1, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    cfg = configs.poiseuille(4, 64, 4)
+    cfg.Amp_x = 0.003072
+    o = Oracle(cfg)
+    o.initial_conditions()
+    o.step(500)
+    po, _ = o.get_populations()
+    pop = read_pop_h5(d1 / "pop.h5", "/population/velocity", (4, 64, 4))
+    assert np.array_equal(pop, po)  # FP64 path: bit-identical to the oracle
+    shutil.copy(d1 / "pop.h5", d2 / "pop.h5")
+    r = subprocess.run([RUN], cwd=d2, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "initialized from file" in r.stderr
+    o.step(500)
+    a = o.dump_averages()
+    last = np.loadtxt(d2 / "velocity_averages.dat")[-1]
+    assert last[0] == 500.0
+    np.testing.assert_allclose(last[3], a["ux"], rtol=1e-6)
+    np.testing.assert_allclose(last[1], a["ene"], rtol=1e-6)

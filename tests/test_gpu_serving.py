@@ -126,3 +126,18 @@ def test_serve_admission_respects_the_footprint():
     big.cfg = workloads.channel3d(64, 64, 64)
     res = serve([big] + [abi.Job.from_buffer_copy(j) for j in jobs[1:3]], hbm_budget=budget)
     assert res[0].status < 0 and all(r.status == 0 for r in res[1:])
+
+
+def test_serve_rejects_unbatchable_jobs_only():
+    """A finite-difference LES job cannot join a batch: the batched mode
+    returns it with an error status and runs the others; the sequential mode
+    runs it."""
+    jobs = small_jobs(4, seed=3)
+    les = abi.Job.from_buffer_copy(jobs[1])
+    les.cfg = workloads.channel3d(16, 16, 8, precision=abi.FP32_DEV)
+    les.cfg.flags |= abi.FLUID_LES
+    mix = [abi.Job.from_buffer_copy(j) for j in jobs] + [les]
+    res = serve(mix, mode=abi.SERVE_BATCHED, chunk_steps=8)
+    assert res[-1].status < 0 and all(r.status == 0 for r in res[:-1])
+    seq = serve([abi.Job.from_buffer_copy(les)], mode=abi.SERVE_SEQUENTIAL)
+    assert seq[0].status == 0
