@@ -173,6 +173,27 @@ def test_fp32_deviation_against_fp64_oracle(name, cfg, steps):
     assert eu <= 1e-5
 
 
+def test_rb2d_full_size_nusselt_fp32_within_one_percent():
+    """512x256 Rayleigh-Benard at Ra=1e8 over 100000 steps: the Nusselt number
+    of the FP32-deviation path, averaged over steps 30000..100000 (dumps every
+    500 steps), is within 1 % of the FP64 path's (the FP64 device path is
+    bit-identical to the oracle; the two trajectories separate once the flow
+    is unsteady, so instantaneous values are not compared)."""
+    nus = {}
+    for prec in (abi.FP64, abi.FP32_DEV):
+        lat = Lattice(configs.rb2d(precision=prec))
+        lat.initial_conditions()
+        series = []
+        for _ in range(200):
+            lat.step(500)
+            series.append(lat.dump_averages()["nuy"])
+        nus[prec] = np.array(series)
+        lat.close()
+    a, b = nus[abi.FP64][60:].mean(), nus[abi.FP32_DEV][60:].mean()
+    print(f"  <Nu> 30000..100000: FP64 {a:.5f}  FP32-deviation {b:.5f}  rel {abs(b / a - 1):.2e}")
+    assert abs(b / a - 1) < 0.01
+
+
 def test_poiseuille_fp32_closed_form():
     """FP32-deviation storage reaches the exact lattice Poiseuille solution
     (parabola + bounce-back slip) to FP32 accuracy."""
