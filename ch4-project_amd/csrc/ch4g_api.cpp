@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "ch4g_lattice.h"
@@ -207,7 +208,7 @@ void gather(ch4g_lattice *L, double *fpost, double *gpost, double *dens, double 
 // Graphs capture kernel arguments: drop them when the step sequence changes.
 void destroy_graphs(ch4g_lattice *L)
 {
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < 4; b++)
         if (L->graph[b]) {
             (void)hipGraphExecDestroy(L->graph[b]);
             L->graph[b] = nullptr;
@@ -263,21 +264,66 @@ int launch_one_step(ch4g_lattice *L)
     return 0;
 }
 
-// Capture (once per starting buffer) and replay GRAPH_STEPS steady-state
-// steps.  Kernel arguments of these steps depend only on the buffer parity.
+// Two steady-state steps in one launch (2-D FP32-storage lattices, see
+// two_step_ok): the lag moves to the other array.
+bool two_step_eligible(const ch4g_lattice *L)
+{
+    return L->dev32 && !L->first && !L->part && two_step_ok(L->base);
+}
+
+int ensure_tlag_alt(ch4g_lattice *L)
+{
+    if (L->base.thermal && !L->tlag_alt) return dalloc(L, &L->tlag_alt, (size_t)L->N * L->rsz);
+    return 0;
+}
+
+int launch_two_steps(ch4g_lattice *L)
+{
+    int rc = ensure_tlag_alt(L); // allocated before any capture (replay_graph)
+    if (rc) return rc;
+    KParams P = params_for(L, 0, (int)L->itime + 1);
+    const int in = L->cur, out = 1 - L->cur;
+    launch_two_steps_fp32(P, (const float *)L->f[in], (const float *)L->g[in], (float *)L->f[out],
+                          (float *)L->g[out], (const float *)L->tlag, (float *)L->tlag_alt, L->stream);
+    HIPCHK(hipGetLastError());
+    L->cur = out;
+    if (L->base.thermal) {
+        std::swap(L->tlag, L->tlag_alt);
+        L->tpar ^= 1;
+    }
+    L->itime += 2;
+    return 0;
+}
+
+// Capture (once per starting buffer pair) and replay GRAPH_STEPS
+// steady-state steps.  Kernel arguments of these steps depend only on the
+// buffer parities.
 int replay_graph(ch4g_lattice *L)
 {
-    const int start = L->cur;
+    if (two_step_eligible(L)) {
+        int rc = ensure_tlag_alt(L);
+        if (rc) return rc;
+    }
+    const int start = L->cur | (L->tpar << 1);
     if (!L->graph[start]) {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
-        int64_t it0 = L->itime;
+        const int64_t it0 = L->itime;
+        const int cur0 = L->cur, tpar0 = L->tpar;
+        void *tl0 = L->tlag, *ta0 = L->tlag_alt;
         int rc = 0;
-        for (int k = 0; k < GRAPH_STEPS && !rc; k++) rc = launch_one_step(L);
+        if (two_step_eligible(L)) {
+            for (int k = 0; k < GRAPH_STEPS / 2 && !rc; k++) rc = launch_two_steps(L);
+        } else {
+            for (int k = 0; k < GRAPH_STEPS && !rc; k++) rc = launch_one_step(L);
+        }
         hipError_t e = hipStreamEndCapture(L->stream, &g);
         // capture records work without running it: restore the counters
         L->itime = it0;
-        L->cur = start;
+        L->cur = cur0;
+        L->tpar = tpar0;
+        L->tlag = tl0;
+        if (ta0) L->tlag_alt = ta0;
         if (rc) {
             if (g) (void)hipGraphDestroy(g);
             return rc;
@@ -544,7 +590,7 @@ void ch4g_destroy(ch4g_lattice *L)
     }
     destroy_graphs(L);
     particles_free(L);
-    void *scr[] = {L->tlag, L->taules, L->vel, L->d_rul, L->d_dens, L->d_u, L->d_t, L->d_part, L->d_res, L->d_fs, L->d_gs,
+    void *scr[] = {L->tlag, L->tlag_alt, L->taules, L->vel, L->d_rul, L->d_dens, L->d_u, L->d_t, L->d_part, L->d_res, L->d_fs, L->d_gs,
                    L->d_row32, L->d_row64};
     for (void *p : scr)
         if (p) (void)hipFree(p);
@@ -676,12 +722,18 @@ int ch4g_step(ch4g_lattice *L, int nsteps)
     int s = 0;
     while (s < nsteps) {
         // Steady-state steps (not the first, no per-step host forcing tables,
-        // no particles) depend only on the buffer parity: replay a captured
+        // no particles) depend only on the buffer parities: replay a captured
         // GRAPH_STEPS-step graph instead of launching.
         if (L->use_graphs && !L->first && L->itime >= 1 && !L->base.hit && !L->part && nsteps - s >= GRAPH_STEPS) {
             int rc = replay_graph(L);
             if (rc) return rc;
             s += GRAPH_STEPS;
+            continue;
+        }
+        if (nsteps - s >= 2 && two_step_eligible(L)) {
+            int rc = launch_two_steps(L);
+            if (rc) return rc;
+            s += 2;
             continue;
         }
         int rc = launch_one_step(L);

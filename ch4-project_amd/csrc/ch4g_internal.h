@@ -248,6 +248,14 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,
                         double *gpost, double *dens, double *u, double *t, hipStream_t s);
 
+// Two steps in one launch for 2-D FP32-storage lattices (NZ = 1, NX a
+// multiple of 64, steady steps without HIT tables, LES or particles): reads
+// the state (f, g in; temperature lag tl_in), writes the state two steps on
+// (f, g out; lag tl_out, a different array).
+bool two_step_ok(const KParams &P);
+void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin, float *fout, float *gout,
+                           const float *tl_in, float *tl_out, hipStream_t s);
+
 // Velocity of the step at every node into P.vel (the finite-difference LES
 // closure reads it at the six neighbours); same gather and moments as the
 // step kernel.

@@ -74,8 +74,13 @@ struct ch4g_lattice {
     // FP32-deviation per-row base state
     float *d_row32 = nullptr;
     double *d_row64 = nullptr;
-    // captured GRAPH_STEPS-step sequences, one per starting buffer
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // captured GRAPH_STEPS-step sequences, one per starting buffer and
+    // temperature-lag buffer (index cur | tpar << 1)
+    hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    // two-step launches (2-D FP32 storage) write the lag into a second array;
+    // tlag stays the current one, tpar counts the swaps mod 2
+    void *tlag_alt = nullptr;
+    int tpar = 0;
     int use_graphs = 1;
     int64_t bytes = 0;
     ch4g::Particles *part = nullptr;

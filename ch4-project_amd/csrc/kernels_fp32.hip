@@ -159,6 +159,36 @@ __device__ __forceinline__ A wa(int q)
     return q == 0 ? A(1) / A(3) : (q <= 6 ? A(1) / A(18) : A(1) / A(36));
 }
 
+// Where a node's post-collision state goes.  NodeOut: the global planes of
+// the single-step kernels (and of the second step of k_step2_fp32), with the
+// previous LES relaxation time read from tau_src[tau_at].  TileOut: the LDS
+// tile of k_step2_fp32's first step: fluid planes from 0, temperature planes
+// from gofs, then the temperature lag, `stride` nodes apart.
+struct NodeOut {
+    float *__restrict__ f;
+    float *__restrict__ g;
+    float *__restrict__ tlag;
+    float *__restrict__ taules;
+    const float *tau_src;
+    uint32_t n, tau_at;
+    size_t N;
+    __device__ void put_f(int q, float v) const { stp(f + (size_t)q * N + n, v); }
+    __device__ void put_g(int q, float v) const { stp(g + (size_t)q * N + n, v); }
+    __device__ void put_tlag(float v) const { tlag[n] = v; }
+    __device__ float get_tau() const { return tau_src[tau_at]; }
+    __device__ void put_tau(float v) const { taules[n] = v; }
+};
+
+struct TileOut {
+    float *t;
+    int at, stride, gofs;  // tile node, plane stride, first temperature plane
+    __device__ void put_f(int q, float v) const { t[q * stride + at] = v; }
+    __device__ void put_g(int q, float v) const { t[(gofs + q) * stride + at] = v; }
+    __device__ void put_tlag(float v) const { t[(gofs + Q) * stride + at] = v; }
+    __device__ float get_tau() const { return 0.f; }  // no LES on the two-step path
+    __device__ void put_tau(float) const {}
+};
+
 // Moments of the gathered fluid populations (deviation form): density
 // deviation and density, momentum, and the velocity with the lagged half
 // force (hydro_fields, lb.c:420-540).
@@ -239,12 +269,11 @@ __device__ __forceinline__ A les_tau_fd32(const KParams &P, const float *__restr
 // half force, optional LES, Guo-forced BGK in FMA form, restoration of the
 // collision invariants, non-temporal stores; the velocity used is returned
 // in ux, uy, uz and the LES relaxation time (if any) in tau_les.
-template <typename A>
+template <typename A, class Out>
 __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, int x, int y, int z, uint32_t n,
-                                              const float f[Q], float dTlag, A dT, float *__restrict__ fout,
-                                              float *__restrict__ taules, A &ux, A &uy, A &uz, A &tau_les)
+                                              const float f[Q], float dTlag, A dT, const Out &out, A &ux, A &uy,
+                                              A &uz, A &tau_les)
 {
-    const size_t N = (size_t)P.N;
     {
         Moments<A> m;
         fluid_moments<A>(P, R, x, y, z, n, f, dTlag, m);
@@ -260,7 +289,7 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
         A omega = (A)P.invtau_u;
         if (P.les && P.les_fd) {
             const A tau = les_tau_fd32<A>(P, (const float *)P.vel, x, y, z, n);
-            taules[n] = (float)tau;
+            out.put_tau((float)tau);
             tau_les = tau;
             omega = A(1) / tau;
         } else if (P.les) {
@@ -280,10 +309,10 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
             }
             // stress_tensor of gradients.c:65-67: S.zz takes S.xz, S.zx stays 0
             const A ss = xx * xx + A(2) * xy * xy + xz * xz + yy * yy + A(2) * yz * yz + xz * xz;
-            A tau_prev = (P.itime == 1) ? (A)P.tau_u : (A)taules[n];
+            A tau_prev = (P.itime == 1) ? (A)P.tau_u : (A)out.get_tau();
             A fac = A(1) / (A(2) * (A)P.cs2 * rho * tau_prev);
             A tau = A(3) * (fac * sqrt(A(2) * ss) * (A)P.Cs2 + (A)P.nu) + A(0.5);
-            taules[n] = (float)tau;
+            out.put_tau((float)tau);
             tau_les = tau;
             omega = A(1) / tau;
         }
@@ -335,19 +364,17 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
             r[0] += drho - sum19(r);
         }
 #pragma unroll
-        for (int q = 0; q < Q; q++) stp(fout + (size_t)q * N + n, (float)r[q]);
+        for (int q = 0; q < Q; q++) out.put_f(q, (float)r[q]);
     }
 
 }
 
 // Temperature part of the step for one node: BGK with the fluid velocity,
 // exact conservation of the temperature, non-temporal stores.
-template <typename A>
-__device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, int y, uint32_t n, const float g[Q],
-                                                A dT, A ux, A uy, A uz, A tau_les, float *__restrict__ gout,
-                                                float *__restrict__ tlag)
+template <typename A, class Out>
+__device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, int y, const float g[Q], A dT, A ux,
+                                                A uy, A uz, A tau_les, const Out &out)
 {
-    const size_t N = (size_t)P.N;
     {
         const A u2 = ux * ux + uy * uy + uz * uz;
         const A T = (sizeof(A) == 4 ? (A)R.tb : (A)P.row64[2 * y + 1]) + dT;
@@ -369,8 +396,8 @@ __device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, 
         // the collision conserves the temperature exactly (float weights do not)
         if constexpr (sizeof(A) == 4) r[0] += dT - sum19(r);
 #pragma unroll
-        for (int q = 0; q < Q; q++) stp(gout + (size_t)q * N + n, (float)r[q]);
-        tlag[n] = (float)dT;
+        for (int q = 0; q < Q; q++) out.put_g(q, (float)r[q]);
+        out.put_tlag((float)dT);
     }
 }
 
@@ -404,8 +431,9 @@ __device__ __forceinline__ void step_node32(const KParams &P, const float *__res
         }
     }
     A ux = 0, uy = 0, uz = 0, tau_les = A(0);
-    if constexpr (FLUID) collide_fluid<A>(P, R, x, y, z, n, f, dTlag, dT, fout, taules, ux, uy, uz, tau_les);
-    if constexpr (THERMAL) collide_thermal<A>(P, R, y, n, g, dT, ux, uy, uz, tau_les, gout, tlag);
+    const NodeOut out{fout, gout, tlag, taules, taules, n, n, (size_t)P.N};
+    if constexpr (FLUID) collide_fluid<A>(P, R, x, y, z, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
+    if constexpr (THERMAL) collide_thermal<A>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
 }
 
 // Row grid (blocks along x of one (y, z) row: row quantities block-uniform)
@@ -433,6 +461,119 @@ __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__res
         n = ((uint32_t)z * P.NY + y) * P.NX + x;
     }
     step_node32<FIRST, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
+}
+
+// ---------------------------------------------------------------------------
+// Two steps per launch for 2-D lattices (NZ = 1, NX a multiple of 64).  A
+// block owns a T2_TX x T2_TY tile.  First step: the tile and a one-node halo
+// (periodic images; rows beyond a wall are computed from their periodic
+// images and never read) are advanced from the global state into LDS with the
+// single-step per-node code.  Second step: the tile's nodes gather from LDS
+// (same wall rules and row terms) and collide into the global output.  Per
+// two steps the state is read 1.55 times and written once instead of twice
+// each.  The temperature lag is read from tl_in and written to tl_out (halo
+// nodes of one block are tile nodes of another).
+constexpr int T2_TX = 64, T2_TY = 4;
+constexpr int T2_W = T2_TX + 2, T2_H = T2_TY + 2, T2_M = T2_W * T2_H;
+
+template <bool NTL>
+__device__ __forceinline__ void gather_f_tile(const KParams &P, const Row &R, const Nbr &b, const float *t, int at,
+                                              float f[Q])
+{
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const bool cross = P.wall_f && crosses_wall<q>(b);
+        float v = cross ? t[invq(q) * T2_M + at] : t[q * T2_M + at - cxq(q) - cyq(q) * T2_W];
+        if constexpr (cyq(q) > 0) v += cross ? 0.f : wf(q) * R.dr_up;
+        if constexpr (cyq(q) < 0) v += cross ? 0.f : wf(q) * R.dr_dn;
+        f[q] = v;
+    });
+}
+
+__device__ __forceinline__ void gather_g_tile(const KParams &P, const Row &R, const Nbr &b, const float *t, int gofs,
+                                              int at, float dTlag, float g[Q])
+{
+    const float fac_top = 2.f * ((float)P.cw_top - dTlag);
+    const float fac_bot = 2.f * ((float)P.cw_bot - dTlag);
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const bool cross = P.wall_t && crosses_wall<q>(b);
+        float v = t[(gofs + q) * T2_M + (cross ? at : at - cxq(q) - cyq(q) * T2_W)];
+        if constexpr (cyq(q) > 0) v += cross ? wf(q) * fac_bot : wf(q) * R.dt_up;
+        if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
+        g[q] = v;
+    });
+}
+
+template <bool FLUID, bool THERMAL, typename A, bool NTL>
+__global__ void __launch_bounds__(256) k_step2_fp32(KParams P, const float *__restrict__ fin,
+                                                    const float *__restrict__ gin, float *__restrict__ fout,
+                                                    float *__restrict__ gout, const float *__restrict__ tl_in,
+                                                    float *__restrict__ tl_out)
+{
+    constexpr int GOFS = FLUID ? Q : 0;
+    constexpr int PLANES = GOFS + (THERMAL ? Q + 1 : 0);
+    __shared__ float t[PLANES * T2_M];
+    const int NX = P.NX, NY = P.NY;
+    const int x0 = blockIdx.x * T2_TX, y0 = blockIdx.y * T2_TY;
+    // first step: tile + halo into LDS
+    for (int m = threadIdx.x; m < T2_M; m += blockDim.x) {
+        const int lx = m % T2_W, ly = m / T2_W;
+        int x = x0 - 1 + lx;
+        x = x < 0 ? x + NX : (x >= NX ? x - NX : x);
+        int y = (y0 - 1 + ly) % NY;
+        y = y < 0 ? y + NY : y;
+        const uint32_t n = (uint32_t)y * NX + x;
+        const Row R = row_of(P, y);
+        const Nbr b = make_nbr(x, y, 0, NX, NY, 1);
+        float dTlag = 0.f;
+        float g[Q], f[Q];
+        if constexpr (THERMAL) {
+            dTlag = tl_in[n];
+            gather_g32<false, NTL>(P, R, b, gin, n, dTlag, g);
+        }
+        if constexpr (FLUID) gather_f32<false, NTL>(P, R, b, fin, n, f);
+        A dT = A(0);
+        if constexpr (THERMAL) {
+            if constexpr (sizeof(A) == 4) {
+                dT = sum19(g);
+            } else {
+                for (int q = 0; q < Q; q++) dT += (A)g[q];
+            }
+        }
+        A ux = 0, uy = 0, uz = 0, tau_les = A(0);
+        const TileOut out{t, m, T2_M, GOFS};
+        if constexpr (FLUID) collide_fluid<A>(P, R, x, y, 0, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
+        if constexpr (THERMAL) collide_thermal<A>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
+    }
+    __syncthreads();
+    // second step: the tile, from LDS into the global state
+    const int tx = threadIdx.x % T2_TX, ty = threadIdx.x / T2_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    if (ty >= T2_TY || y >= NY) return;
+    const int at = (ty + 1) * T2_W + tx + 1;
+    const uint32_t n = (uint32_t)y * NX + x;
+    const Row R = row_of(P, y);
+    const Nbr b = make_nbr(x, y, 0, NX, NY, 1);
+    float dTlag = 0.f;
+    float g[Q], f[Q];
+    if constexpr (THERMAL) {
+        dTlag = t[(GOFS + Q) * T2_M + at];
+        gather_g_tile(P, R, b, t, GOFS, at, dTlag, g);
+    }
+    if constexpr (FLUID) gather_f_tile<NTL>(P, R, b, t, at, f);
+    A dT = A(0);
+    if constexpr (THERMAL) {
+        if constexpr (sizeof(A) == 4) {
+            dT = sum19(g);
+        } else {
+            for (int q = 0; q < Q; q++) dT += (A)g[q];
+        }
+    }
+    A ux = 0, uy = 0, uz = 0, tau_les = A(0);
+    const NodeOut out{fout, gout, tl_out, nullptr, nullptr, n, 0, (size_t)P.N};
+    if constexpr (FLUID) collide_fluid<A>(P, R, x, y, 0, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
+    if constexpr (THERMAL) collide_thermal<A>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
 }
 
 // Velocity of the step at every node (the u[] that add_collision reads,
@@ -630,6 +771,41 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 
 // Cooperative multi-step launch when the lattice's blocks can all be
 // resident; returns 1 if it cannot be used (the caller launches per step).
+// Measured on 2-D fluid+T lattices: 13-18 % faster per step than the
+// single-step kernel once the state no longer fits the last-level cache
+// (2048x1024, 4096x2048), slower while it does (512x256: 9.35 vs 8.94 us,
+// 1024x512: 37.5 vs 32.8 us), so only streamed lattices use it.
+bool two_step_ok(const KParams &P)
+{
+    const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
+    return P.NZ == 1 && P.NX % T2_TX == 0 && !P.first && !P.hit && !P.les && !P.twoway &&
+           lattice_bytes > ((int64_t)192 << 20);
+}
+
+void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin, float *fout, float *gout,
+                           const float *tl_in, float *tl_out, hipStream_t s)
+{
+    dim3 grid((unsigned)(P.NX / T2_TX), (unsigned)((P.NY + T2_TY - 1) / T2_TY)), block(T2_TX * T2_TY);
+    // streamed lattices only (two_step_ok): plain loads
+#define CH4G_T2_(FL_, TH_, A_)                                                                                        \
+    hipLaunchKernelGGL((k_step2_fp32<FL_, TH_, A_, false>), grid, block, 0, s, P, fin, gin, fout, gout, tl_in, tl_out)
+#define CH4G_T2(FL_, TH_)                                                                                             \
+    do {                                                                                                              \
+        if (P.arith64)                                                                                                \
+            CH4G_T2_(FL_, TH_, double);                                                                               \
+        else                                                                                                          \
+            CH4G_T2_(FL_, TH_, float);                                                                                \
+    } while (0)
+    if (P.fluid && P.thermal)
+        CH4G_T2(true, true);
+    else if (P.fluid)
+        CH4G_T2(true, false);
+    else
+        CH4G_T2(false, true);
+#undef CH4G_T2
+#undef CH4G_T2_
+}
+
 void launch_velocity_fp32(const KParams &P, const float *fin, const float *tlag, hipStream_t s)
 {
     int bx = block_x32(P);

@@ -62,3 +62,23 @@ def test_graph_replay_odd_counts_fd_les(precision):
         pe, _ = eager.get_populations()
         pg, _ = graph.get_populations()
         assert np.array_equal(pe, pg), n
+
+
+def test_two_step_launches_match_single_steps():
+    """A 2-D FP32-deviation lattice too large for the last-level cache
+    (1024x704 fluid+T, 219 MB) advances two steps per launch (LDS tile with a
+    one-node halo); the same lattice stepped through ch4g_step_batch uses
+    one launch per step.  Populations must agree bit for bit after odd and
+    even counts and after graph replays."""
+    from ch4g import step_batch
+    cfg = configs.rb2d(NX=1024, NY=704, Ra=1e8, precision=abi.FP32_DEV)
+    two, one = Lattice(cfg), Lattice(cfg)
+    for lat in (two, one):
+        lat.initial_conditions()
+        lat.step(1)
+    for n in (7, 10, 70):
+        two.step(n)
+        step_batch([one], n)
+        pa, ga = two.get_populations()
+        pb, gb = one.get_populations()
+        assert np.array_equal(pa, pb) and np.array_equal(ga, gb), n
