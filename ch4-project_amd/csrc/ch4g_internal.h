@@ -233,8 +233,9 @@ struct BatchSlot {
 // arithmetic, FP64) listed in its slots
 // population sets of a batched launch (one kernel instance per kind)
 enum { BATCH_FLUID = 1, BATCH_THERMAL = 2, BATCH_FLUID_THERMAL = 3 };
+// gen: some lattice of the launch uses the HIT / LES / two-way paths
 void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
-                       bool dense, int arith64, bool ntl, hipStream_t s);
+                       bool dense, bool gen, int arith64, bool ntl, hipStream_t s);
 void launch_batch_fp64(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
                        bool dense, hipStream_t s);
 

@@ -28,6 +28,7 @@ struct Batch {
     struct Group {
         int dev32, arith64, kind;
         bool dense;                       // lanes decode their own rows (dense_rows)
+        bool gen;                         // LES lattices (the kernel instance with those paths)
         bool ntl;
         int first, count;                 // range in the entry table
         std::vector<int64_t> off;         // per launch: first slot in the slot table
@@ -117,7 +118,8 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
     for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
     auto key = [&](int i) {
         const ch4g_lattice *L = lats[(size_t)i];
-        return (L->dev32 << 4) | (L->base.arith64 << 3) | ((int)dense_rows(L->cfg.NX) << 2) | kind_of(L);
+        return (L->base.les << 5) | (L->dev32 << 4) | (L->base.arith64 << 3) | ((int)dense_rows(L->cfg.NX) << 2) |
+               kind_of(L);
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
     for (size_t i = 0; i < order.size();) {
@@ -128,6 +130,7 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
         G.arith64 = L0->base.arith64;
         G.kind = kind_of(L0);
         G.dense = dense_rows(L0->cfg.NX);
+        G.gen = L0->base.les != 0;
         G.first = (int)B.host.size();
         G.count = 0;
         int64_t bytes = 0;
@@ -233,7 +236,7 @@ int launch_steps(Batch &B)
             const BatchSlot *sl = B.dev_slots + G.off[(size_t)k];
             const int ns = G.nslot[(size_t)k];
             if (G.dev32)
-                launch_batch_fp32(tab, sl, ns, nb, G.kind, G.dense, G.arith64, G.ntl, st);
+                launch_batch_fp32(tab, sl, ns, nb, G.kind, G.dense, G.gen, G.arith64, G.ntl, st);
             else
                 launch_batch_fp64(tab, sl, ns, nb, G.kind, G.dense, st);
         }

@@ -56,14 +56,17 @@ __device__ __forceinline__ Row row_of(const KParams &P, int y)
 }
 
 // force of one node from its temperature deviation dT (buoyancy) and the
-// HIT tables; same term order as build_forcing.
+// HIT tables; same term order as build_forcing.  GEN: the HIT, LES and
+// two-way coupling paths are compiled in (lattices without them run an
+// instance without those paths and their register cost).
+template <bool GEN>
 __device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab, const double *fb, uint32_t n, int x,
                                            int y, int z, float theta, float F[3])
 {
     F[0] = (float)P.Fc[0];
     F[1] = (float)P.Fc[1];
     F[2] = (float)P.Fc[2];
-    if (P.hit) {
+    if (GEN && P.hit) {
 #pragma unroll
         for (int ii = 0; ii < CH4G_HIT_NK; ii++) {
             float ax = (float)tab.sx[ii * P.NX + x];
@@ -80,7 +83,7 @@ __device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab
         F[1] += fac * (float)P.grav[1];
         F[2] += fac * (float)P.grav[2];
     }
-    if (P.twoway) {
+    if (GEN && P.twoway) {
         const size_t N = (size_t)P.N;
         F[0] += (float)fb[n];
         F[1] += (float)fb[N + n];
@@ -198,7 +201,7 @@ struct Moments {
     A drho, rho, jx, jy, jz, irho, ux, uy, uz;
 };
 
-template <typename A>
+template <typename A, bool GEN>
 __device__ __forceinline__ void fluid_moments(const KParams &P, const Row &R, int x, int y, int z, uint32_t n,
                                               const float f[Q], float dTlag, Moments<A> &m)
 {
@@ -221,7 +224,7 @@ __device__ __forceinline__ void fluid_moments(const KParams &P, const Row &R, in
     m.uz = m.jz * m.irho;
     if (P.forced) {
         float Fp[3];
-        force_at32(P, P.hit_prev, P.fb_prev, n, x, y, z, R.offT + dTlag, Fp);
+        force_at32<GEN>(P, P.hit_prev, P.fb_prev, n, x, y, z, R.offT + dTlag, Fp);
         m.ux += A(0.5) * Fp[0];
         m.uy += A(0.5) * Fp[1];
         m.uz += A(0.5) * Fp[2];
@@ -269,14 +272,14 @@ __device__ __forceinline__ A les_tau_fd32(const KParams &P, const float *__restr
 // half force, optional LES, Guo-forced BGK in FMA form, restoration of the
 // collision invariants, non-temporal stores; the velocity used is returned
 // in ux, uy, uz and the LES relaxation time (if any) in tau_les.
-template <typename A, class Out>
+template <typename A, bool GEN, class Out>
 __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, int x, int y, int z, uint32_t n,
                                               const float f[Q], float dTlag, A dT, const Out &out, A &ux, A &uy,
                                               A &uz, A &tau_les)
 {
     {
         Moments<A> m;
-        fluid_moments<A>(P, R, x, y, z, n, f, dTlag, m);
+        fluid_moments<A, GEN>(P, R, x, y, z, n, f, dTlag, m);
         const A *fa = m.fa;
         const A drho = m.drho, rho = m.rho, jx = m.jx, jy = m.jy, jz = m.jz, irho = m.irho;
         ux = m.ux;
@@ -287,12 +290,12 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
         const A wr[3] = {wa<A>(0) * rho, wa<A>(1) * rho, wa<A>(7) * rho};
         const A wd[3] = {wa<A>(0) * drho, wa<A>(1) * drho, wa<A>(7) * drho};
         A omega = (A)P.invtau_u;
-        if (P.les && P.les_fd) {
+        if (GEN && P.les && P.les_fd) {
             const A tau = les_tau_fd32<A>(P, (const float *)P.vel, x, y, z, n);
             out.put_tau((float)tau);
             tau_les = tau;
             omega = A(1) / tau;
-        } else if (P.les) {
+        } else if (GEN && P.les) {
             A xx = 0, yy = 0, xy = 0, xz = 0, yz = 0;
 #pragma unroll
             for (int q = 1; q < Q; q++) {
@@ -317,7 +320,7 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
             omega = A(1) / tau;
         }
         float Fn[3] = {0.f, 0.f, 0.f};
-        if (P.forced) force_at32(P, P.hit_now, P.fb_now, n, x, y, z, R.offT + (float)dT, Fn);
+        if (P.forced) force_at32<GEN>(P, P.hit_now, P.fb_now, n, x, y, z, R.offT + (float)dT, Fn);
         const A uF = ux * Fn[0] + uy * Fn[1] + uz * Fn[2];
         // Guo term folded into the equilibrium: f* = f - omega (f - eqG) with
         // eqG = f_eq + G/omega, so the O(population) value is rounded twice
@@ -371,7 +374,7 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
 
 // Temperature part of the step for one node: BGK with the fluid velocity,
 // exact conservation of the temperature, non-temporal stores.
-template <typename A, class Out>
+template <typename A, bool GEN, class Out>
 __device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, int y, const float g[Q], A dT, A ux,
                                                 A uy, A uz, A tau_les, const Out &out)
 {
@@ -379,7 +382,7 @@ __device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, 
         const A u2 = ux * ux + uy * uy + uz * uz;
         const A T = (sizeof(A) == 4 ? (A)R.tb : (A)P.row64[2 * y + 1]) + dT;
         // tau_t_les (les.c:273-297) with prandtl 1: tau_t = 3(kappa + nu_les) + 1/2
-        const A om = (P.les_t && P.fluid) ? A(1) / (A(3) * ((A)P.kappa + (tau_les - A(0.5) - A(3) * (A)P.nu) / A(3)) +
+        const A om = (GEN && P.les_t && P.fluid) ? A(1) / (A(3) * ((A)P.kappa + (tau_les - A(0.5) - A(3) * (A)P.nu) / A(3)) +
                                                      A(0.5))
                                           : (A)P.invtau_t;
         const A wT[3] = {wa<A>(0) * T, wa<A>(1) * T, wa<A>(7) * T};
@@ -405,7 +408,7 @@ __device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, 
 // batched kernel, so a lattice advances bit-identically in either.
 // A = arithmetic type of the collision (float for CH4G_FP32_DEV, double for
 // CH4G_FP32_STORE); storage is float in both.
-template <bool FIRST, bool FLUID, bool THERMAL, typename A, bool NTL>
+template <bool FIRST, bool FLUID, bool THERMAL, typename A, bool NTL, bool GEN>
 __device__ __forceinline__ void step_node32(const KParams &P, const float *__restrict__ fin,
                                             const float *__restrict__ gin, float *__restrict__ fout,
                                             float *__restrict__ gout, float *__restrict__ tlag,
@@ -432,13 +435,13 @@ __device__ __forceinline__ void step_node32(const KParams &P, const float *__res
     }
     A ux = 0, uy = 0, uz = 0, tau_les = A(0);
     const NodeOut out{fout, gout, tlag, taules, taules, n, n, (size_t)P.N};
-    if constexpr (FLUID) collide_fluid<A>(P, R, x, y, z, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
-    if constexpr (THERMAL) collide_thermal<A>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
+    if constexpr (FLUID) collide_fluid<A, GEN>(P, R, x, y, z, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
+    if constexpr (THERMAL) collide_thermal<A, GEN>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
 }
 
 // Row grid (blocks along x of one (y, z) row: row quantities block-uniform)
 // or, DENSE, a linear grid with one node per lane (see dense_rows).
-template <bool FIRST, bool FLUID, bool THERMAL, typename A, bool NTL, bool DENSE>
+template <bool FIRST, bool FLUID, bool THERMAL, typename A, bool NTL, bool DENSE, bool GEN>
 __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__restrict__ fin,
                                                    const float *__restrict__ gin, float *__restrict__ fout,
                                                    float *__restrict__ gout, float *__restrict__ tlag,
@@ -460,7 +463,7 @@ __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__res
         z = blockIdx.z;
         n = ((uint32_t)z * P.NY + y) * P.NX + x;
     }
-    step_node32<FIRST, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
+    step_node32<FIRST, FLUID, THERMAL, A, NTL, GEN>(P, fin, gin, fout, gout, tlag, taules, x, y, z, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -543,8 +546,8 @@ __global__ void __launch_bounds__(256) k_step2_fp32(KParams P, const float *__re
         }
         A ux = 0, uy = 0, uz = 0, tau_les = A(0);
         const TileOut out{t, m, T2_M, GOFS};
-        if constexpr (FLUID) collide_fluid<A>(P, R, x, y, 0, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
-        if constexpr (THERMAL) collide_thermal<A>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
+        if constexpr (FLUID) collide_fluid<A, false>(P, R, x, y, 0, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
+        if constexpr (THERMAL) collide_thermal<A, false>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
     }
     __syncthreads();
     // second step: the tile, from LDS into the global state
@@ -572,8 +575,8 @@ __global__ void __launch_bounds__(256) k_step2_fp32(KParams P, const float *__re
     }
     A ux = 0, uy = 0, uz = 0, tau_les = A(0);
     const NodeOut out{fout, gout, tl_out, nullptr, nullptr, n, 0, (size_t)P.N};
-    if constexpr (FLUID) collide_fluid<A>(P, R, x, y, 0, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
-    if constexpr (THERMAL) collide_thermal<A>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
+    if constexpr (FLUID) collide_fluid<A, false>(P, R, x, y, 0, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
+    if constexpr (THERMAL) collide_thermal<A, false>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
 }
 
 // Velocity of the step at every node (the u[] that add_collision reads,
@@ -593,7 +596,7 @@ __global__ void __launch_bounds__(256) k_velocity32(KParams P, const float *__re
     gather_f32<FIRST, false>(P, R, b, fin, n, f);
     const float dTlag = P.thermal ? tlag[n] : 0.f;
     Moments<A> m;
-    fluid_moments<A>(P, R, x, y, z, n, f, dTlag, m);
+    fluid_moments<A, true>(P, R, x, y, z, n, f, dTlag, m);
     float *vel = (float *)P.vel;
     vel[n] = (float)m.ux;
     vel[N + n] = (float)m.uy;
@@ -605,7 +608,7 @@ __global__ void __launch_bounds__(256) k_velocity32(KParams P, const float *__re
 // holds it (wave-uniform binary search over the launch's few slots), with
 // that lattice's step parity; the state buffer of each lattice alternates
 // with the step parity.  One instance per population-set kind.
-template <bool FLUID, bool THERMAL, typename A, bool NTL, bool DENSE>
+template <bool FLUID, bool THERMAL, typename A, bool NTL, bool DENSE, bool GEN>
 __global__ void __launch_bounds__(256) k_step_batch32(const BatchEntry *__restrict__ tab,
                                                       const BatchSlot *__restrict__ slots, int nslots)
 {
@@ -646,7 +649,7 @@ __global__ void __launch_bounds__(256) k_step_batch32(const BatchEntry *__restri
         z = row / (uint32_t)P.NY;
         n = row * (uint32_t)P.NX + x;
     }
-    step_node32<false, FLUID, THERMAL, A, NTL>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
+    step_node32<false, FLUID, THERMAL, A, NTL, GEN>(P, fin, gin, fout, gout, tl, tau, (int)x, (int)y, (int)z, n);
 }
 
 __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__restrict__ fin,
@@ -697,7 +700,7 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
         double ux = jx / rho, uy = jy / rho, uz = jz / rho;
         if (P.forced) {
             float F[3];
-            force_at32(P, P.hit_now, P.fb_now, n, x, y, z, R.offT + dTlag, F);
+            force_at32<true>(P, P.hit_now, P.fb_now, n, x, y, z, R.offT + dTlag, F);
             ux += 0.5 * (double)F[0];
             uy += 0.5 * (double)F[1];
             uz += 0.5 * (double)F[2];
@@ -734,14 +737,22 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
     // both lattices resident in the 256 MB last-level cache?
     const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
     const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
+    const bool gen = P.hit || P.les || P.twoway;
+#define CH4G_LAUNCH32G_(F_, FL_, TH_, A_, N_, D_)                                                                     \
+    do {                                                                                                              \
+        if (gen)                                                                                                      \
+            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, N_, D_, true>), grid, block, 0, s, P, fin, gin, fout,  \
+                               gout, tlag, taules);                                                                   \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, N_, D_, false>), grid, block, 0, s, P, fin, gin, fout, \
+                               gout, tlag, taules);                                                                   \
+    } while (0)
 #define CH4G_LAUNCH32D_(F_, FL_, TH_, A_, D_)                                                                          \
     do {                                                                                                              \
         if (ntl)                                                                                                      \
-            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, true, D_>), grid, block, 0, s, P, fin, gin, fout,      \
-                               gout, tlag, taules);                                                                   \
+            CH4G_LAUNCH32G_(F_, FL_, TH_, A_, true, D_);                                                              \
         else                                                                                                          \
-            hipLaunchKernelGGL((k_step_fp32<F_, FL_, TH_, A_, false, D_>), grid, block, 0, s, P, fin, gin, fout,     \
-                               gout, tlag, taules);                                                                   \
+            CH4G_LAUNCH32G_(F_, FL_, TH_, A_, false, D_);                                                             \
     } while (0)
 #define CH4G_LAUNCH32_(F_, FL_, TH_, A_)                                                                              \
     do {                                                                                                              \
@@ -767,6 +778,7 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 #undef CH4G_LAUNCH32
 #undef CH4G_LAUNCH32_
 #undef CH4G_LAUNCH32D_
+#undef CH4G_LAUNCH32G_
 }
 
 // Cooperative multi-step launch when the lattice's blocks can all be
@@ -824,15 +836,23 @@ void launch_velocity_fp32(const KParams &P, const float *fin, const float *tlag,
 }
 
 void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
-                       bool dense, int arith64, bool ntl, hipStream_t s)
+                       bool dense, bool gen, int arith64, bool ntl, hipStream_t s)
 {
     dim3 grid((unsigned)nblocks), block(256);
+#define CH4G_BATCH32G_(FL_, TH_, A_, N_, D_)                                                                          \
+    do {                                                                                                              \
+        if (gen)                                                                                                      \
+            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, N_, D_, true>), grid, block, 0, s, tab, slots, nslots); \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, N_, D_, false>), grid, block, 0, s, tab, slots,         \
+                               nslots);                                                                               \
+    } while (0)
 #define CH4G_BATCH32D_(FL_, TH_, A_, D_)                                                                              \
     do {                                                                                                              \
         if (ntl)                                                                                                      \
-            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, true, D_>), grid, block, 0, s, tab, slots, nslots);    \
+            CH4G_BATCH32G_(FL_, TH_, A_, true, D_);                                                                   \
         else                                                                                                          \
-            hipLaunchKernelGGL((k_step_batch32<FL_, TH_, A_, false, D_>), grid, block, 0, s, tab, slots, nslots);   \
+            CH4G_BATCH32G_(FL_, TH_, A_, false, D_);                                                                  \
     } while (0)
 #define CH4G_BATCH32_(FL_, TH_, A_)                                                                                   \
     do {                                                                                                              \
@@ -857,6 +877,7 @@ void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots
 #undef CH4G_BATCH32
 #undef CH4G_BATCH32_
 #undef CH4G_BATCH32D_
+#undef CH4G_BATCH32G_
 }
 
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,
