@@ -590,7 +590,7 @@ void ch4g_destroy(ch4g_lattice *L)
     }
     destroy_graphs(L);
     particles_free(L);
-    void *scr[] = {L->tlag, L->tlag_alt, L->taules, L->vel, L->d_rul, L->d_dens, L->d_u, L->d_t, L->d_part, L->d_res, L->d_fs, L->d_gs,
+    void *scr[] = {L->tlag, L->tlag_alt, L->taules, L->vel, L->d_rul, L->d_u32, L->d_t32, L->d_dens, L->d_u, L->d_t, L->d_part, L->d_res, L->d_fs, L->d_gs,
                    L->d_row32, L->d_row64};
     for (void *p : scr)
         if (p) (void)hipFree(p);

@@ -163,6 +163,7 @@ struct PartParams {
     int *bsum;
     uint64_t *rng;
     const double *u, *t;     // fields of the step: u interleaved [3N], t [N]
+    const float *uf, *tf;    // the same in float (float-storage lattices), else null
     uint32_t flags;          // ch4g_lagrange_flag
     int fluid, thermal, wall_f, wall_t;
     double Tw_bot, Tw_top;   // T_wall - T_ref
@@ -248,6 +249,10 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
                       float *taules, hipStream_t s);
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,
                         double *gpost, double *dens, double *u, double *t, hipStream_t s);
+// u [N][3] and t of the current step in float (particle interpolation on
+// float-storage lattices), either may be null
+void launch_fields_f32(const KParams &P, const float *fin, const float *gin, const float *tlag, float *uf, float *tf,
+                       hipStream_t s);
 
 // Two steps in one launch for 2-D FP32-storage lattices (NZ = 1, NX a
 // multiple of 64, steady steps without HIT tables, LES or particles): reads

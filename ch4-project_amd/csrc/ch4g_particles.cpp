@@ -312,9 +312,22 @@ int particles_after_step(ch4g_lattice *L)
 {
     int rc = ensure_scratch(L);
     if (rc) return rc;
-    gather(L, nullptr, nullptr, nullptr, L->base.fluid ? L->d_u : nullptr, L->base.thermal ? L->d_t : nullptr);
-    HIPCHK(hipGetLastError());
     PartParams P = part_params(L);
+    if (L->dev32) {
+        // float-storage lattices interpolate float fields: half the bytes of
+        // the field pass, and the corner reads of 256^3 u fit the MALL
+        const size_t N = (size_t)L->N;
+        if (L->base.fluid && !L->d_u32 && (rc = dalloc(L, (void **)&L->d_u32, 3 * N * sizeof(float)))) return rc;
+        if (L->base.thermal && !L->d_t32 && (rc = dalloc(L, (void **)&L->d_t32, N * sizeof(float)))) return rc;
+        KParams K = params_for(L, L->first, (int)L->itime);
+        launch_fields_f32(K, (const float *)L->f[L->cur], (const float *)L->g[L->cur], (const float *)L->tlag,
+                          L->base.fluid ? L->d_u32 : nullptr, L->base.thermal ? L->d_t32 : nullptr, L->stream);
+        P.uf = L->base.fluid ? L->d_u32 : nullptr;
+        P.tf = L->base.thermal ? L->d_t32 : nullptr;
+    } else {
+        gather(L, nullptr, nullptr, nullptr, L->base.fluid ? L->d_u : nullptr, L->base.thermal ? L->d_t : nullptr);
+    }
+    HIPCHK(hipGetLastError());
     for (int s = 0; s < L->part->substeps; s++)
         if (launch_particles(P, L->stream)) return fail(-2, "particle launch failed: %s", hipGetErrorString(hipGetLastError()));
     return 0;

@@ -656,7 +656,8 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
                                                      const float *__restrict__ gin, const float *__restrict__ tlag,
                                                      double *__restrict__ fpost, double *__restrict__ gpost,
                                                      double *__restrict__ dens, double *__restrict__ u,
-                                                     double *__restrict__ t)
+                                                     double *__restrict__ t, float *__restrict__ uf,
+                                                     float *__restrict__ tf)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= P.NX) return;
@@ -680,6 +681,7 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
             if (gpost) gpost[q * N + n] = wq(q) * tb + (double)g[q];
         }
         if (t) t[n] = tb + s;
+        if (tf) tf[n] = (float)(tb + s);
     }
     if (P.fluid) {
         float f[Q];
@@ -710,6 +712,11 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
             u[3 * n] = ux;
             u[3 * n + 1] = uy;
             u[3 * n + 2] = uz;
+        }
+        if (uf) {
+            uf[3 * n] = (float)ux;
+            uf[3 * n + 1] = (float)uy;
+            uf[3 * n + 2] = (float)uz;
         }
     }
 }
@@ -885,7 +892,17 @@ void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, co
 {
     int bx = block_x32(P);
     dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ);
-    hipLaunchKernelGGL(k_gather_fp32, grid, dim3(bx), 0, s, P, fin, gin, tlag, fpost, gpost, dens, u, t);
+    hipLaunchKernelGGL(k_gather_fp32, grid, dim3(bx), 0, s, P, fin, gin, tlag, fpost, gpost, dens, u, t, nullptr,
+                       nullptr);
+}
+
+void launch_fields_f32(const KParams &P, const float *fin, const float *gin, const float *tlag, float *uf, float *tf,
+                       hipStream_t s)
+{
+    int bx = block_x32(P);
+    dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ);
+    hipLaunchKernelGGL(k_gather_fp32, grid, dim3(bx), 0, s, P, fin, gin, tlag, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, uf, tf);
 }
 
 }  // namespace ch4g

@@ -90,6 +90,21 @@ __device__ __forceinline__ bool stencil(const PartParams &P, double x, double y,
 // Velocity at a logical node including the halo of boundary_conditions_hydro
 // (boundary_conditions.c:2786-2875): periodic images, then u_g = -u on the
 // first ghost plane of each no-slip y wall.
+// the fields of the step: double for FP64 lattices, float for float storage
+template <typename F>
+__device__ __forceinline__ const F *field_u(const PartParams &P);
+template <>
+__device__ __forceinline__ const double *field_u<double>(const PartParams &P) { return P.u; }
+template <>
+__device__ __forceinline__ const float *field_u<float>(const PartParams &P) { return P.uf; }
+template <typename F>
+__device__ __forceinline__ const F *field_t(const PartParams &P);
+template <>
+__device__ __forceinline__ const double *field_t<double>(const PartParams &P) { return P.t; }
+template <>
+__device__ __forceinline__ const float *field_t<float>(const PartParams &P) { return P.tf; }
+
+template <typename F>
 __device__ __forceinline__ void fetch_u(const PartParams &P, int i, int j, int k, double v[3])
 {
     i = wrapi(i, P.NX);
@@ -105,20 +120,21 @@ __device__ __forceinline__ void fetch_u(const PartParams &P, int i, int j, int k
         j = wrapi(j, P.NY);
     }
     const size_t n = ((size_t)k * P.NY + j) * P.NX + i;
-    const double *u = P.u + 3 * n;
+    const F *u = field_u<F>(P) + 3 * n;
     if (ghost) {
-        v[0] = -1.0 * u[0];
-        v[1] = -1.0 * u[1];
-        v[2] = -1.0 * u[2];
+        v[0] = -1.0 * (double)u[0];
+        v[1] = -1.0 * (double)u[1];
+        v[2] = -1.0 * (double)u[2];
     } else {
-        v[0] = u[0];
-        v[1] = u[1];
-        v[2] = u[2];
+        v[0] = (double)u[0];
+        v[1] = (double)u[1];
+        v[2] = (double)u[2];
     }
 }
 
 // Temperature with the fixed-T ghost t_g = (2 (T_w - T_ref) / t - 1) t
 // (boundary_conditions.c:2928-2945).
+template <typename F>
 __device__ __forceinline__ double fetch_t(const PartParams &P, int i, int j, int k)
 {
     i = wrapi(i, P.NX);
@@ -133,7 +149,7 @@ __device__ __forceinline__ double fetch_t(const PartParams &P, int i, int j, int
     } else {
         j = wrapi(j, P.NY);
     }
-    const double t = P.t[((size_t)k * P.NY + j) * P.NX + i];
+    const double t = (double)field_t<F>(P)[((size_t)k * P.NY + j) * P.NX + i];
     if (side == 0) return t;
     const double Tw = side < 0 ? P.Tw_bot : P.Tw_top;
     const double fac = 2.0 * Tw / t - 1.0;
@@ -155,37 +171,38 @@ __device__ __forceinline__ void one_sided(const double a[3], const double b[3], 
 // gradient_vector with LAGRANGE on a two-node halo (gradients.c:78-499).
 // G = xx, xy, xz, yx, yy, yz, zx, zy, zz (first letter: component of u,
 // second: direction of the derivative).
+template <typename F>
 __device__ void grad_at(const PartParams &P, int i, int j, int k, double G[9])
 {
     double a[3], b[3], c[3];
     // centred differences: both neighbours always exist for nodes -1..N
-    fetch_u(P, i + 1, j, k, a);
-    fetch_u(P, i - 1, j, k, b);
+    fetch_u<F>(P, i + 1, j, k, a);
+    fetch_u<F>(P, i - 1, j, k, b);
     G[0] = (a[0] - b[0]) / 2.0;
     G[3] = (a[1] - b[1]) / 2.0;
     G[6] = (a[2] - b[2]) / 2.0;
-    fetch_u(P, i, j + 1, k, a);
-    fetch_u(P, i, j - 1, k, b);
+    fetch_u<F>(P, i, j + 1, k, a);
+    fetch_u<F>(P, i, j - 1, k, b);
     G[1] = (a[0] - b[0]) / 2.0;
     G[4] = (a[1] - b[1]) / 2.0;
     G[7] = (a[2] - b[2]) / 2.0;
-    fetch_u(P, i, j, k + 1, a);
-    fetch_u(P, i, j, k - 1, b);
+    fetch_u<F>(P, i, j, k + 1, a);
+    fetch_u<F>(P, i, j, k - 1, b);
     G[2] = (a[0] - b[0]) / 2.0;
     G[5] = (a[1] - b[1]) / 2.0;
     G[8] = (a[2] - b[2]) / 2.0;
     if (P.wall_f) {
         // LB_FLUID_BC_Y: one-sided at the first and last fluid rows
         if (j == 0) {
-            fetch_u(P, i, 0, k, a);
-            fetch_u(P, i, 1, k, b);
-            fetch_u(P, i, 2, k, c);
+            fetch_u<F>(P, i, 0, k, a);
+            fetch_u<F>(P, i, 1, k, b);
+            fetch_u<F>(P, i, 2, k, c);
             one_sided(a, b, c, true, G[1], G[4], G[7]);
         }
         if (j == P.NY - 1) {
-            fetch_u(P, i, P.NY - 1, k, a);
-            fetch_u(P, i, P.NY - 2, k, b);
-            fetch_u(P, i, P.NY - 3, k, c);
+            fetch_u<F>(P, i, P.NY - 1, k, a);
+            fetch_u<F>(P, i, P.NY - 2, k, b);
+            fetch_u<F>(P, i, P.NY - 3, k, c);
             one_sided(a, b, c, false, G[1], G[4], G[7]);
         }
         // LAGRANGE_INTERPOLATION_FIX: ghost nodes next to the interior take
@@ -193,44 +210,44 @@ __device__ void grad_at(const PartParams &P, int i, int j, int k, double G[9])
         // is kept for the directions that follow
         if (i == -1) {
             i = 0;
-            fetch_u(P, 0, j, k, a);
-            fetch_u(P, 1, j, k, b);
-            fetch_u(P, 2, j, k, c);
+            fetch_u<F>(P, 0, j, k, a);
+            fetch_u<F>(P, 1, j, k, b);
+            fetch_u<F>(P, 2, j, k, c);
             one_sided(a, b, c, true, G[0], G[3], G[6]);
         }
         if (i == P.NX) {
             i = P.NX - 1;
-            fetch_u(P, i, j, k, a);
-            fetch_u(P, i - 1, j, k, b);
-            fetch_u(P, i - 2, j, k, c);
+            fetch_u<F>(P, i, j, k, a);
+            fetch_u<F>(P, i - 1, j, k, b);
+            fetch_u<F>(P, i - 2, j, k, c);
             one_sided(a, b, c, false, G[0], G[3], G[6]);
         }
         if (j == -1) {
             j = 0;
-            fetch_u(P, i, 0, k, a);
-            fetch_u(P, i, 1, k, b);
-            fetch_u(P, i, 2, k, c);
+            fetch_u<F>(P, i, 0, k, a);
+            fetch_u<F>(P, i, 1, k, b);
+            fetch_u<F>(P, i, 2, k, c);
             one_sided(a, b, c, true, G[1], G[4], G[7]);
         }
         if (j == P.NY) {
             j = P.NY - 1;
-            fetch_u(P, i, j, k, a);
-            fetch_u(P, i, j - 1, k, b);
-            fetch_u(P, i, j - 2, k, c);
+            fetch_u<F>(P, i, j, k, a);
+            fetch_u<F>(P, i, j - 1, k, b);
+            fetch_u<F>(P, i, j - 2, k, c);
             one_sided(a, b, c, false, G[1], G[4], G[7]);
         }
         if (k == -1) {
             k = 0;
-            fetch_u(P, i, j, 0, a);
-            fetch_u(P, i, j, 1, b);
-            fetch_u(P, i, j, 2, c);
+            fetch_u<F>(P, i, j, 0, a);
+            fetch_u<F>(P, i, j, 1, b);
+            fetch_u<F>(P, i, j, 2, c);
             one_sided(a, b, c, true, G[2], G[5], G[8]);
         }
         if (k == P.NZ) {
             k = P.NZ - 1;
-            fetch_u(P, i, j, k, a);
-            fetch_u(P, i, j, k - 1, b);
-            fetch_u(P, i, j, k - 2, c);
+            fetch_u<F>(P, i, j, k, a);
+            fetch_u<F>(P, i, j, k - 1, b);
+            fetch_u<F>(P, i, j, k - 2, c);
             one_sided(a, b, c, false, G[2], G[5], G[8]);
         }
     }
@@ -256,7 +273,7 @@ __device__ __forceinline__ void wrap_keep(const PartParams &P, double &x, double
 // One sub-step of every particle: interpolate_vector_at_particles(u),
 // interpolate_scalar_at_particles(t) (lagrange.c:1211-1593), move_particles
 // (lagrange.c:2934-4025) and the wrap of sendrecv_particles.
-template <bool GRAD>
+template <bool GRAD, typename F>
 __global__ void __launch_bounds__(PB) k_particles(PartParams P)
 {
     const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
@@ -272,10 +289,25 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
         double axo = R(PAXO), ayo = R(PAYO), azo = R(PAZO);
         double ux = R(PUX), uy = R(PUY), uz = R(PUZ);
         double uxo = R(PUXO), uyo = R(PUYO), uzo = R(PUZO);
-        double G[9];
-        for (int m = 0; m < 9; m++) G[m] = R(PG + m);
-        double dtu[3] = {R(PDTU), R(PDTU + 1), R(PDTU + 2)};
-        double t = R(PT), told = R(PTOLD), dtt = R(PDTT);
+        // rows a configuration never changes are neither read nor written:
+        // the gradient without LAGRANGE_GRADIENT, Du/Dt without added mass,
+        // the temperature rows without LB_TEMPERATURE
+        const bool am = lflag(P, CH4G_LAGRANGE_ADDEDMASS);
+        double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if constexpr (GRAD)
+            for (int m = 0; m < 9; m++) G[m] = R(PG + m);
+        double dtu[3] = {0.0, 0.0, 0.0};
+        if (am) {
+            dtu[0] = R(PDTU);
+            dtu[1] = R(PDTU + 1);
+            dtu[2] = R(PDTU + 2);
+        }
+        double t = 0.0, told = 0.0, dtt = 0.0;
+        if (P.thermal) {
+            t = R(PT);
+            told = R(PTOLD);
+            dtt = R(PDTT);
+        }
         const double tau = R(PTAU), beta = R(PBETA);
 
         // ---- interpolation at the (wrapped) position
@@ -285,7 +317,7 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
                 double cu[3];
                 double v0 = 0, v1 = 0, v2 = 0;
                 for (int c = 0; c < 8; c++) {
-                    fetch_u(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), cu);
+                    fetch_u<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), cu);
                     if (c == 0) {
                         v0 = cu[0] * st.w[0];
                         v1 = cu[1] * st.w[0];
@@ -302,7 +334,7 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
                 if constexpr (GRAD) {
                     double acc[9], Gc[9];
                     for (int c = 0; c < 8; c++) {
-                        grad_at(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), Gc);
+                        grad_at<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), Gc);
                         for (int m = 0; m < 9; m++) acc[m] = (c == 0) ? Gc[m] * st.w[0] : acc[m] + Gc[m] * st.w[c];
                     }
                     for (int m = 0; m < 9; m++) G[m] = acc[m];
@@ -311,7 +343,7 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
             if (P.thermal) {
                 double s = 0;
                 for (int c = 0; c < 8; c++) {
-                    double tc = fetch_t(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c));
+                    double tc = fetch_t<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c));
                     s = (c == 0) ? tc * st.w[0] : s + tc * st.w[c];
                 }
                 t = s;
@@ -521,13 +553,18 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
         R(PUXO) = uxo;
         R(PUYO) = uyo;
         R(PUZO) = uzo;
-        for (int m = 0; m < 9; m++) R(PG + m) = G[m];
-        R(PDTU) = dtu[0];
-        R(PDTU + 1) = dtu[1];
-        R(PDTU + 2) = dtu[2];
-        R(PT) = t;
-        R(PTOLD) = told;
-        R(PDTT) = dtt;
+        if constexpr (GRAD)
+            for (int m = 0; m < 9; m++) R(PG + m) = G[m];
+        if (am) {
+            R(PDTU) = dtu[0];
+            R(PDTU + 1) = dtu[1];
+            R(PDTU + 2) = dtu[2];
+        }
+        if (P.thermal) {
+            R(PT) = t;
+            R(PTOLD) = told;
+            R(PDTT) = dtt;
+        }
 #undef R
     }
     if (P.bsum) {
@@ -695,10 +732,18 @@ int launch_particles(const PartParams &P, hipStream_t s)
 {
     if (P.np == 0) return 0;
     const int nb = blocks_of(P.np, PB);
-    if (P.flags & CH4G_LAGRANGE_GRADIENT)
-        hipLaunchKernelGGL(k_particles<true>, dim3(nb), dim3(PB), 0, s, P);
-    else
-        hipLaunchKernelGGL(k_particles<false>, dim3(nb), dim3(PB), 0, s, P);
+    const bool grad = (P.flags & CH4G_LAGRANGE_GRADIENT) != 0;
+    if (P.uf || P.tf) {
+        if (grad)
+            hipLaunchKernelGGL((k_particles<true, float>), dim3(nb), dim3(PB), 0, s, P);
+        else
+            hipLaunchKernelGGL((k_particles<false, float>), dim3(nb), dim3(PB), 0, s, P);
+    } else {
+        if (grad)
+            hipLaunchKernelGGL((k_particles<true, double>), dim3(nb), dim3(PB), 0, s, P);
+        else
+            hipLaunchKernelGGL((k_particles<false, double>), dim3(nb), dim3(PB), 0, s, P);
+    }
     if (P.bsum) {
         hipLaunchKernelGGL(k_draw_offsets, dim3(1), dim3(1024), 0, s, P.bsum, nb, P.rng);
         hipLaunchKernelGGL(k_draw_apply, dim3(nb), dim3(PB), 0, s, P);
