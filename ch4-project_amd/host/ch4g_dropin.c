@@ -272,19 +272,19 @@ static void need_particles(const char *fn)
 /* the particle arrays are allocated with the initial conditions */
 void allocate_particles(void) { need_particles("allocate_particles"); }
 
+/* initial_conditions_particles (lagrange.c:72-1021): types, positions from
+ * the lattice's drand48 stream; with restart the records then come from
+ * part.h5 (read_point_particle_h5, lagrange.c:128-131).                    */
 void initial_conditions_particles(int restart)
 {
     need_particles("initial_conditions_particles");
-    if (restart) {
-        fprintf(stderr, "Error message -> resume: load part.h5 records with ch4g_set_particles. Exit.\n");
-        exit(-1);
-    }
     if (S.itime != 0) {
         fprintf(stderr, "Error message -> initial_conditions_particles after the first step. Exit.\n");
         exit(-1);
     }
     if (ch4g_particles_init(S.lat, &S.rp.pcfg)) die("initial_conditions_particles");
     S.particles = 1;
+    if (restart) read_point_particle_h5();
 }
 
 /* from the second step on, between build_forcing and add_forcing
@@ -332,8 +332,9 @@ void move_particles(void)
     }
 }
 
-/* part.h5 / binary particle dumps are not written by this build (see
- * ch4g_get_particles for the records) */
+/* the periodic particle_<itime>.h5 snapshots (lagrange.c:1598-2595) are not
+ * written; ch4g_get_particles hands out the records, and the end-of-run
+ * part.h5 of write_point_particle_h5 holds them in the reference layout */
 void output_particles(void) { need_particles("output_particles"); }
 
 /* dump_particle_averages (lagrange.c:2598-2930): per particle type, FP64
@@ -581,7 +582,128 @@ void read_pop_h5(void)
     free(buf[0]);
     free(buf[1]);
 }
+/* part.h5 (lagrange.c:4231-4546, :4548-4827): /particles/point_particle, a
+ * compound of the point_particle members this lattice carries (name,
+ * position, velocity, acceleration and their previous values, tau_drag, the
+ * fluid velocity and its previous value; the velocity gradient with
+ * LAGRANGE_GRADIENT and beta_coeff with added mass; t with LB_TEMPERATURE)
+ * over [particle_number] records, the particles in array order first (the
+ * reference sizes the dataset by particle_number and writes the npart it
+ * holds); /properties carries particle_types.  The reference's interpolated
+ * temperature gradient (dx_t, dy_t, dz_t) is not carried by the device
+ * lattice and not written.                                                 */
+static hid_t particle_type(const ch4g_run_params *rp)
+{
+    const unsigned fl = rp->pcfg.flags;
+    hid_t t = H5Tcreate(H5T_COMPOUND, sizeof(ch4g_particle));
+#define M(f) H5Tinsert(t, #f, HOFFSET(ch4g_particle, f), H5T_NATIVE_DOUBLE)
+    M(name); M(x); M(y); M(z);
+    M(vx); M(vy); M(vz); M(vx_old); M(vy_old); M(vz_old);
+    M(ax); M(ay); M(az); M(ax_old); M(ay_old); M(az_old);
+    M(tau_drag);
+    M(ux); M(uy); M(uz); M(ux_old); M(uy_old); M(uz_old);
+    if (fl & CH4G_LAGRANGE_GRADIENT) {
+        M(dx_ux); M(dy_ux); M(dz_ux); M(dx_uy); M(dy_uy); M(dz_uy); M(dx_uz); M(dy_uz); M(dz_uz);
+        if (fl & CH4G_LAGRANGE_ADDEDMASS) M(beta_coeff);
+    }
+    if (rp->cfg.flags & CH4G_LB_TEMPERATURE) M(t);
+#undef M
+    return t;
+}
+
+static double particle_types(const ch4g_run_params *rp)
+{
+    const ch4g_particle_config *pc = &rp->pcfg;
+    double types = pc->tau_drag_types;
+    if ((pc->flags & CH4G_LAGRANGE_GRADIENT) && (pc->flags & CH4G_LAGRANGE_ADDEDMASS)) types *= pc->beta_coeff_types;
+    if (pc->flags & CH4G_LAGRANGE_ADD_TRACER) types += 1.0;
+    return types;
+}
+
+void write_point_particle_h5(void)
+{
+    need_particles("write_point_particle_h5");
+    const ch4g_run_params *rp = &S.rp;
+    int64_t n = 0;
+    if (ch4g_particles_count(S.lat, &n)) die("write_point_particle_h5");
+    const int64_t total = rp->pcfg.particle_number > n ? rp->pcfg.particle_number : n;
+    ch4g_particle *a = (ch4g_particle *)malloc(sizeof(ch4g_particle) * (size_t)(n > 0 ? n : 1));
+    if (!a) die("write_point_particle_h5: out of memory");
+    if (ch4g_get_particles(S.lat, a, n, &n)) die("write_point_particle_h5");
+    fprintf(stderr, "Writing file part.h5\n");
+    hid_t file = H5Fcreate("part.h5", H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (file < 0) die("write_point_particle_h5");
+    hid_t group = H5Gcreate2(file, "/particles", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    hsize_t dims[1] = {(hsize_t)total}, count[1] = {(hsize_t)n}, offset[1] = {0};
+    hid_t fspace = H5Screate_simple(1, dims, NULL);
+    hid_t mspace = H5Screate_simple(1, count, NULL);
+    H5Sselect_hyperslab(fspace, H5S_SELECT_SET, offset, NULL, count, NULL);
+    hid_t pt = particle_type(rp);
+    hid_t d = H5Dcreate2(group, "point_particle", pt, fspace, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    if (d < 0 || (n > 0 && H5Dwrite(d, pt, mspace, fspace, H5P_DEFAULT, a) < 0)) die("write_point_particle_h5");
+    H5Dclose(d);
+    H5Tclose(pt);
+    H5Sclose(mspace);
+    H5Sclose(fspace);
+    H5Gclose(group);
+    hid_t pg = H5Gcreate2(file, "/properties", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    hid_t as = H5Screate(H5S_SCALAR);
+    hid_t at = H5Acreate2(pg, "particle_types", H5T_NATIVE_DOUBLE, as, H5P_DEFAULT, H5P_DEFAULT);
+    const double types = particle_types(rp);
+    H5Awrite(at, H5T_NATIVE_DOUBLE, &types);
+    H5Aclose(at);
+    H5Sclose(as);
+    H5Gclose(pg);
+    H5Fclose(file);
+    free(a);
+}
+
+void read_point_particle_h5(void)
+{
+    need_particles("read_point_particle_h5");
+    const ch4g_run_params *rp = &S.rp;
+    hid_t file = H5Fopen("part.h5", H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (file < 0) {
+        fprintf(stderr, "Error message -> part.h5 cannot be opened. Exit.\n");
+        exit(-1);
+    }
+    hid_t d = H5Dopen2(file, "/particles/point_particle", H5P_DEFAULT);
+    hid_t sp = d >= 0 ? H5Dget_space(d) : -1;
+    hsize_t dims[1] = {0};
+    if (d < 0 || sp < 0 || H5Sget_simple_extent_ndims(sp) != 1 || H5Sget_simple_extent_dims(sp, dims, NULL) != 1) {
+        fprintf(stderr, "Error message -> /particles/point_particle in part.h5 is missing or not 1-D. Exit.\n");
+        exit(-1);
+    }
+    /* the reference reads particle_number records (npart of lagrange.c:128) */
+    const int64_t n = rp->pcfg.particle_number < (int64_t)dims[0] ? rp->pcfg.particle_number : (int64_t)dims[0];
+    ch4g_particle *a = (ch4g_particle *)calloc((size_t)(n > 0 ? n : 1), sizeof(ch4g_particle));
+    if (!a) die("read_point_particle_h5: out of memory");
+    hsize_t count[1] = {(hsize_t)n}, offset[1] = {0};
+    hid_t mspace = H5Screate_simple(1, count, NULL);
+    H5Sselect_hyperslab(sp, H5S_SELECT_SET, offset, NULL, count, NULL);
+    hid_t pt = particle_type(rp);
+    if (n > 0 && H5Dread(d, pt, mspace, sp, H5P_DEFAULT, a) < 0) die("read_point_particle_h5");
+    H5Tclose(pt);
+    H5Sclose(mspace);
+    H5Sclose(sp);
+    H5Dclose(d);
+    H5Fclose(file);
+    if (ch4g_set_particles(S.lat, a, n)) die("read_point_particle_h5");
+    free(a);
+}
 #else
+void write_point_particle_h5(void)
+{
+    fprintf(stderr, "Error message -> this drop-in layer was built without HDF5 (part.h5). Exit.\n");
+    exit(-1);
+}
+
+void read_point_particle_h5(void)
+{
+    fprintf(stderr, "Error message -> this drop-in layer was built without HDF5 (part.h5). Exit.\n");
+    exit(-1);
+}
+
 void write_pop_h5(void)
 {
     fprintf(stderr, "Error message -> this drop-in layer was built without HDF5 (pop.h5). Exit.\n");

@@ -34,7 +34,7 @@ int main(int argc, char **argv)
     initialization_forcing();
     /* the first hydro_fields / build_forcing / hydro_fields of main.c:26-48
        are part of initial_conditions on the device lattice */
-    if (lagr) initial_conditions_particles(0);
+    if (lagr) initial_conditions_particles(rp.resume);
 
     int itime = 0;
     double time_now;
@@ -79,7 +79,10 @@ int main(int argc, char **argv)
         }
     }
     measure_time();
-    if (rp.output_h5) write_pop_h5(); /* main.c:203-204 */
+    if (rp.output_h5) {
+        write_pop_h5(); /* main.c:203-209 */
+        if (lagr) write_point_particle_h5();
+    }
     ch4g_dropin_finalize();
     fprintf(stdout, "The End\n");
     return 0;

@@ -101,6 +101,11 @@ void interpolate_scalar_at_particles(my_double *f, char which_scalar);  /* lagra
 void move_particles(void);                                              /* lagrange.c:2934 */
 void output_particles(void);                                            /* lagrange.c:1598 */
 void dump_particle_averages(void);                                      /* lagrange.c:2598 */
+/* part.h5 (lagrange.c:4231, :4548): /particles/point_particle compound of
+ * the carried point_particle members over [particle_number], /properties
+ * particle_types.  Serial HDF5; without it both report an error and exit. */
+void write_point_particle_h5(void);                                     /* lagrange.c:4231 */
+void read_point_particle_h5(void);                                      /* lagrange.c:4548 */
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,8 @@ REFERENCE_NAMES = ["initial_conditions", "initialization_forcing", "compute_adve
                    "hydro_fields", "dump_averages", "measure_time", "allocate_particles",
                    "initial_conditions_particles", "add_particle_feedbacks", "boundary_conditions_hydro",
                    "interpolate_vector_at_particles", "interpolate_scalar_at_particles", "move_particles",
-                   "output_particles", "dump_particle_averages"]
+                   "output_particles", "dump_particle_averages", "write_pop_h5", "read_pop_h5",
+                   "write_point_particle_h5", "read_point_particle_h5"]
 
 
 class RunParams(C.Structure):
@@ -200,9 +201,9 @@ def test_ch4g_run_particles_matches_oracle(tmp_path):
 H5LIB = "/opt/conda/lib/libhdf5.so"
 
 
-def read_pop_h5(path, name, shape):
-    """Read one /population dataset through the HDF5 C library itself:
-    a [NZ][NY][NX] compound of doubles named p0..p18."""
+def read_compound_h5(path, name, names, shape):
+    """Read members `names` (doubles) of the compound dataset `name` through
+    the HDF5 C library itself; the dataset's extent must equal `shape`."""
     h5 = C.CDLL(H5LIB)
     hid = C.c_int64
     h5.H5open()
@@ -225,17 +226,22 @@ def read_pop_h5(path, name, shape):
     d = h5.H5Dopen2(f, name.encode(), 0)
     assert d >= 0, name
     sp = h5.H5Dget_space(d)
-    dims = (C.c_uint64 * 3)()
-    assert h5.H5Sget_simple_extent_dims(sp, dims, None) == 3
+    dims = (C.c_uint64 * len(shape))()
+    assert h5.H5Sget_simple_extent_dims(sp, dims, None) == len(shape)
     assert tuple(dims) == shape
-    t = h5.H5Tcreate(6, 19 * 8)  # H5T_COMPOUND
-    for i in range(19):
-        assert h5.H5Tinsert(t, b"p%d" % i, 8 * i, native_double) >= 0
-    buf = np.empty(shape + (19,))
+    t = h5.H5Tcreate(6, 8 * len(names))  # H5T_COMPOUND
+    for i, m in enumerate(names):
+        assert h5.H5Tinsert(t, m.encode(), 8 * i, native_double) >= 0, m
+    buf = np.empty(shape + (len(names),))
     assert h5.H5Dread(d, t, 0, 0, 0, buf.ctypes.data) >= 0
     for x, fn in ((t, "H5Tclose"), (sp, "H5Sclose"), (d, "H5Dclose"), (f, "H5Fclose")):
         getattr(h5, fn)(x)
     return buf
+
+
+def read_pop_h5(path, name, shape):
+    """One /population dataset: a [NZ][NY][NX] compound of doubles p0..p18."""
+    return read_compound_h5(path, name, ["p%d" % i for i in range(19)], shape)
 
 
 @pytest.mark.gpu
@@ -273,3 +279,40 @@ def test_pop_h5_dump_and_restart(tmp_path):
     assert last[0] == 500.0
     np.testing.assert_allclose(last[3], a["ux"], rtol=1e-6)
     np.testing.assert_allclose(last[1], a["ene"], rtol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(H5LIB), reason="serial HDF5 not installed")
+def test_part_h5_dump_and_restart(tmp_path):
+    """OUTPUT_H5 with LAGRANGE: part.h5 holds the particle records of the
+    last step under the reference's member names (equal to the oracle's, the
+    FP64 path being bit-identical), and a resume run reads them back."""
+    d1, d2 = tmp_path / "a", tmp_path / "b"
+    define = open(os.path.join(FIXP, "define.h")).read() + "#define OUTPUT_H5\n"
+    param = open(os.path.join(FIXP, "param.in")).read()
+    for d, prm in ((d1, param), (d2, param.replace("resume 0", "resume 1").replace("resume_u 0", "resume_u 1")
+                                  .replace("resume_t 0", "resume_t 1"))):
+        d.mkdir()
+        (d / "define.h").write_text(define)
+        (d / "param.in").write_text(prm)
+    r = subprocess.run([RUN], cwd=d1, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    assert lib.ch4g_read_run_params(str(d1 / "define.h").encode(), str(d1 / "param.in").encode(), C.byref(rp)) == 0
+    from oracle import Oracle
+    o = Oracle(rp.cfg)
+    o.initial_conditions()
+    o.particles_init(rp.pcfg)
+    o.step(int(rp.time_end))
+    ref = o.particles()
+    names = ["name", "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "ux", "uy", "uz", "tau_drag"]
+    got = read_compound_h5(d1 / "part.h5", "/particles/point_particle", names, (rp.pcfg.particle_number,))
+    n = len(ref)
+    for i, m in enumerate(names):
+        np.testing.assert_array_equal(got[:n, i], ref[m], err_msg=m)
+    for f in ("pop.h5", "part.h5"):
+        shutil.copy(d1 / f, d2 / f)
+    r = subprocess.run([RUN], cwd=d2, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "initialized from file" in r.stderr and "The End" in r.stdout
