@@ -96,13 +96,17 @@ def cpu_leg(target_s=12.0):
 
 
 def _serve_both(make_jobs):
-    """ch4g_serve on the same request stream, batched (continuous batching,
-    lock-step chunks) and one job at a time on the plain path."""
+    """ch4g_serve on the same request stream three ways: batched (continuous
+    batching, lock-step chunks, FP32-deviation storage); one job at a time
+    with the same storage (ch4g_step with graphs); one job at a time on the
+    plain FP64 path (the reference's method, no added pattern)."""
     import numpy as np
     from ch4g import abi, serve, workloads
     res = {}
-    for name, mode in (("batched", abi.SERVE_BATCHED), ("sequential", abi.SERVE_SEQUENTIAL)):
-        done = serve(make_jobs(), mode=mode)
+    for name, mode, prec in (("batched", abi.SERVE_BATCHED, abi.FP32_DEV),
+                             ("sequential", abi.SERVE_SEQUENTIAL, abi.FP32_DEV),
+                             ("sequential_plain_fp64", abi.SERVE_SEQUENTIAL, abi.FP64)):
+        done = serve(make_jobs(prec), mode=mode)
         ok = [j for j in done if j.status == 0]
         work = sum(workloads.job_work(j) for j in ok)
         span = max(j.finish_s for j in ok)
@@ -112,6 +116,7 @@ def _serve_both(make_jobs):
                      "p99_latency_s": round(float(np.percentile(lat, 99)), 3), "jobs_done": len(ok),
                      "node_updates": work}
     res["speedup"] = round(res["batched"]["mlups"] / res["sequential"]["mlups"], 3)
+    res["speedup_vs_plain_fp64"] = round(res["batched"]["mlups"] / res["sequential_plain_fp64"]["mlups"], 3)
     return res
 
 
@@ -123,11 +128,11 @@ def serve_leg(njobs, seed=2026, sweep_jobs=48):
     (2) A parameter sweep (workloads.sweep): a burst of small 2-D RB jobs,
     each too small to fill the GPU alone."""
     from ch4g import workloads
-    res = _serve_both(lambda: workloads.ensemble(njobs, seed=seed))
+    res = _serve_both(lambda prec: workloads.ensemble(njobs, seed=seed, precision=prec))
     res["workload"] = (f"{njobs} jobs of the served-ensemble stream (workloads.ensemble seed {seed}): Poisson "
                        "20 jobs/s, 2-D Rayleigh-Benard 64x64..1024x512 Ra 1e6..1e9 and 3-D channel 64^3..192^3, "
                        "steps log-uniform 1e3..1e5, FP32-deviation")
-    sw = _serve_both(lambda: workloads.sweep(sweep_jobs))
+    sw = _serve_both(lambda prec: workloads.sweep(sweep_jobs, precision=prec))
     sw["workload"] = (f"{sweep_jobs} jobs of a parameter sweep (workloads.sweep seed 7): burst at 200 jobs/s of "
                       "2-D Rayleigh-Benard 64x32..256x128, Ra 1e5..1e7, steps log-uniform 2e3..2e4, FP32-deviation")
     res["sweep"] = sw

@@ -468,31 +468,39 @@ __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__res
 
 // ---------------------------------------------------------------------------
 // Two steps per launch for 2-D lattices (NZ = 1, NX a multiple of 64).  A
-// block owns a T2_TX x T2_TY tile.  First step: the tile and a one-node halo
-// (periodic images; rows beyond a wall are computed from their periodic
-// images and never read) are advanced from the global state into LDS with the
+// block owns a 64 x TY tile (TY = 4 or 5) and has one thread per node of the
+// tile with its one-node halo.  First step: the tile and the halo (periodic
+// images; rows beyond a wall are computed from their periodic images and
+// never read) are advanced from the global state into LDS with the
 // single-step per-node code.  Second step: the tile's nodes gather from LDS
 // (same wall rules and row terms) and collide into the global output.  Per
-// two steps the state is read 1.55 times and written once instead of twice
-// each.  The temperature lag is read from tl_in and written to tl_out (halo
-// nodes of one block are tile nodes of another).
-constexpr int T2_TX = 64, T2_TY = 4;
-constexpr int T2_W = T2_TX + 2, T2_H = T2_TY + 2, T2_M = T2_W * T2_H;
+// two steps the state is read 1.55 (TY = 4) or 1.44 (TY = 5) times and
+// written once instead of twice each, and one launch boundary (ramp-up,
+// drain) is paid instead of two.  The temperature lag is read from tl_in and
+// written to tl_out (halo nodes of one block are tile nodes of another).
+constexpr int T2_TX = 64, T2_W = T2_TX + 2;
+template <int TY>
+struct Tile2 {
+    static constexpr int H = TY + 2, M = T2_W * H;
+    static constexpr int THREADS = (M + 63) / 64 * 64;  // 448 (TY = 4), 512 (TY = 5)
+    static_assert(THREADS >= T2_TX * TY && THREADS <= 512, "two-step block");
+};
 
-template <bool NTL>
+template <int M>
 __device__ __forceinline__ void gather_f_tile(const KParams &P, const Row &R, const Nbr &b, const float *t, int at,
                                               float f[Q])
 {
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const bool cross = P.wall_f && crosses_wall<q>(b);
-        float v = cross ? t[invq(q) * T2_M + at] : t[q * T2_M + at - cxq(q) - cyq(q) * T2_W];
+        float v = cross ? t[invq(q) * M + at] : t[q * M + at - cxq(q) - cyq(q) * T2_W];
         if constexpr (cyq(q) > 0) v += cross ? 0.f : wf(q) * R.dr_up;
         if constexpr (cyq(q) < 0) v += cross ? 0.f : wf(q) * R.dr_dn;
         f[q] = v;
     });
 }
 
+template <int M>
 __device__ __forceinline__ void gather_g_tile(const KParams &P, const Row &R, const Nbr &b, const float *t, int gofs,
                                               int at, float dTlag, float g[Q])
 {
@@ -501,26 +509,28 @@ __device__ __forceinline__ void gather_g_tile(const KParams &P, const Row &R, co
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const bool cross = P.wall_t && crosses_wall<q>(b);
-        float v = t[(gofs + q) * T2_M + (cross ? at : at - cxq(q) - cyq(q) * T2_W)];
+        float v = t[(gofs + q) * M + (cross ? at : at - cxq(q) - cyq(q) * T2_W)];
         if constexpr (cyq(q) > 0) v += cross ? wf(q) * fac_bot : wf(q) * R.dt_up;
         if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
         g[q] = v;
     });
 }
 
-template <bool FLUID, bool THERMAL, typename A, bool NTL>
-__global__ void __launch_bounds__(256) k_step2_fp32(KParams P, const float *__restrict__ fin,
-                                                    const float *__restrict__ gin, float *__restrict__ fout,
-                                                    float *__restrict__ gout, const float *__restrict__ tl_in,
-                                                    float *__restrict__ tl_out)
+template <bool FLUID, bool THERMAL, typename A, bool NTL, int TY>
+__global__ void __launch_bounds__(Tile2<TY>::THREADS) k_step2_fp32(KParams P, const float *__restrict__ fin,
+                                                                   const float *__restrict__ gin,
+                                                                   float *__restrict__ fout, float *__restrict__ gout,
+                                                                   const float *__restrict__ tl_in,
+                                                                   float *__restrict__ tl_out)
 {
+    constexpr int M = Tile2<TY>::M;
     constexpr int GOFS = FLUID ? Q : 0;
     constexpr int PLANES = GOFS + (THERMAL ? Q + 1 : 0);
-    __shared__ float t[PLANES * T2_M];
+    __shared__ float t[PLANES * M];
     const int NX = P.NX, NY = P.NY;
-    const int x0 = blockIdx.x * T2_TX, y0 = blockIdx.y * T2_TY;
-    // first step: tile + halo into LDS
-    for (int m = threadIdx.x; m < T2_M; m += blockDim.x) {
+    const int x0 = blockIdx.x * T2_TX, y0 = blockIdx.y * TY;
+    // first step: tile + halo into LDS, one node per thread
+    if (const int m = threadIdx.x; m < M) {
         const int lx = m % T2_W, ly = m / T2_W;
         int x = x0 - 1 + lx;
         x = x < 0 ? x + NX : (x >= NX ? x - NX : x);
@@ -545,7 +555,7 @@ __global__ void __launch_bounds__(256) k_step2_fp32(KParams P, const float *__re
             }
         }
         A ux = 0, uy = 0, uz = 0, tau_les = A(0);
-        const TileOut out{t, m, T2_M, GOFS};
+        const TileOut out{t, m, M, GOFS};
         if constexpr (FLUID) collide_fluid<A, false>(P, R, x, y, 0, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
         if constexpr (THERMAL) collide_thermal<A, false>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
     }
@@ -553,7 +563,7 @@ __global__ void __launch_bounds__(256) k_step2_fp32(KParams P, const float *__re
     // second step: the tile, from LDS into the global state
     const int tx = threadIdx.x % T2_TX, ty = threadIdx.x / T2_TX;
     const int x = x0 + tx, y = y0 + ty;
-    if (ty >= T2_TY || y >= NY) return;
+    if (ty >= TY || y >= NY) return;
     const int at = (ty + 1) * T2_W + tx + 1;
     const uint32_t n = (uint32_t)y * NX + x;
     const Row R = row_of(P, y);
@@ -561,10 +571,10 @@ __global__ void __launch_bounds__(256) k_step2_fp32(KParams P, const float *__re
     float dTlag = 0.f;
     float g[Q], f[Q];
     if constexpr (THERMAL) {
-        dTlag = t[(GOFS + Q) * T2_M + at];
-        gather_g_tile(P, R, b, t, GOFS, at, dTlag, g);
+        dTlag = t[(GOFS + Q) * M + at];
+        gather_g_tile<M>(P, R, b, t, GOFS, at, dTlag, g);
     }
-    if constexpr (FLUID) gather_f_tile<NTL>(P, R, b, t, at, f);
+    if constexpr (FLUID) gather_f_tile<M>(P, R, b, t, at, f);
     A dT = A(0);
     if constexpr (THERMAL) {
         if constexpr (sizeof(A) == 4) {
@@ -788,26 +798,43 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 #undef CH4G_LAUNCH32G_
 }
 
-// Cooperative multi-step launch when the lattice's blocks can all be
-// resident; returns 1 if it cannot be used (the caller launches per step).
-// Measured on 2-D fluid+T lattices: 13-18 % faster per step than the
-// single-step kernel once the state no longer fits the last-level cache
-// (2048x1024, 4096x2048), slower while it does (512x256: 9.35 vs 8.94 us,
-// 1024x512: 37.5 vs 32.8 us), so only streamed lattices use it.
+// Two steps per launch (k_step2_fp32) for 2-D lattices whose rows are whole
+// 64-node tiles, without the HIT / LES / two-way paths.  Measured on 2-D
+// fluid+T lattices against one launch per step: 512x256 9.19 -> 8.05 us per
+// step, 2048x1024 103.0 -> 92.6, 4096x2048 361.4 -> 340.2 (bit-identical).
 bool two_step_ok(const KParams &P)
 {
-    const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
-    return P.NZ == 1 && P.NX % T2_TX == 0 && !P.first && !P.hit && !P.les && !P.twoway &&
-           lattice_bytes > ((int64_t)192 << 20);
+    return P.NZ == 1 && P.NX % T2_TX == 0 && !P.first && !P.hit && !P.les && !P.twoway;
 }
 
 void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin, float *fout, float *gout,
                            const float *tl_in, float *tl_out, hipStream_t s)
 {
-    dim3 grid((unsigned)(P.NX / T2_TX), (unsigned)((P.NY + T2_TY - 1) / T2_TY)), block(T2_TX * T2_TY);
-    // streamed lattices only (two_step_ok): plain loads
+    // non-temporal loads while both lattices fit the last-level cache (as the
+    // single-step kernel); 64x5 tiles (fewer halo nodes) once there are at
+    // least two rounds of blocks (two blocks per CU by LDS), else 64x4 tiles
+    // (more, smaller blocks fill the CUs evenly)
+    const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
+    const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
+    const unsigned bx = (unsigned)(P.NX / T2_TX);
+    const bool ty5 = (int64_t)bx * ((P.NY + 4) / 5) >= 1024;
+    const dim3 g4(bx, (unsigned)((P.NY + 3) / 4)), g5(bx, (unsigned)((P.NY + 4) / 5));
+#define CH4G_T2N_(FL_, TH_, A_, N_)                                                                                   \
+    do {                                                                                                              \
+        if (ty5)                                                                                                      \
+            hipLaunchKernelGGL((k_step2_fp32<FL_, TH_, A_, N_, 5>), g5, dim3(Tile2<5>::THREADS), 0, s, P, fin, gin,  \
+                               fout, gout, tl_in, tl_out);                                                            \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step2_fp32<FL_, TH_, A_, N_, 4>), g4, dim3(Tile2<4>::THREADS), 0, s, P, fin, gin,  \
+                               fout, gout, tl_in, tl_out);                                                            \
+    } while (0)
 #define CH4G_T2_(FL_, TH_, A_)                                                                                        \
-    hipLaunchKernelGGL((k_step2_fp32<FL_, TH_, A_, false>), grid, block, 0, s, P, fin, gin, fout, gout, tl_in, tl_out)
+    do {                                                                                                              \
+        if (ntl)                                                                                                      \
+            CH4G_T2N_(FL_, TH_, A_, true);                                                                            \
+        else                                                                                                          \
+            CH4G_T2N_(FL_, TH_, A_, false);                                                                           \
+    } while (0)
 #define CH4G_T2(FL_, TH_)                                                                                             \
     do {                                                                                                              \
         if (P.arith64)                                                                                                \
@@ -823,6 +850,7 @@ void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin,
         CH4G_T2(false, true);
 #undef CH4G_T2
 #undef CH4G_T2_
+#undef CH4G_T2N_
 }
 
 void launch_velocity_fp32(const KParams &P, const float *fin, const float *tlag, hipStream_t s)

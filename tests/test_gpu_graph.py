@@ -64,14 +64,18 @@ def test_graph_replay_odd_counts_fd_les(precision):
         assert np.array_equal(pe, pg), n
 
 
-def test_two_step_launches_match_single_steps():
-    """A 2-D FP32-deviation lattice too large for the last-level cache
-    (1024x704 fluid+T, 219 MB) advances two steps per launch (LDS tile with a
-    one-node halo); the same lattice stepped through ch4g_step_batch uses
-    one launch per step.  Populations must agree bit for bit after odd and
-    even counts and after graph replays."""
+@pytest.mark.parametrize("NX,NY,precision", [(1024, 704, abi.FP32_DEV), (256, 102, abi.FP32_DEV),
+                                              (128, 62, abi.FP32_STORE)])
+def test_two_step_launches_match_single_steps(NX, NY, precision):
+    """2-D lattices with whole 64-node row tiles advance two steps per launch
+    (LDS tile with a one-node halo): 1024x704 fluid+T (219 MB, streamed, 64x5
+    tiles), 256x102 (cache-resident, non-temporal loads, 64x4 tiles, a
+    partial last tile row) and 128x62 with double arithmetic.  The same
+    lattice stepped through ch4g_step_batch uses one launch per step.
+    Populations must agree bit for bit after odd and even counts and after
+    graph replays."""
     from ch4g import step_batch
-    cfg = configs.rb2d(NX=1024, NY=704, Ra=1e8, precision=abi.FP32_DEV)
+    cfg = configs.rb2d(NX=NX, NY=NY, Ra=1e8 if NY > 500 else 1e6, precision=precision)
     two, one = Lattice(cfg), Lattice(cfg)
     for lat in (two, one):
         lat.initial_conditions()
