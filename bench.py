@@ -41,9 +41,10 @@ WORKLOADS = {
 
 
 def pmc_traffic(workload):
-    """HBM bytes per k_step_fp32 launch measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE (calibrated on a plain copy; profiles/k_step_fp32_pmc_traffic.md)."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_k_step_fp32.json")
+    """HBM bytes per launch of the workload's step kernel measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a plain copy;
+    profiles/*_pmc_traffic.md)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as fh:
             return json.load(fh)[workload]["traffic_bytes"]
@@ -64,8 +65,10 @@ def gpu_leg(precision, steps, warmup, workload="rb2d"):
     lat.synchronize()
     wall = time.perf_counter() - t0
     nodes = cfg.nodes()
+    spl = lat.steps_per_launch()
     out = {"nodes": nodes, "thermal": cfg.has(1 << 1), "wall_s": wall, "event_ms": ms, "mlups": nodes * steps / wall / 1e6,
-           "avg_launch_us": ms * 1e3 / steps}
+           "us_per_step": ms * 1e3 / steps, "steps_per_launch": spl, "avg_launch_us": ms * 1e3 / steps * spl,
+           "kernel": "k_step_fp64" if precision == 0 else ("k_step2_fp32" if spl == 2 else "k_step_fp32")}
     h = lat.dump_averages()
     out["nusselt"] = h["nuy"]
     lat.close()
@@ -266,7 +269,11 @@ def main():
     from ch4g import abi
     r32 = gpu_leg(abi.FP32_DEV, args.steps, args.warmup, args.workload)
     bpn = BYTES_PER_NODE["fp32dev"] if r32["thermal"] else BYTES_PER_NODE["fp32dev"] // 2
-    achieved = bpn * r32["nodes"] / (r32["avg_launch_us"] * 1e-6) / 1e9
+    # algorithmic bytes per launch (bytes per node update x nodes x steps per
+    # launch) over the average launch duration (HIP events on the lattice
+    # stream around the timed steps)
+    alg_launch = bpn * r32["nodes"] * r32["steps_per_launch"]
+    achieved = alg_launch / (r32["avg_launch_us"] * 1e-6) / 1e9
     line = {
         "metric": METRIC,
         "value": round(r32["mlups"], 2),
@@ -284,12 +291,14 @@ def main():
                    "nodes": r32["nodes"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload),
-                     "bytes_per_node": bpn, "avg_launch_us": round(r32["avg_launch_us"], 3)},
+                     "kernel": r32["kernel"], "steps_per_launch": r32["steps_per_launch"],
+                     "bytes_per_node": bpn, "algorithmic_bytes_per_launch": alg_launch,
+                     "avg_launch_us": round(r32["avg_launch_us"], 3)},
     }
     if not args.no_fp64:
         r64 = gpu_leg(abi.FP64, max(args.steps // 2, 1), args.warmup, args.workload)
         b64 = BYTES_PER_NODE["fp64"] if r64["thermal"] else BYTES_PER_NODE["fp64"] // 2
-        a64 = b64 * r64["nodes"] / (r64["avg_launch_us"] * 1e-6) / 1e9
+        a64 = b64 * r64["nodes"] * r64["steps_per_launch"] / (r64["avg_launch_us"] * 1e-6) / 1e9
         line["fp64_plain"] = {"mlups": round(r64["mlups"], 2), "avg_launch_us": round(r64["avg_launch_us"], 3),
                               "roofline_frac": round(a64 / HBM_PEAK_GBS, 4),
                               "fp32dev_speedup": round(r32["mlups"] / r64["mlups"], 3)}

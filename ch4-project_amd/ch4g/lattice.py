@@ -21,7 +21,7 @@ _lib = None
 EXPORTS = [
     "ch4g_last_error", "ch4g_device_count", "ch4g_create", "ch4g_destroy", "ch4g_initial_conditions",
     "ch4g_set_populations", "ch4g_get_populations", "ch4g_step", "ch4g_hydro_fields", "ch4g_dump_averages",
-    "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_dump_rulers", "ch4g_step_timed", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
+    "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_dump_rulers", "ch4g_step_timed", "ch4g_steps_per_launch", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
     "ch4g_footprint_bytes", "ch4g_particles_init", "ch4g_particles_count", "ch4g_get_particles",
     "ch4g_set_particles", "ch4g_step_batch", "ch4g_serve",
 ]
@@ -57,6 +57,7 @@ def lib():
     L.ch4g_set_hit_phases.argtypes = [P, dp]
     L.ch4g_step_timed.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
     L.ch4g_set_graphs.argtypes = [P, C.c_int]
+    L.ch4g_steps_per_launch.argtypes = [P, C.POINTER(C.c_int)]
     L.ch4g_synchronize.argtypes = [P]
     L.ch4g_itime.argtypes = [P]
     L.ch4g_itime.restype = C.c_int64
@@ -165,6 +166,12 @@ class Lattice:
         ms = C.c_double()
         check(self._L.ch4g_step_timed(self._h, int(n), C.byref(ms)))
         return ms.value
+
+    def steps_per_launch(self):
+        """Steps one steady-state launch advances (2: two-step LDS kernel)."""
+        k = C.c_int()
+        check(self._L.ch4g_steps_per_launch(self._h, C.byref(k)))
+        return k.value
 
     def set_graphs(self, enable):
         check(self._L.ch4g_set_graphs(self._h, 1 if enable else 0))

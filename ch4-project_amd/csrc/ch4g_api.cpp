@@ -545,6 +545,7 @@ int ch4g_create(const ch4g_config *cfg, ch4g_lattice **out)
         L->base.row32 = L->d_row32;
         L->base.row64 = L->d_row64;
     }
+    fill_float_params(L->base);
     // srand48(seed): X = (seed << 16) | 0x330E
     L->xsubi[0] = 0x330E;
     L->xsubi[1] = (unsigned short)((uint32_t)cfg->seed & 0xFFFFu);
@@ -747,6 +748,15 @@ int ch4g_set_graphs(ch4g_lattice *L, int enable)
 {
     if (!L) return fail(-1, "null lattice");
     L->use_graphs = enable != 0;
+    return 0;
+}
+
+int ch4g_steps_per_launch(const ch4g_lattice *L, int *steps)
+{
+    if (!L || !steps) return fail(-1, "null argument");
+    ch4g::KParams P = L->base;
+    P.first = 0;  // steady state
+    *steps = (L->dev32 && !L->part && two_step_ok(P)) ? 2 : 1;
     return 0;
 }
 

@@ -83,7 +83,26 @@ struct KParams {
     // planes [3][N] of the previous step (u shift) and of this step (Guo term)
     int twoway;
     const double *fb_prev, *fb_now;
+    // float copies of the step constants used by the float collision (the
+    // same round-to-nearest values the kernels would convert per use);
+    // filled by fill_float_params once the double values are final
+    float Fc_f[3], grav_f[3], beta_t_f, beta2_t_f, invtau_u_f, invtau_t_f, guo_fac_f, cw_top_f, cw_bot_f;
 };
+
+inline void fill_float_params(KParams &P)
+{
+    for (int a = 0; a < 3; a++) {
+        P.Fc_f[a] = (float)P.Fc[a];
+        P.grav_f[a] = (float)P.grav[a];
+    }
+    P.beta_t_f = (float)P.beta_t;
+    P.beta2_t_f = (float)P.beta2_t;
+    P.invtau_u_f = (float)P.invtau_u;
+    P.invtau_t_f = (float)P.invtau_t;
+    P.guo_fac_f = (float)P.guo_fac;
+    P.cw_top_f = (float)P.cw_top;
+    P.cw_bot_f = (float)P.cw_bot;
+}
 
 // Neighbour coordinates of a node for the pull gather: the source of
 // population q is x - c_q, periodic in every direction; crossings of y walls

@@ -63,9 +63,9 @@ template <bool GEN>
 __device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab, const double *fb, uint32_t n, int x,
                                            int y, int z, float theta, float F[3])
 {
-    F[0] = (float)P.Fc[0];
-    F[1] = (float)P.Fc[1];
-    F[2] = (float)P.Fc[2];
+    F[0] = P.Fc_f[0];
+    F[1] = P.Fc_f[1];
+    F[2] = P.Fc_f[2];
     if (GEN && P.hit) {
 #pragma unroll
         for (int ii = 0; ii < CH4G_HIT_NK; ii++) {
@@ -78,10 +78,10 @@ __device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab
         }
     }
     if (P.buoy) {
-        float fac = (float)P.beta_t * theta + (float)P.beta2_t * theta * theta;
-        F[0] += fac * (float)P.grav[0];
-        F[1] += fac * (float)P.grav[1];
-        F[2] += fac * (float)P.grav[2];
+        float fac = P.beta_t_f * theta + P.beta2_t_f * theta * theta;
+        F[0] += fac * P.grav_f[0];
+        F[1] += fac * P.grav_f[1];
+        F[2] += fac * P.grav_f[2];
     }
     if (GEN && P.twoway) {
         const size_t N = (size_t)P.N;
@@ -121,8 +121,8 @@ __device__ __forceinline__ void gather_g32(const KParams &P, const Row &R, const
                                            uint32_t n, float dTlag, float g[Q])
 {
     const size_t N = (size_t)P.N;
-    const float fac_top = 2.f * ((float)P.cw_top - dTlag);
-    const float fac_bot = 2.f * ((float)P.cw_bot - dTlag);
+    const float fac_top = 2.f * (P.cw_top_f - dTlag);
+    const float fac_bot = 2.f * (P.cw_bot_f - dTlag);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (FIRST) {
@@ -135,6 +135,49 @@ __device__ __forceinline__ void gather_g32(const KParams &P, const Row &R, const
             if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
             g[q] = v;
         }
+    });
+}
+
+// The same gathers with 32-bit byte offsets from the array base (arrays of
+// less than 4 GiB): a load is the array's scalar base plus one 32-bit vector
+// offset instead of a 64-bit address per lane and population.
+template <bool NTL>
+__device__ __forceinline__ float ldo(const float *base, uint32_t byteoff)
+{
+    return ldp<NTL>((const float *)((const char *)base + byteoff));
+}
+
+template <bool NTL>
+__device__ __forceinline__ void gather_f32_o32(const KParams &P, const Row &R, const Nbr &b,
+                                               const float *__restrict__ in, uint32_t n, float f[Q])
+{
+    const uint32_t N4 = (uint32_t)P.N * 4u;
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const bool cross = P.wall_f && crosses_wall<q>(b);
+        const uint32_t off = cross ? (uint32_t)invq(q) * N4 + n * 4u : (uint32_t)q * N4 + src_index<q>(b, P.NX, P.NY) * 4u;
+        float v = ldo<NTL>(in, off);
+        if constexpr (cyq(q) > 0) v += cross ? 0.f : wf(q) * R.dr_up;
+        if constexpr (cyq(q) < 0) v += cross ? 0.f : wf(q) * R.dr_dn;
+        f[q] = v;
+    });
+}
+
+template <bool NTL>
+__device__ __forceinline__ void gather_g32_o32(const KParams &P, const Row &R, const Nbr &b,
+                                               const float *__restrict__ in, uint32_t n, float dTlag, float g[Q])
+{
+    const uint32_t N4 = (uint32_t)P.N * 4u;
+    const float fac_top = 2.f * (P.cw_top_f - dTlag);
+    const float fac_bot = 2.f * (P.cw_bot_f - dTlag);
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const bool cross = P.wall_t && crosses_wall<q>(b);
+        const uint32_t off = (uint32_t)q * N4 + (cross ? n : src_index<q>(b, P.NX, P.NY)) * 4u;
+        float v = ldo<NTL>(in, off);
+        if constexpr (cyq(q) > 0) v += cross ? wf(q) * fac_bot : wf(q) * R.dt_up;
+        if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
+        g[q] = v;
     });
 }
 
@@ -154,6 +197,38 @@ __host__ __device__ constexpr int pair_q(int k)
 {
     constexpr int v[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
     return v[k];
+}
+
+// c_q . (x, y, z) for c_q with components in {0, +-1}, written as the one
+// or two signed terms (no multiplications by zero)
+template <int q, typename A>
+__device__ __forceinline__ A cdot(A x, A y, A z)
+{
+    constexpr int a = cxq(q), b = cyq(q), c = czq(q);
+    const A tx = a > 0 ? x : -x, ty = b > 0 ? y : -y, tz = c > 0 ? z : -z;
+    if constexpr (a != 0 && b != 0)
+        return tx + ty;
+    else if constexpr (a != 0 && c != 0)
+        return tx + tz;
+    else if constexpr (b != 0 && c != 0)
+        return ty + tz;
+    else if constexpr (a != 0)
+        return tx;
+    else if constexpr (b != 0)
+        return ty;
+    else
+        return tz;
+}
+
+// a step constant in the collision's arithmetic type: the host's float copy
+// for float arithmetic, the double value otherwise
+template <typename A>
+__device__ __forceinline__ A kc(double d, float f)
+{
+    if constexpr (sizeof(A) == 4)
+        return f;
+    else
+        return (A)d;
 }
 
 template <typename A>
@@ -180,6 +255,19 @@ struct NodeOut {
     __device__ void put_tlag(float v) const { tlag[n] = v; }
     __device__ float get_tau() const { return tau_src[tau_at]; }
     __device__ void put_tau(float v) const { taules[n] = v; }
+};
+
+// NodeOut with 32-bit byte offsets (arrays of less than 4 GiB)
+struct NodeOut32 {
+    float *__restrict__ f;
+    float *__restrict__ g;
+    float *__restrict__ tlag;
+    uint32_t n4, N4;
+    __device__ void put_f(int q, float v) const { stp((float *)((char *)f + ((uint32_t)q * N4 + n4)), v); }
+    __device__ void put_g(int q, float v) const { stp((float *)((char *)g + ((uint32_t)q * N4 + n4)), v); }
+    __device__ void put_tlag(float v) const { *(float *)((char *)tlag + n4) = v; }
+    __device__ float get_tau() const { return 0.f; }  // no LES on the two-step path
+    __device__ void put_tau(float) const {}
 };
 
 struct TileOut {
@@ -289,7 +377,7 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
         // weight-class products (w rho, w drho) shared by the populations
         const A wr[3] = {wa<A>(0) * rho, wa<A>(1) * rho, wa<A>(7) * rho};
         const A wd[3] = {wa<A>(0) * drho, wa<A>(1) * drho, wa<A>(7) * drho};
-        A omega = (A)P.invtau_u;
+        A omega = kc<A>(P.invtau_u, P.invtau_u_f);
         if (GEN && P.les && P.les_fd) {
             const A tau = les_tau_fd32<A>(P, (const float *)P.vel, x, y, z, n);
             out.put_tau((float)tau);
@@ -323,22 +411,34 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
         if (P.forced) force_at32<GEN>(P, P.hit_now, P.fb_now, n, x, y, z, R.offT + (float)dT, Fn);
         const A uF = ux * Fn[0] + uy * Fn[1] + uz * Fn[2];
         // Guo term folded into the equilibrium: f* = f - omega (f - eqG) with
-        // eqG = f_eq + G/omega, so the O(population) value is rounded twice
-        const A kg = (A)P.guo_fac / omega;
+        // eqG = f_eq + G/omega, so the O(population) value is rounded twice.
+        // Pair form: for the opposite pair (q, inv q), eqG = E +- D with
+        //   E = w rho (9/2 (c.u)^2 - 3/2 u^2 + kg (9 (c.u)(c.F) - 3 u.F)) + w drho
+        //   D = w rho (3 c.u + 3 kg c.F),   kg = (1 - omega/2) / omega
+        // (E even, D odd in c); the rest population has c = 0.
+        const A kg = kc<A>(P.guo_fac, P.guo_fac_f) / omega;
+        const A k3 = A(3) * kg, k9 = A(9) * kg;
+        const A c0 = -A(1.5) * u2 - k3 * uF;
         A r[Q];
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int wc = q == 0 ? 0 : (q <= 6 ? 1 : 2);
-            const A cx = cxq(q), cy = cyq(q), cz = czq(q);
-            const A cu = cx * ux + cy * uy + cz * uz;
-            A poly = A(3) * cu + A(4.5) * cu * cu - A(1.5) * u2;
-            if (P.forced) {
-                const A cF = cx * Fn[0] + cy * Fn[1] + cz * Fn[2];
-                poly += kg * (A(3) * (cF - uF) + A(9) * cu * cF);
-            }
-            const A eq = fma(wr[wc], poly, wd[wc]);
-            r[q] = fma(-omega, fa[q] - eq, fa[q]);
+        {
+            const A e = fma(wr[0], c0, wd[0]);
+            r[0] = fma(-omega, fa[0] - e, fa[0]);
         }
+        static_for<0, 9>([&](auto kk) {
+            constexpr int q = pair_q(decltype(kk)::value), qb = invq(q);
+            constexpr int wc = q <= 6 ? 1 : 2;
+            const A cu = cdot<q>(ux, uy, uz);
+            A sym = fma(A(4.5) * cu, cu, c0);
+            A anti = A(3) * cu;
+            if (P.forced) {
+                const A cF = cdot<q>((A)Fn[0], (A)Fn[1], (A)Fn[2]);
+                sym = fma(k9 * cu, cF, sym);
+                anti = fma(k3, cF, anti);
+            }
+            const A E = fma(wr[wc], sym, wd[wc]), D = wr[wc] * anti;
+            r[q] = fma(-omega, fa[q] - (E + D), fa[q]);
+            r[qb] = fma(-omega, fa[qb] - (E - D), fa[qb]);
+        });
         if constexpr (sizeof(A) == 4) {
             // Float weights miss the lattice identities (18 fl(1/18) != 1, ...)
             // by ~1e-8, which would scale mass and momentum systematically at
@@ -351,9 +451,10 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
                 Fp2[1] = (uy - jy * irho) * rho;
                 Fp2[2] = (uz - jz * irho) * rho;
             }
-            const A tx = jx + omega * Fp2[0] + (A)P.guo_fac * rho * (A)Fn[0];
-            const A ty = jy + omega * Fp2[1] + (A)P.guo_fac * rho * (A)Fn[1];
-            const A tz = jz + omega * Fp2[2] + (A)P.guo_fac * rho * (A)Fn[2];
+            const A gf = kc<A>(P.guo_fac, P.guo_fac_f);
+            const A tx = jx + omega * Fp2[0] + gf * rho * (A)Fn[0];
+            const A ty = jy + omega * Fp2[1] + gf * rho * (A)Fn[1];
+            const A tz = jz + omega * Fp2[2] + gf * rho * (A)Fn[2];
             const A mx = (r[1] - r[2]) + (r[7] - r[10]) + (r[8] - r[9]) + (r[11] - r[14]) + (r[13] - r[12]);
             const A my = (r[3] - r[4]) + (r[7] - r[10]) + (r[9] - r[8]) + (r[15] - r[18]) + (r[16] - r[17]);
             const A mz = (r[5] - r[6]) + (r[11] - r[14]) + (r[12] - r[13]) + (r[15] - r[18]) + (r[17] - r[16]);
@@ -384,18 +485,27 @@ __device__ __forceinline__ void collide_thermal(const KParams &P, const Row &R, 
         // tau_t_les (les.c:273-297) with prandtl 1: tau_t = 3(kappa + nu_les) + 1/2
         const A om = (GEN && P.les_t && P.fluid) ? A(1) / (A(3) * ((A)P.kappa + (tau_les - A(0.5) - A(3) * (A)P.nu) / A(3)) +
                                                      A(0.5))
-                                          : (A)P.invtau_t;
+                                          : kc<A>(P.invtau_t, P.invtau_t_f);
         const A wT[3] = {wa<A>(0) * T, wa<A>(1) * T, wa<A>(7) * T};
         const A wdT[3] = {wa<A>(0) * dT, wa<A>(1) * dT, wa<A>(7) * dT};
+        // pair form as for the fluid (no forcing term): eq = E +- D with
+        // E = w T (9/2 (c.u)^2 - 3/2 u^2) + w dT, D = w T 3 c.u
+        const A c0 = -A(1.5) * u2;
         A r[Q];
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            const int wc = q == 0 ? 0 : (q <= 6 ? 1 : 2);
-            const A cu = cxq(q) * ux + cyq(q) * uy + czq(q) * uz;
-            const A eq = fma(wT[wc], A(3) * cu + A(4.5) * cu * cu - A(1.5) * u2, wdT[wc]);
-            const A gq = (A)g[q];
-            r[q] = fma(-om, gq - eq, gq);
+        {
+            const A e = fma(wT[0], c0, wdT[0]);
+            const A g0 = (A)g[0];
+            r[0] = fma(-om, g0 - e, g0);
         }
+        static_for<0, 9>([&](auto kk) {
+            constexpr int q = pair_q(decltype(kk)::value), qb = invq(q);
+            constexpr int wc = q <= 6 ? 1 : 2;
+            const A cu = cdot<q>(ux, uy, uz);
+            const A E = fma(wT[wc], fma(A(4.5) * cu, cu, c0), wdT[wc]), D = wT[wc] * (A(3) * cu);
+            const A ga = (A)g[q], gb = (A)g[qb];
+            r[q] = fma(-om, ga - (E + D), ga);
+            r[qb] = fma(-om, gb - (E - D), gb);
+        });
         // the collision conserves the temperature exactly (float weights do not)
         if constexpr (sizeof(A) == 4) r[0] += dT - sum19(r);
 #pragma unroll
@@ -504,8 +614,8 @@ template <int M>
 __device__ __forceinline__ void gather_g_tile(const KParams &P, const Row &R, const Nbr &b, const float *t, int gofs,
                                               int at, float dTlag, float g[Q])
 {
-    const float fac_top = 2.f * ((float)P.cw_top - dTlag);
-    const float fac_bot = 2.f * ((float)P.cw_bot - dTlag);
+    const float fac_top = 2.f * (P.cw_top_f - dTlag);
+    const float fac_bot = 2.f * (P.cw_bot_f - dTlag);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const bool cross = P.wall_t && crosses_wall<q>(b);
@@ -543,9 +653,9 @@ __global__ void __launch_bounds__(Tile2<TY>::THREADS) k_step2_fp32(KParams P, co
         float g[Q], f[Q];
         if constexpr (THERMAL) {
             dTlag = tl_in[n];
-            gather_g32<false, NTL>(P, R, b, gin, n, dTlag, g);
+            gather_g32_o32<NTL>(P, R, b, gin, n, dTlag, g);
         }
-        if constexpr (FLUID) gather_f32<false, NTL>(P, R, b, fin, n, f);
+        if constexpr (FLUID) gather_f32_o32<NTL>(P, R, b, fin, n, f);
         A dT = A(0);
         if constexpr (THERMAL) {
             if constexpr (sizeof(A) == 4) {
@@ -584,7 +694,7 @@ __global__ void __launch_bounds__(Tile2<TY>::THREADS) k_step2_fp32(KParams P, co
         }
     }
     A ux = 0, uy = 0, uz = 0, tau_les = A(0);
-    const NodeOut out{fout, gout, tl_out, nullptr, nullptr, n, 0, (size_t)P.N};
+    const NodeOut32 out{fout, gout, tl_out, n * 4u, (uint32_t)P.N * 4u};
     if constexpr (FLUID) collide_fluid<A, false>(P, R, x, y, 0, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
     if constexpr (THERMAL) collide_thermal<A, false>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
 }
@@ -799,12 +909,15 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 }
 
 // Two steps per launch (k_step2_fp32) for 2-D lattices whose rows are whole
-// 64-node tiles, without the HIT / LES / two-way paths.  Measured on 2-D
+// 64-node tiles and whose planes span less than 4 GiB, without the HIT / LES /
+// two-way paths.  Measured on 2-D
 // fluid+T lattices against one launch per step: 512x256 9.19 -> 8.05 us per
 // step, 2048x1024 103.0 -> 92.6, 4096x2048 361.4 -> 340.2 (bit-identical).
 bool two_step_ok(const KParams &P)
 {
-    return P.NZ == 1 && P.NX % T2_TX == 0 && !P.first && !P.hit && !P.les && !P.twoway;
+    // 32-bit byte offsets into the Q population planes
+    const bool off32 = (int64_t)Q * P.N * 4 < ((int64_t)1 << 32);
+    return P.NZ == 1 && P.NX % T2_TX == 0 && off32 && !P.first && !P.hit && !P.les && !P.twoway;
 }
 
 void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin, float *fout, float *gout,
