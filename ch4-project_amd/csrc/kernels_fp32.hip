@@ -577,9 +577,10 @@ __global__ void __launch_bounds__(256) k_step_fp32(KParams P, const float *__res
 }
 
 // ---------------------------------------------------------------------------
-// Two steps per launch for 2-D lattices (NZ = 1, NX a multiple of 64).  A
-// block owns a 64 x TY tile (TY = 4 or 5) and has one thread per node of the
-// tile with its one-node halo.  First step: the tile and the halo (periodic
+// Two steps per launch for 2-D lattices (NZ = 1).  A block owns a 64 x TY
+// tile (TY = 4 or 5; the last tile of a row is narrower when NX is not a
+// multiple of 64) and has one thread per node of the tile with its one-node
+// halo.  First step: the tile and the halo (periodic
 // images; rows beyond a wall are computed from their periodic images and
 // never read) are advanced from the global state into LDS with the
 // single-step per-node code.  Second step: the tile's nodes gather from LDS
@@ -639,10 +640,14 @@ __global__ void __launch_bounds__(Tile2<TY>::THREADS) k_step2_fp32(KParams P, co
     __shared__ float t[PLANES * M];
     const int NX = P.NX, NY = P.NY;
     const int x0 = blockIdx.x * T2_TX, y0 = blockIdx.y * TY;
-    // first step: tile + halo into LDS, one node per thread
-    if (const int m = threadIdx.x; m < M) {
+    // columns of this tile: 64, or fewer in the last tile of a row when NX
+    // is not a multiple of 64 (its right halo column is then x = 0)
+    const int w = NX - x0 < T2_TX ? NX - x0 : T2_TX;
+    // first step: tile + halo into LDS, one node per thread (LDS columns
+    // beyond the halo of a narrower last tile are never read: skipped)
+    if (const int m = threadIdx.x; m < M && m % T2_W <= w + 1) {
         const int lx = m % T2_W, ly = m / T2_W;
-        int x = x0 - 1 + lx;
+        int x = x0 - 1 + lx;  // in [-1, NX]
         x = x < 0 ? x + NX : (x >= NX ? x - NX : x);
         int y = (y0 - 1 + ly) % NY;
         y = y < 0 ? y + NY : y;
@@ -673,7 +678,7 @@ __global__ void __launch_bounds__(Tile2<TY>::THREADS) k_step2_fp32(KParams P, co
     // second step: the tile, from LDS into the global state
     const int tx = threadIdx.x % T2_TX, ty = threadIdx.x / T2_TX;
     const int x = x0 + tx, y = y0 + ty;
-    if (ty >= TY || y >= NY) return;
+    if (ty >= TY || y >= NY || tx >= w) return;
     const int at = (ty + 1) * T2_W + tx + 1;
     const uint32_t n = (uint32_t)y * NX + x;
     const Row R = row_of(P, y);
@@ -908,16 +913,17 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 #undef CH4G_LAUNCH32G_
 }
 
-// Two steps per launch (k_step2_fp32) for 2-D lattices whose rows are whole
-// 64-node tiles and whose planes span less than 4 GiB, without the HIT / LES /
-// two-way paths.  Measured on 2-D
-// fluid+T lattices against one launch per step: 512x256 9.19 -> 8.05 us per
-// step, 2048x1024 103.0 -> 92.6, 4096x2048 361.4 -> 340.2 (bit-identical).
+// Two steps per launch (k_step2_fp32) for 2-D lattices whose planes span
+// less than 4 GiB, without the HIT / LES / two-way paths.  Measured on 2-D
+// fluid+T lattices against one launch per step (us per step, bit-identical):
+// 512x256 9.19 -> 7.97, 2048x1024 103.0 -> 88.7, 4096x2048 361.4 -> 325.6;
+// with narrower last tiles 78x40 4.69 -> 3.85, 1000x500 38.6 -> 28.9,
+// 1500x700 76.3 -> 48.2 (against the dense single-step grid).
 bool two_step_ok(const KParams &P)
 {
     // 32-bit byte offsets into the Q population planes
     const bool off32 = (int64_t)Q * P.N * 4 < ((int64_t)1 << 32);
-    return P.NZ == 1 && P.NX % T2_TX == 0 && off32 && !P.first && !P.hit && !P.les && !P.twoway;
+    return P.NZ == 1 && off32 && !P.first && !P.hit && !P.les && !P.twoway;
 }
 
 void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin, float *fout, float *gout,
@@ -929,7 +935,7 @@ void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin,
     // (more, smaller blocks fill the CUs evenly)
     const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
     const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
-    const unsigned bx = (unsigned)(P.NX / T2_TX);
+    const unsigned bx = (unsigned)((P.NX + T2_TX - 1) / T2_TX);
     const bool ty5 = (int64_t)bx * ((P.NY + 4) / 5) >= 1024;
     const dim3 g4(bx, (unsigned)((P.NY + 3) / 4)), g5(bx, (unsigned)((P.NY + 4) / 5));
 #define CH4G_T2N_(FL_, TH_, A_, N_)                                                                                   \

@@ -176,8 +176,8 @@ int ch4g_step_timed(ch4g_lattice *lat, int nsteps, double *elapsed_ms);
 int ch4g_set_graphs(ch4g_lattice *lat, int enable);
 
 /* Steps one steady-state launch advances: 2 for 2-D float-storage lattices
- * whose rows are whole 64-node tiles (k_step2_fp32: two steps through an LDS
- * tile), else 1.  Odd remainders run as single steps; results are identical
+ * without HIT forcing, LES or two-way particles (k_step2_fp32: two steps
+ * through an LDS tile), else 1.  Odd remainders run as single steps; results are identical
  * either way.  No reference counterpart (launch mechanics of this library). */
 int ch4g_steps_per_launch(const ch4g_lattice *lat, int *steps);
 

@@ -64,19 +64,25 @@ def test_graph_replay_odd_counts_fd_les(precision):
         assert np.array_equal(pe, pg), n
 
 
-@pytest.mark.parametrize("NX,NY,precision", [(1024, 704, abi.FP32_DEV), (256, 102, abi.FP32_DEV),
-                                              (128, 62, abi.FP32_STORE)])
-def test_two_step_launches_match_single_steps(NX, NY, precision):
-    """2-D lattices with whole 64-node row tiles advance two steps per launch
-    (LDS tile with a one-node halo): 1024x704 fluid+T (219 MB, streamed, 64x5
-    tiles), 256x102 (cache-resident, non-temporal loads, 64x4 tiles, a
-    partial last tile row) and 128x62 with double arithmetic.  The same
-    lattice stepped through ch4g_step_batch uses one launch per step.
-    Populations must agree bit for bit after odd and even counts and after
-    graph replays."""
-    from ch4g import step_batch
-    cfg = configs.rb2d(NX=NX, NY=NY, Ra=1e8 if NY > 500 else 1e6, precision=precision)
+@pytest.mark.parametrize("NX,NY,precision,fluid_only", [
+    (1024, 704, abi.FP32_DEV, False), (256, 102, abi.FP32_DEV, False), (128, 62, abi.FP32_STORE, False),
+    (300, 97, abi.FP32_DEV, False), (40, 30, abi.FP32_DEV, False), (96, 40, abi.FP32_DEV, True)])
+def test_two_step_launches_match_single_steps(NX, NY, precision, fluid_only):
+    """2-D lattices advance two steps per launch (LDS tile with a one-node
+    halo): 1024x704 fluid+T (219 MB, streamed, 64x5 tiles), 256x102
+    (cache-resident, non-temporal loads, 64x4 tiles, a partial last tile
+    row), 128x62 with double arithmetic, 300x97 (a narrower last tile in x
+    and in y), 40x30 (one tile narrower than 64) and a 96x40 fluid-only
+    channel.  The same lattice stepped through ch4g_step_batch uses one
+    launch per step.  Populations must agree bit for bit after odd and even
+    counts and after graph replays."""
+    from ch4g import step_batch, workloads
+    if fluid_only:
+        cfg = workloads.channel3d(NX, NY, 1, precision=precision)
+    else:
+        cfg = configs.rb2d(NX=NX, NY=NY, Ra=1e8 if NY > 500 else 1e6, precision=precision)
     two, one = Lattice(cfg), Lattice(cfg)
+    assert two.steps_per_launch() == 2
     for lat in (two, one):
         lat.initial_conditions()
         lat.step(1)
