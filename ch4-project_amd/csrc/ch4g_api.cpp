@@ -242,6 +242,14 @@ int launch_one_step(ch4g_lattice *L)
         if (rc) return rc;
     }
     KParams P = params_for(L, L->first, itime);
+    // the particles of the previous step move on the fields this kernel gathers
+    const bool fused = L->part && L->part->pending;
+    if (fused) {
+        int rc = ensure_float_fields(L);
+        if (rc) return rc;
+        P.uf_out = L->base.fluid ? L->d_u32 : nullptr;
+        P.tf_out = L->base.thermal ? L->d_t32 : nullptr;
+    }
     int in = L->cur, out = 1 - L->cur;
     if (P.les_fd) {
         // u of this step at every node first: the closure differentiates it
@@ -260,7 +268,18 @@ int launch_one_step(ch4g_lattice *L)
     L->cur = out;
     L->first = 0;
     L->itime = itime;
-    if (L->part) return particles_after_step(L);
+    if (L->part) {
+        if (fused) {
+            L->part->pending = 0;
+            int rc = move_particles_on_fields(L, itime - 1);
+            if (rc) return rc;
+        }
+        if (fields_fused(L)) {
+            L->part->pending = 1;  // after the next kernel, or at the end of ch4g_step
+            return 0;
+        }
+        return particles_after_step(L);
+    }
     return 0;
 }
 
@@ -741,7 +760,7 @@ int ch4g_step(ch4g_lattice *L, int nsteps)
         if (rc) return rc;
         s++;
     }
-    return 0;
+    return particles_flush(L);
 }
 
 int ch4g_set_graphs(ch4g_lattice *L, int enable)

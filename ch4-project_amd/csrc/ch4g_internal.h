@@ -87,6 +87,11 @@ struct KParams {
     // same round-to-nearest values the kernels would convert per use);
     // filled by fill_float_params once the double values are final
     float Fc_f[3], grav_f[3], beta_t_f, beta2_t_f, invtau_u_f, invtau_t_f, guo_fac_f, cw_top_f, cw_bot_f;
+    // when set (float storage, particles), the step kernel also writes the
+    // hydro fields of the previous step it has just gathered: u interleaved
+    // [3N] and T [N] in float, the fields the particle kernel interpolates
+    float *uf_out, *tf_out;
+    int part;  // particles ride on the step kernel's fields (the instance that can write them)
 };
 
 inline void fill_float_params(KParams &P)

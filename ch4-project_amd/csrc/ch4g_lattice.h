@@ -23,6 +23,9 @@ struct Particles {
     int *d_bsum = nullptr;       // per-block draw sums, then block offsets
     uint64_t *d_rng = nullptr;   // 48-bit drand48 state while particles own the stream
     int rng_on_device = 0;
+    // the particle update of the last step is still to run (fields_fused:
+    // it runs after the next step kernel has written that step's fields)
+    int pending = 0;
     double dt = 1.0;             // time_dt_lagr
     int substeps = 1;
     // two-way coupling: feedback force planes [3][N] per step parity, and the
@@ -100,6 +103,10 @@ void destroy_graphs(ch4g_lattice *L);
 // particle hooks of the step sequence (ch4g_particles.cpp)
 int particles_before_step(ch4g_lattice *L, int itime);  // add_particle_feedbacks
 int particles_after_step(ch4g_lattice *L);               // main.c:163-183
+bool fields_fused(const ch4g_lattice *L);                // the step kernel writes the particles' fields
+int ensure_float_fields(ch4g_lattice *L);
+int move_particles_on_fields(ch4g_lattice *L, int64_t step);
+int particles_flush(ch4g_lattice *L);                    // the update owed by a fused sequence
 void particles_free(ch4g_lattice *L);
 int rng_to_host(ch4g_lattice *L);                        // before host drand48 use
 int rng_to_device(ch4g_lattice *L);                      // after it

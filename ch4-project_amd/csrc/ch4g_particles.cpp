@@ -273,6 +273,7 @@ void particles_free(ch4g_lattice *L)
     delete P;
     L->part = nullptr;
     L->base.twoway = 0;
+    L->base.part = 0;
     L->base.fb_prev = L->base.fb_now = nullptr;
 }
 
@@ -306,31 +307,69 @@ int particles_before_step(ch4g_lattice *L, int itime)
     return 0;
 }
 
-// main.c:163-183: hydro fields of the step (with their halo rules applied in
-// the particle kernel), then time_dt_lagr_factor sub-steps.
-int particles_after_step(ch4g_lattice *L)
+// Float-storage lattices interpolate float fields (half the bytes of the
+// field pass; the corner reads of 256^3 u fit the MALL).
+int ensure_float_fields(ch4g_lattice *L)
+{
+    const size_t N = (size_t)L->N;
+    int rc;
+    if (L->base.fluid && !L->d_u32 && (rc = dalloc(L, (void **)&L->d_u32, 3 * N * sizeof(float)))) return rc;
+    if (L->base.thermal && !L->d_t32 && (rc = dalloc(L, (void **)&L->d_t32, N * sizeof(float)))) return rc;
+    return 0;
+}
+
+// Whether the step kernel writes the fields the particles interpolate: float
+// storage, no two-way feedback (the next step needs the moved particles
+// first) and no device random draws (their order with the host's HIT phase
+// walk is part of the stream).  The particle update of step s then runs
+// after the kernel of step s + 1, which has gathered the fields of step s:
+// the same inputs, without a separate field pass.
+bool fields_fused(const ch4g_lattice *L)
+{
+    return L->part && L->dev32 && !L->base.twoway && !L->part->rng_on_device;
+}
+
+// time_dt_lagr_factor particle sub-steps of step `step` (main.c:169-182) from
+// the fields of that step already in place (float or FP64 planes).
+int move_particles_on_fields(ch4g_lattice *L, int64_t step)
 {
     int rc = ensure_scratch(L);
     if (rc) return rc;
     PartParams P = part_params(L);
+    P.first = step == 1;
     if (L->dev32) {
-        // float-storage lattices interpolate float fields: half the bytes of
-        // the field pass, and the corner reads of 256^3 u fit the MALL
-        const size_t N = (size_t)L->N;
-        if (L->base.fluid && !L->d_u32 && (rc = dalloc(L, (void **)&L->d_u32, 3 * N * sizeof(float)))) return rc;
-        if (L->base.thermal && !L->d_t32 && (rc = dalloc(L, (void **)&L->d_t32, N * sizeof(float)))) return rc;
+        P.uf = L->base.fluid ? L->d_u32 : nullptr;
+        P.tf = L->base.thermal ? L->d_t32 : nullptr;
+    }
+    for (int s = 0; s < L->part->substeps; s++)
+        if (launch_particles(P, L->stream)) return fail(-2, "particle launch failed: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+// main.c:163-183: hydro fields of the step (with their halo rules applied in
+// the particle kernel), then time_dt_lagr_factor sub-steps.
+int particles_after_step(ch4g_lattice *L)
+{
+    int rc = ensure_scratch(L);  // the FP64 field planes the gather writes
+    if (rc) return rc;
+    if (L->dev32) {
+        if ((rc = ensure_float_fields(L))) return rc;
         KParams K = params_for(L, L->first, (int)L->itime);
         launch_fields_f32(K, (const float *)L->f[L->cur], (const float *)L->g[L->cur], (const float *)L->tlag,
                           L->base.fluid ? L->d_u32 : nullptr, L->base.thermal ? L->d_t32 : nullptr, L->stream);
-        P.uf = L->base.fluid ? L->d_u32 : nullptr;
-        P.tf = L->base.thermal ? L->d_t32 : nullptr;
     } else {
         gather(L, nullptr, nullptr, nullptr, L->base.fluid ? L->d_u : nullptr, L->base.thermal ? L->d_t : nullptr);
     }
     HIPCHK(hipGetLastError());
-    for (int s = 0; s < L->part->substeps; s++)
-        if (launch_particles(P, L->stream)) return fail(-2, "particle launch failed: %s", hipGetErrorString(hipGetLastError()));
-    return 0;
+    return move_particles_on_fields(L, L->itime);
+}
+
+// The particle update still owed by a fused sequence (end of ch4g_step).
+int particles_flush(ch4g_lattice *L)
+{
+    if (!L->part || !L->part->pending) return 0;
+    L->part->pending = 0;
+    return particles_after_step(L);
 }
 
 }  // namespace ch4g
@@ -403,6 +442,7 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         if ((rc = rng_to_device(L))) goto err;
     }
     if ((rc = upload(L, h.data(), np))) goto err;
+    L->base.part = fields_fused(L) ? 1 : 0;
     return 0;
 err:
     particles_free(L);

@@ -547,6 +547,17 @@ __device__ __forceinline__ void step_node32(const KParams &P, const float *__res
     const NodeOut out{fout, gout, tlag, taules, taules, n, n, (size_t)P.N};
     if constexpr (FLUID) collide_fluid<A, GEN>(P, R, x, y, z, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
     if constexpr (THERMAL) collide_thermal<A, GEN>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
+    if constexpr (GEN && !FIRST) {
+        // hydro fields of the previous step (hydro_fields, lb.c:420-540) for
+        // the particles: the gathered moments and the velocity with its half
+        // force, as the collision used them
+        if (FLUID && P.uf_out) {
+            P.uf_out[3 * (size_t)n] = (float)ux;
+            P.uf_out[3 * (size_t)n + 1] = (float)uy;
+            P.uf_out[3 * (size_t)n + 2] = (float)uz;
+        }
+        if (THERMAL && P.tf_out) P.tf_out[n] = (float)((sizeof(A) == 4 ? (A)R.tb : (A)P.row64[2 * y + 1]) + dT);
+    }
 }
 
 // Row grid (blocks along x of one (y, z) row: row quantities block-uniform)
@@ -869,7 +880,7 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
     // both lattices resident in the 256 MB last-level cache?
     const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
     const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
-    const bool gen = P.hit || P.les || P.twoway;
+    const bool gen = P.hit || P.les || P.twoway || P.part;
 #define CH4G_LAUNCH32G_(F_, FL_, TH_, A_, N_, D_)                                                                     \
     do {                                                                                                              \
         if (gen)                                                                                                      \
