@@ -283,6 +283,10 @@ int launch_one_step(ch4g_lattice *L)
     return 0;
 }
 
+}  // namespace
+
+namespace ch4g {
+
 // Two steady-state steps in one launch (2-D FP32-storage lattices, see
 // two_step_ok): the lag moves to the other array.
 bool two_step_eligible(const ch4g_lattice *L)
@@ -295,6 +299,10 @@ int ensure_tlag_alt(ch4g_lattice *L)
     if (L->base.thermal && !L->tlag_alt) return dalloc(L, &L->tlag_alt, (size_t)L->N * L->rsz);
     return 0;
 }
+
+}  // namespace ch4g
+
+namespace {
 
 int launch_two_steps(ch4g_lattice *L)
 {

@@ -239,21 +239,30 @@ inline int64_t batch_tiles(int NX, int NY, int NZ)
     return (int64_t)NY * NZ * ((NX + BATCH_WAVE - 1) / BATCH_WAVE);
 }
 
-// One lattice of a batched step: its constants and its two state buffers.
+// One lattice of a batched step: its constants and its two state buffers
+// (and, for two-step launches, its two lag arrays).
 struct BatchEntry {
     KParams P;
     void *f[2], *g[2];
     void *tlag, *taules;
+    void *tlag2;        // second lag array (two-step launches), else null
     int cur;            // buffer holding the state at the start of the chunk
 };
-// One lattice taking part in a batched launch: entry (bit 31: parity of
-// that lattice's step within the chunk) and its first wave tile in the
-// launch.  A launch's slots are sorted by tile0; the last one is a sentinel
-// whose tile0 is the launch's wave-tile count.
+// One lattice taking part in a batched launch: entry index (low 29 bits;
+// bit 31: parity of the lattice's population buffer at this launch; for
+// two-step launches bit 30: a single step, bit 29: parity of its lag array)
+// and its first tile in the launch (wave tiles, or 64 x 4 block tiles for
+// two-step launches).  A launch's slots are sorted by tile0; the last one is
+// a sentinel whose tile0 is the launch's tile count.
 struct BatchSlot {
     uint32_t entry;
     uint32_t tile0;
 };
+constexpr uint32_t BATCH_ENTRY_MASK = 0x1fffffffu;
+constexpr uint32_t BATCH_SINGLE = 1u << 30;
+constexpr uint32_t BATCH_LAG_PARITY = 1u << 29;
+// 64 x 4 block tiles of a 2-D lattice in the batched two-step launch
+inline int64_t batch2_tiles(int NX, int NY) { return (int64_t)((NX + 63) / 64) * ((NY + 3) / 4); }
 // one launch covers the lattices of a storage kind (FP32 float/double
 // arithmetic, FP64) listed in its slots
 // population sets of a batched launch (one kernel instance per kind)
@@ -263,6 +272,10 @@ void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots
                        bool dense, bool gen, int arith64, bool ntl, hipStream_t s);
 void launch_batch_fp64(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
                        bool dense, hipStream_t s);
+// two steps per launch for 2-D float-storage lattices (two_step_ok); nblocks
+// = 64 x 4 tiles of the launch
+void launch_batch2_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
+                        int arith64, bool ntl, hipStream_t s);
 
 // Kernel launchers (stream-ordered).
 void launch_step_fp64(const KParams &P, const double *fin, const double *gin, double *fout, double *gout,

@@ -102,6 +102,8 @@ void destroy_graphs(ch4g_lattice *L);
 
 // particle hooks of the step sequence (ch4g_particles.cpp)
 int particles_before_step(ch4g_lattice *L, int itime);  // add_particle_feedbacks
+bool two_step_eligible(const ch4g_lattice *L);           // steady steps go two per launch
+int ensure_tlag_alt(ch4g_lattice *L);                    // the second lag array of two-step launches
 int particles_after_step(ch4g_lattice *L);               // main.c:163-183
 bool fields_fused(const ch4g_lattice *L);                // the step kernel writes the particles' fields
 int ensure_float_fields(ch4g_lattice *L);

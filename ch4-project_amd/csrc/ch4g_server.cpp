@@ -24,16 +24,21 @@ namespace {
 // storage kind; lattice i takes steps[i] of them, spread evenly over the
 // chunk, so every launch carries a similar share of the work.  Each launch
 // has its own short slot list (lattice, parity of its step, first block).
+// 2-D float-storage lattices (two_step_eligible) form groups of their own
+// whose launches advance two steps each: steps[i] takes steps[i] / 2 such
+// launches and, for an odd share, one single step at the end.
 struct Batch {
     struct Group {
         int dev32, arith64, kind;
         bool dense;                       // lanes decode their own rows (dense_rows)
         bool gen;                         // LES lattices (the kernel instance with those paths)
         bool ntl;
+        bool two;                         // two steps per launch (k_step2_batch32)
         int first, count;                 // range in the entry table
+        int nlaunch;                      // launches of the group per chunk
         std::vector<int64_t> off;         // per launch: first slot in the slot table
         std::vector<int> nslot;           // per launch: slots (without the sentinel)
-        std::vector<int64_t> nblk;        // per launch: wave tiles
+        std::vector<int64_t> nblk;        // per launch: wave tiles (64 x 4 tiles for two-step groups)
     };
     std::vector<BatchEntry> host;
     std::vector<BatchSlot> slots;
@@ -51,6 +56,8 @@ struct Batch {
     hipEvent_t fork = nullptr;
     std::vector<ch4g_lattice *> lats;
     std::vector<int> steps;   // steps of each lattice per chunk
+    std::vector<char> two;    // lattice i goes two steps per launch
+    std::vector<char> flips;  // a chunk changes lattice i's buffer or lag parity
     int chunk = 0;            // launches per chunk = max steps
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev = nullptr;
@@ -107,19 +114,28 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
     B.slots.clear();
     B.groups.clear();
     B.chunk = 0;
+    B.two.assign(lats.size(), 0);
+    B.flips.assign(lats.size(), 0);
     for (size_t i = 0; i < lats.size(); i++) {
         int rc = check_member(lats[i]);
         if (rc) return rc;
         if (steps[i] < 1) return fail(-1, "steps per chunk must be >= 1");
         B.chunk = std::max(B.chunk, steps[i]);
+        ch4g_lattice *L = lats[i];
+        B.two[i] = two_step_eligible(L);
+        if (B.two[i] && (rc = ensure_tlag_alt(L))) return rc;
+        // two-step lattices: the population buffer flips with every launch,
+        // the lag array with every two-step launch
+        const int pairs = steps[i] / 2, launches = pairs + steps[i] % 2;
+        B.flips[i] = B.two[i] ? ((launches & 1) || (pairs & 1)) : (steps[i] & 1);
     }
-    const int K = B.chunk;
     std::vector<int> order(lats.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
     auto key = [&](int i) {
         const ch4g_lattice *L = lats[(size_t)i];
-        return (L->base.les << 5) | (L->dev32 << 4) | (L->base.arith64 << 3) | ((int)dense_rows(L->cfg.NX) << 2) |
-               kind_of(L);
+        const int two = B.two[(size_t)i];
+        return (L->base.les << 6) | (two << 5) | (L->dev32 << 4) | (L->base.arith64 << 3) |
+               ((int)(!two && dense_rows(L->cfg.NX)) << 2) | kind_of(L);
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
     for (size_t i = 0; i < order.size();) {
@@ -129,15 +145,27 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
         G.dev32 = L0->dev32;
         G.arith64 = L0->base.arith64;
         G.kind = kind_of(L0);
-        G.dense = dense_rows(L0->cfg.NX);
+        G.two = B.two[(size_t)order[i]] != 0;
+        G.dense = !G.two && dense_rows(L0->cfg.NX);
         G.gen = L0->base.les != 0;
         G.first = (int)B.host.size();
         G.count = 0;
         int64_t bytes = 0;
-        // step index of each member at each launch (-1: not in that launch)
+        // members of the group and their launches per chunk (two-step
+        // groups: pairs, then the single step of an odd share)
+        size_t i1 = i;
+        while (i1 < order.size() && key(order[i1]) == k0) i1++;
+        int KG = 0;
+        for (size_t j = i; j < i1; j++) {
+            const int st = steps[(size_t)order[j]];
+            KG = std::max(KG, G.two ? st / 2 + st % 2 : st);
+        }
+        G.nlaunch = KG;
+        // launch index of each member at each launch (-1: not in that launch)
         std::vector<std::vector<int>> at;
+        std::vector<int> npair;
         std::vector<int64_t> ntile;
-        for (; i < order.size() && key(order[i]) == k0; i++) {
+        for (; i < i1; i++) {
             ch4g_lattice *L = lats[(size_t)order[i]];
             const int st = steps[(size_t)order[i]];
             BatchEntry E;
@@ -148,26 +176,35 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
                 E.g[b] = L->g[b];
             }
             E.tlag = L->tlag;
+            E.tlag2 = G.two ? L->tlag_alt : nullptr;
             E.taules = L->taules;
             E.cur = L->cur;
             B.host.push_back(E);
-            std::vector<int> a((size_t)K, -1);
-            for (int s_ = 0; s_ < st; s_++) a[(size_t)(((int64_t)s_ * K) / st)] = s_;
+            const int nl = G.two ? st / 2 + st % 2 : st;
+            std::vector<int> a((size_t)KG, -1);
+            for (int s_ = 0; s_ < nl; s_++) a[(size_t)(((int64_t)s_ * KG) / nl)] = s_;
             at.push_back(a);
-            ntile.push_back(batch_tiles(L->cfg.NX, L->cfg.NY, L->cfg.NZ));
+            npair.push_back(G.two ? st / 2 : 0);
+            ntile.push_back(G.two ? batch2_tiles(L->cfg.NX, L->cfg.NY) : batch_tiles(L->cfg.NX, L->cfg.NY, L->cfg.NZ));
             bytes += 2 * (int64_t)Q * L->N * (L->dev32 ? 4 : 8) * ((L->base.fluid ? 1 : 0) + (L->base.thermal ? 1 : 0));
             G.count++;
         }
-        G.off.assign((size_t)K, 0);
-        G.nslot.assign((size_t)K, 0);
-        G.nblk.assign((size_t)K, 0);
-        for (int k = 0; k < K; k++) {
+        G.off.assign((size_t)KG, 0);
+        G.nslot.assign((size_t)KG, 0);
+        G.nblk.assign((size_t)KG, 0);
+        for (int k = 0; k < KG; k++) {
             G.off[(size_t)k] = (int64_t)B.slots.size();
             int64_t t0 = 0;
             for (int e = 0; e < G.count; e++) {
                 const int s_ = at[(size_t)e][(size_t)k];
                 if (s_ < 0) continue;
-                B.slots.push_back({(uint32_t)e | ((uint32_t)(s_ & 1) << 31), (uint32_t)t0});
+                uint32_t ent = (uint32_t)e | ((uint32_t)(s_ & 1) << 31);
+                if (G.two) {
+                    const int np = npair[(size_t)e];
+                    if (s_ >= np) ent |= BATCH_SINGLE;
+                    if (std::min(s_, np) & 1) ent |= BATCH_LAG_PARITY;
+                }
+                B.slots.push_back({ent, (uint32_t)t0});
                 t0 += ntile[(size_t)e];
             }
             G.nslot[(size_t)k] = (int)((int64_t)B.slots.size() - G.off[(size_t)k]);
@@ -228,14 +265,16 @@ int launch_steps(Batch &B)
     for (size_t g = 0; g < ng; g++) {
         const Batch::Group &G = B.groups[g];
         hipStream_t st = g == 0 ? B.stream : B.gstream[g - 1];
-        for (int k = 0; k < B.chunk; k++) {
+        for (int k = 0; k < G.nlaunch; k++) {
             const int64_t nt = G.nblk[(size_t)k];
             if (nt == 0) continue;
             const int64_t nb = (nt + BATCH_WAVES_PER_BLOCK - 1) / BATCH_WAVES_PER_BLOCK;
             const BatchEntry *tab = B.dev + G.first;
             const BatchSlot *sl = B.dev_slots + G.off[(size_t)k];
             const int ns = G.nslot[(size_t)k];
-            if (G.dev32)
+            if (G.two)
+                launch_batch2_fp32(tab, sl, ns, nt, G.kind, G.arith64, G.ntl, st);
+            else if (G.dev32)
                 launch_batch_fp32(tab, sl, ns, nb, G.kind, G.dense, G.gen, G.arith64, G.ntl, st);
             else
                 launch_batch_fp64(tab, sl, ns, nb, G.kind, G.dense, st);
@@ -293,7 +332,16 @@ void commit(Batch &B)
     for (size_t i = 0; i < B.lats.size(); i++) {
         ch4g_lattice *L = B.lats[i];
         const int st = B.steps[i];
-        if (st & 1) L->cur ^= 1;
+        if (B.two[i]) {
+            const int pairs = st / 2;
+            if ((pairs + st % 2) & 1) L->cur ^= 1;
+            if ((pairs & 1) && L->base.thermal) {
+                std::swap(L->tlag, L->tlag_alt);
+                L->tpar ^= 1;
+            }
+        } else if (st & 1) {
+            L->cur ^= 1;
+        }
         L->itime += st;
     }
 }
@@ -472,8 +520,8 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
             HIPCHK(hipStreamSynchronize(B.stream));
             for (size_t a = 0; a < in_chunk; a++) {
                 active[a].remaining -= B.steps[a];
-                // an odd share flips the lattice's buffer: the table follows
-                if (B.steps[a] & 1) table_dirty = true;
+                // a share that flips the lattice's buffer (or lag array): the table follows
+                if (B.flips[a]) table_dirty = true;
                 if (active[a].remaining < B.steps[a]) table_dirty = true;
             }
         } else {

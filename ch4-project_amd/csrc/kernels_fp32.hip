@@ -638,19 +638,25 @@ __device__ __forceinline__ void gather_g_tile(const KParams &P, const Row &R, co
     });
 }
 
+// LDS planes of a two-step tile: fluid populations, then temperature
+// populations and the temperature lag
+template <bool FLUID, bool THERMAL>
+constexpr int tile2_planes()
+{
+    return (FLUID ? Q : 0) + (THERMAL ? Q + 1 : 0);
+}
+
+// The two steps of one 64 x TY tile whose first node is (x0, y0), through
+// the block's LDS array t (shared by the single-lattice and batched kernels).
 template <bool FLUID, bool THERMAL, typename A, bool NTL, int TY>
-__global__ void __launch_bounds__(Tile2<TY>::THREADS) k_step2_fp32(KParams P, const float *__restrict__ fin,
-                                                                   const float *__restrict__ gin,
-                                                                   float *__restrict__ fout, float *__restrict__ gout,
-                                                                   const float *__restrict__ tl_in,
-                                                                   float *__restrict__ tl_out)
+__device__ __forceinline__ void step2_tile(const KParams &P, const float *__restrict__ fin,
+                                           const float *__restrict__ gin, float *__restrict__ fout,
+                                           float *__restrict__ gout, const float *__restrict__ tl_in,
+                                           float *__restrict__ tl_out, int x0, int y0, float *t)
 {
     constexpr int M = Tile2<TY>::M;
     constexpr int GOFS = FLUID ? Q : 0;
-    constexpr int PLANES = GOFS + (THERMAL ? Q + 1 : 0);
-    __shared__ float t[PLANES * M];
     const int NX = P.NX, NY = P.NY;
-    const int x0 = blockIdx.x * T2_TX, y0 = blockIdx.y * TY;
     // columns of this tile: 64, or fewer in the last tile of a row when NX
     // is not a multiple of 64 (its right halo column is then x = 0)
     const int w = NX - x0 < T2_TX ? NX - x0 : T2_TX;
@@ -715,6 +721,62 @@ __global__ void __launch_bounds__(Tile2<TY>::THREADS) k_step2_fp32(KParams P, co
     if constexpr (THERMAL) collide_thermal<A, false>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
 }
 
+template <bool FLUID, bool THERMAL, typename A, bool NTL, int TY>
+__global__ void __launch_bounds__(Tile2<TY>::THREADS) k_step2_fp32(KParams P, const float *__restrict__ fin,
+                                                                   const float *__restrict__ gin,
+                                                                   float *__restrict__ fout, float *__restrict__ gout,
+                                                                   const float *__restrict__ tl_in,
+                                                                   float *__restrict__ tl_out)
+{
+    __shared__ float t[tile2_planes<FLUID, THERMAL>() * Tile2<TY>::M];
+    step2_tile<FLUID, THERMAL, A, NTL, TY>(P, fin, gin, fout, gout, tl_in, tl_out, blockIdx.x * T2_TX,
+                                           blockIdx.y * TY, t);
+}
+
+// Many 2-D lattices in one launch, two steps each: block b advances one
+// 64 x 4 tile of the lattice whose block range in this launch holds it
+// (block-uniform binary search over the launch's few slots).  A slot
+// flagged single advances its tiles by one step instead (the odd remainder
+// of the lattice's share of a chunk; the lag stays in place).  Population
+// buffers alternate with every launch of a lattice, its lag arrays with
+// every two-step launch (parities in the slot).
+template <bool FLUID, bool THERMAL, typename A, bool NTL>
+__global__ void __launch_bounds__(Tile2<4>::THREADS) k_step2_batch32(const BatchEntry *__restrict__ tab,
+                                                                     const BatchSlot *__restrict__ slots, int nslots)
+{
+    constexpr int TY = 4;
+    __shared__ float t[tile2_planes<FLUID, THERMAL>() * Tile2<TY>::M];
+    const uint32_t bt = blockIdx.x;
+    if (bt >= slots[nslots].tile0) return;
+    int lo = 0, hi = nslots - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (slots[mid].tile0 <= bt)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchSlot sl = slots[lo];
+    const BatchEntry &E = tab[sl.entry & BATCH_ENTRY_MASK];
+    const KParams &P = E.P;
+    const int in = E.cur ^ (int)(sl.entry >> 31);
+    const bool single = (sl.entry & BATCH_SINGLE) != 0;
+    const bool tp = (sl.entry & BATCH_LAG_PARITY) != 0;
+    const float *fin = (const float *)E.f[in], *gin = (const float *)E.g[in];
+    float *fout = (float *)E.f[in ^ 1], *gout = (float *)E.g[in ^ 1];
+    float *tl_a = (float *)(tp ? E.tlag2 : E.tlag), *tl_b = (float *)(tp ? E.tlag : E.tlag2);
+    const uint32_t tile = bt - sl.tile0, txs = ((uint32_t)P.NX + T2_TX - 1) / T2_TX;
+    const int x0 = (int)(tile % txs) * T2_TX, y0 = (int)(tile / txs) * TY;
+    if (single) {
+        const int x = x0 + (int)threadIdx.x % T2_TX, y = y0 + (int)threadIdx.x / T2_TX;
+        if ((int)threadIdx.x >= T2_TX * TY || x >= P.NX || y >= P.NY) return;
+        step_node32<false, FLUID, THERMAL, A, NTL, false>(P, fin, gin, fout, gout, tl_a, (float *)E.taules, x, y, 0,
+                                                           (uint32_t)y * (uint32_t)P.NX + (uint32_t)x);
+        return;
+    }
+    step2_tile<FLUID, THERMAL, A, NTL, TY>(P, fin, gin, fout, gout, tl_a, tl_b, x0, y0, t);
+}
+
 // Velocity of the step at every node (the u[] that add_collision reads,
 // fluid.c:1056-1080), for the finite-difference LES closure.
 template <bool FIRST, typename A>
@@ -759,7 +821,7 @@ __global__ void __launch_bounds__(256) k_step_batch32(const BatchEntry *__restri
             hi = mid - 1;
     }
     const BatchSlot sl = slots[lo];
-    const BatchEntry &E = tab[sl.entry & 0x7fffffffu];
+    const BatchEntry &E = tab[sl.entry & BATCH_ENTRY_MASK];
     const KParams &P = E.P;
     const int in = E.cur ^ (int)(sl.entry >> 31);
     const float *fin = (const float *)E.f[in], *gin = (const float *)E.g[in];
@@ -1043,6 +1105,34 @@ void launch_batch_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots
 #undef CH4G_BATCH32_
 #undef CH4G_BATCH32D_
 #undef CH4G_BATCH32G_
+}
+
+void launch_batch2_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslots, int64_t nblocks, int kind,
+                        int arith64, bool ntl, hipStream_t s)
+{
+    dim3 grid((unsigned)nblocks), block(Tile2<4>::THREADS);
+#define CH4G_B2_(FL_, TH_, A_)                                                                                        \
+    do {                                                                                                              \
+        if (ntl)                                                                                                      \
+            hipLaunchKernelGGL((k_step2_batch32<FL_, TH_, A_, true>), grid, block, 0, s, tab, slots, nslots);         \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_step2_batch32<FL_, TH_, A_, false>), grid, block, 0, s, tab, slots, nslots);        \
+    } while (0)
+#define CH4G_B2(FL_, TH_)                                                                                             \
+    do {                                                                                                              \
+        if (arith64)                                                                                                  \
+            CH4G_B2_(FL_, TH_, double);                                                                               \
+        else                                                                                                          \
+            CH4G_B2_(FL_, TH_, float);                                                                                \
+    } while (0)
+    if (kind == BATCH_FLUID_THERMAL)
+        CH4G_B2(true, true);
+    else if (kind == BATCH_FLUID)
+        CH4G_B2(true, false);
+    else
+        CH4G_B2(false, true);
+#undef CH4G_B2
+#undef CH4G_B2_
 }
 
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,

@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(256) k_step_batch64(const BatchEntry *__restri
             hi = mid - 1;
     }
     const BatchSlot sl = slots[lo];
-    const BatchEntry &E = tab[sl.entry & 0x7fffffffu];
+    const BatchEntry &E = tab[sl.entry & BATCH_ENTRY_MASK];
     const KParams &P = E.P;
     const int in = E.cur ^ (int)(sl.entry >> 31);
     const double *fin = (const double *)E.f[in], *gin = (const double *)E.g[in];

@@ -73,9 +73,10 @@ def test_two_step_launches_match_single_steps(NX, NY, precision, fluid_only):
     (cache-resident, non-temporal loads, 64x4 tiles, a partial last tile
     row), 128x62 with double arithmetic, 300x97 (a narrower last tile in x
     and in y), 40x30 (one tile narrower than 64) and a 96x40 fluid-only
-    channel.  The same lattice stepped through ch4g_step_batch uses one
-    launch per step.  Populations must agree bit for bit after odd and even
-    counts and after graph replays."""
+    channel.  The same lattice stepped one step per ch4g_step_batch call
+    runs single steps only (the batch's single-step slots).  Populations
+    must agree bit for bit after odd and even counts and after graph
+    replays."""
     from ch4g import step_batch, workloads
     if fluid_only:
         cfg = workloads.channel3d(NX, NY, 1, precision=precision)
@@ -88,7 +89,8 @@ def test_two_step_launches_match_single_steps(NX, NY, precision, fluid_only):
         lat.step(1)
     for n in (7, 10, 70):
         two.step(n)
-        step_batch([one], n)
+        for _ in range(n):
+            step_batch([one], 1)
         pa, ga = two.get_populations()
         pb, gb = one.get_populations()
         assert np.array_equal(pa, pb) and np.array_equal(ga, gb), n

@@ -23,6 +23,7 @@ def mixed_configs():
         workloads.channel3d(4, 64, 4, tau=1.0, precision=abi.FP64),
         workloads.rb2d(40, 24, 1, Ra=1e5, precision=abi.FP32_STORE, seed=13),
         workloads.rb2d(300, 40, 1, Ra=1e6, precision=abi.FP32_DEV, seed=14),
+        workloads.channel3d(96, 40, 1, precision=abi.FP32_DEV),
     ]
 
 
