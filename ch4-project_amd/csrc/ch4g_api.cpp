@@ -431,9 +431,10 @@ int64_t ch4g_footprint_bytes(const ch4g_config *c)
     if (!c) return -1;
     const int64_t N = (int64_t)c->NX * c->NY * c->NZ;
     const int64_t r = c->precision == CH4G_FP64 ? 8 : 4;
+    const int64_t S = plane_stride(N, r);
     int64_t b = 0;
-    if (has(*c, CH4G_LB_FLUID)) b += 2 * Q * N * r;
-    if (has(*c, CH4G_LB_TEMPERATURE)) b += 2 * Q * N * r + N * r;
+    if (has(*c, CH4G_LB_FLUID)) b += 2 * Q * S * r;
+    if (has(*c, CH4G_LB_TEMPERATURE)) b += 2 * Q * S * r + N * r;
     if (has(*c, CH4G_FLUID_LES)) b += N * r;
     if (les_fd(*c)) b += 3 * N * r;
     return b;
@@ -456,7 +457,7 @@ int ch4g_create(const ch4g_config *cfg, ch4g_lattice **out)
     L->N = (int64_t)cfg->NX * cfg->NY * cfg->NZ;
     L->dev32 = cfg->precision == CH4G_FP32_DEV || cfg->precision == CH4G_FP32_STORE;
     L->rsz = L->dev32 ? 4 : 8;
-    const size_t N = (size_t)L->N, pop = (size_t)Q * N * L->rsz;
+    const size_t N = (size_t)L->N, pop = (size_t)Q * (size_t)plane_stride(L->N, L->rsz) * L->rsz;
     const int fl = has(*cfg, CH4G_LB_FLUID), th = has(*cfg, CH4G_LB_TEMPERATURE);
     if (fl)
         for (int b = 0; b < 2; b++)
@@ -479,6 +480,7 @@ int ch4g_create(const ch4g_config *cfg, ch4g_lattice **out)
         P.NY = cfg->NY;
         P.NZ = cfg->NZ;
         P.N = L->N;
+        P.S = plane_stride(L->N, L->rsz);
         P.fluid = fl;
         P.thermal = th;
         P.wall_f = has(*cfg, CH4G_FLUID_BC_Y);

@@ -48,6 +48,7 @@ struct HitTable {
 struct KParams {
     int NX, NY, NZ;
     int64_t N;          // interior nodes
+    int64_t S;          // population plane stride in elements (plane_stride: N, padded off 64 KiB multiples)
     int fluid, thermal; // population sets present
     int wall_f, wall_t; // y walls: fluid no-slip, temperature fixed
     int forced;         // any fluid forcing (hydro_fields half-force shift + Guo)
@@ -93,6 +94,17 @@ struct KParams {
     float *uf_out, *tf_out;
     int part;  // particles ride on the step kernel's fields (the instance that can write them)
 };
+
+// Stride of the population planes of a lattice of N nodes with rsz-byte
+// reals.  Planes whose byte size is a multiple of 64 KiB would start at the
+// same offset modulo large powers of two, and the 38 streams of a step then
+// collide on the same memory channels (a copy of 38 planes at 512^3: 4.72
+// TB/s with stride N, 5.34 TB/s padded).  The pad (4160 elements) keeps the
+// planes 256-byte aligned.
+inline int64_t plane_stride(int64_t N, int64_t rsz)
+{
+    return (N * rsz) % 65536 == 0 ? N + 4160 : N;
+}
 
 inline void fill_float_params(KParams &P)
 {

@@ -29,14 +29,14 @@ __global__ void k_init(KParams P, const double *__restrict__ prow, const double 
     if (P.fluid) {
         for (int q = 0; q < Q; q++) {
             double v = prow[y * Q + q];
-            f[q * P.N + n] = dev ? (R)(v - wq(q) * rb) : (R)v;
+            f[q * P.S + n] = dev ? (R)(v - wq(q) * rb) : (R)v;
         }
     }
     if (P.thermal) {
         double gq[Q];
         double t = t0[n];
         for (int q = 0; q < Q; q++) gq[q] = wq(q) * t;
-        for (int q = 0; q < Q; q++) g[q * P.N + n] = dev ? (R)(gq[q] - wq(q) * tb) : (R)gq[q];
+        for (int q = 0; q < Q; q++) g[q * P.S + n] = dev ? (R)(gq[q] - wq(q) * tb) : (R)gq[q];
         double T = msum19(gq);
         tlag[n] = dev ? (R)(T - tb) : (R)T;
     }
@@ -54,12 +54,12 @@ __global__ void k_load(KParams P, const double *__restrict__ fs, const double *_
     if (P.fluid && fs)
         for (int q = 0; q < Q; q++) {
             double v = fs[q * P.N + n];
-            f[q * P.N + n] = dev ? (R)(v - wq(q) * rb) : (R)v;
+            f[q * P.S + n] = dev ? (R)(v - wq(q) * rb) : (R)v;
         }
     if (P.thermal && gs) {
         double gq[Q];
         for (int q = 0; q < Q; q++) gq[q] = gs[q * P.N + n];
-        for (int q = 0; q < Q; q++) g[q * P.N + n] = dev ? (R)(gq[q] - wq(q) * tb) : (R)gq[q];
+        for (int q = 0; q < Q; q++) g[q * P.S + n] = dev ? (R)(gq[q] - wq(q) * tb) : (R)gq[q];
         double T = msum19(gq);
         tlag[n] = dev ? (R)(T - tb) : (R)T;
     }

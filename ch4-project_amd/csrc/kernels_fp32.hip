@@ -98,7 +98,7 @@ template <bool FIRST, bool NTL>
 __device__ __forceinline__ void gather_f32(const KParams &P, const Row &R, const Nbr &b, const float *__restrict__ in,
                                            uint32_t n, float f[Q])
 {
-    const size_t N = (size_t)P.N;
+    const size_t N = (size_t)P.S;  // plane stride
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (FIRST) {
@@ -120,7 +120,7 @@ template <bool FIRST, bool NTL>
 __device__ __forceinline__ void gather_g32(const KParams &P, const Row &R, const Nbr &b, const float *__restrict__ in,
                                            uint32_t n, float dTlag, float g[Q])
 {
-    const size_t N = (size_t)P.N;
+    const size_t N = (size_t)P.S;  // plane stride
     const float fac_top = 2.f * (P.cw_top_f - dTlag);
     const float fac_bot = 2.f * (P.cw_bot_f - dTlag);
     static_for<0, Q>([&](auto qc) {
@@ -151,11 +151,11 @@ template <bool NTL>
 __device__ __forceinline__ void gather_f32_o32(const KParams &P, const Row &R, const Nbr &b,
                                                const float *__restrict__ in, uint32_t n, float f[Q])
 {
-    const uint32_t N4 = (uint32_t)P.N * 4u;
+    const uint32_t S4 = (uint32_t)P.S * 4u;
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const bool cross = P.wall_f && crosses_wall<q>(b);
-        const uint32_t off = cross ? (uint32_t)invq(q) * N4 + n * 4u : (uint32_t)q * N4 + src_index<q>(b, P.NX, P.NY) * 4u;
+        const uint32_t off = cross ? (uint32_t)invq(q) * S4 + n * 4u : (uint32_t)q * S4 + src_index<q>(b, P.NX, P.NY) * 4u;
         float v = ldo<NTL>(in, off);
         if constexpr (cyq(q) > 0) v += cross ? 0.f : wf(q) * R.dr_up;
         if constexpr (cyq(q) < 0) v += cross ? 0.f : wf(q) * R.dr_dn;
@@ -167,13 +167,13 @@ template <bool NTL>
 __device__ __forceinline__ void gather_g32_o32(const KParams &P, const Row &R, const Nbr &b,
                                                const float *__restrict__ in, uint32_t n, float dTlag, float g[Q])
 {
-    const uint32_t N4 = (uint32_t)P.N * 4u;
+    const uint32_t S4 = (uint32_t)P.S * 4u;
     const float fac_top = 2.f * (P.cw_top_f - dTlag);
     const float fac_bot = 2.f * (P.cw_bot_f - dTlag);
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const bool cross = P.wall_t && crosses_wall<q>(b);
-        const uint32_t off = (uint32_t)q * N4 + (cross ? n : src_index<q>(b, P.NX, P.NY)) * 4u;
+        const uint32_t off = (uint32_t)q * S4 + (cross ? n : src_index<q>(b, P.NX, P.NY)) * 4u;
         float v = ldo<NTL>(in, off);
         if constexpr (cyq(q) > 0) v += cross ? wf(q) * fac_bot : wf(q) * R.dt_up;
         if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
@@ -249,9 +249,9 @@ struct NodeOut {
     float *__restrict__ taules;
     const float *tau_src;
     uint32_t n, tau_at;
-    size_t N;
-    __device__ void put_f(int q, float v) const { stp(f + (size_t)q * N + n, v); }
-    __device__ void put_g(int q, float v) const { stp(g + (size_t)q * N + n, v); }
+    size_t S;  // plane stride
+    __device__ void put_f(int q, float v) const { stp(f + (size_t)q * S + n, v); }
+    __device__ void put_g(int q, float v) const { stp(g + (size_t)q * S + n, v); }
     __device__ void put_tlag(float v) const { tlag[n] = v; }
     __device__ float get_tau() const { return tau_src[tau_at]; }
     __device__ void put_tau(float v) const { taules[n] = v; }
@@ -262,9 +262,9 @@ struct NodeOut32 {
     float *__restrict__ f;
     float *__restrict__ g;
     float *__restrict__ tlag;
-    uint32_t n4, N4;
-    __device__ void put_f(int q, float v) const { stp((float *)((char *)f + ((uint32_t)q * N4 + n4)), v); }
-    __device__ void put_g(int q, float v) const { stp((float *)((char *)g + ((uint32_t)q * N4 + n4)), v); }
+    uint32_t n4, S4;  // byte offset of the node, plane stride in bytes
+    __device__ void put_f(int q, float v) const { stp((float *)((char *)f + ((uint32_t)q * S4 + n4)), v); }
+    __device__ void put_g(int q, float v) const { stp((float *)((char *)g + ((uint32_t)q * S4 + n4)), v); }
     __device__ void put_tlag(float v) const { *(float *)((char *)tlag + n4) = v; }
     __device__ float get_tau() const { return 0.f; }  // no LES on the two-step path
     __device__ void put_tau(float) const {}
@@ -544,7 +544,7 @@ __device__ __forceinline__ void step_node32(const KParams &P, const float *__res
         }
     }
     A ux = 0, uy = 0, uz = 0, tau_les = A(0);
-    const NodeOut out{fout, gout, tlag, taules, taules, n, n, (size_t)P.N};
+    const NodeOut out{fout, gout, tlag, taules, taules, n, n, (size_t)P.S};
     if constexpr (FLUID) collide_fluid<A, GEN>(P, R, x, y, z, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
     if constexpr (THERMAL) collide_thermal<A, GEN>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
     if constexpr (GEN && !FIRST) {
@@ -716,7 +716,7 @@ __device__ __forceinline__ void step2_tile(const KParams &P, const float *__rest
         }
     }
     A ux = 0, uy = 0, uz = 0, tau_les = A(0);
-    const NodeOut32 out{fout, gout, tl_out, n * 4u, (uint32_t)P.N * 4u};
+    const NodeOut32 out{fout, gout, tl_out, n * 4u, (uint32_t)P.S * 4u};
     if constexpr (FLUID) collide_fluid<A, false>(P, R, x, y, 0, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
     if constexpr (THERMAL) collide_thermal<A, false>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
 }
@@ -995,7 +995,7 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 bool two_step_ok(const KParams &P)
 {
     // 32-bit byte offsets into the Q population planes
-    const bool off32 = (int64_t)Q * P.N * 4 < ((int64_t)1 << 32);
+    const bool off32 = (int64_t)Q * P.S * 4 < ((int64_t)1 << 32);
     return P.NZ == 1 && off32 && !P.first && !P.hit && !P.les && !P.twoway;
 }
 

@@ -78,7 +78,7 @@ template <bool FIRST>
 __device__ __forceinline__ void gather_fluid(const KParams &P, const Nbr &b, const double *__restrict__ in,
                                              uint32_t n, double f[Q])
 {
-    const size_t N = (size_t)P.N;
+    const size_t N = (size_t)P.S;  // plane stride
     static_for<0, Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (FIRST) {
@@ -100,7 +100,7 @@ template <bool FIRST>
 __device__ __forceinline__ void gather_thermal(const KParams &P, const Nbr &b, const double *__restrict__ in,
                                                uint32_t n, double Tlag, double g[Q])
 {
-    const size_t N = (size_t)P.N;
+    const size_t N = (size_t)P.S;  // plane stride
     const double fac_top = 2.0 * (P.Tw_top - Tlag);
     const double fac_bot = 2.0 * (P.Tw_bot - Tlag);
     static_for<0, Q>([&](auto qc) {
@@ -226,7 +226,7 @@ __device__ __forceinline__ void step_node64(const KParams &P, const double *__re
                                             double *__restrict__ gout, double *__restrict__ tlag,
                                             double *__restrict__ taules, int x, int y, int z, uint32_t n)
 {
-    const size_t N = (size_t)P.N;
+    const size_t N = (size_t)P.S;  // plane stride
     const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
 
     double Tlag = 0.0, T = 0.0;

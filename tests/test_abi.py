@@ -53,9 +53,18 @@ def test_struct_layout_matches_header(tmp_path):
 def test_footprint_and_errors_without_gpu():
     lib = C.CDLL(LIB_PATH)
     lib.ch4g_footprint_bytes.restype = C.c_int64
+    # two buffers of 19 fluid and 19 temperature planes plus the lag scalar;
+    # planes whose byte size is a multiple of 64 KiB are padded by 4160
+    # elements (512x256 floats: 512 KiB planes), others are not (500x256)
     cfg = abi.make_config(512, 256, 1, abi.LB_FLUID | abi.LB_TEMPERATURE, precision=abi.FP32_DEV)
     n = 512 * 256
+    assert lib.ch4g_footprint_bytes(C.byref(cfg)) == 4 * 19 * (n + 4160) * 4 + n * 4
+    cfg = abi.make_config(500, 256, 1, abi.LB_FLUID | abi.LB_TEMPERATURE, precision=abi.FP32_DEV)
+    n = 500 * 256
     assert lib.ch4g_footprint_bytes(C.byref(cfg)) == 4 * 19 * n * 4 + n * 4
+    cfg = abi.make_config(512, 256, 1, abi.LB_FLUID, precision=abi.FP64)
+    n = 512 * 256
+    assert lib.ch4g_footprint_bytes(C.byref(cfg)) == 2 * 19 * (n + 4160) * 8
     if lib.ch4g_device_count() == 0:
         h = C.c_void_p()
         rc = lib.ch4g_create(C.byref(cfg), C.byref(h))
