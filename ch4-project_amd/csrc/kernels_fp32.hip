@@ -32,7 +32,19 @@ __device__ __forceinline__ float ldp(const float *p)
         return *p;
 }
 
-__device__ __forceinline__ void stp(float *p, float v) { __builtin_nontemporal_store(v, p); }
+// Stores are non-temporal except where the single-step kernels update a
+// lattice resident in the last-level cache (NT = false there: the next
+// step's reads then find the state in the cache; channel 78^3 21.6 -> 18.8
+// us per step, while the two-step kernel and streamed lattices are faster
+// with non-temporal stores: 512x256 7.24 vs 7.45 us, 256^3 957 vs 999 us).
+template <bool NT = true>
+__device__ __forceinline__ void stp(float *p, float v)
+{
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
 
 __device__ __forceinline__ float wf(int q) { return q == 0 ? 1.f / 3.f : (q <= 6 ? 1.f / 18.f : 1.f / 36.f); }
 
@@ -238,10 +250,12 @@ __device__ __forceinline__ A wa(int q)
 }
 
 // Where a node's post-collision state goes.  NodeOut: the global planes of
-// the single-step kernels (and of the second step of k_step2_fp32), with the
-// previous LES relaxation time read from tau_src[tau_at].  TileOut: the LDS
-// tile of k_step2_fp32's first step: fluid planes from 0, temperature planes
-// from gofs, then the temperature lag, `stride` nodes apart.
+// the single-step kernels (non-temporal stores when NT), with the previous
+// LES relaxation time read from tau_src[tau_at]; NodeOut32 (below): the
+// second step of k_step2_fp32.  TileOut: the LDS tile of k_step2_fp32's
+// first step: fluid planes from 0, temperature planes from gofs, then the
+// temperature lag, `stride` nodes apart.
+template <bool NT>
 struct NodeOut {
     float *__restrict__ f;
     float *__restrict__ g;
@@ -250,8 +264,8 @@ struct NodeOut {
     const float *tau_src;
     uint32_t n, tau_at;
     size_t S;  // plane stride
-    __device__ void put_f(int q, float v) const { stp(f + (size_t)q * S + n, v); }
-    __device__ void put_g(int q, float v) const { stp(g + (size_t)q * S + n, v); }
+    __device__ void put_f(int q, float v) const { stp<NT>(f + (size_t)q * S + n, v); }
+    __device__ void put_g(int q, float v) const { stp<NT>(g + (size_t)q * S + n, v); }
     __device__ void put_tlag(float v) const { tlag[n] = v; }
     __device__ float get_tau() const { return tau_src[tau_at]; }
     __device__ void put_tau(float v) const { taules[n] = v; }
@@ -544,7 +558,7 @@ __device__ __forceinline__ void step_node32(const KParams &P, const float *__res
         }
     }
     A ux = 0, uy = 0, uz = 0, tau_les = A(0);
-    const NodeOut out{fout, gout, tlag, taules, taules, n, n, (size_t)P.S};
+    const NodeOut<!NTL> out{fout, gout, tlag, taules, taules, n, n, (size_t)P.S};
     if constexpr (FLUID) collide_fluid<A, GEN>(P, R, x, y, z, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
     if constexpr (THERMAL) collide_thermal<A, GEN>(P, R, y, g, dT, ux, uy, uz, tau_les, out);
     if constexpr (GEN && !FIRST) {
