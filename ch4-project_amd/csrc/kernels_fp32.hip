@@ -1,7 +1,8 @@
 // FP32-deviation fused stream+collide kernel and its gather kernel.
 //
 // Storage: df_i = f_i - w_i rho_b(y) and dg_i = g_i - w_i T_b(y) in float,
-// SoA planes [q][z][y][x], where rho_b(y), T_b(y) is a per-row base state
+// SoA planes [q][z][y][x] (planes P.S >= N elements apart), where
+// rho_b(y), T_b(y) is a per-row base state
 // chosen on the host (uniform rho0/T0, or the conduction profile with its
 // hydrostatic density for buoyancy-driven runs).  Subtracting the base keeps
 // the stored numbers of the size of the dynamics, so float rounding acts on
@@ -933,12 +934,10 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
     }
 }
 
-#ifndef CH4G_BX32
-#define CH4G_BX32 256
-#endif
+// x extent of a row-grid block: 256 threads, halved for short rows
 static int block_x32(const KParams &P)
 {
-    int bx = CH4G_BX32;
+    int bx = 256;
     while (bx > 64 && bx / 2 >= P.NX) bx /= 2;
     return bx;
 }
