@@ -1018,7 +1018,8 @@ void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin,
     // non-temporal loads while both lattices fit the last-level cache (as the
     // single-step kernel); 64x5 tiles (fewer halo nodes) once there are at
     // least two rounds of blocks (two blocks per CU by LDS), else 64x4 tiles
-    // (more, smaller blocks fill the CUs evenly)
+    // (more, smaller blocks fill the CUs evenly; 64x3 tiles, three blocks per
+    // CU, were slower at 512x256: 9.6 against 7.3 us per step)
     const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
     const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
     const unsigned bx = (unsigned)((P.NX + T2_TX - 1) / T2_TX);
