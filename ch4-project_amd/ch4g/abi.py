@@ -142,7 +142,7 @@ def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_co
 PARTICLE_FIELDS = ["name", "x", "y", "z", "vx", "vy", "vz", "vx_old", "vy_old", "vz_old",
                    "ax", "ay", "az", "ax_old", "ay_old", "az_old", "ux", "uy", "uz", "ux_old", "uy_old", "uz_old",
                    "dx_ux", "dy_ux", "dz_ux", "dx_uy", "dy_uy", "dz_uy", "dx_uz", "dy_uz", "dz_uz",
-                   "Dt_ux", "Dt_uy", "Dt_uz", "t", "t_old", "dt_t", "tau_drag", "beta_coeff"]
+                   "Dt_ux", "Dt_uy", "Dt_uz", "t", "t_old", "dt_t", "tau_drag", "beta_coeff", "dx_t", "dy_t", "dz_t"]
 
 
 class Particle(C.Structure):

@@ -182,7 +182,8 @@ enum PartField {
     PG,                 // 9 rows: dx_ux, dy_ux, dz_ux, dx_uy, dy_uy, dz_uy, dx_uz, dy_uz, dz_uz
     PDTU = PG + 9,      // 3 rows: Dt_ux, Dt_uy, Dt_uz
     PT = PDTU + 3, PTOLD, PDTT,
-    PTAU, PBETA,
+    PGT,                // 3 rows: dx_t, dy_t, dz_t
+    PTAU = PGT + 3, PBETA,
     PVOL,               // (4/3) pi (3 nu beta tau)^(3/2), host pow (lagrange_feedback.c:87-90)
     PDRAT,              // (3 - beta) / (2 beta) (lagrange_feedback.c:84)
     PNAME,

@@ -219,6 +219,9 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         S(PT) = r.t;
         S(PTOLD) = r.t_old;
         S(PDTT) = r.dt_t;
+        S(PGT) = r.dx_t;
+        S(PGT + 1) = r.dy_t;
+        S(PGT + 2) = r.dz_t;
         S(PTAU) = r.tau_drag;
         S(PBETA) = r.beta_coeff;
         double vol = 0.0, drat = 0.0;
@@ -512,6 +515,9 @@ int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t
             r.t = S(PT);
             r.t_old = S(PTOLD);
             r.dt_t = S(PDTT);
+            r.dx_t = S(PGT);
+            r.dy_t = S(PGT + 1);
+            r.dz_t = S(PGT + 2);
             r.tau_drag = S(PTAU);
             r.beta_coeff = S(PBETA);
         }

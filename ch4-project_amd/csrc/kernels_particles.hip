@@ -170,7 +170,10 @@ __device__ __forceinline__ void one_sided(const double a[3], const double b[3], 
 
 // gradient_vector with LAGRANGE on a two-node halo (gradients.c:78-499).
 // G = xx, xy, xz, yx, yy, yz, zx, zy, zz (first letter: component of u,
-// second: direction of the derivative).
+// second: direction of the derivative).  With fluid walls (LB_FLUID_BC) the
+// LAGRANGE_INTERPOLATION_FIX moves a ghost node's evaluation point to the
+// first interior node (:288-368); without them it takes the one-sided
+// difference at the ghost node itself (:413-488).
 template <typename F>
 __device__ void grad_at(const PartParams &P, int i, int j, int k, double G[9])
 {
@@ -250,10 +253,108 @@ __device__ void grad_at(const PartParams &P, int i, int j, int k, double G[9])
             fetch_u<F>(P, i, j, k - 2, c);
             one_sided(a, b, c, false, G[2], G[5], G[8]);
         }
+    } else {
+        if (i == -1) {
+            fetch_u<F>(P, -1, j, k, a);
+            fetch_u<F>(P, 0, j, k, b);
+            fetch_u<F>(P, 1, j, k, c);
+            one_sided(a, b, c, true, G[0], G[3], G[6]);
+        }
+        if (i == P.NX) {
+            fetch_u<F>(P, P.NX, j, k, a);
+            fetch_u<F>(P, P.NX - 1, j, k, b);
+            fetch_u<F>(P, P.NX - 2, j, k, c);
+            one_sided(a, b, c, false, G[0], G[3], G[6]);
+        }
+        if (j == -1) {
+            fetch_u<F>(P, i, -1, k, a);
+            fetch_u<F>(P, i, 0, k, b);
+            fetch_u<F>(P, i, 1, k, c);
+            one_sided(a, b, c, true, G[1], G[4], G[7]);
+        }
+        if (j == P.NY) {
+            fetch_u<F>(P, i, P.NY, k, a);
+            fetch_u<F>(P, i, P.NY - 1, k, b);
+            fetch_u<F>(P, i, P.NY - 2, k, c);
+            one_sided(a, b, c, false, G[1], G[4], G[7]);
+        }
+        if (k == -1) {
+            fetch_u<F>(P, i, j, -1, a);
+            fetch_u<F>(P, i, j, 0, b);
+            fetch_u<F>(P, i, j, 1, c);
+            one_sided(a, b, c, true, G[2], G[5], G[8]);
+        }
+        if (k == P.NZ) {
+            fetch_u<F>(P, i, j, P.NZ, a);
+            fetch_u<F>(P, i, j, P.NZ - 1, b);
+            fetch_u<F>(P, i, j, P.NZ - 2, c);
+            one_sided(a, b, c, false, G[2], G[5], G[8]);
+        }
     }
     if (P.NX == 1) G[0] = G[3] = G[6] = 0.0;
     if (P.NY == 1) G[1] = G[4] = G[7] = 0.0;
     if (P.NZ == 1) G[2] = G[5] = G[8] = 0.0;
+}
+
+// second-order one-sided difference of a scalar (SECOND_ORDER_GRAD with unit
+// spacing: a0 = -3/2, a1 = 2, a2 = -1/2 forward, the negatives backward)
+__device__ __forceinline__ double one_sided_s(double a, double b, double c, bool fwd)
+{
+    const double a0 = fwd ? -1.5 : 1.5, a1 = fwd ? 2.0 : -2.0, a2 = fwd ? -0.5 : 0.5;
+    return a0 * a + a1 * b + a2 * c;
+}
+
+// gradient_scalar of the temperature with LAGRANGE on a two-node halo
+// (gradients.c:505-827): centred differences; with temperature walls
+// (LB_TEMPERATURE_BC_Y) one-sided at the first and last rows of y (:560-626)
+// and the interpolation fix that moves a ghost node's evaluation point to the
+// first interior node (:657-725); without them the one-sided difference at
+// the ghost node itself (:762-825); components along a size-one axis zeroed.
+template <typename F>
+__device__ void grad_t_at(const PartParams &P, int i, int j, int k, double g[3])
+{
+    g[0] = (fetch_t<F>(P, i + 1, j, k) - fetch_t<F>(P, i - 1, j, k)) / 2.0;
+    g[1] = (fetch_t<F>(P, i, j + 1, k) - fetch_t<F>(P, i, j - 1, k)) / 2.0;
+    g[2] = (fetch_t<F>(P, i, j, k + 1) - fetch_t<F>(P, i, j, k - 1)) / 2.0;
+    auto T = [&](int a, int b, int c) { return fetch_t<F>(P, a, b, c); };
+    if (P.wall_t) {
+        if (j == 0) g[1] = one_sided_s(T(i, 0, k), T(i, 1, k), T(i, 2, k), true);
+        if (j == P.NY - 1) g[1] = one_sided_s(T(i, P.NY - 1, k), T(i, P.NY - 2, k), T(i, P.NY - 3, k), false);
+        if (i == -1) {
+            i = 0;
+            g[0] = one_sided_s(T(0, j, k), T(1, j, k), T(2, j, k), true);
+        }
+        if (i == P.NX) {
+            i = P.NX - 1;
+            g[0] = one_sided_s(T(i, j, k), T(i - 1, j, k), T(i - 2, j, k), false);
+        }
+        if (j == -1) {
+            j = 0;
+            g[1] = one_sided_s(T(i, 0, k), T(i, 1, k), T(i, 2, k), true);
+        }
+        if (j == P.NY) {
+            j = P.NY - 1;
+            g[1] = one_sided_s(T(i, j, k), T(i, j - 1, k), T(i, j - 2, k), false);
+        }
+        if (k == -1) {
+            k = 0;
+            g[2] = one_sided_s(T(i, j, 0), T(i, j, 1), T(i, j, 2), true);
+        }
+        if (k == P.NZ) {
+            k = P.NZ - 1;
+            g[2] = one_sided_s(T(i, j, k), T(i, j, k - 1), T(i, j, k - 2), false);
+        }
+    } else {
+        if (i == -1) g[0] = one_sided_s(T(-1, j, k), T(0, j, k), T(1, j, k), true);
+        if (i == P.NX) g[0] = one_sided_s(T(P.NX, j, k), T(P.NX - 1, j, k), T(P.NX - 2, j, k), false);
+        if (j == -1) g[1] = one_sided_s(T(i, -1, k), T(i, 0, k), T(i, 1, k), true);
+        if (j == P.NY) g[1] = one_sided_s(T(i, P.NY, k), T(i, P.NY - 1, k), T(i, P.NY - 2, k), false);
+        if (k == -1) g[2] = one_sided_s(T(i, j, -1), T(i, j, 0), T(i, j, 1), true);
+        if (k == P.NZ) g[2] = one_sided_s(T(i, j, P.NZ), T(i, j, P.NZ - 1), T(i, j, P.NZ - 2), false);
+    }
+    if (P.NX == 1) g[0] = 0.0;
+    if (P.NY == 1) g[1] = 0.0;
+    if (P.NZ == 1) g[2] = 0.0;
 }
 
 // sendrecv_particles on one rank (lagrange.c:4060-4116): wrap (with
@@ -302,11 +403,16 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
             dtu[1] = R(PDTU + 1);
             dtu[2] = R(PDTU + 2);
         }
-        double t = 0.0, told = 0.0, dtt = 0.0;
+        double t = 0.0, told = 0.0, dtt = 0.0, gt[3] = {0.0, 0.0, 0.0};
         if (P.thermal) {
             t = R(PT);
             told = R(PTOLD);
             dtt = R(PDTT);
+            if constexpr (GRAD) {
+                gt[0] = R(PGT);
+                gt[1] = R(PGT + 1);
+                gt[2] = R(PGT + 2);
+            }
         }
         const double tau = R(PTAU), beta = R(PBETA);
 
@@ -347,6 +453,17 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
                     s = (c == 0) ? tc * st.w[0] : s + tc * st.w[c];
                 }
                 t = s;
+                if constexpr (GRAD) {
+                    // interpolate_scalar_at_particles with LAGRANGE_GRADIENT
+                    // (lagrange.c:1500-1576): the corner gradients weighted
+                    // in the corner order of the value
+                    double acc[3], gc[3];
+                    for (int c = 0; c < 8; c++) {
+                        grad_t_at<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), gc);
+                        for (int m = 0; m < 3; m++) acc[m] = (c == 0) ? gc[m] * st.w[0] : acc[m] + gc[m] * st.w[c];
+                    }
+                    for (int m = 0; m < 3; m++) gt[m] = acc[m];
+                }
             }
         }
 
@@ -564,6 +681,11 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
             R(PT) = t;
             R(PTOLD) = told;
             R(PDTT) = dtt;
+            if constexpr (GRAD) {
+                R(PGT) = gt[0];
+                R(PGT + 1) = gt[1];
+                R(PGT + 2) = gt[2];
+            }
         }
 #undef R
     }

@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 #include <time.h>
 
 #include "../../include/ch4g_dropin.h"
@@ -28,6 +29,8 @@ static struct {
     ch4g_output *ruler[3];   /* x, y, z rulers of the last dump_averages */
     ch4g_output *running[3]; /* their running sums (ruler_*_running) */
 } S;
+
+static void write_particle_snapshot(void);
 
 static double now_s(void)
 {
@@ -332,10 +335,17 @@ void move_particles(void)
     }
 }
 
-/* the periodic particle_<itime>.h5 snapshots (lagrange.c:1598-2595) are not
- * written; ch4g_get_particles hands out the records, and the end-of-run
- * part.h5 of write_point_particle_h5 holds them in the reference layout */
-void output_particles(void) { need_particles("output_particles"); }
+/* output_particles (lagrange.c:1598-2595): with OUTPUT_H5, every
+ * time_dump_lagr, the particle snapshot RUN/particle_<itime>.h5 and its XDMF
+ * descriptor (write_particle_snapshot) */
+void output_particles(void)
+{
+    need_particles("output_particles");
+    const ch4g_run_params *rp = &S.rp;
+    const int every = (int)(rp->time_dump_lagr / rp->time_dt);
+    if (!rp->output_h5 || every <= 0 || S.itime % every != 0) return;
+    write_particle_snapshot();
+}
 
 /* dump_particle_averages (lagrange.c:2598-2930): per particle type, FP64
  * averages of v, v^2, v^4, a, a^2, a^4 (and t, t^2, t^4, dt_t, dt_t^2 with
@@ -606,7 +616,12 @@ static hid_t particle_type(const ch4g_run_params *rp)
         M(dx_ux); M(dy_ux); M(dz_ux); M(dx_uy); M(dy_uy); M(dz_uy); M(dx_uz); M(dy_uz); M(dz_uz);
         if (fl & CH4G_LAGRANGE_ADDEDMASS) M(beta_coeff);
     }
-    if (rp->cfg.flags & CH4G_LB_TEMPERATURE) M(t);
+    if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
+        M(t);
+        if (fl & CH4G_LAGRANGE_GRADIENT) {
+            M(dx_t); M(dy_t); M(dz_t);
+        }
+    }
 #undef M
     return t;
 }
@@ -691,7 +706,158 @@ void read_point_particle_h5(void)
     if (ch4g_set_particles(S.lat, a, n)) die("read_point_particle_h5");
     free(a);
 }
+/* Datasets of a particle snapshot, in the order of output_particles
+ * (lagrange.c:1700-2160), for the switches of this build: every record
+ * member is an FP64 array of particle_number entries under /lagrange.      */
+struct snap_member {
+    const char *name;
+    size_t off;
+};
+
+static int snapshot_members(const ch4g_run_params *rp, struct snap_member *m)
+{
+    const unsigned fl = rp->pcfg.flags;
+    const int grad = (fl & CH4G_LAGRANGE_GRADIENT) != 0, am = grad && (fl & CH4G_LAGRANGE_ADDEDMASS);
+    int n = 0;
+#define SM(nm, f) (m[n].name = nm, m[n].off = offsetof(ch4g_particle, f), n++)
+    SM("name", name); SM("tau_drag", tau_drag);
+    SM("x", x); SM("y", y); SM("z", z);
+    SM("vx", vx); SM("vy", vy); SM("vz", vz);
+    SM("ax", ax); SM("ay", ay); SM("az", az);
+    if (grad) {
+        SM("dx_ux", dx_ux); SM("dy_ux", dy_ux); SM("dz_ux", dz_ux);
+        SM("dx_uy", dx_uy); SM("dy_uy", dy_uy); SM("dz_uy", dz_uy);
+        SM("dx_uz", dx_uz); SM("dy_uz", dy_uz); SM("dz_uz", dz_uz);
+        if (am) SM("beta_coeff", beta_coeff);
+        if (rp->lagr_output_fluid) {
+            SM("ux", ux); SM("uy", uy); SM("uz", uz);
+            if (am) {
+                SM("Dt_ux", Dt_ux); SM("Dt_uy", Dt_uy); SM("Dt_uz", Dt_uz);
+            }
+        }
+    }
+    if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
+        SM("temperature", t); SM("dt_t", dt_t);
+        if (grad) {
+            SM("dx_t", dx_t); SM("dy_t", dy_t); SM("dz_t", dz_t);
+        }
+    }
+#undef SM
+    return n;
+}
+
+/* XDMF items of the descriptor (lagrange.c:2176-2590): a scalar attribute,
+ * or a vector / tensor joined from k datasets */
+static void xmf_scalar(FILE *fo, const char *h5, const char *attr, const char *ds, int np)
+{
+    fprintf(fo, "<Attribute Name=\"%s\" AttributeType=\"Scalar\" Center=\"Node\"> \n", attr);
+    fprintf(fo, "<DataItem Dimensions=\"%d\" NumberType=\"Float\" Precision=\"8\" Format=\"HDF\">\n", np);
+    fprintf(fo, "%s:/lagrange/%s\n</DataItem>\n</Attribute>\n", h5, ds);
+}
+
+static void xmf_joined(FILE *fo, const char *h5, const char *attr, const char *type, const char *const *ds, int k,
+                       int np)
+{
+    fprintf(fo, "<Attribute Name=\"%s\" AttributeType=\"%s\" Center=\"Node\"> \n", attr, type);
+    fprintf(fo, "<DataItem ItemType=\"Function\" Dimensions=\"%d %d\" \n   Function=\"JOIN(", np, k);
+    /* the reference's spelling: "$0 , $1, $2" for vectors, "$0,$1,...,$8" for the tensor */
+    for (int i = 0; i < k; i++) fprintf(fo, "%s$%d", i == 0 ? "" : (k == 3 ? (i == 1 ? " , " : ", ") : ","), i);
+    fprintf(fo, ")\">\n");
+    for (int i = 0; i < k; i++) {
+        fprintf(fo, "<DataItem Dimensions=\"%d 1\" NumberType=\"Float\" Precision=\"8\" Format=\"HDF\">\n", np);
+        fprintf(fo, "%s:/lagrange/%s\n</DataItem>\n", h5, ds[i]);
+    }
+    fprintf(fo, "</DataItem>\n</Attribute>\n");
+}
+
+static void write_particle_snapshot(void)
+{
+    const ch4g_run_params *rp = &S.rp;
+    int64_t n = 0;
+    if (ch4g_particles_count(S.lat, &n)) die("output_particles");
+    ch4g_particle *a = (ch4g_particle *)malloc(sizeof(ch4g_particle) * (size_t)(n > 0 ? n : 1));
+    double *aux = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!a || !aux) die("output_particles: out of memory");
+    if (ch4g_get_particles(S.lat, a, n, &n)) die("output_particles");
+    const int np = rp->pcfg.particle_number;
+    const int64_t total = np > n ? np : n;
+    char h5[64], path[96], xmf[96];
+    snprintf(h5, sizeof h5, "particle_%d.h5", S.itime);
+    snprintf(path, sizeof path, "RUN/%s", h5);
+    snprintf(xmf, sizeof xmf, "RUN/particle_%d.xmf", S.itime);
+    mkdir("RUN", S_IWUSR | S_IXUSR | S_IRUSR); /* parameters.c:46-48 */
+    fprintf(stderr, "Writing file %s\n", path);
+    hid_t file = H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    if (file < 0) die("output_particles");
+    hid_t group = H5Gcreate2(file, "/lagrange", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+    hsize_t dims[1] = {(hsize_t)total}, count[1] = {(hsize_t)n}, offset[1] = {0};
+    hid_t fspace = H5Screate_simple(1, dims, NULL);
+    hid_t mspace = H5Screate_simple(1, count, NULL);
+    H5Sselect_hyperslab(fspace, H5S_SELECT_SET, offset, NULL, count, NULL);
+    struct snap_member m[40];
+    const int nm = snapshot_members(rp, m);
+    for (int k = 0; k < nm; k++) {
+        for (int64_t i = 0; i < n; i++) aux[i] = *(const double *)((const char *)&a[i] + m[k].off);
+        hid_t d = H5Dcreate2(group, m[k].name, H5T_NATIVE_DOUBLE, fspace, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+        if (d < 0 || (n > 0 && H5Dwrite(d, H5T_NATIVE_DOUBLE, mspace, fspace, H5P_DEFAULT, aux) < 0))
+            die("output_particles");
+        H5Dclose(d);
+    }
+    H5Sclose(mspace);
+    H5Sclose(fspace);
+    H5Gclose(group);
+    H5Fclose(file);
+    free(aux);
+    free(a);
+
+    FILE *fo = fopen(xmf, "w");
+    if (!fo) die("output_particles: xmf");
+    const unsigned fl = rp->pcfg.flags;
+    const int grad = (fl & CH4G_LAGRANGE_GRADIENT) != 0, am = grad && (fl & CH4G_LAGRANGE_ADDEDMASS);
+    fprintf(fo, "<?xml version=\"1.0\" ?>\n<!DOCTYPE Xdmf SYSTEM \"Xdmf.dtd\" []>\n<Xdmf Version=\"2.0\">\n<Domain>\n");
+    fprintf(fo, "<Grid Name=\"Points\" GridType=\"Uniform\">\n");
+    fprintf(fo, "<Topology TopologyType=\"Polyvertex\" NumberOfElements=\"%d\"/>\n", np);
+    fprintf(fo, "<Geometry GeometryType=\"X_Y_Z\">\n");
+    const char *xyz[3] = {"x", "y", "z"};
+    for (int c = 0; c < 3; c++) {
+        fprintf(fo, "<DataItem Name=\"%s\" Dimensions=\"%d\" NumberType=\"Float\" Precision=\"8\" Format=\"HDF\">\n",
+                xyz[c], np);
+        fprintf(fo, "%s:/lagrange/%s\n</DataItem>\n", h5, xyz[c]);
+    }
+    fprintf(fo, "</Geometry>\n");
+    xmf_scalar(fo, h5, "name", "name", np);
+    xmf_scalar(fo, h5, "tau_drag", "tau_drag", np);
+    const char *v[3] = {"vx", "vy", "vz"}, *ac[3] = {"ax", "ay", "az"};
+    xmf_joined(fo, h5, "velocity", "Vector", v, 3, np);
+    xmf_joined(fo, h5, "acceleration", "Vector", ac, 3, np);
+    if (grad) {
+        const char *g[9] = {"dx_ux", "dy_ux", "dz_ux", "dx_uy", "dy_uy", "dz_uy", "dx_uz", "dy_uz", "dz_uz"};
+        xmf_joined(fo, h5, "fluid velocity gradient", "Tensor", g, 9, np);
+        if (am) xmf_scalar(fo, h5, "beta_coeff", "beta_coeff", np);
+        if (rp->lagr_output_fluid) {
+            const char *u[3] = {"ux", "uy", "uz"}, *du[3] = {"Dt_ux", "Dt_uy", "Dt_uz"};
+            xmf_joined(fo, h5, "fluid_velocity", "Vector", u, 3, np);
+            if (am) xmf_joined(fo, h5, "fluid_acceleration", "Vector", du, 3, np);
+        }
+    }
+    if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
+        xmf_scalar(fo, h5, "temperature", "temperature", np);
+        xmf_scalar(fo, h5, "dt_t", "dt_t", np);
+        if (grad) {
+            const char *gt[3] = {"dx_t", "dy_t", "dz_t"};
+            xmf_joined(fo, h5, "temperature gradient", "Vector", gt, 3, np);
+        }
+    }
+    fprintf(fo, "<Time Value=\" %e\" />\n</Grid>\n</Domain>\n</Xdmf>\n", rp->time_start + S.itime * rp->time_dt);
+    fclose(fo);
+}
 #else
+static void write_particle_snapshot(void)
+{
+    fprintf(stderr, "Error message -> this drop-in layer was built without HDF5 (particle snapshots). Exit.\n");
+    exit(-1);
+}
+
 void write_point_particle_h5(void)
 {
     fprintf(stderr, "Error message -> this drop-in layer was built without HDF5 (part.h5). Exit.\n");

@@ -77,7 +77,7 @@ static const char *NEUTRAL[] = {
     /* particles: umbrella switches, debug output, and the two-node halo the
        particle fields always use here */
     "LAGRANGE", "LAGRANGE_SMALLTAUD", "LAGRANGE_TWOWAY", "LAGRANGE_GRADIENT_BRD2", "LAGRANGE_DEBUG",
-    "LAGRANGE_OUTPUT_DEBUG",
+    "LAGRANGE_OUTPUT_DEBUG", "LAGRANGE_OUTPUT_FLUID",
 };
 
 static int in_list(const char *n, const char *const *l, size_t k)
@@ -216,6 +216,7 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         if (!strcmp(nm, "OUTPUT_AT_START")) out->output_at_start = 1;
         if (!strcmp(nm, "OUTPUT_DIAGN_APPEND")) out->diagn_append = 1;
         if (!strcmp(nm, "OUTPUT_H5")) out->output_h5 = 1;
+        if (!strcmp(nm, "LAGRANGE_OUTPUT_FLUID")) out->lagr_output_fluid = 1;
         if (!strncmp(nm, "GRID_POP_", 9) && strcmp(nm, "GRID_POP_D3Q19"))
             return perr("%.63s: only GRID_POP_D3Q19 is implemented on the device lattice", nm);
         if (!known && !in_list(nm, NEUTRAL, sizeof NEUTRAL / sizeof NEUTRAL[0]))

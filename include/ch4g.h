@@ -244,6 +244,7 @@ typedef struct ch4g_particle {
     double Dt_ux, Dt_uy, Dt_uz;
     double t, t_old, dt_t;
     double tau_drag, beta_coeff;
+    double dx_t, dy_t, dz_t;   /* temperature gradient (LB_TEMPERATURE with LAGRANGE_GRADIENT) */
 } ch4g_particle;
 
 /* allocate_particles (lagrange.c:6-49) + initial_conditions_particles

@@ -49,6 +49,7 @@ typedef struct ch4g_run_params {
     int lagrange;        /* define.h LAGRANGE */
     ch4g_particle_config pcfg;
     double time_dump_lagr;
+    int lagr_output_fluid; /* define.h LAGRANGE_OUTPUT_FLUID: u (and Dt_u) in the particle snapshots */
 } ch4g_run_params;
 
 /* Read a reference-style define.h (active #define names) and param.in

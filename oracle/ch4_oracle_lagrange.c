@@ -250,9 +250,11 @@ static const int CORNER[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1,
  * where both neighbours exist; with fluid walls (LB_FLUID_BC) one-sided
  * second order at the first and last fluid rows of y (:201-225) and the
  * LAGRANGE_INTERPOLATION_FIX at the ghost cells next to the interior in
- * every direction (:372-486), which moves the evaluation point one cell
- * inwards and keeps the moved index for the directions that follow; then
- * components along a size-one axis are zeroed (:494-496).                   */
+ * every direction (:288-368), which moves the evaluation point one cell
+ * inwards and keeps the moved index for the directions that follow; without
+ * fluid walls the fix evaluates the one-sided difference at the ghost cell
+ * itself (:413-488); then components along a size-one axis are zeroed
+ * (:494-496).                                                              */
 static oten grad_vec(const oracle *o, int i, int j, int k)
 {
     const olagr *L = o->lagr;
@@ -282,7 +284,6 @@ static oten grad_vec(const oracle *o, int i, int j, int k)
         g.yz = (t[p].y - t[m].y) / d;
         g.zz = (t[p].z - t[m].z) / d;
     }
-    if (HAS(o, CH4G_FLUID_BC_Y)) {
 #define FWD(c, a)                                                              \
     h1 = c[a + 1] - c[a];                                                      \
     h2 = c[a + 2] - c[a + 1];                                                  \
@@ -299,6 +300,7 @@ static oten grad_vec(const oracle *o, int i, int j, int k)
     X = a0 * t[n0].x + a1 * t[n1].x + a2 * t[n2].x;                            \
     Y = a0 * t[n0].y + a1 * t[n1].y + a2 * t[n2].y;                            \
     Z = a0 * t[n0].z + a1 * t[n1].z + a2 * t[n2].z;
+    if (HAS(o, CH4G_FLUID_BC_Y)) {
         /* LB_FLUID_BC_Y, SECOND_ORDER_GRAD */
         if (j == B2) {
             FWD(L->cy, j)
@@ -339,19 +341,136 @@ static oten grad_vec(const oracle *o, int i, int j, int k)
             BWD(L->cz, k)
             COMB(IDX2(L, i, j, k), IDX2(L, i, j, k - 1), IDX2(L, i, j, k - 2), g.xz, g.yz, g.zz)
         }
-#undef FWD
-#undef BWD
-#undef COMB
+    } else {
+        /* LAGRANGE_INTERPOLATION_FIX, SECOND_ORDER_GRAD, no fluid walls */
+        if (i == B2 - 1) {
+            FWD(L->cx, i)
+            COMB(IDX2(L, i, j, k), IDX2(L, i + 1, j, k), IDX2(L, i + 2, j, k), g.xx, g.yx, g.zx)
+        }
+        if (i == NX + B2) {
+            BWD(L->cx, i)
+            COMB(IDX2(L, i, j, k), IDX2(L, i - 1, j, k), IDX2(L, i - 2, j, k), g.xx, g.yx, g.zx)
+        }
+        if (j == B2 - 1) {
+            FWD(L->cy, j)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j + 1, k), IDX2(L, i, j + 2, k), g.xy, g.yy, g.zy)
+        }
+        if (j == NY + B2) {
+            BWD(L->cy, j)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j - 1, k), IDX2(L, i, j - 2, k), g.xy, g.yy, g.zy)
+        }
+        if (k == B2 - 1) {
+            FWD(L->cz, k)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j, k + 1), IDX2(L, i, j, k + 2), g.xz, g.yz, g.zz)
+        }
+        if (k == NZ + B2) {
+            BWD(L->cz, k)
+            COMB(IDX2(L, i, j, k), IDX2(L, i, j, k - 1), IDX2(L, i, j, k - 2), g.xz, g.yz, g.zz)
+        }
     }
+#undef COMB
     if (NX == 1) g.xx = g.yx = g.zx = 0.0;
     if (NY == 1) g.xy = g.yy = g.zy = 0.0;
     if (NZ == 1) g.xz = g.yz = g.zz = 0.0;
     return g;
 }
 
+/* gradient_scalar with LAGRANGE (gradients.c:505-827) of the temperature
+ * with its halo: centred differences where both neighbours exist; with
+ * temperature walls (LB_TEMPERATURE_BC_Y) one-sided second order at the
+ * first and last rows of y (:560-626) and the LAGRANGE_INTERPOLATION_FIX
+ * that moves the evaluation point one cell inwards (:657-725); without them
+ * the fix evaluates the one-sided difference at the ghost cell itself
+ * (:762-825); components along a size-one axis are zeroed (:829-831).     */
+static ovec grad_sca(const oracle *o, int i, int j, int k)
+{
+    const olagr *L = o->lagr;
+    const double *t = L->t;
+    const int NX = o->NX, NY = o->NY, NZ = o->NZ;
+    ovec g;
+    memset(&g, 0, sizeof g);
+    double h1, h2, a0, a1, a2;
+    if (i > 0 && i < L->GX - 1) g.x = (t[IDX2(L, i + 1, j, k)] - t[IDX2(L, i - 1, j, k)]) / (L->cx[i + 1] - L->cx[i - 1]);
+    if (j > 0 && j < L->GY - 1) g.y = (t[IDX2(L, i, j + 1, k)] - t[IDX2(L, i, j - 1, k)]) / (L->cy[j + 1] - L->cy[j - 1]);
+    if (k > 0 && k < L->GZ - 1) g.z = (t[IDX2(L, i, j, k + 1)] - t[IDX2(L, i, j, k - 1)]) / (L->cz[k + 1] - L->cz[k - 1]);
+#define S3(n0, n1, n2) (a0 * t[n0] + a1 * t[n1] + a2 * t[n2])
+    if (HAS(o, CH4G_TEMPERATURE_BC_Y)) {
+        if (j == B2) {
+            FWD(L->cy, j)
+            g.y = S3(IDX2(L, i, j, k), IDX2(L, i, j + 1, k), IDX2(L, i, j + 2, k));
+        }
+        if (j == NY + B2 - 1) {
+            BWD(L->cy, j)
+            g.y = S3(IDX2(L, i, j, k), IDX2(L, i, j - 1, k), IDX2(L, i, j - 2, k));
+        }
+        if (i == B2 - 1) {
+            i = i + 1;
+            FWD(L->cx, i)
+            g.x = S3(IDX2(L, i, j, k), IDX2(L, i + 1, j, k), IDX2(L, i + 2, j, k));
+        }
+        if (i == NX + B2) {
+            i = i - 1;
+            BWD(L->cx, i)
+            g.x = S3(IDX2(L, i, j, k), IDX2(L, i - 1, j, k), IDX2(L, i - 2, j, k));
+        }
+        if (j == B2 - 1) {
+            j = j + 1;
+            FWD(L->cy, j)
+            g.y = S3(IDX2(L, i, j, k), IDX2(L, i, j + 1, k), IDX2(L, i, j + 2, k));
+        }
+        if (j == NY + B2) {
+            j = j - 1;
+            BWD(L->cy, j)
+            g.y = S3(IDX2(L, i, j, k), IDX2(L, i, j - 1, k), IDX2(L, i, j - 2, k));
+        }
+        if (k == B2 - 1) {
+            k = k + 1;
+            FWD(L->cz, k)
+            g.z = S3(IDX2(L, i, j, k), IDX2(L, i, j, k + 1), IDX2(L, i, j, k + 2));
+        }
+        if (k == NZ + B2) {
+            k = k - 1;
+            BWD(L->cz, k)
+            g.z = S3(IDX2(L, i, j, k), IDX2(L, i, j, k - 1), IDX2(L, i, j, k - 2));
+        }
+    } else {
+        if (i == B2 - 1) {
+            FWD(L->cx, i)
+            g.x = S3(IDX2(L, i, j, k), IDX2(L, i + 1, j, k), IDX2(L, i + 2, j, k));
+        }
+        if (i == NX + B2) {
+            BWD(L->cx, i)
+            g.x = S3(IDX2(L, i, j, k), IDX2(L, i - 1, j, k), IDX2(L, i - 2, j, k));
+        }
+        if (j == B2 - 1) {
+            FWD(L->cy, j)
+            g.y = S3(IDX2(L, i, j, k), IDX2(L, i, j + 1, k), IDX2(L, i, j + 2, k));
+        }
+        if (j == NY + B2) {
+            BWD(L->cy, j)
+            g.y = S3(IDX2(L, i, j, k), IDX2(L, i, j - 1, k), IDX2(L, i, j - 2, k));
+        }
+        if (k == B2 - 1) {
+            FWD(L->cz, k)
+            g.z = S3(IDX2(L, i, j, k), IDX2(L, i, j, k + 1), IDX2(L, i, j, k + 2));
+        }
+        if (k == NZ + B2) {
+            BWD(L->cz, k)
+            g.z = S3(IDX2(L, i, j, k), IDX2(L, i, j, k - 1), IDX2(L, i, j, k - 2));
+        }
+    }
+#undef S3
+    if (NX == 1) g.x = 0.0;
+    if (NY == 1) g.y = 0.0;
+    if (NZ == 1) g.z = 0.0;
+    return g;
+}
+#undef FWD
+#undef BWD
+
 /* interpolate_vector_at_particles(u,'u') and interpolate_scalar_at_particles
- * (t,'t') (lagrange.c:1211-1593); the scalar's gradient feeds only
- * particle-temperature switches that are not restated.                     */
+ * (t,'t') (lagrange.c:1211-1593), with the gradients of both under
+ * LAGRANGE_GRADIENT.                                                       */
 static void interpolate(oracle *o)
 {
     olagr *L = o->lagr;
@@ -395,6 +514,19 @@ static void interpolate(oracle *o)
             double v = L->t[s.n[0]] * s.w[0];
             for (int c = 1; c < 8; c++) v += L->t[s.n[c]] * s.w[c];
             p->t = v;
+            if (grad) { /* lagrange.c:1500-1576 */
+                ovec G[8];
+                for (int c = 0; c < 8; c++) G[c] = grad_sca(o, s.im + CORNER[c][0], s.jm + CORNER[c][1], s.km + CORNER[c][2]);
+                double gx = G[0].x * s.w[0], gy = G[0].y * s.w[0], gz = G[0].z * s.w[0];
+                for (int c = 1; c < 8; c++) {
+                    gx += G[c].x * s.w[c];
+                    gy += G[c].y * s.w[c];
+                    gz += G[c].z * s.w[c];
+                }
+                p->dx_t = gx;
+                p->dy_t = gy;
+                p->dz_t = gz;
+            }
         }
     }
 }

@@ -31,7 +31,7 @@ class RunParams(C.Structure):
                 ("time_end", C.c_double), ("time_dump_field", C.c_double), ("time_dump_diagn", C.c_double),
                 ("output_at_start", C.c_int), ("diagn_append", C.c_int), ("output_h5", C.c_int),
                 ("resume", C.c_int), ("resume_u", C.c_int), ("resume_t", C.c_int), ("lagrange", C.c_int), ("pcfg", abi.ParticleConfig),
-                ("time_dump_lagr", C.c_double)]
+                ("time_dump_lagr", C.c_double), ("lagr_output_fluid", C.c_int)]
 
 
 def test_dropin_exports_reference_names():
@@ -306,7 +306,8 @@ def test_part_h5_dump_and_restart(tmp_path):
     o.particles_init(rp.pcfg)
     o.step(int(rp.time_end))
     ref = o.particles()
-    names = ["name", "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "ux", "uy", "uz", "tau_drag"]
+    names = ["name", "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "ux", "uy", "uz", "tau_drag", "t", "dx_t",
+             "dy_t", "dz_t"]
     got = read_compound_h5(d1 / "part.h5", "/particles/point_particle", names, (rp.pcfg.particle_number,))
     n = len(ref)
     for i, m in enumerate(names):
@@ -316,3 +317,76 @@ def test_part_h5_dump_and_restart(tmp_path):
     r = subprocess.run([RUN], cwd=d2, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "initialized from file" in r.stderr and "The End" in r.stdout
+
+
+def read_dataset_h5(path, name, n):
+    """One 1-D FP64 dataset of n entries, and the member names of its group."""
+    h5 = C.CDLL(H5LIB)
+    hid = C.c_int64
+    h5.H5open()
+    native_double = hid.in_dll(h5, "H5T_NATIVE_DOUBLE_g").value
+    h5.H5Fopen.restype = hid
+    h5.H5Fopen.argtypes = [C.c_char_p, C.c_uint, hid]
+    h5.H5Dopen2.restype = hid
+    h5.H5Dopen2.argtypes = [hid, C.c_char_p, hid]
+    h5.H5Dget_space.restype = hid
+    h5.H5Dget_space.argtypes = [hid]
+    h5.H5Sget_simple_extent_dims.argtypes = [hid, C.POINTER(C.c_uint64), C.c_void_p]
+    h5.H5Dread.argtypes = [hid, hid, hid, hid, hid, C.c_void_p]
+    for f in ("H5Dclose", "H5Sclose", "H5Fclose"):
+        getattr(h5, f).argtypes = [hid]
+    f = h5.H5Fopen(str(path).encode(), 0, 0)
+    assert f >= 0
+    d = h5.H5Dopen2(f, name.encode(), 0)
+    assert d >= 0, name
+    sp = h5.H5Dget_space(d)
+    dims = (C.c_uint64 * 1)()
+    assert h5.H5Sget_simple_extent_dims(sp, dims, None) == 1 and dims[0] == n
+    buf = np.empty(n)
+    assert h5.H5Dread(d, native_double, 0, 0, 0, buf.ctypes.data) >= 0
+    for x, fn in ((sp, "H5Sclose"), (d, "H5Dclose"), (f, "H5Fclose")):
+        getattr(h5, fn)(x)
+    return buf
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(H5LIB), reason="serial HDF5 not installed")
+def test_particle_snapshots(tmp_path):
+    """OUTPUT_H5 with LAGRANGE: output_particles writes RUN/particle_<itime>.h5
+    every time_dump_lagr (lagrange.c:1598-2595), one FP64 dataset per member
+    under /lagrange in the reference's order, equal to the oracle's records
+    (FP64 path: bit-identical), and an XDMF descriptor naming those
+    datasets."""
+    import xml.etree.ElementTree as ET
+    define = open(os.path.join(FIXP, "define.h")).read() + "#define OUTPUT_H5\n#define LAGRANGE_OUTPUT_FLUID\n"
+    (tmp_path / "define.h").write_text(define)
+    shutil.copy(os.path.join(FIXP, "param.in"), tmp_path / "param.in")
+    r = subprocess.run([RUN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    for it in (0, 40, 80, 120):
+        assert (tmp_path / "RUN" / f"particle_{it}.h5").exists() and (tmp_path / "RUN" / f"particle_{it}.xmf").exists()
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    assert lib.ch4g_read_run_params(str(tmp_path / "define.h").encode(), str(tmp_path / "param.in").encode(),
+                                    C.byref(rp)) == 0
+    from oracle import Oracle
+    o = Oracle(rp.cfg)
+    o.initial_conditions()
+    o.particles_init(rp.pcfg)
+    o.step(120)
+    ref = o.particles()
+    n, npart = len(ref), rp.pcfg.particle_number
+    # gradient + added mass + LAGRANGE_OUTPUT_FLUID + temperature, in order
+    members = ["name", "tau_drag", "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "dx_ux", "dy_ux", "dz_ux",
+               "dx_uy", "dy_uy", "dz_uy", "dx_uz", "dy_uz", "dz_uz", "beta_coeff", "ux", "uy", "uz", "Dt_ux", "Dt_uy",
+               "Dt_uz", "temperature", "dt_t", "dx_t", "dy_t", "dz_t"]
+    path = tmp_path / "RUN" / "particle_120.h5"
+    for m in members:
+        got = read_dataset_h5(path, "/lagrange/" + m, npart)
+        want = ref["t" if m == "temperature" else m]
+        np.testing.assert_array_equal(got[:n], want, err_msg=m)
+    root = ET.parse(tmp_path / "RUN" / "particle_120.xmf").getroot()
+    refs = [e.text.strip() for e in root.iter("DataItem") if e.text and e.text.strip()]
+    assert all(t.startswith("particle_120.h5:/lagrange/") for t in refs)
+    assert {t.split(":/lagrange/")[1] for t in refs} == set(members)
+    assert root.find("Domain/Grid/Topology").get("NumberOfElements") == str(npart)

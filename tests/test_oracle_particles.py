@@ -181,6 +181,48 @@ def test_noslip_ghost_and_gradient_near_wall():
     assert r["dy_ux"][0] == pytest.approx(dqdy(2.5, 0.5, 2.5), abs=1e-14)
 
 
+def test_temperature_gradient_of_the_conduction_profile():
+    """gradient_scalar at the particles (gradients.c:505-827 with
+    LB_TEMPERATURE_BC_Y): for the linear conduction profile the fixed-T
+    ghost t_g = 2 T_w - t continues the line, so centred differences, the
+    one-sided wall rows and the ghost-row fix all give dT/dy = -1/H exactly
+    (to rounding), and dT/dx = dT/dz = 0 exactly."""
+    o, cfg = quiescent(6, 8, 5, flags=abi.LB_TEMPERATURE | abi.TEMPERATURE_BC_Y, T_bot=1.0, T_top=0.0, T_ref=0.0)
+    o.particles_init(abi.make_particle_config(abi.LAGRANGE_GRADIENT, 1, tau_drag=(1, 1.0, 1.0)))
+    x, y, z = _centres(cfg)
+    t = 1.0 - y / 8.0
+    rng = np.random.default_rng(4)
+    pos = np.column_stack([rng.uniform(0, 6, 300), rng.uniform(0.01, 7.99, 300), rng.uniform(0, 5, 300)])
+    r = _probe(o, pos, np.zeros(x.shape + (3,)), t)
+    assert np.allclose(r["t"], 1.0 - pos[:, 1] / 8.0, rtol=0, atol=1e-15)
+    assert np.allclose(r["dy_t"], -1.0 / 8.0, rtol=0, atol=1e-15)
+    assert not r["dx_t"].any() and not r["dz_t"].any()
+
+
+def test_gradient_at_a_periodic_ghost_is_one_sided():
+    """Without walls the interpolation fix takes the second-order one-sided
+    difference at the ghost node itself, over its periodic image
+    (gradients.c:413-488 for u, :762-825 for T), not a centred difference of
+    images.  For f = b x_c on the interior centres of an N-periodic axis the
+    ghost at -1 holds f(N-1) = b (N - 1/2), so
+      ghost:        -3/2 f(-1) + 2 f(0) - 1/2 f(1) = b (1 - 3N/2)
+      first node:   (f(1) - f(-1)) / 2              = b (2 - N) / 2
+    and a particle at x = 1/4 (weights 1/4 and 3/4) reads their blend,
+    -5 b for N = 8 (a centred difference at the ghost would give -3 b);
+    the mirror case x = N - 1/4 uses the ghost at N the same way."""
+    o, cfg = quiescent(8, 6, 4, flags=abi.LB_TEMPERATURE)
+    o.particles_init(abi.make_particle_config(abi.LAGRANGE_GRADIENT, 1, tau_drag=(1, 1.0, 1.0)))
+    x, y, z = _centres(cfg)
+    b, N = 0.01, 8
+    u = np.stack([b * x, 0 * x, 0 * x], -1)
+    pos = np.array([[0.25, 2.5, 1.5], [N - 0.25, 2.5, 1.5]])
+    r = _probe(o, pos, u, b * x)
+    want = 0.25 * b * (1 - 1.5 * N) + 0.75 * b * (2 - N) / 2
+    assert want == pytest.approx(-5 * b)
+    assert np.allclose(r["dx_ux"], want, rtol=1e-14, atol=0)
+    assert np.allclose(r["dx_t"], want, rtol=1e-14, atol=0)
+
+
 def test_tracers_move_with_uniform_flow():
     o, cfg = quiescent(16, 8, 8, tau=0.9)
     u0 = np.array([0.031, -0.017, 0.009])
