@@ -44,6 +44,11 @@ LAGRANGE_TWOWAY_MOMENTUM = 1 << 12
 LAGRANGE_TWOWAY_SMALLTAUD = 1 << 13
 LAGRANGE_TAUDRAG_INCREMENT_LOG = 1 << 14
 LAGRANGE_ADDEDMASS_INCREMENT_LOG = 1 << 15
+LAGRANGE_ORIENTATION = 1 << 16
+LAGRANGE_ORIENTATION_JEFFREY = 1 << 17
+LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG = 1 << 18
+LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS = 1 << 19
+LAGRANGE_ORIENTATION_ACTIVE = 1 << 20
 
 FP64 = 0
 FP32_DEV = 1
@@ -124,11 +129,16 @@ class ParticleConfig(C.Structure):
         ("tau_drag_types", C.c_double), ("tau_drag_min", C.c_double), ("tau_drag_max", C.c_double),
         ("beta_coeff_types", C.c_double), ("beta_coeff_min", C.c_double), ("beta_coeff_max", C.c_double),
         ("time_dt_lagr_factor", C.c_double),
+        ("aspect_ratio_types", C.c_double), ("aspect_ratio_min", C.c_double), ("aspect_ratio_max", C.c_double),
+        ("gyrotaxis_velocity_types", C.c_double), ("gyrotaxis_velocity_min", C.c_double),
+        ("gyrotaxis_velocity_max", C.c_double),
+        ("swim_velocity_types", C.c_double), ("swim_velocity_min", C.c_double), ("swim_velocity_max", C.c_double),
     ]
 
 
 def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_coeff=(1, 1.0, 1.0),
-                         time_dt_lagr_factor=1.0):
+                         time_dt_lagr_factor=1.0, aspect_ratio=(1, 1.0, 1.0), gyrotaxis_velocity=(1, 0.0, 0.0),
+                         swim_velocity=(1, 0.0, 0.0)):
     """ParticleConfig from (types, min, max) ladders as param.in gives them."""
     pc = ParticleConfig()
     pc.flags = flags
@@ -136,13 +146,18 @@ def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_co
     pc.tau_drag_types, pc.tau_drag_min, pc.tau_drag_max = (float(v) for v in tau_drag)
     pc.beta_coeff_types, pc.beta_coeff_min, pc.beta_coeff_max = (float(v) for v in beta_coeff)
     pc.time_dt_lagr_factor = float(time_dt_lagr_factor)
+    pc.aspect_ratio_types, pc.aspect_ratio_min, pc.aspect_ratio_max = (float(v) for v in aspect_ratio)
+    (pc.gyrotaxis_velocity_types, pc.gyrotaxis_velocity_min,
+     pc.gyrotaxis_velocity_max) = (float(v) for v in gyrotaxis_velocity)
+    pc.swim_velocity_types, pc.swim_velocity_min, pc.swim_velocity_max = (float(v) for v in swim_velocity)
     return pc
 
 
 PARTICLE_FIELDS = ["name", "x", "y", "z", "vx", "vy", "vz", "vx_old", "vy_old", "vz_old",
                    "ax", "ay", "az", "ax_old", "ay_old", "az_old", "ux", "uy", "uz", "ux_old", "uy_old", "uz_old",
                    "dx_ux", "dy_ux", "dz_ux", "dx_uy", "dy_uy", "dz_uy", "dx_uz", "dy_uz", "dz_uz",
-                   "Dt_ux", "Dt_uy", "Dt_uz", "t", "t_old", "dt_t", "tau_drag", "beta_coeff", "dx_t", "dy_t", "dz_t"]
+                   "Dt_ux", "Dt_uy", "Dt_uz", "t", "t_old", "dt_t", "tau_drag", "beta_coeff", "dx_t", "dy_t", "dz_t",
+                   "px", "py", "pz", "dt_px", "dt_py", "dt_pz", "aspect_ratio", "gyrotaxis_velocity", "swim_velocity"]
 
 
 class Particle(C.Structure):

@@ -187,6 +187,11 @@ enum PartField {
     PVOL,               // (4/3) pi (3 nu beta tau)^(3/2), host pow (lagrange_feedback.c:87-90)
     PDRAT,              // (3 - beta) / (2 beta) (lagrange_feedback.c:84)
     PNAME,
+    PPO,                // 3 rows: px, py, pz (orientation)
+    PDTP = PPO + 3,     // 3 rows: dt_px, dt_py, dt_pz
+    PASP = PDTP + 3,    // aspect_ratio
+    PGYV,               // gyrotaxis_velocity
+    PSWIM,              // swim_velocity
     PF_COUNT
 };
 

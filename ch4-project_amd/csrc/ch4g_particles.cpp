@@ -53,6 +53,24 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
     // reference reads a value left over from another particle
     if (uses_dtu && (!am || pc->beta_coeff_min <= 0.0))
         return fail(-1, "LAGRANGE_SMALLTAUD_* and LAGRANGE_TWOWAY_MOMENTUM need LAGRANGE_ADDEDMASS with beta_coeff_min > 0");
+    // orientation (lagrange.c:389-521 builds its ladders under LAGRANGE_GRADIENT;
+    // Jeffery's equation reads the interpolated velocity gradient)
+    const bool orient = lhas(*pc, CH4G_LAGRANGE_ORIENTATION), jeff = lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY);
+    if (orient && !lhas(*pc, CH4G_LAGRANGE_GRADIENT)) return fail(-1, "LAGRANGE_ORIENTATION needs LAGRANGE_GRADIENT");
+    if (!orient && (jeff || lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE)))
+        return fail(-1, "LAGRANGE_ORIENTATION_JEFFREY / _ACTIVE need LAGRANGE_ORIENTATION");
+    if (!jeff && (lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) ||
+                  lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG)))
+        return fail(-1, "LAGRANGE_ORIENTATION_JEFFREY_* need LAGRANGE_ORIENTATION_JEFFREY");
+    // parameters.c:632, :646, :692
+    if (jeff && (pc->aspect_ratio_types < 1 || pc->aspect_ratio_max < pc->aspect_ratio_min))
+        return fail(-1, "Error in aspect_ratio parameters");
+    if (lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) &&
+        (pc->gyrotaxis_velocity_types < 1 || pc->gyrotaxis_velocity_max < pc->gyrotaxis_velocity_min))
+        return fail(-1, "Error in gyrotaxis_velocity parameters");
+    if (lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE) &&
+        (pc->swim_velocity_types < 1 || pc->swim_velocity_max < pc->swim_velocity_min))
+        return fail(-1, "Error in swim_velocity parameters");
     if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_SMALLTAUD) && !lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM))
         return fail(-1, "LAGRANGE_TWOWAY_SMALLTAUD needs LAGRANGE_TWOWAY_MOMENTUM");
     if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM) && L->N >= ((int64_t)1 << 32) - 1)
@@ -229,6 +247,15 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         S(PVOL) = vol;
         S(PDRAT) = drat;
         S(PNAME) = r.name;
+        S(PPO) = r.px;
+        S(PPO + 1) = r.py;
+        S(PPO + 2) = r.pz;
+        S(PDTP) = r.dt_px;
+        S(PDTP + 1) = r.dt_py;
+        S(PDTP + 2) = r.dt_pz;
+        S(PASP) = r.aspect_ratio;
+        S(PGYV) = r.gyrotaxis_velocity;
+        S(PSWIM) = r.swim_velocity;
         alive[(size_t)p] = 1;
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
@@ -393,14 +420,23 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     Q->dt = 1.0 / pc->time_dt_lagr_factor; // parameters.c:164 with time_dt = 1
     Q->substeps = (int)pc->time_dt_lagr_factor;
 
-    // particle types (parameters.c:559-584, :743-755; lagrange.c:264-336)
+    // particle types (parameters.c:559-584, :621-695, :743-755) and their
+    // ladders in the reference's order (lagrange.c:264-336, :389-521)
     const bool am = lhas(*pc, CH4G_LAGRANGE_GRADIENT) && lhas(*pc, CH4G_LAGRANGE_ADDEDMASS);
+    const bool orient = lhas(*pc, CH4G_LAGRANGE_ORIENTATION);
+    const bool jeff = orient && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY);
+    const bool gyro = jeff && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
+    const bool swim = orient && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
     double types = pc->tau_drag_types;
     if (am) types *= pc->beta_coeff_types;
+    if (jeff) types *= pc->aspect_ratio_types;
+    if (gyro) types *= pc->gyrotaxis_velocity_types;
+    if (swim) types *= pc->swim_velocity_types;
     const int ntr = lhas(*pc, CH4G_LAGRANGE_ADD_TRACER) ? 1 : 0;
     types += ntr;
     const int nt = (int)types;
-    std::vector<double> tau((size_t)nt, 0.0), beta((size_t)nt, 0.0);
+    std::vector<double> tau((size_t)nt, 0.0), beta((size_t)nt, 0.0), asp((size_t)nt, 0.0), gyv((size_t)nt, 0.0),
+        swv((size_t)nt, 0.0);
     double rep = 1.0;
     ladder(tau, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
            lhas(*pc, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
@@ -410,9 +446,24 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
                lhas(*pc, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
         rep *= pc->beta_coeff_types;
     }
+    if (jeff) {
+        ladder(asp, ntr, 1.0, pc->aspect_ratio_types, pc->aspect_ratio_min, pc->aspect_ratio_max,
+               lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG), rep);
+        rep *= pc->aspect_ratio_types;
+    }
+    if (gyro) { // geometric steps only (lagrange.c:448-466)
+        ladder(gyv, ntr, 0.0, pc->gyrotaxis_velocity_types, pc->gyrotaxis_velocity_min, pc->gyrotaxis_velocity_max,
+               true, rep);
+        rep *= pc->gyrotaxis_velocity_types;
+    }
+    if (swim) { // linear steps only (lagrange.c:503-518)
+        ladder(swv, ntr, 0.0, pc->swim_velocity_types, pc->swim_velocity_min, pc->swim_velocity_max, false, rep);
+        rep *= pc->swim_velocity_types;
+    }
 
     // names, properties and positions (lagrange.c:634-641, :711-771): three
-    // draws per particle from the lattice's stream, zero velocity
+    // draws per particle from the lattice's stream, then with orientation
+    // random_vector's two (random.c:139-150, lagrange.c:891), zero velocity
     const int64_t np = pc->particle_number;
     std::vector<ch4g_particle> h((size_t)(np > 0 ? np : 1));
     memset(h.data(), 0, h.size() * sizeof(ch4g_particle));
@@ -426,6 +477,16 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         r.x = 0 + erand48(L->xsubi) * LNX;
         r.y = 0 + erand48(L->xsubi) * LNY;
         r.z = 0 + erand48(L->xsubi) * LNZ;
+        if (jeff) r.aspect_ratio = asp[(size_t)type];
+        if (gyro) r.gyrotaxis_velocity = gyv[(size_t)type];
+        if (swim) r.swim_velocity = swv[(size_t)type];
+        if (orient) {
+            const double theta = acos(2.0 * erand48(L->xsubi) - 1.0);
+            const double phi = 2 * 3.14159265359 * erand48(L->xsubi); // two_pi of typedef.h:367
+            r.px = sin(theta) * sin(phi);
+            r.py = sin(theta) * cos(phi);
+            r.pz = cos(theta);
+        }
     }
     if ((rc = alloc_state(L, np))) goto err;
     if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM)) {
@@ -520,6 +581,15 @@ int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t
             r.dz_t = S(PGT + 2);
             r.tau_drag = S(PTAU);
             r.beta_coeff = S(PBETA);
+            r.px = S(PPO);
+            r.py = S(PPO + 1);
+            r.pz = S(PPO + 2);
+            r.dt_px = S(PDTP);
+            r.dt_py = S(PDTP + 1);
+            r.dt_pz = S(PDTP + 2);
+            r.aspect_ratio = S(PASP);
+            r.gyrotaxis_velocity = S(PGYV);
+            r.swim_velocity = S(PSWIM);
         }
         k++;
     }

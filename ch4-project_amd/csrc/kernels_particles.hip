@@ -415,6 +415,14 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
             }
         }
         const double tau = R(PTAU), beta = R(PBETA);
+        const bool orient = lflag(P, CH4G_LAGRANGE_ORIENTATION);
+        double pv[3] = {0.0, 0.0, 0.0}, fold[3] = {0.0, 0.0, 0.0};
+        if (orient) {
+            for (int i = 0; i < 3; i++) {
+                pv[i] = R(PPO + i);
+                fold[i] = R(PDTP + i);
+            }
+        }
 
         // ---- interpolation at the (wrapped) position
         Stencil st;
@@ -483,6 +491,13 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
             vx = ux;
             vy = uy;
             vz = uz;
+            if (orient && lflag(P, CH4G_LAGRANGE_ORIENTATION_ACTIVE)) {
+                // swimming along p (lagrange.c:3179-3186)
+                const double sw = R(PSWIM);
+                vx += sw * pv[0];
+                vy += sw * pv[1];
+                vz += sw * pv[2];
+            }
             if (first) {
                 vxo = vx;
                 vyo = vy;
@@ -589,6 +604,66 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
             }
         }
 
+        // ---- orientation (lagrange.c:3470-3752)
+        if (orient) {
+            double rhs[3] = {0.0, 0.0, 0.0};
+            if (lflag(P, CH4G_LAGRANGE_ORIENTATION_JEFFREY)) {
+                // Jeffery's equation dp/dt = W p + S' p - (p.S' p) p, S' =
+                // f(alpha) S, with the gradient matrix A[i][j] = d_j u_i (G)
+                const double alpha = R(PASP);
+                const double fa = (alpha * alpha - 1.0) / (1.0 + alpha * alpha);
+                double Sm[3][3], Wm[3][3];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        Sm[i][j] = 0.5 * (G[3 * i + j] + G[3 * j + i]);
+                        Wm[i][j] = 0.5 * (G[3 * i + j] - G[3 * j + i]);
+                    }
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) Sm[i][j] *= fa;
+                if (lflag(P, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS)) {
+                    const double gv = R(PGYV);
+                    if (gv != 0) {
+                        // -1/(2 v0) g_i p_j enters the first column of S'
+                        // only: the reference adds the other six products
+                        // to the gradient matrix after S' and W are formed
+                        // (lagrange.c:3553-3562), where they have no effect
+                        const double gyro = -0.5 / gv;
+                        double va[3];
+                        if (lflag(P, CH4G_LAGRANGE_GRAVITY)) {
+                            va[0] = gyro * (-gx + ax);
+                            va[1] = gyro * (-gy + ay);
+                            va[2] = gyro * (-gz + az);
+                        } else {
+                            va[0] = gyro * 0.0;
+                            va[1] = gyro * 0.0;
+                            va[2] = gyro * 1.0;
+                        }
+                        for (int i = 0; i < 3; i++) Sm[i][0] += va[i] * pv[0];
+                    }
+                }
+                double tmp[3] = {0.0, 0.0, 0.0};
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) tmp[i] += Sm[i][j] * pv[j];
+                double oso = 0.0;
+                for (int i = 0; i < 3; i++) oso += pv[i] * tmp[i];
+                for (int i = 0; i < 3; i++) {
+                    rhs[i] = 0.0;
+                    for (int j = 0; j < 3; j++) rhs[i] += Wm[i][j] * pv[j] + (Sm[i][j] * pv[j]);
+                    rhs[i] -= pv[i] * oso;
+                }
+                for (int i = 0; i < 3; i++) {
+                    if (first)
+                        pv[i] = pv[i] + dt * rhs[i];
+                    else
+                        pv[i] = pv[i] + 0.5 * dt * (3. * rhs[i] - fold[i]);
+                    fold[i] = rhs[i];
+                }
+            }
+            double norm = 0.0;
+            for (int i = 0; i < 3; i++) norm += pv[i] * pv[i];
+            for (int i = 0; i < 3; i++) pv[i] /= sqrt(norm);
+        }
+
         // ---- wall rules at y (lagrange.c:3754-3993); x and z of a random
         // re-injection are filled in by the ordered draw pass
         if (P.wall_f) {
@@ -676,6 +751,12 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
             R(PDTU) = dtu[0];
             R(PDTU + 1) = dtu[1];
             R(PDTU + 2) = dtu[2];
+        }
+        if (orient) {
+            for (int i = 0; i < 3; i++) {
+                R(PPO + i) = pv[i];
+                R(PDTP + i) = fold[i];
+            }
         }
         if (P.thermal) {
             R(PT) = t;

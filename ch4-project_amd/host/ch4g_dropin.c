@@ -347,6 +347,24 @@ void output_particles(void)
     write_particle_snapshot();
 }
 
+/* property.particle_types (parameters.c:559-584, :621-695, :743-755): the
+ * product of the ladder sizes the switches read, plus the fluid tracer. */
+static double ptypes(const ch4g_particle_config *pc)
+{
+    const unsigned fl = pc->flags;
+    double types = pc->tau_drag_types;
+    if ((fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ADDEDMASS)) types *= pc->beta_coeff_types;
+    if ((fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ORIENTATION)) {
+        if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {
+            types *= pc->aspect_ratio_types;
+            if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) types *= pc->gyrotaxis_velocity_types;
+        }
+        if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) types *= pc->swim_velocity_types;
+    }
+    if (fl & CH4G_LAGRANGE_ADD_TRACER) types += 1.0;
+    return types;
+}
+
 /* dump_particle_averages (lagrange.c:2598-2930): per particle type, FP64
  * averages of v, v^2, v^4, a, a^2, a^4 (and t, t^2, t^4, dt_t, dt_t^2 with
  * LB_TEMPERATURE), accumulated in particle order, appended to
@@ -362,12 +380,10 @@ void dump_particle_averages(void)
     ch4g_particle *a = (ch4g_particle *)malloc(sizeof(ch4g_particle) * (size_t)(n > 0 ? n : 1));
     if (!a) die("dump_particle_averages: out of memory");
     if (ch4g_get_particles(S.lat, a, n, &n)) die("dump_particle_averages");
-    const ch4g_particle_config *pc = &rp->pcfg;
-    double types = pc->tau_drag_types;
-    if ((pc->flags & CH4G_LAGRANGE_GRADIENT) && (pc->flags & CH4G_LAGRANGE_ADDEDMASS)) types *= pc->beta_coeff_types;
-    if (pc->flags & CH4G_LAGRANGE_ADD_TRACER) types += 1.0;
-    const int nt = (int)types;
-    enum { NV = 23 };
+    const int nt = (int)ptypes(&rp->pcfg);
+    const unsigned fl = rp->pcfg.flags;
+    const int orient = (fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ORIENTATION);
+    enum { NV = 32 };
     double *acc = (double *)calloc((size_t)nt * NV, sizeof(double));
     double *cnt = (double *)calloc((size_t)nt, sizeof(double));
     if (!acc || !cnt) die("dump_particle_averages: out of memory");
@@ -399,6 +415,15 @@ void dump_particle_averages(void)
         o[20] += pow(p->t, 4.0);
         o[21] += p->dt_t;
         o[22] += p->dt_t * p->dt_t;
+        o[23] += p->dt_px;
+        o[24] += p->dt_py;
+        o[25] += p->dt_pz;
+        o[26] += p->dt_px * p->dt_px;
+        o[27] += p->dt_py * p->dt_py;
+        o[28] += p->dt_pz * p->dt_pz;
+        o[29] += pow(p->dt_px, 4.0);
+        o[30] += pow(p->dt_py, 4.0);
+        o[31] += pow(p->dt_pz, 4.0);
     }
     FILE *fo = fopen("particle_averages.dat", "a");
     if (fo) {
@@ -408,6 +433,8 @@ void dump_particle_averages(void)
             double *o = acc + (size_t)type * NV;
             fprintf(fo, "%e %e ", time_now, (double)type);
             for (int k = 0; k < 18; k++) fprintf(fo, "%e ", o[k] * norm);
+            if (orient) /* lagrange.c:2891-2896: dp/dt moments */
+                for (int k = 23; k < 32; k++) fprintf(fo, "%e ", o[k] * norm);
             if (rp->cfg.flags & CH4G_LB_TEMPERATURE)
                 for (int k = 18; k < 23; k++) fprintf(fo, "%e ", o[k] * norm);
             fprintf(fo, "\n");
@@ -596,12 +623,11 @@ void read_pop_h5(void)
  * compound of the point_particle members this lattice carries (name,
  * position, velocity, acceleration and their previous values, tau_drag, the
  * fluid velocity and its previous value; the velocity gradient with
- * LAGRANGE_GRADIENT and beta_coeff with added mass; t with LB_TEMPERATURE)
- * over [particle_number] records, the particles in array order first (the
- * reference sizes the dataset by particle_number and writes the npart it
- * holds); /properties carries particle_types.  The reference's interpolated
- * temperature gradient (dx_t, dy_t, dz_t) is not carried by the device
- * lattice and not written.                                                 */
+ * LAGRANGE_GRADIENT, then beta_coeff with added mass and the orientation
+ * members with LAGRANGE_ORIENTATION; t, and with the gradient dx_t..dz_t,
+ * with LB_TEMPERATURE) over [particle_number] records, the particles in
+ * array order first (the reference sizes the dataset by particle_number and
+ * writes the npart it holds); /properties carries particle_types.          */
 static hid_t particle_type(const ch4g_run_params *rp)
 {
     const unsigned fl = rp->pcfg.flags;
@@ -615,6 +641,14 @@ static hid_t particle_type(const ch4g_run_params *rp)
     if (fl & CH4G_LAGRANGE_GRADIENT) {
         M(dx_ux); M(dy_ux); M(dz_ux); M(dx_uy); M(dy_uy); M(dz_uy); M(dx_uz); M(dy_uz); M(dz_uz);
         if (fl & CH4G_LAGRANGE_ADDEDMASS) M(beta_coeff);
+        if (fl & CH4G_LAGRANGE_ORIENTATION) { /* lagrange.c:4365-4427 */
+            M(px); M(py); M(pz); M(dt_px); M(dt_py); M(dt_pz);
+            if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {
+                M(aspect_ratio);
+                if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) M(gyrotaxis_velocity);
+            }
+            if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) M(swim_velocity);
+        }
     }
     if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
         M(t);
@@ -624,15 +658,6 @@ static hid_t particle_type(const ch4g_run_params *rp)
     }
 #undef M
     return t;
-}
-
-static double particle_types(const ch4g_run_params *rp)
-{
-    const ch4g_particle_config *pc = &rp->pcfg;
-    double types = pc->tau_drag_types;
-    if ((pc->flags & CH4G_LAGRANGE_GRADIENT) && (pc->flags & CH4G_LAGRANGE_ADDEDMASS)) types *= pc->beta_coeff_types;
-    if (pc->flags & CH4G_LAGRANGE_ADD_TRACER) types += 1.0;
-    return types;
 }
 
 void write_point_particle_h5(void)
@@ -664,7 +689,7 @@ void write_point_particle_h5(void)
     hid_t pg = H5Gcreate2(file, "/properties", H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
     hid_t as = H5Screate(H5S_SCALAR);
     hid_t at = H5Acreate2(pg, "particle_types", H5T_NATIVE_DOUBLE, as, H5P_DEFAULT, H5P_DEFAULT);
-    const double types = particle_types(rp);
+    const double types = ptypes(&rp->pcfg);
     H5Awrite(at, H5T_NATIVE_DOUBLE, &types);
     H5Aclose(at);
     H5Sclose(as);
@@ -735,6 +760,15 @@ static int snapshot_members(const ch4g_run_params *rp, struct snap_member *m)
                 SM("Dt_ux", Dt_ux); SM("Dt_uy", Dt_uy); SM("Dt_uz", Dt_uz);
             }
         }
+        if (fl & CH4G_LAGRANGE_ORIENTATION) { /* lagrange.c:1889-2012 */
+            SM("px", px); SM("py", py); SM("pz", pz);
+            SM("dt_px", dt_px); SM("dt_py", dt_py); SM("dt_pz", dt_pz);
+            if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {
+                SM("aspect_ratio", aspect_ratio);
+                if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) SM("gyrotaxis_velocity", gyrotaxis_velocity);
+            }
+            if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) SM("swim_velocity", swim_velocity);
+        }
     }
     if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
         SM("temperature", t); SM("dt_t", dt_t);
@@ -794,7 +828,7 @@ static void write_particle_snapshot(void)
     hid_t fspace = H5Screate_simple(1, dims, NULL);
     hid_t mspace = H5Screate_simple(1, count, NULL);
     H5Sselect_hyperslab(fspace, H5S_SELECT_SET, offset, NULL, count, NULL);
-    struct snap_member m[40];
+    struct snap_member m[48];
     const int nm = snapshot_members(rp, m);
     for (int k = 0; k < nm; k++) {
         for (int64_t i = 0; i < n; i++) aux[i] = *(const double *)((const char *)&a[i] + m[k].off);
@@ -838,6 +872,17 @@ static void write_particle_snapshot(void)
             const char *u[3] = {"ux", "uy", "uz"}, *du[3] = {"Dt_ux", "Dt_uy", "Dt_uz"};
             xmf_joined(fo, h5, "fluid_velocity", "Vector", u, 3, np);
             if (am) xmf_joined(fo, h5, "fluid_acceleration", "Vector", du, 3, np);
+        }
+        if (fl & CH4G_LAGRANGE_ORIENTATION) { /* lagrange.c:2344-2440 */
+            const char *po[3] = {"px", "py", "pz"}, *dp[3] = {"dt_px", "dt_py", "dt_pz"};
+            xmf_joined(fo, h5, "orientation", "Vector", po, 3, np);
+            xmf_joined(fo, h5, "angular velocity", "Vector", dp, 3, np);
+            if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {
+                xmf_scalar(fo, h5, "aspect_ratio", "aspect_ratio", np);
+                if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS)
+                    xmf_scalar(fo, h5, "gyrotaxis_velocity", "gyrotaxis_velocity", np);
+            }
+            if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) xmf_scalar(fo, h5, "swim_velocity", "swim_velocity", np);
         }
     }
     if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {

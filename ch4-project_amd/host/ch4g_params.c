@@ -64,6 +64,11 @@ static const struct { const char *name; unsigned flag; } LFLAGS[] = {
     {"LAGRANGE_TWOWAY_SMALLTAUD", CH4G_LAGRANGE_TWOWAY_SMALLTAUD},
     {"LAGRANGE_TAUDRAG_INCREMENT_LOG", CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG},
     {"LAGRANGE_ADDEDMASS_INCREMENT_LOG", CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG},
+    {"LAGRANGE_ORIENTATION", CH4G_LAGRANGE_ORIENTATION},
+    {"LAGRANGE_ORIENTATION_JEFFREY", CH4G_LAGRANGE_ORIENTATION_JEFFREY},
+    {"LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG", CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG},
+    {"LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS", CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS},
+    {"LAGRANGE_ORIENTATION_ACTIVE", CH4G_LAGRANGE_ORIENTATION_ACTIVE},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -230,6 +235,13 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     if (!smalltaud) out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_SMALLTAUD_BETA | CH4G_LAGRANGE_SMALLTAUD_FLOATER);
     if (!twoway || !forcing_umbrella)
         out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_TWOWAY_MOMENTUM | CH4G_LAGRANGE_TWOWAY_SMALLTAUD);
+    /* the orientation variants sit inside #ifdef LAGRANGE_ORIENTATION, the
+       aspect-ratio ones inside LAGRANGE_ORIENTATION_JEFFREY (lagrange.c:3470-3752) */
+    if (!(out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION))
+        out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_ORIENTATION_JEFFREY | CH4G_LAGRANGE_ORIENTATION_ACTIVE);
+    if (!(out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_JEFFREY))
+        out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG |
+                                       CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
     if (!out->lagrange && out->pcfg.flags) return perr("%s", "LAGRANGE_* switches without LAGRANGE");
 
     static kv t[1024];
@@ -296,6 +308,25 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         if ((pc->flags & CH4G_LAGRANGE_GRADIENT) && (pc->flags & CH4G_LAGRANGE_ADDEDMASS)) {
             if (need(t, np, "beta_coeff_types", &pc->beta_coeff_types) ||
                 need(t, np, "beta_coeff_min", &pc->beta_coeff_min) || need(t, np, "beta_coeff_max", &pc->beta_coeff_max))
+                return -1;
+        }
+        /* parameters.c:621-695 */
+        const unsigned fl = pc->flags;
+        if ((fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ORIENTATION)) {
+            if ((fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) &&
+                (need(t, np, "aspect_ratio_types", &pc->aspect_ratio_types) ||
+                 need(t, np, "aspect_ratio_min", &pc->aspect_ratio_min) ||
+                 need(t, np, "aspect_ratio_max", &pc->aspect_ratio_max)))
+                return -1;
+            if ((fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) &&
+                (need(t, np, "gyrotaxis_velocity_types", &pc->gyrotaxis_velocity_types) ||
+                 need(t, np, "gyrotaxis_velocity_min", &pc->gyrotaxis_velocity_min) ||
+                 need(t, np, "gyrotaxis_velocity_max", &pc->gyrotaxis_velocity_max)))
+                return -1;
+            if ((fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) &&
+                (need(t, np, "swim_velocity_types", &pc->swim_velocity_types) ||
+                 need(t, np, "swim_velocity_min", &pc->swim_velocity_min) ||
+                 need(t, np, "swim_velocity_max", &pc->swim_velocity_max)))
                 return -1;
         }
     }

@@ -220,7 +220,12 @@ enum ch4g_lagrange_flag {
     CH4G_LAGRANGE_TWOWAY_MOMENTUM                     = 1u << 12, /* lagrange_feedback.c:11-256 */
     CH4G_LAGRANGE_TWOWAY_SMALLTAUD                    = 1u << 13, /* lagrange_feedback.c:106-110 */
     CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG               = 1u << 14, /* lagrange.c:271-297 */
-    CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG             = 1u << 15  /* lagrange.c:313-336 */
+    CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG             = 1u << 15, /* lagrange.c:313-336 */
+    CH4G_LAGRANGE_ORIENTATION                         = 1u << 16, /* orientation p, lagrange.c:877-901, :3470-3752 (needs GRADIENT) */
+    CH4G_LAGRANGE_ORIENTATION_JEFFREY                 = 1u << 17, /* Jeffery equation, lagrange.c:3490-3680 */
+    CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG   = 1u << 18, /* aspect_ratio ladder, lagrange.c:399-421 */
+    CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS       = 1u << 19, /* lagrange.c:445-470, :3533-3566 */
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE                  = 1u << 20  /* tracers swim along p, lagrange.c:502-521, :3179-3186 */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -230,6 +235,9 @@ typedef struct ch4g_particle_config {
     double   tau_drag_types, tau_drag_min, tau_drag_max;
     double   beta_coeff_types, beta_coeff_min, beta_coeff_max; /* read with LAGRANGE_ADDEDMASS */
     double   time_dt_lagr_factor;                              /* sub-steps per lattice step */
+    double   aspect_ratio_types, aspect_ratio_min, aspect_ratio_max;                   /* ORIENTATION_JEFFREY */
+    double   gyrotaxis_velocity_types, gyrotaxis_velocity_min, gyrotaxis_velocity_max; /* ..._GYROTAXIS */
+    double   swim_velocity_types, swim_velocity_min, swim_velocity_max;                /* ORIENTATION_ACTIVE */
 } ch4g_particle_config;
 
 /* One particle: the members of the reference's point_particle
@@ -245,13 +253,16 @@ typedef struct ch4g_particle {
     double t, t_old, dt_t;
     double tau_drag, beta_coeff;
     double dx_t, dy_t, dz_t;   /* temperature gradient (LB_TEMPERATURE with LAGRANGE_GRADIENT) */
+    double px, py, pz, dt_px, dt_py, dt_pz;             /* orientation and its last dp/dt (LAGRANGE_ORIENTATION) */
+    double aspect_ratio, gyrotaxis_velocity, swim_velocity;
 } ch4g_particle;
 
 /* allocate_particles (lagrange.c:6-49) + initial_conditions_particles
  * without restart (lagrange.c:72-1021): type ladders, name % types, positions
  * drawn x, y, z per particle from the lattice's drand48 stream (which has
  * already served initial_conditions and the first build_forcing, as in
- * main.c:23-51), zero velocity.  Call after ch4g_initial_conditions (or
+ * main.c:23-51), then with LAGRANGE_ORIENTATION the particle's random unit
+ * vector p (random_vector, random.c:139-150: two more draws), zero velocity.  Call after ch4g_initial_conditions (or
  * ch4g_set_populations) and before the first step.                          */
 int ch4g_particles_init(ch4g_lattice *lat, const ch4g_particle_config *pcfg);
 

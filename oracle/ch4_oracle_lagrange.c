@@ -111,15 +111,24 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     for (int k = 0; k < L->GZ; k++) L->cz[k] = (double)(k - B2) + 0.5;
     L->dt = 1.0 / pc->time_dt_lagr_factor;
 
-    /* particle_types (parameters.c:559-584, :743-755) */
+    /* particle_types (parameters.c:559-584, :621-695, :743-755) */
     const int am = LHAS(L, CH4G_LAGRANGE_GRADIENT) && LHAS(L, CH4G_LAGRANGE_ADDEDMASS);
+    const int orient = LHAS(L, CH4G_LAGRANGE_ORIENTATION);
+    const int jeff = orient && LHAS(L, CH4G_LAGRANGE_ORIENTATION_JEFFREY);
+    const int gyro = jeff && LHAS(L, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
+    const int swim = orient && LHAS(L, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
     double types = pc->tau_drag_types;
     if (am) types *= pc->beta_coeff_types;
+    if (jeff) types *= pc->aspect_ratio_types;
+    if (gyro) types *= pc->gyrotaxis_velocity_types;
+    if (swim) types *= pc->swim_velocity_types;
     int ntr = LHAS(L, CH4G_LAGRANGE_ADD_TRACER) ? 1 : 0;
     types += ntr;
     const int nt = (int)types;
     double *tau = (double *)calloc(nt, sizeof(double)), *beta = (double *)calloc(nt, sizeof(double));
-    if (!tau || !beta) return -1;
+    double *asp = (double *)calloc(nt, sizeof(double)), *gyv = (double *)calloc(nt, sizeof(double));
+    double *swv = (double *)calloc(nt, sizeof(double));
+    if (!tau || !beta || !asp || !gyv || !swv) return -1;
     double rep = 1.0;
     ladder(tau, nt, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
            LHAS(L, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
@@ -128,6 +137,21 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
         ladder(beta, nt, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
                LHAS(L, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
         rep *= pc->beta_coeff_types;
+    }
+    /* lagrange.c:389-423 (linear or geometric), :445-470 (geometric), :502-521 (linear) */
+    if (jeff) {
+        ladder(asp, nt, ntr, 1.0, pc->aspect_ratio_types, pc->aspect_ratio_min, pc->aspect_ratio_max,
+               LHAS(L, CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG), rep);
+        rep *= pc->aspect_ratio_types;
+    }
+    if (gyro) {
+        ladder(gyv, nt, ntr, 0.0, pc->gyrotaxis_velocity_types, pc->gyrotaxis_velocity_min,
+               pc->gyrotaxis_velocity_max, 1, rep);
+        rep *= pc->gyrotaxis_velocity_types;
+    }
+    if (swim) {
+        ladder(swv, nt, ntr, 0.0, pc->swim_velocity_types, pc->swim_velocity_min, pc->swim_velocity_max, 0, rep);
+        rep *= pc->swim_velocity_types;
     }
 
     L->npart = pc->particle_number;
@@ -146,9 +170,23 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
         p->z = 0 + oracle_drand48(o) * LNZ;
         p->vx = p->vy = p->vz = 0.0;
         p->t = p->t_old = p->dt_t = 0.0;
+        if (jeff) p->aspect_ratio = asp[type];
+        if (gyro) p->gyrotaxis_velocity = gyv[type];
+        if (swim) p->swim_velocity = swv[type];
+        if (orient) { /* random_vector (random.c:139-150), lagrange.c:891-901 */
+            double theta = acos(2.0 * oracle_drand48(o) - 1.0);
+            double phi = 2 * 3.14159265359 * oracle_drand48(o);
+            p->px = sin(theta) * sin(phi);
+            p->py = sin(theta) * cos(phi);
+            p->pz = cos(theta);
+            p->dt_px = p->dt_py = p->dt_pz = 0.0;
+        }
     }
     free(tau);
     free(beta);
+    free(asp);
+    free(gyv);
+    free(swv);
     return 0;
 }
 
@@ -533,11 +571,76 @@ static void interpolate(oracle *o)
 
 /* move_particles (lagrange.c:2934-4025) for the switches of ch4g.h, then
  * sendrecv_particles (lagrange.c:4028-4221) on one rank.                   */
-static void move(oracle *o)
+/* Orientation update of move_particles (lagrange.c:3470-3752) with
+ * LAGRANGE_ORIENTATION_JEFFREY (+ _GYROTAXIS): Jeffery's equation for the
+ * axis p of a spheroid of aspect ratio alpha,
+ *   dp/dt = W p + f S p - (p . f S p) p,   f = (alpha^2 - 1) / (alpha^2 + 1),
+ * S and W the symmetric and antisymmetric parts of A[i][j] = d u_i / d x_j,
+ * Euler on the first step, Adams-Bashforth 2 after; p renormalised. */
+static void orient_step(const olagr *L, ch4g_particle *p, int first, double gx, double gy, double gz)
+{
+    double P[3] = {p->px, p->py, p->pz};
+    double Fold[3] = {p->dt_px, p->dt_py, p->dt_pz};
+    if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_JEFFREY)) {
+        const double al = p->aspect_ratio;
+        const double fa = (al * al - 1.0) / (1.0 + al * al);
+        const double A[3][3] = {{p->dx_ux, p->dy_ux, p->dz_ux}, {p->dx_uy, p->dy_uy, p->dz_uy},
+                                {p->dx_uz, p->dy_uz, p->dz_uz}};
+        double S[3][3], W[3][3], F[3], T[3] = {0, 0, 0};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                S[i][j] = 0.5 * (A[i][j] + A[j][i]);
+                W[i][j] = 0.5 * (A[i][j] - A[j][i]);
+            }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) S[i][j] *= fa;
+        if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) && p->gyrotaxis_velocity != 0) {
+            /* lagrange.c:3533-3566: the first column of S takes a_i p_0; the
+             * six other products land in A, which is not read again */
+            const double g = -0.5 / p->gyrotaxis_velocity;
+            double a[3];
+            if (LHAS(L, CH4G_LAGRANGE_GRAVITY)) {
+                a[0] = g * (-gx + p->ax);
+                a[1] = g * (-gy + p->ay);
+                a[2] = g * (-gz + p->az);
+            } else {
+                a[0] = g * 0.0;
+                a[1] = g * 0.0;
+                a[2] = g * 1.0;
+            }
+            S[0][0] += a[0] * P[0];
+            S[1][0] += a[1] * P[0];
+            S[2][0] += a[2] * P[0];
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) T[i] += S[i][j] * P[j];
+        double oso = 0.0;
+        for (int i = 0; i < 3; i++) oso += P[i] * T[i];
+        for (int i = 0; i < 3; i++) {
+            F[i] = 0.0;
+            for (int j = 0; j < 3; j++) F[i] += W[i][j] * P[j] + (S[i][j] * P[j]);
+            F[i] -= P[i] * oso;
+        }
+        for (int i = 0; i < 3; i++) {
+            P[i] = first ? P[i] + L->dt * F[i] : P[i] + 0.5 * L->dt * (3. * F[i] - Fold[i]);
+            Fold[i] = F[i];
+        }
+    }
+    double norm = 0.0;
+    for (int i = 0; i < 3; i++) norm += P[i] * P[i];
+    for (int i = 0; i < 3; i++) P[i] /= sqrt(norm);
+    p->px = P[0];
+    p->py = P[1];
+    p->pz = P[2];
+    p->dt_px = Fold[0];
+    p->dt_py = Fold[1];
+    p->dt_pz = Fold[2];
+}
+
+static void move_at(oracle *o, int first)
 {
     olagr *L = o->lagr;
     const double dt = L->dt;
-    const int first = (o->itime == 1); /* itime == 1 && resume == 0 */
     const double gx = o->cfg.gravity_x, gy = o->cfg.gravity_y, gz = o->cfg.gravity_z;
     const double SX = o->NX, SY = o->NY, SZ = o->NZ;
     for (int ip = 0; ip < L->npart; ip++) {
@@ -554,6 +657,12 @@ static void move(oracle *o)
             p->vx = p->ux;
             p->vy = p->uy;
             p->vz = p->uz;
+            if (LHAS(L, CH4G_LAGRANGE_ORIENTATION) && LHAS(L, CH4G_LAGRANGE_ORIENTATION_ACTIVE)) {
+                /* swimming along p (lagrange.c:3179-3186) */
+                p->vx += p->swim_velocity * p->px;
+                p->vy += p->swim_velocity * p->py;
+                p->vz += p->swim_velocity * p->pz;
+            }
             if (first) {
                 p->vx_old = p->vx;
                 p->vy_old = p->vy;
@@ -661,6 +770,7 @@ static void move(oracle *o)
                 p->vz += f1 * (-gz - p->Dt_uz);
             }
         }
+        if (LHAS(L, CH4G_LAGRANGE_ORIENTATION)) orient_step(L, p, first, gx, gy, gz);
         /* wall rules (lagrange.c:3754-3993), LB_FLUID_BC_Y only */
         if (HAS(o, CH4G_FLUID_BC_Y)) {
             const int inel = LHAS(L, CH4G_LAGRANGE_BC_INELASTIC);
@@ -742,6 +852,8 @@ static void move(oracle *o)
     }
     L->npart = kept;
 }
+
+static void move(oracle *o) { move_at(o, o->itime == 1); /* itime == 1 && resume == 0 */ }
 
 void olagr_after_step(oracle *o)
 {
@@ -845,5 +957,14 @@ int oracle_particles_probe(oracle *o, const double *u, const double *t)
             }
     bc_hydro(o);
     interpolate(o);
+    return 0;
+}
+
+/* Test hook: one particle sub-step (interpolate + move_particles) on given
+ * interior fields u, t held fixed, Euler start when `first`. */
+int oracle_particles_move(oracle *o, const double *u, const double *t, int first)
+{
+    if (oracle_particles_probe(o, u, t)) return -1;
+    move_at(o, first);
     return 0;
 }

@@ -52,6 +52,7 @@ def lib():
         L.oracle_get_particles.restype = C.c_long
         L.oracle_set_particles.argtypes = [P, P, C.c_long]
         L.oracle_particles_probe.argtypes = [P, dp, dp]
+        L.oracle_particles_move.argtypes = [P, dp, dp, C.c_int]
         _lib = L
     return _lib
 
@@ -141,6 +142,13 @@ class Oracle:
         u = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
         t = None if t is None else np.ascontiguousarray(t, dtype=np.float64)
         self._L.oracle_particles_probe(self._h, _ptr(u), _ptr(t))
+        return self.particles()
+
+    def move(self, u=None, t=None, first=False):
+        """One particle sub-step on given interior fields held fixed (no fluid step)."""
+        u = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        t = None if t is None else np.ascontiguousarray(t, dtype=np.float64)
+        self._L.oracle_particles_move(self._h, _ptr(u), _ptr(t), int(first))
         return self.particles()
 
     def dump_averages(self):

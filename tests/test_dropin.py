@@ -74,7 +74,26 @@ def test_read_run_params_particles():
         open(d, "w").write(text)
         assert lib.ch4g_read_run_params(d.encode(), os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == 0
         assert not rp.pcfg.flags & abi.LAGRANGE_TWOWAY_MOMENTUM
-        open(d, "w").write(text + "#define LAGRANGE_ORIENTATION\n")
+        # orientation: switches nested as in the reference, ladders read from param.in
+        open(d, "w").write(text + "#define LAGRANGE_ORIENTATION\n#define LAGRANGE_ORIENTATION_JEFFREY\n"
+                           "#define LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS\n#define LAGRANGE_ORIENTATION_ACTIVE\n")
+        assert lib.ch4g_read_run_params(d.encode(), os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == -1
+        pin = os.path.join(td, "param.in")
+        open(pin, "w").write(open(os.path.join(FIXP, "param.in")).read() +
+                             "aspect_ratio_types 3\naspect_ratio_min 0.5\naspect_ratio_max 4\n"
+                             "gyrotaxis_velocity_types 1\ngyrotaxis_velocity_min 2\ngyrotaxis_velocity_max 2\n"
+                             "swim_velocity_types 2\nswim_velocity_min 0\nswim_velocity_max 0.01\n")
+        assert lib.ch4g_read_run_params(d.encode(), pin.encode(), C.byref(rp)) == 0
+        o = abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_JEFFREY | abi.LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS
+        assert rp.pcfg.flags & (o | abi.LAGRANGE_ORIENTATION_ACTIVE) == o | abi.LAGRANGE_ORIENTATION_ACTIVE
+        assert (rp.pcfg.aspect_ratio_types, rp.pcfg.aspect_ratio_max, rp.pcfg.gyrotaxis_velocity_min,
+                rp.pcfg.swim_velocity_max) == (3.0, 4.0, 2.0, 0.01)
+        # without LAGRANGE_ORIENTATION its variants have no effect
+        open(d, "w").write(text + "#define LAGRANGE_ORIENTATION_JEFFREY\n")
+        assert lib.ch4g_read_run_params(d.encode(), os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == 0
+        assert not rp.pcfg.flags & abi.LAGRANGE_ORIENTATION_JEFFREY
+        # the stochastic orientation variants are not implemented
+        open(d, "w").write(text + "#define LAGRANGE_ORIENTATION\n#define LAGRANGE_ORIENTATION_DIFFUSION\n")
         assert lib.ch4g_read_run_params(d.encode(), os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == -1
 
 
@@ -349,18 +368,27 @@ def read_dataset_h5(path, name, n):
     return buf
 
 
+ORIENT_DEFINES = ("#define LAGRANGE_ORIENTATION\n#define LAGRANGE_ORIENTATION_JEFFREY\n"
+                  "#define LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS\n#define LAGRANGE_ORIENTATION_ACTIVE\n")
+ORIENT_PARAMS = ("aspect_ratio_types 3\naspect_ratio_min 0.5\naspect_ratio_max 4\n"
+                 "gyrotaxis_velocity_types 1\ngyrotaxis_velocity_min 2\ngyrotaxis_velocity_max 2\n"
+                 "swim_velocity_types 2\nswim_velocity_min 0\nswim_velocity_max 0.001\n")
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(H5LIB), reason="serial HDF5 not installed")
-def test_particle_snapshots(tmp_path):
+@pytest.mark.parametrize("orient", [False, True])
+def test_particle_snapshots(tmp_path, orient):
     """OUTPUT_H5 with LAGRANGE: output_particles writes RUN/particle_<itime>.h5
     every time_dump_lagr (lagrange.c:1598-2595), one FP64 dataset per member
     under /lagrange in the reference's order, equal to the oracle's records
     (FP64 path: bit-identical), and an XDMF descriptor naming those
-    datasets."""
+    datasets; with the orientation switches also their members and the dp/dt
+    moments of particle_averages.dat."""
     import xml.etree.ElementTree as ET
     define = open(os.path.join(FIXP, "define.h")).read() + "#define OUTPUT_H5\n#define LAGRANGE_OUTPUT_FLUID\n"
-    (tmp_path / "define.h").write_text(define)
-    shutil.copy(os.path.join(FIXP, "param.in"), tmp_path / "param.in")
+    (tmp_path / "define.h").write_text(define + (ORIENT_DEFINES if orient else ""))
+    (tmp_path / "param.in").write_text(open(os.path.join(FIXP, "param.in")).read() + (ORIENT_PARAMS if orient else ""))
     r = subprocess.run([RUN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     for it in (0, 40, 80, 120):
@@ -380,6 +408,13 @@ def test_particle_snapshots(tmp_path):
     members = ["name", "tau_drag", "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "dx_ux", "dy_ux", "dz_ux",
                "dx_uy", "dy_uy", "dz_uy", "dx_uz", "dy_uz", "dz_uz", "beta_coeff", "ux", "uy", "uz", "Dt_ux", "Dt_uy",
                "Dt_uz", "temperature", "dt_t", "dx_t", "dy_t", "dz_t"]
+    if orient:
+        i = members.index("temperature")
+        members[i:i] = ["px", "py", "pz", "dt_px", "dt_py", "dt_pz", "aspect_ratio", "gyrotaxis_velocity",
+                        "swim_velocity"]
+        assert np.unique(ref["aspect_ratio"]).size == 4  # the ladder and the tracer's 1
+        rows = [l.split() for l in open(tmp_path / "particle_averages.dat")]
+        assert {len(r) for r in rows} == {2 + 18 + 9 + 5}
     path = tmp_path / "RUN" / "particle_120.h5"
     for m in members:
         got = read_dataset_h5(path, "/lagrange/" + m, npart)

@@ -132,6 +132,36 @@ def test_twoway_smalltaud_feedback_fp64():
     compare_fluid(o, lat)
 
 
+ORIENT = (abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_WRAP | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ORIENTATION |
+          abi.LAGRANGE_ORIENTATION_JEFFREY | abi.LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG |
+          abi.LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS | abi.LAGRANGE_ORIENTATION_ACTIVE)
+
+
+@pytest.mark.parametrize("case", ["hit", "rb_gravity"])
+def test_orientation_fp64(case):
+    """Jeffery orientation with aspect-ratio, gyrotaxis and swimming ladders:
+    p, dp/dt and the particles they move bit-identical to the oracle."""
+    if case == "hit":
+        cfg, flags = configs.hit(16, tau=0.51, amp=1e-4), ORIENT
+    else:
+        cfg, flags = rb_small(), ORIENT | abi.LAGRANGE_GRAVITY | abi.LAGRANGE_ADDEDMASS
+    pc = abi.make_particle_config(flags, 300, tau_drag=(2, 0.0, 4.0), beta_coeff=(1, 0.8, 0.8),
+                                  aspect_ratio=(3, 0.5, 8.0), gyrotaxis_velocity=(2, 0.5, 2.0),
+                                  swim_velocity=(2, 0.0, 0.002), time_dt_lagr_factor=2)
+    o, lat = pair(cfg, pc, 100)
+    a = lat.particles()
+    assert np.unique(a["aspect_ratio"]).size == 4 and np.abs(a["dt_px"]).max() > 0
+    compare_particles(o, lat)
+
+
+def test_orientation_needs_the_gradient():
+    lat = Lattice(rb_small())
+    lat.initial_conditions()
+    pc = abi.make_particle_config(abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_JEFFREY, 4)
+    with pytest.raises(Exception, match="LAGRANGE_GRADIENT"):
+        lat.particles_init(pc)
+
+
 def test_substeps_fp64():
     pc = abi.make_particle_config(HEAVY, 128, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.7, 0.7),
                                   time_dt_lagr_factor=3)

@@ -21,6 +21,11 @@ a wrong ghost sign, a missing factor, an off-by-one cell) fails it:
 * walls and re-injection          <- mirror / stop rules; glibc drand48 draws
 * two-way feedback                <- deposits sum to V_p f_p (weights sum to
                                      one) with f_p in closed form
+* orientation                     <- ladders and random_vector draws from the
+                                     glibc stream; Jeffery orbits in simple
+                                     shear (closed form in the shear plane,
+                                     orbit constant in 3-D); the gyrotaxis
+                                     term by hand; swimming in a straight line
 """
 import ctypes
 import ctypes.util
@@ -335,3 +340,129 @@ def test_twoway_feedback_deposit():
     # trilinear weights: the node at (3,4,2) (centre 3.5,4.5,2.5) gets (1-0.2)(1-0.1)(1-0.3)
     assert F[2, 4, 3, 1] == pytest.approx(vol * fpy * 0.8 * 0.9 * 0.7, rel=1e-12)
     assert F[..., 0].sum() == pytest.approx(0.0, abs=1e-30)
+
+
+# --------------------------------------------------------------------------
+# orientation (lagrange.c:3470-3752)
+ORIENT = abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_JEFFREY
+
+
+def _shear(cfg, gamma):
+    """u_x = gamma (y - NY/2): uniform velocity gradient d u_x / d y = gamma."""
+    x, y, z = _centres(cfg)
+    u = np.zeros(x.shape + (3,))
+    u[..., 0] = gamma * (y - cfg.NY / 2)
+    return u
+
+
+def test_orientation_ladders_and_initial_draws():
+    seed = 41
+    o, cfg = quiescent(8, 8, 4, seed=seed)
+    flags = (abi.LAGRANGE_ADD_TRACER | ORIENT | abi.LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS |
+             abi.LAGRANGE_ORIENTATION_ACTIVE)
+    pc = abi.make_particle_config(flags, 40, tau_drag=(2, 0.0, 1.0), aspect_ratio=(3, 1.0, 5.0),
+                                  gyrotaxis_velocity=(2, 0.5, 2.0), swim_velocity=(2, 0.0, 0.01))
+    o.particles_init(pc)
+    a = o.particles()
+    nt = 2 * 3 * 2 * 2 + 1
+    for r in a:
+        t = int(r["name"]) % nt
+        if t == 0:
+            assert (r["tau_drag"], r["aspect_ratio"], r["gyrotaxis_velocity"], r["swim_velocity"]) == (0, 1, 0, 0)
+            continue
+        h = t - 1  # tau varies fastest, then aspect ratio, gyrotaxis, swimming
+        assert r["tau_drag"] == [0.0, 1.0][h % 2]
+        assert r["aspect_ratio"] == [1.0, 3.0, 5.0][(h // 2) % 3]
+        assert r["gyrotaxis_velocity"] == pytest.approx([0.5, 2.0][(h // 6) % 2], rel=1e-15)
+        assert r["swim_velocity"] == [0.0, 0.01][(h // 12) % 2]
+    # per particle: x, y, z, then random_vector's theta and phi
+    d = glibc_stream(seed, 5 * 40).reshape(40, 5)
+    assert np.array_equal(a["x"], d[:, 0] * 8)
+    theta, phi = np.arccos(2.0 * d[:, 3] - 1.0), 2 * 3.14159265359 * d[:, 4]
+    assert np.allclose(a["px"], np.sin(theta) * np.sin(phi), rtol=0, atol=1e-15)  # numpy vs glibc sin: 1 ulp
+    assert np.allclose(a["pz"], np.cos(theta), rtol=0, atol=1e-15)
+    assert np.allclose(a["px"] ** 2 + a["py"] ** 2 + a["pz"] ** 2, 1.0, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("lam", [1.0, 3.0, 0.25])
+def test_jeffery_orbit_in_plane_matches_the_closed_form(lam):
+    """In u_x = gamma y, p = (cos th, sin th, 0) obeys
+    dth/dt = -gamma (cos^2 th + lam^2 sin^2 th) / (lam^2 + 1), solved by
+    p ~ (lam cos psi, sin psi), psi = psi0 - gamma lam t / (lam^2 + 1)
+    (lam = 1: rigid rotation with the vorticity, -gamma/2)."""
+    gamma, n = 0.01, 600
+    o, cfg = quiescent(8, 16, 4)
+    o.particles_init(abi.make_particle_config(ORIENT, 1, tau_drag=(1, 0.0, 0.0), aspect_ratio=(1, lam, lam)))
+    th0 = np.linspace(0.1, 3.0, 5)
+    a = records(len(th0))
+    a["name"] = np.arange(len(th0))
+    a["x"], a["y"], a["z"] = 4.0, 8.0, 2.0
+    a["px"], a["py"] = np.cos(th0), np.sin(th0)
+    a["aspect_ratio"] = lam
+    o.set_particles(a)
+    u = _shear(cfg, gamma)
+    for s in range(n):
+        r = o.move(u, first=(s == 0))
+    assert np.allclose(r["dy_ux"], gamma, rtol=1e-12)
+    w = gamma * lam / (lam * lam + 1)
+    psi = np.arctan2(lam * np.sin(th0), np.cos(th0)) - w * n
+    e = np.stack([lam * np.cos(psi), np.sin(psi)], -1)
+    e /= np.linalg.norm(e, axis=1, keepdims=True)
+    got = np.stack([r["px"], r["py"]], -1)
+    # AB2 after an Euler step: observed <= 6e-5 over these 600 steps; a
+    # dropped -(p.S'p)p term gives 4e-3, a transposed gradient 0.3-2
+    assert np.abs(got - e).max() < 1e-4
+    assert np.all(r["pz"] == 0.0)
+    # the last dp/dt is the right-hand side at the previous p: its length is
+    # |dth/dt| there (one step, an angle of < 0.01, away)
+    rate = gamma * (r["px"] ** 2 + lam * lam * r["py"] ** 2) / (lam * lam + 1)
+    assert np.allclose(np.hypot(r["dt_px"], r["dt_py"]), rate, rtol=2e-2)
+
+
+def test_jeffery_orbit_constant_in_three_dimensions():
+    """(px^2 + lam^2 py^2) / pz^2 is a constant of Jeffery's equation in
+    u_x = gamma y (the orbit constant)."""
+    gamma, lam, n = 0.02, 4.0, 400
+    o, cfg = quiescent(8, 16, 4, seed=3)
+    o.particles_init(abi.make_particle_config(ORIENT, 6, tau_drag=(1, 0.0, 0.0), aspect_ratio=(1, lam, lam)))
+    a = o.particles()
+    a["x"], a["y"], a["z"] = 4.0, 8.0, 2.0
+    o.set_particles(a)
+    c0 = (a["px"] ** 2 + lam ** 2 * a["py"] ** 2) / a["pz"] ** 2
+    u = _shear(cfg, gamma)
+    for s in range(n):
+        r = o.move(u, first=(s == 0))
+    c1 = (r["px"] ** 2 + lam ** 2 * r["py"] ** 2) / r["pz"] ** 2
+    assert np.allclose(c1, c0, rtol=2e-4)
+    assert np.allclose(r["px"] ** 2 + r["py"] ** 2 + r["pz"] ** 2, 1.0, atol=1e-15)
+    assert not np.allclose(np.stack([r["px"], r["py"]]), np.stack([a["px"], a["py"]]), atol=1e-2)
+
+
+def test_gyrotaxis_first_column_and_swimming():
+    # no flow, no gravity switch: a = -1/(2 v0) (0, 0, 1), added to the first
+    # column of S only (lagrange.c:3553-3562).  By hand for p = (0.6, 0, 0.8),
+    # v0 = 2: S[2][0] = -0.15, S p = (0, 0, -0.09), p.S p = -0.072,
+    # F = S p + 0.072 p = (0.0432, 0, -0.0324); Euler: p + F, normalised.
+    o, cfg = quiescent(6, 6, 6)
+    flags = ORIENT | abi.LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS
+    o.particles_init(abi.make_particle_config(flags, 1, tau_drag=(1, 0.0, 0.0), gyrotaxis_velocity=(1, 2.0, 2.0)))
+    a = records(1)
+    a["x"], a["y"], a["z"] = 3.0, 3.0, 3.0
+    a["px"], a["pz"], a["aspect_ratio"], a["gyrotaxis_velocity"] = 0.6, 0.8, 3.0, 2.0
+    o.set_particles(a)
+    r = o.move(np.zeros((6, 6, 6, 3)), first=True)[0]
+    q = np.array([0.6 + 0.0432, 0.0, 0.8 - 0.0324])
+    q /= np.linalg.norm(q)
+    assert np.allclose([r["px"], r["py"], r["pz"]], q, rtol=1e-14, atol=1e-16)
+    assert np.allclose([r["dt_px"], r["dt_py"], r["dt_pz"]], [0.0432, 0.0, -0.0324], rtol=1e-13)
+    # active tracer in still fluid: straight line at swim_velocity along p
+    o, cfg = quiescent(16, 8, 8)
+    flags = abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_ACTIVE | abi.LAGRANGE_WRAP
+    o.particles_init(abi.make_particle_config(flags, 4, tau_drag=(1, 0.0, 0.0), swim_velocity=(1, 0.01, 0.01)))
+    a0 = o.particles()
+    o.step(30)
+    a1 = o.particles()
+    for c, pc, S in (("x", "px", 16), ("y", "py", 8), ("z", "pz", 8)):
+        d = np.abs(a1[c] - np.mod(a0[c] + 30 * 0.01 * a0[pc], S))
+        assert np.all(np.minimum(d, S - d) < 1e-12)
+    assert np.array_equal(a1["px"], a0["px"]) or np.allclose(a1["px"], a0["px"], rtol=0, atol=2e-16)
