@@ -197,6 +197,11 @@ int ensure_stage(ch4g_lattice *L)
 void gather(ch4g_lattice *L, double *fpost, double *gpost, double *dens, double *u, double *t)
 {
     KParams P = params_for(L, L->first, (int)L->itime);
+    if (L->d_f0) {
+        // the mean force of the last step (its hydro fields carry its half force)
+        launch_force_mean64(P, false, nullptr, (const double *)L->tlag, L->d_f0 + 6, L->d_f0, L->stream);
+        P.f0n = L->d_f0 + 3;
+    }
     if (L->dev32)
         launch_gather_fp32(P, (const float *)L->f[L->cur], (const float *)L->g[L->cur], (const float *)L->tlag, fpost,
                            gpost, dens, u, t, L->stream);
@@ -251,6 +256,13 @@ int launch_one_step(ch4g_lattice *L)
         P.tf_out = L->base.thermal ? L->d_t32 : nullptr;
     }
     int in = L->cur, out = 1 - L->cur;
+    if (L->d_f0) {
+        // NOZEROMODE: mean force of the previous step and of this one first
+        launch_force_mean64(P, true, (const double *)L->g[in], (const double *)L->tlag, L->d_f0 + 6, L->d_f0,
+                            L->stream);
+        P.f0p = L->d_f0;
+        P.f0n = L->d_f0 + 3;
+    }
     if (P.les_fd) {
         // u of this step at every node first: the closure differentiates it
         if (L->dev32)
@@ -401,8 +413,8 @@ int validate(const ch4g_config *c)
     if (!has(*c, CH4G_LB_FLUID) && !has(*c, CH4G_LB_TEMPERATURE)) return fail(-1, "neither LB_FLUID nor LB_TEMPERATURE");
     if (c->precision != CH4G_FP64 && c->precision != CH4G_FP32_DEV && c->precision != CH4G_FP32_STORE)
         return fail(-1, "unknown precision %d", c->precision);
-    if (has(*c, CH4G_FLUID_FORCING_NOZEROMODE))
-        return fail(-1, "LB_FLUID_FORCING_NOZEROMODE is not supported on the device path");
+    if (has(*c, CH4G_FLUID_FORCING_NOZEROMODE) && c->precision != CH4G_FP64)
+        return fail(-1, "LB_FLUID_FORCING_NOZEROMODE is implemented on the FP64 path only");
     if (has(*c, CH4G_TEMPERATURE_BUOYANCY) && !has(*c, CH4G_LB_TEMPERATURE))
         return fail(-1, "LB_TEMPERATURE_BUOYANCY requires LB_TEMPERATURE");
     if (forced(*c) && !has(*c, CH4G_LB_FLUID)) return fail(-1, "fluid forcing requires LB_FLUID");
@@ -580,6 +592,8 @@ int ch4g_create(const ch4g_config *cfg, ch4g_lattice **out)
     L->xsubi[1] = (unsigned short)((uint32_t)cfg->seed & 0xFFFFu);
     L->xsubi[2] = (unsigned short)(((uint32_t)cfg->seed >> 16) & 0xFFFFu);
     memset(L->phi, 0, sizeof L->phi);
+    if (has(*cfg, CH4G_FLUID_FORCING_NOZEROMODE))
+        if ((rc = dalloc(L, (void **)&L->d_f0, (size_t)(6 + 512 * 6) * sizeof(double)))) goto err;
     if (L->base.hit) {
         L->hit_len = (size_t)CH4G_HIT_NK * (cfg->NX + cfg->NY + cfg->NZ);
         for (int b = 0; b < 2; b++)
@@ -614,6 +628,7 @@ void ch4g_destroy(ch4g_lattice *L)
         if (L->g[b]) (void)hipFree(L->g[b]);
         if (L->d_hit[b]) (void)hipFree(L->d_hit[b]);
     }
+    if (L->d_f0) (void)hipFree(L->d_f0);
     for (int r = 0; r < HIT_RING; r++) {
         if (L->h_ring[r]) (void)hipHostFree(L->h_ring[r]);
         if (L->ev_ring[r]) (void)hipEventDestroy(L->ev_ring[r]);

@@ -68,6 +68,10 @@ struct KParams {
     int hit;
     double hit_fa[3][CH4G_HIT_NK];  // pow(k^2,-1/3) * Amp_{x,y,z}
     HitTable hit_prev, hit_now;
+    // LB_FLUID_FORCING_NOZEROMODE (FP64 path): the mean of the built force of
+    // the previous (f0p) and of this step (f0n), removed at every node; null
+    // without the switch
+    const double *f0p, *f0n;
     int buoy;
     double T0b, beta_t, beta2_t, grav[3];
     double Tw_top, Tw_bot;  // (T_wall - T_ref)
@@ -321,6 +325,9 @@ void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin,
 // closure reads it at the six neighbours); same gather and moments as the
 // step kernel.
 void launch_velocity_fp64(const KParams &P, const double *fin, const double *tlag, hipStream_t s);
+// LB_FLUID_FORCING_NOZEROMODE means into f0[6] (scratch part: 512 * 6 doubles)
+void launch_force_mean64(const KParams &P, bool step, const double *gin, const double *tlag, double *part, double *f0,
+                         hipStream_t s);
 void launch_velocity_fp32(const KParams &P, const float *fin, const float *tlag, hipStream_t s);
 
 // Initial state from a per-row fluid profile prow[NY][19] and a temperature

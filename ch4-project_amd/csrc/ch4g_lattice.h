@@ -64,6 +64,8 @@ struct ch4g_lattice {
     double phi[CH4G_HIT_NK][3];
     double vk[CH4G_HIT_NK][3];
     double *d_hit[2] = {nullptr, nullptr};
+    // LB_FLUID_FORCING_NOZEROMODE: means [6] + block partial sums [512][6]
+    double *d_f0 = nullptr;
     double *h_ring[ch4g::HIT_RING] = {};
     hipEvent_t ev_ring[ch4g::HIT_RING] = {};
     int ring = 0;

@@ -97,6 +97,7 @@ int check_member(const ch4g_lattice *L)
     if (L->base.hit) return fail(-1, "HIT-forced lattices need per-step host tables and are not batched");
     if (L->part) return fail(-1, "lattices with particles are not batched");
     if (L->base.les_fd) return fail(-1, "finite-difference LES lattices need a velocity pass and are not batched");
+    if (L->d_f0) return fail(-1, "NOZEROMODE lattices need a force-mean pass and are not batched");
     return 0;
 }
 
@@ -451,9 +452,9 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
             ch4g_lattice *L = nullptr;
             int r = ch4g_create(&J.cfg, &L);
             if (!r) r = ch4g_initial_conditions(L);
-            if (!r && batched && (L->base.hit || L->base.les_fd))
-                r = fail(-1, "HIT-forced and finite-difference LES jobs are not batched (serve them with "
-                             "CH4G_SERVE_SEQUENTIAL)");
+            if (!r && batched && (L->base.hit || L->base.les_fd || L->d_f0))
+                r = fail(-1, "HIT-forced, finite-difference LES and NOZEROMODE jobs are not batched (serve them "
+                             "with CH4G_SERVE_SEQUENTIAL)");
             if (!r && J.nsteps > 0) r = ch4g_step(L, 1); // first step: reads the initial populations
             if (r) {
                 if (L) ch4g_destroy(L);

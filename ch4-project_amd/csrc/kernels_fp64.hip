@@ -36,9 +36,11 @@ __device__ __forceinline__ double m_z(const double *p)
     return p[5] - p[6] + p[11] + p[12] - p[13] - p[14] + p[15] - p[16] + p[17] - p[18];
 }
 
-// build_forcing for one node (forcing.c:361-667), terms added in source order.
-__device__ __forceinline__ void force_at(const KParams &P, const HitTable &tab, const double *fb, uint32_t n, int x,
-                                         int y, int z, double T, double F[3])
+// build_forcing for one node (forcing.c:361-667), terms added in source order;
+// with NOZEROMODE the mean of those terms is removed (forcing.c:1123-1152)
+// before the particle feedback is added (main.c:118-131).
+__device__ __forceinline__ void force_at(const KParams &P, const HitTable &tab, const double *fb, const double *f0,
+                                         uint32_t n, int x, int y, int z, double T, double F[3])
 {
     F[0] = P.Fc[0];
     F[1] = P.Fc[1];
@@ -61,6 +63,11 @@ __device__ __forceinline__ void force_at(const KParams &P, const HitTable &tab, 
         F[0] += fac * P.grav[0];
         F[1] += fac * P.grav[1];
         F[2] += fac * P.grav[2];
+    }
+    if (f0) {
+        F[0] -= f0[0];
+        F[1] -= f0[1];
+        F[2] -= f0[2];
     }
     if (P.twoway) {
         const size_t N = (size_t)P.N;
@@ -207,7 +214,7 @@ __device__ __forceinline__ void fluid_velocity64(const KParams &P, const double 
     rho = m_sum(f);
     if (P.forced) {
         double Fp[3];
-        force_at(P, P.hit_prev, P.fb_prev, n, x, y, z, Tlag, Fp);
+        force_at(P, P.hit_prev, P.fb_prev, P.f0p, n, x, y, z, Tlag, Fp);
         ux = m_x(f) / rho + 0.5 * 1.0 * Fp[0];
         uy = m_y(f) / rho + 0.5 * 1.0 * Fp[1];
         uz = m_z(f) / rho + 0.5 * 1.0 * Fp[2];
@@ -256,7 +263,7 @@ __device__ __forceinline__ void step_node64(const KParams &P, const double *__re
             invtau = 1.0 / tau_les;
         }
         double Fn[3] = {0.0, 0.0, 0.0};
-        if (P.forced) force_at(P, P.hit_now, P.fb_now, n, x, y, z, T, Fn);
+        if (P.forced) force_at(P, P.hit_now, P.fb_now, P.f0n, n, x, y, z, T, Fn);
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             double r = f[q];
@@ -424,7 +431,7 @@ __global__ void __launch_bounds__(256) k_gather_fp64(KParams P, const double *__
         double ux, uy, uz;
         if (P.forced) {
             double F[3];
-            force_at(P, P.hit_now, P.fb_now, n, x, y, z, Tlag, F);
+            force_at(P, P.hit_now, P.fb_now, P.f0n, n, x, y, z, Tlag, F);
             ux = m_x(f) / rho + 0.5 * 1.0 * F[0];
             uy = m_y(f) / rho + 0.5 * 1.0 * F[1];
             uz = m_z(f) / rho + 0.5 * 1.0 * F[2];
@@ -502,6 +509,80 @@ void launch_batch_fp64(const BatchEntry *tab, const BatchSlot *slots, int nslots
     else
         CH4G_BATCH64(false, true);
 #undef CH4G_BATCH64
+}
+
+// LB_FLUID_FORCING_NOZEROMODE (forcing.c:1123-1152): sums over all nodes of
+// the built force (no particle feedback) of the previous step, F(Tlag) with
+// the previous HIT table, and of this step, F(T) with T gathered as the step
+// kernel gathers it (STEP), or of the last step alone, F(Tlag) with the
+// current table (the hydro-fields gather).  Fixed grid and a tree per block:
+// the partial sums are the same on every run.
+constexpr int FM_THREADS = 256, FM_BLOCKS = 512;
+
+template <bool STEP, bool FIRST>
+__global__ void __launch_bounds__(FM_THREADS) k_force_sum64(KParams P, const double *__restrict__ gin,
+                                                            const double *__restrict__ tlag, double *__restrict__ part)
+{
+    P.twoway = 0;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int64_t N = P.N, stride = (int64_t)gridDim.x * FM_THREADS;
+    for (int64_t i = (int64_t)blockIdx.x * FM_THREADS + threadIdx.x; i < N; i += stride) {
+        const uint32_t n = (uint32_t)i;
+        const int x = (int)(i % P.NX), y = (int)((i / P.NX) % P.NY), z = (int)(i / ((int64_t)P.NX * P.NY));
+        const double Tlag = P.thermal ? tlag[n] : 0.0;
+        double F[3];
+        if constexpr (STEP) {
+            force_at(P, P.hit_prev, nullptr, nullptr, n, x, y, z, Tlag, F);
+            for (int c = 0; c < 3; c++) acc[c] += F[c];
+            double T = 0.0;
+            if (P.thermal && P.buoy) {
+                double g[Q];
+                gather_thermal<FIRST>(P, make_nbr(x, y, z, P.NX, P.NY, P.NZ), gin, n, Tlag, g);
+                T = m_sum(g);
+            }
+            force_at(P, P.hit_now, nullptr, nullptr, n, x, y, z, T, F);
+        } else {
+            force_at(P, P.hit_now, nullptr, nullptr, n, x, y, z, Tlag, F);
+        }
+        for (int c = 0; c < 3; c++) acc[3 + c] += F[c];
+    }
+    __shared__ double red[6][FM_THREADS];
+    for (int c = 0; c < 6; c++) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int w = FM_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 6; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Block sums in block order, times 1/(NX NY NZ) as forcing.c:1140 forms it.
+__global__ void k_force_mean64(int nblocks, double NX, double NY, double NZ, const double *__restrict__ part,
+                               double *__restrict__ f0)
+{
+    const int c = threadIdx.x;
+    if (c >= 6) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += part[(size_t)b * 6 + c];
+    const double norm = 1.0 / (NX * NY * NZ);
+    f0[c] = s * norm;
+}
+
+// f0[0..2]: mean force of the previous step, f0[3..5]: of this step (STEP);
+// without STEP only f0[3..5] (the last step's).  part: FM_BLOCKS * 6 doubles.
+void launch_force_mean64(const KParams &P, bool step, const double *gin, const double *tlag, double *part, double *f0,
+                         hipStream_t s)
+{
+    const int64_t need = (P.N + FM_THREADS - 1) / FM_THREADS;
+    const int nb = (int)(need < FM_BLOCKS ? need : FM_BLOCKS);
+    if (!step)
+        hipLaunchKernelGGL((k_force_sum64<false, false>), dim3(nb), dim3(FM_THREADS), 0, s, P, gin, tlag, part);
+    else if (P.first)
+        hipLaunchKernelGGL((k_force_sum64<true, true>), dim3(nb), dim3(FM_THREADS), 0, s, P, gin, tlag, part);
+    else
+        hipLaunchKernelGGL((k_force_sum64<true, false>), dim3(nb), dim3(FM_THREADS), 0, s, P, gin, tlag, part);
+    hipLaunchKernelGGL(k_force_mean64, dim3(1), dim3(64), 0, s, nb, (double)P.NX, (double)P.NY, (double)P.NZ, part, f0);
 }
 
 void launch_gather_fp64(const KParams &P, const double *fin, const double *gin, const double *tlag, double *fpost,

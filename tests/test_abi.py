@@ -76,7 +76,9 @@ def test_footprint_and_errors_without_gpu():
 def test_unsupported_flags_rejected_before_device_use():
     lib = C.CDLL(LIB_PATH)
     lib.ch4g_last_error.restype = C.c_char_p
-    cfg = abi.make_config(8, 8, 8, abi.LB_FLUID | abi.FLUID_FORCING_NOZEROMODE | abi.FLUID_FORCING_HIT)
+    # NOZEROMODE has a force-mean pass on the FP64 path only
+    cfg = abi.make_config(8, 8, 8, abi.LB_FLUID | abi.FLUID_FORCING_NOZEROMODE | abi.FLUID_FORCING_HIT,
+                          precision=abi.FP32_DEV)
     h = C.c_void_p()
     assert lib.ch4g_create(C.byref(cfg), C.byref(h)) == -1
     assert b"NOZEROMODE" in lib.ch4g_last_error()

@@ -79,6 +79,44 @@ def test_hit_les_fp64_matches_oracle():
     compare_fp64(o, lat, exact=False)
 
 
+@pytest.mark.parametrize("case", ["rb", "hit_poiseuille"])
+def test_nozeromode_fp64_matches_oracle(case):
+    """LB_FLUID_FORCING_NOZEROMODE on the FP64 path: the mean of the built
+    force is removed at every node.  The device sums it in a fixed tree, the
+    reference (and the oracle) node by node, so agreement is to rounding
+    (1e-12 relative), not bit-identity; the run without the switch differs."""
+    if case == "rb":
+        cfg = configs.rb2d(NX=64, NY=32, Ra=1e6)
+        steps = 300
+    else:
+        cfg = configs.hit(N=12, tau=0.55, amp=1e-4, les=False)
+        cfg.flags |= abi.FLUID_FORCING_POISEUILLE
+        cfg.Amp_x = 2e-3
+        steps = 60
+    cfg.flags |= abi.FLUID_FORCING_NOZEROMODE
+    o, lat = both(cfg, steps)
+    po, go = o.get_populations()
+    pg, gg = lat.get_populations()
+    ho, hg = o.hydro_fields(), lat.hydro_fields()
+    assert rel_max(pg, po) <= 1e-12 and rel_max(hg["dens"], ho["dens"]) <= 1e-12
+    if cfg.has(abi.LB_TEMPERATURE):
+        assert rel_max(gg, go) <= 1e-12 and rel_max(hg["t"], ho["t"]) <= 1e-12
+    # u is a difference of populations (plus F/2): before convection sets in
+    # |u| is far below |f|, so its rounding scale is the populations'
+    scale = max(np.abs(ho["u"]).max(), np.abs(po).max())
+    err_u = np.abs(hg["u"] - ho["u"]).max() / scale
+    print(f"  p {rel_max(pg, po):.2e}  u {err_u:.2e} (of max(|u|, |f|)), "
+          f"u {rel_max(hg['u'], ho['u']):.2e} of max|u| {np.abs(ho['u']).max():.2e}")
+    assert err_u <= 1e-12
+    u = hg["u"]
+    plain = cfg.copy(flags=cfg.flags & ~abi.FLUID_FORCING_NOZEROMODE)
+    _, lp = both(plain, steps)
+    assert rel_max(lp.hydro_fields()["u"], u) > 1e-6
+    if case == "hit_poiseuille":
+        # the uniform force is gone: the mean streamwise velocity stays ~0
+        assert abs(u[..., 0].mean()) < 1e-3 * abs(lp.hydro_fields()["u"][..., 0].mean())
+
+
 def test_hit_les_fd_fp64_matches_oracle():
     """HIT with the finite-difference Smagorinsky closure (the reference's
     default LB_FLUID_LES): a velocity kernel stores u, the step kernel

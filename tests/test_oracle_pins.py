@@ -20,6 +20,9 @@ mathematics fixes, chosen so that a plausible mistake fails it:
 * force lag (u carries F(T_{n-1}))
                                 <- periodic momentum budget
                                    dJ = sum rho_{n-1}[F_n + (F_{n-1}-F_n)/(2 tau)]
+* NOZEROMODE mean removal       <- a uniform force on a periodic fluid at
+                                   rest then does nothing; added to HIT
+                                   forcing it leaves the mean velocity at ~0
 """
 import ctypes
 import ctypes.util
@@ -257,3 +260,29 @@ def test_buoyancy_momentum_budget_pins_force_lag():
         prev = (h["dens"], h["force"], J)
         o.step(1)
     assert checked == 6
+
+
+def test_nozeromode_removes_the_mean_force():
+    flags = abi.LB_FLUID | abi.FLUID_FORCING_POISEUILLE | abi.FLUID_FORCING_NOZEROMODE
+    o = Oracle(abi.make_config(6, 6, 6, flags, tau_u=0.8, Amp_x=1e-3))
+    o.initial_conditions()
+    o.step(40)
+    h = o.hydro_fields()
+    assert np.abs(h["u"]).max() < 1e-16 and np.abs(h["force"]).max() < 1e-18
+    # HIT modes + a uniform force: the modes drive the flow; their k-free
+    # terms (sin 2 pi phi of the modes without a y or z wave number) and the
+    # uniform part have a non-zero mean, which the switch removes, so the
+    # mean velocity stays near zero (the momentum change is sum rho F, and
+    # rho - 1 = O(u^2) leaves ~1e-9) while without it it grows to ~0.07
+    base = abi.LB_FLUID | abi.FLUID_FORCING_HIT | abi.FLUID_FORCING_POISEUILLE
+    res = {}
+    for nz in (0, abi.FLUID_FORCING_NOZEROMODE):
+        o = Oracle(abi.make_config(8, 8, 8, base | nz, tau_u=0.6, Amp_x=2e-3, Amp_y=1e-4, Amp_z=1e-4, seed=5))
+        o.initial_conditions()
+        o.step(30)
+        res[nz] = o.hydro_fields()["u"]
+    m0 = np.abs(res[0].reshape(-1, 3).mean(0))
+    u = res[abi.FLUID_FORCING_NOZEROMODE]
+    m1 = np.abs(u.reshape(-1, 3).mean(0))
+    assert m0[0] > 1e-2 and np.all(m1 < 1e-7 * m0[0])
+    assert np.abs(u).max() > 1e-5
