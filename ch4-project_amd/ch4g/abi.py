@@ -49,6 +49,7 @@ LAGRANGE_ORIENTATION_JEFFREY = 1 << 17
 LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG = 1 << 18
 LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS = 1 << 19
 LAGRANGE_ORIENTATION_ACTIVE = 1 << 20
+LAGRANGE_ORIENTATION_BC = 1 << 21
 
 FP64 = 0
 FP32_DEV = 1

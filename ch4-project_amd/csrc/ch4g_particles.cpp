@@ -57,8 +57,8 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
     // Jeffery's equation reads the interpolated velocity gradient)
     const bool orient = lhas(*pc, CH4G_LAGRANGE_ORIENTATION), jeff = lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY);
     if (orient && !lhas(*pc, CH4G_LAGRANGE_GRADIENT)) return fail(-1, "LAGRANGE_ORIENTATION needs LAGRANGE_GRADIENT");
-    if (!orient && (jeff || lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE)))
-        return fail(-1, "LAGRANGE_ORIENTATION_JEFFREY / _ACTIVE need LAGRANGE_ORIENTATION");
+    if (!orient && (jeff || lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE) || lhas(*pc, CH4G_LAGRANGE_ORIENTATION_BC)))
+        return fail(-1, "LAGRANGE_ORIENTATION_JEFFREY / _ACTIVE / _BC need LAGRANGE_ORIENTATION");
     if (!jeff && (lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) ||
                   lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG)))
         return fail(-1, "LAGRANGE_ORIENTATION_JEFFREY_* need LAGRANGE_ORIENTATION_JEFFREY");

@@ -699,6 +699,7 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
                         vz /= 2.0;
                     }
                 }
+                if (lflag(P, CH4G_LAGRANGE_ORIENTATION_BC) && pv[1] < 0.0) pv[1] *= -1.0;
             }
             if (y >= SY - radius) {
                 if (!reinj) {
@@ -720,6 +721,7 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
                         vz = -gz * (1.0 - beta) * tau;
                     }
                 }
+                if (lflag(P, CH4G_LAGRANGE_ORIENTATION_BC) && pv[1] > 0.0) pv[1] *= -1.0;
             }
         }
         if (nd == 0) wrap_keep(P, x, y, z, p);

@@ -225,7 +225,8 @@ enum ch4g_lagrange_flag {
     CH4G_LAGRANGE_ORIENTATION_JEFFREY                 = 1u << 17, /* Jeffery equation, lagrange.c:3490-3680 */
     CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG   = 1u << 18, /* aspect_ratio ladder, lagrange.c:399-421 */
     CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS       = 1u << 19, /* lagrange.c:445-470, :3533-3566 */
-    CH4G_LAGRANGE_ORIENTATION_ACTIVE                  = 1u << 20  /* tracers swim along p, lagrange.c:502-521, :3179-3186 */
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE                  = 1u << 20, /* tracers swim along p, lagrange.c:502-521, :3179-3186 */
+    CH4G_LAGRANGE_ORIENTATION_BC                      = 1u << 21  /* p_y turned away from a y wall, lagrange.c:3866-3870, :3942-3946 */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */

@@ -807,6 +807,8 @@ static void move_at(oracle *o, int first)
                         p->vz /= 2.0;
                     }
                 }
+                /* LAGRANGE_ORIENTATION_BC (lagrange.c:3866-3870) */
+                if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_BC) && p->py < 0.0) p->py *= -1.0;
             }
             if (p->y >= SY - radius) {
                 if (!reinj) {
@@ -833,6 +835,8 @@ static void move_at(oracle *o, int first)
                         p->vz = -gz * (1.0 - p->beta_coeff) * p->tau_drag;
                     }
                 }
+                /* LAGRANGE_ORIENTATION_BC (lagrange.c:3942-3946) */
+                if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_BC) && p->py > 0.0) p->py *= -1.0;
             }
         }
     }

@@ -144,7 +144,7 @@ def test_orientation_fp64(case):
     if case == "hit":
         cfg, flags = configs.hit(16, tau=0.51, amp=1e-4), ORIENT
     else:
-        cfg, flags = rb_small(), ORIENT | abi.LAGRANGE_GRAVITY | abi.LAGRANGE_ADDEDMASS
+        cfg, flags = rb_small(), ORIENT | abi.LAGRANGE_GRAVITY | abi.LAGRANGE_ADDEDMASS | abi.LAGRANGE_ORIENTATION_BC
     pc = abi.make_particle_config(flags, 300, tau_drag=(2, 0.0, 4.0), beta_coeff=(1, 0.8, 0.8),
                                   aspect_ratio=(3, 0.5, 8.0), gyrotaxis_velocity=(2, 0.5, 2.0),
                                   swim_velocity=(2, 0.0, 0.002), time_dt_lagr_factor=2)

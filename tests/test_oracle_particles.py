@@ -466,3 +466,21 @@ def test_gyrotaxis_first_column_and_swimming():
         d = np.abs(a1[c] - np.mod(a0[c] + 30 * 0.01 * a0[pc], S))
         assert np.all(np.minimum(d, S - d) < 1e-12)
     assert np.array_equal(a1["px"], a0["px"]) or np.allclose(a1["px"], a0["px"], rtol=0, atol=2e-16)
+
+
+def test_orientation_turned_away_from_the_walls():
+    """LAGRANGE_ORIENTATION_BC: a particle that crosses a y wall leaves with
+    p_y pointing into the fluid (lagrange.c:3866-3870, :3942-3946)."""
+    for bc in (0, abi.LAGRANGE_ORIENTATION_BC):
+        o, cfg = quiescent(6, 8, 6, flags=abi.FLUID_BC_Y)
+        flags = abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ORIENTATION | bc
+        o.particles_init(abi.make_particle_config(flags, 2, tau_drag=(1, 1e9, 1e9)))
+        a = records(2)
+        a["x"], a["y"], a["z"] = [2.0, 3.0], [0.2, 7.9], [3.0, 2.0]
+        a["vy"], a["vy_old"], a["tau_drag"] = [-0.3, 0.3], [-0.3, 0.3], 1e9
+        a["px"], a["py"] = [0.6, 0.6], [-0.8, 0.8]
+        o.set_particles(a)
+        r = o.move(np.zeros((6, 8, 6, 3)), first=True)
+        assert r["y"][0] == pytest.approx(0.1, abs=1e-12) and r["y"][1] == pytest.approx(7.8, abs=1e-12)
+        want = [0.8, -0.8] if bc else [-0.8, 0.8]
+        assert np.allclose(r["py"], want, rtol=1e-15) and np.allclose(r["px"], 0.6, rtol=1e-15)
