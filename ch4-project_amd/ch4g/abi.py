@@ -50,6 +50,7 @@ LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG = 1 << 18
 LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS = 1 << 19
 LAGRANGE_ORIENTATION_ACTIVE = 1 << 20
 LAGRANGE_ORIENTATION_BC = 1 << 21
+LAGRANGE_ORIENTATION_SECONDORIENTATION = 1 << 22
 
 FP64 = 0
 FP32_DEV = 1
@@ -158,7 +159,8 @@ PARTICLE_FIELDS = ["name", "x", "y", "z", "vx", "vy", "vz", "vx_old", "vy_old", 
                    "ax", "ay", "az", "ax_old", "ay_old", "az_old", "ux", "uy", "uz", "ux_old", "uy_old", "uz_old",
                    "dx_ux", "dy_ux", "dz_ux", "dx_uy", "dy_uy", "dz_uy", "dx_uz", "dy_uz", "dz_uz",
                    "Dt_ux", "Dt_uy", "Dt_uz", "t", "t_old", "dt_t", "tau_drag", "beta_coeff", "dx_t", "dy_t", "dz_t",
-                   "px", "py", "pz", "dt_px", "dt_py", "dt_pz", "aspect_ratio", "gyrotaxis_velocity", "swim_velocity"]
+                   "px", "py", "pz", "dt_px", "dt_py", "dt_pz", "aspect_ratio", "gyrotaxis_velocity", "swim_velocity",
+                   "nx", "ny", "nz", "dt_nx", "dt_ny", "dt_nz"]
 
 
 class Particle(C.Structure):

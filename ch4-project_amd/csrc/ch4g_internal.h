@@ -196,7 +196,9 @@ enum PartField {
     PASP = PDTP + 3,    // aspect_ratio
     PGYV,               // gyrotaxis_velocity
     PSWIM,              // swim_velocity
-    PF_COUNT
+    PNO,                // 3 rows: nx, ny, nz (second orientation)
+    PDTN = PNO + 3,     // 3 rows: dt_nx, dt_ny, dt_nz
+    PF_COUNT = PDTN + 3
 };
 
 struct PartParams {

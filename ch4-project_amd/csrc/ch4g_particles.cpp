@@ -57,8 +57,9 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
     // Jeffery's equation reads the interpolated velocity gradient)
     const bool orient = lhas(*pc, CH4G_LAGRANGE_ORIENTATION), jeff = lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY);
     if (orient && !lhas(*pc, CH4G_LAGRANGE_GRADIENT)) return fail(-1, "LAGRANGE_ORIENTATION needs LAGRANGE_GRADIENT");
-    if (!orient && (jeff || lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE) || lhas(*pc, CH4G_LAGRANGE_ORIENTATION_BC)))
-        return fail(-1, "LAGRANGE_ORIENTATION_JEFFREY / _ACTIVE / _BC need LAGRANGE_ORIENTATION");
+    if (!orient && (jeff || lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE) || lhas(*pc, CH4G_LAGRANGE_ORIENTATION_BC) ||
+                    lhas(*pc, CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION)))
+        return fail(-1, "LAGRANGE_ORIENTATION_JEFFREY / _ACTIVE / _BC / _SECONDORIENTATION need LAGRANGE_ORIENTATION");
     if (!jeff && (lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) ||
                   lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG)))
         return fail(-1, "LAGRANGE_ORIENTATION_JEFFREY_* need LAGRANGE_ORIENTATION_JEFFREY");
@@ -256,6 +257,8 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         S(PASP) = r.aspect_ratio;
         S(PGYV) = r.gyrotaxis_velocity;
         S(PSWIM) = r.swim_velocity;
+        const double nv[6] = {r.nx, r.ny, r.nz, r.dt_nx, r.dt_ny, r.dt_nz};
+        for (int m = 0; m < 6; m++) S(PNO + m) = nv[m];
         alive[(size_t)p] = 1;
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
@@ -481,11 +484,33 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         if (gyro) r.gyrotaxis_velocity = gyv[(size_t)type];
         if (swim) r.swim_velocity = swv[(size_t)type];
         if (orient) {
+            // random_vector with glibc sincos for each angle: the call gcc
+            // makes of sin(a) and cos(a) together; separate sin and cos
+            // calls differ from it in the last bit for some angles
             const double theta = acos(2.0 * erand48(L->xsubi) - 1.0);
             const double phi = 2 * 3.14159265359 * erand48(L->xsubi); // two_pi of typedef.h:367
-            r.px = sin(theta) * sin(phi);
-            r.py = sin(theta) * cos(phi);
-            r.pz = cos(theta);
+            double st, ct, sp, cp;
+            sincos(theta, &st, &ct);
+            sincos(phi, &sp, &cp);
+            r.px = st * sp;
+            r.py = st * cp;
+            r.pz = ct;
+            if (lhas(*pc, CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION)) {
+                // n = p x v / |p x v| for a second random_vector v (lagrange.c:904-917)
+                const double t2 = acos(2.0 * erand48(L->xsubi) - 1.0);
+                const double f2 = 2 * 3.14159265359 * erand48(L->xsubi);
+                double s2, c2, sf, cf;
+                sincos(t2, &s2, &c2);
+                sincos(f2, &sf, &cf);
+                const double vx = s2 * sf, vy = s2 * cf, vz = c2;
+                r.nx = r.py * vz - r.pz * vy;
+                r.ny = -r.px * vz + r.pz * vx;
+                r.nz = r.px * vy - r.py * vx;
+                const double val = sqrt(r.nx * r.nx + r.ny * r.ny + r.nz * r.nz);
+                r.nx /= val;
+                r.ny /= val;
+                r.nz /= val;
+            }
         }
     }
     if ((rc = alloc_state(L, np))) goto err;
@@ -590,6 +615,8 @@ int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t
             r.aspect_ratio = S(PASP);
             r.gyrotaxis_velocity = S(PGYV);
             r.swim_velocity = S(PSWIM);
+            double *nv = &r.nx;
+            for (int m = 0; m < 6; m++) nv[m] = S(PNO + m);
         }
         k++;
     }

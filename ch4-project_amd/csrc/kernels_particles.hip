@@ -416,11 +416,19 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
         }
         const double tau = R(PTAU), beta = R(PBETA);
         const bool orient = lflag(P, CH4G_LAGRANGE_ORIENTATION);
+        const bool second = orient && lflag(P, CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
         double pv[3] = {0.0, 0.0, 0.0}, fold[3] = {0.0, 0.0, 0.0};
+        double nv[3] = {0.0, 0.0, 0.0}, nfold[3] = {0.0, 0.0, 0.0};
         if (orient) {
             for (int i = 0; i < 3; i++) {
                 pv[i] = R(PPO + i);
                 fold[i] = R(PDTP + i);
+            }
+        }
+        if (second) {
+            for (int i = 0; i < 3; i++) {
+                nv[i] = R(PNO + i);
+                nfold[i] = R(PDTN + i);
             }
         }
 
@@ -651,6 +659,20 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
                     for (int j = 0; j < 3; j++) rhs[i] += Wm[i][j] * pv[j] + (Sm[i][j] * pv[j]);
                     rhs[i] -= pv[i] * oso;
                 }
+                double rhsn[3] = {0.0, 0.0, 0.0};
+                if (second) {
+                    // n: W n - (p . S' n) p, with p before its update
+                    // (lagrange.c:3586-3616)
+                    double tn[3] = {0.0, 0.0, 0.0};
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) tn[i] += Sm[i][j] * nv[j];
+                    double osn = 0.0;
+                    for (int i = 0; i < 3; i++) osn += pv[i] * tn[i];
+                    for (int i = 0; i < 3; i++) {
+                        for (int j = 0; j < 3; j++) rhsn[i] += Wm[i][j] * nv[j];
+                        rhsn[i] -= pv[i] * osn;
+                    }
+                }
                 for (int i = 0; i < 3; i++) {
                     if (first)
                         pv[i] = pv[i] + dt * rhs[i];
@@ -658,10 +680,23 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
                         pv[i] = pv[i] + 0.5 * dt * (3. * rhs[i] - fold[i]);
                     fold[i] = rhs[i];
                 }
+                if (second)
+                    for (int i = 0; i < 3; i++) {
+                        if (first)
+                            nv[i] = nv[i] + dt * rhsn[i];
+                        else
+                            nv[i] = nv[i] + 0.5 * dt * (3. * rhsn[i] - nfold[i]);
+                        nfold[i] = rhsn[i];
+                    }
             }
             double norm = 0.0;
             for (int i = 0; i < 3; i++) norm += pv[i] * pv[i];
             for (int i = 0; i < 3; i++) pv[i] /= sqrt(norm);
+            if (second) {
+                double nn = 0.0;
+                for (int i = 0; i < 3; i++) nn += nv[i] * nv[i];
+                for (int i = 0; i < 3; i++) nv[i] /= sqrt(nn);
+            }
         }
 
         // ---- wall rules at y (lagrange.c:3754-3993); x and z of a random
@@ -758,6 +793,12 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
             for (int i = 0; i < 3; i++) {
                 R(PPO + i) = pv[i];
                 R(PDTP + i) = fold[i];
+            }
+        }
+        if (second) {
+            for (int i = 0; i < 3; i++) {
+                R(PNO + i) = nv[i];
+                R(PDTN + i) = nfold[i];
             }
         }
         if (P.thermal) {

@@ -383,7 +383,8 @@ void dump_particle_averages(void)
     const int nt = (int)ptypes(&rp->pcfg);
     const unsigned fl = rp->pcfg.flags;
     const int orient = (fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ORIENTATION);
-    enum { NV = 32 };
+    const int second = orient && (fl & CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
+    enum { NV = 41 };
     double *acc = (double *)calloc((size_t)nt * NV, sizeof(double));
     double *cnt = (double *)calloc((size_t)nt, sizeof(double));
     if (!acc || !cnt) die("dump_particle_averages: out of memory");
@@ -424,6 +425,15 @@ void dump_particle_averages(void)
         o[29] += pow(p->dt_px, 4.0);
         o[30] += pow(p->dt_py, 4.0);
         o[31] += pow(p->dt_pz, 4.0);
+        o[32] += p->dt_nx;
+        o[33] += p->dt_ny;
+        o[34] += p->dt_nz;
+        o[35] += p->dt_nx * p->dt_nx;
+        o[36] += p->dt_ny * p->dt_ny;
+        o[37] += p->dt_nz * p->dt_nz;
+        o[38] += pow(p->dt_nx, 4.0);
+        o[39] += pow(p->dt_ny, 4.0);
+        o[40] += pow(p->dt_nz, 4.0);
     }
     FILE *fo = fopen("particle_averages.dat", "a");
     if (fo) {
@@ -435,6 +445,8 @@ void dump_particle_averages(void)
             for (int k = 0; k < 18; k++) fprintf(fo, "%e ", o[k] * norm);
             if (orient) /* lagrange.c:2891-2896: dp/dt moments */
                 for (int k = 23; k < 32; k++) fprintf(fo, "%e ", o[k] * norm);
+            if (second) /* lagrange.c:2897-2902: dn/dt moments */
+                for (int k = 32; k < 41; k++) fprintf(fo, "%e ", o[k] * norm);
             if (rp->cfg.flags & CH4G_LB_TEMPERATURE)
                 for (int k = 18; k < 23; k++) fprintf(fo, "%e ", o[k] * norm);
             fprintf(fo, "\n");
@@ -643,6 +655,9 @@ static hid_t particle_type(const ch4g_run_params *rp)
         if (fl & CH4G_LAGRANGE_ADDEDMASS) M(beta_coeff);
         if (fl & CH4G_LAGRANGE_ORIENTATION) { /* lagrange.c:4365-4427 */
             M(px); M(py); M(pz); M(dt_px); M(dt_py); M(dt_pz);
+            if (fl & CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION) {
+                M(nx); M(ny); M(nz); M(dt_nx); M(dt_ny); M(dt_nz);
+            }
             if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {
                 M(aspect_ratio);
                 if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) M(gyrotaxis_velocity);
@@ -763,6 +778,10 @@ static int snapshot_members(const ch4g_run_params *rp, struct snap_member *m)
         if (fl & CH4G_LAGRANGE_ORIENTATION) { /* lagrange.c:1889-2012 */
             SM("px", px); SM("py", py); SM("pz", pz);
             SM("dt_px", dt_px); SM("dt_py", dt_py); SM("dt_pz", dt_pz);
+            if (fl & CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION) {
+                SM("nx", nx); SM("ny", ny); SM("nz", nz);
+                SM("dt_nx", dt_nx); SM("dt_ny", dt_ny); SM("dt_nz", dt_nz);
+            }
             if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {
                 SM("aspect_ratio", aspect_ratio);
                 if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) SM("gyrotaxis_velocity", gyrotaxis_velocity);
@@ -828,7 +847,7 @@ static void write_particle_snapshot(void)
     hid_t fspace = H5Screate_simple(1, dims, NULL);
     hid_t mspace = H5Screate_simple(1, count, NULL);
     H5Sselect_hyperslab(fspace, H5S_SELECT_SET, offset, NULL, count, NULL);
-    struct snap_member m[48];
+    struct snap_member m[56];
     const int nm = snapshot_members(rp, m);
     for (int k = 0; k < nm; k++) {
         for (int64_t i = 0; i < n; i++) aux[i] = *(const double *)((const char *)&a[i] + m[k].off);
@@ -877,6 +896,11 @@ static void write_particle_snapshot(void)
             const char *po[3] = {"px", "py", "pz"}, *dp[3] = {"dt_px", "dt_py", "dt_pz"};
             xmf_joined(fo, h5, "orientation", "Vector", po, 3, np);
             xmf_joined(fo, h5, "angular velocity", "Vector", dp, 3, np);
+            if (fl & CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION) {
+                const char *no[3] = {"nx", "ny", "nz"}, *dn[3] = {"dt_nx", "dt_ny", "dt_nz"};
+                xmf_joined(fo, h5, "second orientation", "Vector", no, 3, np);
+                xmf_joined(fo, h5, "second angular velocity", "Vector", dn, 3, np);
+            }
             if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {
                 xmf_scalar(fo, h5, "aspect_ratio", "aspect_ratio", np);
                 if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS)

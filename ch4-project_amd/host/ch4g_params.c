@@ -70,6 +70,7 @@ static const struct { const char *name; unsigned flag; } LFLAGS[] = {
     {"LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS", CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS},
     {"LAGRANGE_ORIENTATION_ACTIVE", CH4G_LAGRANGE_ORIENTATION_ACTIVE},
     {"LAGRANGE_ORIENTATION_BC", CH4G_LAGRANGE_ORIENTATION_BC},
+    {"LAGRANGE_ORIENTATION_SECONDORIENTATION", CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -240,7 +241,8 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
        aspect-ratio ones inside LAGRANGE_ORIENTATION_JEFFREY (lagrange.c:3470-3752) */
     if (!(out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION))
         out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_ORIENTATION_JEFFREY | CH4G_LAGRANGE_ORIENTATION_ACTIVE |
-                                       CH4G_LAGRANGE_ORIENTATION_BC);
+                                       CH4G_LAGRANGE_ORIENTATION_BC |
+                                       CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
     if (!(out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_JEFFREY))
         out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG |
                                        CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);

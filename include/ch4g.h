@@ -226,7 +226,8 @@ enum ch4g_lagrange_flag {
     CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG   = 1u << 18, /* aspect_ratio ladder, lagrange.c:399-421 */
     CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS       = 1u << 19, /* lagrange.c:445-470, :3533-3566 */
     CH4G_LAGRANGE_ORIENTATION_ACTIVE                  = 1u << 20, /* tracers swim along p, lagrange.c:502-521, :3179-3186 */
-    CH4G_LAGRANGE_ORIENTATION_BC                      = 1u << 21  /* p_y turned away from a y wall, lagrange.c:3866-3870, :3942-3946 */
+    CH4G_LAGRANGE_ORIENTATION_BC                      = 1u << 21, /* p_y turned away from a y wall, lagrange.c:3866-3870, :3942-3946 */
+    CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION       = 1u << 22  /* second vector n normal to p, lagrange.c:904-924, :3586-3740 */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -256,6 +257,7 @@ typedef struct ch4g_particle {
     double dx_t, dy_t, dz_t;   /* temperature gradient (LB_TEMPERATURE with LAGRANGE_GRADIENT) */
     double px, py, pz, dt_px, dt_py, dt_pz;             /* orientation and its last dp/dt (LAGRANGE_ORIENTATION) */
     double aspect_ratio, gyrotaxis_velocity, swim_velocity;
+    double nx, ny, nz, dt_nx, dt_ny, dt_nz;             /* second orientation (..._SECONDORIENTATION) */
 } ch4g_particle;
 
 /* allocate_particles (lagrange.c:6-49) + initial_conditions_particles

@@ -180,6 +180,19 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
             p->py = sin(theta) * cos(phi);
             p->pz = cos(theta);
             p->dt_px = p->dt_py = p->dt_pz = 0.0;
+            if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION)) {
+                /* n = p x v / |p x v|, v a second random_vector (lagrange.c:904-917) */
+                double t2 = acos(2.0 * oracle_drand48(o) - 1.0);
+                double f2 = 2 * 3.14159265359 * oracle_drand48(o);
+                double vx = sin(t2) * sin(f2), vy = sin(t2) * cos(f2), vz = cos(t2);
+                p->nx = p->py * vz - p->pz * vy;
+                p->ny = -p->px * vz + p->pz * vx;
+                p->nz = p->px * vy - p->py * vx;
+                double val = sqrt(p->nx * p->nx + p->ny * p->ny + p->nz * p->nz);
+                p->nx /= val;
+                p->ny /= val;
+                p->nz /= val;
+            }
         }
     }
     free(tau);
@@ -581,6 +594,9 @@ static void orient_step(const olagr *L, ch4g_particle *p, int first, double gx, 
 {
     double P[3] = {p->px, p->py, p->pz};
     double Fold[3] = {p->dt_px, p->dt_py, p->dt_pz};
+    const int second = LHAS(L, CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
+    double Nv[3] = {p->nx, p->ny, p->nz};
+    double FNold[3] = {p->dt_nx, p->dt_ny, p->dt_nz};
     if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_JEFFREY)) {
         const double al = p->aspect_ratio;
         const double fa = (al * al - 1.0) / (1.0 + al * al);
@@ -621,10 +637,26 @@ static void orient_step(const olagr *L, ch4g_particle *p, int first, double gx, 
             for (int j = 0; j < 3; j++) F[i] += W[i][j] * P[j] + (S[i][j] * P[j]);
             F[i] -= P[i] * oso;
         }
+        /* second orientation (lagrange.c:3586-3616): W n - (p . S n) p */
+        double FN[3] = {0, 0, 0}, TN[3] = {0, 0, 0}, osn = 0.0;
+        if (second) {
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) TN[i] += S[i][j] * Nv[j];
+            for (int i = 0; i < 3; i++) osn += P[i] * TN[i];
+            for (int i = 0; i < 3; i++) {
+                for (int j = 0; j < 3; j++) FN[i] += W[i][j] * Nv[j];
+                FN[i] -= P[i] * osn;
+            }
+        }
         for (int i = 0; i < 3; i++) {
             P[i] = first ? P[i] + L->dt * F[i] : P[i] + 0.5 * L->dt * (3. * F[i] - Fold[i]);
             Fold[i] = F[i];
         }
+        if (second)
+            for (int i = 0; i < 3; i++) {
+                Nv[i] = first ? Nv[i] + L->dt * FN[i] : Nv[i] + 0.5 * L->dt * (3. * FN[i] - FNold[i]);
+                FNold[i] = FN[i];
+            }
     }
     double norm = 0.0;
     for (int i = 0; i < 3; i++) norm += P[i] * P[i];
@@ -635,6 +667,17 @@ static void orient_step(const olagr *L, ch4g_particle *p, int first, double gx, 
     p->dt_px = Fold[0];
     p->dt_py = Fold[1];
     p->dt_pz = Fold[2];
+    if (second) {
+        double nn = 0.0;
+        for (int i = 0; i < 3; i++) nn += Nv[i] * Nv[i];
+        for (int i = 0; i < 3; i++) Nv[i] /= sqrt(nn);
+        p->nx = Nv[0];
+        p->ny = Nv[1];
+        p->nz = Nv[2];
+        p->dt_nx = FNold[0];
+        p->dt_ny = FNold[1];
+        p->dt_nz = FNold[2];
+    }
 }
 
 static void move_at(oracle *o, int first)

@@ -369,7 +369,8 @@ def read_dataset_h5(path, name, n):
 
 
 ORIENT_DEFINES = ("#define LAGRANGE_ORIENTATION\n#define LAGRANGE_ORIENTATION_JEFFREY\n"
-                  "#define LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS\n#define LAGRANGE_ORIENTATION_ACTIVE\n")
+                  "#define LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS\n#define LAGRANGE_ORIENTATION_ACTIVE\n"
+                  "#define LAGRANGE_ORIENTATION_SECONDORIENTATION\n")
 ORIENT_PARAMS = ("aspect_ratio_types 3\naspect_ratio_min 0.5\naspect_ratio_max 4\n"
                  "gyrotaxis_velocity_types 1\ngyrotaxis_velocity_min 2\ngyrotaxis_velocity_max 2\n"
                  "swim_velocity_types 2\nswim_velocity_min 0\nswim_velocity_max 0.001\n")
@@ -410,11 +411,11 @@ def test_particle_snapshots(tmp_path, orient):
                "Dt_uz", "temperature", "dt_t", "dx_t", "dy_t", "dz_t"]
     if orient:
         i = members.index("temperature")
-        members[i:i] = ["px", "py", "pz", "dt_px", "dt_py", "dt_pz", "aspect_ratio", "gyrotaxis_velocity",
-                        "swim_velocity"]
+        members[i:i] = ["px", "py", "pz", "dt_px", "dt_py", "dt_pz", "nx", "ny", "nz", "dt_nx", "dt_ny", "dt_nz",
+                        "aspect_ratio", "gyrotaxis_velocity", "swim_velocity"]
         assert np.unique(ref["aspect_ratio"]).size == 4  # the ladder and the tracer's 1
         rows = [l.split() for l in open(tmp_path / "particle_averages.dat")]
-        assert {len(r) for r in rows} == {2 + 18 + 9 + 5}
+        assert {len(r) for r in rows} == {2 + 18 + 9 + 9 + 5}
     path = tmp_path / "RUN" / "particle_120.h5"
     for m in members:
         got = read_dataset_h5(path, "/lagrange/" + m, npart)

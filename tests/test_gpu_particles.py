@@ -134,7 +134,8 @@ def test_twoway_smalltaud_feedback_fp64():
 
 ORIENT = (abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_WRAP | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ORIENTATION |
           abi.LAGRANGE_ORIENTATION_JEFFREY | abi.LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG |
-          abi.LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS | abi.LAGRANGE_ORIENTATION_ACTIVE)
+          abi.LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS | abi.LAGRANGE_ORIENTATION_ACTIVE |
+          abi.LAGRANGE_ORIENTATION_SECONDORIENTATION)
 
 
 @pytest.mark.parametrize("case", ["hit", "rb_gravity"])

@@ -484,3 +484,31 @@ def test_orientation_turned_away_from_the_walls():
         assert r["y"][0] == pytest.approx(0.1, abs=1e-12) and r["y"][1] == pytest.approx(7.8, abs=1e-12)
         want = [0.8, -0.8] if bc else [-0.8, 0.8]
         assert np.allclose(r["py"], want, rtol=1e-15) and np.allclose(r["px"], 0.6, rtol=1e-15)
+
+
+def test_second_orientation_stays_normal_to_p():
+    """n starts as p x v / |p x v| (one more random_vector per particle) and
+    dn/dt = W n - (p.S'n) p keeps p.n = 0: d(p.n)/dt = (Wp).n + p.(Wn) +
+    (S'p).n - p.(S'n) = 0 for W antisymmetric and S' symmetric."""
+    seed = 17
+    o, cfg = quiescent(8, 16, 4, seed=seed)
+    flags = ORIENT | abi.LAGRANGE_ORIENTATION_SECONDORIENTATION
+    o.particles_init(abi.make_particle_config(flags, 8, tau_drag=(1, 0.0, 0.0), aspect_ratio=(1, 4.0, 4.0)))
+    a = o.particles()
+    d = glibc_stream(seed, 7 * 8).reshape(8, 7)
+    t2, f2 = np.arccos(2.0 * d[:, 5] - 1.0), 2 * 3.14159265359 * d[:, 6]
+    v = np.stack([np.sin(t2) * np.sin(f2), np.sin(t2) * np.cos(f2), np.cos(t2)], -1)
+    p0 = np.stack([a["px"], a["py"], a["pz"]], -1)
+    n0 = np.cross(p0, v)
+    n0 /= np.linalg.norm(n0, axis=1, keepdims=True)
+    assert np.allclose(np.stack([a["nx"], a["ny"], a["nz"]], -1), n0, rtol=0, atol=1e-14)
+    a["x"], a["y"], a["z"] = 4.0, 8.0, 2.0
+    o.set_particles(a)
+    u = _shear(cfg, 0.02)
+    for s in range(400):
+        r = o.move(u, first=(s == 0))
+    p = np.stack([r["px"], r["py"], r["pz"]], -1)
+    n = np.stack([r["nx"], r["ny"], r["nz"]], -1)
+    assert np.abs((p * n).sum(1)).max() < 1e-4
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-15)
+    assert np.abs(n - n0).max() > 0.05  # n did turn
