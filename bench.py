@@ -1,15 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the D3Q19 fluid + temperature streaming step on one MI355X.
 
-Workload: the 2-D Rayleigh-Benard configuration, 512 x 1 x 256 lattice
-(stored as NX=512, NY=256 vertical, NZ=1), D3Q19 fluid + D3Q19 temperature
-populations, Boussinesq buoyancy at Ra = 1e8, Pr = 1, no-slip and fixed-T
-walls, initial T linear plus seeded uniform noise 1e-3.  The timed path is the
-FP32-deviation fused stream+collide kernel (value); the FP64 plain path is
-timed beside it.  A step = one fused kernel launch advancing every node of
-both population sets one time step; inputs are resident in HBM.
+Workload (default): the 3-D Rayleigh-Benard configuration, 512^3 lattice
+(y vertical), D3Q19 fluid + D3Q19 temperature populations, Boussinesq
+buoyancy at Ra = 1e9, Pr = 1, no-slip and fixed-T walls at y, periodic x and
+z, initial T linear plus seeded uniform noise 1e-3 -- the largest
+configuration that fits one GPU (41 GB in FP32-deviation storage, 83 GB in
+FP64).  The timed path is the FP32-deviation fused stream+collide kernel
+(value); the FP64 plain path is timed beside it.  A step = every node of both
+population sets advanced one time step; inputs are resident in HBM.  The 2-D
+512 x 256 configuration (40 MB, resident in the 256 MB last-level cache) is
+reported beside it under its own key.
 
-Prints one JSON line (see README of the driver contract in DESIGN.md).
+Prints one JSON line.
 """
 import argparse
 import json
@@ -40,16 +43,25 @@ WORKLOADS = {
 }
 
 
-def pmc_traffic(workload):
-    """HBM bytes per launch of the workload's step kernel measured with
-    rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a plain copy;
-    profiles/*_pmc_traffic.md)."""
+def pmc_traffic(workload, kernel):
+    """HBM bytes per launch of the workload's step kernel, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/pmc_traffic.json
+    (calibrated on a plain copy; the matching profiles/*_pmc_traffic.md gives
+    the commands).  Counters cannot be read inside this process, so the
+    figure is the recorded one: it is used only when the record names the
+    same kernel, and the line says which commit and configuration it comes
+    from."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh)[workload]["traffic_bytes"]
+            rec = json.load(fh)[workload]
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
+    if rec.get("kernel") != kernel:
+        return None, None
+    src = (f"profiles/pmc_traffic.json[{workload}]: {rec.get('kernel')} at commit {rec.get('commit', '?')}, "
+           f"{rec.get('config', workload)}")
+    return rec["traffic_bytes"], src
 
 
 def gpu_leg(precision, steps, warmup, workload="rb2d"):
@@ -75,27 +87,35 @@ def gpu_leg(precision, steps, warmup, workload="rb2d"):
     return out
 
 
-def cpu_leg(target_s=12.0):
+def cpu_leg(workload, target_s=12.0):
     """The oracle (FP64, reference-structured un-fused sweeps, OpenMP) on
-    this host's cores, on a bounded sample of the same workload."""
+    this host's cores, on a bounded sample of the same workload: the 3-D
+    workloads are sampled on a 128^3 lattice of the same physics (fluid + T,
+    Ra = 1e9; 512^3 needs ~90 GB of host memory in the reference's ghost-cell
+    layout), the 2-D one at its own size."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import configs
     from oracle import Oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    cfg = configs.rb2d()
+    if workload.startswith("rb3d"):
+        cfg = configs.rb2d(NX=128, NY=128, NZ=128, Ra=1e9)
+        what = "3-D Rayleigh-Benard 128^3 fluid+T FP64 (Ra=1e9)"
+    else:
+        cfg = configs.rb2d()
+        what = "2-D Rayleigh-Benard 512x256 fluid+T FP64"
     o = Oracle(cfg)
     o.initial_conditions()
     t0 = time.perf_counter()
-    o.step(5)
+    o.step(2)
     probe = time.perf_counter() - t0
-    n = max(5, min(2000, int(target_s / max(probe / 5, 1e-6))))
+    n = max(2, min(2000, int(target_s / max(probe / 2, 1e-6))))
     t0 = time.perf_counter()
     o.step(n)
     dt = time.perf_counter() - t0
     return {"value": round(cfg.nodes() * n / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"2-D Rayleigh-Benard 512x256 fluid+T FP64, {n} steps ({dt:.1f} s), oracle/ch4_oracle.c "
-                      f"un-fused reference-structured sweeps, OpenMP {threads} threads"}
+            "sample": f"{what}, {n} steps ({dt:.1f} s), oracle/ch4_oracle.c un-fused reference-structured sweeps, "
+                      f"OpenMP {threads} threads"}
 
 
 def _serve_both(make_jobs):
@@ -234,16 +254,61 @@ def particle_leg(steps=40, npart=2_000_000, N=256):
             "particle_updates_per_s": round(npart / (pms * 1e-3), 1) if pms > 0 else None, "alive": alive}
 
 
+def twoway_leg(steps=6, npart=4_000_000, N=512):
+    """The particle part of the 3-D configuration: 512^3 fluid + T
+    (FP32-deviation, Ra = 1e9) with 4e6 heavy particles (beta = 0.6, i.e.
+    density ratio 2; tau_p = 516 steps, Stokes number ~0.5 on the Kolmogorov
+    time of Ra = 1e9) coupled back onto the fluid (feedback sorted per node,
+    summed in particle order).  The particle gravity is 2e-5 with beta_t
+    rescaled so that the fluid's beta_t g is unchanged.  Device time per step
+    without and with the particles."""
+    import configs
+    from ch4g import Lattice, abi
+    c = configs.rb2d(NX=N, NY=N, NZ=N, Ra=1e9, precision=abi.FP32_DEV)
+    g = 2e-5
+    cfg = c.copy(gravity_y=g, beta_t=c.beta_t * c.gravity_y / g)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.step(3)
+    fluid_ms = lat.step_timed(steps) / steps
+    lat.close()
+    flags = (abi.LAGRANGE_WRAP | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS | abi.LAGRANGE_GRAVITY |
+             abi.LAGRANGE_TWOWAY_MOMENTUM)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.particles_init(abi.make_particle_config(flags, npart, tau_drag=(1, 516.0, 516.0), beta_coeff=(1, 0.6, 0.6)))
+    lat.step(3)
+    both_ms = lat.step_timed(steps) / steps
+    alive = lat.particles_count()
+    lat.close()
+    pms = both_ms - fluid_ms
+    return {"workload": f"3-D Rayleigh-Benard {N}^3 FP32-deviation + {npart} two-way heavy particles (beta 0.6, "
+                        "tau_p 516), gradients, added mass, gravity, feedback momentum",
+            "fluid_ms_per_step": round(fluid_ms, 4), "with_particles_ms_per_step": round(both_ms, 4),
+            "particle_ms_per_step": round(pms, 4),
+            "particle_updates_per_s": round(npart / (pms * 1e-3), 1) if pms > 0 else None, "alive": alive}
+
+
+def roofline_of(r, bpn):
+    """Algorithmic bytes per launch (bytes per node update x nodes x steps
+    per launch) over the average launch duration (HIP events on the lattice
+    stream around the timed steps)."""
+    alg_launch = bpn * r["nodes"] * r["steps_per_launch"]
+    achieved = alg_launch / (r["avg_launch_us"] * 1e-6) / 1e9
+    return alg_launch, achieved
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain-path leg")
-    ap.add_argument("--workload", default="rb2d", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="rb3d-512", choices=sorted(WORKLOADS))
     ap.add_argument("--serve-jobs", type=int, default=16, help="jobs of the served-ensemble leg (0: skip)")
-    ap.add_argument("--no-particles", action="store_true", help="skip the particle leg")
+    ap.add_argument("--no-particles", action="store_true", help="skip the particle legs")
+    ap.add_argument("--no-rb2d", action="store_true", help="skip the 2-D (cache-resident) leg")
     ap.add_argument("--batch-compare", action="store_true",
                     help="only compare single-lattice and batched stepping of sets of lattices (no graphs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -268,12 +333,10 @@ def main():
         sys.exit(2)
     from ch4g import abi
     r32 = gpu_leg(abi.FP32_DEV, args.steps, args.warmup, args.workload)
+    print(f"bench: {args.workload} FP32-deviation {r32['mlups']:.1f} MLUPS", file=sys.stderr, flush=True)
     bpn = BYTES_PER_NODE["fp32dev"] if r32["thermal"] else BYTES_PER_NODE["fp32dev"] // 2
-    # algorithmic bytes per launch (bytes per node update x nodes x steps per
-    # launch) over the average launch duration (HIP events on the lattice
-    # stream around the timed steps)
-    alg_launch = bpn * r32["nodes"] * r32["steps_per_launch"]
-    achieved = alg_launch / (r32["avg_launch_us"] * 1e-6) / 1e9
+    alg_launch, achieved = roofline_of(r32, bpn)
+    traffic, traffic_src = pmc_traffic(args.workload, r32["kernel"])
     line = {
         "metric": METRIC,
         "value": round(r32["mlups"], 2),
@@ -290,24 +353,47 @@ def main():
         "config": {"workload": WORKLOADS[args.workload][0] + ", FP32-deviation storage",
                    "nodes": r32["nodes"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(args.workload),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": r32["kernel"], "steps_per_launch": r32["steps_per_launch"],
                      "bytes_per_node": bpn, "algorithmic_bytes_per_launch": alg_launch,
                      "avg_launch_us": round(r32["avg_launch_us"], 3)},
     }
     if not args.no_fp64:
-        r64 = gpu_leg(abi.FP64, max(args.steps // 2, 1), args.warmup, args.workload)
+        r64 = gpu_leg(abi.FP64, args.steps, args.warmup, args.workload)
         b64 = BYTES_PER_NODE["fp64"] if r64["thermal"] else BYTES_PER_NODE["fp64"] // 2
-        a64 = b64 * r64["nodes"] * r64["steps_per_launch"] / (r64["avg_launch_us"] * 1e-6) / 1e9
-        line["fp64_plain"] = {"mlups": round(r64["mlups"], 2), "avg_launch_us": round(r64["avg_launch_us"], 3),
+        _, a64 = roofline_of(r64, b64)
+        line["fp64_plain"] = {"mlups": round(r64["mlups"], 2), "kernel": r64["kernel"],
+                              "avg_launch_us": round(r64["avg_launch_us"], 3),
                               "roofline_frac": round(a64 / HBM_PEAK_GBS, 4),
-                              "fp32dev_speedup": round(r32["mlups"] / r64["mlups"], 3)}
+                              "fp32dev_speedup": round(r32["mlups"] / r64["mlups"], 3),
+                              "fp32dev_speedup_kernel_time": round(r64["us_per_step"] / r32["us_per_step"], 3)}
+        print(f"bench: {args.workload} FP64 {r64['mlups']:.1f} MLUPS", file=sys.stderr, flush=True)
+    if args.workload != "rb2d" and not args.no_rb2d:
+        # the 2-D configuration: 40 MB of state, resident in the 256 MB
+        # last-level cache, so its fraction is not an HBM figure
+        k = max(args.steps, 4000)
+        a = gpu_leg(abi.FP32_DEV, k, 200, "rb2d")
+        _, ach = roofline_of(a, BYTES_PER_NODE["fp32dev"])
+        rb2d = {"workload": WORKLOADS["rb2d"][0] + ", FP32-deviation storage; state resident in the last-level cache",
+                "steps": k, "mlups": round(a["mlups"], 2), "kernel": a["kernel"],
+                "steps_per_launch": a["steps_per_launch"], "avg_launch_us": round(a["avg_launch_us"], 3),
+                "algorithmic_frac_cache_resident": round(ach / HBM_PEAK_GBS, 4)}
+        if not args.no_fp64:
+            b = gpu_leg(abi.FP64, k, 200, "rb2d")
+            rb2d["fp64_mlups"] = round(b["mlups"], 2)
+            rb2d["fp32dev_speedup"] = round(a["mlups"] / b["mlups"], 3)
+        line["rb2d_cache_resident"] = rb2d
     if not args.no_particles:
         line["particles"] = particle_leg()
+        print("bench: HIT particle leg done", file=sys.stderr, flush=True)
+        if args.workload == "rb3d-512":
+            line["particles_twoway_rb3d"] = twoway_leg()
+            print("bench: two-way particle leg done", file=sys.stderr, flush=True)
     if args.serve_jobs > 0:
         line["serving"] = serve_leg(args.serve_jobs)
-    if not args.no_cpu and args.workload == "rb2d":
-        line["cpu_baseline"] = cpu_leg()
+        print("bench: serving leg done", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_leg(args.workload)
     print(json.dumps(line), flush=True)
 
 

@@ -1,0 +1,182 @@
+"""GPU parity of the 3-D Rayleigh-Benard configuration (fluid + temperature,
+buoyancy, no-slip and fixed-T walls at y, periodic x and z) and of the 3-D
+channel forcing, through the C-ABI against the FP64 oracle.
+
+* FP64: populations, density, velocity and temperature bit-identical to the
+  oracle (kernels_fp64.hip follows the reference's evaluation order without
+  contraction).  NX = 45 fills 70 % of the 64-lane waves of a row, so the
+  steady steps run on the dense one-node-per-lane grid; NX = 64 runs the row
+  grid.  Both exercise the 3-D thermal z-neighbour gathers of every step.
+* FP32-deviation (and FP32 storage with double arithmetic): relative L2 error
+  per field against the FP64 oracle <= 1e-5.
+* Two-way coupled heavy particles on a 32^3 lattice: particle records and
+  fluid populations bit-identical.
+* 512^3 (the full 3-D configuration, Ra = 1e9): initial averages against
+  their closed forms, FP32-deviation against the FP64 device path after 100
+  steps, mass conservation and bounded fields after 400 steps.
+"""
+import numpy as np
+import pytest
+
+from ch4g import abi, Lattice
+from oracle import Oracle
+import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_max(a, b):
+    s = np.abs(b).max()
+    return np.abs(a - b).max() / (s if s > 0 else 1.0)
+
+
+def rel_l2(a, b):
+    s = np.sqrt((b * b).sum())
+    return np.sqrt(((a - b) ** 2).sum()) / (s if s > 0 else 1.0)
+
+
+def run(cfg, steps, oracle=False):
+    x = Oracle(cfg) if oracle else Lattice(cfg)
+    x.initial_conditions()
+    x.step(steps)
+    return x
+
+
+def rb3d(NX, NY, NZ, precision=abi.FP64, Ra=2e6):
+    return configs.rb2d(NX=NX, NY=NY, NZ=NZ, Ra=Ra, precision=precision, seed=777)
+
+
+@pytest.mark.parametrize("shape,steps", [((45, 32, 24), 300), ((64, 24, 16), 200)],
+                         ids=["dense_45x32x24", "rows_64x24x16"])
+def test_rb3d_fp64_bit_identical(shape, steps):
+    cfg = rb3d(*shape)
+    print(f"  rb3d FP64 {shape} {steps} steps", flush=True)
+    o = run(cfg, steps, oracle=True)
+    lat = run(cfg, steps)
+    po, go = o.get_populations()
+    pg, gg = lat.get_populations()
+    ho, hg = o.hydro_fields(), lat.hydro_fields()
+    for name, a, b in (("p", pg, po), ("g", gg, go), ("dens", hg["dens"], ho["dens"]), ("u", hg["u"], ho["u"]),
+                       ("t", hg["t"], ho["t"])):
+        print(f"  {name}: max rel err {rel_max(a, b):.3e} bit-identical={np.array_equal(a, b)}")
+        assert np.array_equal(a, b), name
+    # the flow has left the initial state: the comparison is not of zeros
+    assert np.abs(ho["u"]).max() > 1e-6
+    ao, ag = o.dump_averages(), lat.dump_averages()
+    for k in ("nuy", "t", "t2", "ene", "eps", "epst"):
+        assert abs(ag[k] - ao[k]) <= 1e-11 * max(abs(ao[k]), 1e-300), (k, ag[k], ao[k])
+
+
+@pytest.mark.parametrize("precision", [abi.FP32_DEV, abi.FP32_STORE], ids=["fp32dev", "fp32store"])
+def test_rb3d_fp32_against_fp64_oracle(precision):
+    shape, steps = (45, 32, 24), 300
+    o = run(rb3d(*shape), steps, oracle=True)
+    lat = run(rb3d(*shape, precision=precision), steps)
+    ho, hg = o.hydro_fields(), lat.hydro_fields()
+    eu, et, ed = rel_l2(hg["u"], ho["u"]), rel_l2(hg["t"], ho["t"]), rel_l2(hg["dens"], ho["dens"])
+    print(f"  L2 rel err u {eu:.3e}  T {et:.3e}  rho {ed:.3e}")
+    assert eu <= 1e-5 and et <= 1e-5 and ed <= 1e-5
+
+
+def test_channel_forcing_3d_fp64_bit_identical():
+    """LB_FLUID_FORCING_CHANNEL (forcing.c:374-379, F_x = 2 Amp_x^2 / LY) on a
+    3-D channel with no-slip y walls, together with the Poiseuille term."""
+    cfg = abi.make_config(20, 18, 6, abi.LB_FLUID | abi.FLUID_FORCING_CHANNEL | abi.FLUID_BC_Y, tau_u=0.8,
+                          Amp_x=3e-3)
+    o, lat = run(cfg, 400, oracle=True), run(cfg, 400)
+    po, _ = o.get_populations()
+    pg, _ = lat.get_populations()
+    assert np.array_equal(pg, po)
+    u = lat.hydro_fields()["u"]
+    assert np.array_equal(u, o.hydro_fields()["u"])
+    assert u[..., 0].max() > 1e-4  # the channel force drives the flow
+    both = cfg.copy(flags=cfg.flags | abi.FLUID_FORCING_POISEUILLE)
+    o, lat = run(both, 50, oracle=True), run(both, 50)
+    assert np.array_equal(lat.get_populations()[0], o.get_populations()[0])
+
+
+HEAVY = abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_WRAP | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS | \
+    abi.LAGRANGE_GRAVITY
+
+
+def test_rb3d_twoway_heavy_particles_fp64():
+    """32^3 convection cell with 2000 heavy particles (beta = 0.6, density
+    ratio 2) coupled back onto the fluid: every particle field and the fluid
+    populations bit-identical to the oracle after 60 steps."""
+    c = rb3d(32, 32, 32, Ra=1e5)
+    g = 2e-3
+    cfg = c.copy(gravity_y=g, beta_t=c.beta_t * c.gravity_y / g)
+    pc = abi.make_particle_config(HEAVY | abi.LAGRANGE_TWOWAY_MOMENTUM, 2000, tau_drag=(2, 1.0, 4.0),
+                                  beta_coeff=(1, 0.6, 0.6))
+    o, lat = Oracle(cfg), Lattice(cfg)
+    for x in (o, lat):
+        x.initial_conditions()
+        x.particles_init(pc)
+        x.step(60)
+    a, b = o.particles(), lat.particles()
+    assert len(a) == len(b) > 1900
+    for f in abi.PARTICLE_FIELDS:
+        assert np.array_equal(a[f], b[f]), f
+    po, go = o.get_populations()
+    pg, gg = lat.get_populations()
+    assert np.array_equal(po, pg) and np.array_equal(go, gg)
+    # the particles act on the fluid: without the feedback the fluid differs
+    pc1 = abi.make_particle_config(HEAVY, 2000, tau_drag=(2, 1.0, 4.0), beta_coeff=(1, 0.6, 0.6))
+    one = Lattice(cfg)
+    one.initial_conditions()
+    one.particles_init(pc1)
+    one.step(60)
+    assert not np.array_equal(one.get_populations()[0], pg)
+
+
+def test_rb3d_512_full_size():
+    """The full 3-D configuration: 512^3, Ra = 1e9, Pr = 1, initial T linear
+    plus uniform noise 1e-3.
+
+    Initial averages against closed forms (y = (j + 1/2)/H, H = 512):
+    <T> = 1/2, <T^2> = 1/3 - 1/(12 H^2) + a^2/3 (noise amplitude a), and
+    Nu = 1 + (beta g / 2) Var(T) H / (kappa dT): the hydro velocity carries the
+    Guo half force beta g (T - (T_bot + T_top)/2), so <u_y T> = (beta g / 2)
+    Var(T) (a sign slip or another reference temperature changes it).
+    Then FP32-deviation against the FP64 device path (bit-identical to the
+    oracle on the smaller 3-D lattices) after 100 steps, and mass
+    conservation and bounded fields after 400 steps."""
+    H, a = 512, 1e-3
+    c64 = configs.rb2d(NX=H, NY=H, NZ=H, Ra=1e9)
+    kappa = (c64.tau_t - 0.5) / 3.0
+    print("  512^3 FP64", flush=True)
+    lat = Lattice(c64)
+    lat.initial_conditions()
+    a0 = lat.dump_averages()
+    t2 = 1.0 / 3.0 - 1.0 / (12.0 * H * H) + a * a / 3.0
+    var = a0["t2"] - a0["t"] ** 2
+    nu0 = 1.0 + 0.5 * c64.beta_t * c64.gravity_y * var * H / kappa
+    print(f"  <T> {a0['t']:.12f}  <T^2> {a0['t2']:.12f} (closed form {t2:.12f})  Nu {a0['nuy']:.8f} "
+          f"(closed form {nu0:.8f})  <rho> {a0['rho']!r}")
+    assert abs(a0["t"] - 0.5) < 1e-6
+    assert abs(a0["t2"] - t2) < 4e-7
+    assert abs(a0["nuy"] / nu0 - 1) < 1e-4
+    assert abs(a0["rho"] - 1.0) < 1e-12
+    lat.step(100)
+    h64 = lat.hydro_fields()
+    lat.close()
+    print("  512^3 FP32-deviation", flush=True)
+    lat = Lattice(configs.rb2d(NX=H, NY=H, NZ=H, Ra=1e9, precision=abi.FP32_DEV))
+    lat.initial_conditions()
+    b0 = lat.dump_averages()
+    for k in ("t", "t2", "nuy", "rho"):
+        assert abs(b0[k] - a0[k]) <= 1e-6 * abs(a0[k]), k
+    lat.step(100)
+    h32 = lat.hydro_fields()
+    eu, et = rel_l2(h32["u"], h64["u"]), rel_l2(h32["t"], h64["t"])
+    print(f"  100 steps: L2 rel err FP32-deviation vs FP64: u {eu:.3e}  T {et:.3e}  "
+          f"max|u| {np.abs(h64['u']).max():.3e}")
+    assert eu <= 1e-5 and et <= 1e-5
+    del h64
+    lat.step(300)
+    b1 = lat.dump_averages()
+    h = lat.hydro_fields()
+    assert abs(b1["rho"] - 1.0) < 1e-6
+    assert np.isfinite(h["u"]).all() and np.isfinite(h["t"]).all()
+    assert h["t"].min() > -0.05 and h["t"].max() < 1.05
+    lat.close()
