@@ -12,8 +12,16 @@
  * its fields are the reference's fields on the same inputs.
  *
  * Each function cites the reference lines it follows.  How each part is
- * pinned (closed forms, invariants, glibc) is listed in tests/test_oracle_*.py;
- * the parts with no such check are marked "parity unpinned" below.
+ * pinned (closed forms, invariants, glibc) is listed in tests/test_oracle_*.py
+ * (test_oracle_pins.py: RNG, collision, streaming, walls, forcing lag;
+ * test_oracle_forcing.py: HIT, buoyancy and channel forcing; test_oracle_les.py,
+ * test_oracle_rulers.py, test_oracle_particles.py).  One boundary is left
+ * "parity unpinned": the libm routines (sin, pow, sqrt, exp, log) are
+ * glibc's, which the reference links as well; no reference test pins them.
+ *
+ * oracle_set_plant() switches in one deliberate mistake (PLANT_* below) so
+ * that the pinning tests can show that they fail on it; 0 (the default)
+ * is the restatement itself.
  *
  * Build: oracle/Makefile (gcc -O2 -std=c11 -ffp-contract=off -fopenmp).
  */
@@ -29,7 +37,7 @@
 /* drand48: POSIX 48-bit LCG, X <- (a X + c) mod 2^48, a = 0x5DEECE66D,
  * c = 0xB; srand48(s) sets X = (s << 16) | 0x330E.  Used by the reference
  * through myrand() (random.c:4-16, SYSTEM_RANDOM48 define.h:14).
- * Pinned against glibc drand48 in tests/test_oracle_rng.py.                */
+ * Pinned against glibc drand48 in tests/test_oracle_pins.py.               */
 void oracle_srand48(oracle *o, long seed)
 {
     o->x48 = ((((uint64_t)(uint32_t)seed) << 16) | 0x330Eu) & 0xFFFFFFFFFFFFull;
@@ -206,6 +214,20 @@ static void hydro_fields(oracle *o, char which)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Planted mistakes for the pinning tests (test infrastructure only). */
+enum {
+    PLANT_NONE = 0,
+    PLANT_HIT_AX_IN_FX = 1,       /* sin(k_x x) in F_x instead of sin(k_y y) */
+    PLANT_HIT_NO_REVERSION = 2,   /* phase walk without the -phi fac1 term */
+    PLANT_HIT_EXPONENT = 3,       /* pow(k^2, -2/3) instead of pow(k^2, -1/3) */
+    PLANT_BUOY_T_BOT = 4,         /* theta = t - T_bot instead of t - (T_bot + T_top)/2 */
+    PLANT_BUOY_SIGN = 5,          /* -beta_t theta g */
+    PLANT_CHANNEL_LX = 6,         /* channel F_x = 2 Amp_x^2 / LX */
+    PLANT_HIT_NOISE = 7,          /* phase walk noise sqrt(time_dt phi_diffusivity) (factor 2 dropped) */
+};
+static int plant = PLANT_NONE;
+void oracle_set_plant(int which) { plant = which; }
+
 /* build_forcing (forcing.c:5-1246), branches used on the streaming step:
  * HIT phase Ornstein-Uhlenbeck walk on the root (:40-69), per node force
  * reset (:361-364), Poiseuille (:366-371), channel (:374-379), HIT default
@@ -220,6 +242,8 @@ static void build_forcing(oracle *o)
         double phi_diffusivity = 0.01 / t_turnover;
         double fac1 = 1.0 / t_turnover;
         double fac2 = sqrt(2.0 * 1.0 * phi_diffusivity);
+        if (plant == PLANT_HIT_NO_REVERSION) fac1 = 0.0;
+        if (plant == PLANT_HIT_NOISE) fac2 = sqrt(1.0 * phi_diffusivity);
         for (int ii = 0; ii < o->nk; ii++) {
             double val = oracle_random_gauss(o, 0.0, 1.0);
             o->phi[ii].x += -o->phi[ii].x * fac1 + val * fac2;
@@ -243,18 +267,18 @@ static void build_forcing(oracle *o)
                     F.z += 2.0 * o->cfg.Amp_z * ((4. * nu) * pow(LY, -2.0));
                 }
                 if (HAS(o, CH4G_FLUID_FORCING_CHANNEL)) {
-                    F.x += 2.0 * pow(o->cfg.Amp_x, 2.0) / LY;
+                    F.x += 2.0 * pow(o->cfg.Amp_x, 2.0) / (plant == PLANT_CHANNEL_LX ? LX : LY);
                     F.y += 2.0 * pow(o->cfg.Amp_y, 2.0) / LX;
                     F.z += 2.0 * pow(o->cfg.Amp_z, 2.0) / LY;
                 }
                 if (HAS(o, CH4G_FLUID_FORCING_HIT)) {
                     for (int ii = 0; ii < o->nk; ii++) {
-                        double fac = pow(o->vk2[ii], -2. / 6.);
+                        double fac = pow(o->vk2[ii], plant == PLANT_HIT_EXPONENT ? -4. / 6. : -2. / 6.);
                         /* two_pi is `2*3.14159265359` (typedef.h:367) */
                         double ax = sin(2 * 3.14159265359 * (o->vk[ii].x * x / LX + o->phi[ii].x));
                         double ay = sin(2 * 3.14159265359 * (o->vk[ii].y * y / LY + o->phi[ii].y));
                         double az = sin(2 * 3.14159265359 * (o->vk[ii].z * z / LZ + o->phi[ii].z));
-                        F.x += fac * o->cfg.Amp_x * (ay + az);
+                        F.x += fac * o->cfg.Amp_x * ((plant == PLANT_HIT_AX_IN_FX ? ax : ay) + az);
                         F.y += fac * o->cfg.Amp_y * (ax + az);
                         F.z += fac * o->cfg.Amp_z * (ay + ax);
                     }
@@ -263,9 +287,11 @@ static void build_forcing(oracle *o)
                     double temp;
                     if (HAS(o, CH4G_TEMPERATURE_BUOYANCY_T0_REF))
                         temp = (o->t[n] - o->cfg.T_ref);
+                    else if (plant == PLANT_BUOY_T_BOT)
+                        temp = o->t[n] - o->cfg.T_bot;
                     else
                         temp = o->t[n] - 0.5 * (o->cfg.T_bot + o->cfg.T_top);
-                    double fac = o->cfg.beta_t * temp;
+                    double fac = (plant == PLANT_BUOY_SIGN ? -o->cfg.beta_t : o->cfg.beta_t) * temp;
                     if (o->cfg.beta2_t != 0.0) fac += o->cfg.beta2_t * temp * temp;
                     F.x += fac * o->cfg.gravity_x;
                     F.y += fac * o->cfg.gravity_y;

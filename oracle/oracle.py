@@ -53,6 +53,8 @@ def lib():
         L.oracle_set_particles.argtypes = [P, P, C.c_long]
         L.oracle_particles_probe.argtypes = [P, dp, dp]
         L.oracle_particles_move.argtypes = [P, dp, dp, C.c_int]
+        L.oracle_set_plant.argtypes = [C.c_int]
+        L.oracle_set_plant.restype = None
         _lib = L
     return _lib
 
@@ -165,6 +167,25 @@ class Oracle:
         r = [np.zeros(n, dtype=output_dtype()) for n in (self.cfg.NX, self.cfg.NY, self.cfg.NZ)]
         self._L.oracle_dump_rulers(self._h, C.byref(out), *[x.ctypes.data_as(C.c_void_p) for x in r])
         return (out.as_dict(),) + tuple(r)
+
+
+# mistakes that oracle_set_plant() can switch in (ch4_oracle.c, PLANT_*)
+PLANT = {"hit_ax_in_fx": 1, "hit_no_reversion": 2, "hit_exponent": 3, "buoy_t_bot": 4, "buoy_sign": 5,
+         "channel_lx": 6, "hit_noise": 7}
+
+
+class planted:
+    """Context manager: the oracle with one planted mistake (process-wide)."""
+
+    def __init__(self, name):
+        self.code = PLANT[name]
+
+    def __enter__(self):
+        lib().oracle_set_plant(self.code)
+        return self
+
+    def __exit__(self, *a):
+        lib().oracle_set_plant(0)
 
 
 class RNG:
