@@ -80,7 +80,8 @@ def gpu_leg(precision, steps, warmup, workload="rb2d"):
     spl = lat.steps_per_launch()
     out = {"nodes": nodes, "thermal": cfg.has(1 << 1), "wall_s": wall, "event_ms": ms, "mlups": nodes * steps / wall / 1e6,
            "us_per_step": ms * 1e3 / steps, "steps_per_launch": spl, "avg_launch_us": ms * 1e3 / steps * spl,
-           "kernel": "k_step_fp64" if precision == 0 else ("k_step2_fp32" if spl == 2 else "k_step_fp32")}
+           "kernel": "k_step_fp64" if precision == 0 else (
+               ("k_step2z_fp32" if cfg.NZ > 1 else "k_step2_fp32") if spl == 2 else "k_step_fp32")}
     h = lat.dump_averages()
     out["nusselt"] = h["nuy"]
     lat.close()

@@ -123,7 +123,8 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
         if (steps[i] < 1) return fail(-1, "steps per chunk must be >= 1");
         B.chunk = std::max(B.chunk, steps[i]);
         ch4g_lattice *L = lats[i];
-        B.two[i] = two_step_eligible(L);
+        // the batched two-step kernel takes 2-D lattices only
+        B.two[i] = two_step_eligible(L) && L->cfg.NZ == 1;
         if (B.two[i] && (rc = ensure_tlag_alt(L))) return rc;
         // two-step lattices: the population buffer flips with every launch,
         // the lag array with every two-step launch

@@ -121,8 +121,8 @@ __device__ __forceinline__ void gather_f32(const KParams &P, const Row &R, const
             const float *base = in + (size_t)(cross ? invq(q) : q) * N;
             const uint32_t off = cross ? n : src_index<q>(b, P.NX, P.NY);
             float v = ldp<NTL>(base + off);
-            if constexpr (cyq(q) > 0) v += cross ? 0.f : wf(q) * R.dr_up;
-            if constexpr (cyq(q) < 0) v += cross ? 0.f : wf(q) * R.dr_dn;
+            if constexpr (cyq(q) > 0) v = fmaf(wf(q), cross ? 0.f : R.dr_up, v);
+            if constexpr (cyq(q) < 0) v = fmaf(wf(q), cross ? 0.f : R.dr_dn, v);
             f[q] = v;
         }
     });
@@ -144,8 +144,8 @@ __device__ __forceinline__ void gather_g32(const KParams &P, const Row &R, const
             const bool cross = P.wall_t && crosses_wall<q>(b);
             const uint32_t off = cross ? n : src_index<q>(b, P.NX, P.NY);
             float v = ldp<NTL>(in + (size_t)q * N + off);
-            if constexpr (cyq(q) > 0) v += cross ? wf(q) * fac_bot : wf(q) * R.dt_up;
-            if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
+            if constexpr (cyq(q) > 0) v = fmaf(wf(q), cross ? fac_bot : R.dt_up, v);
+            if constexpr (cyq(q) < 0) v = fmaf(wf(q), cross ? fac_top : R.dt_dn, v);
             g[q] = v;
         }
     });
@@ -170,8 +170,8 @@ __device__ __forceinline__ void gather_f32_o32(const KParams &P, const Row &R, c
         const bool cross = P.wall_f && crosses_wall<q>(b);
         const uint32_t off = cross ? (uint32_t)invq(q) * S4 + n * 4u : (uint32_t)q * S4 + src_index<q>(b, P.NX, P.NY) * 4u;
         float v = ldo<NTL>(in, off);
-        if constexpr (cyq(q) > 0) v += cross ? 0.f : wf(q) * R.dr_up;
-        if constexpr (cyq(q) < 0) v += cross ? 0.f : wf(q) * R.dr_dn;
+        if constexpr (cyq(q) > 0) v = fmaf(wf(q), cross ? 0.f : R.dr_up, v);
+        if constexpr (cyq(q) < 0) v = fmaf(wf(q), cross ? 0.f : R.dr_dn, v);
         f[q] = v;
     });
 }
@@ -188,8 +188,8 @@ __device__ __forceinline__ void gather_g32_o32(const KParams &P, const Row &R, c
         const bool cross = P.wall_t && crosses_wall<q>(b);
         const uint32_t off = (uint32_t)q * S4 + (cross ? n : src_index<q>(b, P.NX, P.NY)) * 4u;
         float v = ldo<NTL>(in, off);
-        if constexpr (cyq(q) > 0) v += cross ? wf(q) * fac_bot : wf(q) * R.dt_up;
-        if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
+        if constexpr (cyq(q) > 0) v = fmaf(wf(q), cross ? fac_bot : R.dt_up, v);
+        if constexpr (cyq(q) < 0) v = fmaf(wf(q), cross ? fac_top : R.dt_dn, v);
         g[q] = v;
     });
 }
@@ -631,8 +631,8 @@ __device__ __forceinline__ void gather_f_tile(const KParams &P, const Row &R, co
         constexpr int q = decltype(qc)::value;
         const bool cross = P.wall_f && crosses_wall<q>(b);
         float v = cross ? t[invq(q) * M + at] : t[q * M + at - cxq(q) - cyq(q) * T2_W];
-        if constexpr (cyq(q) > 0) v += cross ? 0.f : wf(q) * R.dr_up;
-        if constexpr (cyq(q) < 0) v += cross ? 0.f : wf(q) * R.dr_dn;
+        if constexpr (cyq(q) > 0) v = fmaf(wf(q), cross ? 0.f : R.dr_up, v);
+        if constexpr (cyq(q) < 0) v = fmaf(wf(q), cross ? 0.f : R.dr_dn, v);
         f[q] = v;
     });
 }
@@ -647,8 +647,8 @@ __device__ __forceinline__ void gather_g_tile(const KParams &P, const Row &R, co
         constexpr int q = decltype(qc)::value;
         const bool cross = P.wall_t && crosses_wall<q>(b);
         float v = t[(gofs + q) * M + (cross ? at : at - cxq(q) - cyq(q) * T2_W)];
-        if constexpr (cyq(q) > 0) v += cross ? wf(q) * fac_bot : wf(q) * R.dt_up;
-        if constexpr (cyq(q) < 0) v += cross ? wf(q) * fac_top : wf(q) * R.dt_dn;
+        if constexpr (cyq(q) > 0) v = fmaf(wf(q), cross ? fac_bot : R.dt_up, v);
+        if constexpr (cyq(q) < 0) v = fmaf(wf(q), cross ? fac_top : R.dt_dn, v);
         g[q] = v;
     });
 }
@@ -746,6 +746,460 @@ __global__ void __launch_bounds__(Tile2<TY>::THREADS) k_step2_fp32(KParams P, co
     __shared__ float t[tile2_planes<FLUID, THERMAL>() * Tile2<TY>::M];
     step2_tile<FLUID, THERMAL, A, NTL, TY>(P, fin, gin, fout, gout, tl_in, tl_out, blockIdx.x * T2_TX,
                                            blockIdx.y * TY, t);
+}
+
+// ---------------------------------------------------------------------------
+// Two steps per launch for 3-D lattices (NZ > 1), marching along z.  A block
+// owns a 64 x TY column of x-y tiles over a range of z planes and has one
+// thread per node of the tile with its one-node x-y halo.  Iteration j:
+//   A  step 1 at plane j for the tile + halo nodes (gather from the global
+//      state, collide into registers); the populations with c_z = -1 go to
+//      the LDS ring M;
+//   B  step 2 at plane j - 1 for the tile nodes, gathering from LDS: c_z = +1
+//      from plane j - 2 (ring P), c_z = 0 from plane j - 1 (ring Z), c_z = -1
+//      from plane j (ring M); collide into the global output;
+//   C  the registers of step A go to ring Z (c_z = 0 and the step-1
+//      temperature lag) and ring P (c_z = +1, two slots by plane parity);
+//      the tile nodes also keep their populations 16 and 18 (c_z = -1) in
+//      ring W, which the wall rules of step B read at the node itself one
+//      plane later (fluid bounce-back at y = 0 takes f_16, f_18 there, the
+//      fixed-T rule at y = NY - 1 takes g_16, g_18).
+// Planes just below and above the block's range are computed in step A
+// (z-halo) but not written.  Per two steps the state is read (1 + 2/ZC)
+// times over the tile + halo area (x-y halo: 396 / 256 nodes for TY = 4) and
+// written once; the temperature lag is read from tl_in and written to tl_out
+// (halo nodes of one block are tile nodes of another).
+namespace z2 {
+// index of population q within its c_z class: c_z = 0: {0,1,2,3,4,7,8,9,10},
+// c_z = +1: {5,11,12,15,17}, c_z = -1: {6,13,14,16,18}
+__host__ __device__ constexpr int slot(int q)
+{
+    constexpr int v[Q] = {0, 1, 2, 3, 4, 0, 0, 5, 6, 7, 8, 1, 2, 1, 2, 3, 3, 4, 4};
+    return v[q];
+}
+// ring W index of population 16 / 18
+__host__ __device__ constexpr int wslot(int q) { return q == 16 ? 0 : 1; }
+}  // namespace z2
+
+// LDS rings of a z-marching tile of TX x TY nodes (one-node x-y halo).
+// HOLD: step A keeps its c_z >= 0 results in registers until step C (one
+// slot of ring Z, two of ring P, one of ring W); otherwise step A writes
+// every result at once into rings with one more slot each (two of Z, three
+// of P, two of W).
+template <bool FLUID, bool THERMAL, int TX_, int TY_, bool HOLD_>
+struct TileZ {
+    static constexpr int TX = TX_, TY = TY_, W = TX + 2;
+    static constexpr bool HOLD = HOLD_;
+    static constexpr int M = W * (TY + 2);            // tile + halo nodes
+    static constexpr int T = TX * TY;                 // tile nodes
+    static constexpr int THREADS = (M + 63) / 64 * 64;
+    static constexpr int F = FLUID ? 1 : 0, H = THERMAL ? 1 : 0;
+    static constexpr int KM = 5 * F + 5 * H;          // ring M planes
+    static constexpr int KZ = 9 * F + 9 * H + H;      // planes of one ring Z slot (+ lag)
+    static constexpr int KP = 5 * F + 5 * H;          // planes of one ring P slot
+    static constexpr int KW = 2 * F + 2 * H;          // planes of one ring W slot (tile nodes)
+    static constexpr int NZS = HOLD ? 1 : 2, NPS = HOLD ? 2 : 3, NWS = HOLD ? 1 : 2;
+    static constexpr int OZ = KM, OP = KM + NZS * KZ, OW = (KM + NZS * KZ + NPS * KP) * M;
+    static constexpr int FLOATS = OW + NWS * KW * T;
+    static_assert(THREADS >= T && THREADS <= 1024, "z-marching block");
+    // plane of population q (fluid: gq = 0, temperature: gq = 1) in its ring
+    static __device__ constexpr int pm(int q, int gq) { return gq * 5 * F + z2::slot(q); }
+    static __device__ constexpr int pz(int q, int gq, int s) { return OZ + s * KZ + gq * 9 * F + z2::slot(q); }
+    static __device__ constexpr int lag(int s) { return OZ + s * KZ + 9 * F + 9 * H; }
+    static __device__ constexpr int pp(int q, int gq, int s) { return OP + s * KP + gq * 5 * F + z2::slot(q); }
+    static __device__ constexpr int pw(int q, int gq, int s) { return s * KW + gq * 2 * F + z2::wslot(q); }
+};
+
+// step-1 output of one node, kept in registers until it goes to the rings
+struct RegOut {
+    float *f, *g, *tl;
+    __device__ void put_f(int q, float v) const { f[q] = v; }
+    __device__ void put_g(int q, float v) const { g[q] = v; }
+    __device__ void put_tlag(float v) const { *tl = v; }
+    __device__ float get_tau() const { return 0.f; }  // no LES on the two-step path
+    __device__ void put_tau(float) const {}
+};
+
+// Ring slots read by step B at iteration it (plane p = it - 1 of the
+// iteration count): ring Z and W of plane p, ring P of plane p and p - 1.
+template <class TZ>
+struct RingSlots {
+    int z, w, p_own, p_below;
+    __device__ RingSlots(int it)
+    {
+        if constexpr (TZ::HOLD) {
+            z = 0;
+            w = 0;
+            p_own = (it - 1) & 1;
+            p_below = it & 1;
+        } else {
+            z = (it - 1) & 1;
+            w = (it - 1) & 1;
+            p_own = (it - 1) % 3;
+            p_below = (it + 1) % 3;  // (it - 2) mod 3
+        }
+    }
+};
+
+// Step-2 gather of population q (gq = 0 fluid, 1 temperature) at tile node
+// `at` (tile + halo index) / `tn` (tile index) from the rings.
+template <class TZ, int q, int gq>
+__device__ __forceinline__ float ring_src(const float *t, int at, int tn, const RingSlots<TZ> &rs, bool cross)
+{
+    constexpr int M = TZ::M;
+    float v;
+    // pull source in the streaming direction
+    const int src = at - cxq(q) - cyq(q) * TZ::W;
+    if constexpr (czq(q) > 0)
+        v = t[TZ::pp(q, gq, rs.p_below) * M + src];
+    else if constexpr (czq(q) < 0)
+        v = t[TZ::pm(q, gq) * M + src];
+    else
+        v = t[TZ::pz(q, gq, rs.z) * M + src];
+    if constexpr (cyq(q) != 0) {
+        // wall rules: fluid bounce-back takes the opposite population at the
+        // node itself, the fixed-T rule the same population there
+        constexpr int qs = gq == 0 ? invq(q) : q;
+        if (cross) {
+            if constexpr (czq(qs) > 0)
+                v = t[TZ::pp(qs, gq, rs.p_own) * M + at];
+            else if constexpr (czq(qs) < 0)
+                v = t[TZ::OW + TZ::pw(qs, gq, rs.w) * TZ::T + tn];
+            else
+                v = t[TZ::pz(qs, gq, rs.z) * M + at];
+        }
+    }
+    return v;
+}
+
+// Step A's loads of one node (load_raw_buf) and the row and wall terms
+// applied to them afterwards (finish_raw): together the same values as
+// gather_f32 / gather_g32.
+struct RawNode {
+    float f[Q], g[Q], tl;
+};
+
+// The Q planes of a population array as three buffer resources of up to
+// seven planes each: a plane is addressed by the resource of its group, a
+// scalar offset of its place in the group and the node's 32-bit byte offset
+// (planes of less than 4 GiB / 7).  Keeps the scalar registers free of 38
+// 64-bit plane addresses and the vector registers of 64-bit lane addresses.
+struct Planes {
+    __amdgpu_buffer_rsrc_t r[3];
+    uint32_t pb;  // plane stride in bytes
+};
+
+__device__ __forceinline__ Planes make_planes(const float *base, int64_t S)
+{
+    Planes p;
+    p.pb = (uint32_t)(S * 4);
+    static_for<0, 3>([&](auto gc) {
+        constexpr int gi = decltype(gc)::value;
+        constexpr int np = gi < 2 ? 7 : Q - 14;
+        p.r[gi] = __builtin_amdgcn_make_buffer_rsrc((void *)(base + (size_t)gi * 7 * (size_t)S), (short)0,
+                                                    (int)(uint32_t)((uint64_t)np * p.pb), 0x00020000);
+    });
+    return p;
+}
+
+// cache-policy bits of buffer instructions: non-temporal
+constexpr int CPOL_NT = 2;
+
+template <int q, bool NT>
+__device__ __forceinline__ float bload(const Planes &p, uint32_t soff, uint32_t voff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(p.r[q / 7], (int)voff, (int)soff,
+                                                                          NT ? CPOL_NT : 0));
+}
+
+template <bool FLUID, bool THERMAL, bool NTL>
+__device__ __forceinline__ void load_raw_buf(const KParams &P, const Nbr &b, const Planes &pf, const Planes &pg,
+                                             const float *__restrict__ tl_in, uint32_t n, RawNode &r)
+{
+    const uint32_t n4 = n * 4u;
+    if constexpr (THERMAL) {
+        r.tl = tl_in[n];
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const bool cross = P.wall_t && crosses_wall<q>(b);
+            r.g[q] = bload<q, NTL>(pg, (uint32_t)(q % 7) * pg.pb, cross ? n4 : src_index<q>(b, P.NX, P.NY) * 4u);
+        });
+    }
+    if constexpr (FLUID) {
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (cyq(q) == 0) {
+                r.f[q] = bload<q, NTL>(pf, (uint32_t)(q % 7) * pf.pb, src_index<q>(b, P.NX, P.NY) * 4u);
+            } else {
+                // bounce-back reads the opposite plane (same group of seven)
+                constexpr int iq = q % 7, ii = invq(q) % 7, mn = iq < ii ? iq : ii;
+                static_assert(q / 7 == invq(q) / 7, "opposite planes share a resource");
+                const bool cross = P.wall_f && crosses_wall<q>(b);
+                const uint32_t voff = cross ? n4 + (uint32_t)(ii - mn) * pf.pb
+                                            : src_index<q>(b, P.NX, P.NY) * 4u + (uint32_t)(iq - mn) * pf.pb;
+                r.f[q] = bload<q, NTL>(pf, (uint32_t)mn * pf.pb, voff);
+            }
+        });
+    }
+}
+
+// NodeOut through buffer resources (non-temporal stores)
+struct BufOut {
+    Planes f, g;
+    float *__restrict__ tlag;
+    uint32_t n, n4;
+    template <int q>
+    __device__ void st(const Planes &p, float v) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), p.r[q / 7], (int)n4,
+                                              (int)((uint32_t)(q % 7) * p.pb), CPOL_NT);
+    }
+    __device__ void put_f(int q, float v) const
+    {
+        static_for<0, Q>([&](auto qc) {
+            constexpr int qq = decltype(qc)::value;
+            if (qq == q) st<qq>(f, v);
+        });
+    }
+    __device__ void put_g(int q, float v) const
+    {
+        static_for<0, Q>([&](auto qc) {
+            constexpr int qq = decltype(qc)::value;
+            if (qq == q) st<qq>(g, v);
+        });
+    }
+    __device__ void put_tlag(float v) const { tlag[n] = v; }
+    __device__ float get_tau() const { return 0.f; }  // no LES on the two-step path
+    __device__ void put_tau(float) const {}
+};
+
+template <bool FLUID, bool THERMAL>
+__device__ __forceinline__ void finish_raw(const KParams &P, const Row &R, const Nbr &b, RawNode &r)
+{
+    if constexpr (THERMAL) {
+        const float fac_top = 2.f * (P.cw_top_f - r.tl);
+        const float fac_bot = 2.f * (P.cw_bot_f - r.tl);
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const bool cross = P.wall_t && crosses_wall<q>(b);
+            if constexpr (cyq(q) > 0) r.g[q] = fmaf(wf(q), cross ? fac_bot : R.dt_up, r.g[q]);
+            if constexpr (cyq(q) < 0) r.g[q] = fmaf(wf(q), cross ? fac_top : R.dt_dn, r.g[q]);
+        });
+    }
+    if constexpr (FLUID) {
+        static_for<0, Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const bool cross = P.wall_f && crosses_wall<q>(b);
+            if constexpr (cyq(q) > 0) r.f[q] = fmaf(wf(q), cross ? 0.f : R.dr_up, r.f[q]);
+            if constexpr (cyq(q) < 0) r.f[q] = fmaf(wf(q), cross ? 0.f : R.dr_dn, r.f[q]);
+        });
+    }
+}
+
+// Step A / C: the results of plane j (iteration it) of node m into the
+// rings: c_z = -1 into ring M (always), the others when `rest`.
+template <class TZ, bool FLUID, bool THERMAL>
+__device__ __forceinline__ void ring_put(float *t, int m, int it, const float rf[Q], const float rg[Q], float rt,
+                                         bool tile, int tn, bool minus, bool rest)
+{
+    constexpr int M = TZ::M;
+    const int sz = TZ::HOLD ? 0 : (it & 1), sw = sz;
+    const int sp = TZ::HOLD ? (it & 1) : (it % 3);
+    static_for<0, Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (czq(q) < 0) {
+            if (minus) {
+                if constexpr (FLUID) t[TZ::pm(q, 0) * M + m] = rf[q];
+                if constexpr (THERMAL) t[TZ::pm(q, 1) * M + m] = rg[q];
+            }
+        } else if constexpr (czq(q) == 0) {
+            if (rest) {
+                if constexpr (FLUID) t[TZ::pz(q, 0, sz) * M + m] = rf[q];
+                if constexpr (THERMAL) t[TZ::pz(q, 1, sz) * M + m] = rg[q];
+            }
+        } else {
+            if (rest) {
+                if constexpr (FLUID) t[TZ::pp(q, 0, sp) * M + m] = rf[q];
+                if constexpr (THERMAL) t[TZ::pp(q, 1, sp) * M + m] = rg[q];
+            }
+        }
+    });
+    if (rest) {
+        if constexpr (THERMAL) t[TZ::lag(sz) * M + m] = rt;
+        if (tile) {
+            if constexpr (FLUID) {
+                t[TZ::OW + TZ::pw(16, 0, sw) * TZ::T + tn] = rf[16];
+                t[TZ::OW + TZ::pw(18, 0, sw) * TZ::T + tn] = rf[18];
+            }
+            if constexpr (THERMAL) {
+                t[TZ::OW + TZ::pw(16, 1, sw) * TZ::T + tn] = rg[16];
+                t[TZ::OW + TZ::pw(18, 1, sw) * TZ::T + tn] = rg[18];
+            }
+        }
+    }
+}
+
+// Two steps per launch for 3-D lattices (NZ > 1), marching along z.  A block
+// owns a TX x TY column of x-y tiles over a range of zc planes and has one
+// thread per node of the tile with its one-node x-y halo.  Iteration j:
+//   A  step 1 at plane j for the tile + halo nodes (gather from the global
+//      state, collision);
+//   B  step 2 at plane j - 1 for the tile nodes, gathering from LDS: c_z = +1
+//      from plane j - 2 (ring P), c_z = 0 from plane j - 1 (ring Z), c_z = -1
+//      from plane j (ring M); collision into the global output.
+// Ring Z also holds the step-1 temperature (the lag of step 2); ring W keeps
+// the tile nodes' populations 16 and 18 (c_z = -1), which the wall rules of
+// step B read at the node itself one plane later (fluid bounce-back at y = 0
+// takes f_16, f_18 there... see ring_src).  With HOLD the c_z >= 0 results of
+// step A wait in registers and go to the rings after step B (C); without it
+// step A writes them at once into the slots step B does not read.  The planes
+// just below and above the block's range are computed in step A (z halo) but
+// not written.  Per two steps the state is read (1 + 2/zc) x M/T times and
+// written once; the temperature lag is read from tl_in and written to tl_out
+// (halo nodes of one block are tile nodes of another).
+template <bool FLUID, bool THERMAL, typename A, bool NTL, class TZ>
+__global__ void __launch_bounds__(TZ::THREADS) k_step2z_fp32(KParams P, const float *__restrict__ fin,
+                                                             const float *__restrict__ gin, float *__restrict__ fout,
+                                                             float *__restrict__ gout,
+                                                             const float *__restrict__ tl_in,
+                                                             float *__restrict__ tl_out, int zc, int remap)
+{
+    constexpr int M = TZ::M, W = TZ::W, TX = TZ::TX, TY = TZ::TY;
+    __shared__ float t[TZ::FLOATS];
+    const int NX = P.NX, NY = P.NY, NZ = P.NZ;
+    // block -> (x tile, y tile, z range); with remap, the blocks that one XCD
+    // receives (every 8th in dispatch order) cover consecutive tiles, so
+    // neighbouring tiles share their halo rows in that XCD's L2
+    const uint32_t nbx = gridDim.x, nby = gridDim.y, total = nbx * nby * gridDim.z;
+    uint32_t bid = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    if (remap && total % 8u == 0u) bid = (bid % 8u) * (total / 8u) + bid / 8u;
+    const int x0 = (int)(bid % nbx) * TX, y0 = (int)((bid / nbx) % nby) * TY;
+    const int zb = (int)(bid / (nbx * nby)) * zc, ze = min(zb + zc, NZ);
+    const int w = NX - x0 < TX ? NX - x0 : TX;
+    // step A node of this thread (tile + halo)
+    const int m = threadIdx.x, lx = m % W, ly = m / W;
+    const bool act_a = m < M && lx <= w + 1;
+    int xa = x0 - 1 + lx;
+    xa = xa < 0 ? xa + NX : (xa >= NX ? xa - NX : xa);
+    int ya = (y0 - 1 + ly) % NY;
+    ya = ya < 0 ? ya + NY : ya;
+    const bool tile_a = act_a && lx >= 1 && lx <= w && ly >= 1 && ly <= TY;
+    const int tn_a = (ly - 1) * TX + (lx - 1);
+    // step B node of this thread (tile)
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+    const int xb = x0 + tx, yb = y0 + ty;
+    const bool act_b = ty < TY && tx < w && yb < NY;
+    const int at = (ty + 1) * W + tx + 1, tn = ty * TX + tx;
+
+    // plane of step A at iteration it (periodic in z) and its node
+    auto node_a = [&](int it_, uint32_t &n, Nbr &b) {
+        const int j = zb - 1 + it_;
+        const int za = j < 0 ? j + NZ : (j >= NZ ? j - NZ : j);
+        n = ((uint32_t)za * NY + ya) * NX + xa;
+        b = make_nbr(xa, ya, za, NX, NY, NZ);
+    };
+    const int nit = ze - zb + 2;
+    Planes pin_f, pin_g, pout_f, pout_g;
+    if constexpr (FLUID) {
+        pin_f = make_planes(fin, P.S);
+        pout_f = make_planes(fout, P.S);
+    }
+    if constexpr (THERMAL) {
+        pin_g = make_planes(gin, P.S);
+        pout_g = make_planes(gout, P.S);
+    }
+    for (int it = 0; it < nit; it++) {
+        const int j = zb - 1 + it;
+        float rf[Q], rg[Q], rt = 0.f;
+        // ---- A: step 1 at plane j
+        if (act_a) {
+            uint32_t n;
+            Nbr b;
+            node_a(it, n, b);
+            const Row R = row_of(P, ya);
+            RawNode cur;
+            load_raw_buf<FLUID, THERMAL, NTL>(P, b, pin_f, pin_g, tl_in, n, cur);
+            finish_raw<FLUID, THERMAL>(P, R, b, cur);
+            const float dTlag = THERMAL ? cur.tl : 0.f;
+            A dT = A(0);
+            if constexpr (THERMAL) {
+                if constexpr (sizeof(A) == 4) {
+                    dT = sum19(cur.g);
+                } else {
+                    for (int q = 0; q < Q; q++) dT += (A)cur.g[q];
+                }
+            }
+            A ux = 0, uy = 0, uz = 0, tau_les = A(0);
+            const RegOut out{rf, rg, &rt};
+            const int za = b.z;
+            if constexpr (FLUID)
+                collide_fluid<A, false>(P, R, xa, ya, za, n, cur.f, dTlag, dT, out, ux, uy, uz, tau_les);
+            if constexpr (THERMAL) collide_thermal<A, false>(P, R, ya, cur.g, dT, ux, uy, uz, tau_les, out);
+            ring_put<TZ, FLUID, THERMAL>(t, m, it, rf, rg, rt, tile_a, tn_a, true, !TZ::HOLD);
+        }
+        __syncthreads();
+        // ---- B: step 2 at plane j - 1 (planes of the block's range only)
+        const int p = j - 1;
+        if (act_b && p >= zb) {
+            const RingSlots<TZ> rs(it);
+            const uint32_t n = ((uint32_t)p * NY + yb) * NX + xb;
+            const Row R = row_of(P, yb);
+            const Nbr b = make_nbr(xb, yb, p, NX, NY, NZ);
+            float dTlag = 0.f;
+            if constexpr (THERMAL) dTlag = t[TZ::lag(rs.z) * M + at];
+            // temperature populations from the rings (read twice: for the
+            // temperature the fluid's force needs, then for their collision,
+            // so they do not occupy registers during the fluid's)
+            auto gather_g = [&](float g[Q]) {
+                const float fac_top = 2.f * (P.cw_top_f - dTlag);
+                const float fac_bot = 2.f * (P.cw_bot_f - dTlag);
+                static_for<0, Q>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    const bool cross = P.wall_t && crosses_wall<q>(b);
+                    float v = ring_src<TZ, q, 1>(t, at, tn, rs, cross);
+                    if constexpr (cyq(q) > 0) v = fmaf(wf(q), cross ? fac_bot : R.dt_up, v);
+                    if constexpr (cyq(q) < 0) v = fmaf(wf(q), cross ? fac_top : R.dt_dn, v);
+                    g[q] = v;
+                });
+            };
+            A dT = A(0);
+            if constexpr (THERMAL) {
+                float g[Q];
+                gather_g(g);
+                if constexpr (sizeof(A) == 4) {
+                    dT = sum19(g);
+                } else {
+                    for (int q = 0; q < Q; q++) dT += (A)g[q];
+                }
+            }
+            A ux = 0, uy = 0, uz = 0, tau_les = A(0);
+            auto collide2 = [&](const auto &out) {
+            if constexpr (FLUID) {
+                float f[Q];
+                static_for<0, Q>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    const bool cross = P.wall_f && crosses_wall<q>(b);
+                    float v = ring_src<TZ, q, 0>(t, at, tn, rs, cross);
+                    if constexpr (cyq(q) > 0) v = fmaf(wf(q), cross ? 0.f : R.dr_up, v);
+                    if constexpr (cyq(q) < 0) v = fmaf(wf(q), cross ? 0.f : R.dr_dn, v);
+                    f[q] = v;
+                });
+                collide_fluid<A, false>(P, R, xb, yb, p, n, f, dTlag, dT, out, ux, uy, uz, tau_les);
+            }
+            if constexpr (THERMAL) {
+                float g[Q];
+                gather_g(g);
+                collide_thermal<A, false>(P, R, yb, g, dT, ux, uy, uz, tau_les, out);
+            }
+            };
+            collide2(BufOut{pout_f, pout_g, tl_out, n, n * 4u});
+        }
+        __syncthreads();
+        // ---- C (HOLD): the c_z >= 0 results of plane j into rings Z, P, W
+        if constexpr (TZ::HOLD) {
+            if (act_a) ring_put<TZ, FLUID, THERMAL>(t, m, it, rf, rg, rt, tile_a, tn_a, false, true);
+        }
+    }
 }
 
 // Many 2-D lattices in one launch, two steps each: block b advances one
@@ -1007,14 +1461,88 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 // 1500x700 76.3 -> 48.2 (against the dense single-step grid).
 bool two_step_ok(const KParams &P)
 {
-    // 32-bit byte offsets into the Q population planes
+    // 2-D: 32-bit byte offsets into the Q population planes; 3-D: 32-bit
+    // byte offsets within a group of seven planes
     const bool off32 = (int64_t)Q * P.S * 4 < ((int64_t)1 << 32);
-    return P.NZ == 1 && off32 && !P.first && !P.hit && !P.les && !P.twoway;
+    const bool fits = P.NZ == 1 ? off32 : 7 * P.S * 4 < ((int64_t)1 << 32);
+    return fits && !P.first && !P.hit && !P.les && !P.twoway;
 }
+
+namespace {
+
+// Two steps per launch for 3-D lattices (k_step2z_fp32), z ranges of zc
+// planes.
+template <class TZ>
+void launch_z2(const KParams &P, const float *fin, const float *gin, float *fout, float *gout, const float *tl_in,
+               float *tl_out, int zc, int remap, bool ntl, hipStream_t s)
+{
+    const unsigned bx = (unsigned)((P.NX + TZ::TX - 1) / TZ::TX), by = (unsigned)((P.NY + TZ::TY - 1) / TZ::TY);
+    const dim3 grid(bx, by, (unsigned)((P.NZ + zc - 1) / zc));
+#define CH4G_Z2N_(FL_, TH_, A_, N_)                                                                                   \
+    hipLaunchKernelGGL((k_step2z_fp32<FL_, TH_, A_, N_, TZ>), grid, dim3(TZ::THREADS), 0, s, P, fin, gin, fout,     \
+                       gout, tl_in, tl_out, zc, remap)
+#define CH4G_Z2_(FL_, TH_, A_)                                                                                        \
+    do {                                                                                                              \
+        if (ntl)                                                                                                      \
+            CH4G_Z2N_(FL_, TH_, A_, true);                                                                            \
+        else                                                                                                          \
+            CH4G_Z2N_(FL_, TH_, A_, false);                                                                           \
+    } while (0)
+    constexpr bool FL = TZ::F == 1, TH = TZ::H == 1;
+    if (P.arith64)
+        CH4G_Z2_(FL, TH, double);
+    else
+        CH4G_Z2_(FL, TH, float);
+#undef CH4G_Z2_
+#undef CH4G_Z2N_
+}
+
+template <bool FL, bool TH, int TX, int TY, bool HOLD>
+using TZ_ = TileZ<FL, TH, TX, TY, HOLD>;
+
+// Tiles (measured at 512^3 fluid + T, FP32-deviation, ms per step, single
+// steps 7.9): 64 x 4 held in registers 6.9, with the next plane's loads in
+// flight during step B 6.5, 32 x 5 all in LDS (two blocks per CU) 7.3,
+// 64 x 8 held 6.0, the same through buffer resources 5.6-5.9; 3-D channel
+// 256^3: single 0.46, 64 x 8 held 0.35.  Double arithmetic needs more
+// registers than an 11-wave block allows: 64 x 4.  z ranges: at least 1024
+// blocks (four per CU) and at most 128 planes (channel 192^3 with ranges of
+// 128 planes: 144 blocks, 0.30 ms; of 32 planes: 0.18 ms).
+void launch_two_steps_3d(const KParams &P, const float *fin, const float *gin, float *fout, float *gout,
+                         const float *tl_in, float *tl_out, hipStream_t s)
+{
+    const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
+    const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
+    const bool tall = P.fluid && !(P.thermal && P.arith64);
+    const int TY = tall ? 8 : 4;
+    const int64_t columns = (int64_t)((P.NX + 63) / 64) * ((P.NY + TY - 1) / TY);
+    int64_t zc = (int64_t)P.NZ * columns / 1024;
+    zc = zc < 8 ? 8 : (zc > 128 ? 128 : zc);
+    zc = zc > P.NZ ? P.NZ : zc;
+    const int z = (int)zc, remap = 1;
+#define Z2(...) launch_z2<TZ_<__VA_ARGS__>>(P, fin, gin, fout, gout, tl_in, tl_out, z, remap, ntl, s)
+    if (P.fluid && P.thermal) {
+        if (P.arith64)
+            Z2(true, true, 64, 4, true);
+        else
+            Z2(true, true, 64, 8, true);
+    } else if (P.fluid) {
+        Z2(true, false, 64, 8, true);
+    } else {
+        Z2(false, true, 64, 4, false);
+    }
+#undef Z2
+}
+
+}  // namespace
 
 void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin, float *fout, float *gout,
                            const float *tl_in, float *tl_out, hipStream_t s)
 {
+    if (P.NZ > 1) {
+        launch_two_steps_3d(P, fin, gin, fout, gout, tl_in, tl_out, s);
+        return;
+    }
     // non-temporal loads while both lattices fit the last-level cache (as the
     // single-step kernel); 64x5 tiles (fewer halo nodes) once there are at
     // least two rounds of blocks (two blocks per CU by LDS), else 64x4 tiles
