@@ -23,7 +23,7 @@ EXPORTS = [
     "ch4g_set_populations", "ch4g_get_populations", "ch4g_step", "ch4g_hydro_fields", "ch4g_dump_averages",
     "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_dump_rulers", "ch4g_step_timed", "ch4g_steps_per_launch", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
     "ch4g_footprint_bytes", "ch4g_particles_init", "ch4g_particles_count", "ch4g_get_particles",
-    "ch4g_set_particles", "ch4g_step_batch", "ch4g_serve",
+    "ch4g_set_particles", "ch4g_step_batch", "ch4g_serve", "ch4g_dump_averages_noalloc",
 ]
 
 
@@ -52,6 +52,7 @@ def lib():
     L.ch4g_step.argtypes = [P, C.c_int]
     L.ch4g_hydro_fields.argtypes = [P, dp, dp, dp]
     L.ch4g_dump_averages.argtypes = [P, C.POINTER(Output)]
+    L.ch4g_dump_averages_noalloc.argtypes = [P, P, P, P, C.POINTER(Output)]
     L.ch4g_dump_rulers.argtypes = [P, C.c_void_p, C.c_void_p, C.c_void_p]
     L.ch4g_get_hit_phases.argtypes = [P, dp]
     L.ch4g_set_hit_phases.argtypes = [P, dp]
@@ -199,6 +200,14 @@ class Lattice:
     def dump_averages(self):
         out = Output()
         check(self._L.ch4g_dump_averages(self._h, C.byref(out)))
+        return out.as_dict()
+
+    def dump_averages_noalloc(self, dens_ptr, u_ptr, t_ptr):
+        """ch4g_dump_averages_noalloc: the caller's device buffers (integer
+        device addresses, or None) receive the FP64 fields dens[N], u[3N],
+        t[N]; the library allocates nothing."""
+        out = Output()
+        check(self._L.ch4g_dump_averages_noalloc(self._h, dens_ptr, u_ptr, t_ptr, C.byref(out)))
         return out.as_dict()
 
     def dump_rulers(self):

@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <utility>
 #include <vector>
@@ -881,6 +882,75 @@ int ch4g_dump_averages(ch4g_lattice *L, ch4g_output *out)
     HIPCHK(hipGetLastError());
     double a[NOUT];
     HIPCHK(hipMemcpyAsync(a, L->d_res, sizeof a, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    const ch4g_config &c = L->cfg;
+    const double norm = 1.0 / ((double)c.NX * (double)c.NY * (double)c.NZ);
+    ch4g_output o;
+    memcpy(&o, a, sizeof o);
+    normalise(L, o, norm, norm);
+    *out = o;
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Block partials of the averages reduction (reduce_blocks caps its grid at
+// AVG_MAX_BLOCKS) and, after them, the NOUT sums, for the allocation-free
+// entry point.  One copy per device, held until the process ends; host code
+// uses the address hipGetSymbolAddress returns for the device, never the
+// array's name.
+constexpr int AVG_MAX_BLOCKS = 2048;
+constexpr size_t AVG_SCRATCH_DOUBLES = (size_t)(AVG_MAX_BLOCKS + 1) * NOUT;
+__device__ __align__(256) double scratch[AVG_SCRATCH_DOUBLES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on device dev, looked up once per device with
+// that device current (the caller's current device is restored).
+int scratch_on(int dev, double **out)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(-1, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        int prev = 0;
+        HIPCHK(hipGetDevice(&prev));
+        if (prev != dev) HIPCHK(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != dev) HIPCHK(hipSetDevice(prev));
+        if (e != hipSuccess || !p) return fail(-2, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        scratch_dev[dev] = (double *)p;
+    }
+    *out = scratch_dev[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ch4g_dump_averages_noalloc(ch4g_lattice *L, double *dens, double *u, double *t, ch4g_output *out)
+{
+    if (!L || !out) return fail(-1, "null argument");
+    if (L->base.fluid && (!dens || !u)) return fail(-1, "a fluid lattice needs device buffers dens[N] and u[3N]");
+    if (L->base.thermal && !t) return fail(-1, "a thermal lattice needs a device buffer t[N]");
+    const int nb = reduce_blocks(L->base);
+    if (nb < 1 || nb > AVG_MAX_BLOCKS)
+        return fail(-3, "averages reduction of %d blocks exceeds the %d-block scratch", nb, AVG_MAX_BLOCKS);
+    double *s = nullptr;
+    int rc = scratch_on(L->device, &s);
+    if (rc) return rc;
+    double *res = s + (size_t)AVG_MAX_BLOCKS * NOUT;
+    // every partial the final sum reads is written first: nothing to clear
+    gather(L, nullptr, nullptr, dens, u, t);
+    launch_averages(L->base, dens, u, t, L->nu, L->kappa, s, res, L->stream);
+    HIPCHK(hipGetLastError());
+    double a[NOUT];
+    HIPCHK(hipMemcpyAsync(a, res, sizeof a, hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
     const ch4g_config &c = L->cfg;
     const double norm = 1.0 / ((double)c.NX * (double)c.NY * (double)c.NZ);

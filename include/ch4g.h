@@ -147,6 +147,20 @@ int ch4g_hydro_fields(ch4g_lattice *lat, double *dens, double *u, double *t);
  * including the Nusselt numbers.  Synchronous.                               */
 int ch4g_dump_averages(ch4g_lattice *lat, ch4g_output *out);
 
+/* No counterpart in the reference.  ch4g_dump_averages for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): the same averages, bit-identical,
+ * with no device allocation.  The caller brings the FP64 field buffers the
+ * averages are computed from, as device pointers: dens[N] and u[3N] (x,y,z
+ * interleaved) for a fluid lattice, t[N] for a thermal one (others may be
+ * NULL); on return they hold the fields ch4g_hydro_fields reports.  The
+ * reduction's block partials and sums live in a static device array of
+ * (2048 + 1) x 26 doubles (426 KB) per GPU, held until the process ends;
+ * the reduction grid is capped at 2048 blocks, so every lattice size fits.
+ * Calls that overlap on one GPU (several host threads, or lattices whose
+ * streams run concurrently) race on that array.  Synchronous.               */
+int ch4g_dump_averages_noalloc(ch4g_lattice *lat, double *dens, double *u, double *t, ch4g_output *out);
+
 /* dump_averages rulers (output.c:212-491, normalisation :520-690): averages
  * over the y-z planes (rx[NX]), x-z planes (ry[NY]) and x-y planes (rz[NZ])
  * of the same members, with the reference's normalisation factors (its
