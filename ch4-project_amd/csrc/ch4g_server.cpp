@@ -492,9 +492,9 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
         // remaining steps, spread over the chunk's launches (even shares keep
         // each lattice's buffer parity, so the tables and their captured
         // graph stay valid until the set changes; a job that can finish
-        // does); shares are refreshed every 16 chunks.  Jobs arriving while
-        // the chunk runs are set up meanwhile and join the next one.  Plain:
-        // the single job runs to its end in one call.
+        // does); shares are refreshed every 16 chunks.  Jobs that arrived
+        // while the chunk ran are set up after it and join the next one.
+        // Plain: the single job runs to its end in one call.
         if (batched) {
             if (table_dirty || ++chunks_since_share >= 16) {
                 int64_t rmax = 1;
@@ -516,10 +516,14 @@ int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
                 chunks_since_share = 0;
             }
             if ((rc = advance(B, sp->use_graphs != 0))) break;
-            commit(B);
             const size_t in_chunk = B.lats.size(); // active[0 .. in_chunk) take part
-            admit_ready();
+            // The chunk (a graph whose branches run on several streams)
+            // completes before any other runtime call: admission allocates,
+            // frees and launches, and doing that while a chunk was in flight
+            // ended the process in the runtime now and then.
             HIPCHK(hipStreamSynchronize(B.stream));
+            commit(B);
+            admit_ready();
             for (size_t a = 0; a < in_chunk; a++) {
                 active[a].remaining -= B.steps[a];
                 // a share that flips the lattice's buffer (or lag array): the table follows

@@ -1459,12 +1459,20 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
 // 512x256 9.19 -> 7.97, 2048x1024 103.0 -> 88.7, 4096x2048 361.4 -> 325.6;
 // with narrower last tiles 78x40 4.69 -> 3.85, 1000x500 38.6 -> 28.9,
 // 1500x700 76.3 -> 48.2 (against the dense single-step grid).
+//
+// 3-D lattices (k_step2z_fp32) need enough tile columns to fill the GPU and
+// rows that fill their 64-wide tiles; measured against single steps with
+// graph replay (us per step): channel 64^3 8.4 vs 20.3, 96^3 28.6 vs 46.0,
+// 128^3 62.2 vs 49.5, 160^3 (rows of 2.5 tiles) 116 vs 122, 192^3 196 vs 185;
+// Rayleigh-Benard 96^3 57.1 vs 66.9, 128^3 126.9 vs 92.6, 192^3 406 vs 344.
 bool two_step_ok(const KParams &P)
 {
     // 2-D: 32-bit byte offsets into the Q population planes; 3-D: 32-bit
     // byte offsets within a group of seven planes
     const bool off32 = (int64_t)Q * P.S * 4 < ((int64_t)1 << 32);
-    const bool fits = P.NZ == 1 ? off32 : 7 * P.S * 4 < ((int64_t)1 << 32);
+    const int64_t tiles_x = (P.NX + 63) / 64;
+    const bool big3d = P.N >= ((int64_t)1 << 21) && 10 * (int64_t)P.NX >= 9 * 64 * tiles_x;
+    const bool fits = P.NZ == 1 ? off32 : (big3d && 7 * P.S * 4 < ((int64_t)1 << 32));
     return fits && !P.first && !P.hit && !P.les && !P.twoway;
 }
 

@@ -97,18 +97,19 @@ def test_two_step_launches_match_single_steps(NX, NY, precision, fluid_only):
 
 
 @pytest.mark.parametrize("shape,precision,fluid_only", [
-    ((45, 32, 24), abi.FP32_DEV, False), ((130, 30, 70), abi.FP32_DEV, False), ((64, 24, 16), abi.FP32_STORE, False),
-    ((70, 20, 3), abi.FP32_DEV, True), ((64, 16, 2), abi.FP32_DEV, False), ((256, 128, 130), abi.FP32_DEV, False)],
-    ids=["45x32x24", "130x30x70", "64x24x16_arith64", "channel70x20x3", "64x16x2", "256x128x130_streamed"])
+    ((120, 30, 602), abi.FP32_DEV, False), ((128, 64, 256), abi.FP32_STORE, False),
+    ((1024, 1024, 2), abi.FP32_DEV, False), ((192, 96, 120), abi.FP32_DEV, True),
+    ((256, 128, 130), abi.FP32_DEV, False)],
+    ids=["120x30x602", "128x64x256_arith64", "1024x1024x2", "channel192x96x120", "256x128x130"])
 def test_two_step_3d_launches_match_single_steps(shape, precision, fluid_only):
-    """3-D lattices advance two steps per launch (z-marching x-y tiles with an
-    LDS ring of three planes): fluid + T with a narrower last tile in x
-    (45, 130), a partial last tile row in y (30), two z ranges of unequal
-    length (70), double arithmetic, a fluid-only channel with NZ = 3, NZ = 2
-    (the z-halo planes are the same plane) and a 1.3 GB lattice streamed from
-    HBM.  The same lattice advanced one step per ch4g_step call runs the
-    single-step kernel.  Populations must agree bit for bit after odd and
-    even counts and after graph replays."""
+    """3-D lattices of at least 2^21 nodes advance two steps per launch
+    (z-marching x-y tiles with an LDS ring of three planes): fluid + T with a
+    narrower last tile in x (120 = 64 + 56), a partial last tile row in y (30)
+    and a last z range of two planes (602), double arithmetic (64 x 4 tiles),
+    NZ = 2 (the z-halo planes are the same plane), a fluid-only channel and a
+    lattice streamed from HBM.  The same lattice advanced one step per
+    ch4g_step call runs the single-step kernel.  Populations must agree bit
+    for bit after odd and even counts and after graph replays."""
     from ch4g import workloads
     NX, NY, NZ = shape
     if fluid_only:
@@ -121,10 +122,22 @@ def test_two_step_3d_launches_match_single_steps(shape, precision, fluid_only):
     for lat in (two, one):
         lat.initial_conditions()
         lat.step(1)
-    for n in ((7, 10, 66) if NX * NY * NZ < 1e6 else (3, 34)):
+    for n in (7, 34):
         two.step(n)
         for _ in range(n):
             one.step(1)
         pa, ga = two.get_populations()
         pb, gb = one.get_populations()
         assert np.array_equal(pa, pb) and np.array_equal(ga, gb), n
+        del pa, ga, pb, gb
+
+
+def test_small_3d_lattices_step_one_per_launch():
+    """Below 2^21 nodes, or with rows that fill less than 90 % of their
+    64-wide tiles, 3-D lattices take one step per launch (faster there)."""
+    from ch4g import workloads
+    for cfg in (workloads.channel3d(96, 96, 96), workloads.channel3d(160, 160, 160),
+                configs.rb2d(NX=45, NY=32, NZ=24, Ra=2e6, precision=abi.FP32_DEV)):
+        lat = Lattice(cfg)
+        assert lat.steps_per_launch() == 1
+        lat.close()
