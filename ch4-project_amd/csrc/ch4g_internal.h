@@ -207,6 +207,7 @@ struct PartParams {
     int64_t np, ld;          // particles, row stride
     double *s;               // state rows
     int *alive;
+    const int *orig;         // reference-order index of each particle (rows may be kept in cell order)
     int *ndraw;
     int *bsum;
     uint64_t *rng;
@@ -227,6 +228,7 @@ int launch_particles(const PartParams &P, hipStream_t s);
 // in particle order); clears the nodes listed in old_keys first
 struct FeedbackBufs {
     uint32_t *kin, *kout, *vin, *vout;
+    double *dep;             // [8 np][3] deposits, particle-major
     void *tmp;
     size_t tmp_bytes;
     int key_bits;
@@ -237,6 +239,17 @@ struct FeedbackBufs {
 };
 int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s);
 size_t feedback_sort_bytes(int64_t nitems, int key_bits);
+// Rows, alive flags and reference-order indices of the particles reordered
+// by cell (z, y, x of the position's node), for locality of the corner reads.
+struct ReorderBufs {
+    uint32_t *kin, *kout, *vin, *vout;   // cell keys / current indices, sorted
+    void *tmp;
+    size_t tmp_bytes;
+    int key_bits;
+    double *s_out;                       // state rows in the new order
+    int *alive_out, *orig_out, *orig_in;
+};
+int launch_reorder(const PartParams &P, const ReorderBufs &B, hipStream_t s);
 
 // The batched step works in wave tiles of BATCH_WAVE lanes: up to 64 x of
 // one (y, z) row, so the row, its base state and its wall flags are uniform

@@ -9,6 +9,8 @@
 namespace ch4g {
 
 constexpr int HIT_RING = 8;
+// steps between reorderings of the particle rows by cell
+constexpr int REORDER_STEPS = 32;
 // steps per captured graph (even: the graph starts and ends on the same buffer)
 constexpr int GRAPH_STEPS = 32;
 
@@ -34,6 +36,16 @@ struct Particles {
     uint32_t *d_keys[2] = {nullptr, nullptr};
     int64_t nkeys[2] = {0, 0};
     uint32_t *d_kin = nullptr, *d_vin = nullptr, *d_vout = nullptr;
+    double *d_dep = nullptr;     // [8 np][3] deposit of each particle corner, reference-order-major
+    // rows kept in cell order (reordered every REORDER_STEPS steps): the
+    // reference-order index of each row, and the buffers of the reordering
+    int *d_orig = nullptr, *d_orig2 = nullptr, *d_alive2 = nullptr;
+    double *d_state2 = nullptr;
+    uint32_t *d_ck = nullptr, *d_ck2 = nullptr, *d_cv = nullptr, *d_cv2 = nullptr;
+    void *d_ro_tmp = nullptr;
+    size_t ro_tmp_bytes = 0;
+    int cell_bits = 32;
+    int64_t last_reorder = 0;
     void *d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int key_bits = 32;

@@ -125,13 +125,20 @@ void xs_set(unsigned short xs[3], uint64_t v)
 void free_state(Particles *P)
 {
     void *ptrs[] = {P->d_state, P->d_alive, P->d_ndraw, P->d_bsum, P->d_kin, P->d_vin, P->d_vout, P->d_sort_tmp,
-                    P->d_keys[0], P->d_keys[1]};
+                    P->d_keys[0], P->d_keys[1], P->d_dep, P->d_orig, P->d_orig2, P->d_alive2, P->d_state2,
+                    P->d_ck, P->d_ck2, P->d_cv, P->d_cv2, P->d_ro_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     P->d_state = nullptr;
     P->d_alive = P->d_ndraw = P->d_bsum = nullptr;
     P->d_kin = P->d_vin = P->d_vout = nullptr;
+    P->d_dep = nullptr;
     P->d_sort_tmp = nullptr;
+    P->d_orig = P->d_orig2 = P->d_alive2 = nullptr;
+    P->d_state2 = nullptr;
+    P->d_ck = P->d_ck2 = P->d_cv = P->d_cv2 = nullptr;
+    P->d_ro_tmp = nullptr;
+    P->ro_tmp_bytes = 0;
     P->d_keys[0] = P->d_keys[1] = nullptr;
     P->nkeys[0] = P->nkeys[1] = 0;
     P->cap = 0;
@@ -147,13 +154,29 @@ int alloc_state(ch4g_lattice *L, int64_t cap)
     if ((rc = dalloc(L, (void **)&P->d_state, (size_t)PF_COUNT * c * sizeof(double)))) return rc;
     if ((rc = dalloc(L, (void **)&P->d_alive, (size_t)c * sizeof(int)))) return rc;
     if ((rc = dalloc(L, (void **)&P->d_ndraw, (size_t)c * sizeof(int)))) return rc;
-    if (random_reinjection(L, P->pc))
+    if ((rc = dalloc(L, (void **)&P->d_orig, (size_t)c * sizeof(int)))) return rc;
+    if (random_reinjection(L, P->pc)) {
         if ((rc = dalloc(L, (void **)&P->d_bsum, (size_t)((c + 255) / 256) * sizeof(int)))) return rc;
+    } else {
+        // rows reordered by cell (not with random re-injection, whose draws
+        // follow the particles' order)
+        if ((rc = dalloc(L, (void **)&P->d_state2, (size_t)PF_COUNT * c * sizeof(double)))) return rc;
+        if ((rc = dalloc(L, (void **)&P->d_alive2, (size_t)c * sizeof(int)))) return rc;
+        if ((rc = dalloc(L, (void **)&P->d_orig2, (size_t)c * sizeof(int)))) return rc;
+        uint32_t **cb[] = {&P->d_ck, &P->d_ck2, &P->d_cv, &P->d_cv2};
+        for (uint32_t **b : cb)
+            if ((rc = dalloc(L, (void **)b, (size_t)c * sizeof(uint32_t)))) return rc;
+        P->cell_bits = 1;
+        while (P->cell_bits < 32 && ((uint64_t)1 << P->cell_bits) <= (uint64_t)L->N) P->cell_bits++;
+        P->ro_tmp_bytes = feedback_sort_bytes(c, P->cell_bits);
+        if ((rc = dalloc(L, &P->d_ro_tmp, P->ro_tmp_bytes > 0 ? P->ro_tmp_bytes : 1))) return rc;
+    }
     if (lhas(P->pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM)) {
         const size_t M = (size_t)8 * c;
         if ((rc = dalloc(L, (void **)&P->d_kin, M * sizeof(uint32_t)))) return rc;
         if ((rc = dalloc(L, (void **)&P->d_vin, M * sizeof(uint32_t)))) return rc;
         if ((rc = dalloc(L, (void **)&P->d_vout, M * sizeof(uint32_t)))) return rc;
+        if ((rc = dalloc(L, (void **)&P->d_dep, 3 * M * sizeof(double)))) return rc;
         for (int b = 0; b < 2; b++)
             if ((rc = dalloc(L, (void **)&P->d_keys[b], M * sizeof(uint32_t)))) return rc;
         P->key_bits = 1;
@@ -178,6 +201,7 @@ PartParams part_params(ch4g_lattice *L)
     P.ld = Q->cap;
     P.s = Q->d_state;
     P.alive = Q->d_alive;
+    P.orig = Q->d_orig;
     P.ndraw = Q->d_ndraw;
     P.bsum = Q->d_bsum;
     P.rng = Q->d_rng;
@@ -263,9 +287,13 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
     HIPCHK(hipMemcpyAsync(Q->d_alive, alive.data(), alive.size() * sizeof(int), hipMemcpyHostToDevice, L->stream));
+    std::vector<int> orig((size_t)cap);
+    for (int64_t p = 0; p < cap; p++) orig[(size_t)p] = (int)p;
+    HIPCHK(hipMemcpyAsync(Q->d_orig, orig.data(), orig.size() * sizeof(int), hipMemcpyHostToDevice, L->stream));
     HIPCHK(hipMemsetAsync(Q->d_ndraw, 0, (size_t)cap * sizeof(int), L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
     Q->np = n;
+    Q->last_reorder = L->itime - REORDER_STEPS;  // reordered at the next step
     return 0;
 }
 
@@ -315,6 +343,29 @@ void particles_free(ch4g_lattice *L)
 int particles_before_step(ch4g_lattice *L, int itime)
 {
     Particles *Q = L->part;
+    if (Q->d_state2 && itime - Q->last_reorder >= REORDER_STEPS) {
+        // rows into cell order (particles move slowly: every REORDER_STEPS
+        // steps keeps neighbouring threads on neighbouring cells)
+        PartParams P = part_params(L);
+        ReorderBufs B;
+        B.kin = Q->d_ck;
+        B.kout = Q->d_ck2;
+        B.vin = Q->d_cv;
+        B.vout = Q->d_cv2;
+        B.tmp = Q->d_ro_tmp;
+        B.tmp_bytes = Q->ro_tmp_bytes;
+        B.key_bits = Q->cell_bits;
+        B.s_out = Q->d_state2;
+        B.alive_out = Q->d_alive2;
+        B.orig_out = Q->d_orig2;
+        B.orig_in = Q->d_orig;
+        if (launch_reorder(P, B, L->stream))
+            return fail(-2, "particle reorder failed: %s", hipGetErrorString(hipGetLastError()));
+        std::swap(Q->d_state, Q->d_state2);
+        std::swap(Q->d_alive, Q->d_alive2);
+        std::swap(Q->d_orig, Q->d_orig2);
+        Q->last_reorder = itime;
+    }
     if (!lhas(Q->pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM) || itime < 2) return 0;
     const int par = itime & 1;
     PartParams P = part_params(L);
@@ -324,6 +375,7 @@ int particles_before_step(ch4g_lattice *L, int itime)
     B.kout = Q->d_keys[par];
     B.vin = Q->d_vin;
     B.vout = Q->d_vout;
+    B.dep = Q->d_dep;
     B.tmp = Q->d_sort_tmp;
     B.tmp_bytes = Q->sort_tmp_bytes;
     B.key_bits = Q->key_bits;
@@ -561,12 +613,17 @@ int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t
     if (!Q || Q->np == 0) return 0;
     const int64_t ld = Q->cap, np = Q->np;
     std::vector<double> rows((size_t)PF_COUNT * ld);
-    std::vector<int> alive((size_t)np);
+    std::vector<int> alive((size_t)np), orig((size_t)np);
     HIPCHK(hipMemcpyAsync(rows.data(), Q->d_state, rows.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipMemcpyAsync(alive.data(), Q->d_alive, alive.size() * sizeof(int), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipMemcpyAsync(orig.data(), Q->d_orig, orig.size() * sizeof(int), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
+    // records in the reference's particle order (rows may be in cell order)
+    std::vector<int64_t> row_of((size_t)np);
+    for (int64_t p = 0; p < np; p++) row_of[(size_t)orig[(size_t)p]] = p;
     int64_t k = 0;
-    for (int64_t p = 0; p < np; p++) {
+    for (int64_t o = 0; o < np; o++) {
+        const int64_t p = row_of[(size_t)o];
         if (!alive[(size_t)p]) continue;
         if (k < cap) {
             auto S = [&](int f) { return rows[(size_t)f * ld + p]; };

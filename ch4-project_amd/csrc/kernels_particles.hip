@@ -384,57 +384,18 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
         const int64_t ld = P.ld;
 #define R(f) S[(int64_t)(f) * ld + p]
         double x = R(PX), y = R(PY), z = R(PZ);
-        double vx = R(PVX), vy = R(PVY), vz = R(PVZ);
-        double vxo = R(PVXO), vyo = R(PVYO), vzo = R(PVZO);
-        double ax = R(PAX), ay = R(PAY), az = R(PAZ);
-        double axo = R(PAXO), ayo = R(PAYO), azo = R(PAZO);
-        double ux = R(PUX), uy = R(PUY), uz = R(PUZ);
-        double uxo = R(PUXO), uyo = R(PUYO), uzo = R(PUZO);
-        // rows a configuration never changes are neither read nor written:
-        // the gradient without LAGRANGE_GRADIENT, Du/Dt without added mass,
-        // the temperature rows without LB_TEMPERATURE
-        const bool am = lflag(P, CH4G_LAGRANGE_ADDEDMASS);
+        // ---- interpolation at the (wrapped) position, first: its dependent
+        // corner reads are the latency-bound part, and the particle's other
+        // rows are read afterwards so that they do not hold registers
+        // meanwhile.  Rows the sub-step overwrites are read only where it
+        // does not (a failed stencil keeps u, G, t, grad t; a tau that is
+        // neither 0 nor positive keeps a and u_old).
+        double ux = 0.0, uy = 0.0, uz = 0.0, t = 0.0;
         double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if constexpr (GRAD)
-            for (int m = 0; m < 9; m++) G[m] = R(PG + m);
-        double dtu[3] = {0.0, 0.0, 0.0};
-        if (am) {
-            dtu[0] = R(PDTU);
-            dtu[1] = R(PDTU + 1);
-            dtu[2] = R(PDTU + 2);
-        }
-        double t = 0.0, told = 0.0, dtt = 0.0, gt[3] = {0.0, 0.0, 0.0};
-        if (P.thermal) {
-            t = R(PT);
-            told = R(PTOLD);
-            dtt = R(PDTT);
-            if constexpr (GRAD) {
-                gt[0] = R(PGT);
-                gt[1] = R(PGT + 1);
-                gt[2] = R(PGT + 2);
-            }
-        }
-        const double tau = R(PTAU), beta = R(PBETA);
-        const bool orient = lflag(P, CH4G_LAGRANGE_ORIENTATION);
-        const bool second = orient && lflag(P, CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
-        double pv[3] = {0.0, 0.0, 0.0}, fold[3] = {0.0, 0.0, 0.0};
-        double nv[3] = {0.0, 0.0, 0.0}, nfold[3] = {0.0, 0.0, 0.0};
-        if (orient) {
-            for (int i = 0; i < 3; i++) {
-                pv[i] = R(PPO + i);
-                fold[i] = R(PDTP + i);
-            }
-        }
-        if (second) {
-            for (int i = 0; i < 3; i++) {
-                nv[i] = R(PNO + i);
-                nfold[i] = R(PDTN + i);
-            }
-        }
-
-        // ---- interpolation at the (wrapped) position
+        double gt[3] = {0.0, 0.0, 0.0};
         Stencil st;
-        if (stencil(P, x, y, z, st)) {
+        const bool ok = stencil(P, x, y, z, st);
+        if (ok) {
             if (P.fluid) {
                 double cu[3];
                 double v0 = 0, v1 = 0, v2 = 0;
@@ -480,6 +441,73 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
                     }
                     for (int m = 0; m < 3; m++) gt[m] = acc[m];
                 }
+            }
+        }
+        if (!(ok && P.fluid)) {
+            ux = R(PUX);
+            uy = R(PUY);
+            uz = R(PUZ);
+            if constexpr (GRAD)
+                for (int m = 0; m < 9; m++) G[m] = R(PG + m);
+        }
+        const double tau = R(PTAU), beta = R(PBETA);
+        const bool inertial = tau > 0.0, moved = tau == 0.0 || inertial;
+        // rows a configuration never changes are neither read nor written:
+        // the gradient without LAGRANGE_GRADIENT, Du/Dt without added mass,
+        // the temperature rows without LB_TEMPERATURE
+        const bool am = lflag(P, CH4G_LAGRANGE_ADDEDMASS);
+        double vx = R(PVX), vy = R(PVY), vz = R(PVZ);
+        double vxo = R(PVXO), vyo = R(PVYO), vzo = R(PVZO);
+        double ax = 0.0, ay = 0.0, az = 0.0, axo = 0.0, ayo = 0.0, azo = 0.0;
+        double uxo = 0.0, uyo = 0.0, uzo = 0.0;
+        if (!moved) {
+            ax = R(PAX);
+            ay = R(PAY);
+            az = R(PAZ);
+        }
+        if (inertial) {
+            axo = R(PAXO);
+            ayo = R(PAYO);
+            azo = R(PAZO);
+        }
+        if (!moved || (inertial && am && beta != 0.0)) {
+            uxo = R(PUXO);
+            uyo = R(PUYO);
+            uzo = R(PUZO);
+        }
+        double dtu[3] = {0.0, 0.0, 0.0};
+        if (am) {
+            dtu[0] = R(PDTU);
+            dtu[1] = R(PDTU + 1);
+            dtu[2] = R(PDTU + 2);
+        }
+        double told = 0.0, dtt = 0.0;
+        if (P.thermal) {
+            if (!ok) {
+                t = R(PT);
+                if constexpr (GRAD) {
+                    gt[0] = R(PGT);
+                    gt[1] = R(PGT + 1);
+                    gt[2] = R(PGT + 2);
+                }
+            }
+            told = R(PTOLD);
+            dtt = R(PDTT);
+        }
+        const bool orient = lflag(P, CH4G_LAGRANGE_ORIENTATION);
+        const bool second = orient && lflag(P, CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
+        double pv[3] = {0.0, 0.0, 0.0}, fold[3] = {0.0, 0.0, 0.0};
+        double nv[3] = {0.0, 0.0, 0.0}, nfold[3] = {0.0, 0.0, 0.0};
+        if (orient) {
+            for (int i = 0; i < 3; i++) {
+                pv[i] = R(PPO + i);
+                fold[i] = R(PDTP + i);
+            }
+        }
+        if (second) {
+            for (int i = 0; i < 3; i++) {
+                nv[i] = R(PNO + i);
+                nfold[i] = R(PDTN + i);
             }
         }
 
@@ -773,9 +801,11 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
         R(PAX) = ax;
         R(PAY) = ay;
         R(PAZ) = az;
-        R(PAXO) = axo;
-        R(PAYO) = ayo;
-        R(PAZO) = azo;
+        if (inertial) {
+            R(PAXO) = axo;
+            R(PAYO) = ayo;
+            R(PAZO) = azo;
+        }
         R(PUX) = ux;
         R(PUY) = uy;
         R(PUZ) = uz;
@@ -902,56 +932,28 @@ __global__ void k_fb_clear(const uint32_t *keys, int64_t n, double *fb, int64_t 
     fb[2 * N + k] = 0.0;
 }
 
-// Deposit keys: the node of each of the eight corners when it is an
-// interior node (add_sendrecv_borders_vector only ever adds into ghost
-// nodes, lagrange_feedback.c:525-836, so ghost deposits never reach the
-// force); N (sorts last) otherwise.  Particles with zero volume (tracers)
+// Deposits, in reference particle order: for each of the eight corners the node when
+// it is an interior node (add_sendrecv_borders_vector only ever adds into
+// ghost nodes, lagrange_feedback.c:525-836, so ghost deposits never reach the
+// force), N (sorts last) otherwise, and the corner's force contribution
+// fac * fp * w (lagrange_feedback.c:149-176), computed here where the
+// particle rows are read in order.  Particles with zero volume (tracers)
 // deposit exact zeros and are skipped.
-__global__ void __launch_bounds__(PB) k_fb_keys(PartParams P, uint32_t *kin, uint32_t *vin)
+__global__ void __launch_bounds__(PB) k_fb_keys(PartParams P, uint32_t *kin, uint32_t *vin, double *dep, double g0,
+                                                double g1, double g2)
 {
     const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
     if (p >= P.np) return;
     const uint32_t sent = (uint32_t)P.N;
     Stencil st;
     const double *S = P.s;
-    bool ok = P.alive[p] && S[(int64_t)PVOL * P.ld + p] != 0.0 &&
-              stencil(P, S[(int64_t)PX * P.ld + p], S[(int64_t)PY * P.ld + p], S[(int64_t)PZ * P.ld + p], st);
-    for (int c = 0; c < 8; c++) {
-        uint32_t key = sent;
-        if (ok) {
-            const int i = st.im + cdx(c), j = st.jm + cdy(c), k = st.km + cdz(c);
-            if (i >= 0 && i < P.NX && j >= 0 && j < P.NY && k >= 0 && k < P.NZ)
-                key = (uint32_t)(((int64_t)k * P.NY + j) * P.NX + i);
-        }
-        kin[8 * p + c] = key;
-        vin[8 * p + c] = (uint32_t)(8 * p + c);
-    }
-}
-
-// Per node, the deposits in particle order (the sort is stable and the
-// input is particle-major): force_twoway starts at 0 and adds
-// fac * fp * vol for each particle (lagrange_feedback.c:149-176).
-__global__ void __launch_bounds__(PB) k_fb_sum(PartParams P, const uint32_t *keys, const uint32_t *vals, int64_t M,
-                                               double *fb, double g0, double g1, double g2)
-{
-    const int64_t e = (int64_t)blockIdx.x * PB + threadIdx.x;
-    if (e >= M) return;
-    const uint32_t key = keys[e];
-    if ((int64_t)key >= P.N) return;
-    if (e > 0 && keys[e - 1] == key) return;
-    const double *S = P.s;
     const int64_t ld = P.ld;
-    const bool small = lflag(P, CH4G_LAGRANGE_TWOWAY_SMALLTAUD);
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int64_t j = e; j < M && keys[j] == key; j++) {
-        const uint32_t v = vals[j];
-        const int64_t p = v >> 3;
-        const int c = (int)(v & 7);
-        Stencil st;
-        stencil(P, S[PX * ld + p], S[PY * ld + p], S[PZ * ld + p], st);
-        const double fac = S[PVOL * ld + p], dr = S[PDRAT * ld + p];
-        double fpx, fpy, fpz;
-        if (small) {
+    const double fac = S[PVOL * ld + p];
+    const bool ok = P.alive[p] && fac != 0.0 && stencil(P, S[PX * ld + p], S[PY * ld + p], S[PZ * ld + p], st);
+    double fpx = 0.0, fpy = 0.0, fpz = 0.0;
+    if (ok) {
+        const double dr = S[PDRAT * ld + p];
+        if (lflag(P, CH4G_LAGRANGE_TWOWAY_SMALLTAUD)) {
             fpx = (S[PDTU * ld + p] - g0) * (1 - dr);
             fpy = (S[(PDTU + 1) * ld + p] - g1) * (1 - dr);
             fpz = (S[(PDTU + 2) * ld + p] - g2) * (1 - dr);
@@ -960,14 +962,84 @@ __global__ void __launch_bounds__(PB) k_fb_sum(PartParams P, const uint32_t *key
             fpy = S[(PDTU + 1) * ld + p] - g1 + dr * (g1 - S[PAY * ld + p]);
             fpz = S[(PDTU + 2) * ld + p] - g2 + dr * (g2 - S[PAZ * ld + p]);
         }
-        sx += fac * fpx * st.w[c];
-        sy += fac * fpy * st.w[c];
-        sz += fac * fpz * st.w[c];
     }
-    const int64_t N = P.N;
+    for (int c = 0; c < 8; c++) {
+        uint32_t key = sent;
+        if (ok) {
+            const int i = st.im + cdx(c), j = st.jm + cdy(c), k = st.km + cdz(c);
+            if (i >= 0 && i < P.NX && j >= 0 && j < P.NY && k >= 0 && k < P.NZ)
+                key = (uint32_t)(((int64_t)k * P.NY + j) * P.NX + i);
+        }
+        // deposits go to the slots of the particle's reference-order index:
+        // the stable sort then keeps that order within a node
+        const int64_t e = 8 * (int64_t)P.orig[p] + c;
+        kin[e] = key;
+        vin[e] = (uint32_t)e;
+        if (key != sent) {
+            dep[3 * e] = fac * fpx * st.w[c];
+            dep[3 * e + 1] = fac * fpy * st.w[c];
+            dep[3 * e + 2] = fac * fpz * st.w[c];
+        }
+    }
+}
+
+// Per node, the deposits in particle order (the sort is stable and the
+// input is particle-major): force_twoway starts at 0 and adds each
+// particle's contribution (lagrange_feedback.c:149-176).
+__global__ void __launch_bounds__(PB) k_fb_sum(const uint32_t *keys, const uint32_t *vals, const double *dep,
+                                               int64_t M, int64_t N, double *fb)
+{
+    const int64_t e = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (e >= M) return;
+    const uint32_t key = keys[e];
+    if ((int64_t)key >= N) return;
+    if (e > 0 && keys[e - 1] == key) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int64_t j = e; j < M && keys[j] == key; j++) {
+        const double *d = dep + 3 * (int64_t)vals[j];
+        sx += d[0];
+        sy += d[1];
+        sz += d[2];
+    }
     fb[key] = sx;
     fb[N + key] = sy;
     fb[2 * N + key] = sz;
+}
+
+// Cell key of each particle: the node of its wrapped position (z, y, x
+// order); dropped particles sort last.
+__global__ void __launch_bounds__(PB) k_cell_keys(PartParams P, uint32_t *key, uint32_t *val)
+{
+    const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (p >= P.np) return;
+    uint32_t k = (uint32_t)P.N;
+    if (P.alive[p]) {
+        const double *S = P.s;
+        const double x = wrapd(S[PX * P.ld + p], (double)P.NX), y = wrapd(S[PY * P.ld + p], (double)P.NY),
+                     z = wrapd(S[PZ * P.ld + p], (double)P.NZ);
+        if (x == x && y == y && z == z) {
+            const int i = min(max((int)x, 0), P.NX - 1), j = min(max((int)y, 0), P.NY - 1),
+                      l = min(max((int)z, 0), P.NZ - 1);
+            k = (uint32_t)(((int64_t)l * P.NY + j) * P.NX + i);
+        }
+    }
+    key[p] = k;
+    val[p] = (uint32_t)p;
+}
+
+// Row f of every particle into its new place (row-major grid: y = row).
+__global__ void __launch_bounds__(PB) k_permute(PartParams P, const uint32_t *src, double *s_out, int *alive_out,
+                                                int *orig_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (i >= P.np) return;
+    const int64_t from = src[i];
+    const int f = blockIdx.y;
+    s_out[(int64_t)f * P.ld + i] = P.s[(int64_t)f * P.ld + from];
+    if (f == 0) {
+        alive_out[i] = P.alive[from];
+        orig_out[i] = P.orig[from];
+    }
 }
 
 int blocks_of(int64_t n, int b) { return (int)((n + b - 1) / b); }
@@ -997,6 +1069,20 @@ int launch_particles(const PartParams &P, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+int launch_reorder(const PartParams &P, const ReorderBufs &B, hipStream_t s)
+{
+    if (P.np == 0) return 0;
+    const int nb = blocks_of(P.np, PB);
+    hipLaunchKernelGGL(k_cell_keys, dim3(nb), dim3(PB), 0, s, P, B.kin, B.vin);
+    size_t bytes = B.tmp_bytes;
+    if (hipcub::DeviceRadixSort::SortPairs(B.tmp, bytes, B.kin, B.kout, B.vin, B.vout, (int)P.np, 0, B.key_bits, s) !=
+        hipSuccess)
+        return -2;
+    hipLaunchKernelGGL(k_permute, dim3(nb, PF_COUNT), dim3(PB), 0, s, P, (const uint32_t *)B.vout, B.s_out,
+                       B.alive_out, B.orig_out);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 size_t feedback_sort_bytes(int64_t nitems, int key_bits)
 {
     size_t bytes = 0;
@@ -1012,13 +1098,14 @@ int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s)
         hipLaunchKernelGGL(k_fb_clear, dim3(blocks_of(B.old_n, 256)), dim3(256), 0, s, B.old_keys, B.old_n, B.fb, P.N);
     if (P.np == 0) return hipGetLastError() == hipSuccess ? 0 : -2;
     const int64_t M = 8 * P.np;
-    hipLaunchKernelGGL(k_fb_keys, dim3(blocks_of(P.np, PB)), dim3(PB), 0, s, P, B.kin, B.vin);
+    hipLaunchKernelGGL(k_fb_keys, dim3(blocks_of(P.np, PB)), dim3(PB), 0, s, P, B.kin, B.vin, B.dep, B.gv[0], B.gv[1],
+                       B.gv[2]);
     size_t bytes = B.tmp_bytes;
     if (hipcub::DeviceRadixSort::SortPairs(B.tmp, bytes, B.kin, B.kout, B.vin, B.vout, (int)M, 0, B.key_bits, s) !=
         hipSuccess)
         return -2;
-    hipLaunchKernelGGL(k_fb_sum, dim3(blocks_of(M, PB)), dim3(PB), 0, s, P, (const uint32_t *)B.kout,
-                       (const uint32_t *)B.vout, M, B.fb, B.gv[0], B.gv[1], B.gv[2]);
+    hipLaunchKernelGGL(k_fb_sum, dim3(blocks_of(M, PB)), dim3(PB), 0, s, (const uint32_t *)B.kout,
+                       (const uint32_t *)B.vout, (const double *)B.dep, M, (int64_t)P.N, B.fb);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
