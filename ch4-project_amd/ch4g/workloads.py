@@ -33,7 +33,12 @@ def channel3d(NX=64, NY=64, NZ=64, tau=0.6, g=1e-6, precision=abi.FP32_DEV):
                            precision=precision, tau_u=tau, Amp_x=g * NY * NY / (8.0 * nu))
 
 
-# smallest viscosity a served convection job is given (tau >= 0.5015)
+# Smallest viscosity a served convection job is given (tau >= 0.5015).  At
+# free-fall velocity Uff, nu = Uff H sqrt(Pr/Ra), so a grid of height H
+# resolves Ra <= (Uff H / NU_MIN)^2: 1e9 needs H >= 316 at Uff = 0.05.  The
+# ensemble draws Ra log-uniform in 1e6..1e9 and caps it there: a job whose
+# grid has fewer rows runs at the largest Ra the grid resolves (H = 100:
+# 1e8, H = 64: 4.1e7).
 NU_MIN = 5e-4
 
 

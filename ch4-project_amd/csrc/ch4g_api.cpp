@@ -200,7 +200,10 @@ void gather(ch4g_lattice *L, double *fpost, double *gpost, double *dens, double 
     KParams P = params_for(L, L->first, (int)L->itime);
     if (L->d_f0) {
         // the mean force of the last step (its hydro fields carry its half force)
-        launch_force_mean64(P, false, nullptr, (const double *)L->tlag, L->d_f0 + 6, L->d_f0, L->stream);
+        if (L->dev32)
+            launch_force_mean32(P, false, nullptr, (const float *)L->tlag, L->d_f0 + 6, L->d_f0, L->stream);
+        else
+            launch_force_mean64(P, false, nullptr, (const double *)L->tlag, L->d_f0 + 6, L->d_f0, L->stream);
         P.f0n = L->d_f0 + 3;
     }
     if (L->dev32)
@@ -259,8 +262,12 @@ int launch_one_step(ch4g_lattice *L)
     int in = L->cur, out = 1 - L->cur;
     if (L->d_f0) {
         // NOZEROMODE: mean force of the previous step and of this one first
-        launch_force_mean64(P, true, (const double *)L->g[in], (const double *)L->tlag, L->d_f0 + 6, L->d_f0,
-                            L->stream);
+        if (L->dev32)
+            launch_force_mean32(P, true, (const float *)L->g[in], (const float *)L->tlag, L->d_f0 + 6, L->d_f0,
+                                L->stream);
+        else
+            launch_force_mean64(P, true, (const double *)L->g[in], (const double *)L->tlag, L->d_f0 + 6, L->d_f0,
+                                L->stream);
         P.f0p = L->d_f0;
         P.f0n = L->d_f0 + 3;
     }
@@ -414,8 +421,6 @@ int validate(const ch4g_config *c)
     if (!has(*c, CH4G_LB_FLUID) && !has(*c, CH4G_LB_TEMPERATURE)) return fail(-1, "neither LB_FLUID nor LB_TEMPERATURE");
     if (c->precision != CH4G_FP64 && c->precision != CH4G_FP32_DEV && c->precision != CH4G_FP32_STORE)
         return fail(-1, "unknown precision %d", c->precision);
-    if (has(*c, CH4G_FLUID_FORCING_NOZEROMODE) && c->precision != CH4G_FP64)
-        return fail(-1, "LB_FLUID_FORCING_NOZEROMODE is implemented on the FP64 path only");
     if (has(*c, CH4G_TEMPERATURE_BUOYANCY) && !has(*c, CH4G_LB_TEMPERATURE))
         return fail(-1, "LB_TEMPERATURE_BUOYANCY requires LB_TEMPERATURE");
     if (forced(*c) && !has(*c, CH4G_LB_FLUID)) return fail(-1, "fluid forcing requires LB_FLUID");
@@ -555,6 +560,7 @@ int ch4g_create(const ch4g_config *cfg, ch4g_lattice **out)
         P.rho0 = cfg->rho0;
         P.T0 = cfg->T0;
         P.arith64 = cfg->precision == CH4G_FP32_STORE;
+        P.nozero = has(*cfg, CH4G_FLUID_FORCING_NOZEROMODE) ? 1 : 0;
     }
     if (L->dev32) {
         std::vector<double> rb, tb;

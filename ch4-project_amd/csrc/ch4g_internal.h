@@ -68,10 +68,11 @@ struct KParams {
     int hit;
     double hit_fa[3][CH4G_HIT_NK];  // pow(k^2,-1/3) * Amp_{x,y,z}
     HitTable hit_prev, hit_now;
-    // LB_FLUID_FORCING_NOZEROMODE (FP64 path): the mean of the built force of
-    // the previous (f0p) and of this step (f0n), removed at every node; null
-    // without the switch
+    // LB_FLUID_FORCING_NOZEROMODE: the mean of the built force of the
+    // previous (f0p) and of this step (f0n), removed at every node; null
+    // without the switch (nozero set: the lattice has it)
     const double *f0p, *f0n;
+    int nozero;
     int buoy;
     double T0b, beta_t, beta2_t, grav[3];
     double Tw_top, Tw_bot;  // (T_wall - T_ref)
@@ -166,6 +167,16 @@ template <int q>
 __device__ __forceinline__ bool crosses_wall(const Nbr &b)
 {
     return (cyq(q) > 0 && b.bot) || (cyq(q) < 0 && b.top);
+}
+
+// Block order for XCD locality: the GPU hands blocks to its 8 XCDs in turn
+// (block b to XCD b mod 8); renumbering them so that XCD x receives a
+// contiguous range makes neighbouring work units share that XCD's L2.  A
+// bijection of [0, nb) for any nb.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    const uint32_t x = b % 8u, q = nb / 8u, r = nb % 8u;
+    return x * q + (x < r ? x : r) + b / 8u;
 }
 
 // compile-time loop over q
@@ -341,6 +352,9 @@ void launch_two_steps_fp32(const KParams &P, const float *fin, const float *gin,
 // step kernel.
 void launch_velocity_fp64(const KParams &P, const double *fin, const double *tlag, hipStream_t s);
 // LB_FLUID_FORCING_NOZEROMODE means into f0[6] (scratch part: 512 * 6 doubles)
+// the same for float-storage lattices, with the float force of the FP32 kernels
+void launch_force_mean32(const KParams &P, bool step, const float *gin, const float *tlag, double *part, double *f0,
+                         hipStream_t s);
 void launch_force_mean64(const KParams &P, bool step, const double *gin, const double *tlag, double *part, double *f0,
                          hipStream_t s);
 void launch_velocity_fp32(const KParams &P, const float *fin, const float *tlag, hipStream_t s);

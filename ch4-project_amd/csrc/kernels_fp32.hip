@@ -73,8 +73,8 @@ __device__ __forceinline__ Row row_of(const KParams &P, int y)
 // two-way coupling paths are compiled in (lattices without them run an
 // instance without those paths and their register cost).
 template <bool GEN>
-__device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab, const double *fb, uint32_t n, int x,
-                                           int y, int z, float theta, float F[3])
+__device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab, const double *fb, const double *f0,
+                                           uint32_t n, int x, int y, int z, float theta, float F[3])
 {
     F[0] = P.Fc_f[0];
     F[1] = P.Fc_f[1];
@@ -95,6 +95,12 @@ __device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab
         F[0] += fac * P.grav_f[0];
         F[1] += fac * P.grav_f[1];
         F[2] += fac * P.grav_f[2];
+    }
+    if (GEN && f0) {
+        // NOZEROMODE: the mean of the built force removed (forcing.c:1123-1152)
+        F[0] -= (float)f0[0];
+        F[1] -= (float)f0[1];
+        F[2] -= (float)f0[2];
     }
     if (GEN && P.twoway) {
         const size_t N = (size_t)P.N;
@@ -327,7 +333,7 @@ __device__ __forceinline__ void fluid_moments(const KParams &P, const Row &R, in
     m.uz = m.jz * m.irho;
     if (P.forced) {
         float Fp[3];
-        force_at32<GEN>(P, P.hit_prev, P.fb_prev, n, x, y, z, R.offT + dTlag, Fp);
+        force_at32<GEN>(P, P.hit_prev, P.fb_prev, P.f0p, n, x, y, z, R.offT + dTlag, Fp);
         m.ux += A(0.5) * Fp[0];
         m.uy += A(0.5) * Fp[1];
         m.uz += A(0.5) * Fp[2];
@@ -423,7 +429,7 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
             omega = A(1) / tau;
         }
         float Fn[3] = {0.f, 0.f, 0.f};
-        if (P.forced) force_at32<GEN>(P, P.hit_now, P.fb_now, n, x, y, z, R.offT + (float)dT, Fn);
+        if (P.forced) force_at32<GEN>(P, P.hit_now, P.fb_now, P.f0n, n, x, y, z, R.offT + (float)dT, Fn);
         const A uF = ux * Fn[0] + uy * Fn[1] + uz * Fn[2];
         // Guo term folded into the equilibrium: f* = f - omega (f - eqG) with
         // eqG = f_eq + G/omega, so the O(population) value is rounded twice.
@@ -1072,7 +1078,7 @@ __global__ void __launch_bounds__(TZ::THREADS) k_step2z_fp32(KParams P, const fl
     // neighbouring tiles share their halo rows in that XCD's L2
     const uint32_t nbx = gridDim.x, nby = gridDim.y, total = nbx * nby * gridDim.z;
     uint32_t bid = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
-    if (remap && total % 8u == 0u) bid = (bid % 8u) * (total / 8u) + bid / 8u;
+    if (remap) bid = xcd_block(bid, total);
     const int x0 = (int)(bid % nbx) * TX, y0 = (int)((bid / nbx) % nby) * TY;
     const int zb = (int)(bid / (nbx * nby)) * zc, ze = min(zb + zc, NZ);
     const int w = NX - x0 < TX ? NX - x0 : TX;
@@ -1369,7 +1375,7 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
         double ux = jx / rho, uy = jy / rho, uz = jz / rho;
         if (P.forced) {
             float F[3];
-            force_at32<true>(P, P.hit_now, P.fb_now, n, x, y, z, R.offT + dTlag, F);
+            force_at32<true>(P, P.hit_now, P.fb_now, P.f0n, n, x, y, z, R.offT + dTlag, F);
             ux += 0.5 * (double)F[0];
             uy += 0.5 * (double)F[1];
             uz += 0.5 * (double)F[2];
@@ -1409,7 +1415,7 @@ void launch_step_fp32(const KParams &P, const float *fin, const float *gin, floa
     // both lattices resident in the 256 MB last-level cache?
     const int64_t lattice_bytes = 2 * (int64_t)Q * P.N * 4 * ((P.fluid ? 1 : 0) + (P.thermal ? 1 : 0));
     const bool ntl = lattice_bytes <= ((int64_t)192 << 20);
-    const bool gen = P.hit || P.les || P.twoway || P.part;
+    const bool gen = P.hit || P.les || P.twoway || P.part || P.f0p || P.f0n;
 #define CH4G_LAUNCH32G_(F_, FL_, TH_, A_, N_, D_)                                                                     \
     do {                                                                                                              \
         if (gen)                                                                                                      \
@@ -1473,7 +1479,7 @@ bool two_step_ok(const KParams &P)
     const int64_t tiles_x = (P.NX + 63) / 64;
     const bool big3d = P.N >= ((int64_t)1 << 21) && 10 * (int64_t)P.NX >= 9 * 64 * tiles_x;
     const bool fits = P.NZ == 1 ? off32 : (big3d && 7 * P.S * 4 < ((int64_t)1 << 32));
-    return fits && !P.first && !P.hit && !P.les && !P.twoway;
+    return fits && !P.first && !P.hit && !P.les && !P.twoway && !P.nozero;
 }
 
 namespace {
@@ -1683,6 +1689,88 @@ void launch_batch2_fp32(const BatchEntry *tab, const BatchSlot *slots, int nslot
         CH4G_B2(false, true);
 #undef CH4G_B2
 #undef CH4G_B2_
+}
+
+// LB_FLUID_FORCING_NOZEROMODE on float storage (forcing.c:1123-1152): sums
+// over all nodes of the float force the step kernel builds (no particle
+// feedback) of the previous step, F(T_lag) with the previous HIT table, and
+// of this step, F(T) with T gathered as the step kernel gathers it (STEP), or
+// of the last step alone (the hydro-fields gather), in double; fixed grid and
+// a tree per block, so the partial sums are the same on every run.  The
+// means are formed by k_force_mean64.
+namespace {
+constexpr int FM32_THREADS = 256, FM32_BLOCKS = 512;
+
+template <bool STEP, bool FIRST>
+__global__ void __launch_bounds__(FM32_THREADS) k_force_sum32(KParams P, const float *__restrict__ gin,
+                                                              const float *__restrict__ tlag, double *__restrict__ part)
+{
+    P.twoway = 0;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int64_t N = P.N, stride = (int64_t)gridDim.x * FM32_THREADS;
+    for (int64_t i = (int64_t)blockIdx.x * FM32_THREADS + threadIdx.x; i < N; i += stride) {
+        const uint32_t n = (uint32_t)i;
+        const int x = (int)(i % P.NX), y = (int)((i / P.NX) % P.NY), z = (int)(i / ((int64_t)P.NX * P.NY));
+        const Row R = row_of(P, y);
+        const float dTlag = P.thermal ? tlag[n] : 0.f;
+        float F[3];
+        if constexpr (STEP) {
+            force_at32<true>(P, P.hit_prev, nullptr, nullptr, n, x, y, z, R.offT + dTlag, F);
+            for (int c = 0; c < 3; c++) acc[c] += (double)F[c];
+            float dT = 0.f;
+            if (P.thermal && P.buoy) {
+                float g[Q];
+                gather_g32<FIRST, false>(P, R, make_nbr(x, y, z, P.NX, P.NY, P.NZ), gin, n, dTlag, g);
+                if (P.arith64) {
+                    double s = 0.0;
+                    for (int q = 0; q < Q; q++) s += (double)g[q];
+                    dT = (float)s;
+                } else {
+                    dT = sum19(g);
+                }
+            }
+            force_at32<true>(P, P.hit_now, nullptr, nullptr, n, x, y, z, R.offT + dT, F);
+        } else {
+            force_at32<true>(P, P.hit_now, nullptr, nullptr, n, x, y, z, R.offT + dTlag, F);
+        }
+        for (int c = 0; c < 3; c++) acc[3 + c] += (double)F[c];
+    }
+    __shared__ double red[6][FM32_THREADS];
+    for (int c = 0; c < 6; c++) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int w = FM32_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int c = 0; c < 6; c++) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_force_mean32(int nblocks, double NX, double NY, double NZ, const double *__restrict__ part,
+                               double *__restrict__ f0)
+{
+    const int c = threadIdx.x;
+    if (c >= 6) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += part[(size_t)b * 6 + c];
+    const double norm = 1.0 / (NX * NY * NZ);
+    f0[c] = s * norm;
+}
+}  // namespace
+
+void launch_force_mean32(const KParams &P, bool step, const float *gin, const float *tlag, double *part, double *f0,
+                         hipStream_t s)
+{
+    const int64_t need = (P.N + FM32_THREADS - 1) / FM32_THREADS;
+    const int nb = (int)(need < FM32_BLOCKS ? need : FM32_BLOCKS);
+    if (!step)
+        hipLaunchKernelGGL((k_force_sum32<false, false>), dim3(nb), dim3(FM32_THREADS), 0, s, P, gin, tlag, part);
+    else if (P.first)
+        hipLaunchKernelGGL((k_force_sum32<true, true>), dim3(nb), dim3(FM32_THREADS), 0, s, P, gin, tlag, part);
+    else
+        hipLaunchKernelGGL((k_force_sum32<true, false>), dim3(nb), dim3(FM32_THREADS), 0, s, P, gin, tlag, part);
+    hipLaunchKernelGGL(k_force_mean32, dim3(1), dim3(64), 0, s, nb, (double)P.NX, (double)P.NY, (double)P.NZ, part,
+                       f0);
 }
 
 void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, const float *tlag, double *fpost,

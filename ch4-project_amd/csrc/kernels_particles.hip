@@ -377,7 +377,9 @@ __device__ __forceinline__ void wrap_keep(const PartParams &P, double &x, double
 template <bool GRAD, typename F>
 __global__ void __launch_bounds__(PB) k_particles(PartParams P)
 {
-    const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
+    // rows are in cell order: consecutive blocks on one XCD share corner lines
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);  // the block of particles this block moves
+    const int64_t p = (int64_t)lb * PB + threadIdx.x;
     int nd = 0;
     if (p < P.np && P.alive[p]) {
         double *S = P.s;
@@ -854,7 +856,7 @@ __global__ void __launch_bounds__(PB) k_particles(PartParams P)
         if (threadIdx.x == 0) {
             int s = 0;
             for (int w = 0; w < PB / 64; w++) s += red[w];
-            P.bsum[blockIdx.x] = s;
+            P.bsum[lb] = s;
         }
     }
 }

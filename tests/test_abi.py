@@ -73,12 +73,15 @@ def test_footprint_and_errors_without_gpu():
         assert b"device" in lib.ch4g_last_error()
 
 
-def test_unsupported_flags_rejected_before_device_use():
+def test_invalid_configs_rejected_before_device_use():
     lib = C.CDLL(LIB_PATH)
     lib.ch4g_last_error.restype = C.c_char_p
-    # NOZEROMODE has a force-mean pass on the FP64 path only
-    cfg = abi.make_config(8, 8, 8, abi.LB_FLUID | abi.FLUID_FORCING_NOZEROMODE | abi.FLUID_FORCING_HIT,
-                          precision=abi.FP32_DEV)
     h = C.c_void_p()
+    # buoyancy without the temperature populations
+    cfg = abi.make_config(8, 8, 8, abi.LB_FLUID | abi.TEMPERATURE_BUOYANCY, precision=abi.FP32_DEV)
     assert lib.ch4g_create(C.byref(cfg), C.byref(h)) == -1
-    assert b"NOZEROMODE" in lib.ch4g_last_error()
+    assert b"LB_TEMPERATURE" in lib.ch4g_last_error()
+    # y walls on a lattice of fewer than three rows
+    cfg = abi.make_config(8, 2, 8, abi.LB_FLUID | abi.FLUID_BC_Y, precision=abi.FP32_DEV)
+    assert lib.ch4g_create(C.byref(cfg), C.byref(h)) == -1
+    assert b"NY >= 3" in lib.ch4g_last_error()

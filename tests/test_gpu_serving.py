@@ -142,3 +142,69 @@ def test_serve_rejects_unbatchable_jobs_only():
     assert res[-1].status < 0 and all(r.status == 0 for r in res[:-1])
     seq = serve([abi.Job.from_buffer_copy(les)], mode=abi.SERVE_SEQUENTIAL)
     assert seq[0].status == 0
+
+
+def _alone(cfg, nsteps):
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.step(nsteps)
+    out = lat.dump_averages()
+    lat.close()
+    return out
+
+
+def test_serve_at_the_stated_sizes_equals_each_job_alone():
+    """Jobs at the ends of the served ensemble's size ranges (2-D
+    Rayleigh-Benard 64x64 and 1024x512 at Ra 1e6 and 1e9 capped as the
+    ensemble caps it, 3-D channels 64^3 and 192^3, the largest of which takes
+    two steps per launch when stepped alone), arriving together and served
+    batched: each job's final averages must equal, bit for bit, those of the
+    same job stepped alone with ch4g_step.  FP32-deviation storage."""
+    import math
+    jobs = []
+    for cfg, n in ((workloads.rb2d(64, 64, 1, Ra=1e6, precision=abi.FP32_DEV, seed=21), 300),
+                   (workloads.rb2d(1024, 512, 1, Ra=min(1e9, (0.05 * 512 / workloads.NU_MIN) ** 2),
+                                   precision=abi.FP32_DEV, seed=22), 200),
+                   (workloads.channel3d(64, 64, 64), 250),
+                   (workloads.channel3d(192, 192, 192), 150)):
+        j = abi.Job()
+        j.cfg, j.nsteps, j.arrival_s = cfg, n, 0.0
+        jobs.append(j)
+    assert Lattice(jobs[3].cfg).steps_per_launch() == 2
+    done = serve([abi.Job.from_buffer_copy(j) for j in jobs], mode=abi.SERVE_BATCHED, chunk_steps=32)
+    assert all(d.status == 0 for d in done)
+    for j, d in zip(jobs, done):
+        ref = _alone(j.cfg, j.nsteps)
+        got = d.out.as_dict()
+        print(f"  {j.cfg.NX}x{j.cfg.NY}x{j.cfg.NZ} {j.nsteps} steps: ene {got['ene']:.10e}")
+        for k, v in ref.items():
+            assert got[k] == v or (math.isnan(v) and math.isnan(got[k])), (j.cfg.NX, k, got[k], v)
+
+
+def test_served_fp64_jobs_match_the_oracle():
+    """Small FP64 jobs (a 2-D Rayleigh-Benard and a channel) served batched
+    with FP32-deviation jobs in the mix: their final averages against the
+    oracle's dump_averages of the same configuration (the batched FP64 kernel
+    advances the populations bit-identically; the averages' reduction order
+    differs: 1e-11)."""
+    from oracle import Oracle
+    cfgs = [(workloads.rb2d(48, 24, 1, Ra=2e5, precision=abi.FP64, seed=31), 120),
+            (workloads.channel3d(8, 24, 6, tau=0.8, g=1e-5, precision=abi.FP64), 90),
+            (workloads.rb2d(96, 48, 1, Ra=1e6, precision=abi.FP32_DEV, seed=32), 100),
+            (workloads.channel3d(40, 24, 20, precision=abi.FP32_DEV), 80)]
+    jobs = []
+    for i, (cfg, n) in enumerate(cfgs):
+        j = abi.Job()
+        j.cfg, j.nsteps, j.arrival_s = cfg, n, 0.001 * i
+        jobs.append(j)
+    done = serve(jobs, mode=abi.SERVE_BATCHED, chunk_steps=16)
+    assert all(d.status == 0 for d in done)
+    for d, (cfg, n) in zip(done[:2], cfgs[:2]):
+        o = Oracle(cfg)
+        o.initial_conditions()
+        o.step(n)
+        ref = o.dump_averages()
+        got = d.out.as_dict()
+        for k in ("ene", "rho", "rho2", "eps", "t", "t2", "nuy", "epst"):
+            tol = 1e-11 * max(abs(ref[k]), 1e-300)
+            assert abs(got[k] - ref[k]) <= tol or abs(ref[k]) < 1e-290, (cfg.NX, k, got[k], ref[k])
