@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from .abi import NPOP, Config, Job, Output, ParticleConfig, ServeParams, output_dtype, particle_dtype
+from .abi import NPOP, Config, Job, Output, Particle, ParticleConfig, ServeParams, output_dtype, particle_dtype
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libch4g.so")
@@ -23,7 +23,7 @@ EXPORTS = [
     "ch4g_set_populations", "ch4g_get_populations", "ch4g_step", "ch4g_hydro_fields", "ch4g_dump_averages",
     "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_dump_rulers", "ch4g_step_timed", "ch4g_steps_per_launch", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
     "ch4g_footprint_bytes", "ch4g_particles_init", "ch4g_particles_count", "ch4g_get_particles",
-    "ch4g_set_particles", "ch4g_step_batch", "ch4g_serve", "ch4g_dump_averages_noalloc",
+    "ch4g_set_particles", "ch4g_step_batch", "ch4g_serve", "ch4g_dump_averages_noalloc", "ch4g_particle_size",
 ]
 
 
@@ -72,6 +72,10 @@ def lib():
     L.ch4g_set_particles.argtypes = [P, P, C.c_int64]
     L.ch4g_step_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
     L.ch4g_serve.argtypes = [C.POINTER(ServeParams), C.POINTER(Job), C.c_int32]
+    L.ch4g_particle_size.restype = C.c_int64
+    if L.ch4g_particle_size() != C.sizeof(Particle):
+        raise Ch4gError(f"ch4g_particle is {L.ch4g_particle_size()} bytes in {LIB_PATH}, "
+                        f"{C.sizeof(Particle)} in ch4g/abi.py")
     _lib = L
     return L
 

@@ -590,6 +590,8 @@ err:
     return rc;
 }
 
+int64_t ch4g_particle_size(void) { return (int64_t)sizeof(ch4g_particle); }
+
 int ch4g_particles_count(ch4g_lattice *L, int64_t *n)
 {
     if (!L || !n) return fail(-1, "null argument");

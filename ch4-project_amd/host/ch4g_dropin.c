@@ -69,6 +69,11 @@ int ch4g_dropin_init(const ch4g_run_params *rp)
     if (S.lat) ch4g_dropin_finalize();
     memset(&S, 0, sizeof S);
     S.rp = *rp;
+    if (ch4g_particle_size() != (int64_t)sizeof(ch4g_particle)) {
+        fprintf(stderr, "Error message -> ch4g_particle is %lld bytes in libch4g, %zu in this build. Exit.\n",
+                (long long)ch4g_particle_size(), sizeof(ch4g_particle));
+        return -1;
+    }
     return ch4g_create(&rp->cfg, &S.lat);
 }
 

@@ -134,7 +134,8 @@ int ch4g_get_populations(ch4g_lattice *lat, double *p, double *g);
 
 /* Advance nsteps time steps of main.c:92-161 (compute_advection, add_collision
  * x2, build_forcing, add_forcing, boundary_and_pbc_conditions_for_streaming,
- * time_stepping x2, hydro_fields x2) with one fused kernel per step.
+ * time_stepping x2, hydro_fields x2) with fused kernels: one launch per step,
+ * or per two steps where ch4g_steps_per_launch reports 2 (bit-identical).
  * Asynchronous on the lattice stream.                                        */
 int ch4g_step(ch4g_lattice *lat, int nsteps);
 
@@ -189,10 +190,13 @@ int ch4g_step_timed(ch4g_lattice *lat, int nsteps, double *elapsed_ms);
  * library).                                                                 */
 int ch4g_set_graphs(ch4g_lattice *lat, int enable);
 
-/* Steps one steady-state launch advances: 2 for 2-D float-storage lattices
- * without HIT forcing, LES or two-way particles (k_step2_fp32: two steps
- * through an LDS tile), else 1.  Odd remainders run as single steps; results are identical
- * either way.  No reference counterpart (launch mechanics of this library). */
+/* Steps one steady-state launch advances: 2 for float-storage lattices
+ * without HIT forcing, LES, NOZEROMODE or two-way particles that are 2-D
+ * (k_step2_fp32: two steps through an LDS tile) or 3-D with at least 2^21
+ * nodes and rows that fill their 64-wide tiles (k_step2z_fp32: x-y tiles
+ * marching along z through an LDS ring of planes), else 1.  Odd remainders
+ * run as single steps; results are identical either way.  No reference
+ * counterpart (launch mechanics of this library).                          */
 int ch4g_steps_per_launch(const ch4g_lattice *lat, int *steps);
 
 /* Wait for all work queued on the lattice stream. */
@@ -273,6 +277,12 @@ typedef struct ch4g_particle {
     double aspect_ratio, gyrotaxis_velocity, swim_velocity;
     double nx, ny, nz, dt_nx, dt_ny, dt_nz;             /* second orientation (..._SECONDORIENTATION) */
 } ch4g_particle;
+
+/* No counterpart in the reference.  sizeof(ch4g_particle) as this library
+ * was built: callers that exchange particle records check it against their
+ * own header before ch4g_get_particles / ch4g_set_particles (the record grew
+ * with the orientation members; a stale header would mis-stride the arrays). */
+int64_t ch4g_particle_size(void);
 
 /* allocate_particles (lagrange.c:6-49) + initial_conditions_particles
  * without restart (lagrange.c:72-1021): type ladders, name % types, positions

@@ -85,3 +85,11 @@ def test_invalid_configs_rejected_before_device_use():
     cfg = abi.make_config(8, 2, 8, abi.LB_FLUID | abi.FLUID_BC_Y, precision=abi.FP32_DEV)
     assert lib.ch4g_create(C.byref(cfg), C.byref(h)) == -1
     assert b"NY >= 3" in lib.ch4g_last_error()
+
+
+def test_particle_record_size_matches_the_library():
+    """ch4g_particle_size reports the record stride the library was built
+    with; the Python binding refuses a library whose stride differs."""
+    lib = C.CDLL(LIB_PATH)
+    lib.ch4g_particle_size.restype = C.c_int64
+    assert lib.ch4g_particle_size() == C.sizeof(abi.Particle) == 8 * len(abi.PARTICLE_FIELDS)
