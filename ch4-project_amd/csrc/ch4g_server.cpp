@@ -58,6 +58,12 @@ struct Batch {
     std::vector<int> steps;   // steps of each lattice per chunk
     std::vector<char> two;    // lattice i goes two steps per launch
     std::vector<char> flips;  // a chunk changes lattice i's buffer or lag parity
+    // 3-D lattices large enough for two steps per launch alone run their own
+    // launches (k_step2z_fp32) as a branch of the chunk on a stream of their
+    // own instead of joining a batched group
+    std::vector<int> solo;    // lattice indices
+    std::vector<hipStream_t> sstream;
+    std::vector<hipEvent_t> sdone;
     int chunk = 0;            // launches per chunk = max steps
     hipGraphExec_t graph = nullptr;
     hipEvent_t ev = nullptr;
@@ -85,6 +91,10 @@ struct Batch {
         gdone.clear();
         for (hipStream_t t : gstream) (void)hipStreamDestroy(t);
         gstream.clear();
+        for (hipEvent_t e : sdone) (void)hipEventDestroy(e);
+        sdone.clear();
+        for (hipStream_t t : sstream) (void)hipStreamDestroy(t);
+        sstream.clear();
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -117,22 +127,28 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
     B.chunk = 0;
     B.two.assign(lats.size(), 0);
     B.flips.assign(lats.size(), 0);
+    B.solo.clear();
+    std::vector<char> solo(lats.size(), 0);
     for (size_t i = 0; i < lats.size(); i++) {
         int rc = check_member(lats[i]);
         if (rc) return rc;
         if (steps[i] < 1) return fail(-1, "steps per chunk must be >= 1");
         B.chunk = std::max(B.chunk, steps[i]);
         ch4g_lattice *L = lats[i];
-        // the batched two-step kernel takes 2-D lattices only
-        B.two[i] = two_step_eligible(L) && L->cfg.NZ == 1;
+        // two steps per launch: 2-D lattices in the batched two-step kernel,
+        // large 3-D lattices on their own branch
+        B.two[i] = two_step_eligible(L);
+        solo[i] = B.two[i] && L->cfg.NZ > 1;
+        if (solo[i]) B.solo.push_back((int)i);
         if (B.two[i] && (rc = ensure_tlag_alt(L))) return rc;
         // two-step lattices: the population buffer flips with every launch,
         // the lag array with every two-step launch
         const int pairs = steps[i] / 2, launches = pairs + steps[i] % 2;
         B.flips[i] = B.two[i] ? ((launches & 1) || (pairs & 1)) : (steps[i] & 1);
     }
-    std::vector<int> order(lats.size());
-    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    std::vector<int> order;
+    for (size_t i = 0; i < lats.size(); i++)
+        if (!solo[i]) order.push_back((int)i);
     auto key = [&](int i) {
         const ch4g_lattice *L = lats[(size_t)i];
         const int two = B.two[(size_t)i];
@@ -228,6 +244,14 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
         HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         B.gdone.push_back(e);
     }
+    while (B.sstream.size() < B.solo.size()) {
+        hipStream_t t = nullptr;
+        hipEvent_t e = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+        B.sstream.push_back(t);
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        B.sdone.push_back(e);
+    }
     // the previous tables may still be read by queued kernels
     HIPCHK(hipStreamSynchronize(B.stream));
     if (B.host.size() > B.dev_cap) {
@@ -259,10 +283,32 @@ int build(Batch &B, const std::vector<ch4g_lattice *> &lats, const std::vector<i
 // this makes parallel branches of one graph).
 int launch_steps(Batch &B)
 {
-    const size_t ng = B.groups.size();
-    if (ng > 1) {
+    const size_t ng = B.groups.size(), ns = B.solo.size();
+    if (ng > 1 || ns > 0) {
         HIPCHK(hipEventRecord(B.fork, B.stream));
         for (size_t g = 1; g < ng; g++) HIPCHK(hipStreamWaitEvent(B.gstream[g - 1], B.fork, 0));
+        for (size_t k = 0; k < ns; k++) HIPCHK(hipStreamWaitEvent(B.sstream[k], B.fork, 0));
+    }
+    // solo 3-D lattices: their share as two-step launches (plus one single
+    // step for an odd share) from the buffers and lag arrays of the chunk's
+    // start, exactly as ch4g_step advances them
+    for (size_t k = 0; k < ns; k++) {
+        ch4g_lattice *L = B.lats[(size_t)B.solo[k]];
+        const int st = B.steps[(size_t)B.solo[k]];
+        hipStream_t t = B.sstream[k];
+        const KParams P = params_for(L, 0, (int)L->itime + 1);
+        int cur = L->cur;
+        void *tl = L->tlag, *ta = L->tlag_alt;
+        for (int s_ = 0; s_ + 1 < st; s_ += 2) {
+            launch_two_steps_fp32(P, (const float *)L->f[cur], (const float *)L->g[cur], (float *)L->f[cur ^ 1],
+                                  (float *)L->g[cur ^ 1], (const float *)tl, (float *)ta, t);
+            cur ^= 1;
+            if (L->base.thermal) std::swap(tl, ta);
+        }
+        if (st & 1)
+            launch_step_fp32(P, (const float *)L->f[cur], (const float *)L->g[cur], (float *)L->f[cur ^ 1],
+                             (float *)L->g[cur ^ 1], (float *)tl, (float *)L->taules, t);
+        HIPCHK(hipEventRecord(B.sdone[k], t));
     }
     for (size_t g = 0; g < ng; g++) {
         const Batch::Group &G = B.groups[g];
@@ -286,6 +332,7 @@ int launch_steps(Batch &B)
         HIPCHK(hipEventRecord(B.gdone[g - 1], B.gstream[g - 1]));
         HIPCHK(hipStreamWaitEvent(B.stream, B.gdone[g - 1], 0));
     }
+    for (size_t k = 0; k < ns; k++) HIPCHK(hipStreamWaitEvent(B.stream, B.sdone[k], 0));
     return 0;
 }
 
