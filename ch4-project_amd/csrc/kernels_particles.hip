@@ -374,8 +374,12 @@ __device__ __forceinline__ void wrap_keep(const PartParams &P, double &x, double
 // One sub-step of every particle: interpolate_vector_at_particles(u),
 // interpolate_scalar_at_particles(t) (lagrange.c:1211-1593), move_particles
 // (lagrange.c:2934-4025) and the wrap of sendrecv_particles.
+// With gradients: three waves per SIMD (167 VGPRs and a few spills) ran the
+// two-way leg's 4e6 particles in 4.54 instead of 4.88 ms; without, four
+// waves (127 VGPRs, 30 spilled) were slower than the unconstrained 143 VGPRs
+// (315 against 262 us for 2e6 particles).
 template <bool GRAD, typename F>
-__global__ void __launch_bounds__(PB) k_particles(PartParams P)
+__global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
 {
     // rows are in cell order: consecutive blocks on one XCD share corner lines
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);  // the block of particles this block moves
