@@ -152,9 +152,15 @@ KParams params_for(const ch4g_lattice *L, int first, int itime)
     P.first = first;
     P.itime = itime;
     if (P.twoway) {
-        // feedback of step s lives in plane set s & 1 (zero at s = 0, 1)
-        P.fb_prev = L->part->d_fb[(itime + 1) & 1];
-        P.fb_now = L->part->d_fb[itime & 1];
+        // feedback of step s lives in plane set s & 1 (zero at s = 0, 1):
+        // double planes on FP64 lattices, float planes on float storage
+        if (L->dev32) {
+            P.fbf_prev = (const float *)L->part->d_fb[(itime + 1) & 1];
+            P.fbf_now = (const float *)L->part->d_fb[itime & 1];
+        } else {
+            P.fb_prev = (const double *)L->part->d_fb[(itime + 1) & 1];
+            P.fb_now = (const double *)L->part->d_fb[itime & 1];
+        }
     }
     if (P.hit) {
         const int NX = L->cfg.NX, NY = L->cfg.NY;

@@ -88,7 +88,8 @@ struct KParams {
     // two-way particle feedback added to the built force (lagrange_feedback.c:245-254):
     // planes [3][N] of the previous step (u shift) and of this step (Guo term)
     int twoway;
-    const double *fb_prev, *fb_now;
+    const double *fb_prev, *fb_now;       // FP64 lattices
+    const float *fbf_prev, *fbf_now;      // float storage: the same sums rounded to float
     // float copies of the step constants used by the float collision (the
     // same round-to-nearest values the kernels would convert per use);
     // filled by fill_float_params once the double values are final
@@ -235,17 +236,26 @@ struct PartParams {
 
 // one sub-step: interpolation + move + walls (+ re-injection draws) + wrap
 int launch_particles(const PartParams &P, hipStream_t s);
-// feedback deposits of the particles into fb[3][N] (sorted per node, summed
-// in particle order); clears the nodes listed in old_keys first
+// Two-way feedback (lagrange_feedback.c:11-256): every particle's deposits
+// summed per node in reference particle order, written to dense planes
+// fb[3][N] (double on FP64 lattices, float on float storage).  The particles
+// are sorted by stencil cell (stable over reference order), a head array
+// marks where each cell's list starts, and one pass over the nodes merges
+// the lists of the eight cells that have the node as a corner.
 struct FeedbackBufs {
-    uint32_t *kin, *kout, *vin, *vout;
-    double *dep;             // [8 np][3] deposits, particle-major
+    uint32_t *kin, *vin;     // [np] by reference index: stencil cell (or the sentinel), row
+    uint32_t *kout, *vout;   // the same sorted by cell
+    const uint32_t *old_keys;  // sorted cells of the previous call, whose heads are cleared
+    int64_t old_n;
+    int *head;               // [(NX+1)(NY+1)(NZ+1)] first sorted position of each cell, -1 if empty
+    double *rec;             // [np][8] by row: fac f_p (3 components), wrapped position (3), reference index
+    double *dep;             // [8][np][3] deposit of corner c of the t-th sorted particle
+    uint32_t *ord;           // [np] reference index of the t-th sorted particle
     void *tmp;
     size_t tmp_bytes;
     int key_bits;
-    const uint32_t *old_keys;
-    int64_t old_n;
-    double *fb;
+    void *fb;                // [3][N]
+    int fb_float;
     double gv[3];            // gravity vector of the feedback (g_x, -g_y, g_z) or 0
 };
 int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s);

@@ -30,13 +30,17 @@ struct Particles {
     int pending = 0;
     double dt = 1.0;             // time_dt_lagr
     int substeps = 1;
-    // two-way coupling: feedback force planes [3][N] per step parity, and the
-    // sorted deposit keys that wrote each (to clear exactly those nodes)
-    double *d_fb[2] = {nullptr, nullptr};
+    // two-way coupling: feedback force planes [3][N] per step parity (double
+    // on FP64 lattices, float on float storage), and the sorted stencil cells
+    // of the particles of each (whose heads the next call clears)
+    void *d_fb[2] = {nullptr, nullptr};
     uint32_t *d_keys[2] = {nullptr, nullptr};
     int64_t nkeys[2] = {0, 0};
     uint32_t *d_kin = nullptr, *d_vin = nullptr, *d_vout = nullptr;
-    double *d_dep = nullptr;     // [8 np][3] deposit of each particle corner, reference-order-major
+    double *d_rec = nullptr;     // [np][8] fac f_p, position and reference index of each row
+    double *d_dep = nullptr;     // [8][np][3] corner deposits of the particles sorted by stencil cell
+    uint32_t *d_ord = nullptr;   // [np] their reference indices
+    int *d_head = nullptr;       // [(NX+1)(NY+1)(NZ+1)] first sorted position of each stencil cell, or -1
     // rows kept in cell order (reordered every REORDER_STEPS steps): the
     // reference-order index of each row, and the buffers of the reordering
     int *d_orig = nullptr, *d_orig2 = nullptr, *d_alive2 = nullptr;

@@ -74,8 +74,10 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
         return fail(-1, "Error in swim_velocity parameters");
     if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_SMALLTAUD) && !lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM))
         return fail(-1, "LAGRANGE_TWOWAY_SMALLTAUD needs LAGRANGE_TWOWAY_MOMENTUM");
-    if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM) && L->N >= ((int64_t)1 << 32) - 1)
-        return fail(-1, "two-way coupling needs fewer than 2^32 - 1 nodes");
+    // the feedback sorts 32-bit stencil-cell keys with a sentinel above every cell
+    if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM) &&
+        (uint64_t)(L->cfg.NX + 1) * (uint64_t)(L->cfg.NY + 1) * (uint64_t)(L->cfg.NZ + 1) >= ((uint64_t)1 << 32))
+        return fail(-1, "two-way coupling needs (NX+1)(NY+1)(NZ+1) < 2^32");
     return 0;
 }
 
@@ -122,17 +124,28 @@ void xs_set(unsigned short xs[3], uint64_t v)
     xs[2] = (unsigned short)((v >> 32) & 0xFFFF);
 }
 
+// stencil cells of the feedback sort: (NX+1)(NY+1)(NZ+1), corners from -1
+uint64_t fb_cells(const ch4g_lattice *L)
+{
+    return (uint64_t)(L->cfg.NX + 1) * (uint64_t)(L->cfg.NY + 1) * (uint64_t)(L->cfg.NZ + 1);
+}
+
+// bytes of one set of feedback planes [3][N]
+size_t fb_bytes(const ch4g_lattice *L) { return (size_t)3 * L->N * (L->dev32 ? sizeof(float) : sizeof(double)); }
+
 void free_state(Particles *P)
 {
     void *ptrs[] = {P->d_state, P->d_alive, P->d_ndraw, P->d_bsum, P->d_kin, P->d_vin, P->d_vout, P->d_sort_tmp,
-                    P->d_keys[0], P->d_keys[1], P->d_dep, P->d_orig, P->d_orig2, P->d_alive2, P->d_state2,
-                    P->d_ck, P->d_ck2, P->d_cv, P->d_cv2, P->d_ro_tmp};
+                    P->d_keys[0], P->d_keys[1], P->d_rec, P->d_dep, P->d_ord, P->d_head, P->d_orig, P->d_orig2, P->d_alive2,
+                    P->d_state2, P->d_ck, P->d_ck2, P->d_cv, P->d_cv2, P->d_ro_tmp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     P->d_state = nullptr;
     P->d_alive = P->d_ndraw = P->d_bsum = nullptr;
     P->d_kin = P->d_vin = P->d_vout = nullptr;
-    P->d_dep = nullptr;
+    P->d_rec = P->d_dep = nullptr;
+    P->d_ord = nullptr;
+    P->d_head = nullptr;
     P->d_sort_tmp = nullptr;
     P->d_orig = P->d_orig2 = P->d_alive2 = nullptr;
     P->d_state2 = nullptr;
@@ -172,15 +185,23 @@ int alloc_state(ch4g_lattice *L, int64_t cap)
         if ((rc = dalloc(L, &P->d_ro_tmp, P->ro_tmp_bytes > 0 ? P->ro_tmp_bytes : 1))) return rc;
     }
     if (lhas(P->pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM)) {
-        const size_t M = (size_t)8 * c;
+        // the stencil-cell sort of the feedback and the cell heads
+        const size_t M = (size_t)c;
         if ((rc = dalloc(L, (void **)&P->d_kin, M * sizeof(uint32_t)))) return rc;
         if ((rc = dalloc(L, (void **)&P->d_vin, M * sizeof(uint32_t)))) return rc;
         if ((rc = dalloc(L, (void **)&P->d_vout, M * sizeof(uint32_t)))) return rc;
-        if ((rc = dalloc(L, (void **)&P->d_dep, 3 * M * sizeof(double)))) return rc;
+        if ((rc = dalloc(L, (void **)&P->d_rec, 8 * M * sizeof(double)))) return rc;
+        if ((rc = dalloc(L, (void **)&P->d_dep, 24 * M * sizeof(double)))) return rc;
+        if ((rc = dalloc(L, (void **)&P->d_ord, M * sizeof(uint32_t)))) return rc;
         for (int b = 0; b < 2; b++)
             if ((rc = dalloc(L, (void **)&P->d_keys[b], M * sizeof(uint32_t)))) return rc;
+        const uint64_t cells = fb_cells(L);
+        if ((rc = dalloc(L, (void **)&P->d_head, cells * sizeof(int)))) return rc;
+        if (hipMemsetAsync(P->d_head, 0xFF, cells * sizeof(int), L->stream) != hipSuccess)
+            return fail(-2, "hipMemsetAsync failed");
+        // the sentinel 2^key_bits - 1 lies beyond every cell and sorts last
         P->key_bits = 1;
-        while (P->key_bits < 32 && ((uint64_t)1 << P->key_bits) <= (uint64_t)L->N) P->key_bits++;
+        while (P->key_bits < 32 && ((uint64_t)1 << P->key_bits) <= cells) P->key_bits++;
         P->sort_tmp_bytes = feedback_sort_bytes((int64_t)M, P->key_bits);
         if ((rc = dalloc(L, &P->d_sort_tmp, P->sort_tmp_bytes > 0 ? P->sort_tmp_bytes : 1))) return rc;
     }
@@ -369,26 +390,34 @@ int particles_before_step(ch4g_lattice *L, int itime)
     if (!lhas(Q->pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM) || itime < 2) return 0;
     const int par = itime & 1;
     PartParams P = part_params(L);
+    double gv[3] = {0.0, 0.0, 0.0};
+    if (lhas(Q->pc, CH4G_LAGRANGE_GRAVITY)) {
+        gv[0] = L->cfg.gravity_x;
+        gv[1] = -L->cfg.gravity_y;
+        gv[2] = L->cfg.gravity_z;
+    }
     FeedbackBufs B;
     memset(&B, 0, sizeof B);
     B.kin = Q->d_kin;
-    B.kout = Q->d_keys[par];
     B.vin = Q->d_vin;
+    B.kout = Q->d_keys[par];
     B.vout = Q->d_vout;
+    // the heads of the previous call's cells go back to -1 first
+    B.old_keys = Q->d_keys[par ^ 1];
+    B.old_n = Q->nkeys[par ^ 1];
+    B.head = Q->d_head;
+    B.rec = Q->d_rec;
     B.dep = Q->d_dep;
+    B.ord = Q->d_ord;
     B.tmp = Q->d_sort_tmp;
     B.tmp_bytes = Q->sort_tmp_bytes;
     B.key_bits = Q->key_bits;
-    B.old_keys = Q->d_keys[par];
-    B.old_n = Q->nkeys[par];
     B.fb = Q->d_fb[par];
-    if (lhas(Q->pc, CH4G_LAGRANGE_GRAVITY)) {
-        B.gv[0] = L->cfg.gravity_x;
-        B.gv[1] = -L->cfg.gravity_y;
-        B.gv[2] = L->cfg.gravity_z;
-    }
+    B.fb_float = L->dev32;
+    for (int c = 0; c < 3; c++) B.gv[c] = gv[c];
     if (launch_feedback(P, B, L->stream)) return fail(-2, "feedback launch failed: %s", hipGetErrorString(hipGetLastError()));
-    Q->nkeys[par] = 8 * Q->np;
+    Q->nkeys[par ^ 1] = 0;
+    Q->nkeys[par] = Q->np;
     return 0;
 }
 
@@ -568,8 +597,8 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     if ((rc = alloc_state(L, np))) goto err;
     if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM)) {
         for (int b = 0; b < 2; b++) {
-            if ((rc = dalloc(L, (void **)&Q->d_fb[b], (size_t)3 * L->N * sizeof(double)))) goto err;
-            if (hipMemsetAsync(Q->d_fb[b], 0, (size_t)3 * L->N * sizeof(double), L->stream) != hipSuccess) {
+            if ((rc = dalloc(L, (void **)&Q->d_fb[b], fb_bytes(L)))) goto err;
+            if (hipMemsetAsync(Q->d_fb[b], 0, fb_bytes(L), L->stream) != hipSuccess) {
                 rc = fail(-2, "hipMemsetAsync failed");
                 goto err;
             }
@@ -692,10 +721,6 @@ int ch4g_set_particles(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
     if (n > L->part->cap) {
         HIPCHK(hipStreamSynchronize(L->stream));
         if ((rc = alloc_state(L, n))) return rc;
-        // the deposit keys that located the feedback nodes are gone
-        for (int b = 0; b < 2; b++)
-            if (L->part->d_fb[b])
-                HIPCHK(hipMemsetAsync(L->part->d_fb[b], 0, (size_t)3 * L->N * sizeof(double), L->stream));
     }
     return upload(L, in, n);
 }

@@ -73,7 +73,7 @@ __device__ __forceinline__ Row row_of(const KParams &P, int y)
 // two-way coupling paths are compiled in (lattices without them run an
 // instance without those paths and their register cost).
 template <bool GEN>
-__device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab, const double *fb, const double *f0,
+__device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab, const float *fb, const double *f0,
                                            uint32_t n, int x, int y, int z, float theta, float F[3])
 {
     F[0] = P.Fc_f[0];
@@ -103,10 +103,11 @@ __device__ __forceinline__ void force_at32(const KParams &P, const HitTable &tab
         F[2] -= (float)f0[2];
     }
     if (GEN && P.twoway) {
+        // per-node sums of the deposits in particle order, rounded to float
         const size_t N = (size_t)P.N;
-        F[0] += (float)fb[n];
-        F[1] += (float)fb[N + n];
-        F[2] += (float)fb[2 * N + n];
+        F[0] += fb[n];
+        F[1] += fb[N + n];
+        F[2] += fb[2 * N + n];
     }
 }
 
@@ -333,7 +334,7 @@ __device__ __forceinline__ void fluid_moments(const KParams &P, const Row &R, in
     m.uz = m.jz * m.irho;
     if (P.forced) {
         float Fp[3];
-        force_at32<GEN>(P, P.hit_prev, P.fb_prev, P.f0p, n, x, y, z, R.offT + dTlag, Fp);
+        force_at32<GEN>(P, P.hit_prev, P.fbf_prev, P.f0p, n, x, y, z, R.offT + dTlag, Fp);
         m.ux += A(0.5) * Fp[0];
         m.uy += A(0.5) * Fp[1];
         m.uz += A(0.5) * Fp[2];
@@ -429,7 +430,7 @@ __device__ __forceinline__ void collide_fluid(const KParams &P, const Row &R, in
             omega = A(1) / tau;
         }
         float Fn[3] = {0.f, 0.f, 0.f};
-        if (P.forced) force_at32<GEN>(P, P.hit_now, P.fb_now, P.f0n, n, x, y, z, R.offT + (float)dT, Fn);
+        if (P.forced) force_at32<GEN>(P, P.hit_now, P.fbf_now, P.f0n, n, x, y, z, R.offT + (float)dT, Fn);
         const A uF = ux * Fn[0] + uy * Fn[1] + uz * Fn[2];
         // Guo term folded into the equilibrium: f* = f - omega (f - eqG) with
         // eqG = f_eq + G/omega, so the O(population) value is rounded twice.
@@ -1375,7 +1376,7 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
         double ux = jx / rho, uy = jy / rho, uz = jz / rho;
         if (P.forced) {
             float F[3];
-            force_at32<true>(P, P.hit_now, P.fb_now, P.f0n, n, x, y, z, R.offT + dTlag, F);
+            force_at32<true>(P, P.hit_now, P.fbf_now, P.f0n, n, x, y, z, R.offT + dTlag, F);
             ux += 0.5 * (double)F[0];
             uy += 0.5 * (double)F[1];
             uz += 0.5 * (double)F[2];

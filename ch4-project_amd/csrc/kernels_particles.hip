@@ -1,8 +1,8 @@
 // Lagrangian point particles on the device: one thread per particle for the
 // interpolation + advance + wall rules, an ordered re-injection pass that
 // hands out drand48 draws in particle order by jumping the 48-bit LCG, and
-// the two-way feedback as a sort of the deposits by node followed by
-// per-node sums in particle order.
+// the two-way feedback as a sort of the particles by stencil cell followed
+// by per-node sums in reference particle order.
 //
 // Compiled with -ffp-contract=off: every expression is evaluated in the
 // reference's order (lagrange.c, lagrange_feedback.c, gradients.c), so
@@ -926,39 +926,41 @@ __global__ void __launch_bounds__(PB) k_draw_apply(PartParams P)
 
 // ---------------------------------------------------------------------------
 // Two-way feedback (lagrange_feedback.c:11-256).
+//
+// The reference loops over its particles in order and adds each particle's
+// fac * f_p * w_c into the force of the eight corners of its stencil cell
+// (lagrange_feedback.c:149-176), from force_twoway = 0.  Here every particle
+// writes its stencil cell under its reference index (fac * f_p, position and
+// the index itself under its row); a stable sort by cell leaves each cell's
+// particles in reference order; and one
+// thread per node merges the lists of the eight cells that have the node as
+// a corner, smallest reference index first, so each node sums exactly the
+// reference's terms in the reference's order.  Deposits into ghost nodes
+// never reach the force (add_sendrecv_borders_vector only adds into ghosts,
+// lagrange_feedback.c:525-836): only interior nodes are summed.
 
-__global__ void k_fb_clear(const uint32_t *keys, int64_t n, double *fb, int64_t N)
+// stencil cell (im, jm, km) in -1 .. N-1, shifted by one: (NX+1)(NY+1)(NZ+1) cells
+__device__ __forceinline__ uint32_t fb_cell(const PartParams &P, int im, int jm, int km)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const uint32_t k = keys[e];
-    if ((int64_t)k >= N) return;
-    fb[k] = 0.0;
-    fb[N + k] = 0.0;
-    fb[2 * N + k] = 0.0;
+    return (uint32_t)((((int64_t)km + 1) * (P.NY + 1) + (jm + 1)) * (P.NX + 1) + (im + 1));
 }
 
-// Deposits, in reference particle order: for each of the eight corners the node when
-// it is an interior node (add_sendrecv_borders_vector only ever adds into
-// ghost nodes, lagrange_feedback.c:525-836, so ghost deposits never reach the
-// force), N (sorts last) otherwise, and the corner's force contribution
-// fac * fp * w (lagrange_feedback.c:149-176), computed here where the
-// particle rows are read in order.  Particles with zero volume (tracers)
-// deposit exact zeros and are skipped.
-__global__ void __launch_bounds__(PB) k_fb_keys(PartParams P, uint32_t *kin, uint32_t *vin, double *dep, double g0,
-                                                double g1, double g2)
+__global__ void __launch_bounds__(PB) k_fb_cells(PartParams P, uint32_t sent, uint32_t *kin, uint32_t *vin,
+                                                 double *rec, double g0, double g1, double g2)
 {
     const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
     if (p >= P.np) return;
-    const uint32_t sent = (uint32_t)P.N;
-    Stencil st;
+    const uint32_t o = (uint32_t)P.orig[p];
+    uint32_t key = sent;
     const double *S = P.s;
     const int64_t ld = P.ld;
     const double fac = S[PVOL * ld + p];
-    const bool ok = P.alive[p] && fac != 0.0 && stencil(P, S[PX * ld + p], S[PY * ld + p], S[PZ * ld + p], st);
-    double fpx = 0.0, fpy = 0.0, fpz = 0.0;
-    if (ok) {
+    const double x = S[PX * ld + p], y = S[PY * ld + p], z = S[PZ * ld + p];
+    Stencil st;
+    // tracers (zero volume) deposit exact zeros: left out
+    if (P.alive[p] && fac != 0.0 && stencil(P, x, y, z, st)) {
         const double dr = S[PDRAT * ld + p];
+        double fpx, fpy, fpz;
         if (lflag(P, CH4G_LAGRANGE_TWOWAY_SMALLTAUD)) {
             fpx = (S[PDTU * ld + p] - g0) * (1 - dr);
             fpy = (S[(PDTU + 1) * ld + p] - g1) * (1 - dr);
@@ -968,48 +970,132 @@ __global__ void __launch_bounds__(PB) k_fb_keys(PartParams P, uint32_t *kin, uin
             fpy = S[(PDTU + 1) * ld + p] - g1 + dr * (g1 - S[PAY * ld + p]);
             fpz = S[(PDTU + 2) * ld + p] - g2 + dr * (g2 - S[PAZ * ld + p]);
         }
+        key = fb_cell(P, st.im, st.jm, st.km);
+        // by row: rows are kept in cell order, so neighbouring nodes read
+        // neighbouring records
+        double *r = rec + 8 * p;
+        r[0] = fac * fpx;
+        r[1] = fac * fpy;
+        r[2] = fac * fpz;
+        // the wrapped position the stencil weights are formed from
+        r[3] = wrapd(x, (double)P.NX);
+        r[4] = wrapd(y, (double)P.NY);
+        r[5] = wrapd(z, (double)P.NZ);
+        r[6] = (double)o;  // the reference index orders the sums
     }
-    for (int c = 0; c < 8; c++) {
-        uint32_t key = sent;
-        if (ok) {
-            const int i = st.im + cdx(c), j = st.jm + cdy(c), k = st.km + cdz(c);
-            if (i >= 0 && i < P.NX && j >= 0 && j < P.NY && k >= 0 && k < P.NZ)
-                key = (uint32_t)(((int64_t)k * P.NY + j) * P.NX + i);
-        }
-        // deposits go to the slots of the particle's reference-order index:
-        // the stable sort then keeps that order within a node
-        const int64_t e = 8 * (int64_t)P.orig[p] + c;
-        kin[e] = key;
-        vin[e] = (uint32_t)e;
-        if (key != sent) {
-            dep[3 * e] = fac * fpx * st.w[c];
-            dep[3 * e + 1] = fac * fpy * st.w[c];
-            dep[3 * e + 2] = fac * fpz * st.w[c];
-        }
-    }
+    // input in reference order: the stable sort keeps that order per cell
+    kin[o] = key;
+    vin[o] = (uint32_t)p;
 }
 
-// Per node, the deposits in particle order (the sort is stable and the
-// input is particle-major): force_twoway starts at 0 and adds each
-// particle's contribution (lagrange_feedback.c:149-176).
-__global__ void __launch_bounds__(PB) k_fb_sum(const uint32_t *keys, const uint32_t *vals, const double *dep,
-                                               int64_t M, int64_t N, double *fb)
+// head[cell] = first sorted position of the cell (set), or -1 (clear)
+__global__ void __launch_bounds__(PB) k_fb_heads(const uint32_t *keys, int64_t M, uint32_t sent, int *head, int set)
 {
-    const int64_t e = (int64_t)blockIdx.x * PB + threadIdx.x;
-    if (e >= M) return;
-    const uint32_t key = keys[e];
-    if ((int64_t)key >= N) return;
-    if (e > 0 && keys[e - 1] == key) return;
+    const int64_t t = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (t >= M) return;
+    const uint32_t key = keys[t];
+    if (key == sent) return;
+    if (!set)
+        head[key] = -1;
+    else if (t == 0 || keys[t - 1] != key)
+        head[key] = (int)t;
+}
+
+// The deposits of every sorted particle, corner-major: dep[c][t][3] =
+// (fac f_p) * w_c with the stencil's weights (lagrange.c:1271-1292), w_c =
+// (X * Y) * Z, X = x - (im + 1/2) on the +x corners and (im + 3/2) - x on the
+// -x corners, likewise Y and Z; ord[t] the particle's reference index.
+__global__ void __launch_bounds__(PB) k_fb_deps(PartParams P, const uint32_t *__restrict__ keys,
+                                                const uint32_t *__restrict__ vals, const double *__restrict__ rec,
+                                                uint32_t sent, int64_t M, double *__restrict__ dep,
+                                                uint32_t *__restrict__ ord)
+{
+    const int64_t t = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (t >= M || keys[t] == sent) return;
+    const double *q = rec + 8 * (int64_t)vals[t];
+    const double px = q[3], py = q[4], pz = q[5];
+    const double im = floor(px - 0.5), jm = floor(py - 0.5), km = floor(pz - 0.5);
+    const double X[2] = {(im + 1.0 + 0.5) - px, px - (im + 0.5)};
+    const double Y[2] = {(jm + 1.0 + 0.5) - py, py - (jm + 0.5)};
+    const double Z[2] = {(km + 1.0 + 0.5) - pz, pz - (km + 0.5)};
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const double w = X[cdx(c)] * Y[cdy(c)] * Z[cdz(c)];
+        double *d = dep + 3 * ((int64_t)c * M + t);
+        d[0] = q[0] * w;
+        d[1] = q[1] * w;
+        d[2] = q[2] * w;
+    }
+    ord[t] = (uint32_t)q[6];
+}
+
+// One thread per node, 64 x 4 nodes per block (x, y), one z plane per grid
+// layer: the lists of the eight cells with this node as a corner, merged by
+// reference index.  Most nodes have no term or one.
+constexpr int FBX = 64, FBY = 4;
+
+template <typename T>
+__global__ void __launch_bounds__(FBX * FBY) k_fb_gather(PartParams P, const uint32_t *__restrict__ keys,
+                                                         const uint32_t *__restrict__ ord,
+                                                         const int *__restrict__ head,
+                                                         const double *__restrict__ dep, int64_t M, T *__restrict__ fb)
+{
+    const int i = blockIdx.x * FBX + threadIdx.x, j = blockIdx.y * FBY + threadIdx.y, k = blockIdx.z;
+    if (i >= P.NX || j >= P.NY) return;
+    const int64_t N = P.N;
+    const int64_t n = ((int64_t)k * P.NY + j) * P.NX + i;
+    // cursor into the sorted list of the cell whose corner c is this node
+    uint32_t cell[8];
+    int cur[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        cell[c] = fb_cell(P, i - cdx(c), j - cdy(c), k - cdz(c));
+        cur[c] = head[cell[c]];
+    }
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int64_t j = e; j < M && keys[j] == key; j++) {
-        const double *d = dep + 3 * (int64_t)vals[j];
+    for (;;) {
+        int nact = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) nact += cur[c] >= 0;
+        if (nact == 0) break;
+        // the next term in reference order: the smallest index at the cursors
+        int bc = -1;
+        if (nact == 1) {
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                if (cur[c] >= 0) bc = c;
+        } else {
+            uint32_t best = 0xFFFFFFFFu;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                if (cur[c] >= 0) {
+                    const uint32_t o = ord[cur[c]];
+                    if (o < best) {
+                        best = o;
+                        bc = c;
+                    }
+                }
+        }
+        int t = 0;
+        uint32_t cb = 0;
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            if (c == bc) {
+                t = cur[c];
+                cb = cell[c];
+            }
+        const double *d = dep + 3 * ((int64_t)bc * M + t);
+        const bool more = t + 1 < M && keys[t + 1] == cb;
         sx += d[0];
         sy += d[1];
         sz += d[2];
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            if (c == bc) cur[c] = more ? t + 1 : -1;
     }
-    fb[key] = sx;
-    fb[N + key] = sy;
-    fb[2 * N + key] = sz;
+    fb[n] = (T)sx;
+    fb[N + n] = (T)sy;
+    fb[2 * N + n] = (T)sz;
 }
 
 // Cell key of each particle: the node of its wrapped position (z, y, x
@@ -1100,18 +1186,30 @@ size_t feedback_sort_bytes(int64_t nitems, int key_bits)
 
 int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s)
 {
+    const uint32_t sent = B.key_bits >= 32 ? 0xFFFFFFFFu : (uint32_t)((1ull << B.key_bits) - 1);
     if (B.old_n > 0)
-        hipLaunchKernelGGL(k_fb_clear, dim3(blocks_of(B.old_n, 256)), dim3(256), 0, s, B.old_keys, B.old_n, B.fb, P.N);
-    if (P.np == 0) return hipGetLastError() == hipSuccess ? 0 : -2;
-    const int64_t M = 8 * P.np;
-    hipLaunchKernelGGL(k_fb_keys, dim3(blocks_of(P.np, PB)), dim3(PB), 0, s, P, B.kin, B.vin, B.dep, B.gv[0], B.gv[1],
-                       B.gv[2]);
-    size_t bytes = B.tmp_bytes;
-    if (hipcub::DeviceRadixSort::SortPairs(B.tmp, bytes, B.kin, B.kout, B.vin, B.vout, (int)M, 0, B.key_bits, s) !=
-        hipSuccess)
-        return -2;
-    hipLaunchKernelGGL(k_fb_sum, dim3(blocks_of(M, PB)), dim3(PB), 0, s, (const uint32_t *)B.kout,
-                       (const uint32_t *)B.vout, (const double *)B.dep, M, (int64_t)P.N, B.fb);
+        hipLaunchKernelGGL(k_fb_heads, dim3(blocks_of(B.old_n, PB)), dim3(PB), 0, s, B.old_keys, B.old_n, sent, B.head,
+                           0);
+    const int64_t M = P.np;
+    if (M > 0) {
+        hipLaunchKernelGGL(k_fb_cells, dim3(blocks_of(M, PB)), dim3(PB), 0, s, P, sent, B.kin, B.vin, B.rec, B.gv[0],
+                           B.gv[1], B.gv[2]);
+        size_t bytes = B.tmp_bytes;
+        if (hipcub::DeviceRadixSort::SortPairs(B.tmp, bytes, B.kin, B.kout, B.vin, B.vout, (int)M, 0, B.key_bits,
+                                               s) != hipSuccess)
+            return -2;
+        hipLaunchKernelGGL(k_fb_heads, dim3(blocks_of(M, PB)), dim3(PB), 0, s, (const uint32_t *)B.kout, M, sent,
+                           B.head, 1);
+        hipLaunchKernelGGL(k_fb_deps, dim3(blocks_of(M, PB)), dim3(PB), 0, s, P, (const uint32_t *)B.kout,
+                           (const uint32_t *)B.vout, (const double *)B.rec, sent, M, B.dep, B.ord);
+    }
+    const dim3 grid(blocks_of(P.NX, FBX), blocks_of(P.NY, FBY), P.NZ), block(FBX, FBY);
+    if (B.fb_float)
+        hipLaunchKernelGGL(k_fb_gather<float>, grid, block, 0, s, P, (const uint32_t *)B.kout, (const uint32_t *)B.ord,
+                           (const int *)B.head, (const double *)B.dep, M, (float *)B.fb);
+    else
+        hipLaunchKernelGGL(k_fb_gather<double>, grid, block, 0, s, P, (const uint32_t *)B.kout, (const uint32_t *)B.ord,
+                           (const int *)B.head, (const double *)B.dep, M, (double *)B.fb);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -217,7 +217,9 @@ int64_t ch4g_device_bytes(const ch4g_lattice *lat);
  * interpolate_vector_at_particles(u) [+ interpolate_scalar_at_particles(t)]
  * and move_particles (with sendrecv_particles' wrap).  With two-way coupling
  * add_particle_feedbacks enters the force of every step from the second on
- * (main.c:125-130).  Positions and fields are FP64 in every precision mode.
+ * (main.c:125-130): per-node sums in the reference's particle order, kept in
+ * double on FP64 lattices and rounded to float on float storage; it needs
+ * (NX+1)(NY+1)(NZ+1) < 2^32.  Positions are FP64 in every precision mode.
  * Interpolated fields use the two-node halo of the reference's particle build
  * (define.h:333-334 LAGRANGE_GRADIENT_BRD2, typedef.h:315-316).            */
 

@@ -10,7 +10,9 @@ channel forcing, through the C-ABI against the FP64 oracle.
 * FP32-deviation (and FP32 storage with double arithmetic): relative L2 error
   per field against the FP64 oracle <= 1e-5.
 * Two-way coupled heavy particles on a 32^3 lattice: particle records and
-  fluid populations bit-identical.
+  fluid populations bit-identical; with float storage (the same per-node
+  sums rounded to float planes) two runs bit-identical and within the FP32
+  tolerance of the oracle.
 * 512^3 (the full 3-D configuration, Ra = 1e9): initial averages against
   their closed forms, FP32-deviation against the FP64 device path after 100
   steps, mass conservation and bounded fields after 400 steps.
@@ -129,6 +131,58 @@ def test_rb3d_twoway_heavy_particles_fp64():
     assert not np.array_equal(one.get_populations()[0], pg)
 
 
+def test_rb3d_twoway_feedback_fp32():
+    """Float storage takes the feedback as float planes of the same per-node
+    sums in particle order (the FP64 path keeps them in double): the same
+    32^3 cell with 2000 heavy particles, 60 steps.  Two FP32-deviation runs
+    are bit-identical, and against the FP64 oracle the fluid velocity and
+    temperature have relative L2 <= 1e-5, positions stay within 1e-6 and
+    velocities within 1e-5 of the largest component (the particle velocities
+    follow the float fluid's velocity, L2 ~ 4e-7, in max norm).  A run without
+    the feedback is far from both: the float planes carry the coupling."""
+    c = rb3d(32, 32, 32, Ra=1e5)
+    g = 2e-3
+    cfg = c.copy(gravity_y=g, beta_t=c.beta_t * c.gravity_y / g)
+    pc = abi.make_particle_config(HEAVY | abi.LAGRANGE_TWOWAY_MOMENTUM, 2000, tau_drag=(2, 1.0, 4.0),
+                                  beta_coeff=(1, 0.6, 0.6))
+    o = Oracle(cfg)
+    o.initial_conditions()
+    o.particles_init(pc)
+    o.step(60)
+    po, ho = o.particles(), o.hydro_fields()
+    runs = []
+    for _ in range(2):
+        lat = Lattice(cfg.copy(precision=abi.FP32_DEV))
+        lat.initial_conditions()
+        lat.particles_init(pc)
+        lat.step(60)
+        runs.append((lat.particles(), lat.get_populations(), lat.hydro_fields()))
+        lat.close()
+    (p1, f1, h1), (p2, f2, h2) = runs
+    for f in abi.PARTICLE_FIELDS:
+        assert np.array_equal(p1[f], p2[f]), f
+    assert np.array_equal(f1[0], f2[0]) and np.array_equal(f1[1], f2[1])
+    assert len(p1["x"]) == len(po["x"]) > 1900
+    eu, et = rel_l2(h1["u"], ho["u"]), rel_l2(h1["t"], ho["t"])
+    xmax = max(np.abs(po[f]).max() for f in ("x", "y", "z"))
+    vmax = max(np.abs(po[f]).max() for f in ("vx", "vy", "vz"))
+    # positions compared as periodic distances (a particle may wrap in one run only)
+    ex = max(np.minimum(np.abs(p1[f] - po[f]), 32 - np.abs(p1[f] - po[f])).max() for f in ("x", "y", "z")) / xmax
+    ev = max(np.abs(p1[f] - po[f]).max() for f in ("vx", "vy", "vz")) / vmax
+    print(f"  FP32 feedback vs FP64 oracle: u {eu:.3e}  T {et:.3e}  x {ex:.3e}  v {ev:.3e}")
+    assert eu <= 1e-5 and et <= 1e-5
+    assert ex <= 1e-6 and ev <= 1e-5
+    # without the feedback the fluid is far from the coupled FP64 run
+    one = Lattice(cfg.copy(precision=abi.FP32_DEV))
+    one.initial_conditions()
+    one.particles_init(abi.make_particle_config(HEAVY, 2000, tau_drag=(2, 1.0, 4.0), beta_coeff=(1, 0.6, 0.6)))
+    one.step(60)
+    e1 = rel_l2(one.hydro_fields()["u"], ho["u"])
+    one.close()
+    print(f"  without feedback: u {e1:.3e}")
+    assert e1 > 100 * eu
+
+
 def test_rb3d_512_full_size():
     """The full 3-D configuration: 512^3, Ra = 1e9, Pr = 1, initial T linear
     plus uniform noise 1e-3.
@@ -187,8 +241,7 @@ def test_rb3d_512_twoway_particles_fp32_against_fp64():
     512^3 (the particle set of bench.py: beta 0.6 i.e. density ratio 2,
     tau_p 516, gradients, added mass, gravity, feedback momentum; the particle
     gravity 2e-5 with beta_t rescaled to keep the fluid's beta_t g): the
-    FP32-deviation run (particle rows in cell order, feedback in float-storage
-    kernels) against the FP64 device path (bit-identical to the oracle on the
+    FP32-deviation run (particle rows in cell order, float feedback planes) against the FP64 device path (bit-identical to the oracle on the
     32^3 two-way case above) after 20 steps: every particle alive and in the
     box, positions within 1e-6 (lattice units) and velocities within 1e-5 of
     the largest component; fluid velocity and temperature relative L2 <= 1e-5;
