@@ -259,8 +259,8 @@ def twoway_leg(steps=6, npart=4_000_000, N=512):
     """The particle part of the 3-D configuration: 512^3 fluid + T
     (FP32-deviation, Ra = 1e9) with 4e6 heavy particles (beta = 0.6, i.e.
     density ratio 2; tau_p = 516 steps, Stokes number ~0.5 on the Kolmogorov
-    time of Ra = 1e9) coupled back onto the fluid (feedback sorted per node,
-    summed in particle order).  The particle gravity is 2e-5 with beta_t
+    time of Ra = 1e9) coupled back onto the fluid (per-node feedback sums in
+    particle order).  The particle gravity is 2e-5 with beta_t
     rescaled so that the fluid's beta_t g is unchanged.  Device time per step
     without and with the particles."""
     import configs

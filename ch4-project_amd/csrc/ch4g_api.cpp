@@ -262,8 +262,7 @@ int launch_one_step(ch4g_lattice *L)
     if (fused) {
         int rc = ensure_float_fields(L);
         if (rc) return rc;
-        P.uf_out = L->base.fluid ? L->d_u32 : nullptr;
-        P.tf_out = L->base.thermal ? L->d_t32 : nullptr;
+        P.uT_out = L->d_uT32;
     }
     int in = L->cur, out = 1 - L->cur;
     if (L->d_f0) {
@@ -648,7 +647,7 @@ void ch4g_destroy(ch4g_lattice *L)
     }
     destroy_graphs(L);
     particles_free(L);
-    void *scr[] = {L->tlag, L->tlag_alt, L->taules, L->vel, L->d_rul, L->d_u32, L->d_t32, L->d_dens, L->d_u, L->d_t, L->d_part, L->d_res, L->d_fs, L->d_gs,
+    void *scr[] = {L->tlag, L->tlag_alt, L->taules, L->vel, L->d_rul, L->d_uT32, L->d_dens, L->d_u, L->d_t, L->d_part, L->d_res, L->d_fs, L->d_gs,
                    L->d_row32, L->d_row64};
     for (void *p : scr)
         if (p) (void)hipFree(p);

@@ -95,9 +95,10 @@ struct KParams {
     // filled by fill_float_params once the double values are final
     float Fc_f[3], grav_f[3], beta_t_f, beta2_t_f, invtau_u_f, invtau_t_f, guo_fac_f, cw_top_f, cw_bot_f;
     // when set (float storage, particles), the step kernel also writes the
-    // hydro fields of the previous step it has just gathered: u interleaved
-    // [3N] and T [N] in float, the fields the particle kernel interpolates
-    float *uf_out, *tf_out;
+    // hydro fields of the previous step it has just gathered: [N][4] float
+    // (u_x, u_y, u_z, T; 0 for a population set the lattice lacks), the
+    // fields the particle kernel interpolates
+    float4 *uT_out;
     int part;  // particles ride on the step kernel's fields (the instance that can write them)
 };
 
@@ -224,7 +225,7 @@ struct PartParams {
     int *bsum;
     uint64_t *rng;
     const double *u, *t;     // fields of the step: u interleaved [3N], t [N]
-    const float *uf, *tf;    // the same in float (float-storage lattices), else null
+    const float4 *uT;        // the same in float, [N] (u_x, u_y, u_z, T) (float-storage lattices), else null
     uint32_t flags;          // ch4g_lagrange_flag
     int fluid, thermal, wall_f, wall_t;
     double Tw_bot, Tw_top;   // T_wall - T_ref
@@ -346,7 +347,7 @@ void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, co
                         double *gpost, double *dens, double *u, double *t, hipStream_t s);
 // u [N][3] and t of the current step in float (particle interpolation on
 // float-storage lattices), either may be null
-void launch_fields_f32(const KParams &P, const float *fin, const float *gin, const float *tlag, float *uf, float *tf,
+void launch_fields_f32(const KParams &P, const float *fin, const float *gin, const float *tlag, float4 *uT,
                        hipStream_t s);
 
 // Two steps in one launch for 2-D FP32-storage lattices (NZ = 1, NX a

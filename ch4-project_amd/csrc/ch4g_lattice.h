@@ -91,7 +91,7 @@ struct ch4g_lattice {
     double *d_dens = nullptr, *d_u = nullptr, *d_t = nullptr;
     double *d_part = nullptr, *d_res = nullptr;
     double *d_rul = nullptr;  // ruler plane sums [NX + NY + NZ][NOUT]
-    float *d_u32 = nullptr, *d_t32 = nullptr;  // float fields for particles (float storage)
+    float4 *d_uT32 = nullptr;  // [N] (u_x, u_y, u_z, T) in float for particles (float storage)
     double *d_fs = nullptr, *d_gs = nullptr;
     // FP32-deviation per-row base state
     float *d_row32 = nullptr;

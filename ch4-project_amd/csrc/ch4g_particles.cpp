@@ -421,14 +421,15 @@ int particles_before_step(ch4g_lattice *L, int itime)
     return 0;
 }
 
-// Float-storage lattices interpolate float fields (half the bytes of the
-// field pass; the corner reads of 256^3 u fit the MALL).
+// Float-storage lattices interpolate float fields, 16 bytes per node (u_x,
+// u_y, u_z, T): half the bytes of the field pass, the 268 MB of 256^3 about
+// the size of the MALL, and one aligned 16-byte load per corner (u and T of a
+// node in one request).
 int ensure_float_fields(ch4g_lattice *L)
 {
     const size_t N = (size_t)L->N;
     int rc;
-    if (L->base.fluid && !L->d_u32 && (rc = dalloc(L, (void **)&L->d_u32, 3 * N * sizeof(float)))) return rc;
-    if (L->base.thermal && !L->d_t32 && (rc = dalloc(L, (void **)&L->d_t32, N * sizeof(float)))) return rc;
+    if (!L->d_uT32 && (rc = dalloc(L, (void **)&L->d_uT32, N * sizeof(float4)))) return rc;
     return 0;
 }
 
@@ -452,8 +453,7 @@ int move_particles_on_fields(ch4g_lattice *L, int64_t step)
     PartParams P = part_params(L);
     P.first = step == 1;
     if (L->dev32) {
-        P.uf = L->base.fluid ? L->d_u32 : nullptr;
-        P.tf = L->base.thermal ? L->d_t32 : nullptr;
+        P.uT = L->d_uT32;
     }
     for (int s = 0; s < L->part->substeps; s++)
         if (launch_particles(P, L->stream)) return fail(-2, "particle launch failed: %s", hipGetErrorString(hipGetLastError()));
@@ -470,7 +470,7 @@ int particles_after_step(ch4g_lattice *L)
         if ((rc = ensure_float_fields(L))) return rc;
         KParams K = params_for(L, L->first, (int)L->itime);
         launch_fields_f32(K, (const float *)L->f[L->cur], (const float *)L->g[L->cur], (const float *)L->tlag,
-                          L->base.fluid ? L->d_u32 : nullptr, L->base.thermal ? L->d_t32 : nullptr, L->stream);
+                          L->d_uT32, L->stream);
     } else {
         gather(L, nullptr, nullptr, nullptr, L->base.fluid ? L->d_u : nullptr, L->base.thermal ? L->d_t : nullptr);
     }

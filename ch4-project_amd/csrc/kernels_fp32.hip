@@ -573,12 +573,16 @@ __device__ __forceinline__ void step_node32(const KParams &P, const float *__res
         // hydro fields of the previous step (hydro_fields, lb.c:420-540) for
         // the particles: the gathered moments and the velocity with its half
         // force, as the collision used them
-        if (FLUID && P.uf_out) {
-            P.uf_out[3 * (size_t)n] = (float)ux;
-            P.uf_out[3 * (size_t)n + 1] = (float)uy;
-            P.uf_out[3 * (size_t)n + 2] = (float)uz;
+        if (P.uT_out) {
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if constexpr (FLUID) {
+                v.x = (float)ux;
+                v.y = (float)uy;
+                v.z = (float)uz;
+            }
+            if constexpr (THERMAL) v.w = (float)((sizeof(A) == 4 ? (A)R.tb : (A)P.row64[2 * y + 1]) + dT);
+            P.uT_out[n] = v;
         }
-        if (THERMAL && P.tf_out) P.tf_out[n] = (float)((sizeof(A) == 4 ? (A)R.tb : (A)P.row64[2 * y + 1]) + dT);
     }
 }
 
@@ -1330,8 +1334,7 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
                                                      const float *__restrict__ gin, const float *__restrict__ tlag,
                                                      double *__restrict__ fpost, double *__restrict__ gpost,
                                                      double *__restrict__ dens, double *__restrict__ u,
-                                                     double *__restrict__ t, float *__restrict__ uf,
-                                                     float *__restrict__ tf)
+                                                     double *__restrict__ t, float4 *__restrict__ uT)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= P.NX) return;
@@ -1342,6 +1345,8 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
     const Nbr b = make_nbr(x, y, z, P.NX, P.NY, P.NZ);
     const double rb = P.row64[2 * y], tb = P.row64[2 * y + 1];
     float dTlag = 0.f;
+    float3 fu = make_float3(0.f, 0.f, 0.f);  // the float fields, for uT
+    float ft = 0.f;
     if (P.thermal) {
         float g[Q];
         dTlag = tlag[n];
@@ -1355,7 +1360,7 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
             if (gpost) gpost[q * N + n] = wq(q) * tb + (double)g[q];
         }
         if (t) t[n] = tb + s;
-        if (tf) tf[n] = (float)(tb + s);
+        ft = (float)(tb + s);
     }
     if (P.fluid) {
         float f[Q];
@@ -1387,12 +1392,9 @@ __global__ void __launch_bounds__(256) k_gather_fp32(KParams P, const float *__r
             u[3 * n + 1] = uy;
             u[3 * n + 2] = uz;
         }
-        if (uf) {
-            uf[3 * n] = (float)ux;
-            uf[3 * n + 1] = (float)uy;
-            uf[3 * n + 2] = (float)uz;
-        }
+        fu = make_float3((float)ux, (float)uy, (float)uz);
     }
+    if (uT) uT[n] = make_float4(fu.x, fu.y, fu.z, ft);
 }
 
 // x extent of a row-grid block: 256 threads, halved for short rows
@@ -1779,17 +1781,16 @@ void launch_gather_fp32(const KParams &P, const float *fin, const float *gin, co
 {
     int bx = block_x32(P);
     dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ);
-    hipLaunchKernelGGL(k_gather_fp32, grid, dim3(bx), 0, s, P, fin, gin, tlag, fpost, gpost, dens, u, t, nullptr,
-                       nullptr);
+    hipLaunchKernelGGL(k_gather_fp32, grid, dim3(bx), 0, s, P, fin, gin, tlag, fpost, gpost, dens, u, t, nullptr);
 }
 
-void launch_fields_f32(const KParams &P, const float *fin, const float *gin, const float *tlag, float *uf, float *tf,
+void launch_fields_f32(const KParams &P, const float *fin, const float *gin, const float *tlag, float4 *uT,
                        hipStream_t s)
 {
     int bx = block_x32(P);
     dim3 grid((P.NX + bx - 1) / bx, P.NY, P.NZ);
     hipLaunchKernelGGL(k_gather_fp32, grid, dim3(bx), 0, s, P, fin, gin, tlag, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, uf, tf);
+                       nullptr, uT);
 }
 
 }  // namespace ch4g

@@ -90,26 +90,40 @@ __device__ __forceinline__ bool stencil(const PartParams &P, double x, double y,
 // Velocity at a logical node including the halo of boundary_conditions_hydro
 // (boundary_conditions.c:2786-2875): periodic images, then u_g = -u on the
 // first ghost plane of each no-slip y wall.
-// the fields of the step: double for FP64 lattices, float for float storage
+// the fields of the step at node n: double u [3N] and t [N] on FP64
+// lattices; on float storage one 16-byte (u_x, u_y, u_z, T) per node
 template <typename F>
-__device__ __forceinline__ const F *field_u(const PartParams &P);
+__device__ __forceinline__ void field_u(const PartParams &P, size_t n, double v[3]);
 template <>
-__device__ __forceinline__ const double *field_u<double>(const PartParams &P) { return P.u; }
+__device__ __forceinline__ void field_u<double>(const PartParams &P, size_t n, double v[3])
+{
+    const double *u = P.u + 3 * n;
+    v[0] = u[0];
+    v[1] = u[1];
+    v[2] = u[2];
+}
 template <>
-__device__ __forceinline__ const float *field_u<float>(const PartParams &P) { return P.uf; }
+__device__ __forceinline__ void field_u<float>(const PartParams &P, size_t n, double v[3])
+{
+    const float4 u = P.uT[n];
+    v[0] = (double)u.x;
+    v[1] = (double)u.y;
+    v[2] = (double)u.z;
+}
 template <typename F>
-__device__ __forceinline__ const F *field_t(const PartParams &P);
+__device__ __forceinline__ double field_t(const PartParams &P, size_t n);
 template <>
-__device__ __forceinline__ const double *field_t<double>(const PartParams &P) { return P.t; }
+__device__ __forceinline__ double field_t<double>(const PartParams &P, size_t n) { return P.t[n]; }
 template <>
-__device__ __forceinline__ const float *field_t<float>(const PartParams &P) { return P.tf; }
+__device__ __forceinline__ double field_t<float>(const PartParams &P, size_t n) { return (double)P.uT[n].w; }
 
-template <typename F>
-__device__ __forceinline__ void fetch_u(const PartParams &P, int i, int j, int k, double v[3])
+// node of (i, j, k) for u: periodic images, and on the first ghost plane of
+// a no-slip y wall the first fluid row (ghost set: u_g = -u)
+__device__ __forceinline__ size_t node_u(const PartParams &P, int i, int j, int k, bool &ghost)
 {
     i = wrapi(i, P.NX);
     k = wrapi(k, P.NZ);
-    bool ghost = false;
+    ghost = false;
     if (P.wall_f && j == -1) {
         j = 0;
         ghost = true;
@@ -119,27 +133,15 @@ __device__ __forceinline__ void fetch_u(const PartParams &P, int i, int j, int k
     } else {
         j = wrapi(j, P.NY);
     }
-    const size_t n = ((size_t)k * P.NY + j) * P.NX + i;
-    const F *u = field_u<F>(P) + 3 * n;
-    if (ghost) {
-        v[0] = -1.0 * (double)u[0];
-        v[1] = -1.0 * (double)u[1];
-        v[2] = -1.0 * (double)u[2];
-    } else {
-        v[0] = (double)u[0];
-        v[1] = (double)u[1];
-        v[2] = (double)u[2];
-    }
+    return ((size_t)k * P.NY + j) * P.NX + i;
 }
 
-// Temperature with the fixed-T ghost t_g = (2 (T_w - T_ref) / t - 1) t
-// (boundary_conditions.c:2928-2945).
-template <typename F>
-__device__ __forceinline__ double fetch_t(const PartParams &P, int i, int j, int k)
+// node of (i, j, k) for t, and the wall side of a ghost plane (-1 bottom, 1 top)
+__device__ __forceinline__ size_t node_t(const PartParams &P, int i, int j, int k, int &side)
 {
     i = wrapi(i, P.NX);
     k = wrapi(k, P.NZ);
-    int side = 0;
+    side = 0;
     if (P.wall_t && j == -1) {
         j = 0;
         side = -1;
@@ -149,11 +151,63 @@ __device__ __forceinline__ double fetch_t(const PartParams &P, int i, int j, int
     } else {
         j = wrapi(j, P.NY);
     }
-    const double t = (double)field_t<F>(P)[((size_t)k * P.NY + j) * P.NX + i];
+    return ((size_t)k * P.NY + j) * P.NX + i;
+}
+
+// Temperature with the fixed-T ghost t_g = (2 (T_w - T_ref) / t - 1) t
+// (boundary_conditions.c:2928-2945).
+__device__ __forceinline__ double t_ghost(const PartParams &P, double t, int side)
+{
     if (side == 0) return t;
     const double Tw = side < 0 ? P.Tw_bot : P.Tw_top;
     const double fac = 2.0 * Tw / t - 1.0;
     return fac * t;
+}
+
+template <typename F>
+__device__ __forceinline__ void fetch_u(const PartParams &P, int i, int j, int k, double v[3])
+{
+    bool ghost;
+    field_u<F>(P, node_u(P, i, j, k, ghost), v);
+    if (ghost) {
+        v[0] = -1.0 * v[0];
+        v[1] = -1.0 * v[1];
+        v[2] = -1.0 * v[2];
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ double fetch_t(const PartParams &P, int i, int j, int k)
+{
+    int side;
+    const double t = field_t<F>(P, node_t(P, i, j, k, side));
+    return t_ghost(P, t, side);
+}
+
+// u and t of one node together, for lattices whose fluid and temperature
+// walls coincide (wall_f == wall_t: both halo rules pick the same node): on
+// float storage one 16-byte load
+template <typename F>
+__device__ __forceinline__ void fetch_ut(const PartParams &P, int i, int j, int k, double v[3], double &t)
+{
+    bool ghost;
+    const size_t n = node_u(P, i, j, k, ghost);
+    if constexpr (std::is_same<F, float>::value) {
+        const float4 a = P.uT[n];
+        v[0] = (double)a.x;
+        v[1] = (double)a.y;
+        v[2] = (double)a.z;
+        t = (double)a.w;
+    } else {
+        field_u<F>(P, n, v);
+        t = field_t<F>(P, n);
+    }
+    if (ghost) {
+        v[0] = -1.0 * v[0];
+        v[1] = -1.0 * v[1];
+        v[2] = -1.0 * v[2];
+    }
+    t = t_ghost(P, t, ghost ? (j < 0 ? -1 : 1) : 0);
 }
 
 // second-order one-sided combinations with unit spacing: h1 = h2 = 1 gives
@@ -174,26 +228,12 @@ __device__ __forceinline__ void one_sided(const double a[3], const double b[3], 
 // LAGRANGE_INTERPOLATION_FIX moves a ghost node's evaluation point to the
 // first interior node (:288-368); without them it takes the one-sided
 // difference at the ghost node itself (:413-488).
+// the rows next to walls and the ghost corners of gradient_vector:
+// one-sided differences, the interpolation fix, size-one axes zeroed
 template <typename F>
-__device__ void grad_at(const PartParams &P, int i, int j, int k, double G[9])
+__device__ void grad_u_edges(const PartParams &P, int i, int j, int k, double G[9])
 {
     double a[3], b[3], c[3];
-    // centred differences: both neighbours always exist for nodes -1..N
-    fetch_u<F>(P, i + 1, j, k, a);
-    fetch_u<F>(P, i - 1, j, k, b);
-    G[0] = (a[0] - b[0]) / 2.0;
-    G[3] = (a[1] - b[1]) / 2.0;
-    G[6] = (a[2] - b[2]) / 2.0;
-    fetch_u<F>(P, i, j + 1, k, a);
-    fetch_u<F>(P, i, j - 1, k, b);
-    G[1] = (a[0] - b[0]) / 2.0;
-    G[4] = (a[1] - b[1]) / 2.0;
-    G[7] = (a[2] - b[2]) / 2.0;
-    fetch_u<F>(P, i, j, k + 1, a);
-    fetch_u<F>(P, i, j, k - 1, b);
-    G[2] = (a[0] - b[0]) / 2.0;
-    G[5] = (a[1] - b[1]) / 2.0;
-    G[8] = (a[2] - b[2]) / 2.0;
     if (P.wall_f) {
         // LB_FLUID_BC_Y: one-sided at the first and last fluid rows
         if (j == 0) {
@@ -296,6 +336,29 @@ __device__ void grad_at(const PartParams &P, int i, int j, int k, double G[9])
     if (P.NZ == 1) G[2] = G[5] = G[8] = 0.0;
 }
 
+template <typename F>
+__device__ void grad_at(const PartParams &P, int i, int j, int k, double G[9])
+{
+    double a[3], b[3];
+    // centred differences: both neighbours always exist for nodes -1..N
+    fetch_u<F>(P, i + 1, j, k, a);
+    fetch_u<F>(P, i - 1, j, k, b);
+    G[0] = (a[0] - b[0]) / 2.0;
+    G[3] = (a[1] - b[1]) / 2.0;
+    G[6] = (a[2] - b[2]) / 2.0;
+    fetch_u<F>(P, i, j + 1, k, a);
+    fetch_u<F>(P, i, j - 1, k, b);
+    G[1] = (a[0] - b[0]) / 2.0;
+    G[4] = (a[1] - b[1]) / 2.0;
+    G[7] = (a[2] - b[2]) / 2.0;
+    fetch_u<F>(P, i, j, k + 1, a);
+    fetch_u<F>(P, i, j, k - 1, b);
+    G[2] = (a[0] - b[0]) / 2.0;
+    G[5] = (a[1] - b[1]) / 2.0;
+    G[8] = (a[2] - b[2]) / 2.0;
+    grad_u_edges<F>(P, i, j, k, G);
+}
+
 // second-order one-sided difference of a scalar (SECOND_ORDER_GRAD with unit
 // spacing: a0 = -3/2, a1 = 2, a2 = -1/2 forward, the negatives backward)
 __device__ __forceinline__ double one_sided_s(double a, double b, double c, bool fwd)
@@ -311,11 +374,8 @@ __device__ __forceinline__ double one_sided_s(double a, double b, double c, bool
 // first interior node (:657-725); without them the one-sided difference at
 // the ghost node itself (:762-825); components along a size-one axis zeroed.
 template <typename F>
-__device__ void grad_t_at(const PartParams &P, int i, int j, int k, double g[3])
+__device__ void grad_t_edges(const PartParams &P, int i, int j, int k, double g[3])
 {
-    g[0] = (fetch_t<F>(P, i + 1, j, k) - fetch_t<F>(P, i - 1, j, k)) / 2.0;
-    g[1] = (fetch_t<F>(P, i, j + 1, k) - fetch_t<F>(P, i, j - 1, k)) / 2.0;
-    g[2] = (fetch_t<F>(P, i, j, k + 1) - fetch_t<F>(P, i, j, k - 1)) / 2.0;
     auto T = [&](int a, int b, int c) { return fetch_t<F>(P, a, b, c); };
     if (P.wall_t) {
         if (j == 0) g[1] = one_sided_s(T(i, 0, k), T(i, 1, k), T(i, 2, k), true);
@@ -357,6 +417,42 @@ __device__ void grad_t_at(const PartParams &P, int i, int j, int k, double g[3])
     if (P.NZ == 1) g[2] = 0.0;
 }
 
+template <typename F>
+__device__ void grad_t_at(const PartParams &P, int i, int j, int k, double g[3])
+{
+    g[0] = (fetch_t<F>(P, i + 1, j, k) - fetch_t<F>(P, i - 1, j, k)) / 2.0;
+    g[1] = (fetch_t<F>(P, i, j + 1, k) - fetch_t<F>(P, i, j - 1, k)) / 2.0;
+    g[2] = (fetch_t<F>(P, i, j, k + 1) - fetch_t<F>(P, i, j, k - 1)) / 2.0;
+    grad_t_edges<F>(P, i, j, k, g);
+}
+
+// grad u and grad t at one node from the same six neighbour reads
+template <typename F>
+__device__ void grad_ut_at(const PartParams &P, int i, int j, int k, double G[9], double g[3])
+{
+    double a[3], b[3], ta, tb;
+    fetch_ut<F>(P, i + 1, j, k, a, ta);
+    fetch_ut<F>(P, i - 1, j, k, b, tb);
+    G[0] = (a[0] - b[0]) / 2.0;
+    G[3] = (a[1] - b[1]) / 2.0;
+    G[6] = (a[2] - b[2]) / 2.0;
+    g[0] = (ta - tb) / 2.0;
+    fetch_ut<F>(P, i, j + 1, k, a, ta);
+    fetch_ut<F>(P, i, j - 1, k, b, tb);
+    G[1] = (a[0] - b[0]) / 2.0;
+    G[4] = (a[1] - b[1]) / 2.0;
+    G[7] = (a[2] - b[2]) / 2.0;
+    g[1] = (ta - tb) / 2.0;
+    fetch_ut<F>(P, i, j, k + 1, a, ta);
+    fetch_ut<F>(P, i, j, k - 1, b, tb);
+    G[2] = (a[0] - b[0]) / 2.0;
+    G[5] = (a[1] - b[1]) / 2.0;
+    G[8] = (a[2] - b[2]) / 2.0;
+    g[2] = (ta - tb) / 2.0;
+    grad_u_edges<F>(P, i, j, k, G);
+    grad_t_edges<F>(P, i, j, k, g);
+}
+
 // sendrecv_particles on one rank (lagrange.c:4060-4116): wrap (with
 // LAGRANGE_WRAP), keep the particle when its wrapped position lies in [0, S).
 __device__ __forceinline__ void wrap_keep(const PartParams &P, double &x, double &y, double &z, int64_t p)
@@ -371,14 +467,131 @@ __device__ __forceinline__ void wrap_keep(const PartParams &P, double &x, double
     if (!(px >= 0.0 && px < SX && py >= 0.0 && py < SY && pz >= 0.0 && pz < SZ)) P.alive[p] = 0;
 }
 
-// One sub-step of every particle: interpolate_vector_at_particles(u),
-// interpolate_scalar_at_particles(t) (lagrange.c:1211-1593), move_particles
-// (lagrange.c:2934-4025) and the wrap of sendrecv_particles.
-// With gradients: three waves per SIMD (167 VGPRs and a few spills) ran the
-// two-way leg's 4e6 particles in 4.54 instead of 4.88 ms; without, four
-// waves (127 VGPRs, 30 spilled) were slower than the unconstrained 143 VGPRs
-// (315 against 262 us for 2e6 particles).
+// interpolate_vector_at_particles(u) [with the gradient] and
+// interpolate_scalar_at_particles(t) [with its gradient] at the stencil
+// (lagrange.c:1211-1593): corner values weighted in the reference's order.
 template <bool GRAD, typename F>
+__device__ __forceinline__ void interpolate(const PartParams &P, const Stencil &st, double u[3], double G[9],
+                                            double &t, double gt[3])
+{
+    if (P.fluid && P.thermal && P.wall_f == P.wall_t) {
+        // both fields: each corner (and gradient neighbour) read once
+        double cu[3], tc;
+        double v0 = 0, v1 = 0, v2 = 0, s = 0;
+        for (int c = 0; c < 8; c++) {
+            fetch_ut<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), cu, tc);
+            if (c == 0) {
+                v0 = cu[0] * st.w[0];
+                v1 = cu[1] * st.w[0];
+                v2 = cu[2] * st.w[0];
+                s = tc * st.w[0];
+            } else {
+                v0 += cu[0] * st.w[c];
+                v1 += cu[1] * st.w[c];
+                v2 += cu[2] * st.w[c];
+                s = s + tc * st.w[c];
+            }
+        }
+        u[0] = v0;
+        u[1] = v1;
+        u[2] = v2;
+        t = s;
+        if constexpr (GRAD) {
+            double acc[9], Gc[9], acct[3], gc[3];
+            for (int c = 0; c < 8; c++) {
+                grad_ut_at<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), Gc, gc);
+                for (int m = 0; m < 9; m++) acc[m] = (c == 0) ? Gc[m] * st.w[0] : acc[m] + Gc[m] * st.w[c];
+                for (int m = 0; m < 3; m++) acct[m] = (c == 0) ? gc[m] * st.w[0] : acct[m] + gc[m] * st.w[c];
+            }
+            for (int m = 0; m < 9; m++) G[m] = acc[m];
+            for (int m = 0; m < 3; m++) gt[m] = acct[m];
+        }
+        return;
+    }
+    if (P.fluid) {
+        double cu[3];
+        double v0 = 0, v1 = 0, v2 = 0;
+        for (int c = 0; c < 8; c++) {
+            fetch_u<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), cu);
+            if (c == 0) {
+                v0 = cu[0] * st.w[0];
+                v1 = cu[1] * st.w[0];
+                v2 = cu[2] * st.w[0];
+            } else {
+                v0 += cu[0] * st.w[c];
+                v1 += cu[1] * st.w[c];
+                v2 += cu[2] * st.w[c];
+            }
+        }
+        u[0] = v0;
+        u[1] = v1;
+        u[2] = v2;
+        if constexpr (GRAD) {
+            double acc[9], Gc[9];
+            for (int c = 0; c < 8; c++) {
+                grad_at<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), Gc);
+                for (int m = 0; m < 9; m++) acc[m] = (c == 0) ? Gc[m] * st.w[0] : acc[m] + Gc[m] * st.w[c];
+            }
+            for (int m = 0; m < 9; m++) G[m] = acc[m];
+        }
+    }
+    if (P.thermal) {
+        double s = 0;
+        for (int c = 0; c < 8; c++) {
+            double tc = fetch_t<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c));
+            s = (c == 0) ? tc * st.w[0] : s + tc * st.w[c];
+        }
+        t = s;
+        if constexpr (GRAD) {
+            // interpolate_scalar_at_particles with LAGRANGE_GRADIENT
+            // (lagrange.c:1500-1576): the corner gradients weighted in the
+            // corner order of the value
+            double acc[3], gc[3];
+            for (int c = 0; c < 8; c++) {
+                grad_t_at<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), gc);
+                for (int m = 0; m < 3; m++) acc[m] = (c == 0) ? gc[m] * st.w[0] : acc[m] + gc[m] * st.w[c];
+            }
+            for (int m = 0; m < 3; m++) gt[m] = acc[m];
+        }
+    }
+}
+
+// The interpolation of a sub-step alone, into the particle's field rows (u,
+// grad u, t, grad t; left as they are where the stencil fails, as the
+// sub-step keeps them), for configurations with gradients: the corner and
+// neighbour reads of the gradients then run without the registers of the
+// sub-step, which follows as a stream over the rows.
+template <bool GRAD, typename F>
+__global__ void __launch_bounds__(PB) k_interp(PartParams P)
+{
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t p = (int64_t)lb * PB + threadIdx.x;
+    if (p >= P.np || !P.alive[p]) return;
+    double *S = P.s;
+    const int64_t ld = P.ld;
+    Stencil st;
+    if (!stencil(P, S[PX * ld + p], S[PY * ld + p], S[PZ * ld + p], st)) return;
+    double u[3], G[9], t, gt[3];
+    interpolate<GRAD, F>(P, st, u, G, t, gt);
+    if (P.fluid) {
+        for (int a = 0; a < 3; a++) S[(PUX + a) * ld + p] = u[a];
+        if constexpr (GRAD)
+            for (int m = 0; m < 9; m++) S[(PG + m) * ld + p] = G[m];
+    }
+    if (P.thermal) {
+        S[PT * ld + p] = t;
+        if constexpr (GRAD)
+            for (int a = 0; a < 3; a++) S[(PGT + a) * ld + p] = gt[a];
+    }
+}
+
+// One sub-step of every particle: interpolate_vector_at_particles(u),
+// interpolate_scalar_at_particles(t) (lagrange.c:1211-1593) unless ROWS (the
+// fields already in the rows, k_interp), move_particles (lagrange.c:2934-4025)
+// and the wrap of sendrecv_particles.  Without gradients four waves per SIMD
+// (spilling) were slower than the unconstrained register count; with
+// gradients the bound of three waves is kept.
+template <bool GRAD, typename F, bool ROWS>
 __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
 {
     // rows are in cell order: consecutive blocks on one XCD share corner lines
@@ -399,54 +612,17 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
         double ux = 0.0, uy = 0.0, uz = 0.0, t = 0.0;
         double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         double gt[3] = {0.0, 0.0, 0.0};
-        Stencil st;
-        const bool ok = stencil(P, x, y, z, st);
-        if (ok) {
-            if (P.fluid) {
-                double cu[3];
-                double v0 = 0, v1 = 0, v2 = 0;
-                for (int c = 0; c < 8; c++) {
-                    fetch_u<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), cu);
-                    if (c == 0) {
-                        v0 = cu[0] * st.w[0];
-                        v1 = cu[1] * st.w[0];
-                        v2 = cu[2] * st.w[0];
-                    } else {
-                        v0 += cu[0] * st.w[c];
-                        v1 += cu[1] * st.w[c];
-                        v2 += cu[2] * st.w[c];
-                    }
-                }
-                ux = v0;
-                uy = v1;
-                uz = v2;
-                if constexpr (GRAD) {
-                    double acc[9], Gc[9];
-                    for (int c = 0; c < 8; c++) {
-                        grad_at<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), Gc);
-                        for (int m = 0; m < 9; m++) acc[m] = (c == 0) ? Gc[m] * st.w[0] : acc[m] + Gc[m] * st.w[c];
-                    }
-                    for (int m = 0; m < 9; m++) G[m] = acc[m];
-                }
-            }
-            if (P.thermal) {
-                double s = 0;
-                for (int c = 0; c < 8; c++) {
-                    double tc = fetch_t<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c));
-                    s = (c == 0) ? tc * st.w[0] : s + tc * st.w[c];
-                }
-                t = s;
-                if constexpr (GRAD) {
-                    // interpolate_scalar_at_particles with LAGRANGE_GRADIENT
-                    // (lagrange.c:1500-1576): the corner gradients weighted
-                    // in the corner order of the value
-                    double acc[3], gc[3];
-                    for (int c = 0; c < 8; c++) {
-                        grad_t_at<F>(P, st.im + cdx(c), st.jm + cdy(c), st.km + cdz(c), gc);
-                        for (int m = 0; m < 3; m++) acc[m] = (c == 0) ? gc[m] * st.w[0] : acc[m] + gc[m] * st.w[c];
-                    }
-                    for (int m = 0; m < 3; m++) gt[m] = acc[m];
-                }
+        // ROWS: k_interp has written the fields of this sub-step into the rows
+        bool ok = false;
+        if constexpr (!ROWS) {
+            Stencil st;
+            ok = stencil(P, x, y, z, st);
+            if (ok) {
+                double u[3];
+                interpolate<GRAD, F>(P, st, u, G, t, gt);
+                ux = u[0];
+                uy = u[1];
+                uz = u[2];
             }
         }
         if (!(ok && P.fluid)) {
@@ -812,13 +988,15 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
             R(PAYO) = ayo;
             R(PAZO) = azo;
         }
-        R(PUX) = ux;
-        R(PUY) = uy;
-        R(PUZ) = uz;
+        if constexpr (!ROWS) {
+            R(PUX) = ux;
+            R(PUY) = uy;
+            R(PUZ) = uz;
+        }
         R(PUXO) = uxo;
         R(PUYO) = uyo;
         R(PUZO) = uzo;
-        if constexpr (GRAD)
+        if constexpr (GRAD && !ROWS)
             for (int m = 0; m < 9; m++) R(PG + m) = G[m];
         if (am) {
             R(PDTU) = dtu[0];
@@ -838,10 +1016,10 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
             }
         }
         if (P.thermal) {
-            R(PT) = t;
+            if constexpr (!ROWS) R(PT) = t;
             R(PTOLD) = told;
             R(PDTT) = dtt;
-            if constexpr (GRAD) {
+            if constexpr (GRAD && !ROWS) {
                 R(PGT) = gt[0];
                 R(PGT + 1) = gt[1];
                 R(PGT + 2) = gt[2];
@@ -1143,16 +1321,24 @@ int launch_particles(const PartParams &P, hipStream_t s)
     if (P.np == 0) return 0;
     const int nb = blocks_of(P.np, PB);
     const bool grad = (P.flags & CH4G_LAGRANGE_GRADIENT) != 0;
-    if (P.uf || P.tf) {
-        if (grad)
-            hipLaunchKernelGGL((k_particles<true, float>), dim3(nb), dim3(PB), 0, s, P);
-        else
-            hipLaunchKernelGGL((k_particles<false, float>), dim3(nb), dim3(PB), 0, s, P);
+    // with gradients the interpolation runs as its own kernel and the
+    // sub-step streams the interpolated rows (18.8 against 19.1-19.4 ms per
+    // step of the 512^3 two-way configuration); without, one kernel (0.25
+    // against 0.28 ms per step for the 2e6 particles of HIT 256^3)
+    if (P.uT) {
+        if (grad) {
+            hipLaunchKernelGGL((k_interp<true, float>), dim3(nb), dim3(PB), 0, s, P);
+            hipLaunchKernelGGL((k_particles<true, float, true>), dim3(nb), dim3(PB), 0, s, P);
+        } else {
+            hipLaunchKernelGGL((k_particles<false, float, false>), dim3(nb), dim3(PB), 0, s, P);
+        }
     } else {
-        if (grad)
-            hipLaunchKernelGGL((k_particles<true, double>), dim3(nb), dim3(PB), 0, s, P);
-        else
-            hipLaunchKernelGGL((k_particles<false, double>), dim3(nb), dim3(PB), 0, s, P);
+        if (grad) {
+            hipLaunchKernelGGL((k_interp<true, double>), dim3(nb), dim3(PB), 0, s, P);
+            hipLaunchKernelGGL((k_particles<true, double, true>), dim3(nb), dim3(PB), 0, s, P);
+        } else {
+            hipLaunchKernelGGL((k_particles<false, double, false>), dim3(nb), dim3(PB), 0, s, P);
+        }
     }
     if (P.bsum) {
         hipLaunchKernelGGL(k_draw_offsets, dim3(1), dim3(1024), 0, s, P.bsum, nb, P.rng);
