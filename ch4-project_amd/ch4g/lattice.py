@@ -21,7 +21,7 @@ _lib = None
 EXPORTS = [
     "ch4g_last_error", "ch4g_device_count", "ch4g_create", "ch4g_destroy", "ch4g_initial_conditions",
     "ch4g_set_populations", "ch4g_get_populations", "ch4g_step", "ch4g_hydro_fields", "ch4g_dump_averages",
-    "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_dump_rulers", "ch4g_step_timed", "ch4g_steps_per_launch", "ch4g_set_graphs", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
+    "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_dump_rulers", "ch4g_step_timed", "ch4g_steps_per_launch", "ch4g_set_graphs", "ch4g_set_fusion", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
     "ch4g_footprint_bytes", "ch4g_particles_init", "ch4g_particles_count", "ch4g_get_particles",
     "ch4g_set_particles", "ch4g_step_batch", "ch4g_serve", "ch4g_dump_averages_noalloc", "ch4g_particle_size",
 ]
@@ -58,6 +58,7 @@ def lib():
     L.ch4g_set_hit_phases.argtypes = [P, dp]
     L.ch4g_step_timed.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
     L.ch4g_set_graphs.argtypes = [P, C.c_int]
+    L.ch4g_set_fusion.argtypes = [P, C.c_int]
     L.ch4g_steps_per_launch.argtypes = [P, C.POINTER(C.c_int)]
     L.ch4g_synchronize.argtypes = [P]
     L.ch4g_itime.argtypes = [P]
@@ -180,6 +181,10 @@ class Lattice:
 
     def set_graphs(self, enable):
         check(self._L.ch4g_set_graphs(self._h, 1 if enable else 0))
+
+    def set_fusion(self, enable):
+        """Two steps per launch on or off (ch4g_set_fusion); results are identical."""
+        check(self._L.ch4g_set_fusion(self._h, 1 if enable else 0))
 
     def synchronize(self):
         check(self._L.ch4g_synchronize(self._h))

@@ -316,7 +316,7 @@ namespace ch4g {
 // two_step_ok): the lag moves to the other array.
 bool two_step_eligible(const ch4g_lattice *L)
 {
-    return L->dev32 && !L->first && !L->part && two_step_ok(L->base);
+    return L->use_fusion && L->dev32 && !L->first && !L->part && two_step_ok(L->base);
 }
 
 int ensure_tlag_alt(ch4g_lattice *L)
@@ -812,7 +812,19 @@ int ch4g_steps_per_launch(const ch4g_lattice *L, int *steps)
     if (!L || !steps) return fail(-1, "null argument");
     ch4g::KParams P = L->base;
     P.first = 0;  // steady state
-    *steps = (L->dev32 && !L->part && two_step_ok(P)) ? 2 : 1;
+    *steps = (L->use_fusion && L->dev32 && !L->part && two_step_ok(P)) ? 2 : 1;
+    return 0;
+}
+
+int ch4g_set_fusion(ch4g_lattice *L, int enable)
+{
+    if (!L) return fail(-1, "null lattice");
+    if (L->use_fusion != (enable != 0)) {
+        // captured step graphs hold the launches of the old setting
+        HIPCHK(hipStreamSynchronize(L->stream));
+        destroy_graphs(L);
+    }
+    L->use_fusion = enable != 0;
     return 0;
 }
 

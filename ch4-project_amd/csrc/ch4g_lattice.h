@@ -104,6 +104,7 @@ struct ch4g_lattice {
     void *tlag_alt = nullptr;
     int tpar = 0;
     int use_graphs = 1;
+    int use_fusion = 1;      // ch4g_set_fusion
     int64_t bytes = 0;
     ch4g::Particles *part = nullptr;
 };

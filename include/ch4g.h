@@ -190,6 +190,11 @@ int ch4g_step_timed(ch4g_lattice *lat, int nsteps, double *elapsed_ms);
  * library).                                                                 */
 int ch4g_set_graphs(ch4g_lattice *lat, int enable);
 
+/* Two steps per launch where ch4g_steps_per_launch says so (default on);
+ * off: one step per launch.  Results are identical either way.  No
+ * reference counterpart (launch mechanics of this library).                 */
+int ch4g_set_fusion(ch4g_lattice *lat, int enable);
+
 /* Steps one steady-state launch advances: 2 for float-storage lattices
  * without HIT forcing, LES, NOZEROMODE or two-way particles that are 2-D
  * (k_step2_fp32: two steps through an LDS tile) or 3-D with at least 2^21

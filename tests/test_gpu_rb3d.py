@@ -183,6 +183,37 @@ def test_rb3d_twoway_feedback_fp32():
     assert e1 > 100 * eu
 
 
+@pytest.mark.parametrize("kind", ["rb", "channel"])
+def test_two_steps_per_launch_against_single_steps(kind):
+    """A 3-D float-storage lattice of 2^21 nodes runs two steps per launch
+    (k_step2z_fp32, x-y tiles marching along z through LDS rings); with
+    ch4g_set_fusion(0) it runs one step per launch.  Populations and hydro
+    fields bit-identical after 25 steps (an odd count: the last step single
+    in both), with graph replay on; RB 128^3 fluid + T and a fluid-only
+    channel 128^3."""
+    if kind == "rb":
+        cfg = rb3d(128, 128, 128, precision=abi.FP32_DEV, Ra=1e6)
+    else:
+        cfg = configs.channel3d(128, 128, 128, g=1e-5)
+    res = []
+    for fusion in (True, False):
+        lat = Lattice(cfg)
+        lat.set_fusion(fusion)
+        assert lat.steps_per_launch() == (2 if fusion else 1)
+        lat.initial_conditions()
+        lat.step(25)
+        res.append((lat.get_populations(), lat.hydro_fields()))
+        lat.close()
+    (f1, h1), (f2, h2) = res
+    for a, b in zip(f1, f2):
+        if a is not None:
+            assert np.array_equal(a, b)
+    for k in h1:
+        if h1[k] is not None:
+            assert np.array_equal(h1[k], h2[k]), k
+    assert np.abs(h1["u"]).max() > 0
+
+
 def test_rb3d_512_full_size():
     """The full 3-D configuration: 512^3, Ra = 1e9, Pr = 1, initial T linear
     plus uniform noise 1e-3.
