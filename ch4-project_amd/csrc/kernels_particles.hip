@@ -632,13 +632,23 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
             if constexpr (GRAD)
                 for (int m = 0; m < 9; m++) G[m] = R(PG + m);
         }
-        const double tau = R(PTAU), beta = R(PBETA);
+        const double tau = R(PTAU);
         const bool inertial = tau > 0.0, moved = tau == 0.0 || inertial;
         // rows a configuration never changes are neither read nor written:
         // the gradient without LAGRANGE_GRADIENT, Du/Dt without added mass,
-        // the temperature rows without LB_TEMPERATURE
+        // the temperature rows without LB_TEMPERATURE; beta is read only by
+        // the switches that use it, the velocity not by tracers (which take
+        // the fluid's)
         const bool am = lflag(P, CH4G_LAGRANGE_ADDEDMASS);
-        double vx = R(PVX), vy = R(PVY), vz = R(PVZ);
+        const bool use_beta = am || lflag(P, CH4G_LAGRANGE_SMALLTAUD_BETA | CH4G_LAGRANGE_SMALLTAUD_FLOATER |
+                                               CH4G_LAGRANGE_BC_INELASTIC_REINJECT);
+        const double beta = use_beta ? R(PBETA) : 0.0;
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        if (tau != 0.0) {
+            vx = R(PVX);
+            vy = R(PVY);
+            vz = R(PVZ);
+        }
         double vxo = R(PVXO), vyo = R(PVYO), vzo = R(PVZO);
         double ax = 0.0, ay = 0.0, az = 0.0, axo = 0.0, ayo = 0.0, azo = 0.0;
         double uxo = 0.0, uyo = 0.0, uzo = 0.0;
