@@ -435,7 +435,10 @@ int ch4g_step_batch(ch4g_lattice *const *lats, int n, int nsteps)
 int ch4g_serve(const ch4g_serve_params *sp, ch4g_job *jobs, int32_t njobs)
 {
     if (!sp || (njobs > 0 && !jobs) || njobs < 0) return fail(-1, "bad arguments");
-    const int chunk = sp->chunk_steps > 0 ? sp->chunk_steps : 32;
+    // 128 launches between admission points: the mixed ensemble ran 23.05
+    // (32), 23.60 (128), 23.72 (512) GLUPS batched against 23.33 one job at a
+    // time, with p50 latency unchanged (2.6-2.7 s)
+    const int chunk = sp->chunk_steps > 0 ? sp->chunk_steps : 128;
     const bool batched = sp->mode == CH4G_SERVE_BATCHED;
     int64_t budget = sp->hbm_budget;
     if (budget <= 0) {

@@ -352,7 +352,7 @@ enum ch4g_schedule {
 typedef struct ch4g_serve_params {
     int64_t hbm_budget;   /* device bytes the jobs may hold at once (0: 90% of free memory) */
     int32_t mode;         /* enum ch4g_serve_mode */
-    int32_t chunk_steps;  /* launches between admission points (0: 32); replayed from a captured graph */
+    int32_t chunk_steps;  /* launches between admission points (0: 128); replayed from a captured graph */
     int32_t max_active;   /* lattices resident at once (0: no limit but the budget) */
     int32_t use_graphs;   /* capture the chunk's launches in a hipGraph and replay it */
     int32_t schedule;     /* enum ch4g_schedule */
