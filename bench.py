@@ -329,8 +329,8 @@ def main():
         print(json.dumps({"batch_compare": batch_compare()}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (one lattice per GPU; multi-GPU slab decomposition "
-              "is a design in DESIGN.md, not built). Use --gpus 1.", file=sys.stderr)
+        print("bench.py: runs on one GPU only (one lattice per GPU; a slab decomposition over several GPUs "
+              "is not built). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
     from ch4g import abi
     r32 = gpu_leg(abi.FP32_DEV, args.steps, args.warmup, args.workload)

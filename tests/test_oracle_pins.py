@@ -129,8 +129,8 @@ def test_poiseuille_steady_closed_form(tau):
     """Steady force-driven channel: the BGK/Guo/halfway-bounce-back lattice
     solution is the parabola plus a tau-dependent slip,
     u(y) = g y (H-y) / (2 nu) + (g/nu)(16 tau^2 - 16 tau + 1)/24, y = j + 1/2,
-    exact in exact arithmetic (derivation: tests/test_oracle_pins.py docstring
-    and DESIGN.md).  The slip is ~1e-4 of u_max; the tolerance is 1e-9."""
+    exact in exact arithmetic (derivation in this file's module docstring).
+    The slip is ~1e-4 of u_max; the tolerance is 1e-9."""
     H, g = 16, 1e-6
     nu = (tau - 0.5) / 3
     o = Oracle(configs.poiseuille(2, H, 1, tau=tau, g=g))
