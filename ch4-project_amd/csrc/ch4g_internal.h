@@ -211,7 +211,11 @@ enum PartField {
     PSWIM,              // swim_velocity
     PNO,                // 3 rows: nx, ny, nz (second orientation)
     PDTN = PNO + 3,     // 3 rows: dt_nx, dt_ny, dt_nz
-    PF_COUNT = PDTN + 3
+    PTP = PDTN + 3,     // t_p, the particle's temperature
+    PCP,                // cp_p
+    PTFAC,              // time_dt_lagr / tau_heat of lagrange.c:4011-4013, host arithmetic
+    PTEXP,              // exp(-PTFAC), glibc (the same bits as the reference's call)
+    PF_COUNT
 };
 
 struct PartParams {
@@ -229,6 +233,7 @@ struct PartParams {
     uint32_t flags;          // ch4g_lagrange_flag
     int fluid, thermal, wall_f, wall_t;
     double Tw_bot, Tw_top;   // T_wall - T_ref
+    double T_bot, T_top;     // the walls' temperatures a re-injected particle takes (LAGRANGE_TEMPERATURE)
     int first;               // itime == 1 (Euler start)
     double dt;               // time_dt_lagr
     double grav[3];
@@ -237,6 +242,9 @@ struct PartParams {
 
 // one sub-step: interpolation + move + walls (+ re-injection draws) + wrap
 int launch_particles(const PartParams &P, hipStream_t s);
+// t, its gradient and t_p = t at the particles from the FP64 field P.t
+// (LAGRANGE_INITIAL_TEMPERATURE_FLUID)
+int launch_particle_tinit(const PartParams &P, hipStream_t s);
 // Two-way feedback (lagrange_feedback.c:11-256): every particle's deposits
 // summed per node in reference particle order, written to dense planes
 // fb[3][N] (double on FP64 lattices, float on float storage).  The particles

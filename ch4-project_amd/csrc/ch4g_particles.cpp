@@ -72,6 +72,19 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
     if (lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE) &&
         (pc->swim_velocity_types < 1 || pc->swim_velocity_max < pc->swim_velocity_min))
         return fail(-1, "Error in swim_velocity parameters");
+    // particle temperature: t_p exists with LB_TEMPERATURE only (typedef.h:467-470);
+    // the log ladder and the fluid-temperature start sit inside
+    // #ifdef LAGRANGE_TEMPERATURE (lagrange.c:587, :1005); parameters.c:616
+    const bool tpart = lhas(*pc, CH4G_LAGRANGE_TEMPERATURE);
+    if (tpart && !L->base.thermal) return fail(-1, "LAGRANGE_TEMPERATURE needs LB_TEMPERATURE");
+    // the relaxation rate reads beta_coeff, a member only with LAGRANGE_GRADIENT
+    // and LAGRANGE_ADDEDMASS (typedef.h:408-413)
+    if (tpart && !am) return fail(-1, "LAGRANGE_TEMPERATURE needs LAGRANGE_GRADIENT and LAGRANGE_ADDEDMASS");
+    if (!tpart && (lhas(*pc, CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG) ||
+                   lhas(*pc, CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID)))
+        return fail(-1, "LAGRANGE_TEMPERATURE_INCREMENT_LOG / LAGRANGE_INITIAL_TEMPERATURE_FLUID need LAGRANGE_TEMPERATURE");
+    if (tpart && (pc->cp_coeff_types < 1 || pc->cp_coeff_max < pc->cp_coeff_min))
+        return fail(-1, "Error in cp_coeff parameters");
     if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_SMALLTAUD) && !lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM))
         return fail(-1, "LAGRANGE_TWOWAY_SMALLTAUD needs LAGRANGE_TWOWAY_MOMENTUM");
     // the feedback sorts 32-bit stencil-cell keys with a sentinel above every cell
@@ -102,6 +115,20 @@ void ladder(std::vector<double> &tab, int ntr, double trval, double types, doubl
                 const int at = ntr + j + i * reps + k * reps * ntyp;
                 if (at < nt) tab[at] = logs ? vmin * pow(step, (double)i) : vmin + i * step;
             }
+}
+
+// time_dt_lagr / tau_heat of the particle temperature (lagrange.c:4011-4013):
+// 2 kappa / (tau_drag nu cp_p (3 - beta)) times the sub-step.  It holds only
+// per-particle constants, so it is formed once here and the kernel reads it
+// with its exponential (glibc exp: the bits the reference's call produces).
+// Tracers (tau_drag 0, cp_p 0) get +inf and, as in the reference, t_p NaN.
+void heat_constants(double kappa, double nu, double dt, double tau, double beta, double cp, double *fac,
+                    double *efac)
+{
+    double fac1 = (kappa * 2.0) / (tau * nu * cp * (3 - beta));
+    fac1 *= dt;
+    *fac = fac1;
+    *efac = exp(-fac1);
 }
 
 // particle volume and density ratio of the feedback (lagrange_feedback.c:84-90)
@@ -235,6 +262,8 @@ PartParams part_params(ch4g_lattice *L)
     P.wall_t = L->base.wall_t;
     P.Tw_bot = L->base.Tw_bot;
     P.Tw_top = L->base.Tw_top;
+    P.T_bot = L->cfg.T_bot;
+    P.T_top = L->cfg.T_top;
     P.first = L->itime == 1;
     P.dt = Q->dt;
     P.grav[0] = L->cfg.gravity_x;
@@ -304,6 +333,13 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         S(PSWIM) = r.swim_velocity;
         const double nv[6] = {r.nx, r.ny, r.nz, r.dt_nx, r.dt_ny, r.dt_nz};
         for (int m = 0; m < 6; m++) S(PNO + m) = nv[m];
+        S(PTP) = r.t_p;
+        S(PCP) = r.cp_p;
+        double hf = 0.0, he = 0.0;
+        if (lhas(Q->pc, CH4G_LAGRANGE_TEMPERATURE))
+            heat_constants(L->kappa, L->nu, Q->dt, r.tau_drag, r.beta_coeff, r.cp_p, &hf, &he);
+        S(PTFAC) = hf;
+        S(PTEXP) = he;
         alive[(size_t)p] = 1;
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
@@ -511,8 +547,10 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     const bool jeff = orient && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY);
     const bool gyro = jeff && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
     const bool swim = orient && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
+    const bool tpart = lhas(*pc, CH4G_LAGRANGE_TEMPERATURE);
     double types = pc->tau_drag_types;
     if (am) types *= pc->beta_coeff_types;
+    if (tpart) types *= pc->cp_coeff_types;  // parameters.c:615
     if (jeff) types *= pc->aspect_ratio_types;
     if (gyro) types *= pc->gyrotaxis_velocity_types;
     if (swim) types *= pc->swim_velocity_types;
@@ -520,7 +558,7 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     types += ntr;
     const int nt = (int)types;
     std::vector<double> tau((size_t)nt, 0.0), beta((size_t)nt, 0.0), asp((size_t)nt, 0.0), gyv((size_t)nt, 0.0),
-        swv((size_t)nt, 0.0);
+        swv((size_t)nt, 0.0), cpv((size_t)nt, 0.0);
     double rep = 1.0;
     ladder(tau, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
            lhas(*pc, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
@@ -544,6 +582,11 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         ladder(swv, ntr, 0.0, pc->swim_velocity_types, pc->swim_velocity_min, pc->swim_velocity_max, false, rep);
         rep *= pc->swim_velocity_types;
     }
+    if (tpart) { // after the orientation ladders (lagrange.c:579-617)
+        ladder(cpv, ntr, 0.0, pc->cp_coeff_types, pc->cp_coeff_min, pc->cp_coeff_max,
+               lhas(*pc, CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG), rep);
+        rep *= pc->cp_coeff_types;
+    }
 
     // names, properties and positions (lagrange.c:634-641, :711-771): three
     // draws per particle from the lattice's stream, then with orientation
@@ -564,6 +607,10 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         if (jeff) r.aspect_ratio = asp[(size_t)type];
         if (gyro) r.gyrotaxis_velocity = gyv[(size_t)type];
         if (swim) r.swim_velocity = swv[(size_t)type];
+        if (tpart) {
+            r.cp_p = cpv[(size_t)type];  // lagrange.c:730-732
+            r.t_p = L->cfg.T_bot;        // lagrange.c:862-864
+        }
         if (orient) {
             // random_vector with glibc sincos for each angle: the call gcc
             // makes of sin(a) and cos(a) together; separate sin and cos
@@ -612,6 +659,16 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         if ((rc = rng_to_device(L))) goto err;
     }
     if ((rc = upload(L, h.data(), np))) goto err;
+    if (lhas(*pc, CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID)) {
+        // the temperature of the initial state (FP64 planes on every storage)
+        if ((rc = ensure_scratch(L))) goto err;
+        gather(L, nullptr, nullptr, nullptr, nullptr, L->d_t);
+        const PartParams P = part_params(L);
+        if (launch_particle_tinit(P, L->stream) || hipStreamSynchronize(L->stream) != hipSuccess) {
+            rc = fail(-2, "particle temperature initialisation failed: %s", hipGetErrorString(hipGetLastError()));
+            goto err;
+        }
+    }
     L->base.part = fields_fused(L) ? 1 : 0;
     return 0;
 err:
@@ -705,6 +762,8 @@ int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t
             r.swim_velocity = S(PSWIM);
             double *nv = &r.nx;
             for (int m = 0; m < 6; m++) nv[m] = S(PNO + m);
+            r.t_p = S(PTP);
+            r.cp_p = S(PCP);
         }
         k++;
     }

@@ -585,6 +585,27 @@ __global__ void __launch_bounds__(PB) k_interp(PartParams P)
     }
 }
 
+// LAGRANGE_INITIAL_TEMPERATURE_FLUID (lagrange.c:1004-1015):
+// interpolate_scalar_at_particles(t) at the initial positions (t and, with
+// the gradient, dx_t..dz_t) and t_p = t, on the FP64 temperature field with
+// the halo rules of every sub-step.  P.fluid is 0: the temperature alone.
+template <bool GRAD>
+__global__ void __launch_bounds__(PB) k_tinit(PartParams P)
+{
+    const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
+    if (p >= P.np || !P.alive[p]) return;
+    double *S = P.s;
+    const int64_t ld = P.ld;
+    Stencil st;
+    if (!stencil(P, S[PX * ld + p], S[PY * ld + p], S[PZ * ld + p], st)) return;
+    double u[3] = {0.0, 0.0, 0.0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, t = 0.0, gt[3] = {0.0, 0.0, 0.0};
+    interpolate<GRAD, double>(P, st, u, G, t, gt);
+    S[PT * ld + p] = t;
+    if constexpr (GRAD)
+        for (int a = 0; a < 3; a++) S[(PGT + a) * ld + p] = gt[a];
+    S[PTP * ld + p] = t;
+}
+
 // One sub-step of every particle: interpolate_vector_at_particles(u),
 // interpolate_scalar_at_particles(t) (lagrange.c:1211-1593) unless ROWS (the
 // fields already in the rows, k_interp), move_particles (lagrange.c:2934-4025)
@@ -920,7 +941,10 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
         }
 
         // ---- wall rules at y (lagrange.c:3754-3993); x and z of a random
-        // re-injection are filled in by the ordered draw pass
+        // re-injection are filled in by the ordered draw pass.  A re-injected
+        // particle takes the temperature of the wall it re-enters at (the
+        // later of the two rules wins, as in the reference's sequence)
+        int rewall = 0;  // 1: re-entered at the top (T_top), 2: at the bottom (T_bot)
         if (P.wall_f) {
             const bool inel = lflag(P, CH4G_LAGRANGE_BC_INELASTIC);
             const bool reinj = lflag(P, CH4G_LAGRANGE_BC_INELASTIC_REINJECT);
@@ -953,6 +977,7 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                         vy /= 2.0;
                         vz /= 2.0;
                     }
+                    rewall = 1;  // lagrange.c:3839-3842
                 }
                 if (lflag(P, CH4G_LAGRANGE_ORIENTATION_BC) && pv[1] < 0.0) pv[1] *= -1.0;
             }
@@ -975,11 +1000,23 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                         vy = -gy * (1.0 - beta) * tau;
                         vz = -gz * (1.0 - beta) * tau;
                     }
+                    rewall = 2;  // lagrange.c:3915-3918
                 }
                 if (lflag(P, CH4G_LAGRANGE_ORIENTATION_BC) && pv[1] > 0.0) pv[1] *= -1.0;
             }
         }
         if (nd == 0) wrap_keep(P, x, y, z, p);
+
+        // ---- particle temperature (lagrange.c:4007-4020): relaxation to the
+        // fluid's t over the heat response time, T_p <- (T_p + fac1 t)
+        // exp(-fac1); fac1 and exp(-fac1) are per-particle constants formed
+        // on the host when the rows are loaded
+        if (P.thermal && lflag(P, CH4G_LAGRANGE_TEMPERATURE)) {
+            double tp = rewall == 1 ? P.T_top : rewall == 2 ? P.T_bot : R(PTP);
+            const double fac1 = R(PTFAC);
+            tp = (tp + fac1 * t) * R(PTEXP);
+            R(PTP) = tp;
+        }
 
         R(PX) = x;
         R(PY) = y;
@@ -1354,6 +1391,21 @@ int launch_particles(const PartParams &P, hipStream_t s)
         hipLaunchKernelGGL(k_draw_offsets, dim3(1), dim3(1024), 0, s, P.bsum, nb, P.rng);
         hipLaunchKernelGGL(k_draw_apply, dim3(nb), dim3(PB), 0, s, P);
     }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_particle_tinit(const PartParams &P, hipStream_t s)
+{
+    if (P.np == 0) return 0;
+    if (!P.t) return -1;
+    PartParams Q = P;
+    Q.fluid = 0;
+    Q.uT = nullptr;
+    const int nb = blocks_of(P.np, PB);
+    if (P.flags & CH4G_LAGRANGE_GRADIENT)
+        hipLaunchKernelGGL(k_tinit<true>, dim3(nb), dim3(PB), 0, s, Q);
+    else
+        hipLaunchKernelGGL(k_tinit<false>, dim3(nb), dim3(PB), 0, s, Q);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -366,14 +366,15 @@ static double ptypes(const ch4g_particle_config *pc)
         }
         if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) types *= pc->swim_velocity_types;
     }
+    if (fl & CH4G_LAGRANGE_TEMPERATURE) types *= pc->cp_coeff_types; /* parameters.c:615 */
     if (fl & CH4G_LAGRANGE_ADD_TRACER) types += 1.0;
     return types;
 }
 
 /* dump_particle_averages (lagrange.c:2598-2930): per particle type, FP64
  * averages of v, v^2, v^4, a, a^2, a^4 (and t, t^2, t^4, dt_t, dt_t^2 with
- * LB_TEMPERATURE), accumulated in particle order, appended to
- * particle_averages.dat every time_dump_diagn.                             */
+ * LB_TEMPERATURE, then t_p with LAGRANGE_TEMPERATURE), accumulated in
+ * particle order, appended to particle_averages.dat every time_dump_diagn. */
 void dump_particle_averages(void)
 {
     need_particles("dump_particle_averages");
@@ -389,7 +390,8 @@ void dump_particle_averages(void)
     const unsigned fl = rp->pcfg.flags;
     const int orient = (fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ORIENTATION);
     const int second = orient && (fl & CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
-    enum { NV = 41 };
+    const int tpart = (fl & CH4G_LAGRANGE_TEMPERATURE) != 0;
+    enum { NV = 42 };
     double *acc = (double *)calloc((size_t)nt * NV, sizeof(double));
     double *cnt = (double *)calloc((size_t)nt, sizeof(double));
     if (!acc || !cnt) die("dump_particle_averages: out of memory");
@@ -439,6 +441,7 @@ void dump_particle_averages(void)
         o[38] += pow(p->dt_nx, 4.0);
         o[39] += pow(p->dt_ny, 4.0);
         o[40] += pow(p->dt_nz, 4.0);
+        o[41] += p->t_p;
     }
     FILE *fo = fopen("particle_averages.dat", "a");
     if (fo) {
@@ -452,8 +455,10 @@ void dump_particle_averages(void)
                 for (int k = 23; k < 32; k++) fprintf(fo, "%e ", o[k] * norm);
             if (second) /* lagrange.c:2897-2902: dn/dt moments */
                 for (int k = 32; k < 41; k++) fprintf(fo, "%e ", o[k] * norm);
-            if (rp->cfg.flags & CH4G_LB_TEMPERATURE)
+            if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
                 for (int k = 18; k < 23; k++) fprintf(fo, "%e ", o[k] * norm);
+                if (tpart) fprintf(fo, "%e ", o[41] * norm); /* lagrange.c:2908-2911 */
+            }
             fprintf(fo, "\n");
         }
         fclose(fo);
@@ -642,7 +647,8 @@ void read_pop_h5(void)
  * fluid velocity and its previous value; the velocity gradient with
  * LAGRANGE_GRADIENT, then beta_coeff with added mass and the orientation
  * members with LAGRANGE_ORIENTATION; t, and with the gradient dx_t..dz_t,
- * with LB_TEMPERATURE) over [particle_number] records, the particles in
+ * with LB_TEMPERATURE, then temperature_particle and cp_particle with
+ * LAGRANGE_TEMPERATURE) over [particle_number] records, the particles in
  * array order first (the reference sizes the dataset by particle_number and
  * writes the npart it holds); /properties carries particle_types.          */
 static hid_t particle_type(const ch4g_run_params *rp)
@@ -674,6 +680,10 @@ static hid_t particle_type(const ch4g_run_params *rp)
         M(t);
         if (fl & CH4G_LAGRANGE_GRADIENT) {
             M(dx_t); M(dy_t); M(dz_t);
+        }
+        if (fl & CH4G_LAGRANGE_TEMPERATURE) { /* lagrange.c:4442-4447 */
+            H5Tinsert(t, "temperature_particle", HOFFSET(ch4g_particle, t_p), H5T_NATIVE_DOUBLE);
+            H5Tinsert(t, "cp_particle", HOFFSET(ch4g_particle, cp_p), H5T_NATIVE_DOUBLE);
         }
     }
 #undef M
@@ -799,6 +809,9 @@ static int snapshot_members(const ch4g_run_params *rp, struct snap_member *m)
         if (grad) {
             SM("dx_t", dx_t); SM("dy_t", dy_t); SM("dz_t", dz_t);
         }
+        if (fl & CH4G_LAGRANGE_TEMPERATURE) { /* lagrange.c:2090-2103 */
+            SM("temperature_particle", t_p); SM("cp_particle", cp_p);
+        }
     }
 #undef SM
     return n;
@@ -852,7 +865,7 @@ static void write_particle_snapshot(void)
     hid_t fspace = H5Screate_simple(1, dims, NULL);
     hid_t mspace = H5Screate_simple(1, count, NULL);
     H5Sselect_hyperslab(fspace, H5S_SELECT_SET, offset, NULL, count, NULL);
-    struct snap_member m[56];
+    struct snap_member m[64];
     const int nm = snapshot_members(rp, m);
     for (int k = 0; k < nm; k++) {
         for (int64_t i = 0; i < n; i++) aux[i] = *(const double *)((const char *)&a[i] + m[k].off);
@@ -920,6 +933,10 @@ static void write_particle_snapshot(void)
         if (grad) {
             const char *gt[3] = {"dx_t", "dy_t", "dz_t"};
             xmf_joined(fo, h5, "temperature gradient", "Vector", gt, 3, np);
+        }
+        if (fl & CH4G_LAGRANGE_TEMPERATURE) { /* lagrange.c:2515-2528 */
+            xmf_scalar(fo, h5, "temperature_particle", "temperature_particle", np);
+            xmf_scalar(fo, h5, "cp_particle", "cp_particle", np);
         }
     }
     fprintf(fo, "<Time Value=\" %e\" />\n</Grid>\n</Domain>\n</Xdmf>\n", rp->time_start + S.itime * rp->time_dt);

@@ -71,6 +71,9 @@ static const struct { const char *name; unsigned flag; } LFLAGS[] = {
     {"LAGRANGE_ORIENTATION_ACTIVE", CH4G_LAGRANGE_ORIENTATION_ACTIVE},
     {"LAGRANGE_ORIENTATION_BC", CH4G_LAGRANGE_ORIENTATION_BC},
     {"LAGRANGE_ORIENTATION_SECONDORIENTATION", CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION},
+    {"LAGRANGE_TEMPERATURE", CH4G_LAGRANGE_TEMPERATURE},
+    {"LAGRANGE_TEMPERATURE_INCREMENT_LOG", CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG},
+    {"LAGRANGE_INITIAL_TEMPERATURE_FLUID", CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -246,6 +249,18 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     if (!(out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_JEFFREY))
         out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG |
                                        CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
+    /* the cp ladder's log steps and the fluid-temperature start sit inside
+       #ifdef LAGRANGE_TEMPERATURE (lagrange.c:587, :1005) */
+    if (!(out->pcfg.flags & CH4G_LAGRANGE_TEMPERATURE))
+        out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG |
+                                       CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID);
+    /* point_particle has t_p only with LB_TEMPERATURE (typedef.h:467-470) */
+    if ((out->pcfg.flags & CH4G_LAGRANGE_TEMPERATURE) && !(c->flags & CH4G_LB_TEMPERATURE))
+        return perr("%s", "LAGRANGE_TEMPERATURE needs LB_TEMPERATURE");
+    /* its relaxation rate reads beta_coeff (typedef.h:408-413) */
+    if ((out->pcfg.flags & CH4G_LAGRANGE_TEMPERATURE) &&
+        (~out->pcfg.flags & (CH4G_LAGRANGE_GRADIENT | CH4G_LAGRANGE_ADDEDMASS)))
+        return perr("%s", "LAGRANGE_TEMPERATURE needs LAGRANGE_GRADIENT and LAGRANGE_ADDEDMASS");
     if (!out->lagrange && out->pcfg.flags) return perr("%s", "LAGRANGE_* switches without LAGRANGE");
 
     static kv t[1024];
@@ -333,6 +348,11 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
                  need(t, np, "swim_velocity_max", &pc->swim_velocity_max)))
                 return -1;
         }
+        /* parameters.c:606-616 */
+        if ((fl & CH4G_LAGRANGE_TEMPERATURE) &&
+            (need(t, np, "cp_coeff_types", &pc->cp_coeff_types) || need(t, np, "cp_coeff_min", &pc->cp_coeff_min) ||
+             need(t, np, "cp_coeff_max", &pc->cp_coeff_max)))
+            return -1;
     }
     /* keys of this implementation (optional) */
     c->precision = CH4G_FP64;

@@ -252,7 +252,10 @@ enum ch4g_lagrange_flag {
     CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS       = 1u << 19, /* lagrange.c:445-470, :3533-3566 */
     CH4G_LAGRANGE_ORIENTATION_ACTIVE                  = 1u << 20, /* tracers swim along p, lagrange.c:502-521, :3179-3186 */
     CH4G_LAGRANGE_ORIENTATION_BC                      = 1u << 21, /* p_y turned away from a y wall, lagrange.c:3866-3870, :3942-3946 */
-    CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION       = 1u << 22  /* second vector n normal to p, lagrange.c:904-924, :3586-3740 */
+    CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION       = 1u << 22, /* second vector n normal to p, lagrange.c:904-924, :3586-3740 */
+    CH4G_LAGRANGE_TEMPERATURE                         = 1u << 23, /* particle temperature t_p relaxing to t, lagrange.c:4007-4020 (needs LB_TEMPERATURE) */
+    CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG           = 1u << 24, /* cp_coeff ladder, lagrange.c:579-617 */
+    CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID           = 1u << 25  /* t_p starts at the fluid's t, lagrange.c:1004-1015 */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -265,6 +268,7 @@ typedef struct ch4g_particle_config {
     double   aspect_ratio_types, aspect_ratio_min, aspect_ratio_max;                   /* ORIENTATION_JEFFREY */
     double   gyrotaxis_velocity_types, gyrotaxis_velocity_min, gyrotaxis_velocity_max; /* ..._GYROTAXIS */
     double   swim_velocity_types, swim_velocity_min, swim_velocity_max;                /* ORIENTATION_ACTIVE */
+    double   cp_coeff_types, cp_coeff_min, cp_coeff_max;                               /* LAGRANGE_TEMPERATURE */
 } ch4g_particle_config;
 
 /* One particle: the members of the reference's point_particle
@@ -283,12 +287,14 @@ typedef struct ch4g_particle {
     double px, py, pz, dt_px, dt_py, dt_pz;             /* orientation and its last dp/dt (LAGRANGE_ORIENTATION) */
     double aspect_ratio, gyrotaxis_velocity, swim_velocity;
     double nx, ny, nz, dt_nx, dt_ny, dt_nz;             /* second orientation (..._SECONDORIENTATION) */
+    double t_p, cp_p;                                   /* particle temperature and specific heat (LAGRANGE_TEMPERATURE) */
 } ch4g_particle;
 
 /* No counterpart in the reference.  sizeof(ch4g_particle) as this library
  * was built: callers that exchange particle records check it against their
  * own header before ch4g_get_particles / ch4g_set_particles (the record grew
- * with the orientation members; a stale header would mis-stride the arrays). */
+ * with the orientation and particle-temperature members; a stale header
+ * would mis-stride the arrays).                                             */
 int64_t ch4g_particle_size(void);
 
 /* allocate_particles (lagrange.c:6-49) + initial_conditions_particles
@@ -296,8 +302,13 @@ int64_t ch4g_particle_size(void);
  * drawn x, y, z per particle from the lattice's drand48 stream (which has
  * already served initial_conditions and the first build_forcing, as in
  * main.c:23-51), then with LAGRANGE_ORIENTATION the particle's random unit
- * vector p (random_vector, random.c:139-150: two more draws), zero velocity.  Call after ch4g_initial_conditions (or
- * ch4g_set_populations) and before the first step.                          */
+ * vector p (random_vector, random.c:139-150: two more draws), zero velocity;
+ * with LAGRANGE_TEMPERATURE the cp_coeff ladder (after the orientation
+ * ones, lagrange.c:579-617) and t_p = T_bot, or with
+ * LAGRANGE_INITIAL_TEMPERATURE_FLUID the fluid temperature interpolated at
+ * the particle (t and, with the gradient, dx_t..dz_t set with it).  Call
+ * after ch4g_initial_conditions (or ch4g_set_populations) and before the
+ * first step.                                                               */
 int ch4g_particles_init(ch4g_lattice *lat, const ch4g_particle_config *pcfg);
 
 /* Number of particles alive (sendrecv_particles drops a particle whose

@@ -18,12 +18,15 @@
  *   boundary_conditions_hydro     boundary_conditions.c:2786-2997
  *   interpolate_*_at_particles    lagrange.c:1211-1593
  *   gradient_vector (LAGRANGE)    gradients.c:78-499
- *   move_particles                lagrange.c:2934-4025
+ *   move_particles                lagrange.c:2934-4025 (thermal relaxation of
+ *                                 t_p :4007-4020, wall temperatures of a
+ *                                 re-injected particle :3839-3842, :3915-3918)
  *   sendrecv_particles (1 rank)   lagrange.c:4028-4221
  *   add_particle_feedbacks        lagrange_feedback.c:11-256, 525-836
  * Pins: tests/test_oracle_particles.py.  Parity unpinned: none of these
- * functions calls libm except pow(r^2, 3/2) of the particle volume, which is
- * computed with the same glibc call by the device library's host code.
+ * functions calls libm except pow(r^2, 3/2) of the particle volume and
+ * exp(-fac1) of the particle temperature, which the device library's host
+ * code computes with the same glibc calls on the same operands.
  */
 #include <float.h>
 #include <math.h>
@@ -88,6 +91,9 @@ static void ladder(double *tab, int ntypes, int ntr, double trval, double types,
             }
 }
 
+static void bc_hydro(oracle *o);
+static void interpolate_fields(oracle *o, int fluid, int thermal);
+
 int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
 {
     olagr_free(o);
@@ -117,8 +123,10 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     const int jeff = orient && LHAS(L, CH4G_LAGRANGE_ORIENTATION_JEFFREY);
     const int gyro = jeff && LHAS(L, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
     const int swim = orient && LHAS(L, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
+    const int tpart = LHAS(L, CH4G_LAGRANGE_TEMPERATURE);
     double types = pc->tau_drag_types;
     if (am) types *= pc->beta_coeff_types;
+    if (tpart) types *= pc->cp_coeff_types; /* parameters.c:615 */
     if (jeff) types *= pc->aspect_ratio_types;
     if (gyro) types *= pc->gyrotaxis_velocity_types;
     if (swim) types *= pc->swim_velocity_types;
@@ -127,8 +135,8 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     const int nt = (int)types;
     double *tau = (double *)calloc(nt, sizeof(double)), *beta = (double *)calloc(nt, sizeof(double));
     double *asp = (double *)calloc(nt, sizeof(double)), *gyv = (double *)calloc(nt, sizeof(double));
-    double *swv = (double *)calloc(nt, sizeof(double));
-    if (!tau || !beta || !asp || !gyv || !swv) return -1;
+    double *swv = (double *)calloc(nt, sizeof(double)), *cpv = (double *)calloc(nt, sizeof(double));
+    if (!tau || !beta || !asp || !gyv || !swv || !cpv) return -1;
     double rep = 1.0;
     ladder(tau, nt, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
            LHAS(L, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
@@ -153,6 +161,12 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
         ladder(swv, nt, ntr, 0.0, pc->swim_velocity_types, pc->swim_velocity_min, pc->swim_velocity_max, 0, rep);
         rep *= pc->swim_velocity_types;
     }
+    /* lagrange.c:579-617: cp_coeff after the orientation ladders, 0 for the tracer */
+    if (tpart) {
+        ladder(cpv, nt, ntr, 0.0, pc->cp_coeff_types, pc->cp_coeff_min, pc->cp_coeff_max,
+               LHAS(L, CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG), rep);
+        rep *= pc->cp_coeff_types;
+    }
 
     L->npart = pc->particle_number;
     L->tr = (ch4g_particle *)calloc(L->npart > 0 ? L->npart : 1, sizeof(ch4g_particle));
@@ -173,6 +187,10 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
         if (jeff) p->aspect_ratio = asp[type];
         if (gyro) p->gyrotaxis_velocity = gyv[type];
         if (swim) p->swim_velocity = swv[type];
+        if (tpart) {
+            p->cp_p = cpv[type];     /* lagrange.c:730-732 */
+            p->t_p = o->cfg.T_bot;   /* lagrange.c:862-864 */
+        }
         if (orient) { /* random_vector (random.c:139-150), lagrange.c:891-901 */
             double theta = acos(2.0 * oracle_drand48(o) - 1.0);
             double phi = 2 * 3.14159265359 * oracle_drand48(o);
@@ -200,6 +218,18 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     free(asp);
     free(gyv);
     free(swv);
+    free(cpv);
+    /* lagrange.c:1004-1015: interpolate_scalar_at_particles(t) (t and, with
+     * the gradient, dx_t..dz_t) and t_p = t.  At that point of main.c
+     * (:31-51) hydro_fields has written the interior only; the halo is
+     * filled here by boundary_conditions_hydro's rules, as for every
+     * sub-step, so the values of particles within one node of a face follow
+     * those rules rather than memory the reference has not set.             */
+    if (tpart && LHAS(L, CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID)) {
+        bc_hydro(o);
+        interpolate_fields(o, 0, 1);
+        for (int i = 0; i < L->npart; i++) L->tr[i].t_p = L->tr[i].t;
+    }
     return 0;
 }
 
@@ -521,8 +551,8 @@ static ovec grad_sca(const oracle *o, int i, int j, int k)
 
 /* interpolate_vector_at_particles(u,'u') and interpolate_scalar_at_particles
  * (t,'t') (lagrange.c:1211-1593), with the gradients of both under
- * LAGRANGE_GRADIENT.                                                       */
-static void interpolate(oracle *o)
+ * LAGRANGE_GRADIENT; `fluid` / `thermal` select which of the two runs.    */
+static void interpolate_fields(oracle *o, int fluid, int thermal)
 {
     olagr *L = o->lagr;
     const int grad = LHAS(L, CH4G_LAGRANGE_GRADIENT);
@@ -530,7 +560,7 @@ static void interpolate(oracle *o)
         ch4g_particle *p = &L->tr[ip];
         ostencil s;
         if (stencil(o, p, &s)) continue;
-        if (HAS(o, CH4G_LB_FLUID)) {
+        if (fluid && HAS(o, CH4G_LB_FLUID)) {
             double vx = L->u[s.n[0]].x * s.w[0], vy = L->u[s.n[0]].y * s.w[0], vz = L->u[s.n[0]].z * s.w[0];
             for (int c = 1; c < 8; c++) {
                 vx += L->u[s.n[c]].x * s.w[c];
@@ -561,7 +591,7 @@ static void interpolate(oracle *o)
                 p->dz_uz = a[8];
             }
         }
-        if (HAS(o, CH4G_LB_TEMPERATURE)) {
+        if (thermal && HAS(o, CH4G_LB_TEMPERATURE)) {
             double v = L->t[s.n[0]] * s.w[0];
             for (int c = 1; c < 8; c++) v += L->t[s.n[c]] * s.w[c];
             p->t = v;
@@ -581,6 +611,8 @@ static void interpolate(oracle *o)
         }
     }
 }
+
+static void interpolate(oracle *o) { interpolate_fields(o, 1, 1); }
 
 /* move_particles (lagrange.c:2934-4025) for the switches of ch4g.h, then
  * sendrecv_particles (lagrange.c:4028-4221) on one rank.                   */
@@ -849,6 +881,8 @@ static void move_at(oracle *o, int first)
                         p->vy /= 2.0;
                         p->vz /= 2.0;
                     }
+                    /* the particle takes the temperature of the wall it re-enters at */
+                    if (LHAS(L, CH4G_LAGRANGE_TEMPERATURE)) p->t_p = o->cfg.T_top;
                 }
                 /* LAGRANGE_ORIENTATION_BC (lagrange.c:3866-3870) */
                 if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_BC) && p->py < 0.0) p->py *= -1.0;
@@ -877,10 +911,19 @@ static void move_at(oracle *o, int first)
                         p->vy = -gy * (1.0 - p->beta_coeff) * p->tau_drag;
                         p->vz = -gz * (1.0 - p->beta_coeff) * p->tau_drag;
                     }
+                    if (LHAS(L, CH4G_LAGRANGE_TEMPERATURE)) p->t_p = o->cfg.T_bot;
                 }
                 /* LAGRANGE_ORIENTATION_BC (lagrange.c:3942-3946) */
                 if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_BC) && p->py > 0.0) p->py *= -1.0;
             }
+        }
+        /* particle temperature (lagrange.c:4007-4020): first-order relaxation
+         * to the fluid's t, T_p <- (T_p + fac1 t) exp(-fac1), fac1 the
+         * sub-step over the heat response time */
+        if (HAS(o, CH4G_LB_TEMPERATURE) && LHAS(L, CH4G_LAGRANGE_TEMPERATURE)) {
+            double fac1 = (o->kappa * 2.0) / (p->tau_drag * o->nu * p->cp_p * (3 - p->beta_coeff));
+            fac1 *= dt;
+            p->t_p = (p->t_p + fac1 * p->t) * exp(-fac1);
         }
     }
     /* sendrecv_particles: wrap, keep the particles inside [0, S) (mesh

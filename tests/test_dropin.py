@@ -97,6 +97,31 @@ def test_read_run_params_particles():
         assert lib.ch4g_read_run_params(d.encode(), os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == -1
 
 
+def test_read_run_params_particle_temperature(tmp_path):
+    """LAGRANGE_TEMPERATURE reads the cp_coeff ladder (parameters.c:606-616);
+    its log and fluid-start variants act only inside it; it needs
+    LB_TEMPERATURE and the added-mass beta_coeff its rate reads."""
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    text = open(os.path.join(FIXP, "define.h")).read()
+    d, pin = tmp_path / "define.h", tmp_path / "param.in"
+    base = open(os.path.join(FIXP, "param.in")).read()
+    d.write_text(text + "#define LAGRANGE_TEMPERATURE\n#define LAGRANGE_TEMPERATURE_INCREMENT_LOG\n"
+                 "#define LAGRANGE_INITIAL_TEMPERATURE_FLUID\n")
+    pin.write_text(base)
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == -1  # no cp_coeff keys
+    pin.write_text(base + "cp_coeff_types 3\ncp_coeff_min 0.5\ncp_coeff_max 2\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == 0
+    t = abi.LAGRANGE_TEMPERATURE | abi.LAGRANGE_TEMPERATURE_INCREMENT_LOG | abi.LAGRANGE_INITIAL_TEMPERATURE_FLUID
+    assert rp.pcfg.flags & t == t
+    assert (rp.pcfg.cp_coeff_types, rp.pcfg.cp_coeff_min, rp.pcfg.cp_coeff_max) == (3.0, 0.5, 2.0)
+    d.write_text(text + "#define LAGRANGE_TEMPERATURE_INCREMENT_LOG\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == 0
+    assert not rp.pcfg.flags & t
+    d.write_text(text.replace("#define LAGRANGE_ADDEDMASS\n", "") + "#define LAGRANGE_TEMPERATURE\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == -1
+
+
 def test_read_run_params_rejects_unimplemented_switch(tmp_path):
     d = tmp_path / "define.h"
     d.write_text(open(os.path.join(FIX, "define.h")).read() + "#define LB_FLUID_BC_Y_P_SLIP\n")
@@ -376,20 +401,28 @@ ORIENT_PARAMS = ("aspect_ratio_types 3\naspect_ratio_min 0.5\naspect_ratio_max 4
                  "swim_velocity_types 2\nswim_velocity_min 0\nswim_velocity_max 0.001\n")
 
 
+TPART_DEFINES = "#define LAGRANGE_TEMPERATURE\n"
+TPART_PARAMS = "cp_coeff_types 2\ncp_coeff_min 0.5\ncp_coeff_max 2\n"
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(H5LIB), reason="serial HDF5 not installed")
-@pytest.mark.parametrize("orient", [False, True])
-def test_particle_snapshots(tmp_path, orient):
+@pytest.mark.parametrize("orient,tpart", [(False, False), (True, False), (False, True)],
+                         ids=["plain", "orientation", "particle_temperature"])
+def test_particle_snapshots(tmp_path, orient, tpart):
     """OUTPUT_H5 with LAGRANGE: output_particles writes RUN/particle_<itime>.h5
     every time_dump_lagr (lagrange.c:1598-2595), one FP64 dataset per member
     under /lagrange in the reference's order, equal to the oracle's records
     (FP64 path: bit-identical), and an XDMF descriptor naming those
     datasets; with the orientation switches also their members and the dp/dt
-    moments of particle_averages.dat."""
+    moments of particle_averages.dat; with LAGRANGE_TEMPERATURE the
+    particle's temperature and heat capacity (lagrange.c:2090-2103,
+    :2515-2528) and the t_p column of particle_averages.dat."""
     import xml.etree.ElementTree as ET
     define = open(os.path.join(FIXP, "define.h")).read() + "#define OUTPUT_H5\n#define LAGRANGE_OUTPUT_FLUID\n"
-    (tmp_path / "define.h").write_text(define + (ORIENT_DEFINES if orient else ""))
-    (tmp_path / "param.in").write_text(open(os.path.join(FIXP, "param.in")).read() + (ORIENT_PARAMS if orient else ""))
+    (tmp_path / "define.h").write_text(define + (ORIENT_DEFINES if orient else "") + (TPART_DEFINES if tpart else ""))
+    (tmp_path / "param.in").write_text(open(os.path.join(FIXP, "param.in")).read() + (ORIENT_PARAMS if orient else "")
+                                       + (TPART_PARAMS if tpart else ""))
     r = subprocess.run([RUN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     for it in (0, 40, 80, 120):
@@ -416,10 +449,16 @@ def test_particle_snapshots(tmp_path, orient):
         assert np.unique(ref["aspect_ratio"]).size == 4  # the ladder and the tracer's 1
         rows = [l.split() for l in open(tmp_path / "particle_averages.dat")]
         assert {len(r) for r in rows} == {2 + 18 + 9 + 9 + 5}
+    if tpart:
+        members += ["temperature_particle", "cp_particle"]
+        assert np.unique(ref["cp_p"]).size == 3  # the ladder and the tracer's 0
+        rows = [l.split() for l in open(tmp_path / "particle_averages.dat")]
+        assert {len(r) for r in rows} == {2 + 18 + 5 + 1}
     path = tmp_path / "RUN" / "particle_120.h5"
+    rename = {"temperature": "t", "temperature_particle": "t_p", "cp_particle": "cp_p"}
     for m in members:
         got = read_dataset_h5(path, "/lagrange/" + m, npart)
-        want = ref["t" if m == "temperature" else m]
+        want = ref[rename.get(m, m)]
         np.testing.assert_array_equal(got[:n], want, err_msg=m)
     root = ET.parse(tmp_path / "RUN" / "particle_120.xmf").getroot()
     refs = [e.text.strip() for e in root.iter("DataItem") if e.text and e.text.strip()]

@@ -49,8 +49,8 @@ def compare_particles(o, lat, exact=True, pos_atol=0.0, rel=0.0):
     bad = []
     for f in FIELDS:
         x, y = a[f], b[f]
-        if exact:
-            if not np.array_equal(x, y):
+        if exact:  # NaN where the reference's arithmetic gives NaN (tracers' t_p)
+            if not np.array_equal(x, y, equal_nan=True):
                 i = int(np.argmax(x != y))
                 bad.append(f"{f}[{i}] oracle {x[i]!r} gpu {y[i]!r}")
     assert not bad, bad[:8]
@@ -161,6 +161,51 @@ def test_orientation_needs_the_gradient():
     pc = abi.make_particle_config(abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_JEFFREY, 4)
     with pytest.raises(Exception, match="LAGRANGE_GRADIENT"):
         lat.particles_init(pc)
+
+
+TPART = HEAVY | abi.LAGRANGE_TEMPERATURE
+
+
+@pytest.mark.parametrize("extra", [
+    abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT,
+    abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT | abi.LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM |
+    abi.LAGRANGE_TEMPERATURE_INCREMENT_LOG | abi.LAGRANGE_INITIAL_TEMPERATURE_FLUID,
+], ids=["reinject", "norandom_log_fluid_start"])
+def test_particle_temperature_fp64(extra):
+    """LAGRANGE_TEMPERATURE: the cp ladder, t_p relaxing to the fluid's t
+    every sub-step (lagrange.c:4007-4020), the wall temperature of a
+    re-injected particle, and the start at T_bot or at the fluid's t:
+    every record bit-identical to the oracle (tracers' t_p NaN in both)."""
+    pc = abi.make_particle_config(TPART | extra, 256, tau_drag=(2, 2.0, 20.0), beta_coeff=(2, 0.2, 0.5),
+                                  cp_coeff=(3, 0.5, 4.0), time_dt_lagr_factor=2)
+    o, lat = pair(rb_small(32, 16, g=5e-3), pc, 120)
+    a = lat.particles()
+    assert np.unique(a["cp_p"]).size == 4
+    fin = np.isfinite(a["t_p"])
+    assert fin.sum() > 0 and np.ptp(a["t_p"][fin]) > 0
+    compare_particles(o, lat)
+
+
+def test_particle_temperature_fp32_dev_close_to_fp64():
+    """Float-storage lattice: t_p within 1e-5 of the wall temperature
+    difference of the FP64 oracle's after 100 steps."""
+    pc = abi.make_particle_config(TPART, 300, tau_drag=(2, 2.0, 8.0), beta_coeff=(1, 0.6, 0.6),
+                                  cp_coeff=(2, 0.5, 2.0))
+    o = Oracle(rb_small())
+    o.initial_conditions()
+    o.particles_init(pc)
+    o.step(100)
+    cfg = rb_small(precision=abi.FP32_DEV)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.particles_init(pc)
+    lat.step(100)
+    a, b = o.particles(), lat.particles()
+    assert len(a) == len(b) and np.array_equal(np.isnan(a["t_p"]), np.isnan(b["t_p"]))
+    fin = np.isfinite(a["t_p"])
+    err = np.abs(a["t_p"][fin] - b["t_p"][fin]).max() / abs(cfg.T_bot - cfg.T_top)
+    print(f"  t_p: max error / |T_bot - T_top| = {err:.3e}")
+    assert err <= 1e-5
 
 
 def test_substeps_fp64():

@@ -26,6 +26,12 @@ a wrong ghost sign, a missing factor, an off-by-one cell) fails it:
                                      shear (closed form in the shear plane,
                                      orbit constant in 3-D); the gyrotaxis
                                      term by hand; swimming in a straight line
+* particle temperature            <- cp ladder after the orientation ones;
+                                     decay rate of a sphere at Nusselt
+                                     number 2 with radius and density from
+                                     tau_drag and beta; the scheme's fixed
+                                     point; re-injection wall temperatures;
+                                     the conduction profile at the start
 """
 import ctypes
 import ctypes.util
@@ -512,3 +518,157 @@ def test_second_orientation_stays_normal_to_p():
     assert np.abs((p * n).sum(1)).max() < 1e-4
     assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-15)
     assert np.abs(n - n0).max() > 0.05  # n did turn
+
+
+# --------------------------------------------------------------------------
+# Particle temperature (LAGRANGE_TEMPERATURE, lagrange.c:579-617, :862-864,
+# :1004-1015, :3839-3842, :3915-3918, :4007-4020).
+TPART = abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS | abi.LAGRANGE_TEMPERATURE
+
+
+def _thermal(NX=6, NY=8, NZ=4, flags=0, tau_u=0.8, **kw):
+    kw.setdefault("tau_t", 0.6)
+    return quiescent(NX, NY, NZ, flags=abi.LB_TEMPERATURE | flags, tau=tau_u, **kw)
+
+
+def test_particle_temperature_ladder_and_start():
+    """cp_coeff steps vary slowest, after the orientation ladders (the
+    tracer gets 0); every particle starts at T_bot; the log switch gives
+    min * step^i, step = (max/min)^(1/(types-1))."""
+    o, _ = _thermal(T_bot=1.25, T_top=0.25)
+    flags = (TPART | abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_ACTIVE)
+    o.particles_init(abi.make_particle_config(flags, 60, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.5, 0.5),
+                                              swim_velocity=(2, 0.0, 0.01), cp_coeff=(3, 0.5, 2.0)))
+    a = o.particles()
+    ntypes = 2 * 2 * 3 + 1
+    for r in a:
+        t = int(r["name"]) % ntypes
+        if t == 0:
+            assert r["cp_p"] == 0.0 and r["tau_drag"] == 0.0
+            continue
+        h = t - 1
+        assert r["tau_drag"] == 1.0 + (h % 2) * 2.0
+        assert r["swim_velocity"] == ((h // 2) % 2) * 0.01
+        assert r["cp_p"] == 0.5 + ((h // 4) % 3) * 0.75
+    assert np.all(a["t_p"] == 1.25)
+    o2, _ = _thermal()
+    o2.particles_init(abi.make_particle_config(TPART | abi.LAGRANGE_TEMPERATURE_INCREMENT_LOG, 9,
+                                               tau_drag=(1, 2.0, 2.0), beta_coeff=(1, 0.5, 0.5),
+                                               cp_coeff=(3, 0.5, 2.0)))
+    assert np.array_equal(o2.particles()["cp_p"][:3], [0.5, 1.0, 2.0])
+
+
+def _heat_rate(nu, kappa, tau, beta, cp):
+    """1 / tau_heat of a small sphere from its heat transfer at Nusselt
+    number 2 (no convection): m c_p dT_p/dt = 4 pi a^2 (k / a) (T - T_p),
+    i.e. 1/tau_h = 3 kappa rho_f c_f / (a^2 rho_p c_p) with the fluid's
+    c_f = 1; the radius from the Stokes response time with added mass,
+    a^2 = 3 nu beta tau (lagrange_feedback.c:87), and the density ratio
+    rho_p / rho_f = (3 - beta) / (2 beta) (lagrange_feedback.c:84)."""
+    a2 = 3.0 * nu * beta * tau
+    rho_ratio = (3.0 - beta) / (2.0 * beta)
+    return 3.0 * kappa / (a2 * rho_ratio * cp)
+
+
+@pytest.mark.parametrize("factor", [1.0, 4.0])
+def test_particle_temperature_relaxation_rate(factor):
+    """In a fluid at t = 0 the particle's temperature decays by exp(-dt /
+    tau_h) per sub-step, tau_h from the sphere's heat transfer (not from
+    the reference's own expression); in a fluid at constant T0 a particle
+    starting at 0 takes f T0 e^{-f} in one sub-step (the scheme's explicit
+    source with the exponential factor), and the map's fixed point is
+    f T0 / (e^f - 1)."""
+    tau_u, tau_t, tau, beta, cp = 0.8, 0.6, 2.5, 0.7, 1.6
+    nu, kappa = (tau_u - 0.5) / 3.0, (tau_t - 0.5) / 3.0
+    o, cfg = _thermal(tau_u=tau_u, tau_t=tau_t)
+    pc = abi.make_particle_config(TPART, 3, tau_drag=(1, tau, tau), beta_coeff=(1, beta, beta),
+                                  cp_coeff=(1, cp, cp), time_dt_lagr_factor=factor)
+    o.particles_init(pc)
+    a = o.particles()
+    a["t_p"] = [1.0, -0.5, 3.0]
+    o.set_particles(a)
+    u = np.zeros((cfg.NZ, cfg.NY, cfg.NX, 3))
+    t0 = np.zeros((cfg.NZ, cfg.NY, cfg.NX))
+    f_want = _heat_rate(nu, kappa, tau, beta, cp) / factor
+    b = o.move(u, t0, first=True)
+    f = -np.log(b["t_p"] / np.array([1.0, -0.5, 3.0]))
+    assert np.allclose(f, f_want, rtol=1e-12, atol=0)
+    # constant fluid temperature T0, particle from 0
+    T0 = 0.75
+    a = o.particles()
+    a["t_p"] = 0.0
+    o.set_particles(a)
+    b = o.move(u, t0 + T0)
+    assert np.allclose(b["t"], T0, rtol=0, atol=1e-15)
+    assert np.allclose(b["t_p"], f_want * T0 * math.exp(-f_want), rtol=1e-12)
+    assert not np.allclose(b["t_p"], T0 * (1 - math.exp(-f_want)), rtol=1e-6)
+    fixed = f_want * T0 / math.expm1(f_want)
+    for _ in range(int(35 / f_want)):  # the start is forgotten by e^{-35}
+        b = o.move(u, t0 + T0)
+    assert np.allclose(b["t_p"], fixed, rtol=1e-9)
+
+
+def test_tracer_temperature_is_not_a_number():
+    """Tracers have tau_drag = 0 and cp_p = 0: the rate is +inf and
+    (t_p + inf t) exp(-inf) is NaN, as the reference computes it."""
+    o, cfg = _thermal()
+    o.particles_init(abi.make_particle_config(TPART | abi.LAGRANGE_ADD_TRACER, 4, tau_drag=(1, 2.0, 2.0),
+                                              beta_coeff=(1, 0.5, 0.5), cp_coeff=(1, 1.0, 1.0)))
+    o.step(1)
+    a = o.particles()
+    tracer = a["name"] % 2 == 0  # two types: the tracer and one inertial
+    assert np.all(np.isnan(a["t_p"][tracer])) and np.all(np.isfinite(a["t_p"][~tracer]))
+
+
+def test_reinjected_particle_takes_the_wall_temperature():
+    """Crossing the bottom wall re-injects at the top with t_p = T_top,
+    crossing the top re-injects at the bottom with t_p = T_bot; the
+    relaxation of the same sub-step then acts on that value."""
+    tau_u, tau_t, tau, beta, cp = 0.8, 0.6, 4.0, 0.5, 1.0
+    nu, kappa = (tau_u - 0.5) / 3.0, (tau_t - 0.5) / 3.0
+    o, cfg = _thermal(6, 8, 4, flags=abi.FLUID_BC_Y | abi.TEMPERATURE_BC_Y, T_bot=1.0, T_top=0.0, T_ref=0.0,
+                      tau_u=tau_u, tau_t=tau_t)
+    flags = (TPART | abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT |
+             abi.LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM | abi.LAGRANGE_WRAP)
+    o.particles_init(abi.make_particle_config(flags, 3, tau_drag=(1, tau, tau), beta_coeff=(1, beta, beta),
+                                              cp_coeff=(1, cp, cp)))
+    a = o.particles()
+    a["x"], a["y"], a["z"] = [1.0, 2.0, 3.0], [0.05, 7.95, 4.0], [1.0, 2.0, 3.0]
+    a["vy"], a["vy_old"] = [-0.2, 0.2, 0.0], [-0.2, 0.2, 0.0]
+    a["t_p"] = 0.5
+    o.set_particles(a)
+    x, y, z = _centres(cfg)
+    b = o.move(np.zeros(x.shape + (3,)), 1.0 - y / 8.0, first=True)
+    f = _heat_rate(nu, kappa, tau, beta, cp)
+    assert b["y"][0] > 7.0 and b["y"][1] < 1.0 and b["y"][2] == 4.0
+    assert b["t_p"][0] == pytest.approx((0.0 + f * b["t"][0]) * math.exp(-f), rel=1e-13)
+    assert b["t_p"][1] == pytest.approx((1.0 + f * b["t"][1]) * math.exp(-f), rel=1e-13)
+    assert b["t_p"][2] == pytest.approx((0.5 + f * b["t"][2]) * math.exp(-f), rel=1e-13)
+
+
+def test_initial_temperature_from_the_fluid():
+    """LAGRANGE_INITIAL_TEMPERATURE_FLUID: t_p (and t, dx_t..dz_t) start as
+    the fluid's temperature at the particle; for the linear conduction
+    profile that is T_bot + (T_top - T_bot) y / H at every height (the
+    fixed-T ghost continues the line)."""
+    o, cfg = _thermal(6, 8, 5, flags=abi.FLUID_BC_Y | abi.TEMPERATURE_BC_Y, T_bot=1.0, T_top=0.0, T_ref=0.0)
+    W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    x, y, z = _centres(cfg)
+    t = 1.0 - y / 8.0
+    p = np.broadcast_to(W, (cfg.NZ, cfg.NY, cfg.NX, 19)).copy()
+    g = t[..., None] * W
+    o.set_populations(p, g)
+    o.particles_init(abi.make_particle_config(TPART | abi.LAGRANGE_INITIAL_TEMPERATURE_FLUID, 200,
+                                              tau_drag=(1, 2.0, 2.0), beta_coeff=(1, 0.5, 0.5),
+                                              cp_coeff=(1, 1.0, 1.0)))
+    a = o.particles()
+    assert np.allclose(a["t_p"], 1.0 - a["y"] / 8.0, rtol=0, atol=1e-15)
+    assert np.array_equal(a["t"], a["t_p"])
+    assert np.allclose(a["dy_t"], -1.0 / 8.0, rtol=0, atol=1e-15)
+    assert not a["ux"].any() and not a["dx_ux"].any()
+    # without the switch: T_bot
+    o2, _ = _thermal(6, 8, 5, flags=abi.FLUID_BC_Y | abi.TEMPERATURE_BC_Y, T_bot=1.0, T_top=0.0, T_ref=0.0)
+    o2.set_populations(p, g)
+    o2.particles_init(abi.make_particle_config(TPART, 5, tau_drag=(1, 2.0, 2.0), beta_coeff=(1, 0.5, 0.5),
+                                               cp_coeff=(1, 1.0, 1.0)))
+    assert np.all(o2.particles()["t_p"] == 1.0) and not o2.particles()["t"].any()
