@@ -54,6 +54,8 @@ LAGRANGE_ORIENTATION_SECONDORIENTATION = 1 << 22
 LAGRANGE_TEMPERATURE = 1 << 23
 LAGRANGE_TEMPERATURE_INCREMENT_LOG = 1 << 24
 LAGRANGE_INITIAL_TEMPERATURE_FLUID = 1 << 25
+LAGRANGE_INITIAL_VELOCITY_FLUID = 1 << 26
+LAGRANGE_INITIAL_VELOCITY_STOKES = 1 << 27
 
 FP64 = 0
 FP32_DEV = 1

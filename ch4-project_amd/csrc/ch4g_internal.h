@@ -242,9 +242,11 @@ struct PartParams {
 
 // one sub-step: interpolation + move + walls (+ re-injection draws) + wrap
 int launch_particles(const PartParams &P, hipStream_t s);
-// t, its gradient and t_p = t at the particles from the FP64 field P.t
+// Fields of the initial state at the particles from the FP64 planes P.u /
+// P.t: with `velocity` u (and grad u) into the rows, v = u when `set_v`
+// (LAGRANGE_INITIAL_VELOCITY_FLUID); otherwise t (and grad t) and t_p = t
 // (LAGRANGE_INITIAL_TEMPERATURE_FLUID)
-int launch_particle_tinit(const PartParams &P, hipStream_t s);
+int launch_particle_finit(const PartParams &P, bool velocity, bool set_v, hipStream_t s);
 // Two-way feedback (lagrange_feedback.c:11-256): every particle's deposits
 // summed per node in reference particle order, written to dense planes
 // fb[3][N] (double on FP64 lattices, float on float storage).  The particles

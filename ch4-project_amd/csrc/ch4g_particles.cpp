@@ -85,6 +85,9 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
         return fail(-1, "LAGRANGE_TEMPERATURE_INCREMENT_LOG / LAGRANGE_INITIAL_TEMPERATURE_FLUID need LAGRANGE_TEMPERATURE");
     if (tpart && (pc->cp_coeff_types < 1 || pc->cp_coeff_max < pc->cp_coeff_min))
         return fail(-1, "Error in cp_coeff parameters");
+    // the Stokes start reads beta_coeff (lagrange.c:981-991)
+    if (lhas(*pc, CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES) && !am)
+        return fail(-1, "LAGRANGE_INITIAL_VELOCITY_STOKES needs LAGRANGE_GRADIENT and LAGRANGE_ADDEDMASS");
     if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_SMALLTAUD) && !lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM))
         return fail(-1, "LAGRANGE_TWOWAY_SMALLTAUD needs LAGRANGE_TWOWAY_MOMENTUM");
     // the feedback sorts 32-bit stencil-cell keys with a sentinel above every cell
@@ -611,6 +614,14 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
             r.cp_p = cpv[(size_t)type];  // lagrange.c:730-732
             r.t_p = L->cfg.T_bot;        // lagrange.c:862-864
         }
+        if (lhas(*pc, CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES)) {
+            // the settling velocity (lagrange.c:981-991); it replaces the
+            // fluid velocity of LAGRANGE_INITIAL_VELOCITY_FLUID, which the
+            // reference sets first
+            r.vx = -L->cfg.gravity_x * (1.0 - r.beta_coeff) * r.tau_drag;
+            r.vy = -L->cfg.gravity_y * (1.0 - r.beta_coeff) * r.tau_drag;
+            r.vz = -L->cfg.gravity_z * (1.0 - r.beta_coeff) * r.tau_drag;
+        }
         if (orient) {
             // random_vector with glibc sincos for each angle: the call gcc
             // makes of sin(a) and cos(a) together; separate sin and cos
@@ -659,13 +670,18 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         if ((rc = rng_to_device(L))) goto err;
     }
     if ((rc = upload(L, h.data(), np))) goto err;
-    if (lhas(*pc, CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID)) {
-        // the temperature of the initial state (FP64 planes on every storage)
+    if (lhas(*pc, CH4G_LAGRANGE_INITIAL_VELOCITY_FLUID) || lhas(*pc, CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID)) {
+        // fields of the initial state (FP64 planes on every storage), then
+        // lagrange.c:968-980 and :1004-1015 in that order
         if ((rc = ensure_scratch(L))) goto err;
-        gather(L, nullptr, nullptr, nullptr, nullptr, L->d_t);
+        gather(L, nullptr, nullptr, nullptr, L->d_u, L->d_t);
         const PartParams P = part_params(L);
-        if (launch_particle_tinit(P, L->stream) || hipStreamSynchronize(L->stream) != hipSuccess) {
-            rc = fail(-2, "particle temperature initialisation failed: %s", hipGetErrorString(hipGetLastError()));
+        const bool vel = lhas(*pc, CH4G_LAGRANGE_INITIAL_VELOCITY_FLUID);
+        const bool keep_v = lhas(*pc, CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES);
+        if ((vel && launch_particle_finit(P, true, !keep_v, L->stream)) ||
+            (lhas(*pc, CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID) && launch_particle_finit(P, false, false, L->stream)) ||
+            hipStreamSynchronize(L->stream) != hipSuccess) {
+            rc = fail(-2, "particle field initialisation failed: %s", hipGetErrorString(hipGetLastError()));
             goto err;
         }
     }

@@ -585,12 +585,16 @@ __global__ void __launch_bounds__(PB) k_interp(PartParams P)
     }
 }
 
-// LAGRANGE_INITIAL_TEMPERATURE_FLUID (lagrange.c:1004-1015):
-// interpolate_scalar_at_particles(t) at the initial positions (t and, with
-// the gradient, dx_t..dz_t) and t_p = t, on the FP64 temperature field with
-// the halo rules of every sub-step.  P.fluid is 0: the temperature alone.
-template <bool GRAD>
-__global__ void __launch_bounds__(PB) k_tinit(PartParams P)
+// The initial-state interpolations of initial_conditions_particles, on the
+// FP64 fields with the halo rules of every sub-step:
+// LAGRANGE_INITIAL_VELOCITY_FLUID (lagrange.c:968-980, after
+// boundary_conditions_hydro): interpolate_vector_at_particles(u) (u and, with
+// the gradient, grad u) and v = u unless the Stokes start replaces it; P has
+// fluid on and thermal off.  LAGRANGE_INITIAL_TEMPERATURE_FLUID
+// (lagrange.c:1004-1015): interpolate_scalar_at_particles(t) (t and, with the
+// gradient, dx_t..dz_t) and t_p = t; P has fluid off and thermal on.
+template <bool GRAD, bool VEL>
+__global__ void __launch_bounds__(PB) k_finit(PartParams P, int set_v)
 {
     const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
     if (p >= P.np || !P.alive[p]) return;
@@ -600,10 +604,18 @@ __global__ void __launch_bounds__(PB) k_tinit(PartParams P)
     if (!stencil(P, S[PX * ld + p], S[PY * ld + p], S[PZ * ld + p], st)) return;
     double u[3] = {0.0, 0.0, 0.0}, G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, t = 0.0, gt[3] = {0.0, 0.0, 0.0};
     interpolate<GRAD, double>(P, st, u, G, t, gt);
-    S[PT * ld + p] = t;
-    if constexpr (GRAD)
-        for (int a = 0; a < 3; a++) S[(PGT + a) * ld + p] = gt[a];
-    S[PTP * ld + p] = t;
+    if constexpr (VEL) {
+        for (int a = 0; a < 3; a++) S[(PUX + a) * ld + p] = u[a];
+        if constexpr (GRAD)
+            for (int m = 0; m < 9; m++) S[(PG + m) * ld + p] = G[m];
+        if (set_v)
+            for (int a = 0; a < 3; a++) S[(PVX + a) * ld + p] = u[a];
+    } else {
+        S[PT * ld + p] = t;
+        if constexpr (GRAD)
+            for (int a = 0; a < 3; a++) S[(PGT + a) * ld + p] = gt[a];
+        S[PTP * ld + p] = t;
+    }
 }
 
 // One sub-step of every particle: interpolate_vector_at_particles(u),
@@ -1394,18 +1406,25 @@ int launch_particles(const PartParams &P, hipStream_t s)
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_particle_tinit(const PartParams &P, hipStream_t s)
+int launch_particle_finit(const PartParams &P, bool velocity, bool set_v, hipStream_t s)
 {
     if (P.np == 0) return 0;
-    if (!P.t) return -1;
+    if (velocity ? !P.u : !P.t) return -1;
     PartParams Q = P;
-    Q.fluid = 0;
+    Q.fluid = velocity ? 1 : 0;
+    Q.thermal = velocity ? 0 : 1;
     Q.uT = nullptr;
     const int nb = blocks_of(P.np, PB);
-    if (P.flags & CH4G_LAGRANGE_GRADIENT)
-        hipLaunchKernelGGL(k_tinit<true>, dim3(nb), dim3(PB), 0, s, Q);
+    const bool grad = (P.flags & CH4G_LAGRANGE_GRADIENT) != 0;
+    const int sv = set_v ? 1 : 0;
+    if (velocity && grad)
+        hipLaunchKernelGGL((k_finit<true, true>), dim3(nb), dim3(PB), 0, s, Q, sv);
+    else if (velocity)
+        hipLaunchKernelGGL((k_finit<false, true>), dim3(nb), dim3(PB), 0, s, Q, sv);
+    else if (grad)
+        hipLaunchKernelGGL((k_finit<true, false>), dim3(nb), dim3(PB), 0, s, Q, sv);
     else
-        hipLaunchKernelGGL(k_tinit<false>, dim3(nb), dim3(PB), 0, s, Q);
+        hipLaunchKernelGGL((k_finit<false, false>), dim3(nb), dim3(PB), 0, s, Q, sv);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -74,6 +74,8 @@ static const struct { const char *name; unsigned flag; } LFLAGS[] = {
     {"LAGRANGE_TEMPERATURE", CH4G_LAGRANGE_TEMPERATURE},
     {"LAGRANGE_TEMPERATURE_INCREMENT_LOG", CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG},
     {"LAGRANGE_INITIAL_TEMPERATURE_FLUID", CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID},
+    {"LAGRANGE_INITIAL_VELOCITY_FLUID", CH4G_LAGRANGE_INITIAL_VELOCITY_FLUID},
+    {"LAGRANGE_INITIAL_VELOCITY_STOKES", CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -261,6 +263,10 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     if ((out->pcfg.flags & CH4G_LAGRANGE_TEMPERATURE) &&
         (~out->pcfg.flags & (CH4G_LAGRANGE_GRADIENT | CH4G_LAGRANGE_ADDEDMASS)))
         return perr("%s", "LAGRANGE_TEMPERATURE needs LAGRANGE_GRADIENT and LAGRANGE_ADDEDMASS");
+    /* the Stokes start reads beta_coeff too (lagrange.c:981-991) */
+    if ((out->pcfg.flags & CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES) &&
+        (~out->pcfg.flags & (CH4G_LAGRANGE_GRADIENT | CH4G_LAGRANGE_ADDEDMASS)))
+        return perr("%s", "LAGRANGE_INITIAL_VELOCITY_STOKES needs LAGRANGE_GRADIENT and LAGRANGE_ADDEDMASS");
     if (!out->lagrange && out->pcfg.flags) return perr("%s", "LAGRANGE_* switches without LAGRANGE");
 
     static kv t[1024];

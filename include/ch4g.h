@@ -255,7 +255,9 @@ enum ch4g_lagrange_flag {
     CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION       = 1u << 22, /* second vector n normal to p, lagrange.c:904-924, :3586-3740 */
     CH4G_LAGRANGE_TEMPERATURE                         = 1u << 23, /* particle temperature t_p relaxing to t, lagrange.c:4007-4020 (needs LB_TEMPERATURE) */
     CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG           = 1u << 24, /* cp_coeff ladder, lagrange.c:579-617 */
-    CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID           = 1u << 25  /* t_p starts at the fluid's t, lagrange.c:1004-1015 */
+    CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID           = 1u << 25, /* t_p starts at the fluid's t, lagrange.c:1004-1015 */
+    CH4G_LAGRANGE_INITIAL_VELOCITY_FLUID              = 1u << 26, /* v starts at the fluid's u, lagrange.c:968-980 */
+    CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES             = 1u << 27  /* v starts at -g (1 - beta) tau, lagrange.c:981-991 (needs ADDEDMASS) */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -302,7 +304,10 @@ int64_t ch4g_particle_size(void);
  * drawn x, y, z per particle from the lattice's drand48 stream (which has
  * already served initial_conditions and the first build_forcing, as in
  * main.c:23-51), then with LAGRANGE_ORIENTATION the particle's random unit
- * vector p (random_vector, random.c:139-150: two more draws), zero velocity;
+ * vector p (random_vector, random.c:139-150: two more draws), zero velocity
+ * (or with LAGRANGE_INITIAL_VELOCITY_FLUID the fluid velocity interpolated
+ * at the particle, u and with the gradient grad u set with it; with
+ * LAGRANGE_INITIAL_VELOCITY_STOKES, which wins over it, -g (1 - beta) tau);
  * with LAGRANGE_TEMPERATURE the cp_coeff ladder (after the orientation
  * ones, lagrange.c:579-617) and t_p = T_bot, or with
  * LAGRANGE_INITIAL_TEMPERATURE_FLUID the fluid temperature interpolated at

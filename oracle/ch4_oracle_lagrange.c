@@ -14,7 +14,9 @@
  * defined value to follow.
  *
  * Sources followed:
- *   initial_conditions_particles  lagrange.c:72-1021 (no restart branch)
+ *   initial_conditions_particles  lagrange.c:72-1021 (no restart branch; the
+ *                                 fluid-velocity, Stokes and fluid-temperature
+ *                                 starts :968-991, :1004-1015)
  *   boundary_conditions_hydro     boundary_conditions.c:2786-2997
  *   interpolate_*_at_particles    lagrange.c:1211-1593
  *   gradient_vector (LAGRANGE)    gradients.c:78-499
@@ -219,6 +221,27 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     free(gyv);
     free(swv);
     free(cpv);
+    /* lagrange.c:968-980: boundary_conditions_hydro, then
+     * interpolate_vector_at_particles(u) (u and, with the gradient, grad u)
+     * and v = u */
+    if (LHAS(L, CH4G_LAGRANGE_INITIAL_VELOCITY_FLUID)) {
+        bc_hydro(o);
+        interpolate_fields(o, 1, 0);
+        for (int i = 0; i < L->npart; i++) {
+            ch4g_particle *p = &L->tr[i];
+            p->vx = p->ux;
+            p->vy = p->uy;
+            p->vz = p->uz;
+        }
+    }
+    /* lagrange.c:981-991: the settling velocity -g (1 - beta) tau */
+    if (LHAS(L, CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES))
+        for (int i = 0; i < L->npart; i++) {
+            ch4g_particle *p = &L->tr[i];
+            p->vx = -o->cfg.gravity_x * (1.0 - p->beta_coeff) * p->tau_drag;
+            p->vy = -o->cfg.gravity_y * (1.0 - p->beta_coeff) * p->tau_drag;
+            p->vz = -o->cfg.gravity_z * (1.0 - p->beta_coeff) * p->tau_drag;
+        }
     /* lagrange.c:1004-1015: interpolate_scalar_at_particles(t) (t and, with
      * the gradient, dx_t..dz_t) and t_p = t.  At that point of main.c
      * (:31-51) hydro_fields has written the interior only; the halo is

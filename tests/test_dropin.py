@@ -122,6 +122,21 @@ def test_read_run_params_particle_temperature(tmp_path):
     assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == -1
 
 
+def test_read_run_params_initial_velocity(tmp_path):
+    """LAGRANGE_INITIAL_VELOCITY_FLUID / _STOKES are read as switches; the
+    Stokes start reads beta_coeff, so it needs the added mass."""
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    text = open(os.path.join(FIXP, "define.h")).read()
+    d, pin = tmp_path / "define.h", os.path.join(FIXP, "param.in")
+    d.write_text(text + "#define LAGRANGE_INITIAL_VELOCITY_FLUID\n#define LAGRANGE_INITIAL_VELOCITY_STOKES\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == 0
+    v = abi.LAGRANGE_INITIAL_VELOCITY_FLUID | abi.LAGRANGE_INITIAL_VELOCITY_STOKES
+    assert rp.pcfg.flags & v == v
+    d.write_text(text.replace("#define LAGRANGE_ADDEDMASS\n", "") + "#define LAGRANGE_INITIAL_VELOCITY_STOKES\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == -1
+
+
 def test_read_run_params_rejects_unimplemented_switch(tmp_path):
     d = tmp_path / "define.h"
     d.write_text(open(os.path.join(FIX, "define.h")).read() + "#define LB_FLUID_BC_Y_P_SLIP\n")
@@ -327,13 +342,17 @@ def test_pop_h5_dump_and_restart(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(H5LIB), reason="serial HDF5 not installed")
-def test_part_h5_dump_and_restart(tmp_path):
+@pytest.mark.parametrize("tpart", [False, True], ids=["plain", "particle_temperature"])
+def test_part_h5_dump_and_restart(tmp_path, tpart):
     """OUTPUT_H5 with LAGRANGE: part.h5 holds the particle records of the
     last step under the reference's member names (equal to the oracle's, the
-    FP64 path being bit-identical), and a resume run reads them back."""
+    FP64 path being bit-identical; with LAGRANGE_TEMPERATURE also
+    temperature_particle and cp_particle), and a resume run reads them back."""
     d1, d2 = tmp_path / "a", tmp_path / "b"
-    define = open(os.path.join(FIXP, "define.h")).read() + "#define OUTPUT_H5\n"
-    param = open(os.path.join(FIXP, "param.in")).read()
+    define = (open(os.path.join(FIXP, "define.h")).read() + "#define OUTPUT_H5\n" +
+              ("#define LAGRANGE_TEMPERATURE\n" if tpart else ""))
+    param = open(os.path.join(FIXP, "param.in")).read() + ("cp_coeff_types 2\ncp_coeff_min 0.5\ncp_coeff_max 2\n"
+                                                            if tpart else "")
     for d, prm in ((d1, param), (d2, param.replace("resume 0", "resume 1").replace("resume_u 0", "resume_u 1")
                                   .replace("resume_t 0", "resume_t 1"))):
         d.mkdir()
@@ -351,11 +370,12 @@ def test_part_h5_dump_and_restart(tmp_path):
     o.step(int(rp.time_end))
     ref = o.particles()
     names = ["name", "x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "ux", "uy", "uz", "tau_drag", "t", "dx_t",
-             "dy_t", "dz_t"]
+             "dy_t", "dz_t"] + (["temperature_particle", "cp_particle"] if tpart else [])
     got = read_compound_h5(d1 / "part.h5", "/particles/point_particle", names, (rp.pcfg.particle_number,))
     n = len(ref)
+    rename = {"temperature_particle": "t_p", "cp_particle": "cp_p"}
     for i, m in enumerate(names):
-        np.testing.assert_array_equal(got[:n, i], ref[m], err_msg=m)
+        np.testing.assert_array_equal(got[:n, i], ref[rename.get(m, m)], err_msg=m)
     for f in ("pop.h5", "part.h5"):
         shutil.copy(d1 / f, d2 / f)
     r = subprocess.run([RUN], cwd=d2, capture_output=True, text=True, timeout=120)

@@ -208,6 +208,33 @@ def test_particle_temperature_fp32_dev_close_to_fp64():
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("extra", [
+    abi.LAGRANGE_INITIAL_VELOCITY_FLUID,
+    abi.LAGRANGE_INITIAL_VELOCITY_STOKES,
+    abi.LAGRANGE_INITIAL_VELOCITY_FLUID | abi.LAGRANGE_INITIAL_VELOCITY_STOKES | abi.LAGRANGE_TEMPERATURE |
+    abi.LAGRANGE_INITIAL_TEMPERATURE_FLUID,
+], ids=["fluid", "stokes", "all_starts"])
+def test_initial_velocity_fp64(extra):
+    """The starts of initial_conditions_particles (lagrange.c:968-991,
+    :1004-1015): the records right after ch4g_particles_init and after 60
+    steps bit-identical to the oracle."""
+    pc = abi.make_particle_config(HEAVY | extra, 256, tau_drag=(2, 2.0, 20.0), beta_coeff=(2, 0.2, 0.5),
+                                  cp_coeff=(2, 0.5, 2.0))
+    cfg = rb_small(32, 16, g=5e-3)
+    o = Oracle(cfg)
+    o.initial_conditions()
+    o.particles_init(pc)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.particles_init(pc)
+    a = lat.particles()
+    assert np.abs(a["vy"]).max() > 0
+    compare_particles(o, lat)
+    o.step(60)
+    lat.step(60)
+    compare_particles(o, lat)
+
+
 def test_substeps_fp64():
     pc = abi.make_particle_config(HEAVY, 128, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.7, 0.7),
                                   time_dt_lagr_factor=3)

@@ -672,3 +672,60 @@ def test_initial_temperature_from_the_fluid():
     o2.particles_init(abi.make_particle_config(TPART, 5, tau_drag=(1, 2.0, 2.0), beta_coeff=(1, 0.5, 0.5),
                                                cp_coeff=(1, 1.0, 1.0)))
     assert np.all(o2.particles()["t_p"] == 1.0) and not o2.particles()["t"].any()
+
+
+# --------------------------------------------------------------------------
+# Initial particle velocity (lagrange.c:968-991).
+def test_initial_velocity_from_the_fluid():
+    """LAGRANGE_INITIAL_VELOCITY_FLUID: v starts as the fluid velocity at the
+    particle (u and grad u set with it); in a uniform flow that is u0 with a
+    zero gradient, for tracers and inertial particles alike."""
+    o, cfg = quiescent(8, 6, 4, tau=0.9)
+    u0 = np.array([0.021, -0.013, 0.007])
+    uniform_flow(o, cfg, u0)
+    flags = (abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS |
+             abi.LAGRANGE_INITIAL_VELOCITY_FLUID)
+    o.particles_init(abi.make_particle_config(flags, 40, tau_drag=(2, 1.0, 5.0), beta_coeff=(1, 0.5, 0.5)))
+    a = o.particles()
+    for v, u, c in (("vx", "ux", 0), ("vy", "uy", 1), ("vz", "uz", 2)):
+        assert np.array_equal(a[v], a[u])
+        assert np.allclose(a[v], u0[c], rtol=1e-14, atol=0)
+    for g in ("dx_ux", "dy_uy", "dz_uz", "dy_ux"):
+        assert np.allclose(a[g], 0.0, atol=1e-15)
+    # without the switch the particles start at rest and u is not interpolated
+    o2, _ = quiescent(8, 6, 4, tau=0.9)
+    uniform_flow(o2, cfg, u0)
+    o2.particles_init(abi.make_particle_config(flags & ~abi.LAGRANGE_INITIAL_VELOCITY_FLUID, 40,
+                                               tau_drag=(2, 1.0, 5.0), beta_coeff=(1, 0.5, 0.5)))
+    b = o2.particles()
+    assert not b["vx"].any() and not b["ux"].any()
+
+
+def test_initial_stokes_velocity_is_the_terminal_velocity():
+    """LAGRANGE_INITIAL_VELOCITY_STOKES starts v at -g (1 - beta) tau, the
+    settling velocity the particle dynamics reach in a fluid at rest: the
+    velocity then stays put, while a particle started at rest is still far
+    from it after the same time; with the fluid-velocity start as well, the
+    Stokes value wins (it is set second) and u keeps the interpolated fluid
+    velocity."""
+    g, tau_p, beta = 1e-3, 5.0, 0.6
+    flags = abi.LAGRANGE_GRAVITY | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS
+    pc = abi.make_particle_config(flags | abi.LAGRANGE_INITIAL_VELOCITY_STOKES, 3, tau_drag=(1, tau_p, tau_p),
+                                  beta_coeff=(1, beta, beta))
+    o, _ = quiescent(6, 6, 6, gravity_y=g)
+    o.particles_init(pc)
+    vt = -g * tau_p * (1 - beta)
+    assert np.all(o.particles()["vy"] == vt) and not o.particles()["vx"].any()
+    o.step(8)
+    assert np.allclose(o.particles()["vy"], vt, rtol=1e-12)
+    o2, _ = quiescent(6, 6, 6, gravity_y=g)
+    o2.particles_init(abi.make_particle_config(flags, 3, tau_drag=(1, tau_p, tau_p), beta_coeff=(1, beta, beta)))
+    o2.step(8)
+    assert np.all(np.abs(o2.particles()["vy"] - vt) > 0.1 * abs(vt))
+    o3, cfg = quiescent(6, 6, 6, gravity_y=g)
+    uniform_flow(o3, cfg, np.array([0.01, 0.0, 0.0]))
+    o3.particles_init(abi.make_particle_config(flags | abi.LAGRANGE_INITIAL_VELOCITY_STOKES |
+                                               abi.LAGRANGE_INITIAL_VELOCITY_FLUID, 3, tau_drag=(1, tau_p, tau_p),
+                                               beta_coeff=(1, beta, beta)))
+    c = o3.particles()
+    assert np.all(c["vy"] == vt) and not c["vx"].any() and np.allclose(c["ux"], 0.01, rtol=1e-14)
