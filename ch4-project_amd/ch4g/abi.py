@@ -56,6 +56,8 @@ LAGRANGE_TEMPERATURE_INCREMENT_LOG = 1 << 24
 LAGRANGE_INITIAL_TEMPERATURE_FLUID = 1 << 25
 LAGRANGE_INITIAL_VELOCITY_FLUID = 1 << 26
 LAGRANGE_INITIAL_VELOCITY_STOKES = 1 << 27
+LAGRANGE_GRAVITY_VARIABLE = 1 << 28
+LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG = 1 << 29
 
 FP64 = 0
 FP32_DEV = 1
@@ -141,12 +143,13 @@ class ParticleConfig(C.Structure):
         ("gyrotaxis_velocity_max", C.c_double),
         ("swim_velocity_types", C.c_double), ("swim_velocity_min", C.c_double), ("swim_velocity_max", C.c_double),
         ("cp_coeff_types", C.c_double), ("cp_coeff_min", C.c_double), ("cp_coeff_max", C.c_double),
+        ("gravity_coeff_types", C.c_double), ("gravity_coeff_min", C.c_double), ("gravity_coeff_max", C.c_double),
     ]
 
 
 def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_coeff=(1, 1.0, 1.0),
                          time_dt_lagr_factor=1.0, aspect_ratio=(1, 1.0, 1.0), gyrotaxis_velocity=(1, 0.0, 0.0),
-                         swim_velocity=(1, 0.0, 0.0), cp_coeff=(1, 1.0, 1.0)):
+                         swim_velocity=(1, 0.0, 0.0), cp_coeff=(1, 1.0, 1.0), gravity_coeff=(1, 1.0, 1.0)):
     """ParticleConfig from (types, min, max) ladders as param.in gives them."""
     pc = ParticleConfig()
     pc.flags = flags
@@ -159,6 +162,7 @@ def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_co
      pc.gyrotaxis_velocity_max) = (float(v) for v in gyrotaxis_velocity)
     pc.swim_velocity_types, pc.swim_velocity_min, pc.swim_velocity_max = (float(v) for v in swim_velocity)
     pc.cp_coeff_types, pc.cp_coeff_min, pc.cp_coeff_max = (float(v) for v in cp_coeff)
+    pc.gravity_coeff_types, pc.gravity_coeff_min, pc.gravity_coeff_max = (float(v) for v in gravity_coeff)
     return pc
 
 
@@ -167,7 +171,8 @@ PARTICLE_FIELDS = ["name", "x", "y", "z", "vx", "vy", "vz", "vx_old", "vy_old", 
                    "dx_ux", "dy_ux", "dz_ux", "dx_uy", "dy_uy", "dz_uy", "dx_uz", "dy_uz", "dz_uz",
                    "Dt_ux", "Dt_uy", "Dt_uz", "t", "t_old", "dt_t", "tau_drag", "beta_coeff", "dx_t", "dy_t", "dz_t",
                    "px", "py", "pz", "dt_px", "dt_py", "dt_pz", "aspect_ratio", "gyrotaxis_velocity", "swim_velocity",
-                   "nx", "ny", "nz", "dt_nx", "dt_ny", "dt_nz", "t_p", "cp_p"]
+                   "nx", "ny", "nz", "dt_nx", "dt_ny", "dt_nz", "t_p", "cp_p",
+                   "gravity_coeff"]
 
 
 class Particle(C.Structure):

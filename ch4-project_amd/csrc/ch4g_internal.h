@@ -215,6 +215,7 @@ enum PartField {
     PCP,                // cp_p
     PTFAC,              // time_dt_lagr / tau_heat of lagrange.c:4011-4013, host arithmetic
     PTEXP,              // exp(-PTFAC), glibc (the same bits as the reference's call)
+    PGC,                // gravity_coeff (LAGRANGE_GRAVITY_VARIABLE)
     PF_COUNT
 };
 

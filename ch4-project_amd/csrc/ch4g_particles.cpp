@@ -85,6 +85,13 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
         return fail(-1, "LAGRANGE_TEMPERATURE_INCREMENT_LOG / LAGRANGE_INITIAL_TEMPERATURE_FLUID need LAGRANGE_TEMPERATURE");
     if (tpart && (pc->cp_coeff_types < 1 || pc->cp_coeff_max < pc->cp_coeff_min))
         return fail(-1, "Error in cp_coeff parameters");
+    // variable gravity (parameters.c:589-600, lagrange.c:347-376)
+    const bool gvar = lhas(*pc, CH4G_LAGRANGE_GRAVITY_VARIABLE);
+    if (gvar && !lhas(*pc, CH4G_LAGRANGE_GRAVITY)) return fail(-1, "LAGRANGE_GRAVITY_VARIABLE needs LAGRANGE_GRAVITY");
+    if (!gvar && lhas(*pc, CH4G_LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG))
+        return fail(-1, "LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG needs LAGRANGE_GRAVITY_VARIABLE");
+    if (gvar && (pc->gravity_coeff_types < 1 || pc->gravity_coeff_max < pc->gravity_coeff_min))
+        return fail(-1, "Error in gravity_coeff parameters");
     // the Stokes start reads beta_coeff (lagrange.c:981-991)
     if (lhas(*pc, CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES) && !am)
         return fail(-1, "LAGRANGE_INITIAL_VELOCITY_STOKES needs LAGRANGE_GRADIENT and LAGRANGE_ADDEDMASS");
@@ -343,6 +350,7 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
             heat_constants(L->kappa, L->nu, Q->dt, r.tau_drag, r.beta_coeff, r.cp_p, &hf, &he);
         S(PTFAC) = hf;
         S(PTEXP) = he;
+        S(PGC) = r.gravity_coeff;
         alive[(size_t)p] = 1;
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
@@ -551,8 +559,10 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     const bool gyro = jeff && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
     const bool swim = orient && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
     const bool tpart = lhas(*pc, CH4G_LAGRANGE_TEMPERATURE);
+    const bool gvar = lhas(*pc, CH4G_LAGRANGE_GRAVITY_VARIABLE);
     double types = pc->tau_drag_types;
     if (am) types *= pc->beta_coeff_types;
+    if (gvar) types *= pc->gravity_coeff_types;  // parameters.c:600
     if (tpart) types *= pc->cp_coeff_types;  // parameters.c:615
     if (jeff) types *= pc->aspect_ratio_types;
     if (gyro) types *= pc->gyrotaxis_velocity_types;
@@ -561,7 +571,7 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     types += ntr;
     const int nt = (int)types;
     std::vector<double> tau((size_t)nt, 0.0), beta((size_t)nt, 0.0), asp((size_t)nt, 0.0), gyv((size_t)nt, 0.0),
-        swv((size_t)nt, 0.0), cpv((size_t)nt, 0.0);
+        swv((size_t)nt, 0.0), cpv((size_t)nt, 0.0), gcv((size_t)nt, 0.0);
     double rep = 1.0;
     ladder(tau, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
            lhas(*pc, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
@@ -570,6 +580,11 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         ladder(beta, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
                lhas(*pc, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
         rep *= pc->beta_coeff_types;
+    }
+    if (gvar) {  // after beta, before the orientation ladders (lagrange.c:347-384)
+        ladder(gcv, ntr, 0.0, pc->gravity_coeff_types, pc->gravity_coeff_min, pc->gravity_coeff_max,
+               lhas(*pc, CH4G_LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG), rep);
+        rep *= pc->gravity_coeff_types;
     }
     if (jeff) {
         ladder(asp, ntr, 1.0, pc->aspect_ratio_types, pc->aspect_ratio_min, pc->aspect_ratio_max,
@@ -604,6 +619,7 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         const int type = ((int)r.name) % nt;
         r.tau_drag = tau[(size_t)type];
         if (am) r.beta_coeff = beta[(size_t)type];
+        if (gvar) r.gravity_coeff = gcv[(size_t)type];  // lagrange.c:719-722
         r.x = 0 + erand48(L->xsubi) * LNX;
         r.y = 0 + erand48(L->xsubi) * LNY;
         r.z = 0 + erand48(L->xsubi) * LNZ;
@@ -780,6 +796,7 @@ int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t
             for (int m = 0; m < 6; m++) nv[m] = S(PNO + m);
             r.t_p = S(PTP);
             r.cp_p = S(PCP);
+            r.gravity_coeff = S(PGC);
         }
         k++;
     }

@@ -676,6 +676,10 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
         const bool use_beta = am || lflag(P, CH4G_LAGRANGE_SMALLTAUD_BETA | CH4G_LAGRANGE_SMALLTAUD_FLOATER |
                                                CH4G_LAGRANGE_BC_INELASTIC_REINJECT);
         const double beta = use_beta ? R(PBETA) : 0.0;
+        // LAGRANGE_GRAVITY_VARIABLE: the factor of the particle's type on g
+        // (lagrange.c:3314-3335, :3824-3827, :3905-3908); 1 otherwise, which
+        // leaves every product it enters unchanged
+        const double gc = lflag(P, CH4G_LAGRANGE_GRAVITY_VARIABLE) ? R(PGC) : 1.0;
         double vx = 0.0, vy = 0.0, vz = 0.0;
         if (tau != 0.0) {
             vx = R(PVX);
@@ -791,15 +795,15 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
             az = (uz - vz) * invtau;
             const bool grav = lflag(P, CH4G_LAGRANGE_GRAVITY);
             if (grav) {
-                ax -= gx;
-                ay -= gy;
-                az -= gz;
+                ax -= gc * gx;
+                ay -= gc * gy;
+                az -= gc * gz;
             }
             if (lflag(P, CH4G_LAGRANGE_ADDEDMASS) && beta != 0.0) {
                 if (grav) {
-                    ax -= (-beta) * gx;
-                    ay -= (-beta) * gy;
-                    az -= (-beta) * gz;
+                    ax -= (-beta) * gc * gx;
+                    ay -= (-beta) * gc * gy;
+                    az -= (-beta) * gc * gz;
                 }
                 if (first) {
                     uxo = ux;
@@ -980,9 +984,9 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                         vy = 0.0;
                         vz = 0.0;
                     } else {
-                        vx = -gx * (1.0 - beta) * tau;
-                        vy = -gy * (1.0 - beta) * tau;
-                        vz = -gz * (1.0 - beta) * tau;
+                        vx = -gx * gc * (1.0 - beta) * tau;
+                        vy = -gy * gc * (1.0 - beta) * tau;
+                        vz = -gz * gc * (1.0 - beta) * tau;
                     }
                     if (lflag(P, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_STOKES_HALF)) {
                         vx /= 2.0;
@@ -1008,9 +1012,9 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                         vy = 0.0;
                         vz = 0.0;
                     } else {
-                        vx = -gx * (1.0 - beta) * tau;
-                        vy = -gy * (1.0 - beta) * tau;
-                        vz = -gz * (1.0 - beta) * tau;
+                        vx = -gx * gc * (1.0 - beta) * tau;
+                        vy = -gy * gc * (1.0 - beta) * tau;
+                        vz = -gz * gc * (1.0 - beta) * tau;
                     }
                     rewall = 2;  // lagrange.c:3915-3918
                 }

@@ -359,6 +359,7 @@ static double ptypes(const ch4g_particle_config *pc)
     const unsigned fl = pc->flags;
     double types = pc->tau_drag_types;
     if ((fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ADDEDMASS)) types *= pc->beta_coeff_types;
+    if (fl & CH4G_LAGRANGE_GRAVITY_VARIABLE) types *= pc->gravity_coeff_types; /* parameters.c:600 */
     if ((fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ORIENTATION)) {
         if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {
             types *= pc->aspect_ratio_types;
@@ -644,7 +645,8 @@ void read_pop_h5(void)
 /* part.h5 (lagrange.c:4231-4546, :4548-4827): /particles/point_particle, a
  * compound of the point_particle members this lattice carries (name,
  * position, velocity, acceleration and their previous values, tau_drag, the
- * fluid velocity and its previous value; the velocity gradient with
+ * fluid velocity and its previous value, gravity_coeff with
+ * LAGRANGE_GRAVITY_VARIABLE; the velocity gradient with
  * LAGRANGE_GRADIENT, then beta_coeff with added mass and the orientation
  * members with LAGRANGE_ORIENTATION; t, and with the gradient dx_t..dz_t,
  * with LB_TEMPERATURE, then temperature_particle and cp_particle with
@@ -661,6 +663,7 @@ static hid_t particle_type(const ch4g_run_params *rp)
     M(ax); M(ay); M(az); M(ax_old); M(ay_old); M(az_old);
     M(tau_drag);
     M(ux); M(uy); M(uz); M(ux_old); M(uy_old); M(uz_old);
+    if (fl & CH4G_LAGRANGE_GRAVITY_VARIABLE) M(gravity_coeff); /* lagrange.c:4336-4341 */
     if (fl & CH4G_LAGRANGE_GRADIENT) {
         M(dx_ux); M(dy_ux); M(dz_ux); M(dx_uy); M(dy_uy); M(dz_uy); M(dx_uz); M(dy_uz); M(dz_uz);
         if (fl & CH4G_LAGRANGE_ADDEDMASS) M(beta_coeff);
@@ -779,6 +782,7 @@ static int snapshot_members(const ch4g_run_params *rp, struct snap_member *m)
     SM("x", x); SM("y", y); SM("z", z);
     SM("vx", vx); SM("vy", vy); SM("vz", vz);
     SM("ax", ax); SM("ay", ay); SM("az", az);
+    if (fl & CH4G_LAGRANGE_GRAVITY_VARIABLE) SM("gravity_coeff", gravity_coeff); /* lagrange.c:1772-1781 */
     if (grad) {
         SM("dx_ux", dx_ux); SM("dy_ux", dy_ux); SM("dz_ux", dz_ux);
         SM("dx_uy", dx_uy); SM("dy_uy", dy_uy); SM("dz_uy", dz_uy);
@@ -901,6 +905,7 @@ static void write_particle_snapshot(void)
     const char *v[3] = {"vx", "vy", "vz"}, *ac[3] = {"ax", "ay", "az"};
     xmf_joined(fo, h5, "velocity", "Vector", v, 3, np);
     xmf_joined(fo, h5, "acceleration", "Vector", ac, 3, np);
+    if (fl & CH4G_LAGRANGE_GRAVITY_VARIABLE) xmf_scalar(fo, h5, "gravity_coeff", "gravity_coeff", np);
     if (grad) {
         const char *g[9] = {"dx_ux", "dy_ux", "dz_ux", "dx_uy", "dy_uy", "dz_uy", "dx_uz", "dy_uz", "dz_uz"};
         xmf_joined(fo, h5, "fluid velocity gradient", "Tensor", g, 9, np);

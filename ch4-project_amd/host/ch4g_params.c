@@ -76,6 +76,8 @@ static const struct { const char *name; unsigned flag; } LFLAGS[] = {
     {"LAGRANGE_INITIAL_TEMPERATURE_FLUID", CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID},
     {"LAGRANGE_INITIAL_VELOCITY_FLUID", CH4G_LAGRANGE_INITIAL_VELOCITY_FLUID},
     {"LAGRANGE_INITIAL_VELOCITY_STOKES", CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES},
+    {"LAGRANGE_GRAVITY_VARIABLE", CH4G_LAGRANGE_GRAVITY_VARIABLE},
+    {"LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG", CH4G_LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -251,6 +253,12 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     if (!(out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_JEFFREY))
         out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG |
                                        CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
+    /* variable gravity sits inside #ifdef LAGRANGE_GRAVITY, its log ladder
+       inside LAGRANGE_GRAVITY_VARIABLE (parameters.c:589-590, lagrange.c:347-376) */
+    if (!(out->pcfg.flags & CH4G_LAGRANGE_GRAVITY))
+        out->pcfg.flags &= ~(unsigned)CH4G_LAGRANGE_GRAVITY_VARIABLE;
+    if (!(out->pcfg.flags & CH4G_LAGRANGE_GRAVITY_VARIABLE))
+        out->pcfg.flags &= ~(unsigned)CH4G_LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG;
     /* the cp ladder's log steps and the fluid-temperature start sit inside
        #ifdef LAGRANGE_TEMPERATURE (lagrange.c:587, :1005) */
     if (!(out->pcfg.flags & CH4G_LAGRANGE_TEMPERATURE))
@@ -337,6 +345,12 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         }
         /* parameters.c:621-695 */
         const unsigned fl = pc->flags;
+        /* parameters.c:589-600 */
+        if ((fl & CH4G_LAGRANGE_GRAVITY_VARIABLE) &&
+            (need(t, np, "gravity_coeff_types", &pc->gravity_coeff_types) ||
+             need(t, np, "gravity_coeff_min", &pc->gravity_coeff_min) ||
+             need(t, np, "gravity_coeff_max", &pc->gravity_coeff_max)))
+            return -1;
         if ((fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ORIENTATION)) {
             if ((fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) &&
                 (need(t, np, "aspect_ratio_types", &pc->aspect_ratio_types) ||

@@ -257,7 +257,9 @@ enum ch4g_lagrange_flag {
     CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG           = 1u << 24, /* cp_coeff ladder, lagrange.c:579-617 */
     CH4G_LAGRANGE_INITIAL_TEMPERATURE_FLUID           = 1u << 25, /* t_p starts at the fluid's t, lagrange.c:1004-1015 */
     CH4G_LAGRANGE_INITIAL_VELOCITY_FLUID              = 1u << 26, /* v starts at the fluid's u, lagrange.c:968-980 */
-    CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES             = 1u << 27  /* v starts at -g (1 - beta) tau, lagrange.c:981-991 (needs ADDEDMASS) */
+    CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES             = 1u << 27, /* v starts at -g (1 - beta) tau, lagrange.c:981-991 (needs ADDEDMASS) */
+    CH4G_LAGRANGE_GRAVITY_VARIABLE                    = 1u << 28, /* gravity scaled per type, lagrange.c:348-384, :3314-3335 (needs GRAVITY) */
+    CH4G_LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG      = 1u << 29  /* gravity_coeff ladder, lagrange.c:356-376 */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -271,6 +273,7 @@ typedef struct ch4g_particle_config {
     double   gyrotaxis_velocity_types, gyrotaxis_velocity_min, gyrotaxis_velocity_max; /* ..._GYROTAXIS */
     double   swim_velocity_types, swim_velocity_min, swim_velocity_max;                /* ORIENTATION_ACTIVE */
     double   cp_coeff_types, cp_coeff_min, cp_coeff_max;                               /* LAGRANGE_TEMPERATURE */
+    double   gravity_coeff_types, gravity_coeff_min, gravity_coeff_max;                /* LAGRANGE_GRAVITY_VARIABLE */
 } ch4g_particle_config;
 
 /* One particle: the members of the reference's point_particle
@@ -290,12 +293,13 @@ typedef struct ch4g_particle {
     double aspect_ratio, gyrotaxis_velocity, swim_velocity;
     double nx, ny, nz, dt_nx, dt_ny, dt_nz;             /* second orientation (..._SECONDORIENTATION) */
     double t_p, cp_p;                                   /* particle temperature and specific heat (LAGRANGE_TEMPERATURE) */
+    double gravity_coeff;                               /* gravity factor of the particle's type (LAGRANGE_GRAVITY_VARIABLE) */
 } ch4g_particle;
 
 /* No counterpart in the reference.  sizeof(ch4g_particle) as this library
  * was built: callers that exchange particle records check it against their
  * own header before ch4g_get_particles / ch4g_set_particles (the record grew
- * with the orientation and particle-temperature members; a stale header
+ * with the orientation, particle-temperature and gravity members; a stale header
  * would mis-stride the arrays).                                             */
 int64_t ch4g_particle_size(void);
 

@@ -126,8 +126,10 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     const int gyro = jeff && LHAS(L, CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
     const int swim = orient && LHAS(L, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
     const int tpart = LHAS(L, CH4G_LAGRANGE_TEMPERATURE);
+    const int gvar = LHAS(L, CH4G_LAGRANGE_GRAVITY) && LHAS(L, CH4G_LAGRANGE_GRAVITY_VARIABLE);
     double types = pc->tau_drag_types;
     if (am) types *= pc->beta_coeff_types;
+    if (gvar) types *= pc->gravity_coeff_types; /* parameters.c:600 */
     if (tpart) types *= pc->cp_coeff_types; /* parameters.c:615 */
     if (jeff) types *= pc->aspect_ratio_types;
     if (gyro) types *= pc->gyrotaxis_velocity_types;
@@ -138,7 +140,8 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     double *tau = (double *)calloc(nt, sizeof(double)), *beta = (double *)calloc(nt, sizeof(double));
     double *asp = (double *)calloc(nt, sizeof(double)), *gyv = (double *)calloc(nt, sizeof(double));
     double *swv = (double *)calloc(nt, sizeof(double)), *cpv = (double *)calloc(nt, sizeof(double));
-    if (!tau || !beta || !asp || !gyv || !swv || !cpv) return -1;
+    double *gcv = (double *)calloc(nt, sizeof(double));
+    if (!tau || !beta || !asp || !gyv || !swv || !cpv || !gcv) return -1;
     double rep = 1.0;
     ladder(tau, nt, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
            LHAS(L, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
@@ -147,6 +150,12 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
         ladder(beta, nt, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
                LHAS(L, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
         rep *= pc->beta_coeff_types;
+    }
+    /* lagrange.c:347-384: gravity_coeff (linear or geometric), 0 for the tracer */
+    if (gvar) {
+        ladder(gcv, nt, ntr, 0.0, pc->gravity_coeff_types, pc->gravity_coeff_min, pc->gravity_coeff_max,
+               LHAS(L, CH4G_LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG), rep);
+        rep *= pc->gravity_coeff_types;
     }
     /* lagrange.c:389-423 (linear or geometric), :445-470 (geometric), :502-521 (linear) */
     if (jeff) {
@@ -180,6 +189,7 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
         int type = ((int)p->name) % nt;
         p->tau_drag = tau[type];
         if (am) p->beta_coeff = beta[type];
+        if (gvar) p->gravity_coeff = gcv[type]; /* lagrange.c:719-722 */
         /* lagrange.c:767-769: three draws per particle, particle order */
         p->x = 0 + oracle_drand48(o) * LNX;
         p->y = 0 + oracle_drand48(o) * LNY;
@@ -221,6 +231,7 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     free(gyv);
     free(swv);
     free(cpv);
+    free(gcv);
     /* lagrange.c:968-980: boundary_conditions_hydro, then
      * interpolate_vector_at_particles(u) (u and, with the gradient, grad u)
      * and v = u */
@@ -791,16 +802,29 @@ static void move_at(oracle *o, int first)
             p->ax = (p->ux - p->vx) * invtau;
             p->ay = (p->uy - p->vy) * invtau;
             p->az = (p->uz - p->vz) * invtau;
+            const int gvar = LHAS(L, CH4G_LAGRANGE_GRAVITY_VARIABLE); /* lagrange.c:3314-3335 */
             if (LHAS(L, CH4G_LAGRANGE_GRAVITY)) {
-                p->ax -= gx;
-                p->ay -= gy;
-                p->az -= gz;
+                if (gvar) {
+                    p->ax -= p->gravity_coeff * gx;
+                    p->ay -= p->gravity_coeff * gy;
+                    p->az -= p->gravity_coeff * gz;
+                } else {
+                    p->ax -= gx;
+                    p->ay -= gy;
+                    p->az -= gz;
+                }
             }
             if (LHAS(L, CH4G_LAGRANGE_ADDEDMASS) && p->beta_coeff != 0.0) {
                 if (LHAS(L, CH4G_LAGRANGE_GRAVITY)) {
-                    p->ax -= (-p->beta_coeff) * gx;
-                    p->ay -= (-p->beta_coeff) * gy;
-                    p->az -= (-p->beta_coeff) * gz;
+                    if (gvar) {
+                        p->ax -= (-p->beta_coeff) * p->gravity_coeff * gx;
+                        p->ay -= (-p->beta_coeff) * p->gravity_coeff * gy;
+                        p->az -= (-p->beta_coeff) * p->gravity_coeff * gz;
+                    } else {
+                        p->ax -= (-p->beta_coeff) * gx;
+                        p->ay -= (-p->beta_coeff) * gy;
+                        p->az -= (-p->beta_coeff) * gz;
+                    }
                 }
                 if (first) {
                     p->ux_old = p->ux;
@@ -894,6 +918,10 @@ static void move_at(oracle *o, int first)
                         p->vx = 0.0;
                         p->vy = 0.0;
                         p->vz = 0.0;
+                    } else if (LHAS(L, CH4G_LAGRANGE_GRAVITY_VARIABLE)) { /* lagrange.c:3824-3827, :3905-3908 */
+                        p->vx = -gx * p->gravity_coeff * (1.0 - p->beta_coeff) * p->tau_drag;
+                        p->vy = -gy * p->gravity_coeff * (1.0 - p->beta_coeff) * p->tau_drag;
+                        p->vz = -gz * p->gravity_coeff * (1.0 - p->beta_coeff) * p->tau_drag;
                     } else {
                         p->vx = -gx * (1.0 - p->beta_coeff) * p->tau_drag;
                         p->vy = -gy * (1.0 - p->beta_coeff) * p->tau_drag;
@@ -929,6 +957,10 @@ static void move_at(oracle *o, int first)
                         p->vx = 0.0;
                         p->vy = 0.0;
                         p->vz = 0.0;
+                    } else if (LHAS(L, CH4G_LAGRANGE_GRAVITY_VARIABLE)) { /* lagrange.c:3824-3827, :3905-3908 */
+                        p->vx = -gx * p->gravity_coeff * (1.0 - p->beta_coeff) * p->tau_drag;
+                        p->vy = -gy * p->gravity_coeff * (1.0 - p->beta_coeff) * p->tau_drag;
+                        p->vz = -gz * p->gravity_coeff * (1.0 - p->beta_coeff) * p->tau_drag;
                     } else {
                         p->vx = -gx * (1.0 - p->beta_coeff) * p->tau_drag;
                         p->vy = -gy * (1.0 - p->beta_coeff) * p->tau_drag;

@@ -137,6 +137,26 @@ def test_read_run_params_initial_velocity(tmp_path):
     assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == -1
 
 
+def test_read_run_params_variable_gravity(tmp_path):
+    """LAGRANGE_GRAVITY_VARIABLE reads the gravity_coeff ladder
+    (parameters.c:589-600) and acts only inside LAGRANGE_GRAVITY."""
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    text = open(os.path.join(FIXP, "define.h")).read()
+    base = open(os.path.join(FIXP, "param.in")).read()
+    d, pin = tmp_path / "define.h", tmp_path / "param.in"
+    d.write_text(text + "#define LAGRANGE_GRAVITY_VARIABLE\n#define LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG\n")
+    pin.write_text(base)
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == -1
+    pin.write_text(base + "gravity_coeff_types 3\ngravity_coeff_min 0.5\ngravity_coeff_max 2\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == 0
+    v = abi.LAGRANGE_GRAVITY_VARIABLE | abi.LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG
+    assert rp.pcfg.flags & v == v and rp.pcfg.gravity_coeff_max == 2.0
+    d.write_text(text.replace("#define LAGRANGE_GRAVITY\n", "") + "#define LAGRANGE_GRAVITY_VARIABLE\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == 0
+    assert not rp.pcfg.flags & v
+
+
 def test_read_run_params_rejects_unimplemented_switch(tmp_path):
     d = tmp_path / "define.h"
     d.write_text(open(os.path.join(FIX, "define.h")).read() + "#define LB_FLUID_BC_Y_P_SLIP\n")
@@ -423,13 +443,16 @@ ORIENT_PARAMS = ("aspect_ratio_types 3\naspect_ratio_min 0.5\naspect_ratio_max 4
 
 TPART_DEFINES = "#define LAGRANGE_TEMPERATURE\n"
 TPART_PARAMS = "cp_coeff_types 2\ncp_coeff_min 0.5\ncp_coeff_max 2\n"
+GVAR_DEFINES = "#define LAGRANGE_GRAVITY_VARIABLE\n"
+GVAR_PARAMS = "gravity_coeff_types 2\ngravity_coeff_min 0.5\ngravity_coeff_max 1.5\n"
 
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(H5LIB), reason="serial HDF5 not installed")
-@pytest.mark.parametrize("orient,tpart", [(False, False), (True, False), (False, True)],
-                         ids=["plain", "orientation", "particle_temperature"])
-def test_particle_snapshots(tmp_path, orient, tpart):
+@pytest.mark.parametrize("orient,tpart,gvar", [(False, False, False), (True, False, False), (False, True, False),
+                                               (False, False, True)],
+                         ids=["plain", "orientation", "particle_temperature", "gravity_variable"])
+def test_particle_snapshots(tmp_path, orient, tpart, gvar):
     """OUTPUT_H5 with LAGRANGE: output_particles writes RUN/particle_<itime>.h5
     every time_dump_lagr (lagrange.c:1598-2595), one FP64 dataset per member
     under /lagrange in the reference's order, equal to the oracle's records
@@ -440,9 +463,10 @@ def test_particle_snapshots(tmp_path, orient, tpart):
     :2515-2528) and the t_p column of particle_averages.dat."""
     import xml.etree.ElementTree as ET
     define = open(os.path.join(FIXP, "define.h")).read() + "#define OUTPUT_H5\n#define LAGRANGE_OUTPUT_FLUID\n"
-    (tmp_path / "define.h").write_text(define + (ORIENT_DEFINES if orient else "") + (TPART_DEFINES if tpart else ""))
+    (tmp_path / "define.h").write_text(define + (ORIENT_DEFINES if orient else "") + (TPART_DEFINES if tpart else "")
+                                       + (GVAR_DEFINES if gvar else ""))
     (tmp_path / "param.in").write_text(open(os.path.join(FIXP, "param.in")).read() + (ORIENT_PARAMS if orient else "")
-                                       + (TPART_PARAMS if tpart else ""))
+                                       + (TPART_PARAMS if tpart else "") + (GVAR_PARAMS if gvar else ""))
     r = subprocess.run([RUN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     for it in (0, 40, 80, 120):
@@ -469,6 +493,10 @@ def test_particle_snapshots(tmp_path, orient, tpart):
         assert np.unique(ref["aspect_ratio"]).size == 4  # the ladder and the tracer's 1
         rows = [l.split() for l in open(tmp_path / "particle_averages.dat")]
         assert {len(r) for r in rows} == {2 + 18 + 9 + 9 + 5}
+    if gvar:
+        i = members.index("az") + 1
+        members[i:i] = ["gravity_coeff"]
+        assert np.unique(ref["gravity_coeff"]).size == 3  # the ladder and the tracer's 0
     if tpart:
         members += ["temperature_particle", "cp_particle"]
         assert np.unique(ref["cp_p"]).size == 3  # the ladder and the tracer's 0

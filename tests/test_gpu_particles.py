@@ -235,6 +235,19 @@ def test_initial_velocity_fp64(extra):
     compare_particles(o, lat)
 
 
+def test_variable_gravity_fp64():
+    """LAGRANGE_GRAVITY_VARIABLE with its log ladder: the type's factor on
+    the weight, the added-mass buoyancy and the Stokes velocity of a
+    re-injected particle, bit-identical to the oracle."""
+    flags = (HEAVY | abi.LAGRANGE_GRAVITY_VARIABLE | abi.LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG |
+             abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT)
+    pc = abi.make_particle_config(flags, 256, tau_drag=(2, 2.0, 20.0), beta_coeff=(2, 0.2, 0.5),
+                                  gravity_coeff=(3, 0.5, 2.0))
+    o, lat = pair(rb_small(32, 16, g=5e-3), pc, 120)
+    assert np.unique(lat.particles()["gravity_coeff"]).size == 4
+    compare_particles(o, lat)
+
+
 def test_substeps_fp64():
     pc = abi.make_particle_config(HEAVY, 128, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.7, 0.7),
                                   time_dt_lagr_factor=3)

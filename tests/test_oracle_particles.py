@@ -729,3 +729,59 @@ def test_initial_stokes_velocity_is_the_terminal_velocity():
                                                beta_coeff=(1, beta, beta)))
     c = o3.particles()
     assert np.all(c["vy"] == vt) and not c["vx"].any() and np.allclose(c["ux"], 0.01, rtol=1e-14)
+
+
+# --------------------------------------------------------------------------
+# Variable gravity (LAGRANGE_GRAVITY_VARIABLE, lagrange.c:347-384,
+# :3314-3335, :3824-3827, :3905-3908).
+GVAR = abi.LAGRANGE_GRAVITY | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS | abi.LAGRANGE_GRAVITY_VARIABLE
+
+
+def test_gravity_coeff_ladder_order():
+    """gravity_coeff steps come after beta and before the orientation and
+    cp ladders; the tracer gets 0; the log switch gives min * step^i."""
+    o, _ = _thermal()
+    flags = GVAR | abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_TEMPERATURE
+    o.particles_init(abi.make_particle_config(flags, 80, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.5, 0.5),
+                                              gravity_coeff=(3, 0.5, 2.0), cp_coeff=(2, 1.0, 2.0)))
+    ntypes = 2 * 3 * 2 + 1
+    for r in o.particles():
+        t = int(r["name"]) % ntypes
+        if t == 0:
+            assert r["gravity_coeff"] == 0.0
+            continue
+        h = t - 1
+        assert r["tau_drag"] == 1.0 + (h % 2) * 2.0
+        assert r["gravity_coeff"] == 0.5 + ((h // 2) % 3) * 0.75
+        assert r["cp_p"] == 1.0 + ((h // 6) % 2) * 1.0
+    o2, _ = quiescent(6, 6, 6)
+    o2.particles_init(abi.make_particle_config(GVAR | abi.LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG, 3,
+                                               tau_drag=(1, 1.0, 1.0), beta_coeff=(1, 0.5, 0.5),
+                                               gravity_coeff=(3, 0.5, 2.0)))
+    assert np.array_equal(o2.particles()["gravity_coeff"], [0.5, 1.0, 2.0])
+
+
+def test_variable_gravity_settling_and_reinjection():
+    """Each type settles at -c g tau (1 - beta): the factor c scales the
+    weight and the added-mass buoyancy alike (with c on one of them only the
+    velocity would be -g tau (c - beta) or -g tau (1 - c beta)); a
+    re-injected particle takes the Stokes velocity of its own c."""
+    g, tau_p, beta = 1e-3, 5.0, 0.6
+    o, _ = quiescent(6, 6, 6, gravity_y=g)
+    o.particles_init(abi.make_particle_config(GVAR, 6, tau_drag=(1, tau_p, tau_p), beta_coeff=(1, beta, beta),
+                                              gravity_coeff=(3, 0.5, 2.0)))
+    o.step(400)
+    a = o.particles()
+    assert np.allclose(a["vy"], -a["gravity_coeff"] * g * tau_p * (1 - beta), rtol=1e-12)
+    o2, _ = quiescent(6, 8, 4, flags=abi.FLUID_BC_Y, gravity_y=g)
+    flags = GVAR | abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT | \
+        abi.LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM | abi.LAGRANGE_WRAP
+    o2.particles_init(abi.make_particle_config(flags, 3, tau_drag=(1, tau_p, tau_p), beta_coeff=(1, beta, beta),
+                                               gravity_coeff=(3, 0.5, 2.0)))
+    b = o2.particles()
+    b["y"], b["vy"], b["vy_old"] = 0.05, -0.2, -0.2
+    o2.set_particles(b)
+    o2.step(1)
+    c = o2.particles()
+    assert np.all(c["y"] > 7.0)
+    assert np.array_equal(c["vy"], -g * c["gravity_coeff"] * (1 - beta) * tau_p)
