@@ -58,6 +58,11 @@ LAGRANGE_INITIAL_VELOCITY_FLUID = 1 << 26
 LAGRANGE_INITIAL_VELOCITY_STOKES = 1 << 27
 LAGRANGE_GRAVITY_VARIABLE = 1 << 28
 LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG = 1 << 29
+LAGRANGE_OUTPUT_FLUID_AVERAGES = 1 << 30
+LAGRANGE_ADD_TRACER_STATIC = 1 << 31
+# enum ch4g_lagrange_flag2 (ParticleConfig.flags2)
+LAGRANGE_INITIAL_GRID = 1 << 0
+LAGRANGE_INITIAL_PAIRS = 1 << 1
 
 FP64 = 0
 FP32_DEV = 1
@@ -144,12 +149,14 @@ class ParticleConfig(C.Structure):
         ("swim_velocity_types", C.c_double), ("swim_velocity_min", C.c_double), ("swim_velocity_max", C.c_double),
         ("cp_coeff_types", C.c_double), ("cp_coeff_min", C.c_double), ("cp_coeff_max", C.c_double),
         ("gravity_coeff_types", C.c_double), ("gravity_coeff_min", C.c_double), ("gravity_coeff_max", C.c_double),
+        ("flags2", C.c_uint32), ("reserved", C.c_int32),
     ]
 
 
 def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_coeff=(1, 1.0, 1.0),
                          time_dt_lagr_factor=1.0, aspect_ratio=(1, 1.0, 1.0), gyrotaxis_velocity=(1, 0.0, 0.0),
-                         swim_velocity=(1, 0.0, 0.0), cp_coeff=(1, 1.0, 1.0), gravity_coeff=(1, 1.0, 1.0)):
+                         swim_velocity=(1, 0.0, 0.0), cp_coeff=(1, 1.0, 1.0), gravity_coeff=(1, 1.0, 1.0),
+                         flags2=0):
     """ParticleConfig from (types, min, max) ladders as param.in gives them."""
     pc = ParticleConfig()
     pc.flags = flags
@@ -163,6 +170,7 @@ def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_co
     pc.swim_velocity_types, pc.swim_velocity_min, pc.swim_velocity_max = (float(v) for v in swim_velocity)
     pc.cp_coeff_types, pc.cp_coeff_min, pc.cp_coeff_max = (float(v) for v in cp_coeff)
     pc.gravity_coeff_types, pc.gravity_coeff_min, pc.gravity_coeff_max = (float(v) for v in gravity_coeff)
+    pc.flags2 = flags2
     return pc
 
 

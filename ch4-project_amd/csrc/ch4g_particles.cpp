@@ -85,6 +85,16 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
         return fail(-1, "LAGRANGE_TEMPERATURE_INCREMENT_LOG / LAGRANGE_INITIAL_TEMPERATURE_FLUID need LAGRANGE_TEMPERATURE");
     if (tpart && (pc->cp_coeff_types < 1 || pc->cp_coeff_max < pc->cp_coeff_min))
         return fail(-1, "Error in cp_coeff parameters");
+    // the Eulerian probe type (parameters.c:744-750); its feedback volume
+    // would be pow of a negative number (lagrange_feedback.c:87-90)
+    if (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC) && !lhas(*pc, CH4G_LAGRANGE_ADD_TRACER))
+        return fail(-1, "LAGRANGE_ADD_TRACER_STATIC needs LAGRANGE_ADD_TRACER");
+    if (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC) && lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM))
+        return fail(-1, "LAGRANGE_ADD_TRACER_STATIC with LAGRANGE_TWOWAY_MOMENTUM: the probes' feedback is NaN");
+    if ((pc->flags2 & ~(uint32_t)(CH4G_LAGRANGE_INITIAL_GRID | CH4G_LAGRANGE_INITIAL_PAIRS)) || pc->reserved)
+        return fail(-1, "unknown bits in flags2 / reserved");
+    if ((pc->flags2 & CH4G_LAGRANGE_INITIAL_GRID) && pc->particle_number < 1)
+        return fail(-1, "LAGRANGE_INITIAL_GRID needs particle_number >= 1");
     // variable gravity (parameters.c:589-600, lagrange.c:347-376)
     const bool gvar = lhas(*pc, CH4G_LAGRANGE_GRAVITY_VARIABLE);
     if (gvar && !lhas(*pc, CH4G_LAGRANGE_GRAVITY)) return fail(-1, "LAGRANGE_GRAVITY_VARIABLE needs LAGRANGE_GRAVITY");
@@ -567,7 +577,9 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     if (jeff) types *= pc->aspect_ratio_types;
     if (gyro) types *= pc->gyrotaxis_velocity_types;
     if (swim) types *= pc->swim_velocity_types;
-    const int ntr = lhas(*pc, CH4G_LAGRANGE_ADD_TRACER) ? 1 : 0;
+    // fluid_tracers (parameters.c:743-755): the tracer, and with
+    // LAGRANGE_ADD_TRACER_STATIC an Eulerian probe type before it
+    const int ntr = lhas(*pc, CH4G_LAGRANGE_ADD_TRACER) ? (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC) ? 2 : 1) : 0;
     types += ntr;
     const int nt = (int)types;
     std::vector<double> tau((size_t)nt, 0.0), beta((size_t)nt, 0.0), asp((size_t)nt, 0.0), gyv((size_t)nt, 0.0),
@@ -575,6 +587,7 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     double rep = 1.0;
     ladder(tau, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
            lhas(*pc, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
+    if (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC)) tau[0] = -1.0;  // lagrange.c:280-282
     rep *= pc->tau_drag_types;
     if (am) {
         ladder(beta, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
@@ -613,6 +626,8 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     std::vector<ch4g_particle> h((size_t)(np > 0 ? np : 1));
     memset(h.data(), 0, h.size() * sizeof(ch4g_particle));
     const double LNX = L->cfg.NX, LNY = L->cfg.NY, LNZ = L->cfg.NZ;
+    double gd = 1.0;
+    int gnx = 1, gny = 1, gnz = 1;
     for (int64_t i = 0; i < np; i++) {
         ch4g_particle &r = h[(size_t)i];
         r.name = (double)i;
@@ -623,6 +638,44 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         r.x = 0 + erand48(L->xsubi) * LNX;
         r.y = 0 + erand48(L->xsubi) * LNY;
         r.z = 0 + erand48(L->xsubi) * LNZ;
+        if (pc->flags2 & CH4G_LAGRANGE_INITIAL_GRID) {
+            // lagrange.c:774-795 (the draws above are still consumed): the
+            // spacing from an integer quotient, rows x-fastest
+            if (i == 0) {
+                const int64_t q = ((int64_t)L->cfg.NX * L->cfg.NY * L->cfg.NZ) / np;
+                gd = pow((double)q, 1. / 3.);
+                gnx = (int)floor(L->cfg.NX / gd);
+                gny = (int)floor(L->cfg.NY / gd);
+                gnz = (int)floor(L->cfg.NZ / gd);
+                if (gnx == 0) gnx = 1;
+                if (gny == 0) gny = 1;
+                if (gnz == 0) gnz = 1;
+            }
+            const int ip = (int)(i % gnx);
+            const int jp = (int)(i / gnx) % gny;
+            const int kp = (int)(i / ((int64_t)gnx * gny)) % gnz;
+            r.x = 0 + gd / 2. + ip * gd;
+            r.y = 0 + gd / 2. + jp * gd;
+            r.z = 0 + gd / 2. + kp * gd;
+        }
+        if ((pc->flags2 & CH4G_LAGRANGE_INITIAL_PAIRS) && i % 2 == 0 && i > 0) {
+            // lagrange.c:798-814: particle i-1 at distance 1 from particle i
+            // along a random_vector (two draws), mirrored per axis if outside
+            const double theta = acos(2.0 * erand48(L->xsubi) - 1.0);
+            const double phi = 2 * 3.14159265359 * erand48(L->xsubi);
+            double st, ct, sp, cp;
+            sincos(theta, &st, &ct);
+            sincos(phi, &sp, &cp);
+            const double d[3] = {st * sp, st * cp, ct};
+            const double dist = 1.0;
+            ch4g_particle &q = h[(size_t)(i - 1)];
+            q.x = r.x + d[0] * dist;
+            q.y = r.y + d[1] * dist;
+            q.z = r.z + d[2] * dist;
+            if (q.x < 0 || q.x >= 0 + LNX) q.x = r.x - d[0] * dist;
+            if (q.y < 0 || q.y >= 0 + LNY) q.y = r.y - d[1] * dist;
+            if (q.z < 0 || q.z >= 0 + LNZ) q.z = r.z - d[2] * dist;
+        }
         if (jeff) r.aspect_ratio = asp[(size_t)type];
         if (gyro) r.gyrotaxis_velocity = gyv[(size_t)type];
         if (swim) r.swim_velocity = swv[(size_t)type];

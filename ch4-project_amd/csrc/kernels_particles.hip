@@ -752,6 +752,11 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                 told = t;
             }
         }
+        if (tau < 0.0) {  // Eulerian probe: stays put, records the fluid velocity (lagrange.c:3024-3033)
+            vx = ux;
+            vy = uy;
+            vz = uz;
+        }
         if (tau == 0.0) {  // fluid tracer (lagrange.c:3010-3227)
             vx = ux;
             vy = uy;

@@ -368,14 +368,16 @@ static double ptypes(const ch4g_particle_config *pc)
         if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) types *= pc->swim_velocity_types;
     }
     if (fl & CH4G_LAGRANGE_TEMPERATURE) types *= pc->cp_coeff_types; /* parameters.c:615 */
-    if (fl & CH4G_LAGRANGE_ADD_TRACER) types += 1.0;
+    if (fl & CH4G_LAGRANGE_ADD_TRACER) types += (fl & CH4G_LAGRANGE_ADD_TRACER_STATIC) ? 2.0 : 1.0;
     return types;
 }
 
 /* dump_particle_averages (lagrange.c:2598-2930): per particle type, FP64
  * averages of v, v^2, v^4, a, a^2, a^4 (and t, t^2, t^4, dt_t, dt_t^2 with
- * LB_TEMPERATURE, then t_p with LAGRANGE_TEMPERATURE), accumulated in
- * particle order, appended to particle_averages.dat every time_dump_diagn. */
+ * LB_TEMPERATURE, then t_p with LAGRANGE_TEMPERATURE; the fluid velocity at
+ * the particles, u, u^2, u^4, with LAGRANGE_OUTPUT_FLUID_AVERAGES),
+ * accumulated in particle order, appended to particle_averages.dat every
+ * time_dump_diagn.                                                         */
 void dump_particle_averages(void)
 {
     need_particles("dump_particle_averages");
@@ -392,7 +394,8 @@ void dump_particle_averages(void)
     const int orient = (fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ORIENTATION);
     const int second = orient && (fl & CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
     const int tpart = (fl & CH4G_LAGRANGE_TEMPERATURE) != 0;
-    enum { NV = 42 };
+    const int favg = (fl & CH4G_LAGRANGE_OUTPUT_FLUID_AVERAGES) != 0;
+    enum { NV = 51 };
     double *acc = (double *)calloc((size_t)nt * NV, sizeof(double));
     double *cnt = (double *)calloc((size_t)nt, sizeof(double));
     if (!acc || !cnt) die("dump_particle_averages: out of memory");
@@ -443,6 +446,15 @@ void dump_particle_averages(void)
         o[39] += pow(p->dt_ny, 4.0);
         o[40] += pow(p->dt_nz, 4.0);
         o[41] += p->t_p;
+        o[42] += p->ux; /* lagrange.c:2769-2782 */
+        o[43] += p->uy;
+        o[44] += p->uz;
+        o[45] += p->ux * p->ux;
+        o[46] += p->uy * p->uy;
+        o[47] += p->uz * p->uz;
+        o[48] += pow(p->ux, 4.0);
+        o[49] += pow(p->uy, 4.0);
+        o[50] += pow(p->uz, 4.0);
     }
     FILE *fo = fopen("particle_averages.dat", "a");
     if (fo) {
@@ -460,6 +472,8 @@ void dump_particle_averages(void)
                 for (int k = 18; k < 23; k++) fprintf(fo, "%e ", o[k] * norm);
                 if (tpart) fprintf(fo, "%e ", o[41] * norm); /* lagrange.c:2908-2911 */
             }
+            if (favg) /* lagrange.c:2913-2917 */
+                for (int k = 42; k < 51; k++) fprintf(fo, "%e ", o[k] * norm);
             fprintf(fo, "\n");
         }
         fclose(fo);

@@ -78,6 +78,14 @@ static const struct { const char *name; unsigned flag; } LFLAGS[] = {
     {"LAGRANGE_INITIAL_VELOCITY_STOKES", CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES},
     {"LAGRANGE_GRAVITY_VARIABLE", CH4G_LAGRANGE_GRAVITY_VARIABLE},
     {"LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG", CH4G_LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG},
+    {"LAGRANGE_OUTPUT_FLUID_AVERAGES", CH4G_LAGRANGE_OUTPUT_FLUID_AVERAGES},
+    {"LAGRANGE_ADD_TRACER_STATIC", CH4G_LAGRANGE_ADD_TRACER_STATIC},
+};
+
+/* the second word: initial positions */
+static const struct { const char *name; unsigned flag; } LFLAGS2[] = {
+    {"LAGRANGE_INITIAL_GRID", CH4G_LAGRANGE_INITIAL_GRID},
+    {"LAGRANGE_INITIAL_PAIRS", CH4G_LAGRANGE_INITIAL_PAIRS},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -221,6 +229,11 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
                 out->pcfg.flags |= LFLAGS[k].flag;
                 known = 1;
             }
+        for (size_t k = 0; k < sizeof LFLAGS2 / sizeof LFLAGS2[0]; k++)
+            if (!strcmp(nm, LFLAGS2[k].name)) {
+                out->pcfg.flags2 |= LFLAGS2[k].flag;
+                known = 1;
+            }
         if (!strcmp(nm, "LAGRANGE")) out->lagrange = 1;
         if (!strcmp(nm, "LAGRANGE_SMALLTAUD")) smalltaud = 1;
         if (!strcmp(nm, "LAGRANGE_TWOWAY")) twoway = 1;
@@ -253,6 +266,8 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     if (!(out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_JEFFREY))
         out->pcfg.flags &= ~(unsigned)(CH4G_LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG |
                                        CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS);
+    /* the Eulerian probe type sits inside #ifdef LAGRANGE_ADD_TRACER (parameters.c:744-755) */
+    if (!(out->pcfg.flags & CH4G_LAGRANGE_ADD_TRACER)) out->pcfg.flags &= ~(unsigned)CH4G_LAGRANGE_ADD_TRACER_STATIC;
     /* variable gravity sits inside #ifdef LAGRANGE_GRAVITY, its log ladder
        inside LAGRANGE_GRAVITY_VARIABLE (parameters.c:589-590, lagrange.c:347-376) */
     if (!(out->pcfg.flags & CH4G_LAGRANGE_GRAVITY))
@@ -275,7 +290,8 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     if ((out->pcfg.flags & CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES) &&
         (~out->pcfg.flags & (CH4G_LAGRANGE_GRADIENT | CH4G_LAGRANGE_ADDEDMASS)))
         return perr("%s", "LAGRANGE_INITIAL_VELOCITY_STOKES needs LAGRANGE_GRADIENT and LAGRANGE_ADDEDMASS");
-    if (!out->lagrange && out->pcfg.flags) return perr("%s", "LAGRANGE_* switches without LAGRANGE");
+    if (!out->lagrange && (out->pcfg.flags || out->pcfg.flags2))
+        return perr("%s", "LAGRANGE_* switches without LAGRANGE");
 
     static kv t[1024];
     int np = read_param_file(param_in, t, 1024);

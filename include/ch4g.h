@@ -259,7 +259,16 @@ enum ch4g_lagrange_flag {
     CH4G_LAGRANGE_INITIAL_VELOCITY_FLUID              = 1u << 26, /* v starts at the fluid's u, lagrange.c:968-980 */
     CH4G_LAGRANGE_INITIAL_VELOCITY_STOKES             = 1u << 27, /* v starts at -g (1 - beta) tau, lagrange.c:981-991 (needs ADDEDMASS) */
     CH4G_LAGRANGE_GRAVITY_VARIABLE                    = 1u << 28, /* gravity scaled per type, lagrange.c:348-384, :3314-3335 (needs GRAVITY) */
-    CH4G_LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG      = 1u << 29  /* gravity_coeff ladder, lagrange.c:356-376 */
+    CH4G_LAGRANGE_GRAVITY_VARIABLE_INCREMENT_LOG      = 1u << 29, /* gravity_coeff ladder, lagrange.c:356-376 */
+    CH4G_LAGRANGE_OUTPUT_FLUID_AVERAGES               = 1u << 30, /* u, u^2, u^4 columns of particle_averages.dat, lagrange.c:2683-2916 (output only) */
+    CH4G_LAGRANGE_ADD_TRACER_STATIC                   = 1u << 31  /* type 0 an Eulerian probe (tau_drag -1), type 1 the tracer, parameters.c:744-750, lagrange.c:280-282 */
+};
+
+/* Particle switches of the initial positions (define.h:321, :325), a second
+ * word (ch4g_particle_config.flags2).                                      */
+enum ch4g_lagrange_flag2 {
+    CH4G_LAGRANGE_INITIAL_GRID  = 1u << 0,  /* positions on a regular grid, lagrange.c:774-795 */
+    CH4G_LAGRANGE_INITIAL_PAIRS = 1u << 1   /* particle 2k-1 at distance 1 from particle 2k, lagrange.c:798-814 */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -274,6 +283,8 @@ typedef struct ch4g_particle_config {
     double   swim_velocity_types, swim_velocity_min, swim_velocity_max;                /* ORIENTATION_ACTIVE */
     double   cp_coeff_types, cp_coeff_min, cp_coeff_max;                               /* LAGRANGE_TEMPERATURE */
     double   gravity_coeff_types, gravity_coeff_min, gravity_coeff_max;                /* LAGRANGE_GRAVITY_VARIABLE */
+    uint32_t flags2;           /* OR of enum ch4g_lagrange_flag2 */
+    int32_t  reserved;         /* 0 */
 } ch4g_particle_config;
 
 /* One particle: the members of the reference's point_particle

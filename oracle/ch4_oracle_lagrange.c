@@ -134,7 +134,8 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     if (jeff) types *= pc->aspect_ratio_types;
     if (gyro) types *= pc->gyrotaxis_velocity_types;
     if (swim) types *= pc->swim_velocity_types;
-    int ntr = LHAS(L, CH4G_LAGRANGE_ADD_TRACER) ? 1 : 0;
+    /* fluid_tracers: 2 with the Eulerian probe type (parameters.c:744-755) */
+    int ntr = LHAS(L, CH4G_LAGRANGE_ADD_TRACER) ? (LHAS(L, CH4G_LAGRANGE_ADD_TRACER_STATIC) ? 2 : 1) : 0;
     types += ntr;
     const int nt = (int)types;
     double *tau = (double *)calloc(nt, sizeof(double)), *beta = (double *)calloc(nt, sizeof(double));
@@ -145,6 +146,7 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     double rep = 1.0;
     ladder(tau, nt, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
            LHAS(L, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
+    if (LHAS(L, CH4G_LAGRANGE_ADD_TRACER_STATIC)) tau[0] = -1.0; /* the probe type, lagrange.c:280-282 */
     rep *= pc->tau_drag_types;
     if (am) {
         ladder(beta, nt, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
@@ -184,6 +186,8 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     if (!L->tr) return -1;
     for (int i = 0; i < L->npart; i++) L->tr[i].name = i; /* lagrange.c:634-641, one rank */
     const double LNX = o->NX, LNY = o->NY, LNZ = o->NZ;
+    int nxpart = 1, nypart = 1, nzpart = 1;
+    double dpart = 1.0;
     for (int i = 0; i < L->npart; i++) {
         ch4g_particle *p = &L->tr[i];
         int type = ((int)p->name) % nt;
@@ -194,6 +198,37 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
         p->x = 0 + oracle_drand48(o) * LNX;
         p->y = 0 + oracle_drand48(o) * LNY;
         p->z = 0 + oracle_drand48(o) * LNZ;
+        if (pc->flags2 & CH4G_LAGRANGE_INITIAL_GRID) { /* lagrange.c:774-795 */
+            if (i == 0) {
+                int cells = o->NX * o->NY * o->NZ;
+                dpart = pow(cells / L->npart, 1. / 3.);
+                nxpart = (int)floor(o->NX / dpart);
+                nypart = (int)floor(o->NY / dpart);
+                nzpart = (int)floor(o->NZ / dpart);
+                if (nxpart == 0) nxpart = 1;
+                if (nypart == 0) nypart = 1;
+                if (nzpart == 0) nzpart = 1;
+            }
+            int ipart = i % nxpart;
+            int jpart = (int)ceil(i / nxpart) % nypart;
+            int kpart = (int)ceil(i / (nxpart * nypart)) % nzpart;
+            p->x = 0 + dpart / 2. + ipart * dpart;
+            p->y = 0 + dpart / 2. + jpart * dpart;
+            p->z = 0 + dpart / 2. + kpart * dpart;
+        }
+        if ((pc->flags2 & CH4G_LAGRANGE_INITIAL_PAIRS) && i % 2 == 0 && i > 0) { /* lagrange.c:798-814 */
+            double theta = acos(2.0 * oracle_drand48(o) - 1.0);
+            double phi = 2 * 3.14159265359 * oracle_drand48(o);
+            double dx = sin(theta) * sin(phi), dy = sin(theta) * cos(phi), dz = cos(theta);
+            double pair_distance = 1.0;
+            ch4g_particle *q = &L->tr[i - 1];
+            q->x = p->x + dx * pair_distance;
+            q->y = p->y + dy * pair_distance;
+            q->z = p->z + dz * pair_distance;
+            if (q->x < 0 || q->x >= 0 + LNX) q->x = p->x - dx * pair_distance;
+            if (q->y < 0 || q->y >= 0 + LNY) q->y = p->y - dy * pair_distance;
+            if (q->z < 0 || q->z >= 0 + LNZ) q->z = p->z - dz * pair_distance;
+        }
         p->vx = p->vy = p->vz = 0.0;
         p->t = p->t_old = p->dt_t = 0.0;
         if (jeff) p->aspect_ratio = asp[type];
@@ -761,6 +796,11 @@ static void move_at(oracle *o, int first)
                 p->dt_t = (p->t - p->t_old) / dt;
                 p->t_old = p->t;
             }
+        }
+        if (p->tau_drag < 0.0) { /* Eulerian probe: not moved, v = u (lagrange.c:3024-3033) */
+            p->vx = p->ux;
+            p->vy = p->uy;
+            p->vz = p->uz;
         }
         if (p->tau_drag == 0.0) { /* fluid tracer (lagrange.c:3010-3227) */
             p->vx = p->ux;

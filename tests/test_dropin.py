@@ -157,6 +157,23 @@ def test_read_run_params_variable_gravity(tmp_path):
     assert not rp.pcfg.flags & v
 
 
+def test_read_run_params_probes_grid_pairs(tmp_path):
+    """LAGRANGE_ADD_TRACER_STATIC (inside LAGRANGE_ADD_TRACER) in the flag
+    word, LAGRANGE_INITIAL_GRID / _PAIRS in the second word."""
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    text = open(os.path.join(FIXP, "define.h")).read()
+    d, pin = tmp_path / "define.h", os.path.join(FIXP, "param.in")
+    d.write_text(text + "#define LAGRANGE_ADD_TRACER_STATIC\n#define LAGRANGE_INITIAL_GRID\n"
+                 "#define LAGRANGE_INITIAL_PAIRS\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == 0
+    assert rp.pcfg.flags & abi.LAGRANGE_ADD_TRACER_STATIC
+    assert rp.pcfg.flags2 == abi.LAGRANGE_INITIAL_GRID | abi.LAGRANGE_INITIAL_PAIRS
+    d.write_text(text.replace("#define LAGRANGE_ADD_TRACER\n", "") + "#define LAGRANGE_ADD_TRACER_STATIC\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == 0
+    assert not rp.pcfg.flags & abi.LAGRANGE_ADD_TRACER_STATIC
+
+
 def test_read_run_params_rejects_unimplemented_switch(tmp_path):
     d = tmp_path / "define.h"
     d.write_text(open(os.path.join(FIX, "define.h")).read() + "#define LB_FLUID_BC_Y_P_SLIP\n")
@@ -212,10 +229,11 @@ def test_ch4g_run_matches_oracle(tmp_path):
     assert np.loadtxt(tmp_path / "velocity_averages_x.dat").shape == (4, 11)
 
 
-def particle_type_averages(a, ntypes, thermal):
+def particle_type_averages(a, ntypes, thermal, fluid_avg=False):
     """dump_particle_averages (lagrange.c:2598-2930) restated: per type, in
     particle order, sequential FP64 sums of v, v^2, v^4, a, a^2, a^4 (and t,
-    t^2, t^4, dt_t, dt_t^2), divided by the count."""
+    t^2, t^4, dt_t, dt_t^2; then u, u^2, u^4 with
+    LAGRANGE_OUTPUT_FLUID_AVERAGES), divided by the count."""
     rows = []
     for t in range(ntypes):
         sel = [r for r in a if int(r["name"]) % ntypes == t]
@@ -238,6 +256,13 @@ def particle_type_averages(a, ntypes, thermal):
             cols.append([float(r["t"]) ** 4.0 for r in sel])
             cols.append([float(r["dt_t"]) for r in sel])
             cols.append([float(r["dt_t"]) * float(r["dt_t"]) for r in sel])
+        if fluid_avg:
+            for f in ("ux", "uy", "uz"):
+                cols.append([float(r[f]) for r in sel])
+            for f in ("ux", "uy", "uz"):
+                cols.append([float(r[f]) * float(r[f]) for r in sel])
+            for f in ("ux", "uy", "uz"):
+                cols.append([float(r[f]) ** 4.0 for r in sel])
         row = []
         for c in cols:
             acc = 0.0
@@ -249,29 +274,32 @@ def particle_type_averages(a, ntypes, thermal):
 
 
 @pytest.mark.gpu
-def test_ch4g_run_particles_matches_oracle(tmp_path):
+@pytest.mark.parametrize("fluid_avg", [False, True], ids=["plain", "fluid_averages"])
+def test_ch4g_run_particles_matches_oracle(tmp_path, fluid_avg):
     """ch4g_run with particles (inelastic re-injection, added mass, two-way
     coupling) writes particle_averages.dat at t = 0, 40, 80, 120 equal (to
-    the printed digits) to the oracle's particles at those steps."""
+    the printed digits) to the oracle's particles at those steps; with
+    LAGRANGE_OUTPUT_FLUID_AVERAGES also the moments of u at the particles."""
     from oracle import Oracle
-    for f in ("define.h", "param.in"):
-        shutil.copy(os.path.join(FIXP, f), tmp_path / f)
+    shutil.copy(os.path.join(FIXP, "param.in"), tmp_path / "param.in")
+    (tmp_path / "define.h").write_text(open(os.path.join(FIXP, "define.h")).read() +
+                                       ("#define LAGRANGE_OUTPUT_FLUID_AVERAGES\n" if fluid_avg else ""))
     r = subprocess.run([RUN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     rows = np.loadtxt(tmp_path / "particle_averages.dat")
     ntypes = 2 * 2 + 1
-    assert rows.shape == (4 * ntypes, 2 + 18 + 5)
+    assert rows.shape == (4 * ntypes, 2 + 18 + 5 + (9 if fluid_avg else 0))
     lib = C.CDLL(DROPIN)
     rp = RunParams()
-    assert lib.ch4g_read_run_params(os.path.join(FIXP, "define.h").encode(),
-                                    os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == 0
+    assert lib.ch4g_read_run_params(str(tmp_path / "define.h").encode(),
+                                    str(tmp_path / "param.in").encode(), C.byref(rp)) == 0
     o = Oracle(rp.cfg)
     o.initial_conditions()
     o.particles_init(rp.pcfg)
     for k in range(4):
         if k:
             o.step(40)
-        want = particle_type_averages(o.particles(), ntypes, True)
+        want = particle_type_averages(o.particles(), ntypes, True, fluid_avg)
         got = rows[k * ntypes:(k + 1) * ntypes]
         assert np.all(got[:, 0] == 40 * k) and np.array_equal(got[:, 1], np.arange(ntypes))
         np.testing.assert_allclose(got[:, 2:], want, rtol=2e-6, atol=1e-300)

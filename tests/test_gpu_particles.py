@@ -248,6 +248,38 @@ def test_variable_gravity_fp64():
     compare_particles(o, lat)
 
 
+@pytest.mark.parametrize("flags,flags2", [
+    (abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_ADD_TRACER_STATIC | abi.LAGRANGE_WRAP | abi.LAGRANGE_GRADIENT |
+     abi.LAGRANGE_ADDEDMASS | abi.LAGRANGE_GRAVITY, abi.LAGRANGE_INITIAL_GRID),
+    (abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_ADD_TRACER_STATIC | abi.LAGRANGE_WRAP | abi.LAGRANGE_GRADIENT |
+     abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_JEFFREY, abi.LAGRANGE_INITIAL_PAIRS),
+], ids=["static_grid", "static_pairs_orientation"])
+def test_probes_and_initial_positions_fp64(flags, flags2):
+    """Eulerian probes (LAGRANGE_ADD_TRACER_STATIC) and the grid / pair
+    starts: records bit-identical to the oracle after ch4g_particles_init
+    (positions, the draws of the pairs before the orientation draws) and
+    after 60 steps (probes in place, v = u)."""
+    pc = abi.make_particle_config(flags, 256, tau_drag=(2, 2.0, 8.0), beta_coeff=(1, 0.5, 0.5),
+                                  aspect_ratio=(2, 0.5, 4.0), flags2=flags2)
+    c = configs.rb2d(24, 16, 8, Ra=1e5, seed=2024)  # 3-D: the grid lies inside the box
+    cfg = c.copy(gravity_y=1e-3, beta_t=c.beta_t * c.gravity_y / 1e-3)
+    o = Oracle(cfg)
+    o.initial_conditions()
+    o.particles_init(pc)
+    lat = Lattice(cfg)
+    lat.initial_conditions()
+    lat.particles_init(pc)
+    compare_particles(o, lat)
+    a0 = lat.particles()
+    o.step(60)
+    lat.step(60)
+    compare_particles(o, lat)
+    a = lat.particles()
+    probe = a0["tau_drag"] == -1.0
+    assert probe.sum() > 0 and np.array_equal(a["x"][probe], a0["x"][probe])
+    assert np.array_equal(a["vx"][probe], a["ux"][probe])
+
+
 def test_substeps_fp64():
     pc = abi.make_particle_config(HEAVY, 128, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.7, 0.7),
                                   time_dt_lagr_factor=3)

@@ -785,3 +785,71 @@ def test_variable_gravity_settling_and_reinjection():
     c = o2.particles()
     assert np.all(c["y"] > 7.0)
     assert np.array_equal(c["vy"], -g * c["gravity_coeff"] * (1 - beta) * tau_p)
+
+
+# --------------------------------------------------------------------------
+# Eulerian probes and the initial-position switches (lagrange.c:280-282,
+# :774-814, :3024-3033).
+def test_static_probes_stay_and_record_the_flow():
+    """LAGRANGE_ADD_TRACER_STATIC: type 0 (tau_drag -1) is a probe that
+    keeps its position and records the fluid velocity as its own; type 1
+    is the tracer, which moves with the flow."""
+    o, cfg = quiescent(8, 6, 4, tau=0.9)
+    u0 = np.array([0.02, -0.01, 0.005])
+    uniform_flow(o, cfg, u0)
+    flags = abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_ADD_TRACER_STATIC | abi.LAGRANGE_WRAP
+    o.particles_init(abi.make_particle_config(flags, 12, tau_drag=(2, 1.0, 4.0)))
+    a0 = o.particles()
+    ntypes = 2 + 2
+    kind = a0["name"].astype(int) % ntypes
+    assert np.all(a0["tau_drag"][kind == 0] == -1.0) and np.all(a0["tau_drag"][kind == 1] == 0.0)
+    assert set(a0["tau_drag"][kind >= 2]) == {1.0, 4.0}
+    o.step(10)
+    a = o.particles()
+    probe, tracer = kind == 0, kind == 1
+    for c, v, u in (("x", "vx", 0), ("y", "vy", 1), ("z", "vz", 2)):
+        assert np.array_equal(a[c][probe], a0[c][probe])
+        assert np.allclose(a[v][probe], u0[u], rtol=1e-13)
+        d = np.abs(a[c][tracer] - np.mod(a0[c][tracer] + 10 * u0[u], (8, 6, 4)[u]))
+        assert np.all(d < 1e-12)
+
+
+def test_initial_grid_positions():
+    """LAGRANGE_INITIAL_GRID with one particle per node: every particle on
+    its own cell centre, x fastest; the position draws are still consumed
+    (the next draws of the stream follow them)."""
+    o, cfg = quiescent(4, 3, 2, seed=11)
+    o.particles_init(abi.make_particle_config(0, 24, tau_drag=(1, 2.0, 2.0), flags2=abi.LAGRANGE_INITIAL_GRID))
+    a = o.particles()
+    i = np.arange(24)
+    assert np.array_equal(a["x"], i % 4 + 0.5)
+    assert np.array_equal(a["y"], (i // 4) % 3 + 0.5)
+    assert np.array_equal(a["z"], (i // 12) % 2 + 0.5)
+    # a coarser grid: 8 particles on 4x3x2 nodes -> spacing 3^(1/3) (integer 24 // 8)
+    o2, _ = quiescent(4, 3, 2, seed=11)
+    o2.particles_init(abi.make_particle_config(0, 8, tau_drag=(1, 2.0, 2.0), flags2=abi.LAGRANGE_INITIAL_GRID))
+    b = o2.particles()
+    dp = 3.0 ** (1.0 / 3.0)
+    nx, ny, nz = int(4 // dp), int(3 // dp), max(int(2 // dp), 1)
+    j = np.arange(8)
+    assert np.allclose(b["x"], dp / 2 + (j % nx) * dp, rtol=1e-15)
+    assert np.allclose(b["y"], dp / 2 + ((j // nx) % ny) * dp, rtol=1e-15)
+    assert np.allclose(b["z"], dp / 2 + ((j // (nx * ny)) % nz) * dp, rtol=1e-15)
+
+
+def test_initial_pairs_at_unit_distance():
+    """LAGRANGE_INITIAL_PAIRS: particle 2k-1 sits at distance 1 from particle
+    2k (k >= 1), on the box side of it; each pair takes two extra draws after
+    the position draws of its second particle."""
+    seed = 5
+    o, cfg = quiescent(16, 12, 8, seed=seed)
+    n = 21
+    o.particles_init(abi.make_particle_config(0, n, tau_drag=(1, 2.0, 2.0), flags2=abi.LAGRANGE_INITIAL_PAIRS))
+    a = o.particles()
+    pos = np.column_stack([a["x"], a["y"], a["z"]])
+    for k in range(2, n, 2):
+        assert np.linalg.norm(pos[k - 1] - pos[k]) == pytest.approx(1.0, abs=1e-14)
+        assert np.all(pos[k - 1] >= 0) and np.all(pos[k - 1] < (16, 12, 8))
+    # draws: 3 for particle 0, 3 for particle 1 (overwritten), 3 + 2 for particle 2, ...
+    r = glibc_stream(seed, 3 * n + 2 * ((n - 1) // 2))
+    assert a["x"][0] == 16 * r[0] and a["x"][2] == 16 * r[6] and a["x"][4] == 16 * r[6 + 5 + 3]
