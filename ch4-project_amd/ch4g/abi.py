@@ -63,6 +63,8 @@ LAGRANGE_ADD_TRACER_STATIC = 1 << 31
 # enum ch4g_lagrange_flag2 (ParticleConfig.flags2)
 LAGRANGE_INITIAL_GRID = 1 << 0
 LAGRANGE_INITIAL_PAIRS = 1 << 1
+LAGRANGE_ADDEDMASS_LIFT = 1 << 2
+LAGRANGE_ADDEDMASS_WAKEDRAG = 1 << 3
 
 FP64 = 0
 FP32_DEV = 1

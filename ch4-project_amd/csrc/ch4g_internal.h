@@ -232,6 +232,8 @@ struct PartParams {
     const double *u, *t;     // fields of the step: u interleaved [3N], t [N]
     const float4 *uT;        // the same in float, [N] (u_x, u_y, u_z, T) (float-storage lattices), else null
     uint32_t flags;          // ch4g_lagrange_flag
+    uint32_t flags2;         // ch4g_lagrange_flag2
+    double nu;               // kinematic viscosity (wake drag)
     int fluid, thermal, wall_f, wall_t;
     double Tw_bot, Tw_top;   // T_wall - T_ref
     double T_bot, T_top;     // the walls' temperatures a re-injected particle takes (LAGRANGE_TEMPERATURE)

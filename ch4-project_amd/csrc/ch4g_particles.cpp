@@ -91,8 +91,11 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
         return fail(-1, "LAGRANGE_ADD_TRACER_STATIC needs LAGRANGE_ADD_TRACER");
     if (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC) && lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM))
         return fail(-1, "LAGRANGE_ADD_TRACER_STATIC with LAGRANGE_TWOWAY_MOMENTUM: the probes' feedback is NaN");
-    if ((pc->flags2 & ~(uint32_t)(CH4G_LAGRANGE_INITIAL_GRID | CH4G_LAGRANGE_INITIAL_PAIRS)) || pc->reserved)
-        return fail(-1, "unknown bits in flags2 / reserved");
+    const uint32_t known2 = CH4G_LAGRANGE_INITIAL_GRID | CH4G_LAGRANGE_INITIAL_PAIRS | CH4G_LAGRANGE_ADDEDMASS_LIFT |
+                            CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG;
+    if ((pc->flags2 & ~known2) || pc->reserved) return fail(-1, "unknown bits in flags2 / reserved");
+    if ((pc->flags2 & (CH4G_LAGRANGE_ADDEDMASS_LIFT | CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG)) && !am)
+        return fail(-1, "LAGRANGE_ADDEDMASS_LIFT / _WAKEDRAG need LAGRANGE_GRADIENT and LAGRANGE_ADDEDMASS");
     if ((pc->flags2 & CH4G_LAGRANGE_INITIAL_GRID) && pc->particle_number < 1)
         return fail(-1, "LAGRANGE_INITIAL_GRID needs particle_number >= 1");
     // variable gravity (parameters.c:589-600, lagrange.c:347-376)
@@ -276,6 +279,8 @@ PartParams part_params(ch4g_lattice *L)
     P.u = L->d_u;
     P.t = L->d_t;
     P.flags = Q->pc.flags;
+    P.flags2 = Q->pc.flags2;
+    P.nu = L->nu;
     P.fluid = L->base.fluid;
     P.thermal = L->base.thermal;
     P.wall_f = L->base.wall_f;

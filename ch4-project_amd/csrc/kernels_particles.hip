@@ -798,6 +798,18 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
             ax = (ux - vx) * invtau;
             ay = (uy - vy) * invtau;
             az = (uz - vz) * invtau;
+            if (P.flags2 & CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG) {
+                // Schiller-Naumann correction (lagrange.c:3242-3264): diameter
+                // from tau = r^2 / (3 beta nu), particle Reynolds number of the
+                // slip velocity, extra rate 0.15 Re_p^0.687 / tau
+                const double diameter_p = 2.0 * sqrt(tau * 3.0 * beta * P.nu);
+                const double re_p = sqrt((ux - vx) * (ux - vx) + (uy - vy) * (uy - vy) + (uz - vz) * (uz - vz)) *
+                                    diameter_p / P.nu;
+                const double wtau = 0.15 * pow(re_p, 0.687) / tau;
+                ax += (ux - vx) * wtau;
+                ay += (uy - vy) * wtau;
+                az += (uz - vz) * wtau;
+            }
             const bool grav = lflag(P, CH4G_LAGRANGE_GRAVITY);
             if (grav) {
                 ax -= gc * gx;
@@ -824,6 +836,14 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                 dtu[0] = dux;
                 dtu[1] = duy;
                 dtu[2] = duz;
+                if (P.flags2 & CH4G_LAGRANGE_ADDEDMASS_LIFT) {
+                    // lift with C_L = 1/2 (lagrange.c:3365-3380): (beta/3) (u - v) x omega
+                    const double ox = G[7] - G[5], oy = G[2] - G[6], oz = G[3] - G[1];
+                    const double lc = beta / 3.0;
+                    ax += lc * ((uy - vy) * oz - (uz - vz) * oy);
+                    ay += lc * ((uz - vz) * ox - (ux - vx) * oz);
+                    az += lc * ((ux - vx) * oy - (uy - vy) * ox);
+                }
             }
             if (first) {
                 x += dt * vx;

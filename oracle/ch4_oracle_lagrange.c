@@ -28,7 +28,9 @@
  * Pins: tests/test_oracle_particles.py.  Parity unpinned: none of these
  * functions calls libm except pow(r^2, 3/2) of the particle volume and
  * exp(-fac1) of the particle temperature, which the device library's host
- * code computes with the same glibc calls on the same operands.
+ * code computes with the same glibc calls on the same operands, and
+ * pow(Re_p, 0.687) of the wake drag, which the device evaluates with its own
+ * pow (agreement to its last bits, not bit for bit).
  */
 #include <float.h>
 #include <math.h>
@@ -842,6 +844,16 @@ static void move_at(oracle *o, int first)
             p->ax = (p->ux - p->vx) * invtau;
             p->ay = (p->uy - p->vy) * invtau;
             p->az = (p->uz - p->vz) * invtau;
+            if (L->pc.flags2 & CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG) { /* lagrange.c:3242-3264 */
+                double diameter_p = 2.0 * sqrt(p->tau_drag * 3.0 * p->beta_coeff * o->nu);
+                double re_p = sqrt((p->ux - p->vx) * (p->ux - p->vx) + (p->uy - p->vy) * (p->uy - p->vy) +
+                                   (p->uz - p->vz) * (p->uz - p->vz)) *
+                              diameter_p / o->nu;
+                invtau = 0.15 * pow(re_p, 0.687) / p->tau_drag;
+                p->ax += (p->ux - p->vx) * invtau;
+                p->ay += (p->uy - p->vy) * invtau;
+                p->az += (p->uz - p->vz) * invtau;
+            }
             const int gvar = LHAS(L, CH4G_LAGRANGE_GRAVITY_VARIABLE); /* lagrange.c:3314-3335 */
             if (LHAS(L, CH4G_LAGRANGE_GRAVITY)) {
                 if (gvar) {
@@ -883,6 +895,13 @@ static void move_at(oracle *o, int first)
                 p->Dt_ux = dux;
                 p->Dt_uy = duy;
                 p->Dt_uz = duz;
+                if (L->pc.flags2 & CH4G_LAGRANGE_ADDEDMASS_LIFT) { /* lagrange.c:3365-3380 */
+                    double ox = p->dy_uz - p->dz_uy, oy = p->dz_ux - p->dx_uz, oz = p->dx_uy - p->dy_ux;
+                    double lift_coeff = p->beta_coeff / 3.0;
+                    p->ax += lift_coeff * ((p->uy - p->vy) * oz - (p->uz - p->vz) * oy);
+                    p->ay += lift_coeff * ((p->uz - p->vz) * ox - (p->ux - p->vx) * oz);
+                    p->az += lift_coeff * ((p->ux - p->vx) * oy - (p->uy - p->vy) * ox);
+                }
             }
             if (first) {
                 p->x += dt * p->vx;

@@ -169,6 +169,12 @@ def test_read_run_params_probes_grid_pairs(tmp_path):
     assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == 0
     assert rp.pcfg.flags & abi.LAGRANGE_ADD_TRACER_STATIC
     assert rp.pcfg.flags2 == abi.LAGRANGE_INITIAL_GRID | abi.LAGRANGE_INITIAL_PAIRS
+    d.write_text(text + "#define LAGRANGE_ADDEDMASS_LIFT\n#define LAGRANGE_ADDEDMASS_WAKEDRAG\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == 0
+    assert rp.pcfg.flags2 == abi.LAGRANGE_ADDEDMASS_LIFT | abi.LAGRANGE_ADDEDMASS_WAKEDRAG
+    d.write_text(text.replace("#define LAGRANGE_ADDEDMASS\n", "") + "#define LAGRANGE_ADDEDMASS_LIFT\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == 0
+    assert rp.pcfg.flags2 == 0
     d.write_text(text.replace("#define LAGRANGE_ADD_TRACER\n", "") + "#define LAGRANGE_ADD_TRACER_STATIC\n")
     assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == 0
     assert not rp.pcfg.flags & abi.LAGRANGE_ADD_TRACER_STATIC

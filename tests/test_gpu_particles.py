@@ -280,6 +280,22 @@ def test_probes_and_initial_positions_fp64(flags, flags2):
     assert np.array_equal(a["vx"][probe], a["ux"][probe])
 
 
+@pytest.mark.parametrize("flags2", [abi.LAGRANGE_ADDEDMASS_LIFT,
+                                    abi.LAGRANGE_ADDEDMASS_LIFT | abi.LAGRANGE_ADDEDMASS_WAKEDRAG],
+                         ids=["lift", "lift_wakedrag"])
+def test_lift_and_wake_drag_fp64(flags2):
+    """The lift of the added-mass model bit-identical to the oracle; with the
+    wake drag, whose pow(Re_p, 0.687) is the device's own, positions within
+    1e-9 and velocities within 1e-9 of the largest velocity component after
+    100 steps."""
+    pc = abi.make_particle_config(HEAVY, 300, tau_drag=(2, 2.0, 8.0), beta_coeff=(2, 0.4, 0.8), flags2=flags2)
+    o, lat = pair(rb_small(), pc, 100)
+    if flags2 & abi.LAGRANGE_ADDEDMASS_WAKEDRAG:
+        compare_particles(o, lat, exact=False, pos_atol=1e-9, rel=1e-9)
+    else:
+        compare_particles(o, lat)
+
+
 def test_substeps_fp64():
     pc = abi.make_particle_config(HEAVY, 128, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.7, 0.7),
                                   time_dt_lagr_factor=3)

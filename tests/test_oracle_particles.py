@@ -853,3 +853,55 @@ def test_initial_pairs_at_unit_distance():
     # draws: 3 for particle 0, 3 for particle 1 (overwritten), 3 + 2 for particle 2, ...
     r = glibc_stream(seed, 3 * n + 2 * ((n - 1) // 2))
     assert a["x"][0] == 16 * r[0] and a["x"][2] == 16 * r[6] and a["x"][4] == 16 * r[6 + 5 + 3]
+
+
+# --------------------------------------------------------------------------
+# Lift and wake drag of the added-mass model (lagrange.c:3242-3264, :3365-3380).
+def _accel(flags2, u_field, v, tau=4.0, beta=0.6, nu_tau=0.8):
+    o, cfg = quiescent(8, 8, 8, tau=nu_tau)
+    flags = abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS
+    o.particles_init(abi.make_particle_config(flags, 1, tau_drag=(1, tau, tau), beta_coeff=(1, beta, beta),
+                                              flags2=flags2))
+    a = o.particles()
+    a["x"], a["y"], a["z"] = 3.3, 4.2, 3.7
+    a["vx"], a["vy"], a["vz"] = v
+    o.set_particles(a)
+    r = o.move(u_field(cfg), None, first=True)[0]
+    return np.array([r["ax"], r["ay"], r["az"]]), r, (nu_tau - 0.5) / 3.0
+
+
+def test_lift_is_the_slip_cross_vorticity():
+    """LAGRANGE_ADDEDMASS_LIFT adds (beta/3) (u - v) x omega: in the shear
+    u = (gamma (y - 4), 0, 0) the vorticity is (0, 0, -gamma)."""
+    gamma, beta = 2e-3, 0.6
+
+    def shear(cfg):
+        x, y, z = _centres(cfg)
+        u = np.zeros(x.shape + (3,))
+        u[..., 0] = gamma * (y - 4.0)
+        return u
+    v = (0.001, -0.0007, 0.0004)
+    a_l, r, _ = _accel(abi.LAGRANGE_ADDEDMASS_LIFT, shear, v, beta=beta)
+    a_0, _, _ = _accel(0, shear, v, beta=beta)
+    slip = np.array([r["ux"], r["uy"], r["uz"]]) - np.array(v)
+    omega = np.array([0.0, 0.0, -gamma])
+    assert np.allclose(a_l - a_0, beta / 3.0 * np.cross(slip, omega), rtol=1e-10, atol=1e-22)
+    assert abs((a_l - a_0)[1]) > 0
+
+
+def test_wake_drag_follows_schiller_naumann():
+    """LAGRANGE_ADDEDMASS_WAKEDRAG adds 0.15 Re_p^0.687 (u - v) / tau, with
+    the particle Reynolds number of the slip velocity and the diameter of
+    the Stokes particle, d = 2 sqrt(3 beta nu tau)."""
+    u0 = np.array([0.03, -0.01, 0.02])
+
+    def uniform(cfg):
+        return np.broadcast_to(u0, (cfg.NZ, cfg.NY, cfg.NX, 3)).copy()
+    v = (0.0, 0.004, -0.002)
+    tau, beta = 4.0, 0.6
+    a_w, r, nu = _accel(abi.LAGRANGE_ADDEDMASS_WAKEDRAG, uniform, v, tau=tau, beta=beta)
+    a_0, _, _ = _accel(0, uniform, v, tau=tau, beta=beta)
+    slip = u0 - np.array(v)
+    d = 2.0 * math.sqrt(3.0 * beta * nu * tau)
+    re = np.linalg.norm(slip) * d / nu
+    assert np.allclose(a_w - a_0, 0.15 * re ** 0.687 * slip / tau, rtol=1e-12)
