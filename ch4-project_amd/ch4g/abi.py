@@ -65,6 +65,7 @@ LAGRANGE_INITIAL_GRID = 1 << 0
 LAGRANGE_INITIAL_PAIRS = 1 << 1
 LAGRANGE_ADDEDMASS_LIFT = 1 << 2
 LAGRANGE_ADDEDMASS_WAKEDRAG = 1 << 3
+LAGRANGE_ORIENTATION_DRAG = 1 << 4
 
 FP64 = 0
 FP32_DEV = 1

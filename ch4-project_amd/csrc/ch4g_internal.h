@@ -216,6 +216,8 @@ enum PartField {
     PTFAC,              // time_dt_lagr / tau_heat of lagrange.c:4011-4013, host arithmetic
     PTEXP,              // exp(-PTFAC), glibc (the same bits as the reference's call)
     PGC,                // gravity_coeff (LAGRANGE_GRAVITY_VARIABLE)
+    PCPE,               // c_perp of the spheroid drag, host libm (lagrange.c:3277-3288)
+    PCPA,               // c_par
     PF_COUNT
 };
 

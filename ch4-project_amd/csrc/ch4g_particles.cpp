@@ -92,8 +92,12 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
     if (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC) && lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM))
         return fail(-1, "LAGRANGE_ADD_TRACER_STATIC with LAGRANGE_TWOWAY_MOMENTUM: the probes' feedback is NaN");
     const uint32_t known2 = CH4G_LAGRANGE_INITIAL_GRID | CH4G_LAGRANGE_INITIAL_PAIRS | CH4G_LAGRANGE_ADDEDMASS_LIFT |
-                            CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG;
+                            CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG | CH4G_LAGRANGE_ORIENTATION_DRAG;
     if ((pc->flags2 & ~known2) || pc->reserved) return fail(-1, "unknown bits in flags2 / reserved");
+    if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) && !jeff)
+        return fail(-1, "LAGRANGE_ORIENTATION_DRAG needs LAGRANGE_ORIENTATION_JEFFREY (it reads aspect_ratio)");
+    if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) && (pc->flags2 & CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG))
+        return fail(-1, "LAGRANGE_ORIENTATION_DRAG replaces the Stokes drag and its wake correction");
     if ((pc->flags2 & (CH4G_LAGRANGE_ADDEDMASS_LIFT | CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG)) && !am)
         return fail(-1, "LAGRANGE_ADDEDMASS_LIFT / _WAKEDRAG need LAGRANGE_GRADIENT and LAGRANGE_ADDEDMASS");
     if ((pc->flags2 & CH4G_LAGRANGE_INITIAL_GRID) && pc->particle_number < 1)
@@ -152,6 +156,22 @@ void heat_constants(double kappa, double nu, double dt, double tau, double beta,
     fac1 *= dt;
     *fac = fac1;
     *efac = exp(-fac1);
+}
+
+// Drag coefficients of a spheroid of aspect ratio alpha = a_par / a_perp
+// across and along its axis, relative to the sphere of radius a_perp
+// (lagrange.c:3276-3288); per-particle constants, formed here with glibc.
+void spheroid_drag_constants(double alpha, double *c_perp, double *c_par)
+{
+    double beta = 0.0;
+    if (alpha == 1) {
+        *c_perp = *c_par = 1.0;
+        return;
+    }
+    if (alpha > 1) beta = log(alpha + sqrt(alpha * alpha - 1.0)) / (alpha * sqrt(alpha * alpha - 1.0));
+    if (alpha < 1) beta = acos(alpha) / (alpha * sqrt(1.0 - alpha * alpha));
+    *c_perp = 8.0 * (alpha * alpha - 1.) / (3.0 * alpha * ((2.0 * alpha * alpha - 3.0) * beta + 1.0));
+    *c_par = 4.0 * (alpha * alpha - 1.) / (3.0 * alpha * ((2.0 * alpha * alpha - 1.0) * beta - 1.0));
 }
 
 // particle volume and density ratio of the feedback (lagrange_feedback.c:84-90)
@@ -366,6 +386,10 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         S(PTFAC) = hf;
         S(PTEXP) = he;
         S(PGC) = r.gravity_coeff;
+        double cpe = 0.0, cpa = 0.0;
+        if (Q->pc.flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) spheroid_drag_constants(r.aspect_ratio, &cpe, &cpa);
+        S(PCPE) = cpe;
+        S(PCPA) = cpa;
         alive[(size_t)p] = 1;
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));

@@ -794,11 +794,28 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
         }
         if (tau > 0.0) {  // inertial particle (lagrange.c:3229-3466)
             double dux = 0.0, duy = 0.0, duz = 0.0;
-            const double invtau = 1.0 / tau;
-            ax = (ux - vx) * invtau;
-            ay = (uy - vy) * invtau;
-            az = (uz - vz) * invtau;
-            if (P.flags2 & CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG) {
+            bool spheroid = false;  // the switches below need the gradient build
+            if constexpr (GRAD) spheroid = (P.flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) != 0;
+            if (spheroid) {
+                // Stokes drag of a spheroid of aspect ratio alpha along p
+                // (lagrange.c:3267-3310): M = c_perp I + (c_par - c_perp) p p,
+                // rate 1 / (tau alpha); c_perp, c_par formed on the host
+                const double alpha = R(PASP), cpe = R(PCPE), cpa = R(PCPA);
+                const double it = 1.0 / (tau * alpha);
+                const double d = cpa - cpe;
+                const double m00 = cpe + d * pv[0] * pv[0], m11 = cpe + d * pv[1] * pv[1], m22 = cpe + d * pv[2] * pv[2];
+                const double m01 = d * pv[0] * pv[1], m02 = d * pv[0] * pv[2], m12 = d * pv[1] * pv[2];
+                const double uvx = ux - vx, uvy = uy - vy, uvz = uz - vz;
+                ax = (m00 * uvx + m01 * uvy + m02 * uvz) * it;
+                ay = (m01 * uvx + m11 * uvy + m12 * uvz) * it;
+                az = (m02 * uvx + m12 * uvy + m22 * uvz) * it;
+            } else {
+                const double invtau = 1.0 / tau;
+                ax = (ux - vx) * invtau;
+                ay = (uy - vy) * invtau;
+                az = (uz - vz) * invtau;
+            }
+            if (GRAD && !spheroid && (P.flags2 & CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG)) {
                 // Schiller-Naumann correction (lagrange.c:3242-3264): diameter
                 // from tau = r^2 / (3 beta nu), particle Reynolds number of the
                 // slip velocity, extra rate 0.15 Re_p^0.687 / tau
@@ -836,7 +853,7 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                 dtu[0] = dux;
                 dtu[1] = duy;
                 dtu[2] = duz;
-                if (P.flags2 & CH4G_LAGRANGE_ADDEDMASS_LIFT) {
+                if (GRAD && (P.flags2 & CH4G_LAGRANGE_ADDEDMASS_LIFT)) {
                     // lift with C_L = 1/2 (lagrange.c:3365-3380): (beta/3) (u - v) x omega
                     const double ox = G[7] - G[5], oy = G[2] - G[6], oz = G[3] - G[1];
                     const double lc = beta / 3.0;

@@ -88,6 +88,7 @@ static const struct { const char *name; unsigned flag; } LFLAGS2[] = {
     {"LAGRANGE_INITIAL_PAIRS", CH4G_LAGRANGE_INITIAL_PAIRS},
     {"LAGRANGE_ADDEDMASS_LIFT", CH4G_LAGRANGE_ADDEDMASS_LIFT},
     {"LAGRANGE_ADDEDMASS_WAKEDRAG", CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG},
+    {"LAGRANGE_ORIENTATION_DRAG", CH4G_LAGRANGE_ORIENTATION_DRAG},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -271,6 +272,13 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     /* lift and wake drag sit inside #ifdef LAGRANGE_ADDEDMASS (lagrange.c:3241, :3325) */
     if (!((out->pcfg.flags & CH4G_LAGRANGE_GRADIENT) && (out->pcfg.flags & CH4G_LAGRANGE_ADDEDMASS)))
         out->pcfg.flags2 &= ~(unsigned)(CH4G_LAGRANGE_ADDEDMASS_LIFT | CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG);
+    /* the spheroid drag reads aspect_ratio (LAGRANGE_ORIENTATION_JEFFREY, typedef.h:423-424)
+       and replaces the Stokes drag with its wake correction (lagrange.c:3234-3268) */
+    if (out->pcfg.flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) {
+        if (!(out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_JEFFREY))
+            return perr("%s", "LAGRANGE_ORIENTATION_DRAG needs LAGRANGE_ORIENTATION_JEFFREY");
+        out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG;
+    }
     /* the Eulerian probe type sits inside #ifdef LAGRANGE_ADD_TRACER (parameters.c:744-755) */
     if (!(out->pcfg.flags & CH4G_LAGRANGE_ADD_TRACER)) out->pcfg.flags &= ~(unsigned)CH4G_LAGRANGE_ADD_TRACER_STATIC;
     /* variable gravity sits inside #ifdef LAGRANGE_GRAVITY, its log ladder

@@ -264,8 +264,9 @@ enum ch4g_lagrange_flag {
     CH4G_LAGRANGE_ADD_TRACER_STATIC                   = 1u << 31  /* type 0 an Eulerian probe (tau_drag -1), type 1 the tracer, parameters.c:744-750, lagrange.c:280-282 */
 };
 
-/* Particle switches of the initial positions (define.h:321, :325) and of
- * the added-mass forces (define.h:337-338), a second word
+/* Particle switches of the initial positions (define.h:321, :325), of the
+ * added-mass forces (define.h:337-338) and of the spheroid drag (define.h:368),
+ * a second word
  * (ch4g_particle_config.flags2).  The wake drag calls pow(): its results
  * follow the reference's to the last bits of the device's pow, not
  * bit for bit.                                                              */
@@ -273,7 +274,8 @@ enum ch4g_lagrange_flag2 {
     CH4G_LAGRANGE_INITIAL_GRID  = 1u << 0,  /* positions on a regular grid, lagrange.c:774-795 */
     CH4G_LAGRANGE_INITIAL_PAIRS = 1u << 1,  /* particle 2k-1 at distance 1 from particle 2k, lagrange.c:798-814 */
     CH4G_LAGRANGE_ADDEDMASS_LIFT     = 1u << 2,  /* lift (beta/3) (u - v) x omega, lagrange.c:3365-3380 (needs ADDEDMASS) */
-    CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG = 1u << 3   /* Schiller-Naumann drag 0.15 Re_p^0.687 / tau, lagrange.c:3242-3264 (needs ADDEDMASS) */
+    CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG = 1u << 3,  /* Schiller-Naumann drag 0.15 Re_p^0.687 / tau, lagrange.c:3242-3264 (needs ADDEDMASS) */
+    CH4G_LAGRANGE_ORIENTATION_DRAG   = 1u << 4   /* Stokes drag of a spheroid along p, lagrange.c:3267-3310 (needs ORIENTATION_JEFFREY) */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */

@@ -22,7 +22,10 @@
  *   gradient_vector (LAGRANGE)    gradients.c:78-499
  *   move_particles                lagrange.c:2934-4025 (thermal relaxation of
  *                                 t_p :4007-4020, wall temperatures of a
- *                                 re-injected particle :3839-3842, :3915-3918)
+ *                                 re-injected particle :3839-3842, :3915-3918;
+ *                                 spheroid drag :3267-3310 with glibc log /
+ *                                 acos / sqrt, which the device library's host
+ *                                 code evaluates on the same operands)
  *   sendrecv_particles (1 rank)   lagrange.c:4028-4221
  *   add_particle_feedbacks        lagrange_feedback.c:11-256, 525-836
  * Pins: tests/test_oracle_particles.py.  Parity unpinned: none of these
@@ -841,10 +844,38 @@ static void move_at(oracle *o, int first)
         if (p->tau_drag > 0.0) { /* inertial particle (lagrange.c:3229-3466) */
             double dux = 0.0, duy = 0.0, duz = 0.0; /* Dt_u */
             double invtau = 1.0 / p->tau_drag;
-            p->ax = (p->ux - p->vx) * invtau;
-            p->ay = (p->uy - p->vy) * invtau;
-            p->az = (p->uz - p->vz) * invtau;
-            if (L->pc.flags2 & CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG) { /* lagrange.c:3242-3264 */
+            if (L->pc.flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) {
+                /* Stokes drag of a spheroid (lagrange.c:3267-3310, PRL 119,
+                 * 254501): resistance across and along the axis p relative to
+                 * the sphere of radius a_perp, time tau_drag * aspect_ratio */
+                double alpha = p->aspect_ratio, beta = 0.0, c_perp, c_par;
+                invtau = 1.0 / (p->tau_drag * alpha);
+                if (alpha == 1) {
+                    beta = c_perp = c_par = 1.0;
+                } else {
+                    if (alpha > 1) beta = log(alpha + sqrt(alpha * alpha - 1.0)) / (alpha * sqrt(alpha * alpha - 1.0));
+                    if (alpha < 1) beta = acos(alpha) / (alpha * sqrt(1.0 - alpha * alpha));
+                    c_perp = 8.0 * (alpha * alpha - 1.) / (3.0 * alpha * ((2.0 * alpha * alpha - 3.0) * beta + 1.0));
+                    c_par = 4.0 * (alpha * alpha - 1.) / (3.0 * alpha * ((2.0 * alpha * alpha - 1.0) * beta - 1.0));
+                }
+                double P0 = p->px, P1 = p->py, P2 = p->pz, M[3][3];
+                M[0][0] = c_perp + (c_par - c_perp) * P0 * P0;
+                M[1][1] = c_perp + (c_par - c_perp) * P1 * P1;
+                M[2][2] = c_perp + (c_par - c_perp) * P2 * P2;
+                M[0][1] = M[1][0] = (c_par - c_perp) * P0 * P1;
+                M[0][2] = M[2][0] = (c_par - c_perp) * P0 * P2;
+                M[1][2] = M[2][1] = (c_par - c_perp) * P1 * P2;
+                double uvx = p->ux - p->vx, uvy = p->uy - p->vy, uvz = p->uz - p->vz;
+                p->ax = (M[0][0] * uvx + M[0][1] * uvy + M[0][2] * uvz) * invtau;
+                p->ay = (M[1][0] * uvx + M[1][1] * uvy + M[1][2] * uvz) * invtau;
+                p->az = (M[2][0] * uvx + M[2][1] * uvy + M[2][2] * uvz) * invtau;
+            } else {
+                p->ax = (p->ux - p->vx) * invtau;
+                p->ay = (p->uy - p->vy) * invtau;
+                p->az = (p->uz - p->vz) * invtau;
+            }
+            if ((L->pc.flags2 & CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG) &&
+                !(L->pc.flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG)) { /* lagrange.c:3242-3264 */
                 double diameter_p = 2.0 * sqrt(p->tau_drag * 3.0 * p->beta_coeff * o->nu);
                 double re_p = sqrt((p->ux - p->vx) * (p->ux - p->vx) + (p->uy - p->vy) * (p->uy - p->vy) +
                                    (p->uz - p->vz) * (p->uz - p->vz)) *

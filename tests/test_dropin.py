@@ -172,6 +172,8 @@ def test_read_run_params_probes_grid_pairs(tmp_path):
     d.write_text(text + "#define LAGRANGE_ADDEDMASS_LIFT\n#define LAGRANGE_ADDEDMASS_WAKEDRAG\n")
     assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == 0
     assert rp.pcfg.flags2 == abi.LAGRANGE_ADDEDMASS_LIFT | abi.LAGRANGE_ADDEDMASS_WAKEDRAG
+    d.write_text(text + "#define LAGRANGE_ORIENTATION_DRAG\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == -1  # needs the Jeffery switch
     d.write_text(text.replace("#define LAGRANGE_ADDEDMASS\n", "") + "#define LAGRANGE_ADDEDMASS_LIFT\n")
     assert lib.ch4g_read_run_params(str(d).encode(), pin.encode(), C.byref(rp)) == 0
     assert rp.pcfg.flags2 == 0

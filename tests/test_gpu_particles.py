@@ -296,6 +296,21 @@ def test_lift_and_wake_drag_fp64(flags2):
         compare_particles(o, lat)
 
 
+def test_spheroid_drag_fp64():
+    """LAGRANGE_ORIENTATION_DRAG with prolate and oblate types (aspect
+    ratios 0.5 .. 6; response times long enough for the explicit AB2 steps): the drag tensor along p, with added mass and gravity,
+    bit-identical to the oracle over 100 steps (c_perp, c_par by glibc on
+    both sides)."""
+    flags = (HEAVY | abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_JEFFREY |
+             abi.LAGRANGE_ORIENTATION_JEFFREY_INCREMENT_LOG)
+    pc = abi.make_particle_config(flags, 300, tau_drag=(2, 8.0, 20.0), beta_coeff=(1, 0.5, 0.5),
+                                  aspect_ratio=(4, 0.5, 6.0), flags2=abi.LAGRANGE_ORIENTATION_DRAG,
+                                  time_dt_lagr_factor=2)
+    o, lat = pair(rb_small(), pc, 100)
+    assert np.unique(lat.particles()["aspect_ratio"]).size == 5
+    compare_particles(o, lat)
+
+
 def test_substeps_fp64():
     pc = abi.make_particle_config(HEAVY, 128, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.7, 0.7),
                                   time_dt_lagr_factor=3)

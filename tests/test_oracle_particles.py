@@ -905,3 +905,49 @@ def test_wake_drag_follows_schiller_naumann():
     d = 2.0 * math.sqrt(3.0 * beta * nu * tau)
     re = np.linalg.norm(slip) * d / nu
     assert np.allclose(a_w - a_0, 0.15 * re ** 0.687 * slip / tau, rtol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# Stokes drag of a spheroid (LAGRANGE_ORIENTATION_DRAG, lagrange.c:3267-3310).
+def _happel_brenner(alpha):
+    """Resistance of a spheroid with semi-axes a_par = alpha a_perp (along
+    the axis) and a_perp, across and along the axis, over that of the
+    sphere of radius a_perp (Happel & Brenner, Low Reynolds number
+    hydrodynamics, the prolate and oblate translation formulas)."""
+    if alpha > 1:
+        e = math.sqrt(1 - 1 / alpha ** 2)
+        L = math.log((1 + e) / (1 - e))
+        across = 32 * alpha * e ** 3 / (6 * (2 * e + (3 * e * e - 1) * L))
+        along = 16 * alpha * e ** 3 / (6 * ((1 + e * e) * L - 2 * e))
+    else:
+        e = math.sqrt(1 - alpha * alpha)
+        s = math.asin(e)
+        across = 16 * e ** 3 / (6 * ((1 + 2 * e * e) * s - e * math.sqrt(1 - e * e)))
+        along = 8 * e ** 3 / (6 * (e * math.sqrt(1 - e * e) - (1 - 2 * e * e) * s))
+    return across, along
+
+
+@pytest.mark.parametrize("alpha", [4.0, 1.0, 0.4])
+def test_spheroid_drag_matches_happel_brenner(alpha):
+    """In a uniform flow a spheroid at rest takes the acceleration
+    c (u - v) / (tau alpha), c the translational resistance along or across
+    its axis relative to the sphere of radius a_perp (tau_drag is that
+    sphere's response time, the mass grows with alpha)."""
+    tau = 4.0
+    u0 = 0.02
+    flags = abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_JEFFREY
+    across, along = _happel_brenner(alpha) if alpha != 1.0 else (1.0, 1.0)
+    for p_axis, c in (((1.0, 0.0, 0.0), along), ((0.0, 1.0, 0.0), across), ((0.0, 0.0, 1.0), across)):
+        o, cfg = quiescent(8, 8, 8, tau=0.8)
+        o.particles_init(abi.make_particle_config(flags, 1, tau_drag=(1, tau, tau), aspect_ratio=(1, alpha, alpha),
+                                                  flags2=abi.LAGRANGE_ORIENTATION_DRAG))
+        a = o.particles()
+        a["x"], a["y"], a["z"] = 3.3, 4.2, 3.7
+        a["px"], a["py"], a["pz"] = p_axis
+        a["vx"] = a["vy"] = a["vz"] = 0.0
+        o.set_particles(a)
+        u = np.zeros((8, 8, 8, 3))
+        u[..., 0] = u0
+        r = o.move(u, None, first=True)[0]
+        assert r["ax"] == pytest.approx(c * u0 / (tau * alpha), rel=1e-13)
+        assert abs(r["ay"]) < 1e-18 and abs(r["az"]) < 1e-18
