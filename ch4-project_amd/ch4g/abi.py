@@ -66,6 +66,10 @@ LAGRANGE_INITIAL_PAIRS = 1 << 1
 LAGRANGE_ADDEDMASS_LIFT = 1 << 2
 LAGRANGE_ADDEDMASS_WAKEDRAG = 1 << 3
 LAGRANGE_ORIENTATION_DRAG = 1 << 4
+LAGRANGE_RADIUSANDDENSITY = 1 << 5
+LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS = 1 << 6
+LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY = 1 << 7
+LAGRANGE_BC_RADIUSANDDENSITY = 1 << 8
 
 FP64 = 0
 FP32_DEV = 1
@@ -153,13 +157,17 @@ class ParticleConfig(C.Structure):
         ("cp_coeff_types", C.c_double), ("cp_coeff_min", C.c_double), ("cp_coeff_max", C.c_double),
         ("gravity_coeff_types", C.c_double), ("gravity_coeff_min", C.c_double), ("gravity_coeff_max", C.c_double),
         ("flags2", C.c_uint32), ("reserved", C.c_int32),
+        ("particle_radius_types", C.c_double), ("particle_radius_min", C.c_double),
+        ("particle_radius_max", C.c_double),
+        ("particle_density_types", C.c_double), ("particle_density_min", C.c_double),
+        ("particle_density_max", C.c_double),
     ]
 
 
 def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_coeff=(1, 1.0, 1.0),
                          time_dt_lagr_factor=1.0, aspect_ratio=(1, 1.0, 1.0), gyrotaxis_velocity=(1, 0.0, 0.0),
                          swim_velocity=(1, 0.0, 0.0), cp_coeff=(1, 1.0, 1.0), gravity_coeff=(1, 1.0, 1.0),
-                         flags2=0):
+                         flags2=0, particle_radius=(1, 0.0, 0.0), particle_density=(1, 1.0, 1.0)):
     """ParticleConfig from (types, min, max) ladders as param.in gives them."""
     pc = ParticleConfig()
     pc.flags = flags
@@ -174,6 +182,10 @@ def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_co
     pc.cp_coeff_types, pc.cp_coeff_min, pc.cp_coeff_max = (float(v) for v in cp_coeff)
     pc.gravity_coeff_types, pc.gravity_coeff_min, pc.gravity_coeff_max = (float(v) for v in gravity_coeff)
     pc.flags2 = flags2
+    (pc.particle_radius_types, pc.particle_radius_min,
+     pc.particle_radius_max) = (float(v) for v in particle_radius)
+    (pc.particle_density_types, pc.particle_density_min,
+     pc.particle_density_max) = (float(v) for v in particle_density)
     return pc
 
 

@@ -37,11 +37,25 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
     if (pc->particle_number < 0 || (int64_t)pc->particle_number * 8 > (int64_t)1 << 31)
         return fail(-1, "particle_number out of range");
     if (!L->base.fluid) return fail(-1, "particles need LB_FLUID");
-    // parameters.c:566, :581, :167
-    if (pc->tau_drag_types < 1 || pc->tau_drag_max < pc->tau_drag_min) return fail(-1, "Error in tau_drag parameters");
+    // parameters.c:538, :551 (radius and density) or :566, :581 (tau_drag, beta_coeff), :167
+    const bool rad = (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY) != 0;
     const bool am = lhas(*pc, CH4G_LAGRANGE_ADDEDMASS);
-    if (am && (pc->beta_coeff_types < 1 || pc->beta_coeff_max < pc->beta_coeff_min))
-        return fail(-1, "Error in beta_coeff parameters");
+    if (rad) {
+        if (pc->particle_radius_types < 1 || pc->particle_radius_max < pc->particle_radius_min)
+            return fail(-1, "Error in particle_radius parameters");
+        if (pc->particle_density_types < 1 || pc->particle_density_max < pc->particle_density_min)
+            return fail(-1, "Error in particle_density parameters");
+    } else {
+        if (pc->tau_drag_types < 1 || pc->tau_drag_max < pc->tau_drag_min)
+            return fail(-1, "Error in tau_drag parameters");
+        if (am && (pc->beta_coeff_types < 1 || pc->beta_coeff_max < pc->beta_coeff_min))
+            return fail(-1, "Error in beta_coeff parameters");
+    }
+    if (!rad && (pc->flags2 & (CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS |
+                               CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY | CH4G_LAGRANGE_BC_RADIUSANDDENSITY)))
+        return fail(-1, "LAGRANGE_RADIUSandDENSITY_* / LAGRANGE_BC_RADIUSandDENSITY need LAGRANGE_RADIUSandDENSITY");
+    if ((pc->flags2 & CH4G_LAGRANGE_BC_RADIUSANDDENSITY) && !am)
+        return fail(-1, "LAGRANGE_BC_RADIUSandDENSITY needs LAGRANGE_ADDEDMASS (the radius is sqrt(3 nu beta tau))");
     if (!(pc->time_dt_lagr_factor >= 1.0)) return fail(-1, "time_dt_lagr > time_dt (time_dt_lagr_factor < 1)");
     // the reference reads beta_coeff and the gradient only when both switches
     // are on (lagrange.c:239-262 allocates beta under GRADIENT && ADDEDMASS)
@@ -92,7 +106,9 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
     if (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC) && lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM))
         return fail(-1, "LAGRANGE_ADD_TRACER_STATIC with LAGRANGE_TWOWAY_MOMENTUM: the probes' feedback is NaN");
     const uint32_t known2 = CH4G_LAGRANGE_INITIAL_GRID | CH4G_LAGRANGE_INITIAL_PAIRS | CH4G_LAGRANGE_ADDEDMASS_LIFT |
-                            CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG | CH4G_LAGRANGE_ORIENTATION_DRAG;
+                            CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG | CH4G_LAGRANGE_ORIENTATION_DRAG |
+                            CH4G_LAGRANGE_RADIUSANDDENSITY | CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS |
+                            CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY | CH4G_LAGRANGE_BC_RADIUSANDDENSITY;
     if ((pc->flags2 & ~known2) || pc->reserved) return fail(-1, "unknown bits in flags2 / reserved");
     if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) && !jeff)
         return fail(-1, "LAGRANGE_ORIENTATION_DRAG needs LAGRANGE_ORIENTATION_JEFFREY (it reads aspect_ratio)");
@@ -599,8 +615,10 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     const bool swim = orient && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
     const bool tpart = lhas(*pc, CH4G_LAGRANGE_TEMPERATURE);
     const bool gvar = lhas(*pc, CH4G_LAGRANGE_GRAVITY_VARIABLE);
+    const bool rad = (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY) != 0;
     double types = pc->tau_drag_types;
     if (am) types *= pc->beta_coeff_types;
+    if (rad) types = pc->particle_radius_types * pc->particle_density_types;  // parameters.c:541, :554
     if (gvar) types *= pc->gravity_coeff_types;  // parameters.c:600
     if (tpart) types *= pc->cp_coeff_types;  // parameters.c:615
     if (jeff) types *= pc->aspect_ratio_types;
@@ -614,14 +632,33 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     std::vector<double> tau((size_t)nt, 0.0), beta((size_t)nt, 0.0), asp((size_t)nt, 0.0), gyv((size_t)nt, 0.0),
         swv((size_t)nt, 0.0), cpv((size_t)nt, 0.0), gcv((size_t)nt, 0.0);
     double rep = 1.0;
-    ladder(tau, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
-           lhas(*pc, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
-    if (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC)) tau[0] = -1.0;  // lagrange.c:280-282
-    rep *= pc->tau_drag_types;
-    if (am) {
-        ladder(beta, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
-               lhas(*pc, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
-        rep *= pc->beta_coeff_types;
+    if (rad) {
+        // lagrange.c:150-260: radius and density ladders (tracers 0 and 1),
+        // tau = (2/9) rho_p r^2 / nu, or with added mass beta = 3 / (1 + 2 rho_p)
+        // and tau = r^2 / (3 beta nu) (rho_f = 1)
+        std::vector<double> rv((size_t)nt, 0.0), dv((size_t)nt, 0.0);
+        ladder(rv, ntr, 0.0, pc->particle_radius_types, pc->particle_radius_min, pc->particle_radius_max,
+               (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS) != 0, rep);
+        rep *= pc->particle_radius_types;
+        ladder(dv, ntr, 1.0, pc->particle_density_types, pc->particle_density_min, pc->particle_density_max,
+               (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY) != 0, rep);
+        rep *= pc->particle_density_types;
+        for (int i = 0; i < nt; i++) tau[(size_t)i] = (2. / 9.) * dv[(size_t)i] * pow(rv[(size_t)i], 2.0) / (L->nu);
+        if (am)
+            for (int i = 0; i < nt; i++) {
+                beta[(size_t)i] = 3.0 / (1.0 + 2.0 * dv[(size_t)i]);
+                tau[(size_t)i] = pow(rv[(size_t)i], 2.0) / (3.0 * beta[(size_t)i] * L->nu);
+            }
+    } else {
+        ladder(tau, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
+               lhas(*pc, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
+        if (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC)) tau[0] = -1.0;  // lagrange.c:280-282
+        rep *= pc->tau_drag_types;
+        if (am) {
+            ladder(beta, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
+                   lhas(*pc, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
+            rep *= pc->beta_coeff_types;
+        }
     }
     if (gvar) {  // after beta, before the orientation ladders (lagrange.c:347-384)
         ladder(gcv, ntr, 0.0, pc->gravity_coeff_types, pc->gravity_coeff_min, pc->gravity_coeff_max,

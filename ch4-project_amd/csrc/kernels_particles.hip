@@ -1009,7 +1009,10 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
             const bool norand = lflag(P, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM);
             const bool vzero = lflag(P, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO);
             const double fac1 = inel ? 0.0 : -1.0, fac2 = inel ? 0.0 : 1.0;
-            const double radius = 0.0;
+            // LAGRANGE_BC_RADIUSandDENSITY: the walls act at the particle's
+            // radius sqrt(3 nu beta tau) (lagrange.c:3777-3788); 0 otherwise
+            const double radius =
+                (P.flags2 & CH4G_LAGRANGE_BC_RADIUSANDDENSITY) ? sqrt(tau * P.nu * 3.0 * beta) : 0.0;
             const double SY = P.NY;
             if (y < radius) {
                 if (!reinj) {

@@ -359,6 +359,8 @@ static double ptypes(const ch4g_particle_config *pc)
     const unsigned fl = pc->flags;
     double types = pc->tau_drag_types;
     if ((fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ADDEDMASS)) types *= pc->beta_coeff_types;
+    if (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY) /* parameters.c:541, :554 */
+        types = pc->particle_radius_types * pc->particle_density_types;
     if (fl & CH4G_LAGRANGE_GRAVITY_VARIABLE) types *= pc->gravity_coeff_types; /* parameters.c:600 */
     if ((fl & CH4G_LAGRANGE_GRADIENT) && (fl & CH4G_LAGRANGE_ORIENTATION)) {
         if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {

@@ -89,6 +89,10 @@ static const struct { const char *name; unsigned flag; } LFLAGS2[] = {
     {"LAGRANGE_ADDEDMASS_LIFT", CH4G_LAGRANGE_ADDEDMASS_LIFT},
     {"LAGRANGE_ADDEDMASS_WAKEDRAG", CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG},
     {"LAGRANGE_ORIENTATION_DRAG", CH4G_LAGRANGE_ORIENTATION_DRAG},
+    {"LAGRANGE_RADIUSandDENSITY", CH4G_LAGRANGE_RADIUSANDDENSITY},
+    {"LAGRANGE_RADIUSandDENSITY_INCREMENT_LOG_RADIUS", CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS},
+    {"LAGRANGE_RADIUSandDENSITY_INCREMENT_LOG_DENSITY", CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY},
+    {"LAGRANGE_BC_RADIUSandDENSITY", CH4G_LAGRANGE_BC_RADIUSANDDENSITY},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -279,6 +283,14 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
             return perr("%s", "LAGRANGE_ORIENTATION_DRAG needs LAGRANGE_ORIENTATION_JEFFREY");
         out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG;
     }
+    /* the log ladders act inside LAGRANGE_RADIUSandDENSITY (lagrange.c:160, :199); the
+       wall radius needs it and the added mass (lagrange.c:3777-3786) */
+    if (!(out->pcfg.flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY))
+        out->pcfg.flags2 &= ~(unsigned)(CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS |
+                                        CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY);
+    if (!((out->pcfg.flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY) && (out->pcfg.flags & CH4G_LAGRANGE_GRADIENT) &&
+          (out->pcfg.flags & CH4G_LAGRANGE_ADDEDMASS)))
+        out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_BC_RADIUSANDDENSITY;
     /* the Eulerian probe type sits inside #ifdef LAGRANGE_ADD_TRACER (parameters.c:744-755) */
     if (!(out->pcfg.flags & CH4G_LAGRANGE_ADD_TRACER)) out->pcfg.flags &= ~(unsigned)CH4G_LAGRANGE_ADD_TRACER_STATIC;
     /* variable gravity sits inside #ifdef LAGRANGE_GRAVITY, its log ladder
@@ -364,10 +376,20 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
             need(t, np, "time_dt_lagr_factor", &pc->time_dt_lagr_factor) || need(t, np, "particle_number", &v2))
             return -1;
         pc->particle_number = (int32_t)v2;
-        if (need(t, np, "tau_drag_types", &pc->tau_drag_types) || need(t, np, "tau_drag_min", &pc->tau_drag_min) ||
-            need(t, np, "tau_drag_max", &pc->tau_drag_max))
+        if (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY) { /* parameters.c:530-556 */
+            if (need(t, np, "particle_radius_types", &pc->particle_radius_types) ||
+                need(t, np, "particle_radius_min", &pc->particle_radius_min) ||
+                need(t, np, "particle_radius_max", &pc->particle_radius_max) ||
+                need(t, np, "particle_density_types", &pc->particle_density_types) ||
+                need(t, np, "particle_density_min", &pc->particle_density_min) ||
+                need(t, np, "particle_density_max", &pc->particle_density_max))
+                return -1;
+        } else if (need(t, np, "tau_drag_types", &pc->tau_drag_types) ||
+                   need(t, np, "tau_drag_min", &pc->tau_drag_min) || need(t, np, "tau_drag_max", &pc->tau_drag_max)) {
             return -1;
-        if ((pc->flags & CH4G_LAGRANGE_GRADIENT) && (pc->flags & CH4G_LAGRANGE_ADDEDMASS)) {
+        }
+        if (!(pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY) && (pc->flags & CH4G_LAGRANGE_GRADIENT) &&
+            (pc->flags & CH4G_LAGRANGE_ADDEDMASS)) {
             if (need(t, np, "beta_coeff_types", &pc->beta_coeff_types) ||
                 need(t, np, "beta_coeff_min", &pc->beta_coeff_min) || need(t, np, "beta_coeff_max", &pc->beta_coeff_max))
                 return -1;

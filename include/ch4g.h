@@ -265,6 +265,7 @@ enum ch4g_lagrange_flag {
 };
 
 /* Particle switches of the initial positions (define.h:321, :325), of the
+ * radius/density parametrisation (define.h:328-330, :345), of the
  * added-mass forces (define.h:337-338) and of the spheroid drag (define.h:368),
  * a second word
  * (ch4g_particle_config.flags2).  The wake drag calls pow(): its results
@@ -275,7 +276,11 @@ enum ch4g_lagrange_flag2 {
     CH4G_LAGRANGE_INITIAL_PAIRS = 1u << 1,  /* particle 2k-1 at distance 1 from particle 2k, lagrange.c:798-814 */
     CH4G_LAGRANGE_ADDEDMASS_LIFT     = 1u << 2,  /* lift (beta/3) (u - v) x omega, lagrange.c:3365-3380 (needs ADDEDMASS) */
     CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG = 1u << 3,  /* Schiller-Naumann drag 0.15 Re_p^0.687 / tau, lagrange.c:3242-3264 (needs ADDEDMASS) */
-    CH4G_LAGRANGE_ORIENTATION_DRAG   = 1u << 4   /* Stokes drag of a spheroid along p, lagrange.c:3267-3310 (needs ORIENTATION_JEFFREY) */
+    CH4G_LAGRANGE_ORIENTATION_DRAG   = 1u << 4,  /* Stokes drag of a spheroid along p, lagrange.c:3267-3310 (needs ORIENTATION_JEFFREY) */
+    CH4G_LAGRANGE_RADIUSANDDENSITY   = 1u << 5,  /* radius and density ladders define tau_drag (and beta), lagrange.c:150-260 */
+    CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS  = 1u << 6,  /* lagrange.c:160-164, :177-180 */
+    CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY = 1u << 7,  /* lagrange.c:199-203, :216-219 */
+    CH4G_LAGRANGE_BC_RADIUSANDDENSITY = 1u << 8  /* walls at the particle radius sqrt(3 nu beta tau), lagrange.c:3777-3788 */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -292,6 +297,9 @@ typedef struct ch4g_particle_config {
     double   gravity_coeff_types, gravity_coeff_min, gravity_coeff_max;                /* LAGRANGE_GRAVITY_VARIABLE */
     uint32_t flags2;           /* OR of enum ch4g_lagrange_flag2 */
     int32_t  reserved;         /* 0 */
+    /* LAGRANGE_RADIUSandDENSITY: read instead of tau_drag_* / beta_coeff_* (parameters.c:530-556) */
+    double   particle_radius_types, particle_radius_min, particle_radius_max;
+    double   particle_density_types, particle_density_min, particle_density_max;
 } ch4g_particle_config;
 
 /* One particle: the members of the reference's point_particle

@@ -132,8 +132,10 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     const int swim = orient && LHAS(L, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
     const int tpart = LHAS(L, CH4G_LAGRANGE_TEMPERATURE);
     const int gvar = LHAS(L, CH4G_LAGRANGE_GRAVITY) && LHAS(L, CH4G_LAGRANGE_GRAVITY_VARIABLE);
+    const int rad = (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY) != 0;
     double types = pc->tau_drag_types;
     if (am) types *= pc->beta_coeff_types;
+    if (rad) types = pc->particle_radius_types * pc->particle_density_types; /* parameters.c:530-556 */
     if (gvar) types *= pc->gravity_coeff_types; /* parameters.c:600 */
     if (tpart) types *= pc->cp_coeff_types; /* parameters.c:615 */
     if (jeff) types *= pc->aspect_ratio_types;
@@ -149,14 +151,35 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     double *gcv = (double *)calloc(nt, sizeof(double));
     if (!tau || !beta || !asp || !gyv || !swv || !cpv || !gcv) return -1;
     double rep = 1.0;
-    ladder(tau, nt, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
-           LHAS(L, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
-    if (LHAS(L, CH4G_LAGRANGE_ADD_TRACER_STATIC)) tau[0] = -1.0; /* the probe type, lagrange.c:280-282 */
-    rep *= pc->tau_drag_types;
-    if (am) {
-        ladder(beta, nt, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
-               LHAS(L, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
-        rep *= pc->beta_coeff_types;
+    if (rad) { /* lagrange.c:150-260 */
+        double *radius = (double *)calloc(nt, sizeof(double)), *density = (double *)calloc(nt, sizeof(double));
+        if (!radius || !density) return -1;
+        ladder(radius, nt, ntr, 0.0, pc->particle_radius_types, pc->particle_radius_min, pc->particle_radius_max,
+               (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS) != 0, rep);
+        rep *= pc->particle_radius_types;
+        ladder(density, nt, ntr, 1.0, pc->particle_density_types, pc->particle_density_min, pc->particle_density_max,
+               (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY) != 0, rep);
+        rep *= pc->particle_density_types;
+        /* tau = (2/9) (rho_p / rho_f) r^2 / nu with rho_f = 1 */
+        for (int i = 0; i < nt; i++) tau[i] = (2. / 9.) * density[i] * pow(radius[i], 2.0) / (o->nu);
+        if (am)
+            for (int i = 0; i < nt; i++) {
+                /* beta from the density; tau = r^2 / (3 beta nu) with the added mass */
+                beta[i] = 3.0 / (1.0 + 2.0 * density[i]);
+                tau[i] = pow(radius[i], 2.0) / (3.0 * beta[i] * o->nu);
+            }
+        free(radius);
+        free(density);
+    } else {
+        ladder(tau, nt, ntr, 0.0, pc->tau_drag_types, pc->tau_drag_min, pc->tau_drag_max,
+               LHAS(L, CH4G_LAGRANGE_TAUDRAG_INCREMENT_LOG), rep);
+        if (LHAS(L, CH4G_LAGRANGE_ADD_TRACER_STATIC)) tau[0] = -1.0; /* the probe type, lagrange.c:280-282 */
+        rep *= pc->tau_drag_types;
+        if (am) {
+            ladder(beta, nt, ntr, 1.0, pc->beta_coeff_types, pc->beta_coeff_min, pc->beta_coeff_max,
+                   LHAS(L, CH4G_LAGRANGE_ADDEDMASS_INCREMENT_LOG), rep);
+            rep *= pc->beta_coeff_types;
+        }
     }
     /* lagrange.c:347-384: gravity_coeff (linear or geometric), 0 for the tracer */
     if (gvar) {
@@ -988,7 +1011,11 @@ static void move_at(oracle *o, int first)
             const int inel = LHAS(L, CH4G_LAGRANGE_BC_INELASTIC);
             const int reinj = LHAS(L, CH4G_LAGRANGE_BC_INELASTIC_REINJECT);
             const double fac1 = inel ? 0.0 : -1.0, fac2 = inel ? 0.0 : 1.0;
-            const double radius = 0.0;
+            /* lagrange.c:3775-3788: 0, or with LAGRANGE_BC_RADIUSandDENSITY the
+             * radius from tau = r^2 / (3 beta nu) */
+            double radius = 0.0;
+            if (L->pc.flags2 & CH4G_LAGRANGE_BC_RADIUSANDDENSITY)
+                radius = sqrt(p->tau_drag * o->nu * 3.0 * p->beta_coeff);
             if (p->y < radius) {
                 if (!reinj) {
                     p->y = fac1 * (p->y - radius) + radius;

@@ -182,6 +182,27 @@ def test_read_run_params_probes_grid_pairs(tmp_path):
     assert not rp.pcfg.flags & abi.LAGRANGE_ADD_TRACER_STATIC
 
 
+def test_read_run_params_radius_and_density(tmp_path):
+    """LAGRANGE_RADIUSandDENSITY reads particle_radius_* and
+    particle_density_* instead of tau_drag_* / beta_coeff_*
+    (parameters.c:530-556)."""
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    text = open(os.path.join(FIXP, "define.h")).read()
+    base = "\n".join(l for l in open(os.path.join(FIXP, "param.in")).read().splitlines()
+                     if not l.startswith(("tau_drag", "beta_coeff"))) + "\n"
+    d, pin = tmp_path / "define.h", tmp_path / "param.in"
+    d.write_text(text + "#define LAGRANGE_RADIUSandDENSITY\n#define LAGRANGE_BC_RADIUSandDENSITY\n")
+    pin.write_text(base)
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == -1
+    pin.write_text(base + "particle_radius_types 2\nparticle_radius_min 0.5\nparticle_radius_max 1\n"
+                   "particle_density_types 1\nparticle_density_min 2\nparticle_density_max 2\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == 0
+    want = abi.LAGRANGE_RADIUSANDDENSITY | abi.LAGRANGE_BC_RADIUSANDDENSITY
+    assert rp.pcfg.flags2 & want == want
+    assert (rp.pcfg.particle_radius_types, rp.pcfg.particle_radius_max, rp.pcfg.particle_density_min) == (2.0, 1.0, 2.0)
+
+
 def test_read_run_params_rejects_unimplemented_switch(tmp_path):
     d = tmp_path / "define.h"
     d.write_text(open(os.path.join(FIX, "define.h")).read() + "#define LB_FLUID_BC_Y_P_SLIP\n")

@@ -311,6 +311,20 @@ def test_spheroid_drag_fp64():
     compare_particles(o, lat)
 
 
+@pytest.mark.parametrize("extra2", [0, abi.LAGRANGE_BC_RADIUSANDDENSITY |
+                                    abi.LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS],
+                         ids=["ladders", "wall_radius_log"])
+def test_radius_and_density_fp64(extra2):
+    """LAGRANGE_RADIUSandDENSITY: tau_drag and beta_coeff from the radius and
+    density ladders, and with LAGRANGE_BC_RADIUSandDENSITY elastic walls at
+    the particle's radius; bit-identical to the oracle over 120 steps."""
+    pc = abi.make_particle_config(HEAVY, 256, particle_radius=(2, 0.4, 1.0), particle_density=(2, 1.5, 3.0),
+                                  flags2=abi.LAGRANGE_RADIUSANDDENSITY | extra2)
+    o, lat = pair(rb_small(32, 16, g=1e-3), pc, 120)
+    assert np.unique(lat.particles()["tau_drag"]).size == 5
+    compare_particles(o, lat)
+
+
 def test_substeps_fp64():
     pc = abi.make_particle_config(HEAVY, 128, tau_drag=(2, 1.0, 3.0), beta_coeff=(1, 0.7, 0.7),
                                   time_dt_lagr_factor=3)

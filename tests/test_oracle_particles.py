@@ -951,3 +951,52 @@ def test_spheroid_drag_matches_happel_brenner(alpha):
         r = o.move(u, None, first=True)[0]
         assert r["ax"] == pytest.approx(c * u0 / (tau * alpha), rel=1e-13)
         assert abs(r["ay"]) < 1e-18 and abs(r["az"]) < 1e-18
+
+
+# --------------------------------------------------------------------------
+# Radius and density parametrisation (lagrange.c:150-260, :3775-3788).
+RAD = abi.LAGRANGE_RADIUSANDDENSITY
+
+
+def test_radius_density_give_the_stokes_response_time():
+    """tau_drag and beta_coeff from the radius and density ladders (radius
+    fastest, then density; the tracer has r = 0): without added mass the
+    Stokes time 2 rho_p r^2 / (9 mu); with it, beta = 3 rho_f / (rho_f +
+    2 rho_p) and the added-mass time 2 (rho_p + rho_f / 2) r^2 / (9 mu),
+    rho_f = 1, mu = nu."""
+    nu = (0.8 - 0.5) / 3.0
+    for flags in (abi.LAGRANGE_ADD_TRACER, abi.LAGRANGE_ADD_TRACER | abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS):
+        o, _ = quiescent(6, 6, 6, tau=0.8)
+        o.particles_init(abi.make_particle_config(flags, 10, particle_radius=(2, 0.5, 1.5),
+                                                  particle_density=(2, 1.0, 4.0), flags2=RAD))
+        a = o.particles()
+        for r in a:
+            t = int(r["name"]) % 5
+            if t == 0:
+                assert r["tau_drag"] == 0.0
+                continue
+            rad, rho = (0.5, 1.5)[(t - 1) % 2], (1.0, 4.0)[((t - 1) // 2) % 2]
+            if flags & abi.LAGRANGE_ADDEDMASS:
+                assert r["beta_coeff"] == pytest.approx(3.0 / (1.0 + 2.0 * rho), rel=1e-15)
+                assert r["tau_drag"] == pytest.approx(2 * (rho + 0.5) * rad ** 2 / (9 * nu), rel=1e-14)
+            else:
+                assert r["tau_drag"] == pytest.approx(2 * rho * rad ** 2 / (9 * nu), rel=1e-14)
+
+
+def test_walls_at_the_particle_radius():
+    """LAGRANGE_BC_RADIUSandDENSITY: an elastic particle of radius r that
+    ends a step at height y < r is mirrored about y = r (its centre cannot
+    come closer to the wall than its radius)."""
+    o, cfg = quiescent(6, 8, 4, flags=abi.FLUID_BC_Y, tau=0.8)
+    flags = abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ADDEDMASS
+    o.particles_init(abi.make_particle_config(flags, 1, particle_radius=(1, 0.5, 0.5),
+                                              particle_density=(1, 2.0, 2.0),
+                                              flags2=RAD | abi.LAGRANGE_BC_RADIUSANDDENSITY))
+    a = o.particles()
+    a["x"], a["y"], a["z"] = 2.0, 0.55, 2.0
+    a["vy"] = -0.3
+    o.set_particles(a)
+    u = np.zeros((4, 8, 6, 3))
+    r = o.move(u, None, first=True)[0]
+    assert r["y"] == pytest.approx(2 * 0.5 - (0.55 - 0.3), abs=1e-12)
+    assert r["vy"] > 0
