@@ -70,6 +70,10 @@ LAGRANGE_RADIUSANDDENSITY = 1 << 5
 LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS = 1 << 6
 LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY = 1 << 7
 LAGRANGE_BC_RADIUSANDDENSITY = 1 << 8
+LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC = 1 << 9
+LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE = 1 << 10
+LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION = 1 << 11
+LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1 << 12
 
 FP64 = 0
 FP32_DEV = 1

@@ -108,8 +108,28 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
     const uint32_t known2 = CH4G_LAGRANGE_INITIAL_GRID | CH4G_LAGRANGE_INITIAL_PAIRS | CH4G_LAGRANGE_ADDEDMASS_LIFT |
                             CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG | CH4G_LAGRANGE_ORIENTATION_DRAG |
                             CH4G_LAGRANGE_RADIUSANDDENSITY | CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS |
-                            CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY | CH4G_LAGRANGE_BC_RADIUSANDDENSITY;
+                            CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY | CH4G_LAGRANGE_BC_RADIUSANDDENSITY |
+                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC | CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE |
+                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION |
+                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION;
     if ((pc->flags2 & ~known2) || pc->reserved) return fail(-1, "unknown bits in flags2 / reserved");
+    // swimming modes: members they read (typedef.h:416-430, :462-466)
+    {
+        const uint32_t f2 = pc->flags2;
+        const bool active = lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
+        const bool second = lhas(*pc, CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
+        if ((f2 & (CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC | CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE |
+                   CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION |
+                   CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION)) && !active)
+            return fail(-1, "LAGRANGE_ORIENTATION_ACTIVE_* modes need LAGRANGE_ORIENTATION_ACTIVE");
+        if ((f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE) && !L->base.thermal)
+            return fail(-1, "LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE needs LB_TEMPERATURE");
+        if ((f2 & (CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION |
+                   CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION)) && !second)
+            return fail(-1, "swimming along n needs LAGRANGE_ORIENTATION_SECONDORIENTATION");
+        if ((f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION) && !jeff)
+            return fail(-1, "LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION needs LAGRANGE_ORIENTATION_JEFFREY");
+    }
     if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) && !jeff)
         return fail(-1, "LAGRANGE_ORIENTATION_DRAG needs LAGRANGE_ORIENTATION_JEFFREY (it reads aspect_ratio)");
     if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) && (pc->flags2 & CH4G_LAGRANGE_ADDEDMASS_WAKEDRAG))

@@ -762,11 +762,29 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
             vy = uy;
             vz = uz;
             if (orient && lflag(P, CH4G_LAGRANGE_ORIENTATION_ACTIVE)) {
-                // swimming along p (lagrange.c:3179-3186)
+                // swimming (lagrange.c:3134-3186): the modes in the
+                // reference's #elif order, else along p on top of u
                 const double sw = R(PSWIM);
-                vx += sw * pv[0];
-                vy += sw * pv[1];
-                vz += sw * pv[2];
+                const uint32_t f2 = P.flags2;
+                if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC) {
+                    vx = sw * pv[0];
+                    vy = sw * pv[1];
+                    vz = sw * pv[2];
+                } else if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE) {
+                    const double react = dtt > 0.0 ? 1.0 : 0.0;
+                    vx += react * sw * pv[0];
+                    vy += react * sw * pv[1];
+                    vz += react * sw * pv[2];
+                } else if ((f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION) ||
+                           ((f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION) && !(R(PASP) >= 1.0))) {
+                    vx += sw * nv[0];
+                    vy += sw * nv[1];
+                    vz += sw * nv[2];
+                } else {
+                    vx += sw * pv[0];
+                    vy += sw * pv[1];
+                    vz += sw * pv[2];
+                }
             }
             if (first) {
                 vxo = vx;

@@ -93,6 +93,10 @@ static const struct { const char *name; unsigned flag; } LFLAGS2[] = {
     {"LAGRANGE_RADIUSandDENSITY_INCREMENT_LOG_RADIUS", CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS},
     {"LAGRANGE_RADIUSandDENSITY_INCREMENT_LOG_DENSITY", CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY},
     {"LAGRANGE_BC_RADIUSandDENSITY", CH4G_LAGRANGE_BC_RADIUSANDDENSITY},
+    {"LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC", CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC},
+    {"LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE", CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE},
+    {"LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION", CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION},
+    {"LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION", CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -291,6 +295,14 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     if (!((out->pcfg.flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY) && (out->pcfg.flags & CH4G_LAGRANGE_GRADIENT) &&
           (out->pcfg.flags & CH4G_LAGRANGE_ADDEDMASS)))
         out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_BC_RADIUSANDDENSITY;
+    /* the swimming modes sit inside #ifdef LAGRANGE_ORIENTATION_ACTIVE (lagrange.c:3134-3178) */
+    {
+        const unsigned modes = CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC | CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE |
+                               CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION |
+                               CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION;
+        if (!((out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_ACTIVE) && (out->pcfg.flags & CH4G_LAGRANGE_GRADIENT)))
+            out->pcfg.flags2 &= ~modes;
+    }
     /* the Eulerian probe type sits inside #ifdef LAGRANGE_ADD_TRACER (parameters.c:744-755) */
     if (!(out->pcfg.flags & CH4G_LAGRANGE_ADD_TRACER)) out->pcfg.flags &= ~(unsigned)CH4G_LAGRANGE_ADD_TRACER_STATIC;
     /* variable gravity sits inside #ifdef LAGRANGE_GRAVITY, its log ladder

@@ -280,7 +280,12 @@ enum ch4g_lagrange_flag2 {
     CH4G_LAGRANGE_RADIUSANDDENSITY   = 1u << 5,  /* radius and density ladders define tau_drag (and beta), lagrange.c:150-260 */
     CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS  = 1u << 6,  /* lagrange.c:160-164, :177-180 */
     CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY = 1u << 7,  /* lagrange.c:199-203, :216-219 */
-    CH4G_LAGRANGE_BC_RADIUSANDDENSITY = 1u << 8  /* walls at the particle radius sqrt(3 nu beta tau), lagrange.c:3777-3788 */
+    CH4G_LAGRANGE_BC_RADIUSANDDENSITY = 1u << 8, /* walls at the particle radius sqrt(3 nu beta tau), lagrange.c:3777-3788 */
+    /* swimming modes of LAGRANGE_ORIENTATION_ACTIVE (lagrange.c:3134-3178), the first set wins */
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC            = 1u << 9,   /* v = swim p, no fluid velocity */
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE          = 1u << 10,  /* swim only while dt_t > 0 (needs LB_TEMPERATURE) */
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION    = 1u << 11,  /* swim along n */
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1u << 12   /* along p if aspect_ratio >= 1, else along n */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */

@@ -835,10 +835,36 @@ static void move_at(oracle *o, int first)
             p->vy = p->uy;
             p->vz = p->uz;
             if (LHAS(L, CH4G_LAGRANGE_ORIENTATION) && LHAS(L, CH4G_LAGRANGE_ORIENTATION_ACTIVE)) {
-                /* swimming along p (lagrange.c:3179-3186) */
-                p->vx += p->swim_velocity * p->px;
-                p->vy += p->swim_velocity * p->py;
-                p->vz += p->swim_velocity * p->pz;
+                /* swimming: the #elif chain of lagrange.c:3134-3186 */
+                const unsigned f2 = L->pc.flags2;
+                if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC) {
+                    p->vx = p->swim_velocity * p->px;
+                    p->vy = p->swim_velocity * p->py;
+                    p->vz = p->swim_velocity * p->pz;
+                } else if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE) {
+                    double reactivity = p->dt_t > 0.0 ? 1.0 : 0.0;
+                    p->vx += reactivity * p->swim_velocity * p->px;
+                    p->vy += reactivity * p->swim_velocity * p->py;
+                    p->vz += reactivity * p->swim_velocity * p->pz;
+                } else if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION) {
+                    p->vx += p->swim_velocity * p->nx;
+                    p->vy += p->swim_velocity * p->ny;
+                    p->vz += p->swim_velocity * p->nz;
+                } else if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION) {
+                    if (p->aspect_ratio >= 1.0) { /* rods */
+                        p->vx += p->swim_velocity * p->px;
+                        p->vy += p->swim_velocity * p->py;
+                        p->vz += p->swim_velocity * p->pz;
+                    } else { /* disks */
+                        p->vx += p->swim_velocity * p->nx;
+                        p->vy += p->swim_velocity * p->ny;
+                        p->vz += p->swim_velocity * p->nz;
+                    }
+                } else {
+                    p->vx += p->swim_velocity * p->px;
+                    p->vy += p->swim_velocity * p->py;
+                    p->vz += p->swim_velocity * p->pz;
+                }
             }
             if (first) {
                 p->vx_old = p->vx;

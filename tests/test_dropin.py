@@ -92,6 +92,11 @@ def test_read_run_params_particles():
         open(d, "w").write(text + "#define LAGRANGE_ORIENTATION_JEFFREY\n")
         assert lib.ch4g_read_run_params(d.encode(), os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == 0
         assert not rp.pcfg.flags & abi.LAGRANGE_ORIENTATION_JEFFREY
+        # the deterministic swimming modes are read into the second word
+        open(d, "w").write(text + "#define LAGRANGE_ORIENTATION\n#define LAGRANGE_ORIENTATION_JEFFREY\n"
+                           "#define LAGRANGE_ORIENTATION_ACTIVE\n#define LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC\n")
+        assert lib.ch4g_read_run_params(d.encode(), pin.encode(), C.byref(rp)) == 0
+        assert rp.pcfg.flags2 == abi.LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC
         # the stochastic orientation variants are not implemented
         open(d, "w").write(text + "#define LAGRANGE_ORIENTATION\n#define LAGRANGE_ORIENTATION_DIFFUSION\n")
         assert lib.ch4g_read_run_params(d.encode(), os.path.join(FIXP, "param.in").encode(), C.byref(rp)) == -1

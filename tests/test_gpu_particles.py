@@ -155,6 +155,22 @@ def test_orientation_fp64(case):
     compare_particles(o, lat)
 
 
+@pytest.mark.parametrize("mode", [abi.LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC, abi.LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE,
+                                  abi.LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION,
+                                  abi.LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION],
+                         ids=["ballistic", "temperature", "second", "selective"])
+def test_swimming_modes_fp64(mode):
+    """The swimming modes of LAGRANGE_ORIENTATION_ACTIVE in Rayleigh-Benard
+    with rods and disks (aspect ratios 0.5 .. 8), bit-identical to the
+    oracle over 100 steps."""
+    flags = ORIENT | abi.LAGRANGE_GRAVITY | abi.LAGRANGE_ADDEDMASS
+    pc = abi.make_particle_config(flags, 300, tau_drag=(2, 0.0, 4.0), beta_coeff=(1, 0.8, 0.8),
+                                  aspect_ratio=(3, 0.5, 8.0), gyrotaxis_velocity=(2, 0.5, 2.0),
+                                  swim_velocity=(2, 0.0, 0.002), time_dt_lagr_factor=2, flags2=mode)
+    o, lat = pair(rb_small(), pc, 100)
+    compare_particles(o, lat)
+
+
 def test_orientation_needs_the_gradient():
     lat = Lattice(rb_small())
     lat.initial_conditions()

@@ -1000,3 +1000,51 @@ def test_walls_at_the_particle_radius():
     r = o.move(u, None, first=True)[0]
     assert r["y"] == pytest.approx(2 * 0.5 - (0.55 - 0.3), abs=1e-12)
     assert r["vy"] > 0
+
+
+# --------------------------------------------------------------------------
+# Swimming modes of LAGRANGE_ORIENTATION_ACTIVE (lagrange.c:3134-3186).
+ACTIVE = (abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_JEFFREY |
+          abi.LAGRANGE_ORIENTATION_ACTIVE | abi.LAGRANGE_ORIENTATION_SECONDORIENTATION)
+
+
+def _swimmer_velocity(mode, aspect=2.0, t_fields=(None,)):
+    o, cfg = quiescent(8, 8, 8, tau=0.8, flags=abi.LB_TEMPERATURE if t_fields[0] is not None else 0)
+    pc = abi.make_particle_config(ACTIVE, 1, tau_drag=(1, 0.0, 0.0), aspect_ratio=(1, aspect, aspect),
+                                  swim_velocity=(1, 0.01, 0.01), flags2=mode)
+    o.particles_init(pc)
+    a = o.particles()
+    a["x"], a["y"], a["z"] = 3.3, 4.2, 3.7
+    a["px"], a["py"], a["pz"] = 1.0, 0.0, 0.0
+    a["nx"], a["ny"], a["nz"] = 0.0, 0.0, 1.0
+    o.set_particles(a)
+    u = np.zeros((8, 8, 8, 3))
+    u[..., 1] = 0.002
+    out = []
+    for k, tf in enumerate(t_fields):
+        t = None if tf is None else np.full((8, 8, 8), tf)
+        r = o.move(u, t, first=(k == 0))[0]
+        out.append(np.array([r["vx"], r["vy"], r["vz"]]))
+    return out
+
+
+def test_swimming_modes():
+    """Default: u + s p; BALLISTIC: s p alone; SECONDORIENTATION: u + s n;
+    SELECTIVEORIENTATION: along p for rods (aspect >= 1), along n for
+    disks; TEMPERATURE: u + s p only while the particle's temperature rises
+    (dt_t > 0)."""
+    s, u = 0.01, np.array([0.0, 0.002, 0.0])
+    p, n = np.array([1.0, 0.0, 0.0]), np.array([0.0, 0.0, 1.0])
+    assert np.allclose(_swimmer_velocity(0)[0], u + s * p, rtol=1e-14, atol=1e-18)
+    assert np.allclose(_swimmer_velocity(abi.LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC)[0], s * p, rtol=1e-14,
+                       atol=1e-18)
+    assert np.allclose(_swimmer_velocity(abi.LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION)[0], u + s * n,
+                       rtol=1e-14, atol=1e-18)
+    sel = abi.LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION
+    assert np.allclose(_swimmer_velocity(sel, aspect=2.0)[0], u + s * p, rtol=1e-14, atol=1e-18)
+    assert np.allclose(_swimmer_velocity(sel, aspect=0.5)[0], u + s * n, rtol=1e-14, atol=1e-18)
+    # temperature: first sub-step keeps dt_t = 0, then warming, then cooling
+    v = _swimmer_velocity(abi.LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE, t_fields=(0.5, 0.6, 0.55))
+    assert np.allclose(v[0], u, rtol=1e-14, atol=1e-18)
+    assert np.allclose(v[1], u + s * p, rtol=1e-14, atol=1e-18)
+    assert np.allclose(v[2], u, rtol=1e-14, atol=1e-18)
