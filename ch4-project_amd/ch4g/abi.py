@@ -74,6 +74,7 @@ LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC = 1 << 9
 LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE = 1 << 10
 LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION = 1 << 11
 LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1 << 12
+LAGRANGE_BC_INELASTIC_SEDIMENT = 1 << 13
 
 FP64 = 0
 FP32_DEV = 1
@@ -199,7 +200,7 @@ PARTICLE_FIELDS = ["name", "x", "y", "z", "vx", "vy", "vz", "vx_old", "vy_old", 
                    "Dt_ux", "Dt_uy", "Dt_uz", "t", "t_old", "dt_t", "tau_drag", "beta_coeff", "dx_t", "dy_t", "dz_t",
                    "px", "py", "pz", "dt_px", "dt_py", "dt_pz", "aspect_ratio", "gyrotaxis_velocity", "swim_velocity",
                    "nx", "ny", "nz", "dt_nx", "dt_ny", "dt_nz", "t_p", "cp_p",
-                   "gravity_coeff"]
+                   "gravity_coeff", "sediment"]
 
 
 class Particle(C.Structure):

@@ -218,6 +218,7 @@ enum PartField {
     PGC,                // gravity_coeff (LAGRANGE_GRAVITY_VARIABLE)
     PCPE,               // c_perp of the spheroid drag, host libm (lagrange.c:3277-3288)
     PCPA,               // c_par
+    PSED,               // sediment (bottom-wall depositions)
     PF_COUNT
 };
 

@@ -111,8 +111,10 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
                             CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY | CH4G_LAGRANGE_BC_RADIUSANDDENSITY |
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC | CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE |
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION |
-                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION;
+                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION | CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT;
     if ((pc->flags2 & ~known2) || pc->reserved) return fail(-1, "unknown bits in flags2 / reserved");
+    if ((pc->flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT) && !lhas(*pc, CH4G_LAGRANGE_BC_INELASTIC))
+        return fail(-1, "LAGRANGE_BC_INELASTIC_SEDIMENT needs LAGRANGE_BC_INELASTIC");
     // swimming modes: members they read (typedef.h:416-430, :462-466)
     {
         const uint32_t f2 = pc->flags2;
@@ -426,6 +428,7 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         if (Q->pc.flags2 & CH4G_LAGRANGE_ORIENTATION_DRAG) spheroid_drag_constants(r.aspect_ratio, &cpe, &cpa);
         S(PCPE) = cpe;
         S(PCPA) = cpa;
+        S(PSED) = r.sediment;
         alive[(size_t)p] = 1;
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
@@ -936,6 +939,7 @@ int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t
             r.t_p = S(PTP);
             r.cp_p = S(PCP);
             r.gravity_coeff = S(PGC);
+            r.sediment = S(PSED);
         }
         k++;
     }

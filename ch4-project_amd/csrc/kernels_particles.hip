@@ -1058,6 +1058,8 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                     }
                     rewall = 1;  // lagrange.c:3839-3842
                 }
+                // LAGRANGE_BC_INELASTIC_SEDIMENT: one more deposition (lagrange.c:3845-3863)
+                if (inel && (P.flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT)) R(PSED) += 1.0;
                 if (lflag(P, CH4G_LAGRANGE_ORIENTATION_BC) && pv[1] < 0.0) pv[1] *= -1.0;
             }
             if (y >= SY - radius) {

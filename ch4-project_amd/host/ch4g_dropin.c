@@ -833,6 +833,7 @@ static int snapshot_members(const ch4g_run_params *rp, struct snap_member *m)
             SM("temperature_particle", t_p); SM("cp_particle", cp_p);
         }
     }
+    if (rp->pcfg.flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT) SM("sediment", sediment); /* lagrange.c:2106-2115 */
 #undef SM
     return n;
 }
@@ -960,6 +961,8 @@ static void write_particle_snapshot(void)
             xmf_scalar(fo, h5, "cp_particle", "cp_particle", np);
         }
     }
+    if (rp->pcfg.flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT) /* lagrange.c:2531-2540 */
+        xmf_scalar(fo, h5, "sediment", "sediment", np);
     fprintf(fo, "<Time Value=\" %e\" />\n</Grid>\n</Domain>\n</Xdmf>\n", rp->time_start + S.itime * rp->time_dt);
     fclose(fo);
 }

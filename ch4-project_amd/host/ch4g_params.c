@@ -97,6 +97,7 @@ static const struct { const char *name; unsigned flag; } LFLAGS2[] = {
     {"LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE", CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE},
     {"LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION", CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION},
     {"LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION", CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION},
+    {"LAGRANGE_BC_INELASTIC_SEDIMENT", CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -303,6 +304,8 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         if (!((out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_ACTIVE) && (out->pcfg.flags & CH4G_LAGRANGE_GRADIENT)))
             out->pcfg.flags2 &= ~modes;
     }
+    /* the deposition count sits inside #ifdef LAGRANGE_BC_INELASTIC (typedef.h:453-457) */
+    if (!(out->pcfg.flags & CH4G_LAGRANGE_BC_INELASTIC)) out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT;
     /* the Eulerian probe type sits inside #ifdef LAGRANGE_ADD_TRACER (parameters.c:744-755) */
     if (!(out->pcfg.flags & CH4G_LAGRANGE_ADD_TRACER)) out->pcfg.flags &= ~(unsigned)CH4G_LAGRANGE_ADD_TRACER_STATIC;
     /* variable gravity sits inside #ifdef LAGRANGE_GRAVITY, its log ladder

@@ -285,7 +285,8 @@ enum ch4g_lagrange_flag2 {
     CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC            = 1u << 9,   /* v = swim p, no fluid velocity */
     CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE          = 1u << 10,  /* swim only while dt_t > 0 (needs LB_TEMPERATURE) */
     CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION    = 1u << 11,  /* swim along n */
-    CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1u << 12   /* along p if aspect_ratio >= 1, else along n */
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1u << 12,  /* along p if aspect_ratio >= 1, else along n */
+    CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT = 1u << 13  /* count of bottom-wall depositions, lagrange.c:3845-3863 (needs BC_INELASTIC) */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -325,12 +326,13 @@ typedef struct ch4g_particle {
     double nx, ny, nz, dt_nx, dt_ny, dt_nz;             /* second orientation (..._SECONDORIENTATION) */
     double t_p, cp_p;                                   /* particle temperature and specific heat (LAGRANGE_TEMPERATURE) */
     double gravity_coeff;                               /* gravity factor of the particle's type (LAGRANGE_GRAVITY_VARIABLE) */
+    double sediment;                                    /* bottom-wall depositions (LAGRANGE_BC_INELASTIC_SEDIMENT) */
 } ch4g_particle;
 
 /* No counterpart in the reference.  sizeof(ch4g_particle) as this library
  * was built: callers that exchange particle records check it against their
  * own header before ch4g_get_particles / ch4g_set_particles (the record grew
- * with the orientation, particle-temperature and gravity members; a stale header
+ * with the orientation, particle-temperature, gravity and sediment members; a stale header
  * would mis-stride the arrays).                                             */
 int64_t ch4g_particle_size(void);
 

@@ -1078,6 +1078,9 @@ static void move_at(oracle *o, int first)
                     /* the particle takes the temperature of the wall it re-enters at */
                     if (LHAS(L, CH4G_LAGRANGE_TEMPERATURE)) p->t_p = o->cfg.T_top;
                 }
+                /* LAGRANGE_BC_INELASTIC_SEDIMENT, without nucleation: count the
+                 * deposition (lagrange.c:3845-3863) */
+                if (inel && (L->pc.flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT)) p->sediment += 1.0;
                 /* LAGRANGE_ORIENTATION_BC (lagrange.c:3866-3870) */
                 if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_BC) && p->py < 0.0) p->py *= -1.0;
             }

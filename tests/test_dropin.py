@@ -511,10 +511,11 @@ GVAR_PARAMS = "gravity_coeff_types 2\ngravity_coeff_min 0.5\ngravity_coeff_max 1
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(H5LIB), reason="serial HDF5 not installed")
-@pytest.mark.parametrize("orient,tpart,gvar", [(False, False, False), (True, False, False), (False, True, False),
-                                               (False, False, True)],
-                         ids=["plain", "orientation", "particle_temperature", "gravity_variable"])
-def test_particle_snapshots(tmp_path, orient, tpart, gvar):
+@pytest.mark.parametrize("orient,tpart,gvar,sed", [(False, False, False, False), (True, False, False, False),
+                                                   (False, True, False, False), (False, False, True, False),
+                                                   (False, False, False, True)],
+                         ids=["plain", "orientation", "particle_temperature", "gravity_variable", "sediment"])
+def test_particle_snapshots(tmp_path, orient, tpart, gvar, sed):
     """OUTPUT_H5 with LAGRANGE: output_particles writes RUN/particle_<itime>.h5
     every time_dump_lagr (lagrange.c:1598-2595), one FP64 dataset per member
     under /lagrange in the reference's order, equal to the oracle's records
@@ -526,7 +527,8 @@ def test_particle_snapshots(tmp_path, orient, tpart, gvar):
     import xml.etree.ElementTree as ET
     define = open(os.path.join(FIXP, "define.h")).read() + "#define OUTPUT_H5\n#define LAGRANGE_OUTPUT_FLUID\n"
     (tmp_path / "define.h").write_text(define + (ORIENT_DEFINES if orient else "") + (TPART_DEFINES if tpart else "")
-                                       + (GVAR_DEFINES if gvar else ""))
+                                       + (GVAR_DEFINES if gvar else "")
+                                       + ("#define LAGRANGE_BC_INELASTIC_SEDIMENT\n" if sed else ""))
     (tmp_path / "param.in").write_text(open(os.path.join(FIXP, "param.in")).read() + (ORIENT_PARAMS if orient else "")
                                        + (TPART_PARAMS if tpart else "") + (GVAR_PARAMS if gvar else ""))
     r = subprocess.run([RUN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
@@ -564,6 +566,8 @@ def test_particle_snapshots(tmp_path, orient, tpart, gvar):
         assert np.unique(ref["cp_p"]).size == 3  # the ladder and the tracer's 0
         rows = [l.split() for l in open(tmp_path / "particle_averages.dat")]
         assert {len(r) for r in rows} == {2 + 18 + 5 + 1}
+    if sed:
+        members += ["sediment"]
     path = tmp_path / "RUN" / "particle_120.h5"
     rename = {"temperature": "t", "temperature_particle": "t_p", "cp_particle": "cp_p"}
     for m in members:

@@ -107,13 +107,18 @@ def test_hit_les_particles_fp64():
     abi.LAGRANGE_BC_INELASTIC_REINJECT_VELOCITY_ZERO,
     abi.LAGRANGE_SMALLTAUD_BETA,
     abi.LAGRANGE_SMALLTAUD_FLOATER,
+    (abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT, abi.LAGRANGE_BC_INELASTIC_SEDIMENT),
 ], ids=["elastic", "inelastic", "reinject", "reinject_half", "reinject_norandom", "reinject_vzero",
-        "smalltaud_beta", "smalltaud_floater"])
+        "smalltaud_beta", "smalltaud_floater", "reinject_sediment"])
 def test_wall_rules_and_small_tau_fp64(extra):
     # strong settling so that particles reach the walls within the run
-    pc = abi.make_particle_config(HEAVY | extra, 256, tau_drag=(2, 2.0, 20.0), beta_coeff=(2, 0.2, 0.5))
+    extra, flags2 = extra if isinstance(extra, tuple) else (extra, 0)
+    pc = abi.make_particle_config(HEAVY | extra, 256, tau_drag=(2, 2.0, 20.0), beta_coeff=(2, 0.2, 0.5),
+                                  flags2=flags2)
     o, lat = pair(rb_small(32, 16, g=5e-3), pc, 120)
     compare_particles(o, lat)
+    if flags2:
+        assert lat.particles()["sediment"].max() >= 1
 
 
 def test_twoway_momentum_fp64():

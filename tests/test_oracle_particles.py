@@ -1048,3 +1048,21 @@ def test_swimming_modes():
     assert np.allclose(v[0], u, rtol=1e-14, atol=1e-18)
     assert np.allclose(v[1], u + s * p, rtol=1e-14, atol=1e-18)
     assert np.allclose(v[2], u, rtol=1e-14, atol=1e-18)
+
+
+def test_sediment_counts_bottom_depositions():
+    """LAGRANGE_BC_INELASTIC_SEDIMENT (no nucleation): each crossing of the
+    bottom wall adds one deposition, a crossing of the top wall none."""
+    o, cfg = quiescent(6, 8, 4, flags=abi.FLUID_BC_Y)
+    flags = abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT | \
+        abi.LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM | abi.LAGRANGE_WRAP
+    o.particles_init(abi.make_particle_config(flags, 2, tau_drag=(1, 1e9, 1e9),
+                                              flags2=abi.LAGRANGE_BC_INELASTIC_SEDIMENT))
+    a = o.particles()
+    a["x"], a["z"] = 2.0, 2.0
+    a["y"], a["vy"], a["vy_old"] = [0.05, 7.95], [-0.2, 0.2], [-0.2, 0.2]
+    o.set_particles(a)
+    u = np.zeros((4, 8, 6, 3))
+    b = o.move(u, None, first=True)
+    assert b["y"][0] > 7.0 and b["y"][1] < 1.0
+    assert list(b["sediment"]) == [1.0, 0.0]
