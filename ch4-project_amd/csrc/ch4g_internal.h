@@ -279,7 +279,15 @@ int launch_feedback(const PartParams &P, const FeedbackBufs &B, hipStream_t s);
 size_t feedback_sort_bytes(int64_t nitems, int key_bits);
 // Rows, alive flags and reference-order indices of the particles reordered
 // by cell (z, y, x of the position's node), for locality of the corner reads.
+// The rows a reordering moves: every row a kernel of the configuration
+// writes, and every row whose values differ between particles; a row that
+// is uniform and never written holds the same value in both buffers.
+struct PermRows {
+    uint8_t r[PF_COUNT];
+    int n;
+};
 struct ReorderBufs {
+    PermRows rows;
     uint32_t *kin, *kout, *vin, *vout;   // cell keys / current indices, sorted
     void *tmp;
     size_t tmp_bytes;

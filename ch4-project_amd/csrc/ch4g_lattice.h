@@ -50,6 +50,7 @@ struct Particles {
     size_t ro_tmp_bytes = 0;
     int cell_bits = 32;
     int64_t last_reorder = 0;
+    PermRows perm{};             // rows the reordering moves (set by each upload)
     void *d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int key_bits = 32;

@@ -432,6 +432,36 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         alive[(size_t)p] = 1;
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
+    if (Q->d_state2) {
+        // the rows the reordering leaves alone: uniform and never written by
+        // this configuration's kernels; both buffers hold them
+        HIPCHK(hipMemcpyAsync(Q->d_state2, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice,
+                              L->stream));
+        bool written[PF_COUNT] = {};
+        const uint32_t fl = Q->pc.flags;
+        auto mark = [&](int f, int k) {
+            for (int m = 0; m < k; m++) written[f + m] = true;
+        };
+        mark(PX, PUZO - PX + 1);  // positions, velocities, accelerations, u and u_old
+        if (fl & CH4G_LAGRANGE_GRADIENT) mark(PG, 9);
+        if (fl & CH4G_LAGRANGE_ADDEDMASS) mark(PDTU, 3);
+        if (L->base.thermal) {
+            mark(PT, 3);
+            mark(PGT, 3);
+        }
+        if (fl & CH4G_LAGRANGE_ORIENTATION) mark(PPO, 6);
+        if (fl & CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION) mark(PNO, 6);
+        if (fl & CH4G_LAGRANGE_TEMPERATURE) mark(PTP, 1);
+        if (Q->pc.flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT) mark(PSED, 1);
+        PermRows pr{};
+        for (int f = 0; f < PF_COUNT; f++) {
+            bool uniform = true;
+            const double *row = rows.data() + (size_t)f * cap;
+            for (int64_t p = 1; p < n && uniform; p++) uniform = memcmp(&row[p], &row[0], sizeof(double)) == 0;
+            if (written[f] || !uniform) pr.r[pr.n++] = (uint8_t)f;
+        }
+        Q->perm = pr;
+    }
     HIPCHK(hipMemcpyAsync(Q->d_alive, alive.data(), alive.size() * sizeof(int), hipMemcpyHostToDevice, L->stream));
     std::vector<int> orig((size_t)cap);
     for (int64_t p = 0; p < cap; p++) orig[(size_t)p] = (int)p;
@@ -505,6 +535,7 @@ int particles_before_step(ch4g_lattice *L, int itime)
         B.alive_out = Q->d_alive2;
         B.orig_out = Q->d_orig2;
         B.orig_in = Q->d_orig;
+        B.rows = Q->perm;
         if (launch_reorder(P, B, L->stream))
             return fail(-2, "particle reorder failed: %s", hipGetErrorString(hipGetLastError()));
         std::swap(Q->d_state, Q->d_state2);

@@ -1427,14 +1427,14 @@ __global__ void __launch_bounds__(PB) k_cell_keys(PartParams P, uint32_t *key, u
 
 // Row f of every particle into its new place (row-major grid: y = row).
 __global__ void __launch_bounds__(PB) k_permute(PartParams P, const uint32_t *src, double *s_out, int *alive_out,
-                                                int *orig_out)
+                                                int *orig_out, PermRows rows)
 {
     const int64_t i = (int64_t)blockIdx.x * PB + threadIdx.x;
     if (i >= P.np) return;
     const int64_t from = src[i];
-    const int f = blockIdx.y;
+    const int f = rows.r[blockIdx.y];
     s_out[(int64_t)f * P.ld + i] = P.s[(int64_t)f * P.ld + from];
-    if (f == 0) {
+    if (blockIdx.y == 0) {
         alive_out[i] = P.alive[from];
         orig_out[i] = P.orig[from];
     }
@@ -1506,8 +1506,9 @@ int launch_reorder(const PartParams &P, const ReorderBufs &B, hipStream_t s)
     if (hipcub::DeviceRadixSort::SortPairs(B.tmp, bytes, B.kin, B.kout, B.vin, B.vout, (int)P.np, 0, B.key_bits, s) !=
         hipSuccess)
         return -2;
-    hipLaunchKernelGGL(k_permute, dim3(nb, PF_COUNT), dim3(PB), 0, s, P, (const uint32_t *)B.vout, B.s_out,
-                       B.alive_out, B.orig_out);
+    if (B.rows.n < 1 || B.rows.n > PF_COUNT) return -1;
+    hipLaunchKernelGGL(k_permute, dim3(nb, B.rows.n), dim3(PB), 0, s, P, (const uint32_t *)B.vout, B.s_out,
+                       B.alive_out, B.orig_out, B.rows);
     return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
