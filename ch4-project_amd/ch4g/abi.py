@@ -75,6 +75,7 @@ LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE = 1 << 10
 LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION = 1 << 11
 LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1 << 12
 LAGRANGE_BC_INELASTIC_SEDIMENT = 1 << 13
+LAGRANGE_ORIENTATION_RANDOM = 1 << 14
 
 FP64 = 0
 FP32_DEV = 1

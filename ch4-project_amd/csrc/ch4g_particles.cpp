@@ -29,6 +29,14 @@ bool random_reinjection(const ch4g_lattice *L, const ch4g_particle_config &pc)
            !lhas(pc, CH4G_LAGRANGE_BC_INELASTIC_REINJECT_NORANDOM);
 }
 
+// The sub-steps draw from the lattice's drand48 stream (random re-injection,
+// random reorientation): the draws are taken on the device in particle order
+// and the rows stay in reference order.
+bool device_draws(const ch4g_lattice *L, const ch4g_particle_config &pc)
+{
+    return random_reinjection(L, pc) || (pc.flags2 & CH4G_LAGRANGE_ORIENTATION_RANDOM) != 0;
+}
+
 // Switch combinations whose reference result is defined and restated.
 int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
 {
@@ -111,8 +119,11 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
                             CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_DENSITY | CH4G_LAGRANGE_BC_RADIUSANDDENSITY |
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC | CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE |
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION |
-                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION | CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT;
+                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION | CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT |
+                            CH4G_LAGRANGE_ORIENTATION_RANDOM;
     if ((pc->flags2 & ~known2) || pc->reserved) return fail(-1, "unknown bits in flags2 / reserved");
+    if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_RANDOM) && !lhas(*pc, CH4G_LAGRANGE_ORIENTATION))
+        return fail(-1, "LAGRANGE_ORIENTATION_RANDOM needs LAGRANGE_ORIENTATION");
     if ((pc->flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT) && !lhas(*pc, CH4G_LAGRANGE_BC_INELASTIC))
         return fail(-1, "LAGRANGE_BC_INELASTIC_SEDIMENT needs LAGRANGE_BC_INELASTIC");
     // swimming modes: members they read (typedef.h:416-430, :462-466)
@@ -276,7 +287,7 @@ int alloc_state(ch4g_lattice *L, int64_t cap)
     if ((rc = dalloc(L, (void **)&P->d_alive, (size_t)c * sizeof(int)))) return rc;
     if ((rc = dalloc(L, (void **)&P->d_ndraw, (size_t)c * sizeof(int)))) return rc;
     if ((rc = dalloc(L, (void **)&P->d_orig, (size_t)c * sizeof(int)))) return rc;
-    if (random_reinjection(L, P->pc)) {
+    if (device_draws(L, P->pc)) {
         if ((rc = dalloc(L, (void **)&P->d_bsum, (size_t)((c + 255) / 256) * sizeof(int)))) return rc;
     } else {
         // rows reordered by cell (not with random re-injection, whose draws
@@ -853,7 +864,7 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         L->base.twoway = 1;
         L->base.forced = 1; // the feedback enters through the force field
     }
-    if (random_reinjection(L, *pc)) {
+    if (device_draws(L, *pc)) {
         if ((rc = dalloc(L, (void **)&Q->d_rng, 2 * sizeof(uint64_t)))) goto err;
         Q->rng_on_device = 1;
         if ((rc = rng_to_device(L))) goto err;

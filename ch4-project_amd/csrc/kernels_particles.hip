@@ -630,7 +630,7 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
     // rows are in cell order: consecutive blocks on one XCD share corner lines
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);  // the block of particles this block moves
     const int64_t p = (int64_t)lb * PB + threadIdx.x;
-    int nd = 0;
+    int nd = 0, pflags = 0;  // draws of the sub-step; 1: random p, 2 / 4: p_y turned at the bottom / top wall
     if (p < P.np && P.alive[p]) {
         double *S = P.s;
         const int64_t ld = P.ld;
@@ -725,6 +725,7 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
         }
         const bool orient = lflag(P, CH4G_LAGRANGE_ORIENTATION);
         const bool second = orient && lflag(P, CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION);
+        const bool rnd = orient && (P.flags2 & CH4G_LAGRANGE_ORIENTATION_RANDOM);
         double pv[3] = {0.0, 0.0, 0.0}, fold[3] = {0.0, 0.0, 0.0};
         double nv[3] = {0.0, 0.0, 0.0}, nfold[3] = {0.0, 0.0, 0.0};
         if (orient) {
@@ -1014,6 +1015,13 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                 for (int i = 0; i < 3; i++) nn += nv[i] * nv[i];
                 for (int i = 0; i < 3; i++) nv[i] /= sqrt(nn);
             }
+            if (rnd) {
+                // LAGRANGE_ORIENTATION_RANDOM (lagrange.c:3698-3713): p is drawn
+                // anew from the particle's first two draws of the sub-step, in
+                // the draw pass; the row keeps the old p that dp/dt is taken from
+                nd += 2;
+                pflags |= 1;
+            }
         }
 
         // ---- wall rules at y (lagrange.c:3754-3993); x and z of a random
@@ -1060,7 +1068,12 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                 }
                 // LAGRANGE_BC_INELASTIC_SEDIMENT: one more deposition (lagrange.c:3845-3863)
                 if (inel && (P.flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT)) R(PSED) += 1.0;
-                if (lflag(P, CH4G_LAGRANGE_ORIENTATION_BC) && pv[1] < 0.0) pv[1] *= -1.0;
+                if (lflag(P, CH4G_LAGRANGE_ORIENTATION_BC)) {
+                    if (rnd)
+                        pflags |= 2;  // applied to the drawn p by the draw pass
+                    else if (pv[1] < 0.0)
+                        pv[1] *= -1.0;
+                }
             }
             if (y >= SY - radius) {
                 if (!reinj) {
@@ -1083,7 +1096,12 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                     }
                     rewall = 2;  // lagrange.c:3915-3918
                 }
-                if (lflag(P, CH4G_LAGRANGE_ORIENTATION_BC) && pv[1] > 0.0) pv[1] *= -1.0;
+                if (lflag(P, CH4G_LAGRANGE_ORIENTATION_BC)) {
+                    if (rnd)
+                        pflags |= 4;
+                    else if (pv[1] > 0.0)
+                        pv[1] *= -1.0;
+                }
             }
         }
         if (nd == 0) wrap_keep(P, x, y, z, p);
@@ -1131,7 +1149,7 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
             R(PDTU + 1) = dtu[1];
             R(PDTU + 2) = dtu[2];
         }
-        if (orient) {
+        if (orient && !rnd) {
             for (int i = 0; i < 3; i++) {
                 R(PPO + i) = pv[i];
                 R(PDTP + i) = fold[i];
@@ -1157,7 +1175,7 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
     }
     if (P.bsum) {
         // draws of this block, for the ordered re-injection pass
-        if (p < P.np) P.ndraw[p] = nd;
+        if (p < P.np) P.ndraw[p] = nd | (pflags << 16);
         __shared__ int red[PB / 64];
         int v = nd;
         for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -1202,12 +1220,16 @@ __global__ void __launch_bounds__(1024) k_draw_offsets(int *bsum, int nb, uint64
     }
 }
 
-// Re-injected particles take x = SX r, z = SZ r from their draws (the last
-// pair wins when a particle crosses both walls), then sendrecv's wrap.
+// The draws of each particle in order: with LAGRANGE_ORIENTATION_RANDOM the
+// first two give p = random_vector (random.c:139-150) and dp/dt = (p - p_old)
+// / dt, then p is normalised and turned at the walls as recorded; a
+// re-injected particle takes x = SX r, z = SZ r from its last pair (the last
+// wall crossed wins); then sendrecv's wrap.
 __global__ void __launch_bounds__(PB) k_draw_apply(PartParams P)
 {
     const int64_t p = (int64_t)blockIdx.x * PB + threadIdx.x;
-    const int nd = p < P.np ? P.ndraw[p] : 0;
+    const int raw = p < P.np ? P.ndraw[p] : 0;
+    const int nd = raw & 0xFFFF, pf = raw >> 16;
     __shared__ int sc[PB];
     sc[threadIdx.x] = nd;
     __syncthreads();
@@ -1220,10 +1242,30 @@ __global__ void __launch_bounds__(PB) k_draw_apply(PartParams P)
     if (nd == 0) return;
     const uint64_t off = (uint64_t)P.bsum[blockIdx.x] + (uint64_t)(sc[threadIdx.x] - nd);
     const uint64_t base = P.rng[1];
-    const double rx = lcg_value(lcg_jump(base, off + nd - 1));
-    const double rz = lcg_value(lcg_jump(base, off + nd));
     double *S = P.s;
-    double x = P.NX * rx, z = P.NZ * rz, y = S[(int64_t)PY * P.ld + p];
+    const int64_t ld = P.ld;
+    if (pf & 1) {
+        const double theta = acos(2.0 * lcg_value(lcg_jump(base, off + 1)) - 1.0);
+        const double phi = 2 * 3.14159265359 * lcg_value(lcg_jump(base, off + 2));
+        double st, ct, sp, cp;
+        sincos(theta, &st, &ct);
+        sincos(phi, &sp, &cp);
+        double v[3] = {st * sp, st * cp, ct};
+        double norm = 0.0;
+        for (int i = 0; i < 3; i++) {
+            S[(PDTP + i) * ld + p] = (v[i] - S[(PPO + i) * ld + p]) / P.dt;
+            norm += v[i] * v[i];
+        }
+        for (int i = 0; i < 3; i++) v[i] /= sqrt(norm);
+        if ((pf & 2) && v[1] < 0.0) v[1] *= -1.0;
+        if ((pf & 4) && v[1] > 0.0) v[1] *= -1.0;
+        for (int i = 0; i < 3; i++) S[(PPO + i) * ld + p] = v[i];
+    }
+    double x = S[(int64_t)PX * ld + p], y = S[(int64_t)PY * ld + p], z = S[(int64_t)PZ * ld + p];
+    if (nd > ((pf & 1) ? 2 : 0)) {
+        x = P.NX * lcg_value(lcg_jump(base, off + nd - 1));
+        z = P.NZ * lcg_value(lcg_jump(base, off + nd));
+    }
     wrap_keep(P, x, y, z, p);
     S[(int64_t)PX * P.ld + p] = x;
     S[(int64_t)PY * P.ld + p] = y;

@@ -98,6 +98,7 @@ static const struct { const char *name; unsigned flag; } LFLAGS2[] = {
     {"LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION", CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION},
     {"LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION", CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION},
     {"LAGRANGE_BC_INELASTIC_SEDIMENT", CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT},
+    {"LAGRANGE_ORIENTATION_RANDOM", CH4G_LAGRANGE_ORIENTATION_RANDOM},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -304,6 +305,9 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         if (!((out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_ACTIVE) && (out->pcfg.flags & CH4G_LAGRANGE_GRADIENT)))
             out->pcfg.flags2 &= ~modes;
     }
+    /* random reorientation sits inside #ifdef LAGRANGE_ORIENTATION (lagrange.c:3698) */
+    if (!((out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION) && (out->pcfg.flags & CH4G_LAGRANGE_GRADIENT)))
+        out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_ORIENTATION_RANDOM;
     /* the deposition count sits inside #ifdef LAGRANGE_BC_INELASTIC (typedef.h:453-457) */
     if (!(out->pcfg.flags & CH4G_LAGRANGE_BC_INELASTIC)) out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT;
     /* the Eulerian probe type sits inside #ifdef LAGRANGE_ADD_TRACER (parameters.c:744-755) */

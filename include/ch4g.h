@@ -286,7 +286,9 @@ enum ch4g_lagrange_flag2 {
     CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE          = 1u << 10,  /* swim only while dt_t > 0 (needs LB_TEMPERATURE) */
     CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION    = 1u << 11,  /* swim along n */
     CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1u << 12,  /* along p if aspect_ratio >= 1, else along n */
-    CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT = 1u << 13  /* count of bottom-wall depositions, lagrange.c:3845-3863 (needs BC_INELASTIC) */
+    CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT = 1u << 13, /* count of bottom-wall depositions, lagrange.c:3845-3863 (needs BC_INELASTIC) */
+    CH4G_LAGRANGE_ORIENTATION_RANDOM    = 1u << 14  /* p drawn anew (random_vector) every sub-step, lagrange.c:3698-3713;
+                                                       the device's acos / sin / cos: parity to their last bits */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */

@@ -719,7 +719,7 @@ static void interpolate(oracle *o) { interpolate_fields(o, 1, 1); }
  *   dp/dt = W p + f S p - (p . f S p) p,   f = (alpha^2 - 1) / (alpha^2 + 1),
  * S and W the symmetric and antisymmetric parts of A[i][j] = d u_i / d x_j,
  * Euler on the first step, Adams-Bashforth 2 after; p renormalised. */
-static void orient_step(const olagr *L, ch4g_particle *p, int first, double gx, double gy, double gz)
+static void orient_step(oracle *o, const olagr *L, ch4g_particle *p, int first, double gx, double gy, double gz)
 {
     double P[3] = {p->px, p->py, p->pz};
     double Fold[3] = {p->dt_px, p->dt_py, p->dt_pz};
@@ -786,6 +786,18 @@ static void orient_step(const olagr *L, ch4g_particle *p, int first, double gx, 
                 Nv[i] = first ? Nv[i] + L->dt * FN[i] : Nv[i] + 0.5 * L->dt * (3. * FN[i] - FNold[i]);
                 FNold[i] = FN[i];
             }
+    }
+    if (L->pc.flags2 & CH4G_LAGRANGE_ORIENTATION_RANDOM) {
+        /* lagrange.c:3698-3713: p replaced by random_vector (random.c:139-150),
+         * dp/dt from the stored p */
+        double theta = acos(2.0 * oracle_drand48(o) - 1.0);
+        double phi = 2 * 3.14159265359 * oracle_drand48(o);
+        P[0] = sin(theta) * sin(phi);
+        P[1] = sin(theta) * cos(phi);
+        P[2] = cos(theta);
+        Fold[0] = (P[0] - p->px) / L->dt;
+        Fold[1] = (P[1] - p->py) / L->dt;
+        Fold[2] = (P[2] - p->pz) / L->dt;
     }
     double norm = 0.0;
     for (int i = 0; i < 3; i++) norm += P[i] * P[i];
@@ -1031,7 +1043,7 @@ static void move_at(oracle *o, int first)
                 p->vz += f1 * (-gz - p->Dt_uz);
             }
         }
-        if (LHAS(L, CH4G_LAGRANGE_ORIENTATION)) orient_step(L, p, first, gx, gy, gz);
+        if (LHAS(L, CH4G_LAGRANGE_ORIENTATION)) orient_step(o, L, p, first, gx, gy, gz);
         /* wall rules (lagrange.c:3754-3993), LB_FLUID_BC_Y only */
         if (HAS(o, CH4G_FLUID_BC_Y)) {
             const int inel = LHAS(L, CH4G_LAGRANGE_BC_INELASTIC);

@@ -176,6 +176,26 @@ def test_swimming_modes_fp64(mode):
     compare_particles(o, lat)
 
 
+def test_random_reorientation_fp64():
+    """LAGRANGE_ORIENTATION_RANDOM with random re-injection and the wall turn
+    of p_y: each particle's draws in order (two for p, two per re-injection)
+    from the device's copy of the stream; p from the device's acos / sincos,
+    so p, dp/dt and the motion within 1e-9 of the oracle after 60 steps, and
+    the re-injection positions exact."""
+    flags = (HEAVY | abi.LAGRANGE_ORIENTATION | abi.LAGRANGE_ORIENTATION_JEFFREY | abi.LAGRANGE_ORIENTATION_BC |
+             abi.LAGRANGE_BC_INELASTIC | abi.LAGRANGE_BC_INELASTIC_REINJECT)
+    pc = abi.make_particle_config(flags, 256, tau_drag=(2, 2.0, 20.0), beta_coeff=(2, 0.2, 0.5),
+                                  aspect_ratio=(2, 0.5, 4.0), flags2=abi.LAGRANGE_ORIENTATION_RANDOM)
+    o, lat = pair(rb_small(32, 16, g=5e-3), pc, 60)
+    a, b = o.particles(), lat.particles()
+    assert len(a) == len(b) and np.array_equal(a["name"], b["name"])
+    for f in ("px", "py", "pz", "dt_px", "dt_py", "dt_pz"):
+        err = np.abs(a[f] - b[f]).max()
+        print(f"  {f}: max |gpu - oracle| = {err:.3e}")
+        assert err <= 1e-9, f
+    compare_particles(o, lat, exact=False, pos_atol=1e-9, rel=1e-9)
+
+
 def test_orientation_needs_the_gradient():
     lat = Lattice(rb_small())
     lat.initial_conditions()

@@ -1066,3 +1066,37 @@ def test_sediment_counts_bottom_depositions():
     b = o.move(u, None, first=True)
     assert b["y"][0] > 7.0 and b["y"][1] < 1.0
     assert list(b["sediment"]) == [1.0, 0.0]
+
+
+# --------------------------------------------------------------------------
+# Random reorientation (LAGRANGE_ORIENTATION_RANDOM, lagrange.c:3698-3713).
+def test_random_reorientation_draws_and_isotropy():
+    """Every sub-step each particle's p is a new random_vector from the next
+    two draws of the stream, in particle order; dp/dt is the change from
+    the previous p over the sub-step; the directions are uniform on the
+    sphere (mean p_z^2 = 1/3, which theta uniform in [0, pi] would make
+    1/2)."""
+    seed = 21
+    o, cfg = quiescent(8, 8, 8, seed=seed)
+    flags = abi.LAGRANGE_GRADIENT | abi.LAGRANGE_ORIENTATION
+    n = 5
+    o.particles_init(abi.make_particle_config(flags, n, tau_drag=(1, 2.0, 2.0),
+                                              flags2=abi.LAGRANGE_ORIENTATION_RANDOM))
+    a0 = o.particles()
+    o.step(1)
+    a = o.particles()
+    r = glibc_stream(seed, 5 * n + 2 * n)[5 * n:].reshape(n, 2)
+    theta, phi = np.arccos(2.0 * r[:, 0] - 1.0), 2 * 3.14159265359 * r[:, 1]
+    v = np.column_stack([np.sin(theta) * np.sin(phi), np.sin(theta) * np.cos(phi), np.cos(theta)])
+    p = np.column_stack([a["px"], a["py"], a["pz"]])
+    assert np.allclose(p, v, rtol=0, atol=1e-15)
+    dp = np.column_stack([a["dt_px"], a["dt_py"], a["dt_pz"]])
+    p0 = np.column_stack([a0["px"], a0["py"], a0["pz"]])
+    assert np.allclose(dp, v - p0, rtol=0, atol=1e-14)
+    o2, _ = quiescent(16, 16, 16, seed=3)
+    o2.particles_init(abi.make_particle_config(flags, 20000, tau_drag=(1, 2.0, 2.0),
+                                               flags2=abi.LAGRANGE_ORIENTATION_RANDOM))
+    o2.step(1)
+    b = o2.particles()
+    assert abs(np.mean(b["pz"] ** 2) - 1.0 / 3.0) < 0.01
+    assert abs(np.mean(b["px"])) < 0.02 and abs(np.mean(b["py"])) < 0.02
