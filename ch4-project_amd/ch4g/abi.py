@@ -208,6 +208,12 @@ class Particle(C.Structure):
     _fields_ = [(n, C.c_double) for n in PARTICLE_FIELDS]
 
 
+class ParticleType(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("count", "particle_radius", "particle_density", "tau_drag", "beta_coeff",
+                                          "gravity_coeff", "cp_coeff", "aspect_ratio", "gyrotaxis_velocity",
+                                          "swim_velocity")]
+
+
 def particle_dtype():
     """numpy structured dtype with the layout of struct ch4g_particle."""
     import numpy as np

@@ -24,6 +24,7 @@ EXPORTS = [
     "ch4g_get_hit_phases", "ch4g_set_hit_phases", "ch4g_dump_rulers", "ch4g_step_timed", "ch4g_steps_per_launch", "ch4g_set_graphs", "ch4g_set_fusion", "ch4g_synchronize", "ch4g_itime", "ch4g_device_bytes",
     "ch4g_footprint_bytes", "ch4g_particles_init", "ch4g_particles_count", "ch4g_get_particles",
     "ch4g_set_particles", "ch4g_step_batch", "ch4g_serve", "ch4g_dump_averages_noalloc", "ch4g_particle_size",
+    "ch4g_particle_types",
 ]
 
 
@@ -70,6 +71,7 @@ def lib():
     L.ch4g_particles_init.argtypes = [P, C.POINTER(ParticleConfig)]
     L.ch4g_particles_count.argtypes = [P, C.POINTER(C.c_int64)]
     L.ch4g_get_particles.argtypes = [P, P, C.c_int64, C.POINTER(C.c_int64)]
+    L.ch4g_particle_types.argtypes = [P, P, C.c_int, C.POINTER(C.c_int)]
     L.ch4g_set_particles.argtypes = [P, P, C.c_int64]
     L.ch4g_step_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
     L.ch4g_serve.argtypes = [C.POINTER(ServeParams), C.POINTER(Job), C.c_int32]
@@ -252,6 +254,15 @@ class Lattice:
         got = C.c_int64()
         check(self._L.ch4g_get_particles(self._h, a.ctypes.data if n else None, n, C.byref(got)))
         return a[:got.value]
+
+    def particle_types(self):
+        """Per-type values of the last particles_init (particle_properties.dat)."""
+        from .abi import ParticleType
+        n = C.c_int()
+        check(self._L.ch4g_particle_types(self._h, None, 0, C.byref(n)))
+        out = (ParticleType * max(n.value, 1))()
+        check(self._L.ch4g_particle_types(self._h, out, n.value, C.byref(n)))
+        return [{f: getattr(out[i], f) for f, _ in ParticleType._fields_} for i in range(n.value)]
 
     def set_particles(self, a):
         a = np.ascontiguousarray(a, dtype=particle_dtype())

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "ch4g_internal.h"
 
 namespace ch4g {
@@ -51,6 +53,7 @@ struct Particles {
     int cell_bits = 32;
     int64_t last_reorder = 0;
     PermRows perm{};             // rows the reordering moves (set by each upload)
+    std::vector<ch4g_particle_type> types;  // per-type values of the last initialisation
     void *d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int key_bits = 32;

@@ -697,11 +697,13 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     std::vector<double> tau((size_t)nt, 0.0), beta((size_t)nt, 0.0), asp((size_t)nt, 0.0), gyv((size_t)nt, 0.0),
         swv((size_t)nt, 0.0), cpv((size_t)nt, 0.0), gcv((size_t)nt, 0.0);
     double rep = 1.0;
+    std::vector<double> rv, dv;  // radius and density ladders (LAGRANGE_RADIUSandDENSITY)
     if (rad) {
         // lagrange.c:150-260: radius and density ladders (tracers 0 and 1),
         // tau = (2/9) rho_p r^2 / nu, or with added mass beta = 3 / (1 + 2 rho_p)
         // and tau = r^2 / (3 beta nu) (rho_f = 1)
-        std::vector<double> rv((size_t)nt, 0.0), dv((size_t)nt, 0.0);
+        rv.assign((size_t)nt, 0.0);
+        dv.assign((size_t)nt, 0.0);
         ladder(rv, ntr, 0.0, pc->particle_radius_types, pc->particle_radius_min, pc->particle_radius_max,
                (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY_INCREMENT_LOG_RADIUS) != 0, rep);
         rep *= pc->particle_radius_types;
@@ -749,6 +751,24 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
                lhas(*pc, CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG), rep);
         rep *= pc->cp_coeff_types;
     }
+
+    // the per-type table of particle_properties.dat (lagrange.c:644-700)
+    Q->types.assign((size_t)nt, ch4g_particle_type{});
+    for (int t = 0; t < nt; t++) {
+        ch4g_particle_type &y = Q->types[(size_t)t];
+        if (rad) {
+            y.particle_radius = rv[(size_t)t];
+            y.particle_density = dv[(size_t)t];
+        }
+        y.tau_drag = tau[(size_t)t];
+        y.beta_coeff = beta[(size_t)t];
+        y.gravity_coeff = gcv[(size_t)t];
+        y.cp_coeff = cpv[(size_t)t];
+        y.aspect_ratio = asp[(size_t)t];
+        y.gyrotaxis_velocity = gyv[(size_t)t];
+        y.swim_velocity = swv[(size_t)t];
+    }
+    for (int64_t i = 0; i < pc->particle_number; i++) Q->types[(size_t)(i % nt)].count += 1.0;
 
     // names, properties and positions (lagrange.c:634-641, :711-771): three
     // draws per particle from the lattice's stream, then with orientation
@@ -893,6 +913,16 @@ err:
 }
 
 int64_t ch4g_particle_size(void) { return (int64_t)sizeof(ch4g_particle); }
+
+int ch4g_particle_types(ch4g_lattice *L, ch4g_particle_type *out, int cap, int *ntypes)
+{
+    if (!L || !ntypes || (cap > 0 && !out)) return fail(-1, "null argument");
+    if (!L->part) return fail(-1, "ch4g_particles_init first");
+    const int nt = (int)L->part->types.size();
+    for (int t = 0; t < nt && t < cap; t++) out[t] = L->part->types[(size_t)t];
+    *ntypes = nt;
+    return 0;
+}
 
 int ch4g_particles_count(ch4g_lattice *L, int64_t *n)
 {

@@ -283,6 +283,44 @@ void allocate_particles(void) { need_particles("allocate_particles"); }
 /* initial_conditions_particles (lagrange.c:72-1021): types, positions from
  * the lattice's drand48 stream; with restart the records then come from
  * part.h5 (read_point_particle_h5, lagrange.c:128-131).                    */
+/* particle_properties.dat (lagrange.c:644-700): one line per type with its
+ * count and the property values the switches define.                      */
+static void write_particle_properties(void)
+{
+    int nt = 0;
+    if (ch4g_particle_types(S.lat, NULL, 0, &nt)) die("initial_conditions_particles");
+    ch4g_particle_type *y = (ch4g_particle_type *)calloc((size_t)(nt > 0 ? nt : 1), sizeof *y);
+    if (!y) die("initial_conditions_particles: out of memory");
+    if (ch4g_particle_types(S.lat, y, nt, &nt)) die("initial_conditions_particles");
+    const unsigned fl = S.rp.pcfg.flags, f2 = S.rp.pcfg.flags2;
+    const int grad = (fl & CH4G_LAGRANGE_GRADIENT) != 0;
+    FILE *fo = fopen("particle_properties.dat", "w");
+    if (fo) {
+        for (int i = 0; i < nt; i++) {
+            fprintf(fo, "type %d counter_all %d ", i, (int)y[i].count);
+            if (f2 & CH4G_LAGRANGE_RADIUSANDDENSITY) {
+                fprintf(fo, "particle_radius %e ", y[i].particle_radius);
+                fprintf(fo, "particle_density %e ", y[i].particle_density);
+            }
+            fprintf(fo, "tau_drag %e ", y[i].tau_drag);
+            if (grad && (fl & CH4G_LAGRANGE_ADDEDMASS)) fprintf(fo, "beta_coeff %e ", y[i].beta_coeff);
+            if (fl & CH4G_LAGRANGE_GRAVITY_VARIABLE) fprintf(fo, "gravity_coeff %e ", y[i].gravity_coeff);
+            if (fl & CH4G_LAGRANGE_TEMPERATURE) fprintf(fo, "cp_coeff %e ", y[i].cp_coeff);
+            if (grad && (fl & CH4G_LAGRANGE_ORIENTATION)) {
+                if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY) {
+                    fprintf(fo, "aspect_ratio %e ", y[i].aspect_ratio);
+                    if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS)
+                        fprintf(fo, "gyrotaxis_velocity %e ", y[i].gyrotaxis_velocity);
+                }
+                if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) fprintf(fo, "swim_velocity %e ", y[i].swim_velocity);
+            }
+            fprintf(fo, "\n");
+        }
+        fclose(fo);
+    }
+    free(y);
+}
+
 void initial_conditions_particles(int restart)
 {
     need_particles("initial_conditions_particles");
@@ -290,9 +328,24 @@ void initial_conditions_particles(int restart)
         fprintf(stderr, "Error message -> initial_conditions_particles after the first step. Exit.\n");
         exit(-1);
     }
+    /* lagrange.c:123-134: with restart and part.h5 present the records come
+     * from the file and none of the initial draws are taken; otherwise the
+     * full initialisation */
+    FILE *fin = restart ? fopen("part.h5", "r") : NULL;
+    if (fin) {
+        fclose(fin);
+        ch4g_particle_config pc = S.rp.pcfg;
+        pc.particle_number = 0;
+        pc.flags2 &= ~(uint32_t)CH4G_LAGRANGE_INITIAL_GRID;
+        if (ch4g_particles_init(S.lat, &pc)) die("initial_conditions_particles");
+        S.particles = 1;
+        read_point_particle_h5();
+        fprintf(stderr, "The part.h5 file is present!\n Particles will be initialized from file.\n");
+        return;
+    }
     if (ch4g_particles_init(S.lat, &S.rp.pcfg)) die("initial_conditions_particles");
     S.particles = 1;
-    if (restart) read_point_particle_h5();
+    write_particle_properties();
 }
 
 /* from the second step on, between build_forcing and add_forcing

@@ -355,6 +355,19 @@ int64_t ch4g_particle_size(void);
  * first step.                                                               */
 int ch4g_particles_init(ch4g_lattice *lat, const ch4g_particle_config *pcfg);
 
+/* No counterpart in the reference, whose initial_conditions_particles keeps
+ * its per-type tables local and prints them to particle_properties.dat
+ * (lagrange.c:644-700): the values of each particle type (name % types) of
+ * the last ch4g_particles_init, for writing that file.  *ntypes receives the
+ * number of types; at most `cap` entries are written.                      */
+typedef struct ch4g_particle_type {
+    double count;                               /* particles of the type */
+    double particle_radius, particle_density;   /* LAGRANGE_RADIUSandDENSITY */
+    double tau_drag, beta_coeff, gravity_coeff, cp_coeff;
+    double aspect_ratio, gyrotaxis_velocity, swim_velocity;
+} ch4g_particle_type;
+int ch4g_particle_types(ch4g_lattice *lat, ch4g_particle_type *out, int cap, int *ntypes);
+
 /* Number of particles alive (sendrecv_particles drops a particle whose
  * wrapped position rounds onto the upper box face, lagrange.c:4090-4116).
  * Synchronous.                                                              */

@@ -330,6 +330,16 @@ def test_ch4g_run_particles_matches_oracle(tmp_path, fluid_avg):
     o = Oracle(rp.cfg)
     o.initial_conditions()
     o.particles_init(rp.pcfg)
+    # particle_properties.dat (lagrange.c:644-700): count, tau_drag, beta_coeff per type
+    a0 = o.particles()
+    props = [l.split() for l in open(tmp_path / "particle_properties.dat")]
+    assert len(props) == ntypes
+    for t, row in enumerate(props):
+        sel = a0[a0["name"].astype(int) % ntypes == t]
+        assert row[:4] == ["type", str(t), "counter_all", str(len(sel))]
+        assert row[4] == "tau_drag" and row[6] == "beta_coeff"
+        assert float(row[5]) == pytest.approx(sel["tau_drag"][0], rel=1e-6)
+        assert float(row[7]) == pytest.approx(sel["beta_coeff"][0], rel=1e-6)
     for k in range(4):
         if k:
             o.step(40)
