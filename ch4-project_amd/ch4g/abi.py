@@ -76,6 +76,7 @@ LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION = 1 << 11
 LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1 << 12
 LAGRANGE_BC_INELASTIC_SEDIMENT = 1 << 13
 LAGRANGE_ORIENTATION_RANDOM = 1 << 14
+LAGRANGE_POLYMER = 1 << 15
 
 FP64 = 0
 FP32_DEV = 1
@@ -167,13 +168,14 @@ class ParticleConfig(C.Structure):
         ("particle_radius_max", C.c_double),
         ("particle_density_types", C.c_double), ("particle_density_min", C.c_double),
         ("particle_density_max", C.c_double),
+        ("tau_polymer", C.c_double),
     ]
 
 
 def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_coeff=(1, 1.0, 1.0),
                          time_dt_lagr_factor=1.0, aspect_ratio=(1, 1.0, 1.0), gyrotaxis_velocity=(1, 0.0, 0.0),
                          swim_velocity=(1, 0.0, 0.0), cp_coeff=(1, 1.0, 1.0), gravity_coeff=(1, 1.0, 1.0),
-                         flags2=0, particle_radius=(1, 0.0, 0.0), particle_density=(1, 1.0, 1.0)):
+                         flags2=0, particle_radius=(1, 0.0, 0.0), particle_density=(1, 1.0, 1.0), tau_polymer=1.0):
     """ParticleConfig from (types, min, max) ladders as param.in gives them."""
     pc = ParticleConfig()
     pc.flags = flags
@@ -192,6 +194,7 @@ def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_co
      pc.particle_radius_max) = (float(v) for v in particle_radius)
     (pc.particle_density_types, pc.particle_density_min,
      pc.particle_density_max) = (float(v) for v in particle_density)
+    pc.tau_polymer = float(tau_polymer)
     return pc
 
 
@@ -201,7 +204,8 @@ PARTICLE_FIELDS = ["name", "x", "y", "z", "vx", "vy", "vz", "vx_old", "vy_old", 
                    "Dt_ux", "Dt_uy", "Dt_uz", "t", "t_old", "dt_t", "tau_drag", "beta_coeff", "dx_t", "dy_t", "dz_t",
                    "px", "py", "pz", "dt_px", "dt_py", "dt_pz", "aspect_ratio", "gyrotaxis_velocity", "swim_velocity",
                    "nx", "ny", "nz", "dt_nx", "dt_ny", "dt_nz", "t_p", "cp_p",
-                   "gravity_coeff", "sediment"]
+                   "gravity_coeff", "sediment", "cxx", "cyy", "czz", "cxy", "cyz", "cxz",
+                   "dt_cxx", "dt_cyy", "dt_czz", "dt_cxy", "dt_cyz", "dt_cxz"]
 
 
 class Particle(C.Structure):

@@ -219,7 +219,9 @@ enum PartField {
     PCPE,               // c_perp of the spheroid drag, host libm (lagrange.c:3277-3288)
     PCPA,               // c_par
     PSED,               // sediment (bottom-wall depositions)
-    PF_COUNT
+    PCT,                // 6 rows: cxx, cyy, czz, cxy, cyz, cxz (polymer conformation)
+    PDCT = PCT + 6,     // 6 rows: dt_cxx .. dt_cxz
+    PF_COUNT = PDCT + 6
 };
 
 struct PartParams {
@@ -240,6 +242,7 @@ struct PartParams {
     int fluid, thermal, wall_f, wall_t;
     double Tw_bot, Tw_top;   // T_wall - T_ref
     double T_bot, T_top;     // the walls' temperatures a re-injected particle takes (LAGRANGE_TEMPERATURE)
+    double inv_tau_polymer;  // 1 / tau_polymer (LAGRANGE_POLYMER)
     int first;               // itime == 1 (Euler start)
     double dt;               // time_dt_lagr
     double grav[3];

@@ -120,8 +120,11 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC | CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE |
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION |
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION | CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT |
-                            CH4G_LAGRANGE_ORIENTATION_RANDOM;
+                            CH4G_LAGRANGE_ORIENTATION_RANDOM | CH4G_LAGRANGE_POLYMER;
     if ((pc->flags2 & ~known2) || pc->reserved) return fail(-1, "unknown bits in flags2 / reserved");
+    if ((pc->flags2 & CH4G_LAGRANGE_POLYMER) && !lhas(*pc, CH4G_LAGRANGE_GRADIENT))
+        return fail(-1, "LAGRANGE_POLYMER needs LAGRANGE_GRADIENT");
+    if ((pc->flags2 & CH4G_LAGRANGE_POLYMER) && !(pc->tau_polymer > 0.0)) return fail(-1, "tau_polymer must be > 0");
     if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_RANDOM) && !lhas(*pc, CH4G_LAGRANGE_ORIENTATION))
         return fail(-1, "LAGRANGE_ORIENTATION_RANDOM needs LAGRANGE_ORIENTATION");
     if ((pc->flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT) && !lhas(*pc, CH4G_LAGRANGE_BC_INELASTIC))
@@ -358,6 +361,7 @@ PartParams part_params(ch4g_lattice *L)
     P.Tw_top = L->base.Tw_top;
     P.T_bot = L->cfg.T_bot;
     P.T_top = L->cfg.T_top;
+    P.inv_tau_polymer = 1. / Q->pc.tau_polymer;
     P.first = L->itime == 1;
     P.dt = Q->dt;
     P.grav[0] = L->cfg.gravity_x;
@@ -440,6 +444,9 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         S(PCPE) = cpe;
         S(PCPA) = cpa;
         S(PSED) = r.sediment;
+        const double ct[12] = {r.cxx, r.cyy, r.czz, r.cxy, r.cyz, r.cxz,
+                               r.dt_cxx, r.dt_cyy, r.dt_czz, r.dt_cxy, r.dt_cyz, r.dt_cxz};
+        for (int m = 0; m < 12; m++) S(PCT + m) = ct[m];
         alive[(size_t)p] = 1;
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
@@ -464,6 +471,7 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         if (fl & CH4G_LAGRANGE_ORIENTATION_SECONDORIENTATION) mark(PNO, 6);
         if (fl & CH4G_LAGRANGE_TEMPERATURE) mark(PTP, 1);
         if (Q->pc.flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT) mark(PSED, 1);
+        if (Q->pc.flags2 & CH4G_LAGRANGE_POLYMER) mark(PCT, 12);
         PermRows pr{};
         for (int f = 0; f < PF_COUNT; f++) {
             bool uniform = true;
@@ -871,6 +879,18 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
                 r.nz /= val;
             }
         }
+        if (pc->flags2 & CH4G_LAGRANGE_POLYMER) {
+            // lagrange.c:937-955: the diagonal of the conformation tensor from
+            // a random_vector (two more draws), the rest 0
+            const double theta = acos(2.0 * erand48(L->xsubi) - 1.0);
+            const double phi = 2 * 3.14159265359 * erand48(L->xsubi);
+            double st, ct, sp, cp;
+            sincos(theta, &st, &ct);
+            sincos(phi, &sp, &cp);
+            r.cxx = st * sp;
+            r.cyy = st * cp;
+            r.czz = ct;
+        }
     }
     if ((rc = alloc_state(L, np))) goto err;
     if (lhas(*pc, CH4G_LAGRANGE_TWOWAY_MOMENTUM)) {
@@ -1012,6 +1032,8 @@ int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t
             r.cp_p = S(PCP);
             r.gravity_coeff = S(PGC);
             r.sediment = S(PSED);
+            double *ct = &r.cxx;
+            for (int m = 0; m < 12; m++) ct[m] = S(PCT + m);
         }
         k++;
     }

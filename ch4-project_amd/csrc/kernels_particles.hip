@@ -1106,6 +1106,37 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
         }
         if (nd == 0) wrap_keep(P, x, y, z, p);
 
+        // ---- polymer conformation tensor along the trajectory
+        // (polymer.c:150-240, after the wall rules, lagrange.c:3998-4002):
+        // dc/dt = A^T c + c A + (I - c) / tau_p with A[i][j] = d_j u_i, Adams-
+        // Bashforth over the lattice step time_dt = 1 (the reference's Euler
+        // branch, itime == 0, is never taken inside move_particles)
+        if constexpr (GRAD) {
+            if (P.flags2 & CH4G_LAGRANGE_POLYMER) {
+                double C[3][3], Fc[3][3];
+                const double cd[6] = {R(PCT), R(PCT + 1), R(PCT + 2), R(PCT + 3), R(PCT + 4), R(PCT + 5)};
+                C[0][0] = cd[0];
+                C[1][1] = cd[1];
+                C[2][2] = cd[2];
+                C[0][1] = C[1][0] = cd[3];
+                C[1][2] = C[2][1] = cd[4];
+                C[0][2] = C[2][0] = cd[5];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        double f = 0.0;
+                        for (int k = 0; k < 3; k++) f += G[3 * k + i] * C[k][j] + C[i][k] * G[3 * k + j];
+                        f += ((i == j ? 1.0 : 0.0) - C[i][j]) * P.inv_tau_polymer;
+                        Fc[i][j] = f;
+                    }
+                const int ui[6] = {0, 1, 2, 0, 1, 0}, uj[6] = {0, 1, 2, 1, 2, 2};
+                for (int m = 0; m < 6; m++) {
+                    const int i = ui[m], j = uj[m];
+                    R(PCT + m) = C[i][j] + 0.5 * 1.0 * (3. * Fc[i][j] - R(PDCT + m));
+                    R(PDCT + m) = Fc[i][j];
+                }
+            }
+        }
+
         // ---- particle temperature (lagrange.c:4007-4020): relaxation to the
         // fluid's t over the heat response time, T_p <- (T_p + fac1 t)
         // exp(-fac1); fac1 and exp(-fac1) are per-particle constants formed

@@ -876,6 +876,9 @@ static int snapshot_members(const ch4g_run_params *rp, struct snap_member *m)
             }
             if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) SM("swim_velocity", swim_velocity);
         }
+        if (rp->pcfg.flags2 & CH4G_LAGRANGE_POLYMER) { /* lagrange.c:2015-2052 */
+            SM("cxx", cxx); SM("cyy", cyy); SM("czz", czz); SM("cxy", cxy); SM("cyz", cyz); SM("cxz", cxz);
+        }
     }
     if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
         SM("temperature", t); SM("dt_t", dt_t);
@@ -1000,6 +1003,10 @@ static void write_particle_snapshot(void)
                     xmf_scalar(fo, h5, "gyrotaxis_velocity", "gyrotaxis_velocity", np);
             }
             if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) xmf_scalar(fo, h5, "swim_velocity", "swim_velocity", np);
+        }
+        if (rp->pcfg.flags2 & CH4G_LAGRANGE_POLYMER) { /* lagrange.c:2457-2482 */
+            const char *c6[6] = {"cxx", "cxy", "cxz", "cyy", "cyz", "czz"};
+            xmf_joined(fo, h5, "polymer conformation tensor", "Tensor6", c6, 6, np);
         }
     }
     if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {

@@ -99,6 +99,7 @@ static const struct { const char *name; unsigned flag; } LFLAGS2[] = {
     {"LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION", CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION},
     {"LAGRANGE_BC_INELASTIC_SEDIMENT", CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT},
     {"LAGRANGE_ORIENTATION_RANDOM", CH4G_LAGRANGE_ORIENTATION_RANDOM},
+    {"LAGRANGE_POLYMER", CH4G_LAGRANGE_POLYMER},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -308,6 +309,8 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     /* random reorientation sits inside #ifdef LAGRANGE_ORIENTATION (lagrange.c:3698) */
     if (!((out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION) && (out->pcfg.flags & CH4G_LAGRANGE_GRADIENT)))
         out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_ORIENTATION_RANDOM;
+    /* the conformation tensor sits inside #ifdef LAGRANGE_GRADIENT (lagrange.c:3998-4002) */
+    if (!(out->pcfg.flags & CH4G_LAGRANGE_GRADIENT)) out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_POLYMER;
     /* the deposition count sits inside #ifdef LAGRANGE_BC_INELASTIC (typedef.h:453-457) */
     if (!(out->pcfg.flags & CH4G_LAGRANGE_BC_INELASTIC)) out->pcfg.flags2 &= ~(unsigned)CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT;
     /* the Eulerian probe type sits inside #ifdef LAGRANGE_ADD_TRACER (parameters.c:744-755) */
@@ -415,6 +418,8 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         }
         /* parameters.c:621-695 */
         const unsigned fl = pc->flags;
+        /* parameters.c:729-733 */
+        if ((pc->flags2 & CH4G_LAGRANGE_POLYMER) && need(t, np, "tau_polymer", &pc->tau_polymer)) return -1;
         /* parameters.c:589-600 */
         if ((fl & CH4G_LAGRANGE_GRAVITY_VARIABLE) &&
             (need(t, np, "gravity_coeff_types", &pc->gravity_coeff_types) ||

@@ -287,8 +287,9 @@ enum ch4g_lagrange_flag2 {
     CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION    = 1u << 11,  /* swim along n */
     CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1u << 12,  /* along p if aspect_ratio >= 1, else along n */
     CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT = 1u << 13, /* count of bottom-wall depositions, lagrange.c:3845-3863 (needs BC_INELASTIC) */
-    CH4G_LAGRANGE_ORIENTATION_RANDOM    = 1u << 14  /* p drawn anew (random_vector) every sub-step, lagrange.c:3698-3713;
+    CH4G_LAGRANGE_ORIENTATION_RANDOM    = 1u << 14, /* p drawn anew (random_vector) every sub-step, lagrange.c:3698-3713;
                                                        the device's acos / sin / cos: parity to their last bits */
+    CH4G_LAGRANGE_POLYMER               = 1u << 15  /* conformation tensor along the trajectory, polymer.c:150-240 (needs GRADIENT) */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -308,6 +309,7 @@ typedef struct ch4g_particle_config {
     /* LAGRANGE_RADIUSandDENSITY: read instead of tau_drag_* / beta_coeff_* (parameters.c:530-556) */
     double   particle_radius_types, particle_radius_min, particle_radius_max;
     double   particle_density_types, particle_density_min, particle_density_max;
+    double   tau_polymer;      /* LAGRANGE_POLYMER relaxation time (parameters.c:730-733) */
 } ch4g_particle_config;
 
 /* One particle: the members of the reference's point_particle
@@ -329,12 +331,14 @@ typedef struct ch4g_particle {
     double t_p, cp_p;                                   /* particle temperature and specific heat (LAGRANGE_TEMPERATURE) */
     double gravity_coeff;                               /* gravity factor of the particle's type (LAGRANGE_GRAVITY_VARIABLE) */
     double sediment;                                    /* bottom-wall depositions (LAGRANGE_BC_INELASTIC_SEDIMENT) */
+    double cxx, cyy, czz, cxy, cyz, cxz;                /* polymer conformation tensor (LAGRANGE_POLYMER) */
+    double dt_cxx, dt_cyy, dt_czz, dt_cxy, dt_cyz, dt_cxz;   /* and its last dc/dt */
 } ch4g_particle;
 
 /* No counterpart in the reference.  sizeof(ch4g_particle) as this library
  * was built: callers that exchange particle records check it against their
  * own header before ch4g_get_particles / ch4g_set_particles (the record grew
- * with the orientation, particle-temperature, gravity and sediment members; a stale header
+ * with the orientation, particle-temperature, gravity, sediment and polymer members; a stale header
  * would mis-stride the arrays).                                             */
 int64_t ch4g_particle_size(void);
 

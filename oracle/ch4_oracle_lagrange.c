@@ -287,6 +287,13 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
                 p->nz /= val;
             }
         }
+        if (pc->flags2 & CH4G_LAGRANGE_POLYMER) { /* lagrange.c:937-955 */
+            double theta = acos(2.0 * oracle_drand48(o) - 1.0);
+            double phi = 2 * 3.14159265359 * oracle_drand48(o);
+            p->cxx = sin(theta) * sin(phi);
+            p->cyy = sin(theta) * cos(phi);
+            p->czz = cos(theta);
+        }
     }
     free(tau);
     free(beta);
@@ -1129,6 +1136,39 @@ static void move_at(oracle *o, int first)
                 /* LAGRANGE_ORIENTATION_BC (lagrange.c:3942-3946) */
                 if (LHAS(L, CH4G_LAGRANGE_ORIENTATION_BC) && p->py > 0.0) p->py *= -1.0;
             }
+        }
+        /* polymer conformation tensor (polymer.c:150-240), after the wall
+         * rules (lagrange.c:3998-4002); time_dt = 1, and the Euler branch of
+         * itime == 0 is never reached inside move_particles */
+        if (LHAS(L, CH4G_LAGRANGE_GRADIENT) && (L->pc.flags2 & CH4G_LAGRANGE_POLYMER)) {
+            double A[3][3] = {{p->dx_ux, p->dy_ux, p->dz_ux}, {p->dx_uy, p->dy_uy, p->dz_uy},
+                              {p->dx_uz, p->dy_uz, p->dz_uz}};
+            double C[3][3] = {{p->cxx, p->cxy, p->cxz}, {p->cxy, p->cyy, p->cyz}, {p->cxz, p->cyz, p->czz}};
+            double Fo[3][3] = {{p->dt_cxx, p->dt_cxy, p->dt_cxz}, {p->dt_cxy, p->dt_cyy, p->dt_cyz},
+                               {p->dt_cxz, p->dt_cyz, p->dt_czz}};
+            double I3[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+            double F[3][3];
+            double invtau = 1. / L->pc.tau_polymer;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    F[i][j] = 0.0;
+                    for (int k = 0; k < 3; k++) F[i][j] += A[k][i] * C[k][j] + C[i][k] * A[k][j];
+                    F[i][j] += (I3[i][j] - C[i][j]) * invtau;
+                }
+            for (int j = 0; j < 3; j++)
+                for (int i = 0; i <= j; i++) C[i][j] = C[i][j] + 0.5 * 1.0 * (3. * F[i][j] - Fo[i][j]);
+            p->cxx = C[0][0];
+            p->cyy = C[1][1];
+            p->czz = C[2][2];
+            p->cxy = C[0][1];
+            p->cyz = C[1][2];
+            p->cxz = C[0][2];
+            p->dt_cxx = F[0][0];
+            p->dt_cyy = F[1][1];
+            p->dt_czz = F[2][2];
+            p->dt_cxy = F[0][1];
+            p->dt_cyz = F[1][2];
+            p->dt_cxz = F[0][2];
         }
         /* particle temperature (lagrange.c:4007-4020): first-order relaxation
          * to the fluid's t, T_p <- (T_p + fac1 t) exp(-fac1), fac1 the

@@ -208,6 +208,22 @@ def test_read_run_params_radius_and_density(tmp_path):
     assert (rp.pcfg.particle_radius_types, rp.pcfg.particle_radius_max, rp.pcfg.particle_density_min) == (2.0, 1.0, 2.0)
 
 
+def test_read_run_params_polymer(tmp_path):
+    """LAGRANGE_POLYMER reads tau_polymer (parameters.c:729-733) and acts only
+    with LAGRANGE_GRADIENT."""
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    text = open(os.path.join(FIXP, "define.h")).read()
+    base = open(os.path.join(FIXP, "param.in")).read()
+    d, pin = tmp_path / "define.h", tmp_path / "param.in"
+    d.write_text(text + "#define LAGRANGE_POLYMER\n")
+    pin.write_text(base)
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == -1
+    pin.write_text(base + "tau_polymer 12.5\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == 0
+    assert rp.pcfg.flags2 & abi.LAGRANGE_POLYMER and rp.pcfg.tau_polymer == 12.5
+
+
 def test_read_run_params_rejects_unimplemented_switch(tmp_path):
     d = tmp_path / "define.h"
     d.write_text(open(os.path.join(FIX, "define.h")).read() + "#define LB_FLUID_BC_Y_P_SLIP\n")

@@ -196,6 +196,20 @@ def test_random_reorientation_fp64():
     compare_particles(o, lat, exact=False, pos_atol=1e-9, rel=1e-9)
 
 
+@pytest.mark.parametrize("case", ["heavy", "orientation"])
+def test_polymer_conformation_fp64(case):
+    """LAGRANGE_POLYMER: the conformation tensor along each trajectory (its
+    initial diagonal from two draws after the orientation draws) bit-identical
+    to the oracle over 100 steps."""
+    flags = HEAVY if case == "heavy" else ORIENT | abi.LAGRANGE_GRAVITY | abi.LAGRANGE_ADDEDMASS
+    pc = abi.make_particle_config(flags, 300, tau_drag=(2, 0.0, 4.0), beta_coeff=(1, 0.8, 0.8),
+                                  aspect_ratio=(2, 0.5, 4.0), gyrotaxis_velocity=(1, 1.0, 1.0),
+                                  swim_velocity=(1, 0.001, 0.001), flags2=abi.LAGRANGE_POLYMER, tau_polymer=20.0)
+    o, lat = pair(rb_small(), pc, 100)
+    assert np.abs(lat.particles()["dt_cxy"]).max() > 0
+    compare_particles(o, lat)
+
+
 def test_orientation_needs_the_gradient():
     lat = Lattice(rb_small())
     lat.initial_conditions()

@@ -1100,3 +1100,50 @@ def test_random_reorientation_draws_and_isotropy():
     b = o2.particles()
     assert abs(np.mean(b["pz"] ** 2) - 1.0 / 3.0) < 0.01
     assert abs(np.mean(b["px"])) < 0.02 and abs(np.mean(b["py"])) < 0.02
+
+
+# --------------------------------------------------------------------------
+# Polymer conformation tensor (LAGRANGE_POLYMER, polymer.c:150-240).
+def test_polymer_conformation_steady_shear_and_relaxation():
+    """The reference's evolution dc/dt = A^T c + c A + (I - c) / tau_p with
+    A[i][j] = d_j u_i: in the shear u_x = g y its fixed point is c_xx = 1,
+    c_xy = g tau_p, c_yy = 1 + 2 (g tau_p)^2, c_zz = 1 (the transpose of the
+    upper-convected form, whose c_xx would grow instead), reached by the
+    Adams-Bashforth steps; at rest c - I decays as exp(-t / tau_p) to the
+    steps' second order.  The initial diagonal is a random_vector of two
+    draws after the position draws."""
+    g, tau_p = 0.01, 5.0
+    flags = abi.LAGRANGE_GRADIENT
+    o, cfg = quiescent(8, 8, 8, seed=13)
+    o.particles_init(abi.make_particle_config(flags, 2, tau_drag=(1, 0.0, 0.0), flags2=abi.LAGRANGE_POLYMER,
+                                              tau_polymer=tau_p))
+    a = o.particles()
+    r = glibc_stream(13, 10).reshape(2, 5)[:, 3:]
+    th, ph = np.arccos(2 * r[:, 0] - 1), 2 * 3.14159265359 * r[:, 1]
+    assert np.allclose(a["cxx"], np.sin(th) * np.sin(ph), atol=1e-15)
+    assert np.allclose(a["czz"], np.cos(th), atol=1e-15) and not a["cxy"].any()
+    a["x"], a["y"], a["z"] = [2.0, 5.0], [4.3, 3.6], [4.0, 4.0]
+    o.set_particles(a)
+    x, y, z = _centres(cfg)
+    u = np.zeros(x.shape + (3,))
+    u[..., 0] = g * (y - 4.0)
+    for k in range(400):
+        b = o.move(u, None, first=(k == 0))
+    gt = g * tau_p
+    assert np.allclose(b["cxx"], 1.0, rtol=1e-9)
+    assert np.allclose(b["cxy"], gt, rtol=1e-9)
+    assert np.allclose(b["cyy"], 1.0 + 2 * gt * gt, rtol=1e-9)
+    assert np.allclose(b["czz"], 1.0, rtol=1e-9) and np.allclose(b["cxz"], 0.0, atol=1e-12)
+    # at rest, tau_p = 50 lattice steps: c - I decays as exp(-t / tau_p) (the
+    # first step's Adams-Bashforth start is off by 1/2 of dt / tau_p)
+    tau_p = 50.0
+    o2, cfg2 = quiescent(8, 8, 8, seed=13)
+    o2.particles_init(abi.make_particle_config(flags, 2, tau_drag=(1, 0.0, 0.0), flags2=abi.LAGRANGE_POLYMER,
+                                               tau_polymer=tau_p))
+    c0 = o2.particles()
+    u0 = np.zeros(x.shape + (3,))
+    n = 100
+    for k in range(n):
+        c = o2.move(u0, None, first=(k == 0))
+    for f in ("cxx", "cyy", "czz"):
+        assert np.allclose(c[f] - 1.0, (c0[f] - 1.0) * math.exp(-n / tau_p), rtol=2e-2)
