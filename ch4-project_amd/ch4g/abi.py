@@ -77,6 +77,9 @@ LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION = 1 << 12
 LAGRANGE_BC_INELASTIC_SEDIMENT = 1 << 13
 LAGRANGE_ORIENTATION_RANDOM = 1 << 14
 LAGRANGE_POLYMER = 1 << 15
+LAGRANGE_ORIENTATION_ACTIVE_JUMP = 1 << 16
+LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS = 1 << 17
+LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE = 1 << 18
 
 FP64 = 0
 FP32_DEV = 1
@@ -169,13 +172,17 @@ class ParticleConfig(C.Structure):
         ("particle_density_types", C.c_double), ("particle_density_min", C.c_double),
         ("particle_density_max", C.c_double),
         ("tau_polymer", C.c_double),
+        ("critical_shear_rate_types", C.c_double), ("critical_shear_rate_min", C.c_double),
+        ("critical_shear_rate_max", C.c_double),
+        ("jump_time_types", C.c_double), ("jump_time_min", C.c_double), ("jump_time_max", C.c_double),
     ]
 
 
 def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_coeff=(1, 1.0, 1.0),
                          time_dt_lagr_factor=1.0, aspect_ratio=(1, 1.0, 1.0), gyrotaxis_velocity=(1, 0.0, 0.0),
                          swim_velocity=(1, 0.0, 0.0), cp_coeff=(1, 1.0, 1.0), gravity_coeff=(1, 1.0, 1.0),
-                         flags2=0, particle_radius=(1, 0.0, 0.0), particle_density=(1, 1.0, 1.0), tau_polymer=1.0):
+                         flags2=0, particle_radius=(1, 0.0, 0.0), particle_density=(1, 1.0, 1.0), tau_polymer=1.0,
+                         critical_shear_rate=(1, 0.0, 0.0), jump_time=(1, 1.0, 1.0)):
     """ParticleConfig from (types, min, max) ladders as param.in gives them."""
     pc = ParticleConfig()
     pc.flags = flags
@@ -195,6 +202,9 @@ def make_particle_config(flags, particle_number, tau_drag=(1, 0.0, 0.0), beta_co
     (pc.particle_density_types, pc.particle_density_min,
      pc.particle_density_max) = (float(v) for v in particle_density)
     pc.tau_polymer = float(tau_polymer)
+    (pc.critical_shear_rate_types, pc.critical_shear_rate_min,
+     pc.critical_shear_rate_max) = (float(v) for v in critical_shear_rate)
+    pc.jump_time_types, pc.jump_time_min, pc.jump_time_max = (float(v) for v in jump_time)
     return pc
 
 
@@ -205,7 +215,8 @@ PARTICLE_FIELDS = ["name", "x", "y", "z", "vx", "vy", "vz", "vx_old", "vy_old", 
                    "px", "py", "pz", "dt_px", "dt_py", "dt_pz", "aspect_ratio", "gyrotaxis_velocity", "swim_velocity",
                    "nx", "ny", "nz", "dt_nx", "dt_ny", "dt_nz", "t_p", "cp_p",
                    "gravity_coeff", "sediment", "cxx", "cyy", "czz", "cxy", "cyz", "cxz",
-                   "dt_cxx", "dt_cyy", "dt_czz", "dt_cxy", "dt_cyz", "dt_cxz"]
+                   "dt_cxx", "dt_cyy", "dt_czz", "dt_cxy", "dt_cyz", "dt_cxz",
+                   "critical_shear_rate", "shear_rate", "jump_time", "time_from_jump", "px_jump", "py_jump", "pz_jump"]
 
 
 class Particle(C.Structure):
@@ -215,7 +226,7 @@ class Particle(C.Structure):
 class ParticleType(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("count", "particle_radius", "particle_density", "tau_drag", "beta_coeff",
                                           "gravity_coeff", "cp_coeff", "aspect_ratio", "gyrotaxis_velocity",
-                                          "swim_velocity")]
+                                          "swim_velocity", "jump_time", "critical_shear_rate")]
 
 
 def particle_dtype():

@@ -221,7 +221,12 @@ enum PartField {
     PSED,               // sediment (bottom-wall depositions)
     PCT,                // 6 rows: cxx, cyy, czz, cxy, cyz, cxz (polymer conformation)
     PDCT = PCT + 6,     // 6 rows: dt_cxx .. dt_cxz
-    PF_COUNT = PDCT + 6
+    PCSR = PDCT + 6,    // critical_shear_rate (jumps)
+    PSR,                // shear_rate
+    PJT,                // jump_time
+    PTFJ,               // time_from_jump
+    PPJ,                // 3 rows: px_jump, py_jump, pz_jump
+    PF_COUNT = PPJ + 3
 };
 
 struct PartParams {
@@ -243,6 +248,7 @@ struct PartParams {
     double Tw_bot, Tw_top;   // T_wall - T_ref
     double T_bot, T_top;     // the walls' temperatures a re-injected particle takes (LAGRANGE_TEMPERATURE)
     double inv_tau_polymer;  // 1 / tau_polymer (LAGRANGE_POLYMER)
+    double log_001;          // log(0.01), host libm (the jump duration -jump_time log(0.01))
     int first;               // itime == 1 (Euler start)
     double dt;               // time_dt_lagr
     double grav[3];

@@ -120,8 +120,23 @@ int validate_particles(const ch4g_lattice *L, const ch4g_particle_config *pc)
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC | CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE |
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION |
                             CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION | CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT |
-                            CH4G_LAGRANGE_ORIENTATION_RANDOM | CH4G_LAGRANGE_POLYMER;
+                            CH4G_LAGRANGE_ORIENTATION_RANDOM | CH4G_LAGRANGE_POLYMER |
+                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP | CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS |
+                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE;
     if ((pc->flags2 & ~known2) || pc->reserved) return fail(-1, "unknown bits in flags2 / reserved");
+    if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) &&
+        (pc->critical_shear_rate_types < 1 || pc->critical_shear_rate_max < pc->critical_shear_rate_min ||
+         pc->jump_time_types < 1 || pc->jump_time_max < pc->jump_time_min))
+        return fail(-1, "Error in critical_shear_rate / jump_time parameters");
+    if ((pc->flags2 & (CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS | CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE)) &&
+        !(pc->flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP))
+        return fail(-1, "LAGRANGE_ORIENTATION_ACTIVE_JUMP_* need LAGRANGE_ORIENTATION_ACTIVE_JUMP");
+    // the jump reads the tracer's velocity gradient (lagrange.c:3056-3066),
+    // which the kernel carries only in its LAGRANGE_GRADIENT variant
+    if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) && !lhas(*pc, CH4G_LAGRANGE_GRADIENT))
+        return fail(-1, "LAGRANGE_ORIENTATION_ACTIVE_JUMP needs LAGRANGE_GRADIENT");
+    if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE) && !L->base.thermal)
+        return fail(-1, "LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE needs LB_TEMPERATURE");
     if ((pc->flags2 & CH4G_LAGRANGE_POLYMER) && !lhas(*pc, CH4G_LAGRANGE_GRADIENT))
         return fail(-1, "LAGRANGE_POLYMER needs LAGRANGE_GRADIENT");
     if ((pc->flags2 & CH4G_LAGRANGE_POLYMER) && !(pc->tau_polymer > 0.0)) return fail(-1, "tau_polymer must be > 0");
@@ -362,6 +377,7 @@ PartParams part_params(ch4g_lattice *L)
     P.T_bot = L->cfg.T_bot;
     P.T_top = L->cfg.T_top;
     P.inv_tau_polymer = 1. / Q->pc.tau_polymer;
+    P.log_001 = log(0.01);
     P.first = L->itime == 1;
     P.dt = Q->dt;
     P.grav[0] = L->cfg.gravity_x;
@@ -447,6 +463,9 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         const double ct[12] = {r.cxx, r.cyy, r.czz, r.cxy, r.cyz, r.cxz,
                                r.dt_cxx, r.dt_cyy, r.dt_czz, r.dt_cxy, r.dt_cyz, r.dt_cxz};
         for (int m = 0; m < 12; m++) S(PCT + m) = ct[m];
+        const double jr[7] = {r.critical_shear_rate, r.shear_rate, r.jump_time, r.time_from_jump,
+                              r.px_jump, r.py_jump, r.pz_jump};
+        for (int m = 0; m < 7; m++) S(PCSR + m) = jr[m];
         alive[(size_t)p] = 1;
     }
     HIPCHK(hipMemcpyAsync(Q->d_state, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice, L->stream));
@@ -472,6 +491,10 @@ int upload(ch4g_lattice *L, const ch4g_particle *in, int64_t n)
         if (fl & CH4G_LAGRANGE_TEMPERATURE) mark(PTP, 1);
         if (Q->pc.flags2 & CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT) mark(PSED, 1);
         if (Q->pc.flags2 & CH4G_LAGRANGE_POLYMER) mark(PCT, 12);
+        if (Q->pc.flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) {
+            mark(PSR, 1);
+            mark(PTFJ, 4);
+        }
         PermRows pr{};
         for (int f = 0; f < PF_COUNT; f++) {
             bool uniform = true;
@@ -688,6 +711,7 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     const bool swim = orient && lhas(*pc, CH4G_LAGRANGE_ORIENTATION_ACTIVE);
     const bool tpart = lhas(*pc, CH4G_LAGRANGE_TEMPERATURE);
     const bool gvar = lhas(*pc, CH4G_LAGRANGE_GRAVITY_VARIABLE);
+    const bool jump = swim && (pc->flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP);
     const bool rad = (pc->flags2 & CH4G_LAGRANGE_RADIUSANDDENSITY) != 0;
     double types = pc->tau_drag_types;
     if (am) types *= pc->beta_coeff_types;
@@ -697,13 +721,14 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
     if (jeff) types *= pc->aspect_ratio_types;
     if (gyro) types *= pc->gyrotaxis_velocity_types;
     if (swim) types *= pc->swim_velocity_types;
+    if (jump) types *= pc->critical_shear_rate_types * pc->jump_time_types;  // parameters.c:709, :720
     // fluid_tracers (parameters.c:743-755): the tracer, and with
     // LAGRANGE_ADD_TRACER_STATIC an Eulerian probe type before it
     const int ntr = lhas(*pc, CH4G_LAGRANGE_ADD_TRACER) ? (lhas(*pc, CH4G_LAGRANGE_ADD_TRACER_STATIC) ? 2 : 1) : 0;
     types += ntr;
     const int nt = (int)types;
     std::vector<double> tau((size_t)nt, 0.0), beta((size_t)nt, 0.0), asp((size_t)nt, 0.0), gyv((size_t)nt, 0.0),
-        swv((size_t)nt, 0.0), cpv((size_t)nt, 0.0), gcv((size_t)nt, 0.0);
+        swv((size_t)nt, 0.0), cpv((size_t)nt, 0.0), gcv((size_t)nt, 0.0), csr((size_t)nt, 0.0), jtv((size_t)nt, 0.0);
     double rep = 1.0;
     std::vector<double> rv, dv;  // radius and density ladders (LAGRANGE_RADIUSandDENSITY)
     if (rad) {
@@ -754,6 +779,13 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         ladder(swv, ntr, 0.0, pc->swim_velocity_types, pc->swim_velocity_min, pc->swim_velocity_max, false, rep);
         rep *= pc->swim_velocity_types;
     }
+    if (jump) {  // lagrange.c:526-570: linear steps; tracers 1e10 and 0
+        ladder(csr, ntr, 1.e+10, pc->critical_shear_rate_types, pc->critical_shear_rate_min,
+               pc->critical_shear_rate_max, false, rep);
+        rep *= pc->critical_shear_rate_types;
+        ladder(jtv, ntr, 0.0, pc->jump_time_types, pc->jump_time_min, pc->jump_time_max, false, rep);
+        rep *= pc->jump_time_types;
+    }
     if (tpart) { // after the orientation ladders (lagrange.c:579-617)
         ladder(cpv, ntr, 0.0, pc->cp_coeff_types, pc->cp_coeff_min, pc->cp_coeff_max,
                lhas(*pc, CH4G_LAGRANGE_TEMPERATURE_INCREMENT_LOG), rep);
@@ -775,6 +807,8 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         y.aspect_ratio = asp[(size_t)t];
         y.gyrotaxis_velocity = gyv[(size_t)t];
         y.swim_velocity = swv[(size_t)t];
+        y.jump_time = jtv[(size_t)t];
+        y.critical_shear_rate = csr[(size_t)t];
     }
     for (int64_t i = 0; i < pc->particle_number; i++) Q->types[(size_t)(i % nt)].count += 1.0;
 
@@ -838,6 +872,10 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
         if (jeff) r.aspect_ratio = asp[(size_t)type];
         if (gyro) r.gyrotaxis_velocity = gyv[(size_t)type];
         if (swim) r.swim_velocity = swv[(size_t)type];
+        if (jump) {
+            r.critical_shear_rate = csr[(size_t)type];
+            r.jump_time = jtv[(size_t)type];
+        }
         if (tpart) {
             r.cp_p = cpv[(size_t)type];  // lagrange.c:730-732
             r.t_p = L->cfg.T_bot;        // lagrange.c:862-864
@@ -878,6 +916,14 @@ int ch4g_particles_init(ch4g_lattice *L, const ch4g_particle_config *pc)
                 r.ny /= val;
                 r.nz /= val;
             }
+        }
+        if (jump) {
+            // lagrange.c:927-934: time from the last jump in [0, 10 jump_time),
+            // one more draw; the jump direction starts as p
+            r.time_from_jump = 10. * erand48(L->xsubi) * r.jump_time;
+            r.px_jump = r.px;
+            r.py_jump = r.py;
+            r.pz_jump = r.pz;
         }
         if (pc->flags2 & CH4G_LAGRANGE_POLYMER) {
             // lagrange.c:937-955: the diagonal of the conformation tensor from
@@ -1034,6 +1080,8 @@ int ch4g_get_particles(ch4g_lattice *L, ch4g_particle *out, int64_t cap, int64_t
             r.sediment = S(PSED);
             double *ct = &r.cxx;
             for (int m = 0; m < 12; m++) ct[m] = S(PCT + m);
+            double *jr = &r.critical_shear_rate;
+            for (int m = 0; m < 7; m++) jr[m] = S(PCSR + m);
         }
         k++;
     }

@@ -767,7 +767,47 @@ __global__ void __launch_bounds__(PB, GRAD ? 3 : 1) k_particles(PartParams P)
                 // reference's #elif order, else along p on top of u
                 const double sw = R(PSWIM);
                 const uint32_t f2 = P.flags2;
-                if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC) {
+                bool jumped = false;
+                if constexpr (GRAD) {
+                    if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) {
+                        // copepod jumps (lagrange.c:3048-3132): react to the
+                        // shear rate sqrt(2) sqrt(...) as the reference writes
+                        // it (its second term is dy_uy dz_uy), or to t
+                        double sr;
+                        if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE) {
+                            sr = t;
+                        } else {
+                            const double s01 = G[1] + G[3], s02 = G[2] + G[6], s12 = G[7] + G[5];
+                            sr = sqrt(2.) * sqrt(G[0] * G[0] + G[4] * G[5] + G[8] * G[8] + 0.5 * s01 * s01 +
+                                                 0.5 * s02 * s02 + 0.5 * s12 * s12);
+                            R(PSR) = sr;
+                        }
+                        const double jt = R(PJT);
+                        double tfj = R(PTFJ);
+                        double amp;
+                        if (tfj > -jt * P.log_001) {
+                            amp = 0.0;
+                            const double csr = R(PCSR);
+                            const bool go = (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS) ? sr < csr : sr > csr;
+                            if (go) {
+                                tfj = 0.0;
+                                amp = sw;
+                                for (int i = 0; i < 3; i++) R(PPJ + i) = pv[i];
+                            }
+                        } else {
+                            amp = sw * exp(-tfj / jt);
+                            if (jt == 0.0) amp = 0.0;
+                        }
+                        vx += amp * R(PPJ);
+                        vy += amp * R(PPJ + 1);
+                        vz += amp * R(PPJ + 2);
+                        R(PTFJ) = tfj + dt;
+                        jumped = true;
+                    }
+                }
+                if (jumped) {
+                    // the jump is the first of the reference's #elif modes
+                } else if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC) {
                     vx = sw * pv[0];
                     vy = sw * pv[1];
                     vz = sw * pv[2];

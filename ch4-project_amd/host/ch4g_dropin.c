@@ -312,7 +312,13 @@ static void write_particle_properties(void)
                     if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS)
                         fprintf(fo, "gyrotaxis_velocity %e ", y[i].gyrotaxis_velocity);
                 }
-                if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) fprintf(fo, "swim_velocity %e ", y[i].swim_velocity);
+                if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) {
+                    fprintf(fo, "swim_velocity %e ", y[i].swim_velocity);
+                    if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) {
+                        fprintf(fo, "jump_time %e ", y[i].jump_time);
+                        fprintf(fo, "critical_shear_rate %e ", y[i].critical_shear_rate);
+                    }
+                }
             }
             fprintf(fo, "\n");
         }
@@ -421,6 +427,8 @@ static double ptypes(const ch4g_particle_config *pc)
             if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) types *= pc->gyrotaxis_velocity_types;
         }
         if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) types *= pc->swim_velocity_types;
+        if (pc->flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) /* parameters.c:709, :720 */
+            types *= pc->critical_shear_rate_types * pc->jump_time_types;
     }
     if (fl & CH4G_LAGRANGE_TEMPERATURE) types *= pc->cp_coeff_types; /* parameters.c:615 */
     if (fl & CH4G_LAGRANGE_ADD_TRACER) types += (fl & CH4G_LAGRANGE_ADD_TRACER_STATIC) ? 2.0 : 1.0;
@@ -746,6 +754,9 @@ static hid_t particle_type(const ch4g_run_params *rp)
                 if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) M(gyrotaxis_velocity);
             }
             if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) M(swim_velocity);
+            if (rp->pcfg.flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) { /* lagrange.c:4411-4424 */
+                M(critical_shear_rate); M(shear_rate); M(jump_time); M(px_jump); M(py_jump); M(pz_jump);
+            }
         }
     }
     if (rp->cfg.flags & CH4G_LB_TEMPERATURE) {
@@ -875,6 +886,10 @@ static int snapshot_members(const ch4g_run_params *rp, struct snap_member *m)
                 if (fl & CH4G_LAGRANGE_ORIENTATION_JEFFREY_GYROTAXIS) SM("gyrotaxis_velocity", gyrotaxis_velocity);
             }
             if (fl & CH4G_LAGRANGE_ORIENTATION_ACTIVE) SM("swim_velocity", swim_velocity);
+            if (rp->pcfg.flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) { /* lagrange.c:1993-2010 */
+                SM("critical_shear_rate", critical_shear_rate); SM("shear_rate", shear_rate);
+                SM("jump_time", jump_time);
+            }
         }
         if (rp->pcfg.flags2 & CH4G_LAGRANGE_POLYMER) { /* lagrange.c:2015-2052 */
             SM("cxx", cxx); SM("cyy", cyy); SM("czz", czz); SM("cxy", cxy); SM("cyz", cyz); SM("cxz", cxz);

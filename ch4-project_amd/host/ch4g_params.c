@@ -100,6 +100,9 @@ static const struct { const char *name; unsigned flag; } LFLAGS2[] = {
     {"LAGRANGE_BC_INELASTIC_SEDIMENT", CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT},
     {"LAGRANGE_ORIENTATION_RANDOM", CH4G_LAGRANGE_ORIENTATION_RANDOM},
     {"LAGRANGE_POLYMER", CH4G_LAGRANGE_POLYMER},
+    {"LAGRANGE_ORIENTATION_ACTIVE_JUMP", CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP},
+    {"LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS", CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS},
+    {"LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE", CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE},
 };
 
 /* switches with no effect on the streaming step of these configurations
@@ -302,9 +305,15 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
     {
         const unsigned modes = CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC | CH4G_LAGRANGE_ORIENTATION_ACTIVE_TEMPERATURE |
                                CH4G_LAGRANGE_ORIENTATION_ACTIVE_SECONDORIENTATION |
-                               CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION;
+                               CH4G_LAGRANGE_ORIENTATION_ACTIVE_SELECTIVEORIENTATION |
+                               CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP | CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS |
+                               CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE;
         if (!((out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION_ACTIVE) && (out->pcfg.flags & CH4G_LAGRANGE_GRADIENT)))
             out->pcfg.flags2 &= ~modes;
+        /* the jump's variants sit inside #ifdef LAGRANGE_ORIENTATION_ACTIVE_JUMP (lagrange.c:3052-3087) */
+        if (!(out->pcfg.flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP))
+            out->pcfg.flags2 &= ~(unsigned)(CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS |
+                                            CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE);
     }
     /* random reorientation sits inside #ifdef LAGRANGE_ORIENTATION (lagrange.c:3698) */
     if (!((out->pcfg.flags & CH4G_LAGRANGE_ORIENTATION) && (out->pcfg.flags & CH4G_LAGRANGE_GRADIENT)))
@@ -418,6 +427,14 @@ int ch4g_read_run_params(const char *define_h, const char *param_in, ch4g_run_pa
         }
         /* parameters.c:621-695 */
         const unsigned fl = pc->flags;
+        /* parameters.c:696-722 */
+        if ((pc->flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) &&
+            (need(t, np, "critical_shear_rate_types", &pc->critical_shear_rate_types) ||
+             need(t, np, "critical_shear_rate_min", &pc->critical_shear_rate_min) ||
+             need(t, np, "critical_shear_rate_max", &pc->critical_shear_rate_max) ||
+             need(t, np, "jump_time_types", &pc->jump_time_types) || need(t, np, "jump_time_min", &pc->jump_time_min) ||
+             need(t, np, "jump_time_max", &pc->jump_time_max)))
+            return -1;
         /* parameters.c:729-733 */
         if ((pc->flags2 & CH4G_LAGRANGE_POLYMER) && need(t, np, "tau_polymer", &pc->tau_polymer)) return -1;
         /* parameters.c:589-600 */

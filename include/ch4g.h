@@ -289,7 +289,12 @@ enum ch4g_lagrange_flag2 {
     CH4G_LAGRANGE_BC_INELASTIC_SEDIMENT = 1u << 13, /* count of bottom-wall depositions, lagrange.c:3845-3863 (needs BC_INELASTIC) */
     CH4G_LAGRANGE_ORIENTATION_RANDOM    = 1u << 14, /* p drawn anew (random_vector) every sub-step, lagrange.c:3698-3713;
                                                        the device's acos / sin / cos: parity to their last bits */
-    CH4G_LAGRANGE_POLYMER               = 1u << 15  /* conformation tensor along the trajectory, polymer.c:150-240 (needs GRADIENT) */
+    CH4G_LAGRANGE_POLYMER               = 1u << 15, /* conformation tensor along the trajectory, polymer.c:150-240 (needs GRADIENT) */
+    /* copepod-like jumps of active tracers (lagrange.c:3048-3132; the first of the swimming modes):
+       needs LAGRANGE_GRADIENT; the decay exp(-t / jump_time) is the device's exp, parity to its last bits */
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP             = 1u << 16,
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS    = 1u << 17,  /* jump below the critical shear rate */
+    CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE = 1u << 18   /* react to t instead of the shear rate */
 };
 
 /* Particle parameters of param.in (parameters.c:155-167, :524-584). */
@@ -310,6 +315,8 @@ typedef struct ch4g_particle_config {
     double   particle_radius_types, particle_radius_min, particle_radius_max;
     double   particle_density_types, particle_density_min, particle_density_max;
     double   tau_polymer;      /* LAGRANGE_POLYMER relaxation time (parameters.c:730-733) */
+    double   critical_shear_rate_types, critical_shear_rate_min, critical_shear_rate_max;  /* ..._ACTIVE_JUMP */
+    double   jump_time_types, jump_time_min, jump_time_max;
 } ch4g_particle_config;
 
 /* One particle: the members of the reference's point_particle
@@ -333,12 +340,14 @@ typedef struct ch4g_particle {
     double sediment;                                    /* bottom-wall depositions (LAGRANGE_BC_INELASTIC_SEDIMENT) */
     double cxx, cyy, czz, cxy, cyz, cxz;                /* polymer conformation tensor (LAGRANGE_POLYMER) */
     double dt_cxx, dt_cyy, dt_czz, dt_cxy, dt_cyz, dt_cxz;   /* and its last dc/dt */
+    double critical_shear_rate, shear_rate, jump_time, time_from_jump;   /* LAGRANGE_ORIENTATION_ACTIVE_JUMP */
+    double px_jump, py_jump, pz_jump;                   /* direction of the current jump */
 } ch4g_particle;
 
 /* No counterpart in the reference.  sizeof(ch4g_particle) as this library
  * was built: callers that exchange particle records check it against their
  * own header before ch4g_get_particles / ch4g_set_particles (the record grew
- * with the orientation, particle-temperature, gravity, sediment and polymer members; a stale header
+ * with the orientation, particle-temperature, gravity, sediment, polymer and jump members; a stale header
  * would mis-stride the arrays).                                             */
 int64_t ch4g_particle_size(void);
 
@@ -369,6 +378,7 @@ typedef struct ch4g_particle_type {
     double particle_radius, particle_density;   /* LAGRANGE_RADIUSandDENSITY */
     double tau_drag, beta_coeff, gravity_coeff, cp_coeff;
     double aspect_ratio, gyrotaxis_velocity, swim_velocity;
+    double jump_time, critical_shear_rate;
 } ch4g_particle_type;
 int ch4g_particle_types(ch4g_lattice *lat, ch4g_particle_type *out, int cap, int *ntypes);
 

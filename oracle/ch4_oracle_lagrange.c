@@ -141,6 +141,8 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     if (jeff) types *= pc->aspect_ratio_types;
     if (gyro) types *= pc->gyrotaxis_velocity_types;
     if (swim) types *= pc->swim_velocity_types;
+    if (swim && (pc->flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP))
+        types *= pc->critical_shear_rate_types * pc->jump_time_types; /* parameters.c:709, :720 */
     /* fluid_tracers: 2 with the Eulerian probe type (parameters.c:744-755) */
     int ntr = LHAS(L, CH4G_LAGRANGE_ADD_TRACER) ? (LHAS(L, CH4G_LAGRANGE_ADD_TRACER_STATIC) ? 2 : 1) : 0;
     types += ntr;
@@ -202,6 +204,17 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
         ladder(swv, nt, ntr, 0.0, pc->swim_velocity_types, pc->swim_velocity_min, pc->swim_velocity_max, 0, rep);
         rep *= pc->swim_velocity_types;
     }
+    /* lagrange.c:526-570: critical_shear_rate (tracer 1e10) and jump_time (tracer 0), linear */
+    const int jump = swim && (pc->flags2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP);
+    double *csr = (double *)calloc(nt, sizeof(double)), *jtv = (double *)calloc(nt, sizeof(double));
+    if (!csr || !jtv) return -1;
+    if (jump) {
+        ladder(csr, nt, ntr, 1.e+10, pc->critical_shear_rate_types, pc->critical_shear_rate_min,
+               pc->critical_shear_rate_max, 0, rep);
+        rep *= pc->critical_shear_rate_types;
+        ladder(jtv, nt, ntr, 0.0, pc->jump_time_types, pc->jump_time_min, pc->jump_time_max, 0, rep);
+        rep *= pc->jump_time_types;
+    }
     /* lagrange.c:579-617: cp_coeff after the orientation ladders, 0 for the tracer */
     if (tpart) {
         ladder(cpv, nt, ntr, 0.0, pc->cp_coeff_types, pc->cp_coeff_min, pc->cp_coeff_max,
@@ -262,6 +275,10 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
         if (jeff) p->aspect_ratio = asp[type];
         if (gyro) p->gyrotaxis_velocity = gyv[type];
         if (swim) p->swim_velocity = swv[type];
+        if (jump) {
+            p->critical_shear_rate = csr[type];
+            p->jump_time = jtv[type];
+        }
         if (tpart) {
             p->cp_p = cpv[type];     /* lagrange.c:730-732 */
             p->t_p = o->cfg.T_bot;   /* lagrange.c:862-864 */
@@ -287,6 +304,12 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
                 p->nz /= val;
             }
         }
+        if (jump) { /* lagrange.c:927-934 */
+            p->time_from_jump = 10. * oracle_drand48(o) * p->jump_time;
+            p->px_jump = p->px;
+            p->py_jump = p->py;
+            p->pz_jump = p->pz;
+        }
         if (pc->flags2 & CH4G_LAGRANGE_POLYMER) { /* lagrange.c:937-955 */
             double theta = acos(2.0 * oracle_drand48(o) - 1.0);
             double phi = 2 * 3.14159265359 * oracle_drand48(o);
@@ -302,6 +325,8 @@ int oracle_particles_init(oracle *o, const ch4g_particle_config *pc)
     free(swv);
     free(cpv);
     free(gcv);
+    free(csr);
+    free(jtv);
     /* lagrange.c:968-980: boundary_conditions_hydro, then
      * interpolate_vector_at_particles(u) (u and, with the gradient, grad u)
      * and v = u */
@@ -854,9 +879,40 @@ static void move_at(oracle *o, int first)
             p->vy = p->uy;
             p->vz = p->uz;
             if (LHAS(L, CH4G_LAGRANGE_ORIENTATION) && LHAS(L, CH4G_LAGRANGE_ORIENTATION_ACTIVE)) {
-                /* swimming: the #elif chain of lagrange.c:3134-3186 */
+                /* swimming: the #elif chain of lagrange.c:3048-3186 */
                 const unsigned f2 = L->pc.flags2;
-                if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC) {
+                if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP) {
+                    double shear_rate, velocity_amplitude, jump_time_duration;
+                    if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE) {
+                        shear_rate = p->t;
+                    } else {
+                        p->shear_rate = shear_rate =
+                            sqrt(2.) * sqrt(p->dx_ux * p->dx_ux + p->dy_uy * p->dz_uy + p->dz_uz * p->dz_uz +
+                                            0.5 * (p->dy_ux + p->dx_uy) * (p->dy_ux + p->dx_uy) +
+                                            0.5 * (p->dz_ux + p->dx_uz) * (p->dz_ux + p->dx_uz) +
+                                            0.5 * (p->dy_uz + p->dz_uy) * (p->dy_uz + p->dz_uy));
+                    }
+                    jump_time_duration = -(p->jump_time) * log(0.01);
+                    if (p->time_from_jump > jump_time_duration) {
+                        velocity_amplitude = 0.0;
+                        int go = (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS) ? shear_rate < p->critical_shear_rate
+                                                                                        : shear_rate > p->critical_shear_rate;
+                        if (go) {
+                            p->time_from_jump = 0.0;
+                            velocity_amplitude = p->swim_velocity;
+                            p->px_jump = p->px;
+                            p->py_jump = p->py;
+                            p->pz_jump = p->pz;
+                        }
+                    } else {
+                        velocity_amplitude = p->swim_velocity * exp(-(p->time_from_jump) / (p->jump_time));
+                        if (p->jump_time == 0.0) velocity_amplitude = 0.0;
+                    }
+                    p->vx += velocity_amplitude * p->px_jump;
+                    p->vy += velocity_amplitude * p->py_jump;
+                    p->vz += velocity_amplitude * p->pz_jump;
+                    p->time_from_jump += dt;
+                } else if (f2 & CH4G_LAGRANGE_ORIENTATION_ACTIVE_BALLISTIC) {
                     p->vx = p->swim_velocity * p->px;
                     p->vy = p->swim_velocity * p->py;
                     p->vz = p->swim_velocity * p->pz;

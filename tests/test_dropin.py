@@ -224,6 +224,34 @@ def test_read_run_params_polymer(tmp_path):
     assert rp.pcfg.flags2 & abi.LAGRANGE_POLYMER and rp.pcfg.tau_polymer == 12.5
 
 
+def test_read_run_params_jump(tmp_path):
+    """LAGRANGE_ORIENTATION_ACTIVE_JUMP reads critical_shear_rate_* and
+    jump_time_* (parameters.c:700-722); it and its HIGHPASS variant act only
+    inside LAGRANGE_ORIENTATION_ACTIVE."""
+    lib = C.CDLL(DROPIN)
+    rp = RunParams()
+    text = open(os.path.join(FIXP, "define.h")).read()
+    base = open(os.path.join(FIXP, "param.in")).read() + \
+        "aspect_ratio_types 1\naspect_ratio_min 2\naspect_ratio_max 2\n" \
+        "swim_velocity_types 1\nswim_velocity_min 0.01\nswim_velocity_max 0.01\n"
+    d, pin = tmp_path / "define.h", tmp_path / "param.in"
+    active = "#define LAGRANGE_ORIENTATION\n#define LAGRANGE_ORIENTATION_JEFFREY\n#define LAGRANGE_ORIENTATION_ACTIVE\n"
+    jump = "#define LAGRANGE_ORIENTATION_ACTIVE_JUMP\n#define LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS\n"
+    d.write_text(text + active + jump)
+    pin.write_text(base)
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == -1
+    pin.write_text(base + "critical_shear_rate_types 2\ncritical_shear_rate_min 1e-3\ncritical_shear_rate_max 2e-3\n"
+                   "jump_time_types 1\njump_time_min 4\njump_time_max 4\n")
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == 0
+    want = abi.LAGRANGE_ORIENTATION_ACTIVE_JUMP | abi.LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS
+    assert rp.pcfg.flags2 == want
+    assert (rp.pcfg.critical_shear_rate_types, rp.pcfg.critical_shear_rate_max, rp.pcfg.jump_time_min) == \
+        (2.0, 2e-3, 4.0)
+    d.write_text(text + "#define LAGRANGE_ORIENTATION\n#define LAGRANGE_ORIENTATION_JEFFREY\n" + jump)
+    assert lib.ch4g_read_run_params(str(d).encode(), str(pin).encode(), C.byref(rp)) == 0
+    assert rp.pcfg.flags2 == 0
+
+
 def test_read_run_params_rejects_unimplemented_switch(tmp_path):
     d = tmp_path / "define.h"
     d.write_text(open(os.path.join(FIX, "define.h")).read() + "#define LB_FLUID_BC_Y_P_SLIP\n")

@@ -176,6 +176,48 @@ def test_swimming_modes_fp64(mode):
     compare_particles(o, lat)
 
 
+@pytest.mark.parametrize("mode", ["shear", "highpass", "temperature"])
+def test_copepod_jumps_fp64(mode):
+    """LAGRANGE_ORIENTATION_ACTIVE_JUMP in Rayleigh-Benard: swimming tracers
+    jump when the shear rate (or, with JUMP_TEMPERATURE, the temperature)
+    crosses their critical value, then decay as exp(-t / jump_time); the
+    decay uses the device's exp, so the motion is held within 1e-9 of the
+    oracle after 100 steps, and the set of tracers that jumped is the
+    oracle's."""
+    f2 = abi.LAGRANGE_ORIENTATION_ACTIVE_JUMP
+    csr = (2, 2e-7, 6e-7)
+    if mode == "highpass":
+        f2 |= abi.LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS
+    elif mode == "temperature":
+        f2 |= abi.LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE
+        csr = (2, 0.35, 0.65)
+    flags = ORIENT | abi.LAGRANGE_GRAVITY | abi.LAGRANGE_ADDEDMASS
+    pc = abi.make_particle_config(flags, 300, tau_drag=(2, 0.0, 4.0), beta_coeff=(1, 0.8, 0.8),
+                                  aspect_ratio=(3, 0.5, 8.0), gyrotaxis_velocity=(2, 0.5, 2.0),
+                                  swim_velocity=(2, 0.0, 0.002), critical_shear_rate=csr, jump_time=(2, 2.0, 5.0),
+                                  time_dt_lagr_factor=2, flags2=f2)
+    o = Oracle(rb_small())
+    o.initial_conditions()
+    o.particles_init(pc)
+    t0 = o.particles()["time_from_jump"].copy()
+    o, lat = pair(rb_small(), pc, 100)
+    a, b = o.particles(), lat.particles()
+    assert len(a) == len(b) and np.array_equal(a["name"], b["name"])
+    a_s, b_s = np.sort(a, order="name"), np.sort(b, order="name")
+    tracer = a_s["tau_drag"] == 0.0
+    jumped_a = a_s["time_from_jump"] < t0[a_s["name"].astype(int)] + 100 * 0.5 - 1e-9
+    jumped_b = b_s["time_from_jump"] < t0[b_s["name"].astype(int)] + 100 * 0.5 - 1e-9
+    swim = tracer & (a_s["swim_velocity"] > 0)
+    print(f"  {mode}: {int(jumped_a[swim].sum())} of {int(swim.sum())} swimmers jumped")
+    assert np.array_equal(jumped_a[tracer], jumped_b[tracer])
+    assert jumped_a[swim].sum() > 0
+    for f in ("time_from_jump", "px_jump", "py_jump", "pz_jump", "shear_rate"):
+        err = np.abs(a[f] - b[f]).max()
+        print(f"  {f}: max |gpu - oracle| = {err:.3e}")
+        assert err <= 1e-9, f
+    compare_particles(o, lat, exact=False, pos_atol=1e-9, rel=1e-9)
+
+
 def test_random_reorientation_fp64():
     """LAGRANGE_ORIENTATION_RANDOM with random re-injection and the wall turn
     of p_y: each particle's draws in order (two for p, two per re-injection)

@@ -1050,6 +1050,71 @@ def test_swimming_modes():
     assert np.allclose(v[2], u, rtol=1e-14, atol=1e-18)
 
 
+# Copepod jumps (LAGRANGE_ORIENTATION_ACTIVE_JUMP, lagrange.c:3048-3132).
+def _jumper(flags2, gamma, csr, jt, steps, t_field=None, seed=7):
+    """One swimming tracer with p along z, the vorticity axis of the shear
+    u_x = gamma (y - 4), which Jeffery's equation leaves fixed; returns the
+    initial records and those after each sub-step."""
+    o, cfg = quiescent(8, 8, 8, tau=0.8, seed=seed, flags=abi.LB_TEMPERATURE if t_field is not None else 0)
+    pc = abi.make_particle_config(ACTIVE & ~abi.LAGRANGE_ORIENTATION_SECONDORIENTATION, 1, tau_drag=(1, 0.0, 0.0),
+                                  aspect_ratio=(1, 2.0, 2.0), swim_velocity=(1, 0.01, 0.01),
+                                  critical_shear_rate=(1, csr, csr), jump_time=(1, jt, jt),
+                                  flags2=abi.LAGRANGE_ORIENTATION_ACTIVE_JUMP | flags2)
+    o.particles_init(pc)
+    a0 = o.particles()
+    a = a0.copy()
+    a["x"], a["y"], a["z"] = 3.3, 4.2, 3.7
+    a["px"], a["py"], a["pz"] = 0.0, 0.0, 1.0
+    a["time_from_jump"] = 100.0 * jt
+    o.set_particles(a)
+    u = np.zeros((8, 8, 8, 3))
+    u[..., 0] = gamma * (np.arange(8.0)[None, :, None] - 4.0)
+    t = None if t_field is None else np.full((8, 8, 8), t_field)
+    return a0, [o.move(u, t, first=(k == 0))[0] for k in range(steps)]
+
+
+def test_jump_timing_and_decay():
+    """The first draw after the orientation sets time_from_jump = 10 r
+    jump_time; the shear rate sqrt(2 S:S) of a simple shear is gamma; above
+    the critical rate the tracer jumps along p with the swim speed, the
+    amplitude then decays as exp(-t / jump_time) until t exceeds
+    -jump_time ln 0.01, when the next jump starts; below it nothing
+    happens; HIGHPASS jumps below the critical rate instead; with
+    JUMP_TEMPERATURE the fluid temperature takes the shear rate's place."""
+    s, jt, gamma = 0.01, 2.0, 1e-3
+    a0, rs = _jumper(0, gamma, 5e-4, jt, 24, seed=11)
+    assert a0["jump_time"][0] == jt and a0["critical_shear_rate"][0] == 5e-4
+    assert a0["time_from_jump"][0] == 10.0 * glibc_stream(11, 6)[5] * jt
+    assert a0["px_jump"][0] == a0["px"][0] and a0["pz_jump"][0] == a0["pz"][0]
+    dt = rs[0]["time_from_jump"]
+    assert dt == 1.0
+    assert rs[0]["shear_rate"] == pytest.approx(gamma, rel=1e-12)
+    vz = np.array([r["vz"] for r in rs])
+    dur = -jt * np.log(0.01)
+    want, tfj = [], 100.0 * jt
+    for k in range(24):
+        if tfj > dur:
+            want.append(s)
+            tfj = 0.0
+        else:
+            want.append(s * np.exp(-tfj / jt))
+        tfj += dt
+    assert np.allclose(vz, want, rtol=1e-14, atol=0)
+    assert want[10] == s and want[9] == pytest.approx(s * np.exp(-4.5))  # 9 dt <= dur < 10 dt
+    assert np.array_equal([r["pz_jump"] for r in rs], np.ones(24))
+    # below the critical rate: no jump; HIGHPASS inverts the test
+    _, quiet = _jumper(0, gamma, 2e-3, jt, 3)
+    assert all(r["vz"] == 0.0 for r in quiet)
+    _, hp = _jumper(abi.LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS, gamma, 2e-3, jt, 2)
+    assert hp[0]["vz"] == s and hp[1]["vz"] == pytest.approx(s * np.exp(-1.0 / jt), rel=1e-14)
+    _, hp = _jumper(abi.LAGRANGE_ORIENTATION_ACTIVE_JUMP_HIGHPASS, gamma, 5e-4, jt, 2)
+    assert hp[0]["vz"] == 0.0
+    # temperature as the stimulus
+    T = abi.LAGRANGE_ORIENTATION_ACTIVE_JUMP_TEMPERATURE
+    assert _jumper(T, gamma, 0.3, jt, 1, t_field=0.5)[1][0]["vz"] == s
+    assert _jumper(T, gamma, 0.6, jt, 1, t_field=0.5)[1][0]["vz"] == 0.0
+
+
 def test_sediment_counts_bottom_depositions():
     """LAGRANGE_BC_INELASTIC_SEDIMENT (no nucleation): each crossing of the
     bottom wall adds one deposition, a crossing of the top wall none."""
